@@ -1,0 +1,506 @@
+"""formoniq_amd: formoniq's FEEC assembly-and-solve path on MI355X.
+
+A host-side mirror of the reference's API for this path (luiswirth/formoniq,
+paths relative to crates/), over the C ABI of libfeec (include/feec.h):
+
+* ``CartesianGrid``      regge/src/mesher/cartesian.rs:109-233
+* ``Complex``            simplicial/src/topology/complex.rs:278-336
+* ``MeshLengthsSq``      regge/src/lengths/mesh.rs, regge/src/coord/mesh.rs:208-216
+* ``WhitneyComplex``     formoniq/src/whitney_complex.rs:55-277, 321-465, 523-714
+                         (``mass``, ``dif_trial``, ``dif_test``, ``dif_both``,
+                         ``hdif_gram``, ``dif``, ``ndofs``, ``element``)
+* ``solve_source``       formoniq/src/problems/elliptic.rs:132-182
+* ``HodgeDirac``         formoniq/src/problems/dirac.rs:181-275
+* ``solve_dirac_source`` formoniq/src/problems/dirac.rs:473-534
+
+Everything computes on the GPU through libfeec; the module raises if the
+library is not built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import FeecError, Report, Request, Result, ServerOpts, SolveOpts, check, load
+
+MASS, DIF_TRIAL, DIF_TEST, DIF_BOTH, HDIF_GRAM = 0, 1, 2, 3, 4
+FULL, RELATIVE = 0, 1
+EUCLIDEAN, MINKOWSKI = 0, 1
+FP64, MIXED = 0, 1
+
+__all__ = [
+    "Context", "CartesianGrid", "Complex", "MeshLengthsSq", "WhitneyComplex", "Csr",
+    "HodgeDirac", "Problem", "solve_source", "solve_dirac_source", "solve_opts", "FeecError",
+    "normal_vector", "MASS", "DIF_TRIAL", "DIF_TEST", "DIF_BOTH", "HDIF_GRAM", "FULL",
+    "RELATIVE", "EUCLIDEAN", "MINKOWSKI", "FP64", "MIXED", "Report", "Request", "Result",
+    "ServerOpts", "SolveOpts", "Server", "batch_run",
+]
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def solve_opts(**kw) -> SolveOpts:
+    o = SolveOpts()
+    load().feec_solve_opts_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One device and one HIP stream (feec_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(load().feec_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().feec_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def synchronize(self):
+        check(load().feec_ctx_synchronize(self._h))
+
+    def memory(self) -> int:
+        b = C.c_int64()
+        check(load().feec_ctx_memory(self._h, C.byref(b)))
+        return b.value
+
+
+_DEFAULT_CTX = None
+
+
+def default_context() -> Context:
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+class CartesianGrid:
+    """Unit box grid of n^dim cubes, Kuhn-triangulated (CartesianGrid::new_unit)."""
+
+    def __init__(self, dim: int, n: int):
+        self.dim, self.n = dim, n
+
+    @staticmethod
+    def new_unit(dim, n):
+        return CartesianGrid(dim, n)
+
+    def counts(self):
+        nc, nv = C.c_int64(), C.c_int64()
+        check(load().feec_kuhn_counts(self.dim, self.n, C.byref(nc), C.byref(nv)))
+        return nc.value, nv.value
+
+    def triangulate(self):
+        """(cells [N_d, dim+1] int64 in generation order, coords [V, dim])."""
+        nc, nv = self.counts()
+        cells = np.zeros((nc, self.dim + 1), np.int64)
+        coords = np.zeros((nv, self.dim), np.float64)
+        check(load().feec_kuhn_grid(self.dim, self.n, _ptr(cells, _lib.i64p), _ptr(coords, _lib.f64p)))
+        return cells, coords
+
+    def jitter(self, coords, amplitude, seed, axes=None):
+        """Interior-vertex jitter amplitude*h*(2U-1) per axis (SplitMix64(seed))."""
+        coords = np.ascontiguousarray(coords, dtype=np.float64).copy()
+        mask = 0
+        for a in (range(self.dim) if axes is None else axes):
+            mask |= 1 << a
+        check(load().feec_jitter_interior(self.dim, self.n, _ptr(coords, _lib.f64p), amplitude,
+                                          seed, mask))
+        return coords
+
+
+def normal_vector(n, seed):
+    out = np.zeros(n)
+    check(load().feec_normal_vector(n, seed, _ptr(out, _lib.f64p)))
+    return out
+
+
+class Csr:
+    """A device CSR (feec_csr) and its host copy."""
+
+    def __init__(self, handle):
+        self._h = handle
+        nr, nc, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(load().feec_csr_shape(handle, C.byref(nr), C.byref(nc), C.byref(nnz)))
+        self.shape = (nr.value, nc.value)
+        self.nnz = nnz.value
+        self._host = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().feec_csr_destroy(self._h)
+            self._h = None
+
+    def host(self):
+        if self._host is None:
+            rp = np.zeros(self.shape[0] + 1, np.int64)
+            col = np.zeros(max(self.nnz, 1), np.int32)
+            val = np.zeros(max(self.nnz, 1), np.float64)
+            check(load().feec_csr_copy(self._h, _ptr(rp, _lib.i64p), _ptr(col, _lib.i32p),
+                                       _ptr(val, _lib.f64p)))
+            self._host = (rp, col[: self.nnz], val[: self.nnz])
+        return self._host
+
+    @property
+    def row_ptr(self):
+        return self.host()[0]
+
+    @property
+    def col(self):
+        return self.host()[1]
+
+    @property
+    def val(self):
+        return self.host()[2]
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp, col, val = self.host()
+        return sp.csr_matrix((val, col, rp), shape=self.shape)
+
+    def __matmul__(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.shape[0])
+        check(load().feec_csr_spmv(self._h, _ptr(x, _lib.f64p), _ptr(y, _lib.f64p)))
+        return y
+
+
+class Complex:
+    """A simplicial complex numbered on the device (Complex::from_cells)."""
+
+    def __init__(self, dim, cells, nverts=None, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        cells = np.ascontiguousarray(cells, dtype=np.int64)
+        if nverts is None:
+            nverts = int(cells.max()) + 1
+        h = C.c_void_p()
+        check(load().feec_mesh_create(self.ctx._h, dim, nverts, cells.shape[0], _ptr(cells, _lib.i64p),
+                                      C.byref(h)))
+        self._h = h
+        self.dim = dim
+        cnt = np.zeros(dim + 1, np.int64)
+        check(load().feec_mesh_counts(h, _ptr(cnt, _lib.i64p)))
+        self.counts = cnt
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().feec_mesh_destroy(self._h)
+            self._h = None
+
+    def nsimplices(self, k):
+        return int(self.counts[k]) if 0 <= k <= self.dim else 0
+
+    def skeleton(self, k):
+        out = np.zeros((self.nsimplices(k), k + 1), np.int64)
+        check(load().feec_mesh_skeleton(self._h, k, _ptr(out, _lib.i64p)))
+        return out
+
+    def cells(self):
+        return self.skeleton(self.dim)
+
+    def cell_faces(self, k):
+        from math import comb
+        out = np.zeros((self.nsimplices(self.dim), comb(self.dim + 1, k + 1)), np.int64)
+        check(load().feec_mesh_cell_faces(self._h, k, _ptr(out, _lib.i64p)))
+        return out
+
+    def boundary_flags(self, k):
+        out = np.zeros(self.nsimplices(k), np.uint8)
+        check(load().feec_mesh_boundary(self._h, k, _ptr(out, _lib.u8p)))
+        return out
+
+    def orientation(self):
+        out = np.zeros(self.nsimplices(self.dim), np.int8)
+        check(load().feec_mesh_orientation(self._h, _ptr(out, _lib.i8p)))
+        return out
+
+    def coboundary(self, k, bc=FULL):
+        h = C.c_void_p()
+        check(load().feec_mesh_coboundary(self._h, bc, k, C.byref(h)))
+        return Csr(h)
+
+
+class MeshLengthsSq:
+    """Signed squared edge lengths on the device (a Regge metric)."""
+
+    def __init__(self, complex_: Complex, handle):
+        self.complex = complex_
+        self._h = handle
+
+    @staticmethod
+    def from_lengths(complex_: Complex, lengths):
+        lengths = np.ascontiguousarray(lengths, dtype=np.float64)
+        h = C.c_void_p()
+        check(load().feec_geom_from_lengths(complex_._h, _ptr(lengths, _lib.f64p), C.byref(h)))
+        return MeshLengthsSq(complex_, h)
+
+    @staticmethod
+    def from_coords(complex_: Complex, coords, minkowski=False):
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        h = C.c_void_p()
+        check(load().feec_geom_from_coords(complex_._h, _ptr(coords, _lib.f64p),
+                                           MINKOWSKI if minkowski else EUCLIDEAN, C.byref(h)))
+        return MeshLengthsSq(complex_, h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().feec_geom_destroy(self._h)
+            self._h = None
+
+    def lengths(self):
+        out = np.zeros(self.complex.nsimplices(1))
+        check(load().feec_geom_lengths(self._h, _ptr(out, _lib.f64p)))
+        return out
+
+    def cell_metrics(self):
+        d, nc = self.complex.dim, self.complex.nsimplices(self.complex.dim)
+        g, gi, det = np.zeros((nc, d, d)), np.zeros((nc, d, d)), np.zeros(nc)
+        check(load().feec_cell_metrics(self._h, _ptr(g, _lib.f64p), _ptr(gi, _lib.f64p),
+                                       _ptr(det, _lib.f64p)))
+        return g, gi, det
+
+    def causal_census(self):
+        counts = np.zeros(3, np.int64)
+        riem = np.zeros(self.complex.nsimplices(self.complex.dim), np.uint8)
+        check(load().feec_causal_census(self._h, _ptr(counts, _lib.i64p), _ptr(riem, _lib.u8p)))
+        return counts, riem
+
+
+def element_shape(dim, kind, grade):
+    r, c = C.c_int(), C.c_int()
+    check(load().feec_element_shape(dim, kind, grade, C.byref(r), C.byref(c)))
+    return r.value, c.value
+
+
+class WhitneyComplex:
+    """HilbertComplex over the Whitney forms of a mesh, full or relative."""
+
+    def __init__(self, geometry: MeshLengthsSq, relative: bool = False):
+        self.geometry = geometry
+        self.topology = geometry.complex
+        self.relative = relative
+        self.bc = RELATIVE if relative else FULL
+        self.dim = self.topology.dim
+
+    def ndofs(self, k):
+        if not 0 <= k <= self.dim:
+            return 0
+        if self.relative:
+            return int((self.topology.boundary_flags(k) == 0).sum())
+        return self.topology.nsimplices(k)
+
+    def element(self, kind, grade):
+        r, c = element_shape(self.dim, kind, grade)
+        out = np.zeros((self.topology.nsimplices(self.dim), r, c))
+        if r * c:
+            check(load().feec_element_matrices(self.geometry._h, kind, grade, _ptr(out, _lib.f64p)))
+        return out
+
+    def assemble(self, kind, grade) -> Csr:
+        h = C.c_void_p()
+        check(load().feec_assemble(self.geometry._h, self.bc, kind, grade, C.byref(h)))
+        return Csr(h)
+
+    def mass(self, k):
+        return self.assemble(MASS, k)
+
+    def dif_trial(self, k):
+        return self.assemble(DIF_TRIAL, k)
+
+    def dif_test(self, k):
+        return self.assemble(DIF_TEST, k)
+
+    def dif_both(self, k):
+        return self.assemble(DIF_BOTH, k)
+
+    def hdif_gram(self, k):
+        return self.assemble(HDIF_GRAM, k)
+
+    def dif(self, k):
+        return self.topology.coboundary(k, self.bc)
+
+    def harmonics(self, k):
+        cnt = C.c_int64()
+        check(load().feec_harmonics(self.geometry._h, self.bc, k, C.byref(cnt), None))
+        n = self.ndofs(k)
+        out = np.zeros((max(cnt.value, 1), n))
+        if cnt.value:
+            check(load().feec_harmonics(self.geometry._h, self.bc, k, C.byref(cnt), _ptr(out, _lib.f64p)))
+        return out[: cnt.value].T.copy()
+
+    def mixed_kkt(self, k):
+        h = C.c_void_p()
+        sizes = np.zeros(3, np.int64)
+        check(load().feec_mixed_kkt(self.geometry._h, self.bc, k, C.byref(h), _ptr(sizes, _lib.i64p)))
+        return Csr(h), tuple(int(s) for s in sizes)
+
+
+def solve_source(complex_: WhitneyComplex, source_galvec, grade, opts: SolveOpts | None = None,
+                 raise_on_failure=True):
+    """elliptic::solve_source: (sigma, u, p, report) with ambient cochains."""
+    topo = complex_.topology
+    src = np.ascontiguousarray(source_galvec, dtype=np.float64)
+    assert src.shape == (topo.nsimplices(grade),)
+    sigma = np.zeros(max(topo.nsimplices(grade - 1), 1))
+    u = np.zeros(topo.nsimplices(grade))
+    p = np.zeros(8)
+    rep = Report()
+    st = load().feec_solve_source(complex_.geometry._h, complex_.bc, grade, _ptr(src, _lib.f64p),
+                                  C.byref(opts) if opts is not None else None, _ptr(sigma, _lib.f64p),
+                                  _ptr(u, _lib.f64p), _ptr(p, _lib.f64p), C.byref(rep))
+    if raise_on_failure:
+        check(st)
+    nh = complex_.harmonics(grade).shape[1] if grade in (0, complex_.dim) else 0
+    return sigma[: topo.nsimplices(grade - 1)], u, p[:nh], rep
+
+
+class Problem:
+    """A device-resident mixed Hodge-Laplace problem (pattern and Krylov state kept)."""
+
+    def __init__(self, complex_: WhitneyComplex, grade, opts: SolveOpts | None = None):
+        self.complex = complex_
+        self.grade = grade
+        h = C.c_void_p()
+        check(load().feec_problem_create(complex_.geometry._h, complex_.bc, grade,
+                                         C.byref(opts) if opts is not None else None, C.byref(h)))
+        self._h = h
+        sizes = np.zeros(3, np.int64)
+        nnz = C.c_int64()
+        check(load().feec_problem_sizes(h, _ptr(sizes, _lib.i64p), C.byref(nnz)))
+        self.sizes = tuple(int(s) for s in sizes)
+        self.nnz = nnz.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().feec_problem_destroy(self._h)
+            self._h = None
+
+    def assemble(self):
+        check(load().feec_problem_assemble(self._h))
+
+    def source_buffer(self):
+        p = C.c_void_p()
+        check(load().feec_problem_source_buffer(self._h, C.byref(p)))
+        return p
+
+    def solve(self, source=None) -> Report:
+        rep = Report()
+        if source is None:
+            check(load().feec_problem_solve_device(self._h, self.source_buffer(), C.byref(rep)))
+        else:
+            src = np.ascontiguousarray(source, dtype=np.float64)
+            check(load().feec_problem_solve(self._h, _ptr(src, _lib.f64p), C.byref(rep)))
+        return rep
+
+    def solution(self):
+        topo = self.complex.topology
+        k = self.grade
+        sigma = np.zeros(max(topo.nsimplices(k - 1), 1))
+        u = np.zeros(topo.nsimplices(k))
+        p = np.zeros(max(self.sizes[2], 1))
+        check(load().feec_problem_solution(self._h, _ptr(sigma, _lib.f64p), _ptr(u, _lib.f64p),
+                                           _ptr(p, _lib.f64p)))
+        return sigma[: topo.nsimplices(k - 1)], u, p[: self.sizes[2]]
+
+
+@dataclass
+class HodgeDirac:
+    offsets: np.ndarray
+    mass_block: Csr
+    op: Csr
+
+    @staticmethod
+    def assemble_signed(complex_: WhitneyComplex, sign):
+        mb, op = C.c_void_p(), C.c_void_p()
+        offs = np.zeros(complex_.dim + 2, np.int64)
+        check(load().feec_hodge_dirac_assemble(complex_.geometry._h, complex_.bc, sign, C.byref(mb),
+                                               C.byref(op), _ptr(offs, _lib.i64p)))
+        return HodgeDirac(offs, Csr(mb), Csr(op))
+
+    @staticmethod
+    def assemble(complex_):
+        return HodgeDirac.assemble_signed(complex_, -1.0)
+
+    @staticmethod
+    def assemble_selfadjoint(complex_):
+        return HodgeDirac.assemble_signed(complex_, 1.0)
+
+    def ndofs_total(self):
+        return int(self.offsets[-1])
+
+
+def solve_dirac_source(geometry: MeshLengthsSq, mass_term, load_field, lift, opts=None):
+    """solve_dirac_source on the relative complex; flat full-layout fields."""
+    f = np.ascontiguousarray(load_field, dtype=np.float64)
+    l = np.ascontiguousarray(lift, dtype=np.float64)
+    out = np.zeros_like(f)
+    rep = Report()
+    check(load().feec_solve_dirac_source(geometry._h, mass_term, _ptr(f, _lib.f64p), _ptr(l, _lib.f64p),
+                                         C.byref(opts) if opts is not None else None,
+                                         _ptr(out, _lib.f64p), C.byref(rep)))
+    return out, rep
+
+
+def batch_run(requests, opts: SolveOpts | None = None, ctx: Context | None = None):
+    ctx = ctx or default_context()
+    n = len(requests)
+    reqs = (Request * n)(*requests)
+    res = (Result * n)()
+    check(load().feec_batch_run(ctx._h, n, reqs, C.byref(opts) if opts is not None else None, res))
+    return list(res)
+
+
+class Server:
+    """Continuous-batching request loop (feec_server_*)."""
+
+    def __init__(self, opts: ServerOpts, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        check(load().feec_server_create(self.ctx._h, C.byref(opts), C.byref(h)))
+        self._h = h
+
+    def submit(self, req: Request) -> int:
+        t = C.c_int64()
+        check(load().feec_server_submit(self._h, C.byref(req), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        r = Result()
+        st = load().feec_server_poll(self._h, ticket, C.byref(r))
+        if st == 10:
+            return None
+        check(st)
+        return r
+
+    def solution(self, ticket):
+        n = C.c_int64()
+        check(load().feec_server_solution(self._h, ticket, None, C.byref(n)))
+        u = np.zeros(n.value)
+        check(load().feec_server_solution(self._h, ticket, _ptr(u, _lib.f64p), C.byref(n)))
+        return u
+
+    def drain(self):
+        check(load().feec_server_drain(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().feec_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

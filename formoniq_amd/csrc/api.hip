@@ -1,0 +1,978 @@
+// The C ABI of libfeec (include/feec.h): handles, host-side meshers and
+// synthetic inputs, topology and assembly queries, and the problem entry
+// points built on the device kernels of the other translation units.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <vector>
+
+#include "feec_internal.hpp"
+
+namespace feec {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+void* device_alloc(Ctx* ctx, size_t bytes) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fail(e == hipErrorOutOfMemory ? FEEC_E_OOM : FEEC_E_DEVICE,
+         "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+  }
+  ctx->bytes += (int64_t)bytes;
+  return p;
+}
+
+void device_free(Ctx* ctx, void* p, size_t bytes) {
+  (void)hipFree(p);
+  ctx->bytes -= (int64_t)bytes;
+}
+
+void* Ctx::ensure_temp(size_t b) {
+  if (b <= temp_bytes && temp) return temp;
+  if (temp) {
+    FEEC_HIP(hipStreamSynchronize(stream));
+    device_free(this, temp, temp_bytes);
+  }
+  temp_bytes = std::max<size_t>(b, 1 << 20);
+  temp = device_alloc(this, temp_bytes);
+  return temp;
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014): the generator of every synthetic input.
+struct SplitMix64 {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }  // [0,1)
+};
+
+// ---------------------------------------------------------------------------
+// Mixed Hodge-Laplace KKT problem (elliptic.rs:54-182), device resident.
+// ---------------------------------------------------------------------------
+struct Problem {
+  Geom* g = nullptr;
+  int bc = 0, k = 0;
+  feec_solve_opts opts{};
+  int64_t ns = 0, nu = 0, nh = 0;
+  int nfm = 0, nf = 0;
+  // per-cell element blocks, one allocation: mass_sigma | dif_test | dif_both | diag_lo | diag_mk
+  DBuf<double> el;
+  int64_t off[6] = {0, 0, 0, 0, 0, 0};
+  CooPlan kkt_plan, diag_plan;
+  CompactMap kkt_map;
+  DBuf<double> diag;  // 2 (ns + nu)
+  KktSystem sys;
+  SolverState* ws = nullptr;
+  DBuf<double> rhs, x, source;
+  DBuf<double> harmonics;  // [nh][nu] M-orthonormal basis
+  std::vector<double> harmonics_host;
+  feec_report last{};
+  ~Problem() { solver_state_free(ws); }
+};
+
+namespace {
+
+constexpr int TPB = 256;
+inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
+
+__global__ void k_inv_diag(const double* diag, int64_t n, int64_t nh, double* inv) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) inv[i] = 1.0 / (diag[i] + diag[n + i]);
+  else if (i < n + nh) inv[i] = 1.0;
+}
+
+__global__ void k_rhs(const double* src, const int32_t* map, int64_t nfull, int64_t ns, int64_t N,
+                      double* rhs) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nfull) return;
+  int32_t j = map ? map[i] : (int32_t)i;
+  if (j >= 0) rhs[ns + j] = src[i];
+}
+
+__global__ void k_extend(const double* x, const int32_t* map, int64_t nfull, int64_t off,
+                         double* out) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nfull) return;
+  int32_t j = map ? map[i] : (int32_t)i;
+  out[i] = (j >= 0) ? x[off + j] : 0.0;
+}
+
+__global__ void k_soa_to_aos(const double* soa, int64_t nc, int per, double* aos) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nc * per) return;
+  int64_t cell = t / per;
+  int e = (int)(t % per);
+  aos[t] = soa[(int64_t)e * nc + cell];
+}
+
+// vector contributions: entry rows[t] = base_row + map(faces[cell][i]), col 0
+__global__ void k_diag_coo(const int32_t* faces, int nf, const int32_t* map, int64_t nc, int64_t roff,
+                           int stride_diag /* element entry of (i,i) = i*stride_diag */,
+                           uint32_t base, int32_t* rows, int32_t* cols, uint32_t* srcs) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nc * nf) return;
+  int64_t cell = t / nf;
+  int i = (int)(t % nf);
+  int32_t g = faces[cell * nf + i];
+  if (map) g = map[g];
+  rows[t] = g >= 0 ? (int32_t)(g + roff) : -1;
+  cols[t] = g >= 0 ? 0 : -1;
+  srcs[t] = base + (uint32_t)((int64_t)i * stride_diag * nc + cell);
+}
+
+__global__ void k_fill_border(const double* mh, int64_t nu, int64_t ns, int64_t n, int nh,
+                              double* border) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int h = 0; h < nh; ++h) border[h * n + i] = (i >= ns) ? mh[h * nu + (i - ns)] : 0.0;
+}
+
+}  // namespace
+
+int64_t euler_characteristic(const Mesh& m) {
+  int64_t chi = 0;
+  for (int k = 0; k <= m.d; ++k) chi += (k & 1) ? -m.n[k] : m.n[k];
+  return chi;
+}
+
+// Harmonic basis on a ball (elliptic.rs:197-220): b_0 = 1 on the full complex
+// (the normalized constant), b_n = 1 on the relative complex (M_n^-1 times the
+// orientation, dirac.rs:553-568); every other space is trivial. Anything that
+// is not a ball (chi != 1) is outside what the closed forms cover.
+void harmonic_basis(Geom& g, int bc, int k, std::vector<double>& basis, int64_t& count) {
+  Mesh& m = *g.mesh;
+  count = 0;
+  basis.clear();
+  bool candidate = (bc == FEEC_FULL && k == 0) || (bc == FEEC_RELATIVE && k == m.d);
+  if (!candidate) return;
+  FEEC_REQUIRE(euler_characteristic(m) == 1, FEEC_E_UNSUPPORTED,
+               "harmonic forms are implemented in closed form for balls (Euler characteristic 1)");
+  Csr M;
+  assemble_form(g, bc, FEEC_MASS, k, M);
+  int64_t n = M.nrows;
+  std::vector<double> v(n);
+  if (bc == FEEC_FULL) {
+    std::fill(v.begin(), v.end(), 1.0);
+  } else {
+    m.ensure_orientation();
+    FEEC_REQUIRE(m.orientable, FEEC_E_TOPOLOGY, "the mesh is not orientable");
+    std::vector<int64_t> rp(n + 1);
+    std::vector<int32_t> col(M.nnz);
+    std::vector<double> val(M.nnz);
+    FEEC_HIP(hipMemcpy(rp.data(), M.row_ptr.p, (n + 1) * 8, hipMemcpyDeviceToHost));
+    FEEC_HIP(hipMemcpy(col.data(), M.col.p, M.nnz * 4, hipMemcpyDeviceToHost));
+    FEEC_HIP(hipMemcpy(val.data(), M.val.p, M.nnz * 8, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) {
+      double dgl = 0.0;
+      for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
+        if (col[p] == i) dgl = val[p];
+      v[i] = (double)m.orientation[i] / dgl;  // top-grade Whitney mass is diagonal
+    }
+  }
+  // normalize in the M_k inner product
+  DBuf<double> dv(m.ctx, n), mv(m.ctx, n);
+  FEEC_HIP(hipMemcpy(dv.p, v.data(), n * 8, hipMemcpyHostToDevice));
+  spmv64(m.ctx, M, dv.p, mv.p);
+  m.ctx->sync();
+  std::vector<double> hmv(n);
+  FEEC_HIP(hipMemcpy(hmv.data(), mv.p, n * 8, hipMemcpyDeviceToHost));
+  double q = 0.0;
+  for (int64_t i = 0; i < n; ++i) q += v[i] * hmv[i];
+  double s = 1.0 / std::sqrt(q);
+  for (auto& x : v) x *= s;
+  basis = std::move(v);
+  count = 1;
+}
+
+void problem_build(Problem& P) {
+  Geom& g = *P.g;
+  Mesh& m = *g.mesh;
+  Ctx* ctx = m.ctx;
+  const int d = m.d, k = P.k, bc = P.bc;
+  FEEC_REQUIRE(k >= 0 && k <= d, FEEC_E_ARG, "grade out of range");
+  P.ns = ndofs(m, bc, k - 1);
+  P.nu = ndofs(m, bc, k);
+  P.nfm = (k >= 1) ? binom(d + 1, k) : 0;
+  P.nf = binom(d + 1, k + 1);
+  const int64_t nc = m.n[d];
+  const int nfm = P.nfm, nf = P.nf;
+  const bool has_up = k + 1 <= d;
+  // element block layout
+  int64_t sizes[5] = {(int64_t)nfm * nfm * nc, (int64_t)nfm * nf * nc, has_up ? (int64_t)nf * nf * nc : 0,
+                      (int64_t)nfm * nc, (int64_t)nf * nc};
+  P.off[0] = 0;
+  for (int i = 0; i < 5; ++i) P.off[i + 1] = P.off[i] + sizes[i];
+  FEEC_REQUIRE(P.off[5] < (int64_t(1) << 31), FEEC_E_ARG, "problem too large for 31-bit sources");
+  P.el.alloc(ctx, std::max<int64_t>(P.off[5], 1));
+
+  // harmonic border
+  std::vector<double> hb;
+  harmonic_basis(g, bc, k, hb, P.nh);
+  P.harmonics_host = hb;
+
+  // KKT contribution plan: [[-M_{k-1}, B], [B^T, dif_both(k+1)]]
+  const int64_t n = P.ns + P.nu;
+  int64_t total = sizes[0] + 2 * sizes[1] + sizes[2];
+  {
+    DBuf<int32_t> rows(ctx, std::max<int64_t>(total, 1)), cols(ctx, std::max<int64_t>(total, 1));
+    DBuf<uint32_t> srcs(ctx, std::max<int64_t>(total, 1));
+    int64_t at = 0;
+    if (k >= 1) {
+      form_contributions(m, bc, k - 1, k - 1, nfm, nfm, 0, 0, false, (uint32_t)P.off[0], true,
+                         rows.p + at, cols.p + at, srcs.p + at);
+      at += sizes[0];
+      form_contributions(m, bc, k - 1, k, nfm, nf, 0, P.ns, false, (uint32_t)P.off[1], false,
+                         rows.p + at, cols.p + at, srcs.p + at);
+      at += sizes[1];
+      form_contributions(m, bc, k - 1, k, nfm, nf, 0, P.ns, true, (uint32_t)P.off[1], false,
+                         rows.p + at, cols.p + at, srcs.p + at);
+      at += sizes[1];
+    }
+    if (has_up) {
+      form_contributions(m, bc, k, k, nf, nf, P.ns, P.ns, false, (uint32_t)P.off[2], false,
+                         rows.p + at, cols.p + at, srcs.p + at);
+      at += sizes[2];
+    }
+    coo_plan_build(ctx, P.kkt_plan, n, n, at, rows.p, cols.p, srcs.p);
+  }
+  // diagonal plan: [diag M_{k-1} | diag M_k] then [diag dif_both(k) | diag dif_both(k+1)]
+  {
+    int64_t nd = nc * (2 * nfm + (has_up ? 2 : 1) * nf);
+    DBuf<int32_t> rows(ctx, std::max<int64_t>(nd, 1)), cols(ctx, std::max<int64_t>(nd, 1));
+    DBuf<uint32_t> srcs(ctx, std::max<int64_t>(nd, 1));
+    if (bc == FEEC_RELATIVE) m.ensure_interior();
+    const int32_t* map_lo = (bc == FEEC_RELATIVE && k >= 1) ? m.interior[k - 1].p : nullptr;
+    const int32_t* map_k = (bc == FEEC_RELATIVE) ? m.interior[k].p : nullptr;
+    int64_t at = 0;
+    if (k >= 1) {
+      k_diag_coo<<<grid_for(nc * nfm), TPB, 0, ctx->stream>>>(m.faces[k - 1].p, nfm, map_lo, nc, 0,
+                                                               nfm + 1, (uint32_t)P.off[0],
+                                                               rows.p + at, cols.p + at, srcs.p + at);
+      at += nc * nfm;
+      k_diag_coo<<<grid_for(nc * nfm), TPB, 0, ctx->stream>>>(m.faces[k - 1].p, nfm, map_lo, nc, n,
+                                                               1, (uint32_t)P.off[3], rows.p + at,
+                                                               cols.p + at, srcs.p + at);
+      at += nc * nfm;
+    }
+    k_diag_coo<<<grid_for(nc * nf), TPB, 0, ctx->stream>>>(m.faces[k].p, nf, map_k, nc, P.ns, 1,
+                                                            (uint32_t)P.off[4], rows.p + at,
+                                                            cols.p + at, srcs.p + at);
+    at += nc * nf;
+    if (has_up) {
+      k_diag_coo<<<grid_for(nc * nf), TPB, 0, ctx->stream>>>(m.faces[k].p, nf, map_k, nc, n + P.ns,
+                                                              nf + 1, (uint32_t)P.off[2],
+                                                              rows.p + at, cols.p + at, srcs.p + at);
+      at += nc * nf;
+    }
+    FEEC_HIP(hipGetLastError());
+    coo_plan_build(ctx, P.diag_plan, 2 * n, 1, at, rows.p, cols.p, srcs.p);
+  }
+  P.diag.alloc(ctx, 2 * n);
+  P.sys.ctx = ctx;
+  P.sys.n = n;
+  P.sys.nh = P.nh;
+  P.sys.inv_diag.alloc(ctx, n + P.nh);
+  P.rhs.alloc(ctx, n + P.nh);
+  P.x.alloc(ctx, n + P.nh);
+  P.source.alloc(ctx, std::max<int64_t>(m.n[k], 1));
+  if (P.nh) {
+    // border = M_k H, stored as full-length columns (zero on the sigma rows)
+    Csr M;
+    assemble_form(g, bc, FEEC_MASS, k, M);
+    DBuf<double> h(ctx, P.nu), mh(ctx, P.nu);
+    FEEC_HIP(hipMemcpy(h.p, P.harmonics_host.data(), P.nu * 8, hipMemcpyHostToDevice));
+    spmv64(ctx, M, h.p, mh.p);
+    P.sys.border.alloc(ctx, P.nh * n);
+    k_fill_border<<<grid_for(n), TPB, 0, ctx->stream>>>(mh.p, P.nu, P.ns, n, (int)P.nh,
+                                                        P.sys.border.p);
+    P.harmonics = std::move(h);
+  }
+  ctx->sync();
+}
+
+void problem_assemble(Problem& P) {
+  Geom& g = *P.g;
+  Mesh& m = *g.mesh;
+  Ctx* ctx = m.ctx;
+  double* outs[5] = {P.el.p + P.off[0], P.el.p + P.off[1], P.el.p + P.off[2], P.el.p + P.off[3],
+                     P.el.p + P.off[4]};
+  kkt_element_blocks(g, P.k, outs, true);
+  bool rebuilt = assemble_compacted(ctx, P.kkt_plan, P.el.p, P.kkt_map, P.sys.A);
+  coo_plan_dense_column(ctx, P.diag_plan, P.el.p, P.diag.p);
+  k_inv_diag<<<grid_for(P.sys.n + P.nh), TPB, 0, ctx->stream>>>(P.diag.p, P.sys.n, P.nh,
+                                                                P.sys.inv_diag.p);
+  FEEC_HIP(hipGetLastError());
+  if (rebuilt && P.ws) {  // a new pattern invalidates captured graphs
+    solver_state_free(P.ws);
+    P.ws = nullptr;
+  }
+  kkt_refresh_fp32(P.sys);
+  ctx->sync();
+}
+
+}  // namespace feec
+
+using namespace feec;
+
+struct feec_problem_s : feec::Problem {};
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* feec_last_error(void) { return g_last_error.c_str(); }
+
+const char* feec_status_string(feec_status s) {
+  switch (s) {
+    case FEEC_OK: return "ok";
+    case FEEC_E_ARG: return "invalid argument";
+    case FEEC_E_DEVICE: return "device error";
+    case FEEC_E_OOM: return "out of device memory";
+    case FEEC_E_TOPOLOGY: return "invalid topology";
+    case FEEC_E_DEGENERATE_CELL: return "degenerate cell";
+    case FEEC_E_NOT_SPD: return "not positive definite";
+    case FEEC_E_NOT_CONVERGED: return "not converged";
+    case FEEC_E_UNSUPPORTED: return "unsupported";
+    case FEEC_E_ADMISSION: return "admission refused";
+    case FEEC_E_AGAIN: return "not ready";
+  }
+  return "unknown";
+}
+
+void feec_solve_opts_default(feec_solve_opts* o) {
+  if (!o) return;
+  o->rtol = 1e-10;
+  o->max_iters = 200000;
+  o->precision = FEEC_PREC_MIXED;
+  o->max_refinements = 12;
+  o->inner_rtol = 1e-5;
+  o->use_graph = 1;
+  o->graph_iters = 32;
+}
+
+feec_status feec_ctx_create(int device, feec_ctx** out) {
+  return guard([&] {
+    FEEC_REQUIRE(out, FEEC_E_ARG, "null output");
+    int count = 0;
+    FEEC_HIP(hipGetDeviceCount(&count));
+    FEEC_REQUIRE(device >= 0 && device < count, FEEC_E_ARG, "no such device");
+    FEEC_HIP(hipSetDevice(device));
+    auto* c = new feec_ctx_s();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->dflag, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->dcount, sizeof(unsigned long long));
+    if (e != hipSuccess) {
+      if (c->stream) (void)hipStreamDestroy(c->stream);
+      delete c;
+      fail(FEEC_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    *out = c;
+  });
+}
+
+void feec_ctx_destroy(feec_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  elmat_release_tables(ctx);
+  if (ctx->temp) device_free(ctx, ctx->temp, ctx->temp_bytes);
+  (void)hipFree(ctx->dflag);
+  (void)hipFree(ctx->dcount);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+feec_status feec_ctx_synchronize(feec_ctx* ctx) {
+  return guard([&] { ctx->sync(); });
+}
+
+feec_status feec_ctx_memory(const feec_ctx* ctx, int64_t* bytes) {
+  return guard([&] { *bytes = ctx->bytes.load(); });
+}
+
+// ---- meshers --------------------------------------------------------------
+feec_status feec_kuhn_counts(int dim, int n, int64_t* ncells, int64_t* nverts) {
+  return guard([&] {
+    FEEC_REQUIRE(dim >= 1 && dim <= 4 && n >= 1, FEEC_E_ARG, "dim 1..4, n >= 1");
+    int64_t c = 1, v = 1, f = 1;
+    for (int a = 0; a < dim; ++a) {
+      c *= n;
+      v *= (n + 1);
+      f *= (a + 1);
+    }
+    if (ncells) *ncells = c * f;
+    if (nverts) *nverts = v;
+  });
+}
+
+// CartesianTopology::cell_skeleton (simplicial/src/mesher/grid.rs:77-98) and
+// CartesianGrid::vertex_pos (regge/src/mesher/cartesian.rs:158-168).
+feec_status feec_kuhn_grid(int dim, int n, int64_t* cells, double* coords) {
+  return guard([&] {
+    int64_t nc = 0, nv = 0;
+    FEEC_REQUIRE(feec_kuhn_counts(dim, n, &nc, &nv) == FEEC_OK, FEEC_E_ARG, "bad grid");
+    int64_t stride[4];
+    int64_t s = 1;
+    for (int a = 0; a < dim; ++a) {
+      stride[a] = s;
+      s *= (n + 1);
+    }
+    if (coords)
+      for (int64_t v = 0; v < nv; ++v) {
+        int64_t r = v;
+        for (int a = 0; a < dim; ++a) {
+          coords[v * dim + a] = (double)(r % (n + 1)) / (double)n;
+          r /= (n + 1);
+        }
+      }
+    if (!cells) return;
+    int64_t nbox = 1;
+    for (int a = 0; a < dim; ++a) nbox *= n;
+    std::vector<int> perm(dim);
+    int64_t out = 0;
+    for (int64_t b = 0; b < nbox; ++b) {
+      int64_t r = b, origin = 0;
+      for (int a = 0; a < dim; ++a) {
+        origin += (r % n) * stride[a];
+        r /= n;
+      }
+      std::iota(perm.begin(), perm.end(), 0);
+      do {
+        int64_t v = origin;
+        cells[out * (dim + 1)] = v;
+        for (int i = 0; i < dim; ++i) {
+          v += stride[perm[i]];
+          cells[out * (dim + 1) + i + 1] = v;
+        }
+        ++out;
+      } while (std::next_permutation(perm.begin(), perm.end()));
+    }
+  });
+}
+
+feec_status feec_jitter_interior(int dim, int n, double* coords, double amplitude, uint64_t seed,
+                                 uint32_t axis_mask) {
+  return guard([&] {
+    FEEC_REQUIRE(dim >= 1 && dim <= 4 && n >= 1 && coords, FEEC_E_ARG, "bad jitter arguments");
+    int64_t nv = 1;
+    for (int a = 0; a < dim; ++a) nv *= (n + 1);
+    SplitMix64 rng{seed};
+    const double h = 1.0 / n;
+    for (int64_t v = 0; v < nv; ++v) {
+      int64_t r = v;
+      bool interior = true;
+      for (int a = 0; a < dim; ++a) {
+        int64_t c = r % (n + 1);
+        r /= (n + 1);
+        if (c == 0 || c == n) interior = false;
+      }
+      if (!interior) continue;
+      for (int a = 0; a < dim; ++a)
+        if (axis_mask & (1u << a)) coords[v * dim + a] += amplitude * h * (2.0 * rng.uniform() - 1.0);
+    }
+  });
+}
+
+feec_status feec_normal_vector(int64_t n, uint64_t seed, double* out) {
+  return guard([&] {
+    FEEC_REQUIRE(n >= 0 && (n == 0 || out), FEEC_E_ARG, "bad output");
+    SplitMix64 rng{seed};
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int64_t i = 0; i < n; i += 2) {
+      double u1 = 1.0 - rng.uniform();  // (0, 1]
+      double u2 = rng.uniform();
+      double rad = std::sqrt(-2.0 * std::log(u1));
+      out[i] = rad * std::cos(two_pi * u2);
+      if (i + 1 < n) out[i + 1] = rad * std::sin(two_pi * u2);
+    }
+  });
+}
+
+// ---- topology ---------------------------------------------------------------
+feec_status feec_mesh_create(feec_ctx* ctx, int dim, int64_t nverts, int64_t ncells,
+                             const int64_t* cells, feec_mesh** out) {
+  return guard([&] {
+    FEEC_REQUIRE(ctx && cells && out, FEEC_E_ARG, "null argument");
+    FEEC_HIP(hipSetDevice(ctx->device));
+    auto* m = new feec_mesh_s();
+    m->ctx = ctx;
+    try {
+      mesh_build(*m, dim, nverts, ncells, cells);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+void feec_mesh_destroy(feec_mesh* m) { delete m; }
+
+feec_status feec_mesh_dim(const feec_mesh* m, int* dim) {
+  return guard([&] { *dim = m->d; });
+}
+
+feec_status feec_mesh_counts(const feec_mesh* m, int64_t* counts) {
+  return guard([&] {
+    for (int k = 0; k <= m->d; ++k) counts[k] = m->n[k];
+  });
+}
+
+feec_status feec_mesh_skeleton(const feec_mesh* mc, int k, int64_t* verts) {
+  return guard([&] {
+    feec_mesh* m = const_cast<feec_mesh*>(mc);
+    FEEC_REQUIRE(k >= 0 && k <= m->d && verts, FEEC_E_ARG, "bad grade");
+    const int d = m->d, nv = d + 1;
+    const int64_t nc = m->n[d];
+    std::vector<int32_t> cells((size_t)nc * nv);
+    FEEC_HIP(hipMemcpy(cells.data(), m->faces[0].p, cells.size() * 4, hipMemcpyDeviceToHost));
+    if (k == 0) {
+      for (int64_t v = 0; v < m->n[0]; ++v) verts[v] = v;
+      return;
+    }
+    if (k == d) {
+      for (size_t i = 0; i < cells.size(); ++i) verts[i] = cells[i];
+      return;
+    }
+    std::vector<int64_t> rep(m->n[k]);
+    FEEC_HIP(hipMemcpy(rep.data(), m->rep[k].p, rep.size() * 8, hipMemcpyDeviceToHost));
+    std::vector<std::vector<int>> subs;
+    for (unsigned msk = 0; msk < (1u << nv); ++msk) {
+      if (__builtin_popcount(msk) != k + 1) continue;
+      std::vector<int> e;
+      for (int b = 0; b < nv; ++b)
+        if (msk & (1u << b)) e.push_back(b);
+      subs.push_back(e);
+    }
+    const int nf = (int)subs.size();
+    for (int64_t s = 0; s < m->n[k]; ++s) {
+      int64_t cell = rep[s] / nf;
+      int f = (int)(rep[s] % nf);
+      for (int p = 0; p <= k; ++p) verts[s * (k + 1) + p] = cells[cell * nv + subs[f][p]];
+    }
+  });
+}
+
+feec_status feec_mesh_cell_faces(const feec_mesh* m, int k, int64_t* out) {
+  return guard([&] {
+    FEEC_REQUIRE(k >= 0 && k <= m->d && out, FEEC_E_ARG, "bad grade");
+    int64_t n = m->n[m->d] * (int64_t)binom(m->d + 1, k + 1);
+    std::vector<int32_t> h(n);
+    FEEC_HIP(hipMemcpy(h.data(), m->faces[k].p, n * 4, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) out[i] = h[i];
+  });
+}
+
+feec_status feec_mesh_boundary(const feec_mesh* m, int k, uint8_t* flags) {
+  return guard([&] {
+    FEEC_REQUIRE(k >= 0 && k <= m->d && flags, FEEC_E_ARG, "bad grade");
+    FEEC_HIP(hipMemcpy(flags, m->bnd[k].p, m->n[k], hipMemcpyDeviceToHost));
+  });
+}
+
+feec_status feec_mesh_orientation(feec_mesh* m, int8_t* signs) {
+  return guard([&] {
+    m->ensure_orientation();
+    FEEC_REQUIRE(m->orientable, FEEC_E_TOPOLOGY, "the mesh is not orientable");
+    std::copy(m->orientation.begin(), m->orientation.end(), signs);
+  });
+}
+
+feec_status feec_mesh_coboundary(feec_mesh* m, feec_bc bc, int k, feec_csr** out) {
+  return guard([&] {
+    FEEC_HIP(hipSetDevice(m->ctx->device));
+    auto* a = new feec_csr_s();
+    a->ctx = m->ctx;
+    try {
+      if (k < 0 || k >= m->d) {  // HilbertComplex::dif is total in grade
+        a->nrows = ndofs(*m, bc, k + 1);
+        a->ncols = ndofs(*m, bc, k);
+        a->nnz = 0;
+        a->row_ptr.alloc(m->ctx, a->nrows + 1);
+        FEEC_HIP(hipMemsetAsync(a->row_ptr.p, 0, (a->nrows + 1) * 8, m->ctx->stream));
+        m->ctx->sync();
+      } else if (bc == FEEC_RELATIVE) {
+        Csr full;
+        mesh_coboundary(*m, k, full);
+        m->ensure_interior();
+        csr_select(full, m->interior[k + 1].p, m->n_interior[k + 1], m->interior[k].p,
+                   m->n_interior[k], *a);
+      } else {
+        mesh_coboundary(*m, k, *a);
+      }
+    } catch (...) {
+      delete a;
+      throw;
+    }
+    *out = a;
+  });
+}
+
+// ---- geometry ---------------------------------------------------------------
+feec_status feec_geom_from_lengths(feec_mesh* m, const double* lengths, feec_geom** out) {
+  return guard([&] {
+    FEEC_REQUIRE(m && lengths && out, FEEC_E_ARG, "null argument");
+    FEEC_HIP(hipSetDevice(m->ctx->device));
+    auto* g = new feec_geom_s();
+    g->mesh = m;
+    g->lengths.alloc(m->ctx, m->n[1]);
+    FEEC_HIP(hipMemcpy(g->lengths.p, lengths, m->n[1] * 8, hipMemcpyHostToDevice));
+    *out = g;
+  });
+}
+
+feec_status feec_geom_from_coords(feec_mesh* m, const double* coords, feec_ambient amb,
+                                  feec_geom** out) {
+  return guard([&] {
+    FEEC_REQUIRE(m && coords && out, FEEC_E_ARG, "null argument");
+    FEEC_HIP(hipSetDevice(m->ctx->device));
+    auto* g = new feec_geom_s();
+    g->mesh = m;
+    g->lengths.alloc(m->ctx, m->n[1]);
+    DBuf<double> c(m->ctx, m->n[0] * m->d);
+    FEEC_HIP(hipMemcpy(c.p, coords, m->n[0] * m->d * 8, hipMemcpyHostToDevice));
+    lengths_from_coords(*m, c.p, amb == FEEC_MINKOWSKI, g->lengths.p);
+    m->ctx->sync();
+    *out = g;
+  });
+}
+
+void feec_geom_destroy(feec_geom* g) { delete g; }
+
+feec_status feec_geom_lengths(const feec_geom* g, double* out) {
+  return guard([&] {
+    FEEC_HIP(hipMemcpy(out, g->lengths.p, g->mesh->n[1] * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+feec_status feec_cell_metrics(feec_geom* g, double* gm, double* gi, double* det) {
+  return guard([&] {
+    Mesh& m = *g->mesh;
+    int64_t nc = m.n[m.d], dd = (int64_t)m.d * m.d;
+    DBuf<double> a(m.ctx, gm ? nc * dd : 1), b(m.ctx, gi ? nc * dd : 1), c(m.ctx, det ? nc : 1);
+    cell_metrics(*g, gm ? a.p : nullptr, gi ? b.p : nullptr, det ? c.p : nullptr);
+    m.ctx->sync();
+    if (gm) FEEC_HIP(hipMemcpy(gm, a.p, nc * dd * 8, hipMemcpyDeviceToHost));
+    if (gi) FEEC_HIP(hipMemcpy(gi, b.p, nc * dd * 8, hipMemcpyDeviceToHost));
+    if (det) FEEC_HIP(hipMemcpy(det, c.p, nc * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+feec_status feec_causal_census(feec_geom* g, int64_t counts[3], uint8_t* riem) {
+  return guard([&] {
+    Mesh& m = *g->mesh;
+    DBuf<uint8_t> r(m.ctx, riem ? m.n[m.d] : 1);
+    causal_census(*g, counts, riem ? r.p : nullptr);
+    if (riem) FEEC_HIP(hipMemcpy(riem, r.p, m.n[m.d], hipMemcpyDeviceToHost));
+  });
+}
+
+// ---- element matrices and assembly ---------------------------------------------
+feec_status feec_element_shape(int dim, feec_form_kind kind, int grade, int* rows, int* cols) {
+  return guard([&] {
+    FEEC_REQUIRE(kind >= FEEC_MASS && kind <= FEEC_DIF_BOTH, FEEC_E_ARG, "not an element form");
+    *rows = (grade < 0 || grade > dim) ? 0 : element_rows(dim, kind, grade);
+    *cols = (grade < 0 || grade > dim) ? 0 : element_cols(dim, kind, grade);
+  });
+}
+
+feec_status feec_element_matrices(feec_geom* g, feec_form_kind kind, int grade, double* out) {
+  return guard([&] {
+    Mesh& m = *g->mesh;
+    FEEC_REQUIRE(kind >= FEEC_MASS && kind <= FEEC_DIF_BOTH, FEEC_E_ARG, "not an element form");
+    FEEC_REQUIRE(grade >= 0 && grade <= m.d, FEEC_E_ARG, "grade out of range");
+    int r = element_rows(m.d, kind, grade), c = element_cols(m.d, kind, grade);
+    int64_t nc = m.n[m.d], total = nc * r * c;
+    if (total == 0) return;
+    DBuf<double> soa(m.ctx, total), aos(m.ctx, total);
+    element_matrices_soa(*g, kind, grade, soa.p);
+    k_soa_to_aos<<<grid_for(total), TPB, 0, m.ctx->stream>>>(soa.p, nc, r * c, aos.p);
+    FEEC_HIP(hipGetLastError());
+    m.ctx->sync();
+    FEEC_HIP(hipMemcpy(out, aos.p, total * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+feec_status feec_assemble(feec_geom* g, feec_bc bc, feec_form_kind kind, int grade,
+                          feec_csr** out) {
+  return guard([&] {
+    FEEC_REQUIRE(g && out, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(kind >= FEEC_MASS && kind <= FEEC_HDIF_GRAM, FEEC_E_ARG, "bad form kind");
+    FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
+    auto* a = new feec_csr_s();
+    try {
+      assemble_form(*g, bc, kind, grade, *a);
+    } catch (...) {
+      delete a;
+      throw;
+    }
+    *out = a;
+  });
+}
+
+feec_status feec_csr_shape(const feec_csr* a, int64_t* nrows, int64_t* ncols, int64_t* nnz) {
+  return guard([&] {
+    if (nrows) *nrows = a->nrows;
+    if (ncols) *ncols = a->ncols;
+    if (nnz) *nnz = a->nnz;
+  });
+}
+
+feec_status feec_csr_copy(const feec_csr* a, int64_t* row_ptr, int32_t* col, double* val) {
+  return guard([&] {
+    if (row_ptr) FEEC_HIP(hipMemcpy(row_ptr, a->row_ptr.p, (a->nrows + 1) * 8, hipMemcpyDeviceToHost));
+    if (col && a->nnz) FEEC_HIP(hipMemcpy(col, a->col.p, a->nnz * 4, hipMemcpyDeviceToHost));
+    if (val && a->nnz) FEEC_HIP(hipMemcpy(val, a->val.p, a->nnz * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+feec_status feec_csr_spmv(feec_csr* a, const double* x, double* y) {
+  return guard([&] {
+    DBuf<double> dx(a->ctx, std::max<int64_t>(a->ncols, 1)), dy(a->ctx, std::max<int64_t>(a->nrows, 1));
+    FEEC_HIP(hipMemcpy(dx.p, x, a->ncols * 8, hipMemcpyHostToDevice));
+    spmv64(a->ctx, *a, dx.p, dy.p);
+    a->ctx->sync();
+    FEEC_HIP(hipMemcpy(y, dy.p, a->nrows * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+void feec_csr_destroy(feec_csr* a) { delete a; }
+
+// ---- problems ---------------------------------------------------------------------
+feec_status feec_harmonics(feec_geom* g, feec_bc bc, int grade, int64_t* count, double* basis) {
+  return guard([&] {
+    std::vector<double> hb;
+    int64_t c = 0;
+    harmonic_basis(*g, bc, grade, hb, c);
+    *count = c;
+    if (basis && c) std::copy(hb.begin(), hb.end(), basis);
+  });
+}
+
+feec_status feec_mixed_kkt(feec_geom* g, feec_bc bc, int grade, feec_csr** out, int64_t sizes[3]) {
+  return guard([&] {
+    Problem P;
+    P.g = g;
+    P.bc = bc;
+    P.k = grade;
+    feec_solve_opts_default(&P.opts);
+    problem_build(P);
+    problem_assemble(P);
+    Ctx* ctx = g->mesh->ctx;
+    auto* a = new feec_csr_s();
+    a->ctx = ctx;
+    if (P.nh == 0) {
+      a->nrows = P.sys.A.nrows;
+      a->ncols = P.sys.A.ncols;
+      a->nnz = P.sys.A.nnz;
+      a->row_ptr = std::move(P.sys.A.row_ptr);
+      a->col = std::move(P.sys.A.col);
+      a->val = std::move(P.sys.A.val);
+    } else {
+      // the border as dense CSR blocks, every entry kept (elliptic.rs:62-84)
+      int64_t n = P.sys.n, nh = P.nh, nu = P.nu;
+      std::vector<double> bh(nh * n);
+      FEEC_HIP(hipMemcpy(bh.data(), P.sys.border.p, nh * n * 8, hipMemcpyDeviceToHost));
+      Csr col_blk, row_blk;
+      std::vector<int64_t> rp(nu + 1), rp2(nh + 1);
+      std::vector<int32_t> c1(nu * nh), c2(nh * nu);
+      std::vector<double> v1(nu * nh), v2(nh * nu);
+      for (int64_t i = 0; i < nu; ++i) {
+        rp[i] = i * nh;
+        for (int64_t h = 0; h < nh; ++h) {
+          c1[i * nh + h] = (int32_t)h;
+          v1[i * nh + h] = bh[h * n + P.ns + i];
+        }
+      }
+      rp[nu] = nu * nh;
+      for (int64_t h = 0; h < nh; ++h) {
+        rp2[h] = h * nu;
+        for (int64_t i = 0; i < nu; ++i) {
+          c2[h * nu + i] = (int32_t)i;
+          v2[h * nu + i] = bh[h * n + P.ns + i];
+        }
+      }
+      rp2[nh] = nh * nu;
+      auto up = [&](Csr& c, int64_t r, int64_t cc, std::vector<int64_t>& rpv,
+                    std::vector<int32_t>& cv, std::vector<double>& vv) {
+        c.ctx = ctx;
+        c.nrows = r;
+        c.ncols = cc;
+        c.nnz = (int64_t)cv.size();
+        c.row_ptr.alloc(ctx, r + 1);
+        c.col.alloc(ctx, c.nnz);
+        c.val.alloc(ctx, c.nnz);
+        FEEC_HIP(hipMemcpy(c.row_ptr.p, rpv.data(), (r + 1) * 8, hipMemcpyHostToDevice));
+        FEEC_HIP(hipMemcpy(c.col.p, cv.data(), c.nnz * 4, hipMemcpyHostToDevice));
+        FEEC_HIP(hipMemcpy(c.val.p, vv.data(), c.nnz * 8, hipMemcpyHostToDevice));
+      };
+      up(col_blk, nu, nh, rp, c1, v1);
+      up(row_blk, nh, nu, rp2, c2, v2);
+      // [[A_ss, A_su, 0],[A_us, A_uu, MH],[0, (MH)^T, 0]] from the KKT plus the border
+      BlockRef blocks[4] = {{&P.sys.A, 1.0, false}, {nullptr, 1.0, false}, {nullptr, 1.0, false},
+                            {nullptr, 1.0, false}};
+      (void)blocks;
+      // place the border blocks at rows/cols offset by building a 3x3 grid over (n_s+n_u, nh)
+      Csr border_col, border_row;
+      // pad the u-only border to the full KKT row range [0, n)
+      std::vector<int64_t> rp3(n + 1);
+      std::vector<int32_t> c3;
+      std::vector<double> v3;
+      for (int64_t i = 0; i < n; ++i) {
+        rp3[i] = (int64_t)c3.size();
+        if (i >= P.ns)
+          for (int64_t h = 0; h < nh; ++h) {
+            c3.push_back((int32_t)h);
+            v3.push_back(bh[h * n + i]);
+          }
+      }
+      rp3[n] = (int64_t)c3.size();
+      std::vector<int64_t> rp4(nh + 1);
+      std::vector<int32_t> c4;
+      std::vector<double> v4;
+      for (int64_t h = 0; h < nh; ++h) {
+        rp4[h] = (int64_t)c4.size();
+        for (int64_t i = P.ns; i < n; ++i) {
+          c4.push_back((int32_t)i);
+          v4.push_back(bh[h * n + i]);
+        }
+      }
+      rp4[nh] = (int64_t)c4.size();
+      up(border_col, n, nh, rp3, c3, v3);
+      up(border_row, nh, n, rp4, c4, v4);
+      BlockRef grid[4] = {{&P.sys.A, 1.0, false}, {&border_col, 1.0, false},
+                          {&border_row, 1.0, false}, {nullptr, 1.0, false}};
+      int64_t rs[2] = {n, nh};
+      csr_block(ctx, 2, grid, rs, rs, *a);
+    }
+    sizes[0] = P.ns;
+    sizes[1] = P.nu;
+    sizes[2] = P.nh;
+    *out = a;
+  });
+}
+
+feec_status feec_problem_create(feec_geom* g, feec_bc bc, int grade, const feec_solve_opts* opts,
+                                feec_problem** out) {
+  return guard([&] {
+    FEEC_REQUIRE(g && out, FEEC_E_ARG, "null argument");
+    FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
+    auto* P = new feec_problem_s();
+    P->g = g;
+    P->bc = bc;
+    P->k = grade;
+    if (opts)
+      P->opts = *opts;
+    else
+      feec_solve_opts_default(&P->opts);
+    try {
+      problem_build(*P);
+      problem_assemble(*P);
+    } catch (...) {
+      delete P;
+      throw;
+    }
+    *out = P;
+  });
+}
+
+void feec_problem_destroy(feec_problem* p) { delete p; }
+
+feec_status feec_problem_assemble(feec_problem* p) {
+  return guard([&] { problem_assemble(*p); });
+}
+
+feec_status feec_problem_source_buffer(feec_problem* p, double** ptr) {
+  return guard([&] { *ptr = p->source.p; });
+}
+
+feec_status feec_problem_sizes(const feec_problem* p, int64_t sizes[3], int64_t* nnz) {
+  return guard([&] {
+    sizes[0] = p->ns;
+    sizes[1] = p->nu;
+    sizes[2] = p->nh;
+    if (nnz) *nnz = p->sys.A.nnz;
+  });
+}
+
+feec_status feec_problem_solve_device(feec_problem* p, const double* src, feec_report* report) {
+  return guard([&] {
+    Mesh& m = *p->g->mesh;
+    Ctx* ctx = m.ctx;
+    const int64_t N = p->sys.n + p->nh;
+    FEEC_HIP(hipMemsetAsync(p->rhs.p, 0, N * 8, ctx->stream));
+    if (p->bc == FEEC_RELATIVE) m.ensure_interior();
+    const int32_t* map = (p->bc == FEEC_RELATIVE) ? m.interior[p->k].p : nullptr;
+    k_rhs<<<grid_for(m.n[p->k]), TPB, 0, ctx->stream>>>(src, map, m.n[p->k], p->ns, N, p->rhs.p);
+    FEEC_HIP(hipGetLastError());
+    feec_report r = kkt_solve(p->sys, p->rhs.p, p->x.p, p->opts, &p->ws);
+    p->last = r;
+    if (report) *report = r;
+    FEEC_REQUIRE(r.converged, FEEC_E_NOT_CONVERGED,
+                 "Krylov budget spent before the residual reached rtol");
+  });
+}
+
+feec_status feec_problem_solve(feec_problem* p, const double* src, feec_report* report) {
+  return guard([&] {
+    Mesh& m = *p->g->mesh;
+    FEEC_HIP(hipMemcpy(p->source.p, src, m.n[p->k] * 8, hipMemcpyHostToDevice));
+    feec_status s = feec_problem_solve_device(p, p->source.p, report);
+    if (s != FEEC_OK) fail(s, g_last_error);
+  });
+}
+
+feec_status feec_problem_solution(feec_problem* p, double* sigma, double* u, double* ph) {
+  return guard([&] {
+    Mesh& m = *p->g->mesh;
+    Ctx* ctx = m.ctx;
+    const int k = p->k;
+    if (p->bc == FEEC_RELATIVE) m.ensure_interior();
+    if (sigma && k >= 1) {
+      DBuf<double> o(ctx, m.n[k - 1]);
+      const int32_t* map = (p->bc == FEEC_RELATIVE) ? m.interior[k - 1].p : nullptr;
+      k_extend<<<grid_for(m.n[k - 1]), TPB, 0, ctx->stream>>>(p->x.p, map, m.n[k - 1], 0, o.p);
+      ctx->sync();
+      FEEC_HIP(hipMemcpy(sigma, o.p, m.n[k - 1] * 8, hipMemcpyDeviceToHost));
+    }
+    if (u) {
+      DBuf<double> o(ctx, m.n[k]);
+      const int32_t* map = (p->bc == FEEC_RELATIVE) ? m.interior[k].p : nullptr;
+      k_extend<<<grid_for(m.n[k]), TPB, 0, ctx->stream>>>(p->x.p, map, m.n[k], p->ns, o.p);
+      ctx->sync();
+      FEEC_HIP(hipMemcpy(u, o.p, m.n[k] * 8, hipMemcpyDeviceToHost));
+    }
+    if (ph && p->nh) FEEC_HIP(hipMemcpy(ph, p->x.p + p->sys.n, p->nh * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+feec_status feec_solve_source(feec_geom* g, feec_bc bc, int grade, const double* src,
+                              const feec_solve_opts* opts, double* sigma, double* u, double* ph,
+                              feec_report* report) {
+  feec_problem* p = nullptr;
+  feec_status s = feec_problem_create(g, bc, grade, opts, &p);
+  if (s != FEEC_OK) return s;
+  s = feec_problem_solve(p, src, report);
+  if (s == FEEC_OK || s == FEEC_E_NOT_CONVERGED) {
+    feec_status s2 = feec_problem_solution(p, sigma, u, ph);
+    if (s == FEEC_OK) s = s2;
+  }
+  std::string err = g_last_error;
+  feec_problem_destroy(p);
+  g_last_error = err;
+  return s;
+}
+
+}  // extern "C"

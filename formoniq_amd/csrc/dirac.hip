@@ -1,0 +1,266 @@
+// Hodge-Dirac operator on the device and its source problem.
+//
+// Reference behaviour (luiswirth/formoniq, crates/):
+//   HodgeDirac::assemble_signed   formoniq/src/problems/dirac.rs:213-258
+//   solve_dirac_source            formoniq/src/problems/dirac.rs:473-534
+//   top_harmonic                  formoniq/src/problems/dirac.rs:553-568
+//   solve_bordered                formoniq/src/problems/dirac.rs:578-598
+//
+// mass_block is the block diagonal of the grade masses M_k; op carries
+// U_k = M_k D_{k-1} (global CSR x CSR product, as the reference) below the
+// diagonal and sign * U_k^T above it. The source problem (the reference
+// factors it with sparse LU) is solved here with MINRES on the symmetric
+// (self-adjoint sign) system, preconditioned by the inverse absolute diagonal
+// of the mass block, bordered with the top-grade harmonic when m = 0.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "feec_internal.hpp"
+
+namespace feec {
+
+void assemble_form(Geom& g, int bc, int kind, int grade, Csr& out);
+
+namespace {
+constexpr int TPB = 256;
+inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
+
+__global__ void k_abs_inv(const double* d, int64_t n, double* out) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = d[i] != 0.0 ? 1.0 / fabs(d[i]) : 1.0;
+}
+}  // namespace
+
+struct Dirac {
+  std::vector<int64_t> offsets;
+  std::vector<Csr> masses;
+  Csr mass_block, op;
+};
+
+static void coboundary_bc(Mesh& m, int bc, int k, Csr& out) {
+  out.ctx = m.ctx;
+  if (bc == FEEC_RELATIVE) {
+    Csr full;
+    mesh_coboundary(m, k, full);
+    m.ensure_interior();
+    csr_select(full, m.interior[k + 1].p, m.n_interior[k + 1], m.interior[k].p, m.n_interior[k],
+               out);
+  } else {
+    mesh_coboundary(m, k, out);
+  }
+}
+
+void dirac_assemble(Geom& g, int bc, double sign, Dirac& D) {
+  Mesh& m = *g.mesh;
+  Ctx* ctx = m.ctx;
+  const int d = m.d;
+  D.masses.resize(d + 1);
+  D.offsets.assign(d + 2, 0);
+  for (int k = 0; k <= d; ++k) {
+    assemble_form(g, bc, FEEC_MASS, k, D.masses[k]);
+    D.offsets[k + 1] = D.offsets[k] + D.masses[k].nrows;
+  }
+  std::vector<Csr> U(d + 1);
+  for (int k = 1; k <= d; ++k) {
+    Csr dif;
+    coboundary_bc(m, bc, k - 1, dif);
+    csr_mul(D.masses[k], dif, U[k]);
+  }
+  const int nb = d + 1;
+  std::vector<int64_t> sizes(nb);
+  for (int k = 0; k <= d; ++k) sizes[k] = D.masses[k].nrows;
+  std::vector<BlockRef> mb(nb * nb, BlockRef{nullptr, 1.0, false});
+  for (int k = 0; k <= d; ++k) mb[k * nb + k] = BlockRef{&D.masses[k], 1.0, false};
+  csr_block(ctx, nb, mb.data(), sizes.data(), sizes.data(), D.mass_block);
+  std::vector<BlockRef> ob(nb * nb, BlockRef{nullptr, 1.0, false});
+  for (int k = 1; k <= d; ++k) {
+    ob[k * nb + (k - 1)] = BlockRef{&U[k], 1.0, false};
+    ob[(k - 1) * nb + k] = BlockRef{&U[k], sign, true};
+  }
+  csr_block(ctx, nb, ob.data(), sizes.data(), sizes.data(), D.op);
+}
+
+}  // namespace feec
+
+using namespace feec;
+
+extern "C" {
+
+feec_status feec_hodge_dirac_assemble(feec_geom* g, feec_bc bc, double sign, feec_csr** mass_block,
+                                      feec_csr** op, int64_t* offsets) {
+  return guard([&] {
+    FEEC_REQUIRE(g, FEEC_E_ARG, "null geometry");
+    FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
+    Dirac D;
+    dirac_assemble(*g, bc, sign, D);
+    if (offsets)
+      for (size_t i = 0; i < D.offsets.size(); ++i) offsets[i] = D.offsets[i];
+    if (mass_block) {
+      auto* a = new feec_csr_s();
+      static_cast<Csr&>(*a) = std::move(D.mass_block);
+      *mass_block = a;
+    }
+    if (op) {
+      auto* a = new feec_csr_s();
+      static_cast<Csr&>(*a) = std::move(D.op);
+      *op = a;
+    }
+  });
+}
+
+feec_status feec_solve_dirac_source(feec_geom* g, double mass_term, const double* load,
+                                    const double* lift, const feec_solve_opts* opts, double* out,
+                                    feec_report* report) {
+  return guard([&] {
+    Mesh& m = *g->mesh;
+    Ctx* ctx = m.ctx;
+    FEEC_HIP(hipSetDevice(ctx->device));
+    feec_solve_opts o;
+    if (opts)
+      o = *opts;
+    else
+      feec_solve_opts_default(&o);
+    o.precision = FEEC_PREC_FP64;  // the reference solve is a direct FP64 factorization
+    const int d = m.d;
+    Dirac D;
+    dirac_assemble(*g, FEEC_FULL, 1.0, D);
+    const int64_t nfull = D.offsets[d + 1];
+    // S = A + m M (full), as one COO sum
+    Csr S;
+    if (mass_term != 0.0) {
+      int64_t sz[1] = {nfull};
+      BlockRef scaled{&D.mass_block, mass_term, false};
+      Csr mm;
+      csr_block(ctx, 1, &scaled, sz, sz, mm);
+      csr_add(D.op, mm, S);
+    } else {
+      int64_t sz[1] = {nfull};
+      BlockRef same{&D.op, 1.0, false};
+      csr_block(ctx, 1, &same, sz, sz, S);
+    }
+    // the flat inclusion E over all grades
+    m.ensure_interior();
+    std::vector<int32_t> map(nfull);
+    int64_t nrel = 0;
+    for (int k = 0; k <= d; ++k) {
+      std::vector<int32_t> ik(m.n[k]);
+      FEEC_HIP(hipMemcpy(ik.data(), m.interior[k].p, m.n[k] * 4, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < m.n[k]; ++i)
+        map[D.offsets[k] + i] = ik[i] >= 0 ? (int32_t)(nrel + ik[i]) : -1;
+      nrel += m.n_interior[k];
+    }
+    DBuf<int32_t> dmap(ctx, nfull);
+    FEEC_HIP(hipMemcpy(dmap.p, map.data(), nfull * 4, hipMemcpyHostToDevice));
+    // rhs = E^T (f - S lift)
+    std::vector<double> f(load, load + nfull), l(lift, lift + nfull), sl(nfull);
+    DBuf<double> dl(ctx, nfull), dsl(ctx, nfull);
+    FEEC_HIP(hipMemcpy(dl.p, l.data(), nfull * 8, hipMemcpyHostToDevice));
+    spmv64(ctx, S, dl.p, dsl.p);
+    ctx->sync();
+    FEEC_HIP(hipMemcpy(sl.data(), dsl.p, nfull * 8, hipMemcpyDeviceToHost));
+    Csr Srel;
+    csr_select(S, dmap.p, nrel, dmap.p, nrel, Srel);
+    std::vector<double> rhs(nrel, 0.0);
+    for (int64_t i = 0; i < nfull; ++i)
+      if (map[i] >= 0) rhs[map[i]] = f[i] - sl[i];
+    // m = 0: border with the top harmonic h = M_n^-1 o when it is in the kernel
+    std::vector<double> h;
+    bool bordered = false;
+    if (mass_term == 0.0) {
+      m.ensure_orientation();
+      if (m.orientable) {
+        Csr Mn;
+        assemble_form(*g, FEEC_RELATIVE, FEEC_MASS, d, Mn);
+        std::vector<double> dg(Mn.nrows);
+        DBuf<double> dd(ctx, Mn.nrows);
+        csr_diagonal(Mn, dd.p);
+        ctx->sync();
+        FEEC_HIP(hipMemcpy(dg.data(), dd.p, Mn.nrows * 8, hipMemcpyDeviceToHost));
+        h.assign(nrel, 0.0);
+        int64_t off = nrel - m.n_interior[d];
+        for (int64_t c = 0; c < m.n_interior[d]; ++c) h[off + c] = m.orientation[c] / dg[c];
+        DBuf<double> dh(ctx, nrel), sh(ctx, nrel);
+        FEEC_HIP(hipMemcpy(dh.p, h.data(), nrel * 8, hipMemcpyHostToDevice));
+        spmv64(ctx, Srel, dh.p, sh.p);
+        ctx->sync();
+        std::vector<double> shh(nrel);
+        FEEC_HIP(hipMemcpy(shh.data(), sh.p, nrel * 8, hipMemcpyDeviceToHost));
+        double a = 0.0, b = 0.0;
+        for (int64_t i = 0; i < nrel; ++i) {
+          a += shh[i] * shh[i];
+          b += h[i] * h[i];
+        }
+        bordered = std::sqrt(a) <= 1e-8 * std::sqrt(b);
+      }
+    }
+    KktSystem sys;
+    sys.ctx = ctx;
+    int64_t n = nrel + (bordered ? 1 : 0);
+    sys.n = n;
+    sys.nh = 0;
+    if (bordered) {
+      // [[S, h], [h^T, 0]] with the nonzero entries of h (solve_bordered)
+      std::vector<int64_t> rp(nrel + 1), rp2(2);
+      std::vector<int32_t> c1, c2;
+      std::vector<double> v1, v2;
+      for (int64_t i = 0; i < nrel; ++i) {
+        rp[i] = (int64_t)c1.size();
+        if (h[i] != 0.0) {
+          c1.push_back(0);
+          v1.push_back(h[i]);
+          c2.push_back((int32_t)i);
+          v2.push_back(h[i]);
+        }
+      }
+      rp[nrel] = (int64_t)c1.size();
+      rp2[0] = 0;
+      rp2[1] = (int64_t)c2.size();
+      auto up = [&](Csr& c, int64_t r, int64_t cc, std::vector<int64_t>& rpv,
+                    std::vector<int32_t>& cv, std::vector<double>& vv) {
+        c.ctx = ctx;
+        c.nrows = r;
+        c.ncols = cc;
+        c.nnz = (int64_t)cv.size();
+        c.row_ptr.alloc(ctx, r + 1);
+        c.col.alloc(ctx, c.nnz);
+        c.val.alloc(ctx, c.nnz);
+        FEEC_HIP(hipMemcpy(c.row_ptr.p, rpv.data(), (r + 1) * 8, hipMemcpyHostToDevice));
+        FEEC_HIP(hipMemcpy(c.col.p, cv.data(), c.nnz * 4, hipMemcpyHostToDevice));
+        FEEC_HIP(hipMemcpy(c.val.p, vv.data(), c.nnz * 8, hipMemcpyHostToDevice));
+      };
+      Csr bc_col, bc_row;
+      up(bc_col, nrel, 1, rp, c1, v1);
+      up(bc_row, 1, nrel, rp2, c2, v2);
+      BlockRef grid[4] = {{&Srel, 1.0, false}, {&bc_col, 1.0, false}, {&bc_row, 1.0, false},
+                          {nullptr, 1.0, false}};
+      int64_t rs[2] = {nrel, 1};
+      csr_block(ctx, 2, grid, rs, rs, sys.A);
+      rhs.push_back(0.0);
+    } else {
+      sys.A = std::move(Srel);
+    }
+    // preconditioner: inverse absolute diagonal of the relative mass block
+    Csr Mrel;
+    csr_select(D.mass_block, dmap.p, nrel, dmap.p, nrel, Mrel);
+    DBuf<double> md(ctx, n);
+    FEEC_HIP(hipMemsetAsync(md.p, 0, n * 8, ctx->stream));
+    csr_diagonal(Mrel, md.p);
+    sys.inv_diag.alloc(ctx, n);
+    k_abs_inv<<<grid_for(n), TPB, 0, ctx->stream>>>(md.p, n, sys.inv_diag.p);
+    FEEC_HIP(hipGetLastError());
+    DBuf<double> b(ctx, n), x(ctx, n);
+    ctx->sync();
+    FEEC_HIP(hipMemcpy(b.p, rhs.data(), n * 8, hipMemcpyHostToDevice));
+    SolverState* ws = nullptr;
+    feec_report r = kkt_solve(sys, b.p, x.p, o, &ws);
+    solver_state_free(ws);
+    if (report) *report = r;
+    std::vector<double> xs(n);
+    FEEC_HIP(hipMemcpy(xs.data(), x.p, n * 8, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nfull; ++i) out[i] = l[i] + (map[i] >= 0 ? xs[map[i]] : 0.0);
+    FEEC_REQUIRE(r.converged, FEEC_E_NOT_CONVERGED, "Dirac source solve did not converge");
+  });
+}
+
+}  // extern "C"

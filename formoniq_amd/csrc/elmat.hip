@@ -1,0 +1,676 @@
+// Per-cell Regge metric and Whitney element matrices, FP64, one thread per cell.
+//
+// Reference behaviour (luiswirth/formoniq, crates/):
+//   SimplexLengthsSq::metric       regge/src/lengths/simplex.rs:308-326
+//   Metric::{det, dual}            metric/src/lib.rs:179-190, 250-257
+//   cell_volume                    regge/src/lib.rs:26-28
+//   multiform_metric               metric/src/tensor.rs:127-131
+//   HodgeMass::element             formoniq/src/operators.rs:84-94
+//   WhitneyExpansion::pullback     derham/src/interpolate/form.rs:195-232
+//   WhitneyPairing::element        formoniq/src/operators.rs:201-211
+//
+// Formulation. The reference forms H = P (Lambda^k g^-1) P^T with P the k-th
+// compound of the barycentric differentials. By Cauchy-Binet this is the k-th
+// compound of the barycentric Gram matrix G = Dlambda g^-1 Dlambda^T, so each
+// H entry is one k x k minor of the (d+1) x (d+1) matrix G, which is g^-1
+// bordered by its negated row/column sums. The pullback sums (k+1)^2 signed H
+// entries per mass entry with weights 1 or 2 (the barycentric Gramian
+// Q = (1 + delta)/((d+1)(d+2))), and the exterior derivative sides are +-1
+// incidences, so every element block -- mass, dif_trial, dif_test, dif_both --
+// is a fixed integer-coefficient linear map of the symmetric H, times
+// vol (k!)^2/((d+1)(d+2)). Those maps are built once on the host in exact
+// integer arithmetic.
+//
+// Kernel shape: one wavefront per group of 64 cells, one lane per cell. The
+// lane computes g, det g, g^-1 and G in registers, stages G and then the
+// minors H of each grade in LDS (layout [entry][lane], so a wavefront's LDS
+// access is one contiguous 512-byte row), and emits every requested block by
+// walking the coefficient tables (wave-uniform, read through the scalar
+// cache). Element matrices are written structure-of-arrays,
+// out[entry * ncells + cell], so each store instruction of a wavefront covers
+// 64 consecutive doubles.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "feec_internal.hpp"
+
+namespace feec {
+
+namespace {
+
+constexpr int TPB = 256;
+
+constexpr int popc(unsigned m) {
+  int c = 0;
+  while (m) {
+    c += m & 1u;
+    m >>= 1;
+  }
+  return c;
+}
+
+constexpr int colex_rank(unsigned m) {
+  int r = 0, i = 0;
+  for (int e = 0; e < 32; ++e)
+    if (m & (1u << e)) {
+      r += binom(e, i + 1);
+      ++i;
+    }
+  return r;
+}
+
+constexpr double factorial(int n) { return n <= 1 ? 1.0 : n * factorial(n - 1); }
+
+template <int K>
+__device__ __forceinline__ double det(const double (&a)[K > 0 ? K : 1][K > 0 ? K : 1]) {
+  if constexpr (K == 0) {
+    return 1.0;
+  } else if constexpr (K == 1) {
+    return a[0][0];
+  } else if constexpr (K == 2) {
+    return a[0][0] * a[1][1] - a[0][1] * a[1][0];
+  } else if constexpr (K == 3) {
+    return a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+           a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  } else {
+    static_assert(K == 4, "K <= 4");
+    // expansion along row 0 with 2x2 minors of rows 2,3 shared
+    double s01 = a[2][0] * a[3][1] - a[2][1] * a[3][0];
+    double s02 = a[2][0] * a[3][2] - a[2][2] * a[3][0];
+    double s03 = a[2][0] * a[3][3] - a[2][3] * a[3][0];
+    double s12 = a[2][1] * a[3][2] - a[2][2] * a[3][1];
+    double s13 = a[2][1] * a[3][3] - a[2][3] * a[3][1];
+    double s23 = a[2][2] * a[3][3] - a[2][3] * a[3][2];
+    double c0 = a[1][1] * s23 - a[1][2] * s13 + a[1][3] * s12;
+    double c1 = a[1][0] * s23 - a[1][2] * s03 + a[1][3] * s02;
+    double c2 = a[1][0] * s13 - a[1][1] * s03 + a[1][3] * s01;
+    double c3 = a[1][0] * s12 - a[1][1] * s02 + a[1][2] * s01;
+    return a[0][0] * c0 - a[0][1] * c1 + a[0][2] * c2 - a[0][3] * c3;
+  }
+}
+
+// Metric by polarization, det and inverse by the adjugate.
+template <int D>
+struct CellGeom {
+  double g[D][D];
+  double ginv[D][D];
+  double det;
+};
+
+__device__ __forceinline__ constexpr int edge_index(int i, int j) { return i + j * (j - 1) / 2; }
+
+template <int D>
+__device__ __forceinline__ void cell_geometry(const double* s, CellGeom<D>& cg) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) cg.g[i][i] = s[edge_index(0, i + 1)];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = i + 1; j < D; ++j) {
+      double v = 0.5 * (s[edge_index(0, i + 1)] + s[edge_index(0, j + 1)] -
+                        s[edge_index(i + 1, j + 1)]);
+      cg.g[i][j] = v;
+      cg.g[j][i] = v;
+    }
+  cg.det = det<D>(cg.g);
+  const double inv = 1.0 / cg.det;
+  if constexpr (D == 1) {
+    cg.ginv[0][0] = inv;
+  } else {
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        double m[D - 1][D - 1];
+#pragma unroll
+        for (int a = 0; a < D - 1; ++a)
+#pragma unroll
+          for (int b = 0; b < D - 1; ++b) m[a][b] = cg.g[a < i ? a : a + 1][b < j ? b : b + 1];
+        double cof = det<D - 1>(m);
+        cg.ginv[j][i] = (((i + j) & 1) ? -cof : cof) * inv;
+      }
+  }
+}
+
+// Barycentric Gram G = Dlambda g^-1 Dlambda^T, Dlambda row 0 = -1, rows 1.. = I.
+template <int D>
+__device__ __forceinline__ void bary_gram(const double (&gi)[D][D], double (&G)[D + 1][D + 1]) {
+  double colsum[D];
+  double total = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    double c = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) c += gi[i][j];
+    colsum[j] = c;
+    total += c;
+  }
+  G[0][0] = total;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    G[0][j + 1] = -colsum[j];
+    G[j + 1][0] = -colsum[j];  // g^-1 is symmetric
+#pragma unroll
+    for (int i = 0; i < D; ++i) G[i + 1][j + 1] = gi[i][j];
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// coefficient tables (host)
+// ---------------------------------------------------------------------------
+// Block kinds: 0 mass, 1 dif_trial, 2 dif_test, 3 dif_both, 4 diag(mass), 5 diag(dif_both)
+constexpr int KIND_DIAG_MASS = 4, KIND_DIAG_BOTH = 5;
+
+struct HostBlock {
+  int K = 0, kind = 0, rows = 0, cols = 0;
+  std::vector<int32_t> start;  // nout + 1
+  std::vector<uint16_t> h;
+  std::vector<int16_t> c;
+};
+
+static std::vector<unsigned> masks_of(int n, int size) {
+  std::vector<unsigned> out;
+  if (size < 0 || size > n) return out;
+  for (unsigned m = 0; m < (1u << n); ++m)
+    if (popc(m) == size) out.push_back(m);
+  return out;
+}
+static int elem_of(unsigned m, int p) {
+  for (int e = 0; e < 32; ++e)
+    if (m & (1u << e)) {
+      if (p == 0) return e;
+      --p;
+    }
+  return -1;
+}
+// index of the symmetric pair (a, b), a <= b, row-major over the upper triangle
+static int sym_index(int a, int b, int nb) {
+  if (a > b) std::swap(a, b);
+  return a * nb - a * (a - 1) / 2 + (b - a);
+}
+
+using LinH = std::map<int, long>;  // H index -> integer coefficient
+
+static HostBlock make_block(int D, int K, int kind) {
+  const int nv = D + 1;
+  auto blades = masks_of(nv, K);
+  auto dofs = masks_of(nv, K + 1);
+  const int nb = (int)blades.size(), nd = (int)dofs.size();
+  auto blade_idx = [&](unsigned m) {
+    for (int i = 0; i < nb; ++i)
+      if (blades[i] == m) return i;
+    return -1;
+  };
+  // Whitney mass M[s][t] as a combination of H entries (form.rs:195-232)
+  std::vector<LinH> M((size_t)nd * nd);
+  for (int i = 0; i < nd; ++i)
+    for (int j = 0; j < nd; ++j) {
+      LinH acc;
+      for (int p = 0; p <= K; ++p)
+        for (int q = 0; q <= K; ++q) {
+          int va = elem_of(dofs[i], p), vb = elem_of(dofs[j], q);
+          int a = blade_idx(dofs[i] & ~(1u << va)), b = blade_idx(dofs[j] & ~(1u << vb));
+          long w = (((p + q) & 1) ? -1 : 1) * (va == vb ? 2 : 1);
+          acc[sym_index(a, b, nb)] += w;
+        }
+      M[(size_t)i * nd + j] = acc;
+    }
+  // local boundary d_K: rows = K-subsets (faces), cols = dofs (simplex.rs:223-238)
+  std::vector<int> bd((size_t)nb * nd, 0);
+  if (K >= 1)
+    for (int s = 0; s < nd; ++s)
+      for (int p = 0; p <= K; ++p) {
+        int f = blade_idx(dofs[s] & ~(1u << elem_of(dofs[s], p)));
+        bd[(size_t)f * nd + s] = (p & 1) ? -1 : 1;
+      }
+  auto add = [](LinH& dst, const LinH& src, long w) {
+    for (auto& kv : src) dst[kv.first] += w * kv.second;
+  };
+  HostBlock hb;
+  hb.K = K;
+  hb.kind = kind;
+  std::vector<LinH> out;
+  if (kind == 0 || kind == KIND_DIAG_MASS) {
+    hb.rows = nd;
+    hb.cols = nd;
+    out = M;
+  } else if (kind == 1) {  // M d^T
+    hb.rows = nd;
+    hb.cols = nb;
+    out.assign((size_t)nd * nb, {});
+    for (int s = 0; s < nd; ++s)
+      for (int f = 0; f < nb; ++f)
+        for (int t = 0; t < nd; ++t)
+          if (bd[(size_t)f * nd + t]) add(out[(size_t)s * nb + f], M[(size_t)s * nd + t], bd[(size_t)f * nd + t]);
+  } else if (kind == 2) {  // d M
+    hb.rows = nb;
+    hb.cols = nd;
+    out.assign((size_t)nb * nd, {});
+    for (int f = 0; f < nb; ++f)
+      for (int t = 0; t < nd; ++t)
+        for (int s = 0; s < nd; ++s)
+          if (bd[(size_t)f * nd + s]) add(out[(size_t)f * nd + t], M[(size_t)s * nd + t], bd[(size_t)f * nd + s]);
+  } else {  // d M d^T
+    hb.rows = nb;
+    hb.cols = nb;
+    out.assign((size_t)nb * nb, {});
+    for (int f = 0; f < nb; ++f)
+      for (int g = 0; g < nb; ++g)
+        for (int s = 0; s < nd; ++s)
+          for (int t = 0; t < nd; ++t) {
+            long w = (long)bd[(size_t)f * nd + s] * bd[(size_t)g * nd + t];
+            if (w) add(out[(size_t)f * nb + g], M[(size_t)s * nd + t], w);
+          }
+  }
+  std::vector<int> pick;
+  if (kind == KIND_DIAG_MASS || kind == KIND_DIAG_BOTH) {
+    for (int i = 0; i < hb.rows; ++i) pick.push_back(i * hb.cols + i);
+    hb.cols = 1;
+  } else {
+    for (int e = 0; e < hb.rows * hb.cols; ++e) pick.push_back(e);
+  }
+  hb.start.push_back(0);
+  for (int e : pick) {
+    for (auto& kv : out[e])
+      if (kv.second != 0) {
+        hb.h.push_back((uint16_t)kv.first);
+        hb.c.push_back((int16_t)kv.second);
+      }
+    hb.start.push_back((int32_t)hb.h.size());
+  }
+  return hb;
+}
+
+// Device-resident tables of one (d): the minor pair lists of every grade and
+// the coefficient tables of every (grade, kind) block.
+struct DevTables {
+  DBuf<uint8_t> pairs;   // per grade: nH * 2K vertex positions
+  DBuf<int32_t> start;
+  DBuf<uint16_t> h;
+  DBuf<int16_t> c;
+  int pair_off[5] = {0, 0, 0, 0, 0};
+  int nH[5] = {0, 0, 0, 0, 0};
+  int block_start_off[5][6];  // offset into `start` of each block table
+  int block_term_base[5][6];  // offset added to start[] values
+  int block_nout[5][6];
+};
+
+static std::mutex g_tab_mu;
+static std::map<std::pair<Ctx*, int>, std::unique_ptr<DevTables>> g_tabs;
+
+}  // namespace
+
+void elmat_release_tables(Ctx* ctx) {
+  std::lock_guard<std::mutex> lock(g_tab_mu);
+  for (auto it = g_tabs.begin(); it != g_tabs.end();)
+    it = (it->first.first == ctx) ? g_tabs.erase(it) : std::next(it);
+}
+
+namespace {
+
+static DevTables& tables_for(Ctx* ctx, int D) {
+  std::lock_guard<std::mutex> lock(g_tab_mu);
+  auto key = std::make_pair(ctx, D);
+  auto it = g_tabs.find(key);
+  if (it != g_tabs.end()) return *it->second;
+  auto T = std::make_unique<DevTables>();
+  std::vector<uint8_t> pairs;
+  std::vector<int32_t> start;
+  std::vector<uint16_t> hs;
+  std::vector<int16_t> cs;
+  const int nv = D + 1;
+  for (int K = 0; K <= D; ++K) {
+    auto blades = masks_of(nv, K);
+    int nb = (int)blades.size();
+    T->pair_off[K] = (int)pairs.size();
+    int cnt = 0;
+    for (int a = 0; a < nb; ++a)
+      for (int b = a; b < nb; ++b) {
+        for (int p = 0; p < K; ++p) pairs.push_back((uint8_t)elem_of(blades[a], p));
+        for (int p = 0; p < K; ++p) pairs.push_back((uint8_t)elem_of(blades[b], p));
+        ++cnt;
+      }
+    T->nH[K] = cnt;
+    for (int kind = 0; kind < 6; ++kind) {
+      T->block_nout[K][kind] = 0;
+      T->block_start_off[K][kind] = (int)start.size();
+      T->block_term_base[K][kind] = (int)hs.size();
+      if (K == 0 && kind != 0 && kind != KIND_DIAG_MASS) {
+        start.push_back(0);
+        continue;
+      }
+      HostBlock hb = make_block(D, K, kind);
+      for (int32_t s : hb.start) start.push_back(s + (int32_t)hs.size());
+      T->block_nout[K][kind] = (int)hb.start.size() - 1;
+      hs.insert(hs.end(), hb.h.begin(), hb.h.end());
+      cs.insert(cs.end(), hb.c.begin(), hb.c.end());
+    }
+  }
+  T->pairs.alloc(ctx, std::max<size_t>(pairs.size(), 1));
+  T->start.alloc(ctx, start.size());
+  T->h.alloc(ctx, std::max<size_t>(hs.size(), 1));
+  T->c.alloc(ctx, std::max<size_t>(cs.size(), 1));
+  FEEC_HIP(hipMemcpy(T->pairs.p, pairs.data(), pairs.size(), hipMemcpyHostToDevice));
+  FEEC_HIP(hipMemcpy(T->start.p, start.data(), start.size() * 4, hipMemcpyHostToDevice));
+  FEEC_HIP(hipMemcpy(T->h.p, hs.data(), hs.size() * 2, hipMemcpyHostToDevice));
+  FEEC_HIP(hipMemcpy(T->c.p, cs.data(), cs.size() * 2, hipMemcpyHostToDevice));
+  auto& slot = g_tabs[key];
+  slot = std::move(T);
+  return *slot;
+}
+
+// A launch request: the grades whose minors are needed and the blocks emitted.
+struct JobBlock {
+  int K;
+  int nout;
+  int start_off;  // into the start table
+  double* out;    // SoA [nout][ncells]
+};
+struct Job {
+  int ngrades;
+  int grades[5];
+  int pair_off[5];
+  int nH[5];
+  int nblocks;
+  JobBlock b[12];
+};
+
+constexpr int WAVE = 64;
+
+template <int K>
+__device__ __forceinline__ double minor_det(const double* sG, int nv, const uint8_t* pr, int lane) {
+  if constexpr (K == 0) {
+    return 1.0;
+  } else {
+    double m[K][K];
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+      for (int b = 0; b < K; ++b) m[a][b] = sG[(pr[a] * nv + pr[K + b]) * WAVE + lane];
+    return det<K>(m);
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(WAVE) k_elmat(const int32_t* __restrict__ edges,
+                                                const double* __restrict__ len, int64_t nc, Job job,
+                                                const uint8_t* __restrict__ pairs,
+                                                const int32_t* __restrict__ tstart,
+                                                const uint16_t* __restrict__ th,
+                                                const int16_t* __restrict__ tc, int* degenerate) {
+  constexpr int NV = D + 1;
+  constexpr int NE = binom(D + 1, 2);
+  extern __shared__ double lds[];
+  double* sG = lds;                 // [NV*NV][WAVE]
+  double* sH = lds + NV * NV * WAVE;  // [nH][WAVE]
+  const int lane = threadIdx.x;
+  const int64_t cell = blockIdx.x * (int64_t)WAVE + lane;
+  const bool active = cell < nc;
+  double s[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) s[e] = active ? len[edges[cell * NE + e]] : ((e < D) ? 1.0 : 2.0);
+  CellGeom<D> cg;
+  cell_geometry<D>(s, cg);
+  if (active && cg.det == 0.0) atomicOr(degenerate, 1);
+  const double vol = (1.0 / factorial(D)) * sqrt(fabs(cg.det));
+  double G[NV][NV];
+  bary_gram<D>(cg.ginv, G);
+#pragma unroll
+  for (int a = 0; a < NV; ++a)
+#pragma unroll
+    for (int b = 0; b < NV; ++b) sG[(a * NV + b) * WAVE + lane] = G[a][b];
+  for (int gi = 0; gi < job.ngrades; ++gi) {
+    const int K = job.grades[gi];
+    const uint8_t* pr = pairs + job.pair_off[gi];
+    const int nH = job.nH[gi];
+    for (int t = 0; t < nH; ++t, pr += 2 * K) {
+      double v;
+      switch (K) {
+        case 0: v = 1.0; break;
+        case 1: v = minor_det<1>(sG, NV, pr, lane); break;
+        case 2: v = minor_det<2>(sG, NV, pr, lane); break;
+        case 3: v = minor_det<3>(sG, NV, pr, lane); break;
+        default: v = minor_det<4>(sG, NV, pr, lane); break;
+      }
+      sH[t * WAVE + lane] = v;
+    }
+    const double kf = factorial(K);
+    const double scale = vol * (kf * kf / double((D + 1) * (D + 2)));
+    for (int bi = 0; bi < job.nblocks; ++bi) {
+      if (job.b[bi].K != K) continue;
+      const int32_t* st = tstart + job.b[bi].start_off;
+      double* out = job.b[bi].out;
+      const int nout = job.b[bi].nout;
+      for (int e = 0; e < nout; ++e) {
+        double acc = 0.0;
+        const int t1 = st[e + 1];
+        for (int t = st[e]; t < t1; ++t) acc += (double)tc[t] * sH[th[t] * WAVE + lane];
+        if (active) out[(int64_t)e * nc + cell] = scale * acc;
+      }
+    }
+  }
+}
+template <int D>
+__global__ void k_metrics(const int32_t* __restrict__ edges, const double* __restrict__ len,
+                          int64_t nc, double* g, double* ginv, double* det) {
+  int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (cell >= nc) return;
+  constexpr int NE = binom(D + 1, 2);
+  double s[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) s[e] = len[edges[cell * NE + e]];
+  CellGeom<D> cg;
+  cell_geometry<D>(s, cg);
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      if (g) g[cell * D * D + i * D + j] = cg.g[i][j];
+      if (ginv) ginv[cell * D * D + i * D + j] = cg.ginv[i][j];
+    }
+  if (det) det[cell] = cg.det;
+}
+
+// Riemannian iff the Cholesky factorization of g succeeds (metric/src/lib.rs:203-205).
+template <int D>
+__global__ void k_riemannian(const int32_t* __restrict__ edges, const double* __restrict__ len,
+                             int64_t nc, uint8_t* riem) {
+  int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (cell >= nc) return;
+  constexpr int NE = binom(D + 1, 2);
+  double s[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) s[e] = len[edges[cell * NE + e]];
+  CellGeom<D> cg;
+  cell_geometry<D>(s, cg);
+  double L[D][D] = {};
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    double dsum = cg.g[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) dsum -= L[j][k] * L[j][k];
+    if (!(dsum > 0.0)) ok = false;
+    double ljj = sqrt(fmax(dsum, 0.0));
+    L[j][j] = ljj;
+#pragma unroll
+    for (int i = j + 1; i < D; ++i) {
+      double v = cg.g[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+      L[i][j] = ljj > 0.0 ? v / ljj : 0.0;
+    }
+  }
+  riem[cell] = ok ? 1 : 0;
+}
+
+__global__ void k_census(const double* len, int64_t ne, unsigned long long* counts) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double s = len[e];
+  atomicAdd(&counts[s < 0.0 ? 0 : (s == 0.0 ? 1 : 2)], 1ull);
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
+
+void check_degenerate(Ctx* ctx, int* deg) {
+  int h = 0;
+  FEEC_HIP(hipMemcpyAsync(&h, deg, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  FEEC_REQUIRE(h == 0, FEEC_E_DEGENERATE_CELL, "a cell metric is degenerate (det g == 0)");
+}
+
+}  // namespace
+
+int element_rows(int d, int kind, int grade) {
+  bool test_dif = (kind == 2 || kind == 3);
+  return test_dif ? binom(d + 1, grade) : binom(d + 1, grade + 1);
+}
+int element_cols(int d, int kind, int grade) {
+  bool trial_dif = (kind == 1 || kind == 3);
+  return trial_dif ? binom(d + 1, grade) : binom(d + 1, grade + 1);
+}
+
+namespace {
+
+// Launch k_elmat for a job on the mesh of g.
+void run_job(Geom& g, Job& job, bool check) {
+  Mesh& m = *g.mesh;
+  Ctx* ctx = m.ctx;
+  const int D = m.d;
+  int64_t nc = m.n[D];
+  DevTables& T = tables_for(ctx, D);
+  int maxH = 1;
+  for (int i = 0; i < job.ngrades; ++i) {
+    int K = job.grades[i];
+    job.pair_off[i] = T.pair_off[K];
+    job.nH[i] = T.nH[K];
+    maxH = std::max(maxH, T.nH[K]);
+  }
+  size_t lds = (size_t)((D + 1) * (D + 1) + maxH) * WAVE * sizeof(double);
+  int* deg = ctx->dflag;
+  FEEC_HIP(hipMemsetAsync(deg, 0, sizeof(int), ctx->stream));
+  unsigned grid = (unsigned)std::max<int64_t>(1, (nc + WAVE - 1) / WAVE);
+  const int32_t* edges = m.faces[1].p;
+  switch (D) {
+    case 1: k_elmat<1><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, T.pairs.p, T.start.p, T.h.p, T.c.p, deg); break;
+    case 2: k_elmat<2><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, T.pairs.p, T.start.p, T.h.p, T.c.p, deg); break;
+    case 3: k_elmat<3><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, T.pairs.p, T.start.p, T.h.p, T.c.p, deg); break;
+    case 4: k_elmat<4><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, T.pairs.p, T.start.p, T.h.p, T.c.p, deg); break;
+  }
+  FEEC_HIP(hipGetLastError());
+  if (check) check_degenerate(ctx, deg);
+}
+
+void job_add_grade(Job& job, int K) {
+  for (int i = 0; i < job.ngrades; ++i)
+    if (job.grades[i] == K) return;
+  job.grades[job.ngrades++] = K;
+}
+
+void job_add_block(Job& job, DevTables& T, int K, int kind, double* out) {
+  FEEC_REQUIRE(job.nblocks < 12, FEEC_E_ARG, "too many element blocks in one launch");
+  job_add_grade(job, K);
+  JobBlock& b = job.b[job.nblocks++];
+  b.K = K;
+  b.nout = T.block_nout[K][kind];
+  b.start_off = T.block_start_off[K][kind];
+  b.out = out;
+}
+
+}  // namespace
+
+void element_matrices_soa(Geom& g, int kind, int grade, double* out_dev) {
+  Mesh& m = *g.mesh;
+  DevTables& T = tables_for(m.ctx, m.d);
+  Job job{};
+  job_add_block(job, T, grade, kind, out_dev);
+  run_job(g, job, true);
+}
+
+// outs: mass_sigma (M_{k-1}), dif_test(k), dif_both(k+1), diag(dif_both(k)), diag(M_k)
+void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check) {
+  Mesh& m = *g.mesh;
+  const int d = m.d, k = grade;
+  DevTables& T = tables_for(m.ctx, d);
+  Job job{};
+  if (k >= 1) {
+    job_add_block(job, T, k - 1, 0, outs[0]);
+    job_add_block(job, T, k, 2, outs[1]);
+    job_add_block(job, T, k, KIND_DIAG_BOTH, outs[3]);
+  }
+  job_add_block(job, T, k, KIND_DIAG_MASS, outs[4]);
+  if (k + 1 <= d) job_add_block(job, T, k + 1, 3, outs[2]);
+  run_job(g, job, check);
+}
+
+int dirac_element_count(int d) {
+  int total = 0;
+  for (int k = 0; k <= d; ++k) {
+    total += binom(d + 1, k + 1) * binom(d + 1, k + 1);
+    if (k >= 1) total += binom(d + 1, k + 1) * binom(d + 1, k);
+  }
+  return total;
+}
+
+// All-grade Hodge-Dirac element matrices: M_k (k = 0..d) and dif_trial(k)
+// (k = 1..d), back to back per cell in entry-major SoA order.
+void dirac_element_matrices(Geom& g, double* out_dev, bool check) {
+  Mesh& m = *g.mesh;
+  const int d = m.d;
+  const int64_t nc = m.n[d];
+  DevTables& T = tables_for(m.ctx, d);
+  Job job{};
+  int64_t off = 0;
+  for (int k = 0; k <= d; ++k) {
+    job_add_block(job, T, k, 0, out_dev + off * nc);
+    off += T.block_nout[k][0];
+    if (k >= 1) {
+      job_add_block(job, T, k, 1, out_dev + off * nc);
+      off += T.block_nout[k][1];
+    }
+  }
+  run_job(g, job, check);
+}
+
+void cell_metrics(Geom& g, double* gmat, double* ginv, double* det) {
+  Mesh& m = *g.mesh;
+  Ctx* ctx = m.ctx;
+  int64_t nc = m.n[m.d];
+  const int32_t* edges = m.faces[1].p;
+  switch (m.d) {
+    case 1: k_metrics<1><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, g.lengths.p, nc, gmat, ginv, det); break;
+    case 2: k_metrics<2><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, g.lengths.p, nc, gmat, ginv, det); break;
+    case 3: k_metrics<3><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, g.lengths.p, nc, gmat, ginv, det); break;
+    case 4: k_metrics<4><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, g.lengths.p, nc, gmat, ginv, det); break;
+  }
+  FEEC_HIP(hipGetLastError());
+}
+
+void causal_census(Geom& g, int64_t counts[3], uint8_t* riem_dev) {
+  Mesh& m = *g.mesh;
+  Ctx* ctx = m.ctx;
+  DBuf<unsigned long long> c(ctx, 3);
+  FEEC_HIP(hipMemsetAsync(c.p, 0, 3 * sizeof(unsigned long long), ctx->stream));
+  k_census<<<grid_for(m.n[1]), TPB, 0, ctx->stream>>>(g.lengths.p, m.n[1], c.p);
+  if (riem_dev) {
+    int64_t nc = m.n[m.d];
+    const int32_t* edges = m.faces[1].p;
+    switch (m.d) {
+      case 1: k_riemannian<1><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, g.lengths.p, nc, riem_dev); break;
+      case 2: k_riemannian<2><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, g.lengths.p, nc, riem_dev); break;
+      case 3: k_riemannian<3><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, g.lengths.p, nc, riem_dev); break;
+      case 4: k_riemannian<4><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, g.lengths.p, nc, riem_dev); break;
+    }
+  }
+  unsigned long long h[3];
+  FEEC_HIP(hipMemcpyAsync(h, c.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  for (int i = 0; i < 3; ++i) counts[i] = (int64_t)h[i];
+}
+
+}  // namespace feec

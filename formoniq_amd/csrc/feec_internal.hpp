@@ -1,0 +1,278 @@
+// Internal declarations shared by the libfeec translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/feec.h"
+
+namespace feec {
+
+// ---------------------------------------------------------------------------
+// errors: exceptions inside, feec_status at the C boundary
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  feec_status status;
+  Error(feec_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+void set_last_error(const std::string& msg);
+
+[[noreturn]] inline void fail(feec_status s, const std::string& m) { throw Error(s, m); }
+
+#define FEEC_HIP(call)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      ::feec::fail(e_ == hipErrorOutOfMemory ? FEEC_E_OOM : FEEC_E_DEVICE,                  \
+                   std::string(#call) + ": " + hipGetErrorString(e_));                      \
+  } while (0)
+
+#define FEEC_REQUIRE(cond, status, msg) \
+  do {                                      \
+    if (!(cond)) ::feec::fail(status, msg); \
+  } while (0)
+
+template <class F>
+feec_status guard(F&& f) {
+  try {
+    f();
+    return FEEC_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return FEEC_E_OOM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return FEEC_E_ARG;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small combinatorics shared by host and device
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  int r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// device memory
+// ---------------------------------------------------------------------------
+struct Ctx;
+
+void* device_alloc(Ctx* ctx, size_t bytes);
+void device_free(Ctx* ctx, void* p, size_t bytes);
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  Ctx* ctx = nullptr;
+  DBuf() = default;
+  DBuf(Ctx* c, size_t count) { alloc(c, count); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), ctx(o.ctx) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      n = o.n;
+      ctx = o.ctx;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  void alloc(Ctx* c, size_t count) {
+    reset();
+    ctx = c;
+    n = count;
+    p = static_cast<T*>(device_alloc(c, (count ? count : 1) * sizeof(T)));
+  }
+  void reset() {
+    if (p) device_free(ctx, p, (n ? n : 1) * sizeof(T));
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { reset(); }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::atomic<int64_t> bytes{0};
+  // rocPRIM temporary storage, grown on demand
+  void* temp = nullptr;
+  size_t temp_bytes = 0;
+  void* ensure_temp(size_t bytes);
+  // small device scalars owned by the context: an int flag and a counter
+  int* dflag = nullptr;
+  unsigned long long* dcount = nullptr;
+  void sync() { FEEC_HIP(hipStreamSynchronize(stream)); }
+};
+
+// Device COO -> CSR plan: the sorted order of (row, col) keys, kept so that a
+// later value pass is a segmented sum (duplicates summed in input order).
+struct CooPlan {
+  int64_t nrows = 0, ncols = 0;
+  int64_t nnz = 0;        // unique (row, col) entries
+  DBuf<int64_t> row_ptr;  // nrows+1
+  DBuf<int32_t> col;      // nnz
+  DBuf<int64_t> seg;      // nnz+1: range of each entry in `src`
+  DBuf<uint32_t> src;     // sorted contributor -> index into the value array
+};
+
+struct Csr {
+  Ctx* ctx = nullptr;
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  DBuf<int64_t> row_ptr;
+  DBuf<int32_t> col;
+  DBuf<double> val;
+};
+
+struct Mesh {
+  Ctx* ctx = nullptr;
+  int d = 0;
+  int64_t n[5] = {0, 0, 0, 0, 0};
+  // faces[k]: [ncells][C(d+1,k+1)] global index of each local face, k < d;
+  // faces[0] are the (sorted) cell vertices.
+  DBuf<int32_t> faces[5];
+  // rep[k]: one (cell * nf + f) slot per k-simplex, 0 < k < d, decoding its vertices.
+  DBuf<int64_t> rep[5];
+  DBuf<uint8_t> bnd[5];
+  // interior renumbering for the relative complex: -1 on the boundary
+  DBuf<int32_t> interior[5];
+  int64_t n_interior[5] = {0, 0, 0, 0, 0};
+  bool have_interior = false;
+  std::vector<int8_t> orientation;  // host, lazily
+  bool have_orientation = false;
+  bool orientable = false;
+  // cached assembly plans keyed by (bc, row grade, col grade)
+  std::map<std::tuple<int, int, int>, std::unique_ptr<CooPlan>> plans;
+  std::mutex mu;
+
+  int nfaces(int k) const { return binom(d + 1, k + 1); }
+  void ensure_interior();
+  void ensure_orientation();
+};
+
+struct Geom {
+  Mesh* mesh = nullptr;
+  DBuf<double> lengths;  // N_1 signed squared edge lengths
+};
+
+// ---------------------------------------------------------------------------
+// module entry points (internal)
+// ---------------------------------------------------------------------------
+// topology.hip
+void mesh_build(Mesh& m, int d, int64_t nverts, int64_t ncells, const int64_t* cells);
+void mesh_coboundary(Mesh& m, int k, Csr& out);
+void lengths_from_coords(Mesh& m, const double* coords_dev, int minkowski, double* out_dev);
+
+// assemble.hip
+void coo_plan_build(Ctx* ctx, CooPlan& plan, int64_t nrows, int64_t ncols, int64_t n,
+                    const int32_t* rows, const int32_t* cols, const uint32_t* srcs);
+// Segmented sum of vals[src] per plan entry; zero_drop removes entries whose
+// contributors are all exactly 0.0 (the reference's `val != 0.0` push rule).
+void coo_plan_apply(Ctx* ctx, const CooPlan& plan, const double* vals, bool zero_drop, Csr& out);
+// Sparse utilities on device CSRs (nalgebra-sparse semantics).
+void csr_transpose(const Csr& a, Csr& out);
+void csr_add(const Csr& a, const Csr& b, Csr& out);
+void csr_mul(const Csr& a, const Csr& b, Csr& out);
+void csr_select(const Csr& a, const int32_t* row_map, int64_t nrows, const int32_t* col_map,
+                int64_t ncols, Csr& out);
+struct BlockRef {
+  const Csr* a;
+  double scale;
+  bool transpose;
+};
+void csr_block(Ctx* ctx, int nb, const BlockRef* blocks, const int64_t* row_sizes,
+               const int64_t* col_sizes, Csr& out);
+void csr_diagonal(const Csr& a, double* diag_dev);
+int64_t coo_plan_values(Ctx* ctx, const CooPlan& plan, const double* vals, double* out,
+                        uint8_t* keep);
+struct CompactMap {
+  DBuf<uint8_t> keep;
+  DBuf<int64_t> pos;
+  bool valid = false;
+};
+bool assemble_compacted(Ctx* ctx, const CooPlan& plan, const double* vals, CompactMap& cm,
+                        Csr& A);
+// Values of a plan whose columns are all 0 scattered to a dense vector of
+// plan.nrows entries (rows without contributions get 0).
+void coo_plan_dense_column(Ctx* ctx, const CooPlan& plan, const double* vals, double* out);
+void form_contributions(Mesh& m, int bc, int rg, int cg, int r, int c, int64_t roff, int64_t coff,
+                        bool transpose, uint32_t base, bool negate, int32_t* rows, int32_t* cols,
+                        uint32_t* srcs);
+int64_t ndofs(Mesh& m, int bc, int k);
+
+// elmat.hip
+// Element matrices of one pairing, SoA [entry][cell].
+void element_matrices_soa(Geom& g, int kind, int grade, double* out_dev);
+int element_rows(int d, int kind, int grade);
+int element_cols(int d, int kind, int grade);
+void cell_metrics(Geom& g, double* gmat, double* ginv, double* det);
+void elmat_release_tables(Ctx* ctx);
+void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check);
+int dirac_element_count(int d);
+void dirac_element_matrices(Geom& g, double* out_dev, bool check);
+void causal_census(Geom& g, int64_t counts[3], uint8_t* riem_dev);
+
+// assembly driver
+void assemble_form(Geom& g, int bc, int kind, int grade, Csr& out);
+
+// krylov.hip
+struct KrylovStats {
+  int64_t iters = 0;
+  double precond_residual = 1.0;
+  bool converged = false;
+};
+struct SolverState;  // opaque per-problem Krylov workspace
+struct KktSystem {
+  Ctx* ctx = nullptr;
+  int64_t n = 0;        // KKT size without the harmonic border
+  int64_t nh = 0;       // harmonic border columns
+  Csr A;                // FP64 KKT (sigma,u blocks)
+  DBuf<float> val32;    // FP32 copy of A's values
+  DBuf<double> inv_diag;
+  DBuf<float> inv_diag32;
+  DBuf<double> border;  // [nh][n] dense border columns (M_k H) placed at u rows
+};
+void kkt_refresh_fp32(KktSystem& s);
+// Solve s x = b (b, x device, FP64) to the options' tolerance.
+feec_report kkt_solve(KktSystem& s, const double* b_dev, double* x_dev, const feec_solve_opts& o,
+                      SolverState** ws);
+void solver_state_free(SolverState* ws);
+void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y);
+KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
+                     double rtol, int64_t max_iters);
+
+}  // namespace feec
+
+struct feec_ctx_s : feec::Ctx {};
+struct feec_mesh_s : feec::Mesh {};
+struct feec_geom_s : feec::Geom {};
+struct feec_csr_s : feec::Csr {};

@@ -1,0 +1,758 @@
+// Krylov solvers on the device: CSR SpMV (FP64/FP32), preconditioned MINRES
+// with every scalar of the recurrence kept in device memory, FP64 iterative
+// refinement around an FP32 MINRES, and Jacobi-preconditioned CG.
+//
+// Reference behaviour (luiswirth/formoniq, crates/):
+//   minres          iterative/src/krylov.rs:113-211 (Paige-Saunders, x0 = 0,
+//                   stops on phibar/beta1 <= rtol)
+//   cg              iterative/src/krylov.rs:48-95
+//   Jacobi          iterative/src/precond.rs:53-123
+//   BlockDiagonal   iterative/src/precond.rs:138-167
+//   solve_source    formoniq/src/problems/elliptic.rs:132-182 (rtol 1e-10)
+//
+// One MINRES iteration is five launches that never return to the host:
+//   spmv      yn = A (y/beta) - (beta/oldb) r1       + partial alpha = v.yn
+//   alpha     alpha = sum of partials (fixed order)
+//   update    yn -= (alpha/beta) r2, z = P yn         + partial beta^2 = yn.z
+//   rotate    Givens rotation and w-update coefficients (one thread)
+//   wx        w_new = (v - oldeps w2 - delta w)/gamma, x += phi w_new
+// Reductions go through a fixed grid of per-block partials summed in a fixed
+// order, so a run is bit-reproducible. The buffer roles rotate with period two
+// (r1/r2, y/z, w/w2), so an even number of iterations is captured once into a
+// hipGraph and replayed; a `done` flag in the scalar block turns the tail of a
+// segment into no-ops once the tolerance is met, and the host reads the flag
+// only between segments.
+//
+// Mixed precision: the matrix values and every Krylov vector are FP32, SpMV
+// accumulates in FP32; the orthogonalisation scalars (alpha, beta, the
+// rotation) and their dot products are accumulated in FP64. The outer loop
+// computes the residual b - A x with the FP64 matrix and FP64 x, and stops on
+// the true relative residual.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "feec_internal.hpp"
+
+namespace feec {
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int NBLK = 1024;  // fixed reduction grid: 4 blocks per CU on 256 CUs
+constexpr int LANES = 8;    // lanes per CSR row in the SpMV
+
+struct MinresS {
+  double beta, oldb, beta1, dbar, epsln, phibar, cs, sn;
+  double alfa, residual;
+  double invb_prev, oldeps, delta, invg, phi;
+  double rtol;
+  double extra_alpha;
+  long long iter, max_iters;
+  int done;
+  int pad;
+};
+
+template <class T>
+__device__ __forceinline__ double to_d(T v) {
+  return (double)v;
+}
+
+__device__ __forceinline__ double block_reduce(double v, double* sh) {
+  // wavefront (64-lane) tree, then across the 4 wavefronts
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
+  }
+  return r;  // valid on thread 0
+}
+
+// y = A x, LANES lanes per row, accumulation in T
+template <class T>
+__global__ void __launch_bounds__(TPB) k_spmv(const int64_t* __restrict__ rp,
+                                              const int32_t* __restrict__ col,
+                                              const T* __restrict__ val, int64_t n,
+                                              const T* __restrict__ x, T* __restrict__ y) {
+  const int lane = threadIdx.x % LANES;
+  const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
+  for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
+       row += groups) {
+    T acc = 0;
+    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LANES) acc += val[p] * x[col[p]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    if (lane == 0) y[row] = acc;
+  }
+}
+
+// r = b - A x in FP64 with partial |r|^2
+__global__ void __launch_bounds__(TPB) k_residual(const int64_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ col,
+                                                  const double* __restrict__ val, int64_t n,
+                                                  const double* __restrict__ x,
+                                                  const double* __restrict__ b,
+                                                  double* __restrict__ r, double* part) {
+  __shared__ double sh[TPB / 64];
+  const int lane = threadIdx.x % LANES;
+  const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
+  double loc = 0.0;
+  for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
+       row += groups) {
+    double acc = 0.0;
+    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LANES) acc += val[p] * x[col[p]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    if (lane == 0) {
+      double ri = b[row] - acc;
+      r[row] = ri;
+      loc += ri * ri;
+    }
+  }
+  double t = block_reduce(loc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// sum of NBLK partials in a fixed order (one block)
+__global__ void k_finish_sum(const double* part, int np, double* out) {
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+  double t = block_reduce(v, sh);
+  if (threadIdx.x == 0) *out = t;
+}
+
+template <class T>
+__global__ void k_dot(const T* a, const T* b, int64_t n, double* part) {
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v += to_d(a[i]) * to_d(b[i]);
+  double t = block_reduce(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// --- MINRES -----------------------------------------------------------------
+
+// r1 = r2 = b, y = P b, x = w = w2 = 0, partial beta1^2 = b.y
+template <class T>
+__global__ void k_mr_init(const T* __restrict__ b, const T* __restrict__ dinv, int64_t n,
+                          T* r1, T* r2, T* y, T* x, T* w, T* w2, double* part) {
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T bi = b[i];
+    T yi = bi * dinv[i];
+    r1[i] = bi;
+    r2[i] = bi;
+    y[i] = yi;
+    x[i] = 0;
+    w[i] = 0;
+    w2[i] = 0;
+    v += to_d(bi) * to_d(yi);
+  }
+  double t = block_reduce(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_mr_start(MinresS* S, const double* part, int np, double rtol, long long max_iters) {
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+  double b2 = block_reduce(v, sh);
+  if (threadIdx.x == 0) {
+    MinresS s = {};
+    s.rtol = rtol;
+    s.max_iters = max_iters;
+    s.iter = 0;
+    if (b2 <= 0.0) {
+      s.done = 1;
+      s.residual = 0.0;
+    } else {
+      double beta1 = sqrt(b2);
+      s.beta1 = beta1;
+      s.beta = beta1;
+      s.oldb = 0.0;
+      s.dbar = 0.0;
+      s.epsln = 0.0;
+      s.phibar = beta1;
+      s.cs = -1.0;
+      s.sn = 0.0;
+      s.residual = 1.0;
+      s.done = max_iters <= 0;
+    }
+    *S = s;
+  }
+}
+
+// yn = A (y/beta) - (beta/oldb) r1, written over r1; partial alpha = (y/beta).yn
+template <class T>
+__global__ void __launch_bounds__(TPB) k_mr_spmv(const int64_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ col,
+                                                 const T* __restrict__ val, int64_t n,
+                                                 const T* __restrict__ y, T* __restrict__ r1,
+                                                 const MinresS* __restrict__ S, double* part) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  const T invb = (T)(1.0 / S->beta);
+  const T cr = (S->iter >= 1) ? (T)(S->beta / S->oldb) : (T)0;
+  const int lane = threadIdx.x % LANES;
+  const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
+  double loc = 0.0;
+  for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
+       row += groups) {
+    T acc = 0;
+    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LANES) acc += val[p] * y[col[p]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    if (lane == 0) {
+      T yn = acc * invb - cr * r1[row];
+      r1[row] = yn;
+      loc += to_d(y[row] * invb) * to_d(yn);
+    }
+  }
+  double t = block_reduce(loc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Harmonic border (rank nh): rows/cols n..n+nh-1 of the KKT system.
+// partial dots y[0:n] . border_h
+template <class T>
+__global__ void k_mr_border_dot(const T* __restrict__ border, int nh, int64_t n,
+                                const T* __restrict__ y, const MinresS* S, double* part) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  for (int h = 0; h < nh; ++h) {
+    double v = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      v += to_d(border[h * n + i]) * to_d(y[i]);
+    double t = block_reduce(v, sh);
+    if (threadIdx.x == 0) part[h * NBLK + blockIdx.x] = t;
+    __syncthreads();
+  }
+}
+
+// border rows of yn and the alpha correction (one block)
+template <class T>
+__global__ void k_mr_border_rows(int nh, int64_t n, const T* y, T* r1, MinresS* S,
+                                 const double* part) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  __shared__ double dh[8];
+  for (int h = 0; h < nh; ++h) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < NBLK; i += blockDim.x) v += part[h * NBLK + i];
+    double t = block_reduce(v, sh);
+    if (threadIdx.x == 0) dh[h] = t;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double invb = 1.0 / S->beta;
+    double cr = (S->iter >= 1) ? S->beta / S->oldb : 0.0;
+    double extra = 0.0;
+    for (int h = 0; h < nh; ++h) {
+      double vh = to_d(y[n + h]) * invb;
+      T ynh = (T)(dh[h] * invb - cr * to_d(r1[n + h]));
+      r1[n + h] = ynh;
+      extra += vh * (dh[h] * invb) + vh * to_d(ynh);
+    }
+    S->extra_alpha = extra;
+  }
+}
+
+// yn[0:n] += (y[n+h]/beta) border_h
+template <class T>
+__global__ void k_mr_border_axpy(const T* __restrict__ border, int nh, int64_t n,
+                                 const T* __restrict__ y, T* __restrict__ r1, const MinresS* S) {
+  if (S->done) return;
+  const T invb = (T)(1.0 / S->beta);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T acc = r1[i];
+    for (int h = 0; h < nh; ++h) acc += (y[n + h] * invb) * border[h * n + i];
+    r1[i] = acc;
+  }
+}
+
+__global__ void k_mr_alpha(MinresS* S, const double* part, int np, int with_extra) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+  double t = block_reduce(v, sh);
+  if (threadIdx.x == 0) S->alfa = t + (with_extra ? S->extra_alpha : 0.0);
+}
+
+// yn -= (alpha/beta) r2; z = dinv * yn; partial beta^2 = yn.z
+template <class T>
+__global__ void __launch_bounds__(TPB) k_mr_update(T* __restrict__ yn, const T* __restrict__ r2,
+                                                   const T* __restrict__ dinv, T* __restrict__ z,
+                                                   int64_t n, const MinresS* __restrict__ S,
+                                                   double* part) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  const T c = (T)(-S->alfa / S->beta);
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T a = yn[i] + c * r2[i];
+    T zi = a * dinv[i];
+    yn[i] = a;
+    z[i] = zi;
+    v += to_d(a) * to_d(zi);
+  }
+  double t = block_reduce(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Givens rotation (krylov.rs:168-186), one thread after the beta^2 reduction.
+__global__ void k_mr_rotate(MinresS* S, const double* part, int np) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+  double b2 = block_reduce(v, sh);
+  if (threadIdx.x != 0) return;
+  MinresS s = *S;
+  const double eps = 2.220446049250313e-16;
+  double beta_new = sqrt(fmax(b2, 0.0));
+  double alfa = s.alfa;
+  s.invb_prev = 1.0 / s.beta;
+  s.oldb = s.beta;
+  s.beta = beta_new;
+  double oldeps = s.epsln;
+  double delta = s.cs * s.dbar + s.sn * alfa;
+  double gbar = s.sn * s.dbar - s.cs * alfa;
+  s.epsln = s.sn * beta_new;
+  s.dbar = -s.cs * beta_new;
+  double gamma = fmax(sqrt(gbar * gbar + beta_new * beta_new), eps);
+  s.cs = gbar / gamma;
+  s.sn = beta_new / gamma;
+  double phi = s.cs * s.phibar;
+  s.phibar *= s.sn;
+  s.oldeps = oldeps;
+  s.delta = delta;
+  s.invg = 1.0 / gamma;
+  s.phi = phi;
+  s.iter += 1;
+  s.residual = s.phibar / s.beta1;
+  *S = s;
+}
+
+// w_new = (y_old/beta_old - oldeps w2 - delta w)/gamma -> into w2's buffer; x += phi w_new.
+// Runs in the same iteration as the rotation; the done flag is raised after it.
+template <class T>
+__global__ void __launch_bounds__(TPB) k_mr_wx(const T* __restrict__ yold, T* __restrict__ w,
+                                               T* __restrict__ w2, T* __restrict__ x, int64_t n,
+                                               MinresS* S) {
+  if (S->done) return;
+  const T ib = (T)S->invb_prev, oe = (T)S->oldeps, de = (T)S->delta, ig = (T)S->invg,
+          ph = (T)S->phi;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T wn = (yold[i] * ib - oe * w2[i] - de * w[i]) * ig;
+    w2[i] = wn;
+    x[i] += ph * wn;
+  }
+}
+
+__global__ void k_mr_check(MinresS* S) {
+  if (S->done) return;
+  if (S->residual <= S->rtol || S->iter >= S->max_iters) S->done = 1;
+}
+
+// --- refinement helpers --------------------------------------------------------
+__global__ void k_to_f32_scaled(const double* a, int64_t n, double s, float* b) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)(a[i] * s);
+}
+__global__ void k_axpy_from_f32(const float* a, int64_t n, double s, double* x) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] += s * (double)a[i];
+}
+__global__ void k_axpy(const double* a, int64_t n, double s, double* x) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] += s * a[i];
+}
+__global__ void k_f64_to_f32(const double* a, int64_t n, float* b) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+// border rows of r = b - A x
+__global__ void k_border_residual(const double* border, int nh, int64_t n, const double* x,
+                                  const double* b, double* r, double* part_extra) {
+  __shared__ double sh[TPB / 64];
+  for (int h = 0; h < nh; ++h) {
+    double v = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v += border[h * n + i] * x[i];
+    double t = block_reduce(v, sh);
+    if (threadIdx.x == 0) {
+      double rh = b[n + h] - t;
+      r[n + h] = rh;
+      part_extra[h] = rh * rh;
+    }
+    __syncthreads();
+  }
+}
+__global__ void k_border_residual_axpy(const double* border, int nh, int64_t n, const double* x,
+                                       double* r, double* part) {
+  // r[0:n] -= sum_h x[n+h] border_h, then partial |r[0:n]|^2 recomputed
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double ri = r[i];
+    for (int h = 0; h < nh; ++h) ri -= x[n + h] * border[h * n + i];
+    r[i] = ri;
+    v += ri * ri;
+  }
+  double t = block_reduce(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_jacobi(const double* dd, const double* ii, double* oo, int64_t nn) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < nn) oo[i] = dd ? ii[i] * dd[i] : ii[i];
+}
+__global__ void k_cg_p(double* pp, const double* zz, double bb, int64_t nn) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < nn) pp[i] = bb * pp[i] + zz[i];
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
+inline unsigned red_grid(int64_t n) {
+  return (unsigned)std::min<int64_t>(NBLK, std::max<int64_t>(1, (n + TPB - 1) / TPB));
+}
+inline unsigned spmv_grid(int64_t n) {
+  int64_t rows_per_block = TPB / LANES;
+  return (unsigned)std::min<int64_t>(NBLK, std::max<int64_t>(1, (n + rows_per_block - 1) / rows_per_block));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// solver workspace
+// ---------------------------------------------------------------------------
+template <class T>
+struct MinresWork {
+  DBuf<T> r1, r2, y, z, w, w2, x, b, dinv, border;
+  DBuf<double> part, part2;
+  DBuf<MinresS> S;
+  hipGraphExec_t graph = nullptr;
+  int graph_iters = 0;
+  int64_t n = 0, nh = 0;
+  ~MinresWork() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+  }
+};
+
+struct SolverState {
+  MinresWork<double> w64;
+  MinresWork<float> w32;
+  DBuf<double> r, x, part;
+  MinresS* host_s = nullptr;  // pinned
+  ~SolverState() {
+    if (host_s) (void)hipHostFree(host_s);
+  }
+};
+
+void solver_state_free(SolverState* ws) { delete ws; }
+
+template <class T>
+static void minres_alloc(Ctx* ctx, MinresWork<T>& w, int64_t n, int64_t nh) {
+  if (w.n == n && w.nh == nh && w.r1.p) return;
+  int64_t N = n + nh;
+  w.r1.alloc(ctx, N);
+  w.r2.alloc(ctx, N);
+  w.y.alloc(ctx, N);
+  w.z.alloc(ctx, N);
+  w.w.alloc(ctx, N);
+  w.w2.alloc(ctx, N);
+  w.x.alloc(ctx, N);
+  w.b.alloc(ctx, N);
+  w.dinv.alloc(ctx, N);
+  w.border.alloc(ctx, std::max<int64_t>(nh * n, 1));
+  w.part.alloc(ctx, NBLK * 8);
+  w.part2.alloc(ctx, NBLK);
+  w.S.alloc(ctx, 1);
+  w.n = n;
+  w.nh = nh;
+  if (w.graph) {
+    (void)hipGraphExecDestroy(w.graph);
+    w.graph = nullptr;
+  }
+}
+
+// Enqueue one MINRES iteration; `par` selects the buffer roles.
+template <class T>
+static void minres_iteration(hipStream_t st, MinresWork<T>& w, const int64_t* rp,
+                             const int32_t* col, const T* val, int par) {
+  const int64_t n = w.n, N = w.n + w.nh;
+  T* r1 = par ? w.r2.p : w.r1.p;
+  T* r2 = par ? w.r1.p : w.r2.p;
+  T* y = par ? w.z.p : w.y.p;
+  T* z = par ? w.y.p : w.z.p;
+  T* wc = par ? w.w2.p : w.w.p;
+  T* w2 = par ? w.w.p : w.w2.p;
+  k_mr_spmv<T><<<spmv_grid(n), TPB, 0, st>>>(rp, col, val, n, y, r1, w.S.p, w.part.p);
+  if (w.nh) {
+    k_mr_border_dot<T><<<NBLK, TPB, 0, st>>>(w.border.p, (int)w.nh, n, y, w.S.p, w.part.p + NBLK);
+    k_mr_border_rows<T><<<1, TPB, 0, st>>>((int)w.nh, n, y, r1, w.S.p, w.part.p + NBLK);
+    k_mr_border_axpy<T><<<red_grid(n), TPB, 0, st>>>(w.border.p, (int)w.nh, n, y, r1, w.S.p);
+  }
+  k_mr_alpha<<<1, TPB, 0, st>>>(w.S.p, w.part.p, spmv_grid(n), w.nh ? 1 : 0);
+  k_mr_update<T><<<red_grid(N), TPB, 0, st>>>(r1, r2, w.dinv.p, z, N, w.S.p, w.part2.p);
+  k_mr_rotate<<<1, TPB, 0, st>>>(w.S.p, w.part2.p, red_grid(N));
+  k_mr_wx<T><<<grid_for(N), TPB, 0, st>>>(y, wc, w2, w.x.p, N, w.S.p);
+  k_mr_check<<<1, 1, 0, st>>>(w.S.p);
+}
+
+// Run MINRES on w.b (already filled) into w.x. Returns stats.
+template <class T>
+static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const int64_t* rp,
+                              const int32_t* col, const T* val, double rtol, int64_t max_iters,
+                              const feec_solve_opts& o) {
+  hipStream_t st = ctx->stream;
+  const int64_t N = w.n + w.nh;
+  k_mr_init<T><<<red_grid(N), TPB, 0, st>>>(w.b.p, w.dinv.p, N, w.r1.p, w.r2.p, w.y.p, w.x.p,
+                                            w.w.p, w.w2.p, w.part.p);
+  k_mr_start<<<1, TPB, 0, st>>>(w.S.p, w.part.p, red_grid(N), rtol, (long long)max_iters);
+  FEEC_HIP(hipGetLastError());
+  int seg = std::max(2, o.graph_iters - (o.graph_iters & 1));
+  if (o.use_graph && (!w.graph || w.graph_iters != seg)) {
+    if (w.graph) {
+      (void)hipGraphExecDestroy(w.graph);
+      w.graph = nullptr;
+    }
+    hipGraph_t g;
+    FEEC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < seg; ++i) minres_iteration<T>(st, w, rp, col, val, i & 1);
+    FEEC_HIP(hipStreamEndCapture(st, &g));
+    FEEC_HIP(hipGraphInstantiate(&w.graph, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    w.graph_iters = seg;
+  }
+  KrylovStats ks;
+  int64_t launched = 0;
+  // Keep two segments in flight: enqueue, then read the scalars of the one before.
+  while (true) {
+    if (o.use_graph) {
+      FEEC_HIP(hipGraphLaunch(w.graph, st));
+    } else {
+      for (int i = 0; i < seg; ++i) minres_iteration<T>(st, w, rp, col, val, i & 1);
+    }
+    launched += seg;
+    FEEC_HIP(hipMemcpyAsync(ss.host_s, w.S.p, sizeof(MinresS), hipMemcpyDeviceToHost, st));
+    FEEC_HIP(hipStreamSynchronize(st));
+    if (ss.host_s->done) break;
+  }
+  ks.iters = ss.host_s->iter;
+  ks.precond_residual = ss.host_s->residual;
+  ks.converged = ss.host_s->residual <= rtol;
+  return ks;
+}
+
+void kkt_refresh_fp32(KktSystem& s) {
+  if (!s.val32.p || (int64_t)s.val32.n != s.A.nnz) s.val32.alloc(s.ctx, s.A.nnz);
+  if (s.A.nnz)
+    k_f64_to_f32<<<grid_for(s.A.nnz), TPB, 0, s.ctx->stream>>>(s.A.val.p, s.A.nnz, s.val32.p);
+  int64_t N = s.n + s.nh;
+  if (!s.inv_diag32.p || (int64_t)s.inv_diag32.n != N) s.inv_diag32.alloc(s.ctx, N);
+  k_f64_to_f32<<<grid_for(N), TPB, 0, s.ctx->stream>>>(s.inv_diag.p, N, s.inv_diag32.p);
+  FEEC_HIP(hipGetLastError());
+}
+
+// |b - A x| / |b| with the FP64 KKT operator (CSR + border); r into ss.r
+static double true_residual(KktSystem& s, SolverState& ss, const double* b, const double* x,
+                            double bnorm) {
+  Ctx* ctx = s.ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n, nh = s.nh;
+  k_residual<<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.A.val.p, n, x, b, ss.r.p,
+                                           ss.part.p);
+  double* fin = ss.part.p + NBLK * 4;
+  if (nh) {
+    k_border_residual<<<1, TPB, 0, st>>>(s.border.p, (int)nh, n, x, b, ss.r.p, ss.part.p + NBLK * 2);
+    k_border_residual_axpy<<<red_grid(n), TPB, 0, st>>>(s.border.p, (int)nh, n, x, ss.r.p, ss.part.p);
+    k_finish_sum<<<1, TPB, 0, st>>>(ss.part.p, red_grid(n), fin);
+    k_finish_sum<<<1, TPB, 0, st>>>(ss.part.p + NBLK * 2, (int)nh, fin + 1);
+  } else {
+    k_finish_sum<<<1, TPB, 0, st>>>(ss.part.p, spmv_grid(n), fin);
+  }
+  double h[2] = {0.0, 0.0};
+  FEEC_HIP(hipMemcpyAsync(h, fin, (nh ? 2 : 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  return std::sqrt(h[0] + h[1]) / bnorm;
+}
+
+static double device_norm(Ctx* ctx, SolverState& ss, const double* a, int64_t n) {
+  k_dot<double><<<red_grid(n), TPB, 0, ctx->stream>>>(a, a, n, ss.part.p);
+  double* fin = ss.part.p + NBLK * 4;
+  k_finish_sum<<<1, TPB, 0, ctx->stream>>>(ss.part.p, red_grid(n), fin);
+  double h = 0.0;
+  FEEC_HIP(hipMemcpyAsync(&h, fin, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  return std::sqrt(h);
+}
+
+feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_solve_opts& o,
+                      SolverState** wsp) {
+  Ctx* ctx = s.ctx;
+  hipStream_t st = ctx->stream;
+  if (!*wsp) {
+    *wsp = new SolverState();
+    FEEC_HIP(hipHostMalloc((void**)&(*wsp)->host_s, sizeof(MinresS), hipHostMallocDefault));
+  }
+  SolverState& ss = **wsp;
+  const int64_t n = s.n, nh = s.nh, N = n + nh;
+  if ((int64_t)ss.r.n != N) {
+    ss.r.alloc(ctx, N);
+    ss.x.alloc(ctx, N);
+    ss.part.alloc(ctx, NBLK * 8);
+  }
+  feec_report rep{};
+  hipEvent_t e0, e1;
+  FEEC_HIP(hipEventCreate(&e0));
+  FEEC_HIP(hipEventCreate(&e1));
+  FEEC_HIP(hipEventRecord(e0, st));
+
+  const double bnorm = device_norm(ctx, ss, b, N);
+  FEEC_HIP(hipMemsetAsync(x_out, 0, N * sizeof(double), st));
+  if (bnorm == 0.0) {
+    rep.converged = 1;
+    rep.residual = 0.0;
+    rep.precond_residual = 0.0;
+  } else if (o.precision == FEEC_PREC_FP64) {
+    // FP64 MINRES on the reference's stopping quantity, then FP64 residual
+    // correction steps until the true residual meets rtol.
+    MinresWork<double>& w = ss.w64;
+    minres_alloc(ctx, w, n, nh);
+    FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (nh) FEEC_HIP(hipMemcpyAsync(w.border.p, s.border.p, nh * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    FEEC_HIP(hipMemcpyAsync(w.b.p, b, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double res = 1.0;
+    int steps = 0;
+    while (true) {
+      int64_t budget = o.max_iters - rep.iters;
+      if (budget <= 0) break;
+      KrylovStats ks = minres_run<double>(ctx, ss, w, s.A.row_ptr.p, s.A.col.p, s.A.val.p,
+                                          o.rtol, budget, o);
+      rep.iters += ks.iters;
+      rep.precond_residual = ks.precond_residual;
+      k_axpy<<<grid_for(N), TPB, 0, st>>>(w.x.p, N, 1.0, x_out);
+      res = true_residual(s, ss, b, x_out, bnorm);
+      if (res <= o.rtol || steps >= o.max_refinements) break;
+      ++steps;
+      FEEC_HIP(hipMemcpyAsync(w.b.p, ss.r.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    rep.refinements = steps;
+    rep.residual = res;
+    rep.converged = res <= o.rtol;
+  } else {
+    MinresWork<float>& w = ss.w32;
+    minres_alloc(ctx, w, n, nh);
+    if (!s.val32.p) kkt_refresh_fp32(s);
+    FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag32.p, N * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if (nh) k_f64_to_f32<<<grid_for(nh * n), TPB, 0, st>>>(s.border.p, nh * n, w.border.p);
+    // r = b
+    FEEC_HIP(hipMemcpyAsync(ss.r.p, b, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double rnorm = bnorm, res = 1.0;
+    int steps = 0;
+    while (steps < o.max_refinements) {
+      int64_t budget = o.max_iters - rep.iters;
+      if (budget <= 0) break;
+      k_to_f32_scaled<<<grid_for(N), TPB, 0, st>>>(ss.r.p, N, 1.0 / rnorm, w.b.p);
+      KrylovStats ks = minres_run<float>(ctx, ss, w, s.A.row_ptr.p, s.A.col.p, s.val32.p,
+                                         o.inner_rtol, budget, o);
+      rep.iters += ks.iters;
+      rep.precond_residual = ks.precond_residual;
+      k_axpy_from_f32<<<grid_for(N), TPB, 0, st>>>(w.x.p, N, rnorm, x_out);
+      ++steps;
+      res = true_residual(s, ss, b, x_out, bnorm);
+      if (res <= o.rtol) break;
+      rnorm = res * bnorm;
+    }
+    rep.refinements = steps;
+    rep.residual = res;
+    rep.converged = res <= o.rtol;
+  }
+  FEEC_HIP(hipEventRecord(e1, st));
+  FEEC_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
+  rep.seconds = ms * 1e-3;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rep;
+}
+
+void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y) {
+  if (a.nrows == 0) return;
+  k_spmv<double><<<spmv_grid(a.nrows), TPB, 0, ctx->stream>>>(a.row_ptr.p, a.col.p, a.val.p,
+                                                              a.nrows, x, y);
+  FEEC_HIP(hipGetLastError());
+}
+
+// Jacobi-preconditioned CG (krylov.rs:48-95) on an FP64 CSR; host-checked
+// every iteration (used for mass solves and the harmonic normal equations).
+KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
+                     double rtol, int64_t max_iters) {
+  const int64_t n = a.nrows;
+  hipStream_t st = ctx->stream;
+  DBuf<double> r(ctx, n), z(ctx, n), p(ctx, n), ap(ctx, n), part(ctx, NBLK * 2);
+  auto dot = [&](const double* u, const double* v) {
+    k_dot<double><<<red_grid(n), TPB, 0, st>>>(u, v, n, part.p);
+    k_finish_sum<<<1, TPB, 0, st>>>(part.p, red_grid(n), part.p + NBLK);
+    double h = 0.0;
+    FEEC_HIP(hipMemcpyAsync(&h, part.p + NBLK, sizeof(double), hipMemcpyDeviceToHost, st));
+    FEEC_HIP(hipStreamSynchronize(st));
+    return h;
+  };
+  KrylovStats ks;
+  FEEC_HIP(hipMemsetAsync(x, 0, n * sizeof(double), st));
+  double bn = std::sqrt(dot(b, b));
+  if (bn == 0.0) {
+    ks.converged = true;
+    ks.precond_residual = 0.0;
+    return ks;
+  }
+  FEEC_HIP(hipMemcpyAsync(r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  auto apply_p = [&](double* out, const double* in) {
+    k_jacobi<<<grid_for(n), TPB, 0, st>>>(dinv, in, out, n);
+  };
+  apply_p(z.p, r.p);
+  FEEC_HIP(hipMemcpyAsync(p.p, z.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  double rz = dot(r.p, z.p);
+  int64_t iters = 0;
+  double res = 1.0;
+  while (true) {
+    res = std::sqrt(dot(r.p, r.p)) / bn;
+    if (res <= rtol || iters >= max_iters) break;
+    spmv64(ctx, a, p.p, ap.p);
+    double alpha = rz / dot(p.p, ap.p);
+    k_axpy<<<grid_for(n), TPB, 0, st>>>(p.p, n, alpha, x);
+    k_axpy<<<grid_for(n), TPB, 0, st>>>(ap.p, n, -alpha, r.p);
+    apply_p(z.p, r.p);
+    double rzn = dot(r.p, z.p);
+    double beta = rzn / rz;
+    k_cg_p<<<grid_for(n), TPB, 0, st>>>(p.p, z.p, beta, n);
+    rz = rzn;
+    ++iters;
+  }
+  ks.iters = iters;
+  ks.precond_residual = res;
+  ks.converged = res <= rtol;
+  return ks;
+}
+
+}  // namespace feec

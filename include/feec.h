@@ -1,0 +1,259 @@
+/*
+ * feec.h -- C ABI of libfeec, the MI355X (gfx950) implementation of formoniq's
+ * FEEC assembly-and-solve hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes. Every call returns a
+ * feec_status; feec_last_error() gives a thread-local message for the last
+ * failure on the calling thread. Host buffers passed in or out are
+ * caller-owned and sized by the count queries. Handles are library-owned and
+ * freed by their *_destroy function. Calls on one feec_ctx are serialized on
+ * that context's HIP stream; use one context per host thread.
+ *
+ * Reference interfaces replaced (luiswirth/formoniq, paths relative to crates/):
+ *   CartesianTopology::cell_skeleton      simplicial/src/mesher/grid.rs:77-98
+ *   Complex::from_cells                    simplicial/src/topology/complex.rs:278-336
+ *   Complex::boundary_simplices            simplicial/src/topology/complex.rs:140-149
+ *   Complex::coboundary_operator           simplicial/src/topology/complex.rs:168-217
+ *   MeshCoords::to_edge_lengths_sq         regge/src/coord/mesh.rs:208-216
+ *   MeshLengthsSq::cell_metric             regge/src/lengths/mesh.rs:182-184
+ *   BilinearForm::{element, assemble}      formoniq/src/galerkin.rs:42-58, 138-188
+ *   WhitneyPairing::{mass,dif_trial,dif_test,dif_both}  formoniq/src/operators.rs:163-211
+ *   HilbertComplex::{mass,...,hdif_gram,dif,inclusion}  formoniq/src/whitney_complex.rs:55-277
+ *   elliptic::solve_source                 formoniq/src/problems/elliptic.rs:132-182
+ *   HodgeDirac::assemble_signed            formoniq/src/problems/dirac.rs:213-258
+ *   solve_dirac_source                     formoniq/src/problems/dirac.rs:473-534
+ *   krylov::{cg, minres}                   iterative/src/krylov.rs:48-211
+ */
+#ifndef FEEC_H
+#define FEEC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum feec_status {
+  FEEC_OK = 0,
+  FEEC_E_ARG = 1,             /* bad argument or shape (the reference panics) */
+  FEEC_E_DEVICE = 2,          /* HIP runtime error */
+  FEEC_E_OOM = 3,             /* device allocation failed */
+  FEEC_E_TOPOLOGY = 4,        /* dangling/duplicate vertices, non-pseudomanifold */
+  FEEC_E_DEGENERATE_CELL = 5, /* a cell metric with det g == 0 */
+  FEEC_E_NOT_SPD = 6,         /* an SPD-only path met an indefinite block */
+  FEEC_E_NOT_CONVERGED = 7,   /* Krylov budget spent; the report holds the state */
+  FEEC_E_UNSUPPORTED = 8,     /* a case outside what this build implements */
+  FEEC_E_ADMISSION = 9,       /* serving: request larger than the HBM budget */
+  FEEC_E_AGAIN = 10           /* serving: result not ready yet */
+} feec_status;
+
+const char *feec_last_error(void);
+const char *feec_status_string(feec_status s);
+
+typedef struct feec_ctx_s feec_ctx;
+typedef struct feec_mesh_s feec_mesh;
+typedef struct feec_geom_s feec_geom;
+typedef struct feec_csr_s feec_csr;
+typedef struct feec_problem_s feec_problem;
+typedef struct feec_server_s feec_server;
+
+/* The four WhitneyPairing blocks (operators.rs:163-193); the grade argument is
+ * always the grade of the inner product. FEEC_HDIF_GRAM is
+ * HilbertComplex::hdif_gram(k) = mass(k) + dif_both(k+1) (whitney_complex.rs:180-183). */
+typedef enum feec_form_kind {
+  FEEC_MASS = 0,
+  FEEC_DIF_TRIAL = 1,
+  FEEC_DIF_TEST = 2,
+  FEEC_DIF_BOTH = 3,
+  FEEC_HDIF_GRAM = 4
+} feec_form_kind;
+
+/* Full complex (natural BC, WhitneyComplex) or relative complex (essential BC,
+ * RelativeWhitneyComplex::new, whitney_complex.rs:545-557). */
+typedef enum feec_bc { FEEC_FULL = 0, FEEC_RELATIVE = 1 } feec_bc;
+
+typedef enum feec_ambient { FEEC_EUCLIDEAN = 0, FEEC_MINKOWSKI = 1 } feec_ambient;
+
+typedef enum feec_precision {
+  FEEC_PREC_FP64 = 0,  /* FP64 matrix, vectors and Krylov */
+  FEEC_PREC_MIXED = 1  /* FP32 SpMV/Krylov inside FP64 iterative refinement */
+} feec_precision;
+
+typedef struct feec_solve_opts {
+  double rtol;           /* target true relative residual |b-Ax|/|b| (default 1e-10) */
+  int64_t max_iters;     /* Krylov iteration budget over all refinements (default 200000) */
+  int32_t precision;     /* feec_precision (default FEEC_PREC_MIXED) */
+  int32_t max_refinements; /* mixed: outer FP64 refinement steps (default 12) */
+  double inner_rtol;     /* mixed: FP32 inner MINRES tolerance (default 1e-5) */
+  int32_t use_graph;     /* capture Krylov iterations in a hipGraph (default 1) */
+  int32_t graph_iters;   /* iterations per captured segment, even (default 32) */
+} feec_solve_opts;
+
+typedef struct feec_report {
+  int64_t iters;           /* Krylov iterations over all refinements */
+  int32_t refinements;     /* FP64 refinement steps taken (0 for plain FP64) */
+  int32_t converged;       /* 1 when residual <= rtol */
+  double residual;         /* true FP64 |b-Ax|_2/|b|_2 of the returned solution */
+  double precond_residual; /* MINRES phibar/beta1 of the last Krylov run (the reference's stopping quantity) */
+  double seconds;          /* wall time of the solve on the device */
+} feec_report;
+
+void feec_solve_opts_default(feec_solve_opts *opts);
+
+/* ---- context --------------------------------------------------------- */
+feec_status feec_ctx_create(int device, feec_ctx **out);
+void feec_ctx_destroy(feec_ctx *ctx);
+feec_status feec_ctx_synchronize(feec_ctx *ctx);
+/* Device bytes currently held by handles of this context. */
+feec_status feec_ctx_memory(const feec_ctx *ctx, int64_t *bytes_in_use);
+
+/* ---- meshers (host) ---------------------------------------------------- */
+/* CartesianGrid::new_unit(dim, n).triangulate() (regge/src/mesher/cartesian.rs:109-233):
+ * Kuhn cells [ncells][dim+1] in generation order and coordinates [nverts][dim], axis 0 fastest. */
+feec_status feec_kuhn_counts(int dim, int n, int64_t *ncells, int64_t *nverts);
+feec_status feec_kuhn_grid(int dim, int n, int64_t *cells, double *coords);
+/* Interior-vertex jitter (no reference counterpart; build-defined synthetic
+ * input): each interior vertex of the n^dim unit grid, in vertex order, each axis
+ * a with bit a of axis_mask set, gets amplitude*h*(2U-1), U from SplitMix64(seed). */
+feec_status feec_jitter_interior(int dim, int n, double *coords, double amplitude, uint64_t seed,
+                                 uint32_t axis_mask);
+/* N(0,1) values from SplitMix64(seed) by Box-Muller (build-defined). */
+feec_status feec_normal_vector(int64_t n, uint64_t seed, double *out);
+
+/* ---- topology ---------------------------------------------------------- */
+/* Complex::from_cells: colex numbering of every skeleton, built on the device
+ * by sort. Cells: [ncells][dim+1], vertices 0..nverts-1, all used. */
+feec_status feec_mesh_create(feec_ctx *ctx, int dim, int64_t nverts, int64_t ncells,
+                             const int64_t *cells, feec_mesh **out);
+void feec_mesh_destroy(feec_mesh *mesh);
+feec_status feec_mesh_dim(const feec_mesh *mesh, int *dim);
+feec_status feec_mesh_counts(const feec_mesh *mesh, int64_t *counts /* dim+1 */);
+feec_status feec_mesh_skeleton(const feec_mesh *mesh, int k, int64_t *verts /* N_k*(k+1) */);
+feec_status feec_mesh_cell_faces(const feec_mesh *mesh, int k, int64_t *out /* N_d*C(d+1,k+1) */);
+feec_status feec_mesh_boundary(const feec_mesh *mesh, int k, uint8_t *flags /* N_k */);
+/* Coherent orientation signs of the cells (cell 0 positive); FEEC_E_TOPOLOGY if non-orientable. */
+feec_status feec_mesh_orientation(feec_mesh *mesh, int8_t *signs /* N_d */);
+/* HilbertComplex::dif(k): the coboundary D_k as a CSR (rows N_{k+1}, cols N_k). */
+feec_status feec_mesh_coboundary(feec_mesh *mesh, feec_bc bc, int k, feec_csr **out);
+
+/* ---- geometry ---------------------------------------------------------- */
+feec_status feec_geom_from_lengths(feec_mesh *mesh, const double *lengths_sq /* N_1 */,
+                                   feec_geom **out);
+feec_status feec_geom_from_coords(feec_mesh *mesh, const double *coords /* nverts*dim */,
+                                  feec_ambient ambient, feec_geom **out);
+void feec_geom_destroy(feec_geom *geom);
+feec_status feec_geom_lengths(const feec_geom *geom, double *out /* N_1 */);
+/* Per cell: g [d*d], g^-1 [d*d], det g; any pointer may be NULL. */
+feec_status feec_cell_metrics(feec_geom *geom, double *g, double *ginv, double *det);
+/* Signature guards (metric/src/lib.rs:47-58, 203-205; regge/src/lengths/mesh.rs:219-236):
+ * edges with s<0, s==0, s>0, and per-cell Riemannian flags (Cholesky succeeds). */
+feec_status feec_causal_census(feec_geom *geom, int64_t counts[3], uint8_t *cell_riemannian);
+
+/* ---- element matrices and assembly ------------------------------------- */
+/* Row-major [N_d][rows][cols] in local colex face order. */
+feec_status feec_element_shape(int dim, feec_form_kind kind, int grade, int *rows, int *cols);
+feec_status feec_element_matrices(feec_geom *geom, feec_form_kind kind, int grade, double *out);
+/* The Galerkin matrix: pattern, numbering and signs bit-exact with the
+ * reference (exact-zero element entries dropped, duplicates summed in cell
+ * order, summed zeros kept); relative = E^T A E. */
+feec_status feec_assemble(feec_geom *geom, feec_bc bc, feec_form_kind kind, int grade,
+                          feec_csr **out);
+feec_status feec_csr_shape(const feec_csr *a, int64_t *nrows, int64_t *ncols, int64_t *nnz);
+feec_status feec_csr_copy(const feec_csr *a, int64_t *row_ptr, int32_t *col, double *val);
+feec_status feec_csr_spmv(feec_csr *a, const double *x, double *y); /* host vectors */
+void feec_csr_destroy(feec_csr *a);
+
+/* ---- problems ------------------------------------------------------------ */
+/* b_k and an M_k-orthonormal harmonic basis [ndofs][count] (basis may be NULL).
+ * Closed forms on a ball (full k=0: the constant; relative k=n: M_n^-1 times the
+ * orientation); other harmonic spaces give FEEC_E_UNSUPPORTED. */
+feec_status feec_harmonics(feec_geom *geom, feec_bc bc, int grade, int64_t *count, double *basis);
+/* The symmetric mixed KKT (elliptic.rs:54-114), with its harmonic border and
+ * the sigma-row sign flip; sizes = {n_sigma, n_u, n_harmonic}. */
+feec_status feec_mixed_kkt(feec_geom *geom, feec_bc bc, int grade, feec_csr **out,
+                           int64_t sizes[3]);
+/* elliptic::solve_source: source_galvec has N_k entries (ambient); sigma (N_{k-1}),
+ * u (N_k) are ambient cochains, p has n_harmonic entries. */
+feec_status feec_solve_source(feec_geom *geom, feec_bc bc, int grade, const double *source_galvec,
+                              const feec_solve_opts *opts, double *sigma, double *u, double *p,
+                              feec_report *report);
+/* HodgeDirac::assemble_signed(sign): mass_block, op and offsets[dim+2]. */
+feec_status feec_hodge_dirac_assemble(feec_geom *geom, feec_bc bc, double sign,
+                                      feec_csr **mass_block, feec_csr **op, int64_t *offsets);
+/* solve_dirac_source on the relative complex: (A + m M) u = f with lift;
+ * load and lift are flat full-complex fields, out is the flat full field. */
+feec_status feec_solve_dirac_source(feec_geom *geom, double mass_term, const double *load,
+                                    const double *lift, const feec_solve_opts *opts, double *out,
+                                    feec_report *report);
+
+/* ---- resident problems (device-resident state, pattern reused) ----------- */
+feec_status feec_problem_create(feec_geom *geom, feec_bc bc, int grade, const feec_solve_opts *opts,
+                                feec_problem **out);
+void feec_problem_destroy(feec_problem *prob);
+/* Recompute element matrices and CSR values from the geometry (pattern kept). */
+feec_status feec_problem_assemble(feec_problem *prob);
+/* Solve with the source already on the device (N_k doubles); solution stays
+ * resident; feec_problem_solution copies sigma/u/p out. */
+feec_status feec_problem_solve_device(feec_problem *prob, const double *source_device,
+                                      feec_report *report);
+feec_status feec_problem_solve(feec_problem *prob, const double *source_host, feec_report *report);
+feec_status feec_problem_solution(feec_problem *prob, double *sigma, double *u, double *p);
+feec_status feec_problem_sizes(const feec_problem *prob, int64_t sizes[3], int64_t *nnz);
+/* Device pointer to an N_k scratch vector for feec_problem_solve_device. */
+feec_status feec_problem_source_buffer(feec_problem *prob, double **device_ptr);
+
+/* ---- batching and serving -------------------------------------------------- */
+/* One request: a jittered Kuhn box and a Hodge-Laplace source solve. */
+typedef struct feec_request {
+  int32_t dim;          /* 2..4 */
+  int32_t grade;        /* 0..dim */
+  int32_t n;            /* cells per axis */
+  int32_t bc;           /* feec_bc */
+  double jitter;        /* amplitude in units of h */
+  uint64_t mesh_seed;   /* jitter seed */
+  uint64_t source_seed; /* N(0,1) source cochain seed */
+  int32_t precision;    /* feec_precision */
+  int32_t reserved;
+} feec_request;
+
+typedef struct feec_result {
+  int64_t ticket;
+  int64_t ndofs;        /* KKT size */
+  int64_t ncells;
+  int64_t iters;
+  int32_t refinements;
+  int32_t status;       /* feec_status of this request */
+  double residual;      /* true relative residual */
+  double u_norm;        /* |u|_2 of the solution (for alone-vs-mixed checks) */
+  double u_checksum;    /* sum_i u_i * ((i % 7) - 3) */
+  double admit_time;    /* seconds since server start */
+  double finish_time;
+  double latency;       /* finish - submit, seconds */
+} feec_result;
+
+/* Batched: all element matrices of all requests in one persistent launch over
+ * a (mesh, cell-range) work queue, then the solves. */
+feec_status feec_batch_run(feec_ctx *ctx, int nreq, const feec_request *reqs,
+                           const feec_solve_opts *opts, feec_result *results);
+
+typedef struct feec_server_opts {
+  int64_t hbm_budget_bytes; /* admission budget for resident state */
+  int32_t max_inflight;     /* slots: requests resident on the device at once */
+  int32_t reserved;
+  feec_solve_opts solve;
+} feec_server_opts;
+
+feec_status feec_server_create(feec_ctx *ctx, const feec_server_opts *opts, feec_server **out);
+feec_status feec_server_submit(feec_server *srv, const feec_request *req, int64_t *ticket);
+/* FEEC_E_AGAIN while the request is queued or running. */
+feec_status feec_server_poll(feec_server *srv, int64_t ticket, feec_result *result);
+/* Copy the solution u (N_k) of a finished request; NULL-safe size query. */
+feec_status feec_server_solution(feec_server *srv, int64_t ticket, double *u, int64_t *n);
+feec_status feec_server_drain(feec_server *srv);
+void feec_server_destroy(feec_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEEC_H */

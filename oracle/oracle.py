@@ -1,0 +1,592 @@
+"""ctypes face of the CPU FP64 oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: this module is the parity checker and the CPU
+baseline. The tests, ``__graft_entry__.smoke()`` and the benchmark's
+``cpu_baseline`` leg import it; the product package ``formoniq_amd`` never
+does.
+
+The classes mirror the reference's host API for the hot path so that tests read
+like formoniq's own:
+
+* ``Complex``            simplicial/src/topology/complex.rs:298-336
+* ``WhitneyComplex``     formoniq/src/whitney_complex.rs:321-465 (full)
+                         formoniq/src/whitney_complex.rs:523-714 (relative)
+* ``HodgeBlocks``        formoniq/src/hodge.rs:62-99
+* ``assemble_mixed_kkt`` formoniq/src/problems/elliptic.rs:54-114
+* ``solve_source``       formoniq/src/problems/elliptic.rs:132-182
+* ``hodge_dirac``        formoniq/src/problems/dirac.rs:213-258
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+MASS, DIF_TRIAL, DIF_TEST, DIF_BOTH = 0, 1, 2, 3
+
+i64p = C.POINTER(C.c_int64)
+f64p = C.POINTER(C.c_double)
+u8p = C.POINTER(C.c_uint8)
+
+
+class _Csr(C.Structure):
+    _fields_ = [("nrows", C.c_int64), ("ncols", C.c_int64), ("nnz", C.c_int64),
+                ("row_ptr", i64p), ("col", i64p), ("val", f64p)]
+
+
+class _Report(C.Structure):
+    _fields_ = [("iters", C.c_int64), ("residual", C.c_double), ("converged", C.c_int)]
+
+
+def lib(path=None):
+    """The oracle library; `path` loads another build of the same source (the
+    mutation test compiles deliberately broken variants)."""
+    global _LIB
+    if path is not None or _LIB is None:
+        path = path or os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        sig = {
+            "orc_binom": (C.c_int64, [C.c_int64, C.c_int64]),
+            "orc_local_subsets": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int)]),
+            "orc_form_shape": (None, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "orc_kuhn_ncells": (C.c_int64, [C.c_int, C.c_int]),
+            "orc_kuhn_nverts": (C.c_int64, [C.c_int, C.c_int]),
+            "orc_kuhn": (None, [C.c_int, C.c_int, i64p, f64p]),
+            "orc_complex_new": (P, [C.c_int, C.c_int64, i64p]),
+            "orc_complex_free": (None, [P]),
+            "orc_nsimplices": (C.c_int64, [P, C.c_int]),
+            "orc_skeleton": (None, [P, C.c_int, i64p]),
+            "orc_cell_faces": (None, [P, C.c_int, i64p]),
+            "orc_boundary_flags": (None, [P, C.c_int, u8p]),
+            "orc_coboundary": (C.POINTER(_Csr), [P, C.c_int]),
+            "orc_orientation": (C.c_int, [P, C.POINTER(C.c_int8)]),
+            "orc_lengths_from_coords": (None, [P, f64p, C.c_int, f64p]),
+            "orc_cell_metric": (None, [C.c_int, f64p, f64p, f64p, f64p]),
+            "orc_element_matrix": (None, [C.c_int, C.c_int, C.c_int, f64p, f64p]),
+            "orc_element_matrices": (None, [P, f64p, C.c_int, C.c_int, f64p, C.c_int]),
+            "orc_assemble": (C.POINTER(_Csr), [P, f64p, C.c_int, C.c_int, C.c_int]),
+            "orc_csr_free": (None, [C.POINTER(_Csr)]),
+            "orc_csr_add": (C.POINTER(_Csr), [C.POINTER(_Csr), C.POINTER(_Csr)]),
+            "orc_csr_mul": (C.POINTER(_Csr), [C.POINTER(_Csr), C.POINTER(_Csr)]),
+            "orc_csr_transpose": (C.POINTER(_Csr), [C.POINTER(_Csr)]),
+            "orc_csr_select": (C.POINTER(_Csr), [C.POINTER(_Csr), u8p, u8p]),
+            "orc_csr_block": (C.POINTER(_Csr), [C.c_int, C.POINTER(C.POINTER(_Csr)), f64p, i64p, i64p]),
+            "orc_spmv": (None, [C.POINTER(_Csr), f64p, f64p]),
+            "orc_cg": (_Report, [C.POINTER(_Csr), f64p, f64p, C.c_double, C.c_int64, f64p]),
+            "orc_minres": (_Report, [C.POINTER(_Csr), f64p, f64p, C.c_double, C.c_int64, f64p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path != os.path.join(_HERE, "liboracle.so"):
+            return L
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+# --------------------------------------------------------------------------
+# CSR
+# --------------------------------------------------------------------------
+
+class Csr:
+    """Owned copy of an oracle CSR: row_ptr, col (int64), val (float64)."""
+
+    def __init__(self, shape, row_ptr, col, val):
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int64)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+
+    @property
+    def nnz(self):
+        return int(self.col.size)
+
+    @staticmethod
+    def _take(ptr):
+        s = ptr.contents
+        n, nnz = s.nrows, s.nnz
+        rp = np.ctypeslib.as_array(s.row_ptr, shape=(n + 1,)).copy() if n >= 0 else np.zeros(1, np.int64)
+        col = np.ctypeslib.as_array(s.col, shape=(max(nnz, 1),))[:nnz].copy()
+        val = np.ctypeslib.as_array(s.val, shape=(max(nnz, 1),))[:nnz].copy()
+        out = Csr((s.nrows, s.ncols), rp, col, val)
+        lib().orc_csr_free(ptr)
+        return out
+
+    def _struct(self):
+        s = _Csr(self.shape[0], self.shape[1], self.nnz, _p(self.row_ptr, i64p),
+                 _p(self.col, i64p), _p(self.val, f64p))
+        s._keep = self
+        return s
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col, self.row_ptr), shape=self.shape)
+
+    def todense(self):
+        out = np.zeros(self.shape)
+        for i in range(self.shape[0]):
+            for p in range(self.row_ptr[i], self.row_ptr[i + 1]):
+                out[i, self.col[p]] += self.val[p]
+        return out
+
+    def __add__(self, other):
+        a, b = self._struct(), other._struct()
+        return Csr._take(lib().orc_csr_add(C.byref(a), C.byref(b)))
+
+    def __matmul__(self, other):
+        if isinstance(other, np.ndarray):
+            x = np.ascontiguousarray(other, dtype=np.float64)
+            y = np.zeros(self.shape[0])
+            a = self._struct()
+            lib().orc_spmv(C.byref(a), _p(x, f64p), _p(y, f64p))
+            return y
+        a, b = self._struct(), other._struct()
+        return Csr._take(lib().orc_csr_mul(C.byref(a), C.byref(b)))
+
+    @property
+    def T(self):
+        a = self._struct()
+        return Csr._take(lib().orc_csr_transpose(C.byref(a)))
+
+    def select(self, row_keep=None, col_keep=None):
+        a = self._struct()
+        rk = None if row_keep is None else np.ascontiguousarray(row_keep, dtype=np.uint8)
+        ck = None if col_keep is None else np.ascontiguousarray(col_keep, dtype=np.uint8)
+        return Csr._take(lib().orc_csr_select(C.byref(a), None if rk is None else _p(rk, u8p),
+                                              None if ck is None else _p(ck, u8p)))
+
+    def diagonal(self):
+        d = np.zeros(min(self.shape))
+        for i in range(d.size):
+            for p in range(self.row_ptr[i], self.row_ptr[i + 1]):
+                if self.col[p] == i:
+                    d[i] = self.val[p]
+        return d
+
+    @staticmethod
+    def zeros(nrows, ncols):
+        return Csr((nrows, ncols), np.zeros(nrows + 1, np.int64), np.zeros(0, np.int64), np.zeros(0))
+
+    @staticmethod
+    def from_dense_rows(dense):
+        """Dense block kept structurally in full (used for harmonic borders)."""
+        dense = np.asarray(dense, dtype=np.float64)
+        n, m = dense.shape
+        rp = np.arange(n + 1, dtype=np.int64) * m
+        col = np.tile(np.arange(m, dtype=np.int64), n)
+        return Csr((n, m), rp, col, dense.reshape(-1).copy())
+
+
+def block(blocks, scales=None, row_sizes=None, col_sizes=None):
+    """CooMatrix::block: a square grid of CSR blocks (None = empty)."""
+    nb = len(blocks)
+    flat = [blocks[i][j] for i in range(nb) for j in range(nb)]
+    if row_sizes is None:
+        row_sizes = [next(b.shape[0] for b in blocks[i] if b is not None) for i in range(nb)]
+    if col_sizes is None:
+        col_sizes = [next(blocks[i][j].shape[1] for i in range(nb) if blocks[i][j] is not None)
+                     for j in range(nb)]
+    structs = [None if b is None else b._struct() for b in flat]
+    arr = (C.POINTER(_Csr) * (nb * nb))(*[C.cast(C.pointer(s), C.POINTER(_Csr)) if s is not None
+                                          else C.POINTER(_Csr)() for s in structs])
+    sc = np.ones(nb * nb) if scales is None else np.ascontiguousarray(np.asarray(scales, float).reshape(-1))
+    rs = np.ascontiguousarray(row_sizes, dtype=np.int64)
+    cs = np.ascontiguousarray(col_sizes, dtype=np.int64)
+    res = lib().orc_csr_block(nb, arr, _p(sc, f64p), _p(rs, i64p), _p(cs, i64p))
+    del structs
+    return Csr._take(res)
+
+
+# --------------------------------------------------------------------------
+# meshes, topology, geometry
+# --------------------------------------------------------------------------
+
+def kuhn(dim, n):
+    """CartesianGrid::new_unit(dim, n).triangulate_cells(): Kuhn cells in
+    generation order and vertex coordinates [V, dim]."""
+    L = lib()
+    nc, nv = L.orc_kuhn_ncells(dim, n), L.orc_kuhn_nverts(dim, n)
+    cells = np.zeros((nc, dim + 1), np.int64)
+    coords = np.zeros((nv, dim), np.float64)
+    L.orc_kuhn(dim, n, _p(cells, i64p), _p(coords, f64p))
+    return cells, coords
+
+
+def local_subsets(d, k):
+    n = int(lib().orc_binom(d + 1, k + 1))
+    out = np.zeros(n * (k + 1), dtype=np.int32)
+    lib().orc_local_subsets(d, k, out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out.reshape(n, k + 1)
+
+
+def form_shape(d, kind, grade):
+    r, c = C.c_int(), C.c_int()
+    lib().orc_form_shape(d, kind, grade, C.byref(r), C.byref(c))
+    return r.value, c.value
+
+
+class Complex:
+    def __init__(self, dim, cells):
+        cells = np.ascontiguousarray(cells, dtype=np.int64)
+        h = lib().orc_complex_new(dim, cells.shape[0], _p(cells, i64p))
+        if not h:
+            raise ValueError("invalid cell list (dangling/duplicate vertices)")
+        self._h = h
+        self.dim = dim
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_complex_free(self._h)
+            self._h = None
+
+    def nsimplices(self, k):
+        return int(lib().orc_nsimplices(self._h, k))
+
+    def skeleton(self, k):
+        out = np.zeros((self.nsimplices(k), k + 1), np.int64)
+        lib().orc_skeleton(self._h, k, _p(out, i64p))
+        return out
+
+    def cells(self):
+        return self.skeleton(self.dim)
+
+    def cell_faces(self, k):
+        nf = int(lib().orc_binom(self.dim + 1, k + 1))
+        out = np.zeros((self.nsimplices(self.dim), nf), np.int64)
+        lib().orc_cell_faces(self._h, k, _p(out, i64p))
+        return out
+
+    def boundary_flags(self, k):
+        out = np.zeros(self.nsimplices(k), np.uint8)
+        lib().orc_boundary_flags(self._h, k, _p(out, u8p))
+        return out
+
+    def coboundary(self, k):
+        return Csr._take(lib().orc_coboundary(self._h, k))
+
+    def orientation(self):
+        out = np.zeros(self.nsimplices(self.dim), np.int8)
+        ok = lib().orc_orientation(self._h, out.ctypes.data_as(C.POINTER(C.c_int8)))
+        return out if ok else None
+
+    def lengths_from_coords(self, coords, minkowski=False):
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        out = np.zeros(self.nsimplices(1))
+        lib().orc_lengths_from_coords(self._h, _p(coords, f64p), int(minkowski), _p(out, f64p))
+        return out
+
+
+def cell_metric(d, s):
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    g, gi, det = np.zeros(d * d), np.zeros(d * d), np.zeros(1)
+    lib().orc_cell_metric(d, _p(s, f64p), _p(g, f64p), _p(gi, f64p), _p(det, f64p))
+    return g.reshape(d, d), gi.reshape(d, d), float(det[0])
+
+
+def element_matrix(d, kind, grade, s, L=None):
+    r, c = form_shape(d, kind, grade)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    out = np.zeros(max(r * c, 1))
+    (L or lib()).orc_element_matrix(d, kind, grade, _p(s, f64p), _p(out, f64p))
+    return out[: r * c].reshape(r, c)
+
+
+def unit_lengths(d):
+    """SimplexLengthsSq::unit (regge/src/lengths/simplex.rs:81-88): edges from
+    vertex 0 have squared length 1, the others 2."""
+    edges = local_subsets(d, 1)
+    return np.array([1.0 if e[0] == 0 else 2.0 for e in edges])
+
+
+def element_matrices(cx, lengths, kind, grade, nthreads=0):
+    r, c = form_shape(cx.dim, kind, grade)
+    lengths = np.ascontiguousarray(lengths, dtype=np.float64)
+    out = np.zeros((cx.nsimplices(cx.dim), r, c))
+    lib().orc_element_matrices(cx._h, _p(lengths, f64p), kind, grade, _p(out, f64p), nthreads)
+    return out
+
+
+def assemble(cx, lengths, kind, grade, nthreads=0):
+    lengths = np.ascontiguousarray(lengths, dtype=np.float64)
+    return Csr._take(lib().orc_assemble(cx._h, _p(lengths, f64p), kind, grade, nthreads))
+
+
+@dataclass
+class Report:
+    iters: int
+    residual: float
+    converged: bool
+
+
+def _krylov(fn, a, b, inv_diag, rtol, max_iters):
+    s = a._struct()
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(a.shape[0])
+    idg = None if inv_diag is None else np.ascontiguousarray(inv_diag, dtype=np.float64)
+    rep = fn(C.byref(s), None if idg is None else _p(idg, f64p), _p(b, f64p), rtol, max_iters, _p(x, f64p))
+    return x, Report(int(rep.iters), float(rep.residual), bool(rep.converged))
+
+
+def cg(a, b, inv_diag=None, rtol=1e-10, max_iters=10_000):
+    return _krylov(lib().orc_cg, a, b, inv_diag, rtol, max_iters)
+
+
+def minres(a, b, inv_diag=None, rtol=1e-10, max_iters=10_000):
+    return _krylov(lib().orc_minres, a, b, inv_diag, rtol, max_iters)
+
+
+# --------------------------------------------------------------------------
+# the discrete Hilbert complex and the problems built on it
+# --------------------------------------------------------------------------
+
+class WhitneyComplex:
+    """HilbertComplex over a Whitney complex (full or relative)."""
+
+    def __init__(self, cx: Complex, lengths, relative=False):
+        self.cx = cx
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.float64)
+        self.relative = relative
+        self.dim = cx.dim
+        self._interior = [None] * (cx.dim + 1)
+        if relative:
+            for k in range(cx.dim + 1):
+                self._interior[k] = (cx.boundary_flags(k) == 0).astype(np.uint8)
+
+    def in_range(self, k):
+        return 0 <= k <= self.dim
+
+    def ndofs(self, k):
+        if not self.in_range(k):
+            return 0
+        if self.relative:
+            return int(self._interior[k].sum())
+        return self.cx.nsimplices(k)
+
+    def interior(self, k):
+        return self._interior[k]
+
+    def _assemble(self, kind, grade):
+        tg = grade - 1 if kind in (DIF_TEST, DIF_BOTH) else grade
+        rg = grade - 1 if kind in (DIF_TRIAL, DIF_BOTH) else grade
+        rows, cols = self.ndofs(tg), self.ndofs(rg)
+        if rows == 0 or cols == 0:
+            return Csr.zeros(rows, cols)
+        a = assemble(self.cx, self.lengths, kind, grade)
+        if self.relative:
+            a = a.select(self._interior[tg], self._interior[rg])
+        return a
+
+    def mass(self, k):
+        return self._assemble(MASS, k)
+
+    def dif_trial(self, k):
+        return self._assemble(DIF_TRIAL, k)
+
+    def dif_test(self, k):
+        return self._assemble(DIF_TEST, k)
+
+    def dif_both(self, k):
+        return self._assemble(DIF_BOTH, k)
+
+    def hdif_gram(self, k):
+        return self.mass(k) + self.dif_both(k + 1)
+
+    def dif(self, k):
+        if not (0 <= k < self.dim):
+            return Csr.zeros(self.ndofs(k + 1), self.ndofs(k))
+        d = self.cx.coboundary(k)
+        if self.relative:
+            d = d.select(self._interior[k + 1], self._interior[k])
+        return d
+
+    def inclusion_indices(self, k):
+        if self.relative:
+            return np.nonzero(self._interior[k])[0]
+        return np.arange(self.ndofs(k))
+
+    def harmonics(self, k):
+        """An M_k-orthonormal harmonic basis on a ball: b_0 = 1 on the full
+        complex (the constant), b_n = 1 on the relative one (M_n^{-1} times the
+        orientation, dirac.rs:553-568), zero otherwise."""
+        n = self.ndofs(k)
+        if not self.relative and k == 0:
+            h = np.ones(n)
+        elif self.relative and k == self.dim:
+            o = self.cx.orientation().astype(np.float64)
+            m = self.mass(k).diagonal()
+            h = o / m
+        else:
+            return np.zeros((n, 0))
+        m = self.mass(k)
+        h = h / math.sqrt(h @ (m @ h))
+        return h.reshape(n, 1)
+
+
+def hodge_blocks(c: WhitneyComplex, k):
+    return dict(n_sigma=c.ndofs(k - 1), n_u=c.ndofs(k), mass_sigma=c.mass(k - 1),
+                mass_u=c.mass(k), dif_test=c.dif_test(k), dif_both=c.dif_both(k + 1))
+
+
+def assemble_mixed_kkt(c: WhitneyComplex, source, k, harmonics):
+    """elliptic.rs:54-114: [[-M_{k-1}, B, 0], [B^T, K, M H], [0, H^T M, 0]],
+    rhs [0; E^T f; 0]."""
+    b = hodge_blocks(c, k)
+    ns, nu, nh = b["n_sigma"], b["n_u"], harmonics.shape[1]
+    mh = b["mass_u"].to_scipy() @ harmonics if nh else np.zeros((nu, 0))
+    blocks = [[b["mass_sigma"], b["dif_test"], None],
+              [b["dif_test"].T, b["dif_both"], Csr.from_dense_rows(mh) if nh else None],
+              [None, Csr.from_dense_rows(mh.T) if nh else None, None]]
+    scales = [[-1.0, -1.0, 1.0], [1.0, 1.0, 1.0], [1.0, 1.0, 1.0]]
+    # mixed_hodge_laplacian has -dif_test in the sigma row; the sign flip of
+    # the sigma rows turns [M, -B] into [-M, B].
+    scales[0][1] = 1.0
+    kkt = block(blocks, scales, [ns, nu, nh], [ns, nu, nh])
+    src = np.asarray(source, dtype=np.float64)[c.inclusion_indices(k)]
+    rhs = np.concatenate([np.zeros(ns), src, np.zeros(nh)])
+    return kkt, rhs, ns, nu
+
+
+def mixed_jacobi(c: WhitneyComplex, k, nh):
+    """Inverse diagonal of the block preconditioner diag(hdif_gram(k-1),
+    hdif_gram(k), I) (elliptic.rs:29-47), Jacobi-approximated per block."""
+    parts = []
+    if k > 0:
+        parts.append(1.0 / c.hdif_gram(k - 1).diagonal())
+    parts.append(1.0 / c.hdif_gram(k).diagonal())
+    parts.append(np.ones(nh))
+    return np.concatenate(parts)
+
+
+def solve_source(c: WhitneyComplex, source, k, precond="jacobi", rtol=1e-10, max_iters=100_000):
+    """elliptic.rs:132-182 with the block preconditioner applied by Jacobi
+    ("jacobi") or by sparse LU of each block ("direct", the reference's
+    DirectInverse with SuperLU standing in for faer). Returns sigma, u, p (ambient
+    cochains) and the MINRES report."""
+    h = c.harmonics(k)
+    kkt, rhs, ns, nu = assemble_mixed_kkt(c, source, k, h)
+    if precond == "jacobi":
+        x, rep = minres(kkt, rhs, mixed_jacobi(c, k, h.shape[1]), rtol, max_iters)
+    elif precond == "direct":
+        x, rep = _minres_direct_blocks(c, k, kkt, rhs, h.shape[1], rtol, max_iters)
+    else:
+        raise ValueError(precond)
+    sig = np.zeros(c.cx.nsimplices(k - 1) if k > 0 else 0)
+    if k > 0:
+        sig[c.inclusion_indices(k - 1)] = x[:ns]
+    u = np.zeros(c.cx.nsimplices(k))
+    u[c.inclusion_indices(k)] = x[ns:ns + nu]
+    return sig, u, x[ns + nu:], rep
+
+
+def _minres_direct_blocks(c, k, kkt, rhs, nh, rtol, max_iters):
+    """MINRES (krylov.rs:113-211) in Python with the block-direct
+    preconditioner; SuperLU stands in for faer's Cholesky."""
+    import scipy.sparse.linalg as spla
+    facs = []
+    if k > 0:
+        facs.append(spla.splu(c.hdif_gram(k - 1).to_scipy().tocsc()))
+    facs.append(spla.splu(c.hdif_gram(k).to_scipy().tocsc()))
+    sizes = [f.shape[0] for f in facs] + [nh]
+
+    def prec(r):
+        out, off = np.empty_like(r), 0
+        for i, n in enumerate(sizes):
+            out[off:off + n] = facs[i].solve(r[off:off + n]) if i < len(facs) else r[off:off + n]
+            off += n
+        return out
+
+    A = kkt.to_scipy()
+    return minres_py(lambda v: A @ v, prec, rhs, rtol, max_iters)
+
+
+def minres_py(op, prec, b, rtol=1e-10, max_iters=10_000):
+    """krylov.rs:113-211 with callables (for non-diagonal preconditioners)."""
+    r1 = b.copy()
+    y = prec(r1)
+    beta1_sq = r1 @ y
+    x = np.zeros_like(b)
+    if beta1_sq <= 0:
+        return x, Report(0, 0.0, True)
+    beta1 = math.sqrt(beta1_sq)
+    oldb, beta, dbar, epsln, phibar, cs, sn = 0.0, beta1, 0.0, 0.0, beta1, -1.0, 0.0
+    w, w2, r2 = np.zeros_like(b), np.zeros_like(b), r1.copy()
+    residual, iters = 1.0, 0
+    while iters < max_iters:
+        iters += 1
+        v = y * (1.0 / beta)
+        yn = op(v)
+        if iters >= 2:
+            yn = yn + (-beta / oldb) * r1
+        alfa = v @ yn
+        yn = yn + (-alfa / beta) * r2
+        r1, r2 = r2, yn
+        y = prec(r2)
+        oldb = beta
+        beta = math.sqrt(max(r2 @ y, 0.0))
+        oldeps = epsln
+        delta = cs * dbar + sn * alfa
+        gbar = sn * dbar - cs * alfa
+        epsln = sn * beta
+        dbar = -cs * beta
+        gamma = max(math.sqrt(gbar * gbar + beta * beta), 2.220446049250313e-16)
+        cs, sn = gbar / gamma, beta / gamma
+        phi = cs * phibar
+        phibar *= sn
+        wn = (v - oldeps * w2 - delta * w) * (1.0 / gamma)
+        w2, w = w, wn
+        x = x + phi * wn
+        residual = phibar / beta1
+        if residual <= rtol:
+            return x, Report(iters, residual, True)
+    return x, Report(iters, residual, False)
+
+
+def hodge_dirac(c: WhitneyComplex, sign):
+    """HodgeDirac::assemble_signed (dirac.rs:213-258): offsets, mass_block and
+    op with U_k = M_k D_{k-1} below the diagonal and sign * U_k^T above."""
+    n = c.dim
+    masses = [c.mass(k) for k in range(n + 1)]
+    difs = [c.dif(k) for k in range(n)]
+    offsets = np.cumsum([0] + [m.shape[0] for m in masses])
+    nb = n + 1
+    mb = [[masses[i] if i == j else None for j in range(nb)] for i in range(nb)]
+    sizes = [m.shape[0] for m in masses]
+    mass_block = block(mb, None, sizes, sizes)
+    # op: emit (r,c,v) and (c,r,sign v) from the same triplets, then compress.
+    import scipy.sparse as sp
+    rows, cols, vals = [], [], []
+    for k in range(1, n + 1):
+        u = masses[k] @ difs[k - 1]
+        coo_r = np.repeat(np.arange(u.shape[0]), np.diff(u.row_ptr))
+        r0, c0 = offsets[k], offsets[k - 1]
+        rows += [r0 + coo_r, c0 + u.col]
+        cols += [c0 + u.col, r0 + coo_r]
+        vals += [u.val, sign * u.val]
+    total = int(offsets[-1])
+    if rows:
+        r = np.concatenate(rows)
+        cc = np.concatenate(cols)
+        v = np.concatenate(vals)
+    else:
+        r = cc = np.zeros(0, np.int64)
+        v = np.zeros(0)
+    # No duplicate (r, c) pairs across the emitted blocks, so compression is a
+    # pure reorder and scipy's conversion changes no value.
+    op = sp.csr_matrix((v, (r, cc)), shape=(total, total))
+    op.sort_indices()
+    opc = Csr((total, total), op.indptr.astype(np.int64), op.indices.astype(np.int64), op.data)
+    return offsets, mass_block, opc
