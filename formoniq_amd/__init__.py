@@ -393,6 +393,17 @@ class Problem:
     def assemble(self):
         check(load().feec_problem_assemble(self._h))
 
+    def kernel_times(self, reps=20):
+        """Seconds per launch: element kernel, KKT scatter, FP32 and FP64 MINRES SpMV."""
+        out = np.zeros(4)
+        check(load().feec_problem_kernel_times(self._h, reps, _ptr(out, _lib.f64p)))
+        return dict(elmat=out[0], scatter=out[1], spmv32=out[2], spmv64=out[3])
+
+    def upload_source(self, source):
+        """Make `source` (N_k) resident on the device for solve()."""
+        src = np.ascontiguousarray(source, dtype=np.float64)
+        check(load().feec_problem_upload_source(self._h, _ptr(src, _lib.f64p)))
+
     def source_buffer(self):
         p = C.c_void_p()
         check(load().feec_problem_source_buffer(self._h, C.byref(p)))

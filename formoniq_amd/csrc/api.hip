@@ -300,13 +300,16 @@ void problem_build(Problem& P) {
   ctx->sync();
 }
 
-void problem_assemble(Problem& P) {
+void problem_element_outputs(Problem& P, double** outs) {
+  for (int i = 0; i < 5; ++i) outs[i] = P.el.p + P.off[i];
+}
+
+// CSR values, preconditioner diagonal and FP32 copy from the element blocks
+// already in P.el.
+void problem_assemble_from_elements(Problem& P) {
   Geom& g = *P.g;
   Mesh& m = *g.mesh;
   Ctx* ctx = m.ctx;
-  double* outs[5] = {P.el.p + P.off[0], P.el.p + P.off[1], P.el.p + P.off[2], P.el.p + P.off[3],
-                     P.el.p + P.off[4]};
-  kkt_element_blocks(g, P.k, outs, true);
   bool rebuilt = assemble_compacted(ctx, P.kkt_plan, P.el.p, P.kkt_map, P.sys.A);
   coo_plan_dense_column(ctx, P.diag_plan, P.el.p, P.diag.p);
   k_inv_diag<<<grid_for(P.sys.n + P.nh), TPB, 0, ctx->stream>>>(P.diag.p, P.sys.n, P.nh,
@@ -320,11 +323,69 @@ void problem_assemble(Problem& P) {
   ctx->sync();
 }
 
+void problem_assemble(Problem& P) {
+  double* outs[5];
+  problem_element_outputs(P, outs);
+  kkt_element_blocks(*P.g, P.k, outs, true);
+  problem_assemble_from_elements(P);
+}
+
 }  // namespace feec
 
 using namespace feec;
 
 struct feec_problem_s : feec::Problem {};
+
+namespace feec {
+
+Problem* problem_new(Geom* g, int bc, int k, const feec_solve_opts& o) {
+  auto* P = new feec_problem_s();
+  P->g = g;
+  P->bc = bc;
+  P->k = k;
+  P->opts = o;
+  try {
+    problem_build(*P);
+  } catch (...) {
+    delete P;
+    throw;
+  }
+  return P;
+}
+
+void problem_delete(Problem* P) { delete static_cast<feec_problem_s*>(P); }
+
+int64_t problem_ndofs(Problem* P) { return P->sys.n + P->nh; }
+
+// Upload a host source, solve; the report says whether it converged.
+feec_report problem_solve_host(Problem* P, const double* src) {
+  Mesh& m = *P->g->mesh;
+  Ctx* ctx = m.ctx;
+  FEEC_HIP(hipMemcpy(P->source.p, src, m.n[P->k] * 8, hipMemcpyHostToDevice));
+  const int64_t N = P->sys.n + P->nh;
+  FEEC_HIP(hipMemsetAsync(P->rhs.p, 0, N * 8, ctx->stream));
+  if (P->bc == FEEC_RELATIVE) m.ensure_interior();
+  const int32_t* map = (P->bc == FEEC_RELATIVE) ? m.interior[P->k].p : nullptr;
+  k_rhs<<<grid_for(m.n[P->k]), TPB, 0, ctx->stream>>>(P->source.p, map, m.n[P->k], P->ns, N,
+                                                      P->rhs.p);
+  FEEC_HIP(hipGetLastError());
+  feec_report r = kkt_solve(P->sys, P->rhs.p, P->x.p, P->opts, &P->ws);
+  P->last = r;
+  return r;
+}
+
+void problem_u(Problem* P, double* u) {
+  Mesh& m = *P->g->mesh;
+  Ctx* ctx = m.ctx;
+  if (P->bc == FEEC_RELATIVE) m.ensure_interior();
+  DBuf<double> o(ctx, m.n[P->k]);
+  const int32_t* map = (P->bc == FEEC_RELATIVE) ? m.interior[P->k].p : nullptr;
+  k_extend<<<grid_for(m.n[P->k]), TPB, 0, ctx->stream>>>(P->x.p, map, m.n[P->k], P->ns, o.p);
+  ctx->sync();
+  FEEC_HIP(hipMemcpy(u, o.p, m.n[P->k] * 8, hipMemcpyDeviceToHost));
+}
+
+}  // namespace feec
 
 // ===========================================================================
 // C ABI
@@ -898,6 +959,14 @@ feec_status feec_problem_source_buffer(feec_problem* p, double** ptr) {
   return guard([&] { *ptr = p->source.p; });
 }
 
+feec_status feec_problem_upload_source(feec_problem* p, const double* src) {
+  return guard([&] {
+    Mesh& m = *p->g->mesh;
+    m.ctx->sync();
+    FEEC_HIP(hipMemcpy(p->source.p, src, m.n[p->k] * 8, hipMemcpyHostToDevice));
+  });
+}
+
 feec_status feec_problem_sizes(const feec_problem* p, int64_t sizes[3], int64_t* nnz) {
   return guard([&] {
     sizes[0] = p->ns;
@@ -955,6 +1024,39 @@ feec_status feec_problem_solution(feec_problem* p, double* sigma, double* u, dou
       FEEC_HIP(hipMemcpy(u, o.p, m.n[k] * 8, hipMemcpyDeviceToHost));
     }
     if (ph && p->nh) FEEC_HIP(hipMemcpy(ph, p->x.p + p->sys.n, p->nh * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+feec_status feec_problem_kernel_times(feec_problem* p, int reps, double out[4]) {
+  return guard([&] {
+    Mesh& m = *p->g->mesh;
+    Ctx* ctx = m.ctx;
+    hipStream_t st = ctx->stream;
+    hipEvent_t e0, e1, e2;
+    FEEC_HIP(hipEventCreate(&e0));
+    FEEC_HIP(hipEventCreate(&e1));
+    FEEC_HIP(hipEventCreate(&e2));
+    double* outs[5] = {p->el.p + p->off[0], p->el.p + p->off[1], p->el.p + p->off[2],
+                       p->el.p + p->off[3], p->el.p + p->off[4]};
+    kkt_element_blocks(*p->g, p->k, outs, true);
+    DBuf<double> v(ctx, std::max<int64_t>(p->kkt_plan.nnz, 1));
+    coo_plan_values(ctx, p->kkt_plan, p->el.p, v.p, nullptr);
+    FEEC_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < reps; ++i) kkt_element_blocks(*p->g, p->k, outs, false);
+    FEEC_HIP(hipEventRecord(e1, st));
+    for (int i = 0; i < reps; ++i) coo_plan_values(ctx, p->kkt_plan, p->el.p, v.p, nullptr);
+    FEEC_HIP(hipEventRecord(e2, st));
+    FEEC_HIP(hipEventSynchronize(e2));
+    float a = 0.f, b = 0.f;
+    FEEC_HIP(hipEventElapsedTime(&a, e0, e1));
+    FEEC_HIP(hipEventElapsedTime(&b, e1, e2));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    out[0] = a * 1e-3 / reps;
+    out[1] = b * 1e-3 / reps;
+    out[2] = time_minres_spmv(p->sys, true, reps);
+    out[3] = time_minres_spmv(p->sys, false, reps);
   });
 }
 

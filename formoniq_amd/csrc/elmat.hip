@@ -395,20 +395,30 @@ __device__ __forceinline__ double minor_det(const double* sG, int nv, const uint
   }
 }
 
+// Coefficient tables of one dimension, as device pointers.
+struct TabPtrs {
+  const uint8_t* pairs;
+  const int32_t* tstart;
+  const uint16_t* th;
+  const int16_t* tc;
+};
+
+// One wavefront, 64 consecutive cells starting at cell0.
 template <int D>
-__global__ void __launch_bounds__(WAVE) k_elmat(const int32_t* __restrict__ edges,
-                                                const double* __restrict__ len, int64_t nc, Job job,
-                                                const uint8_t* __restrict__ pairs,
-                                                const int32_t* __restrict__ tstart,
-                                                const uint16_t* __restrict__ th,
-                                                const int16_t* __restrict__ tc, int* degenerate) {
+__device__ __forceinline__ void elmat_group(const int32_t* __restrict__ edges,
+                                            const double* __restrict__ len, int64_t nc,
+                                            int64_t cell0, const Job& job, const TabPtrs& T,
+                                            int* degenerate, double* lds) {
+  const uint8_t* __restrict__ pairs = T.pairs;
+  const int32_t* __restrict__ tstart = T.tstart;
+  const uint16_t* __restrict__ th = T.th;
+  const int16_t* __restrict__ tc = T.tc;
   constexpr int NV = D + 1;
   constexpr int NE = binom(D + 1, 2);
-  extern __shared__ double lds[];
-  double* sG = lds;                 // [NV*NV][WAVE]
+  double* sG = lds;                   // [NV*NV][WAVE]
   double* sH = lds + NV * NV * WAVE;  // [nH][WAVE]
   const int lane = threadIdx.x;
-  const int64_t cell = blockIdx.x * (int64_t)WAVE + lane;
+  const int64_t cell = cell0 + lane;
   const bool active = cell < nc;
   double s[NE];
 #pragma unroll
@@ -451,6 +461,54 @@ __global__ void __launch_bounds__(WAVE) k_elmat(const int32_t* __restrict__ edge
         for (int t = st[e]; t < t1; ++t) acc += (double)tc[t] * sH[th[t] * WAVE + lane];
         if (active) out[(int64_t)e * nc + cell] = scale * acc;
       }
+    }
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(WAVE) k_elmat(const int32_t* __restrict__ edges,
+                                                const double* __restrict__ len, int64_t nc, Job job,
+                                                TabPtrs T, int* degenerate) {
+  extern __shared__ double lds[];
+  elmat_group<D>(edges, len, nc, blockIdx.x * (int64_t)WAVE, job, T, degenerate, lds);
+}
+
+// Many meshes of mixed dimension in one persistent launch: every wavefront
+// takes (mesh, 64-cell range) items from a global counter until the queue is
+// empty, so the grid drains whatever the mix of sizes.
+struct BatchMesh {
+  int D;
+  int64_t nc;
+  const int32_t* edges;
+  const double* len;
+  int64_t item0;  // first work item of this mesh
+  Job job;
+};
+
+__global__ void __launch_bounds__(WAVE) k_elmat_batched(const BatchMesh* __restrict__ meshes,
+                                                        int nmeshes, int64_t nitems,
+                                                        unsigned long long* counter, TabPtrs T2,
+                                                        TabPtrs T3, TabPtrs T4, int* degenerate) {
+  extern __shared__ double lds[];
+  __shared__ long long item;
+  for (;;) {
+    if (threadIdx.x == 0) item = (long long)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const long long it = item;
+    __syncthreads();
+    if (it >= nitems) break;
+    int lo = 0, hi = nmeshes - 1;  // last mesh with item0 <= it
+    while (lo < hi) {
+      int mid = (lo + hi + 1) / 2;
+      if (meshes[mid].item0 <= it) lo = mid; else hi = mid - 1;
+    }
+    const BatchMesh& m = meshes[lo];
+    const int64_t cell0 = (it - m.item0) * WAVE;
+    switch (m.D) {
+      case 2: elmat_group<2>(m.edges, m.len, m.nc, cell0, m.job, T2, degenerate, lds); break;
+      case 3: elmat_group<3>(m.edges, m.len, m.nc, cell0, m.job, T3, degenerate, lds); break;
+      case 4: elmat_group<4>(m.edges, m.len, m.nc, cell0, m.job, T4, degenerate, lds); break;
+      default: break;
     }
   }
 }
@@ -556,15 +614,22 @@ void run_job(Geom& g, Job& job, bool check) {
   FEEC_HIP(hipMemsetAsync(deg, 0, sizeof(int), ctx->stream));
   unsigned grid = (unsigned)std::max<int64_t>(1, (nc + WAVE - 1) / WAVE);
   const int32_t* edges = m.faces[1].p;
+  TabPtrs tp{T.pairs.p, T.start.p, T.h.p, T.c.p};
   switch (D) {
-    case 1: k_elmat<1><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, T.pairs.p, T.start.p, T.h.p, T.c.p, deg); break;
-    case 2: k_elmat<2><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, T.pairs.p, T.start.p, T.h.p, T.c.p, deg); break;
-    case 3: k_elmat<3><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, T.pairs.p, T.start.p, T.h.p, T.c.p, deg); break;
-    case 4: k_elmat<4><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, T.pairs.p, T.start.p, T.h.p, T.c.p, deg); break;
+    case 1: k_elmat<1><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, tp, deg); break;
+    case 2: k_elmat<2><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, tp, deg); break;
+    case 3: k_elmat<3><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, tp, deg); break;
+    case 4: k_elmat<4><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, tp, deg); break;
   }
   FEEC_HIP(hipGetLastError());
   if (check) check_degenerate(ctx, deg);
 }
+
+Job kkt_job(DevTables& T, int d, int k, double* const* outs);
+
+// Autotuned persistent grid (in wavefronts) per device, chosen by timing.
+std::mutex g_tune_mu;
+std::map<int, unsigned> g_tuned_waves;
 
 void job_add_grade(Job& job, int K) {
   for (int i = 0; i < job.ngrades; ++i)
@@ -592,11 +657,8 @@ void element_matrices_soa(Geom& g, int kind, int grade, double* out_dev) {
   run_job(g, job, true);
 }
 
-// outs: mass_sigma (M_{k-1}), dif_test(k), dif_both(k+1), diag(dif_both(k)), diag(M_k)
-void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check) {
-  Mesh& m = *g.mesh;
-  const int d = m.d, k = grade;
-  DevTables& T = tables_for(m.ctx, d);
+namespace {
+Job kkt_job(DevTables& T, int d, int k, double* const* outs) {
   Job job{};
   if (k >= 1) {
     job_add_block(job, T, k - 1, 0, outs[0]);
@@ -605,7 +667,98 @@ void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check) {
   }
   job_add_block(job, T, k, KIND_DIAG_MASS, outs[4]);
   if (k + 1 <= d) job_add_block(job, T, k + 1, 3, outs[2]);
+  for (int i = 0; i < job.ngrades; ++i) {
+    job.pair_off[i] = T.pair_off[job.grades[i]];
+    job.nH[i] = T.nH[job.grades[i]];
+  }
+  return job;
+}
+}  // namespace
+
+// outs: mass_sigma (M_{k-1}), dif_test(k), dif_both(k+1), diag(dif_both(k)), diag(M_k)
+void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check) {
+  Mesh& m = *g.mesh;
+  DevTables& T = tables_for(m.ctx, m.d);
+  Job job = kkt_job(T, m.d, grade, outs);
   run_job(g, job, check);
+}
+
+// Element blocks of many KKT problems (dimensions 2..4) in one persistent
+// launch on ctx's stream. The grid size is picked once per device by timing a
+// few candidates on the first batch (the launch is idempotent).
+double kkt_element_blocks_batched(Ctx* ctx, const std::vector<BatchElementRequest>& reqs) {
+  if (reqs.empty()) return 0.0;
+  std::vector<BatchMesh> host(reqs.size());
+  int64_t items = 0;
+  int maxH = 1, maxD = 2;
+  for (size_t i = 0; i < reqs.size(); ++i) {
+    Mesh& m = *reqs[i].g->mesh;
+    FEEC_REQUIRE(m.d >= 2 && m.d <= 4, FEEC_E_UNSUPPORTED, "batched assembly covers dimensions 2..4");
+    DevTables& T = tables_for(ctx, m.d);
+    BatchMesh& b = host[i];
+    b.D = m.d;
+    b.nc = m.n[m.d];
+    b.edges = m.faces[1].p;
+    b.len = reqs[i].g->lengths.p;
+    b.item0 = items;
+    b.job = kkt_job(T, m.d, reqs[i].grade, reqs[i].outs);
+    for (int gi = 0; gi < b.job.ngrades; ++gi) maxH = std::max(maxH, b.job.nH[gi]);
+    maxD = std::max(maxD, m.d);
+    items += (b.nc + WAVE - 1) / WAVE;
+  }
+  DevTables& T2 = tables_for(ctx, 2);
+  DevTables& T3 = tables_for(ctx, 3);
+  DevTables& T4 = tables_for(ctx, 4);
+  TabPtrs t2{T2.pairs.p, T2.start.p, T2.h.p, T2.c.p}, t3{T3.pairs.p, T3.start.p, T3.h.p, T3.c.p},
+      t4{T4.pairs.p, T4.start.p, T4.h.p, T4.c.p};
+  DBuf<BatchMesh> dm(ctx, host.size());
+  FEEC_HIP(hipMemcpyAsync(dm.p, host.data(), host.size() * sizeof(BatchMesh), hipMemcpyHostToDevice,
+                          ctx->stream));
+  const size_t lds = (size_t)((maxD + 1) * (maxD + 1) + maxH) * WAVE * sizeof(double);
+  FEEC_HIP(hipMemsetAsync(ctx->dflag, 0, sizeof(int), ctx->stream));
+  auto launch = [&](unsigned waves) {
+    FEEC_HIP(hipMemsetAsync(ctx->dcount, 0, sizeof(unsigned long long), ctx->stream));
+    k_elmat_batched<<<waves, WAVE, lds, ctx->stream>>>(dm.p, (int)host.size(), items, ctx->dcount,
+                                                       t2, t3, t4, ctx->dflag);
+    FEEC_HIP(hipGetLastError());
+  };
+  unsigned waves = 0;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tuned_waves.find(ctx->device);
+    if (it != g_tuned_waves.end()) waves = it->second;
+  }
+  hipEvent_t e0, e1;
+  FEEC_HIP(hipEventCreate(&e0));
+  FEEC_HIP(hipEventCreate(&e1));
+  float best_ms = 0.f;
+  if (waves == 0) {
+    const unsigned cand[] = {1024, 2048, 4096, 8192, 16384};
+    float best = 1e30f;
+    for (unsigned c : cand) {
+      FEEC_HIP(hipEventRecord(e0, ctx->stream));
+      launch(c);
+      FEEC_HIP(hipEventRecord(e1, ctx->stream));
+      FEEC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        waves = c;
+      }
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned_waves[ctx->device] = waves;
+  }
+  FEEC_HIP(hipEventRecord(e0, ctx->stream));
+  launch(waves);
+  FEEC_HIP(hipEventRecord(e1, ctx->stream));
+  FEEC_HIP(hipEventSynchronize(e1));
+  FEEC_HIP(hipEventElapsedTime(&best_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  check_degenerate(ctx, ctx->dflag);
+  return best_ms * 1e-3;
 }
 
 int dirac_element_count(int d) {

@@ -237,6 +237,13 @@ int element_cols(int d, int kind, int grade);
 void cell_metrics(Geom& g, double* gmat, double* ginv, double* det);
 void elmat_release_tables(Ctx* ctx);
 void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check);
+struct BatchElementRequest {
+  Geom* g;
+  int grade;
+  double* outs[5];
+};
+// All requests in one persistent launch; returns the kernel's seconds.
+double kkt_element_blocks_batched(Ctx* ctx, const std::vector<BatchElementRequest>& reqs);
 int dirac_element_count(int d);
 void dirac_element_matrices(Geom& g, double* out_dev, bool check);
 void causal_census(Geom& g, int64_t counts[3], uint8_t* riem_dev);
@@ -267,6 +274,7 @@ feec_report kkt_solve(KktSystem& s, const double* b_dev, double* x_dev, const fe
                       SolverState** ws);
 void solver_state_free(SolverState* ws);
 void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y);
+double time_minres_spmv(KktSystem& s, bool fp32, int reps);
 KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
                      double rtol, int64_t max_iters);
 

@@ -696,6 +696,53 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
   return rep;
 }
 
+// Average duration of the fused MINRES SpMV launch (the kernel that moves the
+// matrix) on this system, FP32 or FP64, from HIP events on the context stream.
+double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
+  Ctx* ctx = s.ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n, N = s.n + s.nh;
+  if (fp32 && !s.val32.p) kkt_refresh_fp32(s);
+  DBuf<MinresS> S(ctx, 1);
+  MinresS h{};
+  h.beta = 1.0;
+  h.oldb = 1.0;
+  h.iter = 1;
+  h.done = 0;
+  FEEC_HIP(hipMemcpy(S.p, &h, sizeof(h), hipMemcpyHostToDevice));
+  DBuf<double> part(ctx, NBLK);
+  hipEvent_t e0, e1;
+  FEEC_HIP(hipEventCreate(&e0));
+  FEEC_HIP(hipEventCreate(&e1));
+  float ms = 0.f;
+  if (fp32) {
+    DBuf<float> y(ctx, N), r(ctx, N);
+    FEEC_HIP(hipMemsetAsync(y.p, 0, N * 4, st));
+    FEEC_HIP(hipMemsetAsync(r.p, 0, N * 4, st));
+    k_mr_spmv<float><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.val32.p, n, y.p, r.p, S.p, part.p);
+    FEEC_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < reps; ++i)
+      k_mr_spmv<float><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.val32.p, n, y.p, r.p, S.p, part.p);
+    FEEC_HIP(hipEventRecord(e1, st));
+    FEEC_HIP(hipEventSynchronize(e1));
+  } else {
+    DBuf<double> y(ctx, N), r(ctx, N);
+    FEEC_HIP(hipMemsetAsync(y.p, 0, N * 8, st));
+    FEEC_HIP(hipMemsetAsync(r.p, 0, N * 8, st));
+    k_mr_spmv<double><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.A.val.p, n, y.p, r.p, S.p, part.p);
+    FEEC_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < reps; ++i)
+      k_mr_spmv<double><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.A.val.p, n, y.p, r.p, S.p, part.p);
+    FEEC_HIP(hipEventRecord(e1, st));
+    FEEC_HIP(hipEventSynchronize(e1));
+  }
+  FEEC_HIP(hipGetLastError());
+  FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms * 1e-3 / std::max(reps, 1);
+}
+
 void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y) {
   if (a.nrows == 0) return;
   k_spmv<double><<<spmv_grid(a.nrows), TPB, 0, ctx->stream>>>(a.row_ptr.p, a.col.p, a.val.p,

@@ -202,6 +202,12 @@ feec_status feec_problem_solution(feec_problem *prob, double *sigma, double *u, 
 feec_status feec_problem_sizes(const feec_problem *prob, int64_t sizes[3], int64_t *nnz);
 /* Device pointer to an N_k scratch vector for feec_problem_solve_device. */
 feec_status feec_problem_source_buffer(feec_problem *prob, double **device_ptr);
+/* Copy an N_k host source into that device buffer. */
+feec_status feec_problem_upload_source(feec_problem *prob, const double *source_host);
+/* Diagnostics (no counterpart in the reference): average seconds per launch,
+ * from HIP events on the context stream, of {element kernel, KKT value
+ * scatter, fused MINRES SpMV in FP32, the same in FP64}. */
+feec_status feec_problem_kernel_times(feec_problem *prob, int reps, double out[4]);
 
 /* ---- batching and serving -------------------------------------------------- */
 /* One request: a jittered Kuhn box and a Hodge-Laplace source solve. */

@@ -837,6 +837,9 @@ orc_csr *orc_csr_block(int nb, const orc_csr *const *blocks, const double *scale
 }
 
 void orc_spmv(const orc_csr *a, const double *x, double *y) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (a->nnz > 200000)
+#endif
   for (int64_t i = 0; i < a->nrows; ++i) {
     double acc = 0.0;
     for (int64_t p = a->row_ptr[i]; p < a->row_ptr[i + 1]; ++p) acc += a->val[p] * x[a->col[p]];
@@ -1028,6 +1031,9 @@ orc_csr *orc_assemble(const orc_complex *cx, const double *lengths, int kind, in
 
 static double dot(const double *a, const double *b, int64_t n) {
   double s = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) reduction(+ : s) if (n > 100000)
+#endif
   for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }

@@ -31,7 +31,7 @@ def test_numbering_bit_exact(d, n, amp):
             oc = ocx.coboundary(k)
             if bc == fa.RELATIVE:
                 oc = oc.select(ocx.boundary_flags(k + 1) == 0, ocx.boundary_flags(k) == 0)
-            assert_csr_parity(g, oc, 0.0, f"D_{k} bc={bc}")
+            assert_csr_parity(g, oc, 0.0, f"D_{k} bc={bc}", exact_pattern=True)
     np.testing.assert_array_equal(geo.lengths(), oL)
 
 
@@ -99,6 +99,21 @@ def test_galerkin_matrices(d, n, amp):
                 assert_csr_parity(gc.dif_test(k), oc.dif_test(k), 1e-12, f"dif_test {k} rel={rel}")
                 assert_csr_parity(gc.dif_both(k), oc.dif_both(k), 1e-12, f"dif_both {k} rel={rel}")
             assert_csr_parity(gc.hdif_gram(k), oc.hdif_gram(k), 1e-12, f"hdif_gram {k} rel={rel}")
+
+
+@pytest.mark.parametrize("d,n", [(2, 6), (3, 3), (4, 2)])
+def test_galerkin_patterns_bit_exact_when_no_cell_is_exact(d, n):
+    """Every vertex perturbed, boundary included: no element entry is zero
+    by cancellation, so every pattern must match the oracle exactly."""
+    ocx, oL, gcx, geo, _, _ = both(d, n, 0.0, seed=d, perturb_all=0.1)
+    for rel in (False, True):
+        oc = o.WhitneyComplex(ocx, oL, relative=rel)
+        gc = fa.WhitneyComplex(geo, relative=rel)
+        for k in range(d + 1):
+            assert_csr_parity(gc.mass(k), oc.mass(k), 1e-12, f"mass {k}", exact_pattern=True)
+            if k >= 1:
+                assert_csr_parity(gc.dif_test(k), oc.dif_test(k), 1e-12, f"dif_test {k}", exact_pattern=True)
+                assert_csr_parity(gc.dif_both(k), oc.dif_both(k), 1e-12, f"dif_both {k}", exact_pattern=True)
 
 
 def test_exact_zero_drop_unjittered():
