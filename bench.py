@@ -173,7 +173,9 @@ def main():
             tr = json.load(fh)
         if tr.get("n_kkt") == n_kkt:
             traffic = tr.get("hbm_bytes_per_launch")
-    elmat_bytes = ncells * (48 + 8 * (16 + 24 + 36 + 6 + 4))
+    # per tet: 6 edge lengths + 4 vertex indices in; M_0, dif_test(1), dif_both(2),
+    # two diagonals, M_1 and dif_both(1) out (138 doubles)
+    elmat_bytes = ncells * (48 + 8 * (16 + 24 + 36 + 6 + 4 + 36 + 16))
 
     line = {
         "metric": METRIC,
@@ -200,7 +202,7 @@ def main():
         "iters_mixed": int(reps[-1].iters), "refinements_mixed": int(reps[-1].refinements),
         "iters_fp64": int(r64.iters),
         "residual_mixed": float(reps[-1].residual), "residual_fp64": float(r64.residual),
-        "elmat_blocks_per_s": 5 * ncells / times["elmat"],
+        "elmat_blocks_per_s": 7 * ncells / times["elmat"],
         "elmat_cells_per_s": ncells / times["elmat"],
         "elmat_hbm_gbs": elmat_bytes / times["elmat"] / 1e9,
         "kernel_seconds": times,

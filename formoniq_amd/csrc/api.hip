@@ -67,8 +67,8 @@ struct Problem {
   int nfm = 0, nf = 0;
   // per-cell element blocks, one allocation: mass_sigma | dif_test | dif_both | diag_lo | diag_mk
   DBuf<double> el;
-  int64_t off[6] = {0, 0, 0, 0, 0, 0};
-  CooPlan kkt_plan, diag_plan;
+  int64_t off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  CooPlan kkt_plan, diag_plan, pc_plan;
   CompactMap kkt_map;
   DBuf<double> diag;  // 2 (ns + nu)
   KktSystem sys;
@@ -208,12 +208,15 @@ void problem_build(Problem& P) {
   const int nfm = P.nfm, nf = P.nf;
   const bool has_up = k + 1 <= d;
   // element block layout
-  int64_t sizes[5] = {(int64_t)nfm * nfm * nc, (int64_t)nfm * nf * nc, has_up ? (int64_t)nf * nf * nc : 0,
-                      (int64_t)nfm * nc, (int64_t)nf * nc};
+  // element blocks: 0 M_{k-1} | 1 dif_test(k) | 2 dif_both(k+1) | 3 diag dif_both(k) |
+  // 4 diag M_k | 5 M_k | 6 dif_both(k)
+  int64_t sizes[7] = {(int64_t)nfm * nfm * nc, (int64_t)nfm * nf * nc, has_up ? (int64_t)nf * nf * nc : 0,
+                      (int64_t)nfm * nc, (int64_t)nf * nc, (int64_t)nf * nf * nc,
+                      (int64_t)nfm * nfm * nc};
   P.off[0] = 0;
-  for (int i = 0; i < 5; ++i) P.off[i + 1] = P.off[i] + sizes[i];
-  FEEC_REQUIRE(P.off[5] < (int64_t(1) << 31), FEEC_E_ARG, "problem too large for 31-bit sources");
-  P.el.alloc(ctx, std::max<int64_t>(P.off[5], 1));
+  for (int i = 0; i < 7; ++i) P.off[i + 1] = P.off[i] + sizes[i];
+  FEEC_REQUIRE(P.off[7] < (int64_t(1) << 31), FEEC_E_ARG, "problem too large for 31-bit sources");
+  P.el.alloc(ctx, std::max<int64_t>(P.off[7], 1));
 
   // harmonic border
   std::vector<double> hb;
@@ -277,6 +280,45 @@ void problem_build(Problem& P) {
     FEEC_HIP(hipGetLastError());
     coo_plan_build(ctx, P.diag_plan, 2 * n, 1, at, rows.p, cols.p, srcs.p);
   }
+  // preconditioner matrix blockdiag(hdif_gram(k-1), hdif_gram(k)) with
+  // hdif_gram(j) = M_j + dif_both(j+1) (whitney_complex.rs:180-183)
+  {
+    int64_t total_pc = sizes[0] + sizes[6] + sizes[5] + sizes[2];
+    DBuf<int32_t> rows(ctx, std::max<int64_t>(total_pc, 1)), cols(ctx, std::max<int64_t>(total_pc, 1));
+    DBuf<uint32_t> srcs(ctx, std::max<int64_t>(total_pc, 1));
+    int64_t at = 0;
+    if (k >= 1) {
+      form_contributions(m, bc, k - 1, k - 1, nfm, nfm, 0, 0, false, (uint32_t)P.off[0], false,
+                         rows.p + at, cols.p + at, srcs.p + at);
+      at += sizes[0];
+      form_contributions(m, bc, k - 1, k - 1, nfm, nfm, 0, 0, false, (uint32_t)P.off[6], false,
+                         rows.p + at, cols.p + at, srcs.p + at);
+      at += sizes[6];
+    }
+    form_contributions(m, bc, k, k, nf, nf, P.ns, P.ns, false, (uint32_t)P.off[5], false,
+                       rows.p + at, cols.p + at, srcs.p + at);
+    at += sizes[5];
+    if (has_up) {
+      form_contributions(m, bc, k, k, nf, nf, P.ns, P.ns, false, (uint32_t)P.off[2], false,
+                         rows.p + at, cols.p + at, srcs.p + at);
+      at += sizes[2];
+    }
+    coo_plan_build(ctx, P.pc_plan, n, n, at, rows.p, cols.p, srcs.p);
+    Csr& Pm = P.sys.P;
+    Pm.ctx = ctx;
+    Pm.nrows = n;
+    Pm.ncols = n;
+    Pm.nnz = P.pc_plan.nnz;
+    Pm.row_ptr.alloc(ctx, n + 1);
+    Pm.col.alloc(ctx, std::max<int64_t>(Pm.nnz, 1));
+    Pm.val.alloc(ctx, std::max<int64_t>(Pm.nnz, 1));
+    FEEC_HIP(hipMemcpyAsync(Pm.row_ptr.p, P.pc_plan.row_ptr.p, (n + 1) * 8, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    if (Pm.nnz)
+      FEEC_HIP(hipMemcpyAsync(Pm.col.p, P.pc_plan.col.p, Pm.nnz * 4, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    P.sys.split = P.ns;
+  }
   P.diag.alloc(ctx, 2 * n);
   P.sys.ctx = ctx;
   P.sys.n = n;
@@ -301,7 +343,7 @@ void problem_build(Problem& P) {
 }
 
 void problem_element_outputs(Problem& P, double** outs) {
-  for (int i = 0; i < 5; ++i) outs[i] = P.el.p + P.off[i];
+  for (int i = 0; i < 7; ++i) outs[i] = P.el.p + P.off[i];
 }
 
 // CSR values, preconditioner diagonal and FP32 copy from the element blocks
@@ -319,12 +361,14 @@ void problem_assemble_from_elements(Problem& P) {
     solver_state_free(P.ws);
     P.ws = nullptr;
   }
+  if (P.sys.P.nnz) coo_plan_values(ctx, P.pc_plan, P.el.p, P.sys.P.val.p, nullptr);
   kkt_refresh_fp32(P.sys);
+  kkt_set_chebyshev(P.sys, P.opts.cheb_degree);
   ctx->sync();
 }
 
 void problem_assemble(Problem& P) {
-  double* outs[5];
+  double* outs[7];
   problem_element_outputs(P, outs);
   kkt_element_blocks(*P.g, P.k, outs, true);
   problem_assemble_from_elements(P);
@@ -420,6 +464,8 @@ void feec_solve_opts_default(feec_solve_opts* o) {
   o->inner_rtol = 1e-5;
   o->use_graph = 1;
   o->graph_iters = 32;
+  o->cheb_degree = 8;
+  o->reserved = 0;
 }
 
 feec_status feec_ctx_create(int device, feec_ctx** out) {
@@ -1036,8 +1082,8 @@ feec_status feec_problem_kernel_times(feec_problem* p, int reps, double out[4]) 
     FEEC_HIP(hipEventCreate(&e0));
     FEEC_HIP(hipEventCreate(&e1));
     FEEC_HIP(hipEventCreate(&e2));
-    double* outs[5] = {p->el.p + p->off[0], p->el.p + p->off[1], p->el.p + p->off[2],
-                       p->el.p + p->off[3], p->el.p + p->off[4]};
+    double* outs[7];
+    problem_element_outputs(*p, outs);
     kkt_element_blocks(*p->g, p->k, outs, true);
     DBuf<double> v(ctx, std::max<int64_t>(p->kkt_plan.nnz, 1));
     coo_plan_values(ctx, p->kkt_plan, p->el.p, v.p, nullptr);

@@ -667,6 +667,9 @@ Job kkt_job(DevTables& T, int d, int k, double* const* outs) {
   }
   job_add_block(job, T, k, KIND_DIAG_MASS, outs[4]);
   if (k + 1 <= d) job_add_block(job, T, k + 1, 3, outs[2]);
+  // the two hdif_gram blocks of the preconditioner need M_k and dif_both(k) in full
+  job_add_block(job, T, k, 0, outs[5]);
+  if (k >= 1) job_add_block(job, T, k, 3, outs[6]);
   for (int i = 0; i < job.ngrades; ++i) {
     job.pair_off[i] = T.pair_off[job.grades[i]];
     job.nH[i] = T.nH[job.grades[i]];
@@ -675,7 +678,7 @@ Job kkt_job(DevTables& T, int d, int k, double* const* outs) {
 }
 }  // namespace
 
-// outs: mass_sigma (M_{k-1}), dif_test(k), dif_both(k+1), diag(dif_both(k)), diag(M_k)
+// outs: M_{k-1}, dif_test(k), dif_both(k+1), diag(dif_both(k)), diag(M_k), M_k, dif_both(k)
 void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check) {
   Mesh& m = *g.mesh;
   DevTables& T = tables_for(m.ctx, m.d);

@@ -33,9 +33,11 @@ void set_last_error(const std::string& msg);
 #define FEEC_HIP(call)                                                                      \
   do {                                                                                      \
     hipError_t e_ = (call);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
+    if (e_ != hipSuccess) {                                                                 \
+      (void)hipGetLastError();                                                              \
       ::feec::fail(e_ == hipErrorOutOfMemory ? FEEC_E_OOM : FEEC_E_DEVICE,                  \
                    std::string(#call) + ": " + hipGetErrorString(e_));                      \
+    }                                                                                       \
   } while (0)
 
 #define FEEC_REQUIRE(cond, status, msg) \
@@ -43,8 +45,13 @@ void set_last_error(const std::string& msg);
     if (!(cond)) ::feec::fail(status, msg); \
   } while (0)
 
+// Every C-ABI entry runs in guard(). HIP keeps a per-thread "last error" that
+// library code (rocPRIM) reads after its launches; a failure already reported,
+// or one of a cleanup call whose status is ignored, is cleared on entry so it
+// is never attributed to an unrelated later call.
 template <class F>
 feec_status guard(F&& f) {
+  (void)hipGetLastError();
   try {
     f();
     return FEEC_OK;
@@ -240,7 +247,7 @@ void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check);
 struct BatchElementRequest {
   Geom* g;
   int grade;
-  double* outs[5];
+  double* outs[7];
 };
 // All requests in one persistent launch; returns the kernel's seconds.
 double kkt_element_blocks_batched(Ctx* ctx, const std::vector<BatchElementRequest>& reqs);
@@ -267,7 +274,19 @@ struct KktSystem {
   DBuf<double> inv_diag;
   DBuf<float> inv_diag32;
   DBuf<double> border;  // [nh][n] dense border columns (M_k H) placed at u rows
+  // Block preconditioner matrix blockdiag(hdif_gram(k-1), hdif_gram(k)), n x n
+  // (nnz == 0: Jacobi with inv_diag), its FP32 values and the Chebyshev
+  // coefficients of its two blocks (rows < split: block 0).
+  Csr P;
+  DBuf<float> P32;
+  int64_t split = 0;
+  int cheb_m = 0;
+  DBuf<double> cheb;
+  double lambda_max[2] = {0.0, 0.0};
 };
+// Estimate the spectra of D^-1 P per block and set degree-m Chebyshev
+// coefficients (m = 0 disables the polynomial).
+void kkt_set_chebyshev(KktSystem& s, int m);
 void kkt_refresh_fp32(KktSystem& s);
 // Solve s x = b (b, x device, FP64) to the options' tolerance.
 feec_report kkt_solve(KktSystem& s, const double* b_dev, double* x_dev, const feec_solve_opts& o,

@@ -21,7 +21,8 @@
 // (r1/r2, y/z, w/w2), so an even number of iterations is captured once into a
 // hipGraph and replayed; a `done` flag in the scalar block turns the tail of a
 // segment into no-ops once the tolerance is met, and the host reads the flag
-// only between segments.
+// only between segments. The graph is built node by node (no stream capture),
+// so solves on other threads' streams are never constrained by a capture.
 //
 // Mixed precision: the matrix values and every Krylov vector are FP32, SpMV
 // accumulates in FP32; the orthogonalisation scalars (alpha, beta, the
@@ -33,6 +34,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "feec_internal.hpp"
@@ -44,6 +47,41 @@ namespace {
 constexpr int TPB = 256;
 constexpr int NBLK = 1024;  // fixed reduction grid: 4 blocks per CU on 256 CUs
 constexpr int LANES = 8;    // lanes per CSR row in the SpMV
+
+// Enqueues kernels on a stream, or appends them as a linear chain of kernel
+// nodes to a graph under construction.
+struct Enq {
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+
+  template <class... P, class... A>
+  void operator()(void (*kernel)(P...), dim3 grid, dim3 block, A... a) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    std::tuple<std::decay_t<P>...> args(static_cast<std::decay_t<P>>(a)...);
+    void* ptrs[sizeof...(P) > 0 ? sizeof...(P) : 1];
+    std::apply(
+        [&](auto&... x) {
+          int i = 0;
+          ((ptrs[i++] = static_cast<void*>(&x)), ...);
+        },
+        args);
+    if (!graph) {
+      FEEC_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, 0, st));
+      return;
+    }
+    hipKernelNodeParams p{};
+    p.func = reinterpret_cast<void*>(kernel);
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = ptrs;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    FEEC_HIP(hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &p));
+    last = node;
+  }
+};
 
 struct MinresS {
   double beta, oldb, beta1, dbar, epsln, phibar, cs, sn;
@@ -366,6 +404,112 @@ __global__ void __launch_bounds__(TPB) k_mr_wx(const T* __restrict__ yold, T* __
   }
 }
 
+// --- Chebyshev block preconditioner ---------------------------------------
+// z = p_m(D^-1 A) D^-1 r per diagonal block of A = blockdiag(hdif_gram(k-1),
+// hdif_gram(k)): m steps of the Chebyshev semi-iteration for D^-1 A z = D^-1 r
+// from z = 0 on [a_b, b_b] (Saad, Iterative Methods for Sparse Linear Systems,
+// Alg. 12.1). The polynomial is fixed and positive on (0, b_b], so the
+// preconditioner is SPD, as MINRES needs. coef: [init scale per block] then
+// per step k: (rho_{k+1} rho_k, 2 rho_{k+1}/delta) per block. Rows >= n (the
+// harmonic border) get the identity.
+
+// yn -= (alpha/beta) r2 (the Lanczos vector update alone)
+template <class T>
+__global__ void __launch_bounds__(TPB) k_mr_update_only(T* __restrict__ yn, const T* __restrict__ r2,
+                                                        int64_t N, const MinresS* __restrict__ S) {
+  if (S->done) return;
+  const T c = (T)(-S->alfa / S->beta);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x)
+    yn[i] = yn[i] + c * r2[i];
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_cheb_init(const T* __restrict__ r, const T* __restrict__ dinv,
+                                                   T* __restrict__ res, T* __restrict__ d,
+                                                   T* __restrict__ z, int64_t n, int64_t N,
+                                                   int64_t split, const double* __restrict__ coef,
+                                                   const MinresS* __restrict__ S) {
+  if (S && S->done) return;
+  const T s0 = (T)coef[0], s1 = (T)coef[1];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T ri = r[i];
+    if (i < n) {
+      T q = ri * dinv[i];
+      res[i] = q;
+      d[i] = q * (i < split ? s0 : s1);
+      z[i] = 0;
+    } else {
+      z[i] = ri;
+    }
+  }
+}
+
+// one step: t = (A d)_i; res_i -= dinv_i t; z_i += d_i; dn_i = c1 d_i + c2 res_i
+template <class T>
+__global__ void __launch_bounds__(TPB) k_cheb_step(const int64_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col,
+                                                   const T* __restrict__ val, int64_t n, int64_t split,
+                                                   const T* __restrict__ dinv, const T* __restrict__ d,
+                                                   T* __restrict__ res, T* __restrict__ z,
+                                                   T* __restrict__ dn, const double* __restrict__ coef,
+                                                   int step, const MinresS* __restrict__ S) {
+  if (S && S->done) return;
+  const double* c = coef + 2 + 4 * step;
+  const T a0 = (T)c[0], b0 = (T)c[1], a1 = (T)c[2], b1 = (T)c[3];
+  const int lane = threadIdx.x % LANES;
+  const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
+  for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
+       row += groups) {
+    T acc = 0;
+    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LANES) acc += val[p] * d[col[p]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    if (lane == 0) {
+      T di = d[row];
+      T ri = res[row] - dinv[row] * acc;
+      res[row] = ri;
+      z[row] += di;
+      dn[row] = (row < split) ? (a0 * di + b0 * ri) : (a1 * di + b1 * ri);
+    }
+  }
+}
+
+// z_i += d_i (the last step's direction); partial dot r.z over all N rows
+template <class T>
+__global__ void __launch_bounds__(TPB) k_cheb_finish(const T* __restrict__ r, const T* __restrict__ d,
+                                                     T* __restrict__ z, int64_t n, int64_t N,
+                                                     const MinresS* __restrict__ S, double* part) {
+  if (S && S->done) return;
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T zi = z[i];
+    if (i < n) {
+      zi += d[i];
+      z[i] = zi;
+    }
+    v += to_d(r[i]) * to_d(zi);
+  }
+  double t = block_reduce(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <class T>
+__global__ void k_mr_zero(const T* __restrict__ b, int64_t n, T* r1, T* r2, T* x, T* w, T* w2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T bi = b[i];
+    r1[i] = bi;
+    r2[i] = bi;
+    x[i] = 0;
+    w[i] = 0;
+    w2[i] = 0;
+  }
+}
+
 __global__ void k_mr_check(MinresS* S) {
   if (S->done) return;
   if (S->residual <= S->rtol || S->iter >= S->max_iters) S->done = 1;
@@ -424,6 +568,10 @@ __global__ void k_jacobi(const double* dd, const double* ii, double* oo, int64_t
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < nn) oo[i] = dd ? ii[i] * dd[i] : ii[i];
 }
+__global__ void k_to_scale(const double* a, int64_t n, double s, double* out) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = s * a[i];
+}
 __global__ void k_cg_p(double* pp, const double* zz, double bb, int64_t nn) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < nn) pp[i] = bb * pp[i] + zz[i];
@@ -446,11 +594,19 @@ inline unsigned spmv_grid(int64_t n) {
 template <class T>
 struct MinresWork {
   DBuf<T> r1, r2, y, z, w, w2, x, b, dinv, border;
+  DBuf<T> res, d0, d1;  // Chebyshev work vectors
   DBuf<double> part, part2;
   DBuf<MinresS> S;
   hipGraphExec_t graph = nullptr;
-  int graph_iters = 0;
+  int graph_iters = 0, graph_cheb = -1;
   int64_t n = 0, nh = 0;
+  // block preconditioner (cheb_m == 0: Jacobi with dinv)
+  int cheb_m = 0;
+  int64_t split = 0;
+  const int64_t* prp = nullptr;
+  const int32_t* pcol = nullptr;
+  const T* pval = nullptr;
+  const double* coef = nullptr;
   ~MinresWork() {
     if (graph) (void)hipGraphExecDestroy(graph);
   }
@@ -482,6 +638,9 @@ static void minres_alloc(Ctx* ctx, MinresWork<T>& w, int64_t n, int64_t nh) {
   w.b.alloc(ctx, N);
   w.dinv.alloc(ctx, N);
   w.border.alloc(ctx, std::max<int64_t>(nh * n, 1));
+  w.res.alloc(ctx, N);
+  w.d0.alloc(ctx, N);
+  w.d1.alloc(ctx, N);
   w.part.alloc(ctx, NBLK * 8);
   w.part2.alloc(ctx, NBLK);
   w.S.alloc(ctx, 1);
@@ -493,10 +652,45 @@ static void minres_alloc(Ctx* ctx, MinresWork<T>& w, int64_t n, int64_t nh) {
   }
 }
 
+// Point the workspace at the system's preconditioner (matrix values of
+// precision T); a captured graph holding other pointers is dropped.
+template <class T>
+static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval) {
+  int m = (s.P.nnz > 0 && s.cheb_m > 0 && s.cheb.p) ? s.cheb_m : 0;
+  if (w.graph && (w.pval != pval || w.coef != s.cheb.p || w.prp != s.P.row_ptr.p ||
+                  w.pcol != s.P.col.p || w.cheb_m != m)) {
+    (void)hipGraphExecDestroy(w.graph);
+    w.graph = nullptr;
+  }
+  w.cheb_m = m;
+  w.split = s.split;
+  w.prp = s.P.row_ptr.p;
+  w.pcol = s.P.col.p;
+  w.pval = pval;
+  w.coef = s.cheb.p;
+}
+
+// z = P r with the Chebyshev block preconditioner, plus partial sums of r.z
+// into `part` (red_grid(N) blocks).
+template <class T>
+static void cheb_apply(Enq& q, MinresWork<T>& w, const T* r, T* z, const MinresS* S,
+                       double* part) {
+  const int64_t n = w.n, N = w.n + w.nh;
+  q(k_cheb_init<T>, red_grid(N), TPB, r, w.dinv.p, w.res.p, w.d0.p, z, n, N, w.split, w.coef, S);
+  T* d = w.d0.p;
+  T* dn = w.d1.p;
+  for (int k = 0; k + 1 < w.cheb_m; ++k) {
+    q(k_cheb_step<T>, spmv_grid(n), TPB, w.prp, w.pcol, w.pval, n, w.split, w.dinv.p, d, w.res.p,
+      z, dn, w.coef, k, S);
+    std::swap(d, dn);
+  }
+  q(k_cheb_finish<T>, red_grid(N), TPB, r, d, z, n, N, S, part);
+}
+
 // Enqueue one MINRES iteration; `par` selects the buffer roles.
 template <class T>
-static void minres_iteration(hipStream_t st, MinresWork<T>& w, const int64_t* rp,
-                             const int32_t* col, const T* val, int par) {
+static void minres_iteration(Enq& q, MinresWork<T>& w, const int64_t* rp, const int32_t* col,
+                             const T* val, int par) {
   const int64_t n = w.n, N = w.n + w.nh;
   T* r1 = par ? w.r2.p : w.r1.p;
   T* r2 = par ? w.r1.p : w.r2.p;
@@ -504,17 +698,22 @@ static void minres_iteration(hipStream_t st, MinresWork<T>& w, const int64_t* rp
   T* z = par ? w.y.p : w.z.p;
   T* wc = par ? w.w2.p : w.w.p;
   T* w2 = par ? w.w.p : w.w2.p;
-  k_mr_spmv<T><<<spmv_grid(n), TPB, 0, st>>>(rp, col, val, n, y, r1, w.S.p, w.part.p);
+  q(k_mr_spmv<T>, spmv_grid(n), TPB, rp, col, val, n, y, r1, w.S.p, w.part.p);
   if (w.nh) {
-    k_mr_border_dot<T><<<NBLK, TPB, 0, st>>>(w.border.p, (int)w.nh, n, y, w.S.p, w.part.p + NBLK);
-    k_mr_border_rows<T><<<1, TPB, 0, st>>>((int)w.nh, n, y, r1, w.S.p, w.part.p + NBLK);
-    k_mr_border_axpy<T><<<red_grid(n), TPB, 0, st>>>(w.border.p, (int)w.nh, n, y, r1, w.S.p);
+    q(k_mr_border_dot<T>, NBLK, TPB, w.border.p, (int)w.nh, n, y, w.S.p, w.part.p + NBLK);
+    q(k_mr_border_rows<T>, 1, TPB, (int)w.nh, n, y, r1, w.S.p, w.part.p + NBLK);
+    q(k_mr_border_axpy<T>, red_grid(n), TPB, w.border.p, (int)w.nh, n, y, r1, w.S.p);
   }
-  k_mr_alpha<<<1, TPB, 0, st>>>(w.S.p, w.part.p, spmv_grid(n), w.nh ? 1 : 0);
-  k_mr_update<T><<<red_grid(N), TPB, 0, st>>>(r1, r2, w.dinv.p, z, N, w.S.p, w.part2.p);
-  k_mr_rotate<<<1, TPB, 0, st>>>(w.S.p, w.part2.p, red_grid(N));
-  k_mr_wx<T><<<grid_for(N), TPB, 0, st>>>(y, wc, w2, w.x.p, N, w.S.p);
-  k_mr_check<<<1, 1, 0, st>>>(w.S.p);
+  q(k_mr_alpha, 1, TPB, w.S.p, w.part.p, spmv_grid(n), w.nh ? 1 : 0);
+  if (w.cheb_m > 0) {
+    q(k_mr_update_only<T>, red_grid(N), TPB, r1, r2, N, w.S.p);
+    cheb_apply<T>(q, w, r1, z, w.S.p, w.part2.p);
+  } else {
+    q(k_mr_update<T>, red_grid(N), TPB, r1, r2, w.dinv.p, z, N, w.S.p, w.part2.p);
+  }
+  q(k_mr_rotate, 1, TPB, w.S.p, w.part2.p, red_grid(N));
+  q(k_mr_wx<T>, grid_for(N), TPB, y, wc, w2, w.x.p, N, w.S.p);
+  q(k_mr_check, 1, 1, w.S.p);
 }
 
 // Run MINRES on w.b (already filled) into w.x. Returns stats.
@@ -524,34 +723,45 @@ static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const
                               const feec_solve_opts& o) {
   hipStream_t st = ctx->stream;
   const int64_t N = w.n + w.nh;
-  k_mr_init<T><<<red_grid(N), TPB, 0, st>>>(w.b.p, w.dinv.p, N, w.r1.p, w.r2.p, w.y.p, w.x.p,
-                                            w.w.p, w.w2.p, w.part.p);
-  k_mr_start<<<1, TPB, 0, st>>>(w.S.p, w.part.p, red_grid(N), rtol, (long long)max_iters);
-  FEEC_HIP(hipGetLastError());
+  Enq q{st};
+  if (w.cheb_m > 0) {
+    q(k_mr_zero<T>, red_grid(N), TPB, w.b.p, N, w.r1.p, w.r2.p, w.x.p, w.w.p, w.w2.p);
+    cheb_apply<T>(q, w, w.b.p, w.y.p, nullptr, w.part.p);
+  } else {
+    q(k_mr_init<T>, red_grid(N), TPB, w.b.p, w.dinv.p, N, w.r1.p, w.r2.p, w.y.p, w.x.p, w.w.p,
+      w.w2.p, w.part.p);
+  }
+  q(k_mr_start, 1, TPB, w.S.p, w.part.p, red_grid(N), rtol, (long long)max_iters);
   int seg = std::max(2, o.graph_iters - (o.graph_iters & 1));
-  if (o.use_graph && (!w.graph || w.graph_iters != seg)) {
+  if (o.use_graph && (!w.graph || w.graph_iters != seg || w.graph_cheb != w.cheb_m)) {
     if (w.graph) {
       (void)hipGraphExecDestroy(w.graph);
       w.graph = nullptr;
     }
-    hipGraph_t g;
-    FEEC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < seg; ++i) minres_iteration<T>(st, w, rp, col, val, i & 1);
-    FEEC_HIP(hipStreamEndCapture(st, &g));
-    FEEC_HIP(hipGraphInstantiate(&w.graph, g, nullptr, nullptr, 0));
+    // Build the segment graph node by node; it is destroyed on any failure.
+    hipGraph_t g = nullptr;
+    FEEC_HIP(hipGraphCreate(&g, 0));
+    Enq gq{st, g};
+    try {
+      for (int i = 0; i < seg; ++i) minres_iteration<T>(gq, w, rp, col, val, i & 1);
+    } catch (...) {
+      (void)hipGraphDestroy(g);
+      throw;
+    }
+    hipError_t ie = hipGraphInstantiate(&w.graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
+    FEEC_HIP(ie);
     w.graph_iters = seg;
+    w.graph_cheb = w.cheb_m;
   }
   KrylovStats ks;
-  int64_t launched = 0;
-  // Keep two segments in flight: enqueue, then read the scalars of the one before.
+  // One segment at a time; the host reads the scalar block between segments.
   while (true) {
     if (o.use_graph) {
       FEEC_HIP(hipGraphLaunch(w.graph, st));
     } else {
-      for (int i = 0; i < seg; ++i) minres_iteration<T>(st, w, rp, col, val, i & 1);
+      for (int i = 0; i < seg; ++i) minres_iteration<T>(q, w, rp, col, val, i & 1);
     }
-    launched += seg;
     FEEC_HIP(hipMemcpyAsync(ss.host_s, w.S.p, sizeof(MinresS), hipMemcpyDeviceToHost, st));
     FEEC_HIP(hipStreamSynchronize(st));
     if (ss.host_s->done) break;
@@ -569,7 +779,78 @@ void kkt_refresh_fp32(KktSystem& s) {
   int64_t N = s.n + s.nh;
   if (!s.inv_diag32.p || (int64_t)s.inv_diag32.n != N) s.inv_diag32.alloc(s.ctx, N);
   k_f64_to_f32<<<grid_for(N), TPB, 0, s.ctx->stream>>>(s.inv_diag.p, N, s.inv_diag32.p);
+  if (s.P.nnz) {
+    if (!s.P32.p || (int64_t)s.P32.n != s.P.nnz) s.P32.alloc(s.ctx, s.P.nnz);
+    k_f64_to_f32<<<grid_for(s.P.nnz), TPB, 0, s.ctx->stream>>>(s.P.val.p, s.P.nnz, s.P32.p);
+  }
   FEEC_HIP(hipGetLastError());
+}
+
+// Power iteration on D^-1 P, one estimate per diagonal block; then the
+// Chebyshev interval [b lo, b] with b = 1.05 lambda_max and lo chosen by degree.
+void kkt_set_chebyshev(KktSystem& s, int m) {
+  Ctx* ctx = s.ctx;
+  hipStream_t st = ctx->stream;
+  s.cheb_m = (s.P.nnz > 0) ? std::max(m, 0) : 0;
+  if (s.cheb_m == 0) return;
+  const int64_t n = s.n;
+  DBuf<double> x(ctx, n), y(ctx, n), part(ctx, NBLK + 1);
+  std::vector<double> hx(n);
+  uint64_t st_rng = 0x2545F4914F6CDD1Dull;
+  for (int64_t i = 0; i < n; ++i) {
+    st_rng ^= st_rng << 13;
+    st_rng ^= st_rng >> 7;
+    st_rng ^= st_rng << 17;
+    hx[i] = 0.5 + (double)(st_rng >> 11) * (1.0 / 9007199254740992.0);
+  }
+  FEEC_HIP(hipMemcpy(x.p, hx.data(), n * 8, hipMemcpyHostToDevice));
+  auto block_norm = [&](const double* v, int64_t lo, int64_t hi) {
+    if (hi <= lo) return 0.0;
+    k_dot<double><<<red_grid(hi - lo), TPB, 0, st>>>(v + lo, v + lo, hi - lo, part.p);
+    k_finish_sum<<<1, TPB, 0, st>>>(part.p, red_grid(hi - lo), part.p + NBLK);
+    double h = 0.0;
+    FEEC_HIP(hipMemcpyAsync(&h, part.p + NBLK, sizeof(double), hipMemcpyDeviceToHost, st));
+    FEEC_HIP(hipStreamSynchronize(st));
+    return std::sqrt(h);
+  };
+  const int64_t lo[2] = {0, s.split}, hi[2] = {s.split, n};
+  double lam[2] = {1.0, 1.0};
+  for (int it = 0; it < 30; ++it) {
+    spmv64(ctx, s.P, x.p, y.p);
+    k_jacobi<<<grid_for(n), TPB, 0, st>>>(s.inv_diag.p, y.p, y.p, n);
+    // lambda per block from |y|/|x|, then x = y normalized per block
+    for (int b = 0; b < 2; ++b) {
+      if (hi[b] <= lo[b]) continue;
+      double ny = block_norm(y.p, lo[b], hi[b]), nx = block_norm(x.p, lo[b], hi[b]);
+      if (nx > 0) lam[b] = ny / nx;
+      if (ny > 0)
+        k_to_scale<<<grid_for(hi[b] - lo[b]), TPB, 0, st>>>(y.p + lo[b], hi[b] - lo[b], 1.0 / ny,
+                                                            x.p + lo[b]);
+    }
+  }
+  // interval lower ends (fraction of b) tuned on 3D k=1 Kuhn meshes
+  const double frac = m <= 4 ? 1e-2 : m <= 8 ? 3e-3 : m <= 12 ? 1.5e-3 : 1e-3;
+  std::vector<double> coef(2 + 4 * std::max(m - 1, 0), 0.0);
+  double rho[2], sigma[2], delta[2];
+  for (int b = 0; b < 2; ++b) {
+    s.lambda_max[b] = lam[b];
+    double bb = 1.05 * lam[b], aa = bb * frac;
+    double theta = 0.5 * (bb + aa);
+    delta[b] = 0.5 * (bb - aa);
+    sigma[b] = theta / delta[b];
+    rho[b] = 1.0 / sigma[b];
+    coef[b] = 1.0 / theta;
+  }
+  for (int k = 0; k + 1 < m; ++k)
+    for (int b = 0; b < 2; ++b) {
+      double rn = 1.0 / (2.0 * sigma[b] - rho[b]);
+      coef[2 + 4 * k + 2 * b] = rn * rho[b];
+      coef[2 + 4 * k + 2 * b + 1] = 2.0 * rn / delta[b];
+      rho[b] = rn;
+    }
+  if ((int64_t)s.cheb.n != (int64_t)coef.size()) s.cheb.alloc(ctx, coef.size());
+  FEEC_HIP(hipMemcpyAsync(s.cheb.p, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, st));
+  ctx->sync();
 }
 
 // |b - A x| / |b| with the FP64 KKT operator (CSR + border); r into ss.r
@@ -637,6 +918,7 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
     // correction steps until the true residual meets rtol.
     MinresWork<double>& w = ss.w64;
     minres_alloc(ctx, w, n, nh);
+    bind_preconditioner(s, w, s.P.val.p);
     FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
     if (nh) FEEC_HIP(hipMemcpyAsync(w.border.p, s.border.p, nh * n * sizeof(double), hipMemcpyDeviceToDevice, st));
     FEEC_HIP(hipMemcpyAsync(w.b.p, b, N * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -662,6 +944,7 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
     MinresWork<float>& w = ss.w32;
     minres_alloc(ctx, w, n, nh);
     if (!s.val32.p) kkt_refresh_fp32(s);
+    bind_preconditioner(s, w, s.P32.p);
     FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag32.p, N * sizeof(float), hipMemcpyDeviceToDevice, st));
     if (nh) k_f64_to_f32<<<grid_for(nh * n), TPB, 0, st>>>(s.border.p, nh * n, w.border.p);
     // r = b

@@ -250,6 +250,11 @@ __global__ void k_coboundary(const int32_t* cells, int nv, LocalSubsets co, cons
 
 __global__ void k_lengths(const int32_t* cells, int nv, LocalSubsets edges, const int64_t* rep,
                           int64_t ne, const double* coords, int d, int minkowski, double* out) {
+  // Each square rounded on its own, then summed in axis order, with no fused
+  // multiply-add: the squared lengths are bit-identical to the reference's
+  // (Rust never contracts a*b+c), so exact-zero element entries of
+  // exactly-representable cells come out the same.
+#pragma clang fp contract(off)
   int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= ne) return;
   int64_t cell, vi, vj;
@@ -262,11 +267,6 @@ __global__ void k_lengths(const int32_t* cells, int nv, LocalSubsets edges, cons
     vi = cells[e * nv];
     vj = cells[e * nv + 1];
   }
-  // Each square rounded on its own, then summed in axis order, with no fused
-  // multiply-add: the squared lengths are bit-identical to the reference's
-  // (Rust never contracts a*b+c), so exact-zero element entries of
-  // exactly-representable cells come out the same.
-#pragma clang fp contract(off)
   double acc = 0.0;
   for (int a = 0; a < d; ++a) {
     double t = coords[vj * d + a] - coords[vi * d + a];

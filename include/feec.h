@@ -88,6 +88,9 @@ typedef struct feec_solve_opts {
   double inner_rtol;     /* mixed: FP32 inner MINRES tolerance (default 1e-5) */
   int32_t use_graph;     /* capture Krylov iterations in a hipGraph (default 1) */
   int32_t graph_iters;   /* iterations per captured segment, even (default 32) */
+  int32_t cheb_degree;   /* degree of the Chebyshev-Jacobi polynomial applied to each
+                            hdif_gram block of the preconditioner; 0 = Jacobi only (default 8) */
+  int32_t reserved;
 } feec_solve_opts;
 
 typedef struct feec_report {

@@ -513,6 +513,97 @@ def _minres_direct_blocks(c, k, kkt, rhs, nh, rtol, max_iters):
     return minres_py(lambda v: A @ v, prec, rhs, rtol, max_iters)
 
 
+CHEB_POWER_ITERS = 30
+CHEB_LMAX_MARGIN = 1.05
+
+
+def cheb_interval_fraction(m):
+    """Lower end of the Chebyshev interval as a fraction of its upper end."""
+    return 1e-2 if m <= 4 else 3e-3 if m <= 8 else 1.5e-3 if m <= 12 else 1e-3
+
+
+def _xorshift_start(n):
+    """Power-iteration start vector: 0.5 + U[0,1) from xorshift64 (13, 7, 17)."""
+    out = np.empty(n)
+    s = 0x2545F4914F6CDD1D
+    mask = (1 << 64) - 1
+    for i in range(n):
+        s ^= (s << 13) & mask
+        s ^= s >> 7
+        s ^= (s << 17) & mask
+        out[i] = 0.5 + (s >> 11) * (1.0 / 9007199254740992.0)
+    return out
+
+
+def chebyshev_coefficients(lmax, m):
+    """(1/theta, [(rho_{k+1} rho_k, 2 rho_{k+1}/delta)] for k < m-1) of the
+    Chebyshev iteration on [f b, b], b = 1.05 lmax (Saad, Iterative Methods
+    for Sparse Linear Systems, 2nd ed., Algorithm 12.1)."""
+    b = CHEB_LMAX_MARGIN * lmax
+    a = b * cheb_interval_fraction(m)
+    theta, delta = 0.5 * (b + a), 0.5 * (b - a)
+    sigma = theta / delta
+    rho = 1.0 / sigma
+    steps = []
+    for _ in range(m - 1):
+        rn = 1.0 / (2.0 * sigma - rho)
+        steps.append((rn * rho, 2.0 * rn / delta))
+        rho = rn
+    return 1.0 / theta, steps
+
+
+def chebyshev_apply(P, dinv, r, c0, steps):
+    """z = p(D^-1 P) D^-1 r by the Chebyshev recurrence: m directions summed,
+    so 1 - lambda p(lambda) = T_m((theta - lambda)/delta) / T_m(theta/delta)."""
+    res = dinv * r
+    d = c0 * res
+    z = np.zeros_like(r)
+    for a, b in steps:
+        res = res - dinv * (P @ d)
+        z = z + d
+        d = a * d + b * res
+    return z + d
+
+
+def chebyshev_jacobi(c: WhitneyComplex, k, nh, m):
+    """Build-defined stand-in for the reference's direct block preconditioner
+    diag(hdif_gram(k-1)^-1, hdif_gram(k)^-1, I) (elliptic.rs:29-47): per
+    block, a degree-(m-1) Chebyshev polynomial in D^-1 hdif_gram with D its
+    diagonal and lambda_max from 30 power iterations. Returns prec(r)."""
+    import scipy.sparse as sp
+    blocks = ([c.hdif_gram(k - 1).to_scipy()] if k > 0 else []) + [c.hdif_gram(k).to_scipy()]
+    P = sp.block_diag(blocks, format="csr")
+    n = P.shape[0]
+    dinv = 1.0 / P.diagonal()
+    split = blocks[0].shape[0] if k > 0 else 0
+    bounds = [(0, split), (split, n)]
+    x = _xorshift_start(n)
+    lam = [1.0, 1.0]
+    for _ in range(CHEB_POWER_ITERS):
+        y = dinv * (P @ x)
+        for b, (lo, hi) in enumerate(bounds):
+            if hi <= lo:
+                continue
+            ny, nx = np.linalg.norm(y[lo:hi]), np.linalg.norm(x[lo:hi])
+            if nx > 0:
+                lam[b] = ny / nx
+            if ny > 0:
+                x[lo:hi] = y[lo:hi] / ny
+    coefs = [chebyshev_coefficients(lam[b], m) for b in range(2)]
+    Pb = [blocks[0] if k > 0 else None, blocks[-1]]
+
+    def prec(r):
+        out = np.empty_like(r)
+        for b, (lo, hi) in enumerate(bounds):
+            if hi > lo:
+                out[lo:hi] = chebyshev_apply(Pb[b], dinv[lo:hi], r[lo:hi], *coefs[b])
+        out[n:] = r[n:]
+        return out
+
+    prec.lambda_max = lam
+    return prec
+
+
 def minres_py(op, prec, b, rtol=1e-10, max_iters=10_000):
     """krylov.rs:113-211 with callables (for non-diagonal preconditioners)."""
     r1 = b.copy()

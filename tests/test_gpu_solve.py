@@ -80,16 +80,23 @@ def test_c1_configuration_fp64_and_mixed():
     f = fa.normal_vector(ocx.nsimplices(1), 1)
     kkt, rhs, ns, nu = o.assemble_mixed_kkt(oc, f, 1, oc.harmonics(1))
     x_lu = spla.spsolve(kkt.to_scipy().tocsc(), rhs)
-    x_or, rep_or = o.minres(kkt, rhs, o.mixed_jacobi(oc, 1, 0), 1e-10, 100000)
-    for prec in (fa.FP64, fa.MIXED):
-        sig, u, p, rep = fa.solve_source(gc, f, 1, fa.solve_opts(precision=prec))
+    A = kkt.to_scipy()
+    # the oracle's FP64 MINRES with the two preconditioners the GPU offers
+    _, rep_jac = o.minres(kkt, rhs, o.mixed_jacobi(oc, 1, 0), 1e-10, 100000)
+    _, rep_cheb = o.minres_py(lambda v: A @ v, o.chebyshev_jacobi(oc, 1, 0, 8), rhs, 1e-10, 100000)
+    for prec, cheb in ((fa.FP64, 0), (fa.FP64, 8), (fa.MIXED, 8), (fa.MIXED, 0)):
+        sig, u, p, rep = fa.solve_source(gc, f, 1, fa.solve_opts(precision=prec, cheb_degree=cheb))
         x = np.concatenate([sig, u])
         assert rep.residual <= 1e-10
-        assert np.linalg.norm(kkt.to_scipy() @ x - rhs) / np.linalg.norm(rhs) <= 1e-10
+        assert np.linalg.norm(A @ x - rhs) / np.linalg.norm(rhs) <= 1e-10
         assert np.linalg.norm(x - x_lu) / np.linalg.norm(x_lu) <= 1e-7
-        if prec == fa.FP64:
-            # same algorithm and preconditioner as the oracle's FP64 MINRES
-            assert abs(rep.iters - rep_or.iters) <= max(10, rep_or.iters // 20), (rep.iters, rep_or.iters)
+    for cheb, ref in ((0, rep_jac), (8, rep_cheb)):
+        # one FP64 MINRES run with the oracle's algorithm and preconditioner
+        # stops after the same number of iterations
+        opts = fa.solve_opts(precision=fa.FP64, cheb_degree=cheb, max_refinements=0)
+        rep = fa.solve_source(gc, f, 1, opts, raise_on_failure=False)[3]
+        assert abs(rep.iters - ref.iters) <= max(10, ref.iters // 20), (cheb, rep.iters, ref.iters)
+        assert rep.precond_residual <= 1e-10
 
 
 def test_resident_problem_reassembly_and_device_source():

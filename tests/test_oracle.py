@@ -387,6 +387,48 @@ def test_cg_matches_cholesky():
     np.testing.assert_allclose(x, spla.spsolve(M.to_scipy().tocsc(), b), rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("m", [2, 5, 8, 16])
+def test_chebyshev_recurrence_is_the_chebyshev_residual_polynomial(m):
+    """1 - lambda p(lambda) = T_m((theta-lambda)/delta) / T_m(theta/delta) on a
+    diagonal operator, with T_m from numpy's Chebyshev series (not the
+    recurrence): a wrong coefficient or a dropped direction breaks it."""
+    lam = np.concatenate([np.linspace(1e-3, 1.0, 40), [0.37, 0.999]])
+    c0, steps = o.chebyshev_coefficients(1.0, m)
+    b = o.CHEB_LMAX_MARGIN
+    a = b * o.cheb_interval_fraction(m)
+    theta, delta = (b + a) / 2, (b - a) / 2
+    P = sp.diags(lam).tocsr()
+    z = o.chebyshev_apply(P, np.ones_like(lam), np.ones_like(lam), c0, steps)
+    Tm = np.polynomial.chebyshev.chebval
+    coef = [0] * m + [1]
+    want = Tm((theta - lam) / delta, coef) / Tm(theta / delta, coef)
+    np.testing.assert_allclose(1 - lam * z, want, rtol=1e-10, atol=1e-13)
+
+
+def test_chebyshev_jacobi_preconditioner():
+    """lambda_max of D^-1 hdif_gram per block against a dense eigensolve, and
+    a preconditioned MINRES that needs far fewer iterations than Jacobi."""
+    cells, coords = jittered(2, 12, 0.1, 0)
+    cx = o.Complex(2, cells)
+    L = cx.lengths_from_coords(coords)
+    c = o.WhitneyComplex(cx, L)
+    prec = o.chebyshev_jacobi(c, 1, 0, 8)
+    for b, g in enumerate((c.hdif_gram(0), c.hdif_gram(1))):
+        A = g.todense()
+        dh = 1 / np.sqrt(np.diag(A))
+        lmax = np.linalg.eigvalsh(dh[:, None] * A * dh[None, :]).max()
+        assert abs(prec.lambda_max[b] - lmax) <= 0.02 * lmax, (b, prec.lambda_max[b], lmax)
+    f = np.sin(np.arange(cx.nsimplices(1)) * 0.7)
+    kkt, rhs, ns, nu = o.assemble_mixed_kkt(c, f, 1, c.harmonics(1))
+    A = kkt.to_scipy()
+    x, rep = o.minres_py(lambda v: A @ v, prec, rhs, 1e-10, 20000)
+    xj, repj = o.minres(kkt, rhs, o.mixed_jacobi(c, 1, 0), 1e-10, 20000)
+    assert rep.converged and repj.converged
+    assert rep.iters * 4 < repj.iters, (rep.iters, repj.iters)
+    x_lu = spla.spsolve(A.tocsc(), rhs)
+    assert np.linalg.norm(x - x_lu) / np.linalg.norm(x_lu) < 1e-7
+
+
 def test_minres_finite_termination():
     """krylov.rs:225-319: symmetric indefinite from a known spectrum."""
     rng = np.random.default_rng(5)
