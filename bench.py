@@ -46,12 +46,13 @@ def build_problem(fa, n, precision, ctx):
     return topo, geo, wc
 
 
-def cpu_baseline(n_sample, gpu_iters, c2_cells, c2_nnz):
+def cpu_baseline(n_sample, gpu_iters, c2_cells, c2_nnz, cheb_degree):
     """The oracle (C restatement of the reference algorithm) timed on this
     host on a bounded sample: the element matrices and CSR assembly of the KKT
-    blocks of an n_sample^3 Kuhn mesh, and FP64 Jacobi-MINRES iterations on
-    its KKT system; scaled to the 64^3 workload per cell / per nonzero, with the
-    iteration count of the GPU FP64 path (same algorithm and preconditioner)."""
+    blocks of an n_sample^3 Kuhn mesh, and FP64 MINRES iterations with the
+    same Chebyshev block preconditioner as the GPU, on its KKT system; scaled
+    to the 64^3 workload per cell / per nonzero, with the iteration count of
+    the GPU FP64 path (same algorithm and preconditioner)."""
     from oracle import oracle as o
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cells, coords = o.kuhn(3, n_sample)
@@ -69,11 +70,11 @@ def cpu_baseline(n_sample, gpu_iters, c2_cells, c2_nnz):
     t0 = time.perf_counter()
     f = np.random.default_rng(2).standard_normal(cx.nsimplices(1))
     kkt, rhs, ns, nu = o.assemble_mixed_kkt(c, f, 1, c.harmonics(1))
-    dinv = o.mixed_jacobi(c, 1, 0)
     t_asm = time.perf_counter() - t0
-    iters = 400
+    prec = o.chebyshev_jacobi(c, 1, 0, cheb_degree)
+    iters = 60
     t0 = time.perf_counter()
-    o.minres(kkt, rhs, dinv, 1e-30, iters)
+    o.minres_cheb(kkt, rhs, prec, 1e-30, iters)
     t_it = (time.perf_counter() - t0) / iters
     scale_cells = c2_cells / nc
     t_step = (t_el + t_asm) * scale_cells + t_it * (c2_nnz / kkt.nnz) * gpu_iters
@@ -81,8 +82,9 @@ def cpu_baseline(n_sample, gpu_iters, c2_cells, c2_nnz):
         "value": 1.0 / t_step, "unit": "solves/s", "cores": cores, "kind": "port",
         "sample": (f"oracle on a {n_sample}^3 Kuhn mesh ({nc} tets, KKT nnz {kkt.nnz}): element matrices "
                    f"{t_el:.2f}s + assembly {t_asm:.2f}s scaled x{scale_cells:.1f} by cells, and {iters} "
-                   f"FP64 Jacobi-MINRES iterations ({t_it * 1e3:.2f} ms each) scaled by nnz to the 64^3 KKT "
-                   f"and multiplied by the GPU FP64 path's {gpu_iters} iterations"),
+                   f"FP64 MINRES iterations with the degree-{cheb_degree} Chebyshev block preconditioner "
+                   f"({t_it * 1e3:.2f} ms each) scaled by nnz to the 64^3 KKT and multiplied by the GPU "
+                   f"FP64 path's {gpu_iters} iterations"),
         "seconds_per_step_estimate": t_step,
         "elmat_cells_per_s": nc / t_el,
     }
@@ -215,7 +217,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         try:
-            line["cpu_baseline"] = cpu_baseline(args.cpu_n, int(r64.iters), ncells, nnz)
+            line["cpu_baseline"] = cpu_baseline(args.cpu_n, int(r64.iters), ncells, nnz,
+                                                int(fp64.opts.cheb_degree))
         except Exception as e:  # reported, never fatal for the GPU line
             line["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(line), flush=True)

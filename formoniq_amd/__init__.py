@@ -375,6 +375,7 @@ class Problem:
     def __init__(self, complex_: WhitneyComplex, grade, opts: SolveOpts | None = None):
         self.complex = complex_
         self.grade = grade
+        self.opts = opts if opts is not None else solve_opts()
         h = C.c_void_p()
         check(load().feec_problem_create(complex_.geometry._h, complex_.bc, grade,
                                          C.byref(opts) if opts is not None else None, C.byref(h)))

@@ -22,13 +22,14 @@
 // integer arithmetic.
 //
 // Kernel shape: one wavefront per group of 64 cells, one lane per cell. The
-// lane computes g, det g, g^-1 and G in registers, stages G and then the
-// minors H of each grade in LDS (layout [entry][lane], so a wavefront's LDS
-// access is one contiguous 512-byte row), and emits every requested block by
-// walking the coefficient tables (wave-uniform, read through the scalar
-// cache). Element matrices are written structure-of-arrays,
-// out[entry * ncells + cell], so each store instruction of a wavefront covers
-// 64 consecutive doubles.
+// lane computes g, det g, g^-1 and G in registers, then for each grade K the
+// unique K x K minors H of G (symmetric, so NU = n(n+1)/2 of them) in
+// registers, and emits every block of that grade as a dense contraction
+// out_e = scale * sum_u C[e][u] H[u]. C is wave-uniform: it is read with
+// scalar loads and enters the FP64 FMAs as an SGPR operand, so the inner loop
+// touches neither LDS nor vector memory. Element matrices are written
+// structure-of-arrays, out[entry * ncells + cell], so each store instruction
+// of a wavefront covers 64 consecutive doubles.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -285,17 +286,13 @@ static HostBlock make_block(int D, int K, int kind) {
   return hb;
 }
 
-// Device-resident tables of one (d): the minor pair lists of every grade and
-// the coefficient tables of every (grade, kind) block.
+// Device-resident coefficient tables of one dimension: per (grade, kind)
+// block, a dense [nout][NU] table of integer coefficients (stored as double)
+// over the NU unique minors of that grade.
 struct DevTables {
-  DBuf<uint8_t> pairs;   // per grade: nH * 2K vertex positions
-  DBuf<int32_t> start;
-  DBuf<uint16_t> h;
-  DBuf<int16_t> c;
-  int pair_off[5] = {0, 0, 0, 0, 0};
-  int nH[5] = {0, 0, 0, 0, 0};
-  int block_start_off[5][6];  // offset into `start` of each block table
-  int block_term_base[5][6];  // offset added to start[] values
+  DBuf<double> coef;
+  int nU[5] = {0, 0, 0, 0, 0};
+  int block_coef_off[5][6];
   int block_nout[5][6];
 };
 
@@ -318,46 +315,28 @@ static DevTables& tables_for(Ctx* ctx, int D) {
   auto it = g_tabs.find(key);
   if (it != g_tabs.end()) return *it->second;
   auto T = std::make_unique<DevTables>();
-  std::vector<uint8_t> pairs;
-  std::vector<int32_t> start;
-  std::vector<uint16_t> hs;
-  std::vector<int16_t> cs;
+  std::vector<double> coef;
   const int nv = D + 1;
   for (int K = 0; K <= D; ++K) {
-    auto blades = masks_of(nv, K);
-    int nb = (int)blades.size();
-    T->pair_off[K] = (int)pairs.size();
-    int cnt = 0;
-    for (int a = 0; a < nb; ++a)
-      for (int b = a; b < nb; ++b) {
-        for (int p = 0; p < K; ++p) pairs.push_back((uint8_t)elem_of(blades[a], p));
-        for (int p = 0; p < K; ++p) pairs.push_back((uint8_t)elem_of(blades[b], p));
-        ++cnt;
-      }
-    T->nH[K] = cnt;
+    const int nb = binom(nv, K);
+    const int nu = nb * (nb + 1) / 2;
+    T->nU[K] = nu;
     for (int kind = 0; kind < 6; ++kind) {
       T->block_nout[K][kind] = 0;
-      T->block_start_off[K][kind] = (int)start.size();
-      T->block_term_base[K][kind] = (int)hs.size();
-      if (K == 0 && kind != 0 && kind != KIND_DIAG_MASS) {
-        start.push_back(0);
-        continue;
-      }
+      T->block_coef_off[K][kind] = (int)coef.size();
+      if (K == 0 && kind != 0 && kind != KIND_DIAG_MASS) continue;
       HostBlock hb = make_block(D, K, kind);
-      for (int32_t s : hb.start) start.push_back(s + (int32_t)hs.size());
-      T->block_nout[K][kind] = (int)hb.start.size() - 1;
-      hs.insert(hs.end(), hb.h.begin(), hb.h.end());
-      cs.insert(cs.end(), hb.c.begin(), hb.c.end());
+      const int nout = (int)hb.start.size() - 1;
+      T->block_nout[K][kind] = nout;
+      size_t base = coef.size();
+      coef.resize(base + (size_t)nout * nu, 0.0);
+      for (int e = 0; e < nout; ++e)
+        for (int t = hb.start[e]; t < hb.start[e + 1]; ++t)
+          coef[base + (size_t)e * nu + hb.h[t]] += (double)hb.c[t];
     }
   }
-  T->pairs.alloc(ctx, std::max<size_t>(pairs.size(), 1));
-  T->start.alloc(ctx, start.size());
-  T->h.alloc(ctx, std::max<size_t>(hs.size(), 1));
-  T->c.alloc(ctx, std::max<size_t>(cs.size(), 1));
-  FEEC_HIP(hipMemcpy(T->pairs.p, pairs.data(), pairs.size(), hipMemcpyHostToDevice));
-  FEEC_HIP(hipMemcpy(T->start.p, start.data(), start.size() * 4, hipMemcpyHostToDevice));
-  FEEC_HIP(hipMemcpy(T->h.p, hs.data(), hs.size() * 2, hipMemcpyHostToDevice));
-  FEEC_HIP(hipMemcpy(T->c.p, cs.data(), cs.size() * 2, hipMemcpyHostToDevice));
+  T->coef.alloc(ctx, std::max<size_t>(coef.size(), 1));
+  FEEC_HIP(hipMemcpy(T->coef.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
   auto& slot = g_tabs[key];
   slot = std::move(T);
   return *slot;
@@ -367,58 +346,87 @@ static DevTables& tables_for(Ctx* ctx, int D) {
 struct JobBlock {
   int K;
   int nout;
-  int start_off;  // into the start table
-  double* out;    // SoA [nout][ncells]
+  int coef_off;  // into the dense coefficient table of the dimension
+  double* out;   // SoA [nout][ncells]
 };
 struct Job {
   int ngrades;
   int grades[5];
-  int pair_off[5];
-  int nH[5];
   int nblocks;
   JobBlock b[12];
 };
 
 constexpr int WAVE = 64;
 
-template <int K>
-__device__ __forceinline__ double minor_det(const double* sG, int nv, const uint8_t* pr, int lane) {
-  if constexpr (K == 0) {
-    return 1.0;
-  } else {
-    double m[K][K];
+// The K-subsets of {0..N-1} in colex order (= increasing bit masks).
+template <int N, int K>
+struct Blades {
+  static constexpr int count = binom(N, K);
+  int e[count > 0 ? count : 1][K > 0 ? K : 1];
+  constexpr Blades() : e() {
+    int i = 0;
+    for (unsigned m = 0; m < (1u << N); ++m) {
+      int c = 0;
+      for (int b = 0; b < N; ++b) c += (m >> b) & 1u;
+      if (c != K) continue;
+      int p = 0;
+      for (int b = 0; b < N; ++b)
+        if ((m >> b) & 1u) e[i][p++] = b;
+      ++i;
+    }
+  }
+};
+
+// Coefficients are read through the constant address space, so the compiler
+// issues wave-uniform scalar loads (s_load_dwordx16) and the FMAs take them as
+// SGPR operands.
+typedef const __attribute__((address_space(4))) double* ConstDouble;
+
+// Every block of grade K: the unique K x K minors H of G in registers, then
+// out_e = scale * sum_u C[e][u] H[u] with the dense integer table C.
+template <int D, int K>
+__device__ __forceinline__ void grade_blocks(const double (&G)[D + 1][D + 1], double vol,
+                                             const Job& job, const double* __restrict__ coef,
+                                             int64_t nc, int64_t cell, bool active) {
+  constexpr Blades<D + 1, K> B{};
+  constexpr int NB = Blades<D + 1, K>::count;
+  constexpr int NU = NB * (NB + 1) / 2;
+  double H[NU];
 #pragma unroll
-    for (int a = 0; a < K; ++a)
+  for (int a = 0; a < NB; ++a)
 #pragma unroll
-      for (int b = 0; b < K; ++b) m[a][b] = sG[(pr[a] * nv + pr[K + b]) * WAVE + lane];
-    return det<K>(m);
+    for (int b = a; b < NB; ++b) {
+      double m[K > 0 ? K : 1][K > 0 ? K : 1];
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+#pragma unroll
+        for (int j = 0; j < K; ++j) m[i][j] = G[B.e[a][i]][B.e[b][j]];
+      H[a * NB - a * (a - 1) / 2 + (b - a)] = det<K>(m);
+    }
+  constexpr double kf = factorial(K);
+  const double scale = vol * (kf * kf / double((D + 1) * (D + 2)));
+  for (int bi = 0; bi < job.nblocks; ++bi) {
+    if (job.b[bi].K != K) continue;
+    ConstDouble c = (ConstDouble)(coef + job.b[bi].coef_off);
+    double* __restrict__ out = job.b[bi].out;
+    const int nout = job.b[bi].nout;
+    for (int e = 0; e < nout; ++e, c += NU) {
+      double acc = 0.0;
+#pragma unroll
+      for (int u = 0; u < NU; ++u) acc += c[u] * H[u];
+      if (active) out[(int64_t)e * nc + cell] = scale * acc;
+    }
   }
 }
 
-// Coefficient tables of one dimension, as device pointers.
-struct TabPtrs {
-  const uint8_t* pairs;
-  const int32_t* tstart;
-  const uint16_t* th;
-  const int16_t* tc;
-};
-
-// One wavefront, 64 consecutive cells starting at cell0.
+// One wavefront, 64 consecutive cells starting at cell0, one lane per cell.
 template <int D>
 __device__ __forceinline__ void elmat_group(const int32_t* __restrict__ edges,
                                             const double* __restrict__ len, int64_t nc,
-                                            int64_t cell0, const Job& job, const TabPtrs& T,
-                                            int* degenerate, double* lds) {
-  const uint8_t* __restrict__ pairs = T.pairs;
-  const int32_t* __restrict__ tstart = T.tstart;
-  const uint16_t* __restrict__ th = T.th;
-  const int16_t* __restrict__ tc = T.tc;
-  constexpr int NV = D + 1;
+                                            int64_t cell0, const Job& job,
+                                            const double* __restrict__ coef, int* degenerate) {
   constexpr int NE = binom(D + 1, 2);
-  double* sG = lds;                   // [NV*NV][WAVE]
-  double* sH = lds + NV * NV * WAVE;  // [nH][WAVE]
-  const int lane = threadIdx.x;
-  const int64_t cell = cell0 + lane;
+  const int64_t cell = cell0 + threadIdx.x;
   const bool active = cell < nc;
   double s[NE];
 #pragma unroll
@@ -427,40 +435,16 @@ __device__ __forceinline__ void elmat_group(const int32_t* __restrict__ edges,
   cell_geometry<D>(s, cg);
   if (active && cg.det == 0.0) atomicOr(degenerate, 1);
   const double vol = (1.0 / factorial(D)) * sqrt(fabs(cg.det));
-  double G[NV][NV];
+  double G[D + 1][D + 1];
   bary_gram<D>(cg.ginv, G);
-#pragma unroll
-  for (int a = 0; a < NV; ++a)
-#pragma unroll
-    for (int b = 0; b < NV; ++b) sG[(a * NV + b) * WAVE + lane] = G[a][b];
   for (int gi = 0; gi < job.ngrades; ++gi) {
-    const int K = job.grades[gi];
-    const uint8_t* pr = pairs + job.pair_off[gi];
-    const int nH = job.nH[gi];
-    for (int t = 0; t < nH; ++t, pr += 2 * K) {
-      double v;
-      switch (K) {
-        case 0: v = 1.0; break;
-        case 1: v = minor_det<1>(sG, NV, pr, lane); break;
-        case 2: v = minor_det<2>(sG, NV, pr, lane); break;
-        case 3: v = minor_det<3>(sG, NV, pr, lane); break;
-        default: v = minor_det<4>(sG, NV, pr, lane); break;
-      }
-      sH[t * WAVE + lane] = v;
-    }
-    const double kf = factorial(K);
-    const double scale = vol * (kf * kf / double((D + 1) * (D + 2)));
-    for (int bi = 0; bi < job.nblocks; ++bi) {
-      if (job.b[bi].K != K) continue;
-      const int32_t* st = tstart + job.b[bi].start_off;
-      double* out = job.b[bi].out;
-      const int nout = job.b[bi].nout;
-      for (int e = 0; e < nout; ++e) {
-        double acc = 0.0;
-        const int t1 = st[e + 1];
-        for (int t = st[e]; t < t1; ++t) acc += (double)tc[t] * sH[th[t] * WAVE + lane];
-        if (active) out[(int64_t)e * nc + cell] = scale * acc;
-      }
+    switch (job.grades[gi]) {
+      case 0: grade_blocks<D, 0>(G, vol, job, coef, nc, cell, active); break;
+      case 1: grade_blocks<D, 1>(G, vol, job, coef, nc, cell, active); break;
+      case 2: if constexpr (D >= 2) grade_blocks<D, 2>(G, vol, job, coef, nc, cell, active); break;
+      case 3: if constexpr (D >= 3) grade_blocks<D, 3>(G, vol, job, coef, nc, cell, active); break;
+      case 4: if constexpr (D >= 4) grade_blocks<D, 4>(G, vol, job, coef, nc, cell, active); break;
+      default: break;
     }
   }
 }
@@ -468,9 +452,8 @@ __device__ __forceinline__ void elmat_group(const int32_t* __restrict__ edges,
 template <int D>
 __global__ void __launch_bounds__(WAVE) k_elmat(const int32_t* __restrict__ edges,
                                                 const double* __restrict__ len, int64_t nc, Job job,
-                                                TabPtrs T, int* degenerate) {
-  extern __shared__ double lds[];
-  elmat_group<D>(edges, len, nc, blockIdx.x * (int64_t)WAVE, job, T, degenerate, lds);
+                                                const double* __restrict__ coef, int* degenerate) {
+  elmat_group<D>(edges, len, nc, blockIdx.x * (int64_t)WAVE, job, coef, degenerate);
 }
 
 // Many meshes of mixed dimension in one persistent launch: every wavefront
@@ -487,9 +470,11 @@ struct BatchMesh {
 
 __global__ void __launch_bounds__(WAVE) k_elmat_batched(const BatchMesh* __restrict__ meshes,
                                                         int nmeshes, int64_t nitems,
-                                                        unsigned long long* counter, TabPtrs T2,
-                                                        TabPtrs T3, TabPtrs T4, int* degenerate) {
-  extern __shared__ double lds[];
+                                                        unsigned long long* counter,
+                                                        const double* __restrict__ coef2,
+                                                        const double* __restrict__ coef3,
+                                                        const double* __restrict__ coef4,
+                                                        int* degenerate) {
   __shared__ long long item;
   for (;;) {
     if (threadIdx.x == 0) item = (long long)atomicAdd(counter, 1ull);
@@ -505,9 +490,9 @@ __global__ void __launch_bounds__(WAVE) k_elmat_batched(const BatchMesh* __restr
     const BatchMesh& m = meshes[lo];
     const int64_t cell0 = (it - m.item0) * WAVE;
     switch (m.D) {
-      case 2: elmat_group<2>(m.edges, m.len, m.nc, cell0, m.job, T2, degenerate, lds); break;
-      case 3: elmat_group<3>(m.edges, m.len, m.nc, cell0, m.job, T3, degenerate, lds); break;
-      case 4: elmat_group<4>(m.edges, m.len, m.nc, cell0, m.job, T4, degenerate, lds); break;
+      case 2: elmat_group<2>(m.edges, m.len, m.nc, cell0, m.job, coef2, degenerate); break;
+      case 3: elmat_group<3>(m.edges, m.len, m.nc, cell0, m.job, coef3, degenerate); break;
+      case 4: elmat_group<4>(m.edges, m.len, m.nc, cell0, m.job, coef4, degenerate); break;
       default: break;
     }
   }
@@ -602,24 +587,16 @@ void run_job(Geom& g, Job& job, bool check) {
   const int D = m.d;
   int64_t nc = m.n[D];
   DevTables& T = tables_for(ctx, D);
-  int maxH = 1;
-  for (int i = 0; i < job.ngrades; ++i) {
-    int K = job.grades[i];
-    job.pair_off[i] = T.pair_off[K];
-    job.nH[i] = T.nH[K];
-    maxH = std::max(maxH, T.nH[K]);
-  }
-  size_t lds = (size_t)((D + 1) * (D + 1) + maxH) * WAVE * sizeof(double);
   int* deg = ctx->dflag;
   FEEC_HIP(hipMemsetAsync(deg, 0, sizeof(int), ctx->stream));
   unsigned grid = (unsigned)std::max<int64_t>(1, (nc + WAVE - 1) / WAVE);
   const int32_t* edges = m.faces[1].p;
-  TabPtrs tp{T.pairs.p, T.start.p, T.h.p, T.c.p};
+  const double* cf = T.coef.p;
   switch (D) {
-    case 1: k_elmat<1><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, tp, deg); break;
-    case 2: k_elmat<2><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, tp, deg); break;
-    case 3: k_elmat<3><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, tp, deg); break;
-    case 4: k_elmat<4><<<grid, WAVE, lds, ctx->stream>>>(edges, g.lengths.p, nc, job, tp, deg); break;
+    case 1: k_elmat<1><<<grid, WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, job, cf, deg); break;
+    case 2: k_elmat<2><<<grid, WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, job, cf, deg); break;
+    case 3: k_elmat<3><<<grid, WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, job, cf, deg); break;
+    case 4: k_elmat<4><<<grid, WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, job, cf, deg); break;
   }
   FEEC_HIP(hipGetLastError());
   if (check) check_degenerate(ctx, deg);
@@ -643,7 +620,7 @@ void job_add_block(Job& job, DevTables& T, int K, int kind, double* out) {
   JobBlock& b = job.b[job.nblocks++];
   b.K = K;
   b.nout = T.block_nout[K][kind];
-  b.start_off = T.block_start_off[K][kind];
+  b.coef_off = T.block_coef_off[K][kind];
   b.out = out;
 }
 
@@ -670,10 +647,6 @@ Job kkt_job(DevTables& T, int d, int k, double* const* outs) {
   // the two hdif_gram blocks of the preconditioner need M_k and dif_both(k) in full
   job_add_block(job, T, k, 0, outs[5]);
   if (k >= 1) job_add_block(job, T, k, 3, outs[6]);
-  for (int i = 0; i < job.ngrades; ++i) {
-    job.pair_off[i] = T.pair_off[job.grades[i]];
-    job.nH[i] = T.nH[job.grades[i]];
-  }
   return job;
 }
 }  // namespace
@@ -693,7 +666,6 @@ double kkt_element_blocks_batched(Ctx* ctx, const std::vector<BatchElementReques
   if (reqs.empty()) return 0.0;
   std::vector<BatchMesh> host(reqs.size());
   int64_t items = 0;
-  int maxH = 1, maxD = 2;
   for (size_t i = 0; i < reqs.size(); ++i) {
     Mesh& m = *reqs[i].g->mesh;
     FEEC_REQUIRE(m.d >= 2 && m.d <= 4, FEEC_E_UNSUPPORTED, "batched assembly covers dimensions 2..4");
@@ -705,24 +677,19 @@ double kkt_element_blocks_batched(Ctx* ctx, const std::vector<BatchElementReques
     b.len = reqs[i].g->lengths.p;
     b.item0 = items;
     b.job = kkt_job(T, m.d, reqs[i].grade, reqs[i].outs);
-    for (int gi = 0; gi < b.job.ngrades; ++gi) maxH = std::max(maxH, b.job.nH[gi]);
-    maxD = std::max(maxD, m.d);
     items += (b.nc + WAVE - 1) / WAVE;
   }
   DevTables& T2 = tables_for(ctx, 2);
   DevTables& T3 = tables_for(ctx, 3);
   DevTables& T4 = tables_for(ctx, 4);
-  TabPtrs t2{T2.pairs.p, T2.start.p, T2.h.p, T2.c.p}, t3{T3.pairs.p, T3.start.p, T3.h.p, T3.c.p},
-      t4{T4.pairs.p, T4.start.p, T4.h.p, T4.c.p};
   DBuf<BatchMesh> dm(ctx, host.size());
   FEEC_HIP(hipMemcpyAsync(dm.p, host.data(), host.size() * sizeof(BatchMesh), hipMemcpyHostToDevice,
                           ctx->stream));
-  const size_t lds = (size_t)((maxD + 1) * (maxD + 1) + maxH) * WAVE * sizeof(double);
   FEEC_HIP(hipMemsetAsync(ctx->dflag, 0, sizeof(int), ctx->stream));
   auto launch = [&](unsigned waves) {
     FEEC_HIP(hipMemsetAsync(ctx->dcount, 0, sizeof(unsigned long long), ctx->stream));
-    k_elmat_batched<<<waves, WAVE, lds, ctx->stream>>>(dm.p, (int)host.size(), items, ctx->dcount,
-                                                       t2, t3, t4, ctx->dflag);
+    k_elmat_batched<<<waves, WAVE, 0, ctx->stream>>>(dm.p, (int)host.size(), items, ctx->dcount,
+                                                     T2.coef.p, T3.coef.p, T4.coef.p, ctx->dflag);
     FEEC_HIP(hipGetLastError());
   };
   unsigned waves = 0;

@@ -1092,10 +1092,63 @@ orc_report orc_cg(const orc_csr *a, const double *inv_diag, const double *b, dou
   return rep;
 }
 
+/* A preconditioner z = B r: Jacobi (diag) or the Chebyshev block polynomial. */
+typedef struct precond {
+  const double *inv_diag;  /* Jacobi when cheb == NULL */
+  const orc_csr *cheb_p;   /* block-diagonal hdif_gram matrix, np x np */
+  const double *coef;      /* 2 + 4 (m-1): 1/theta per block, then (a, b) per step and block */
+  int64_t split;           /* rows < split belong to block 0 */
+  int m;
+  double *res, *d, *dn, *t; /* np work vectors */
+} precond;
+
+/* Chebyshev iteration (Saad, Iterative Methods for Sparse Linear Systems,
+ * Algorithm 12.1) on D^-1 P from z = 0, m directions summed; rows >= np are
+ * the identity. */
+static void cheb_apply(const precond *pc, const double *r, double *z, int64_t n) {
+  const orc_csr *P = pc->cheb_p;
+  const int64_t np = P->nrows;
+  const double *dinv = pc->inv_diag, *c = pc->coef;
+  double *res = pc->res, *d = pc->d, *dn = pc->dn, *t = pc->t;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (np > 100000)
+#endif
+  for (int64_t i = 0; i < np; ++i) {
+    res[i] = dinv[i] * r[i];
+    d[i] = res[i] * c[i < pc->split ? 0 : 1];
+    z[i] = 0.0;
+  }
+  for (int k = 0; k + 1 < pc->m; ++k) {
+    orc_spmv(P, d, t);
+    const double *ck = c + 2 + 4 * k;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (np > 100000)
+#endif
+    for (int64_t i = 0; i < np; ++i) {
+      const int blk = i < pc->split ? 0 : 1;
+      res[i] -= dinv[i] * t[i];
+      z[i] += d[i];
+      dn[i] = ck[2 * blk] * d[i] + ck[2 * blk + 1] * res[i];
+    }
+    double *sw = d;
+    d = dn;
+    dn = sw;
+  }
+  for (int64_t i = 0; i < np; ++i) z[i] += d[i];
+  for (int64_t i = np; i < n; ++i) z[i] = r[i];
+}
+
+static void precond_run(const precond *pc, const double *r, double *z, int64_t n) {
+  if (pc->cheb_p)
+    cheb_apply(pc, r, z, n);
+  else
+    precond_apply(pc->inv_diag, r, z, n);
+}
+
 /* Preconditioned MINRES (Paige-Saunders) from x0 = 0 (iterative/src/krylov.rs:113-211);
  * stops on phibar/beta1 <= rtol, the preconditioner-norm relative residual. */
-orc_report orc_minres(const orc_csr *a, const double *inv_diag, const double *b, double rtol,
-                      int64_t max_iters, double *x) {
+static orc_report minres_core(const orc_csr *a, const precond *pc, const double *b, double rtol,
+                              int64_t max_iters, double *x) {
   int64_t n = a->nrows;
   orc_report rep = {0, 1.0, 0};
   double *r1 = (double *)malloc(sizeof(double) * n);
@@ -1106,7 +1159,7 @@ orc_report orc_minres(const orc_csr *a, const double *inv_diag, const double *b,
   double *w2 = (double *)calloc(n, sizeof(double));
   double *yn = (double *)malloc(sizeof(double) * n);
   memcpy(r1, b, sizeof(double) * n);
-  precond_apply(inv_diag, r1, y, n);
+  precond_run(pc, r1, y, n);
   double beta1_sq = dot(r1, y, n);
   memset(x, 0, sizeof(double) * n);
   if (beta1_sq <= 0.0) {
@@ -1139,7 +1192,7 @@ orc_report orc_minres(const orc_csr *a, const double *inv_diag, const double *b,
       r1 = r2;
       r2 = yn;
       yn = t;
-      precond_apply(inv_diag, r2, y, n);
+      precond_run(pc, r2, y, n);
       oldb = beta;
       double bb = dot(r2, y, n);
       beta = sqrt(bb > 0.0 ? bb : 0.0);
@@ -1181,5 +1234,36 @@ done:
   free(w);
   free(w2);
   free(yn);
+  return rep;
+}
+
+orc_report orc_minres(const orc_csr *a, const double *inv_diag, const double *b, double rtol,
+                      int64_t max_iters, double *x) {
+  precond pc;
+  memset(&pc, 0, sizeof(pc));
+  pc.inv_diag = inv_diag;
+  return minres_core(a, &pc, b, rtol, max_iters, x);
+}
+
+orc_report orc_minres_cheb(const orc_csr *a, const orc_csr *p, const double *inv_diag,
+                           int64_t split, int m, const double *coef, const double *b, double rtol,
+                           int64_t max_iters, double *x) {
+  precond pc;
+  memset(&pc, 0, sizeof(pc));
+  pc.inv_diag = inv_diag;
+  pc.cheb_p = p;
+  pc.coef = coef;
+  pc.split = split;
+  pc.m = m < 1 ? 1 : m;
+  int64_t np = p->nrows;
+  pc.res = (double *)malloc(sizeof(double) * (np ? np : 1));
+  pc.d = (double *)malloc(sizeof(double) * (np ? np : 1));
+  pc.dn = (double *)malloc(sizeof(double) * (np ? np : 1));
+  pc.t = (double *)malloc(sizeof(double) * (np ? np : 1));
+  orc_report rep = minres_core(a, &pc, b, rtol, max_iters, x);
+  free(pc.res);
+  free(pc.d);
+  free(pc.dn);
+  free(pc.t);
   return rep;
 }

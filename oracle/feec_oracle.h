@@ -91,6 +91,13 @@ orc_report orc_cg(const orc_csr *a, const double *inv_diag, const double *b, dou
                   int64_t max_iters, double *x);
 orc_report orc_minres(const orc_csr *a, const double *inv_diag, const double *b, double rtol,
                       int64_t max_iters, double *x);
+/* MINRES with the Chebyshev block preconditioner: p is the block-diagonal
+ * np x np hdif_gram matrix (rows < split: block 0), inv_diag its inverse
+ * diagonal, coef = {1/theta_0, 1/theta_1, then (a_k0, b_k0, a_k1, b_k1) for
+ * k < m-1}; rows >= np are left unpreconditioned. */
+orc_report orc_minres_cheb(const orc_csr *a, const orc_csr *p, const double *inv_diag,
+                           int64_t split, int m, const double *coef, const double *b, double rtol,
+                           int64_t max_iters, double *x);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,8 @@ def lib(path=None):
             "orc_spmv": (None, [C.POINTER(_Csr), f64p, f64p]),
             "orc_cg": (_Report, [C.POINTER(_Csr), f64p, f64p, C.c_double, C.c_int64, f64p]),
             "orc_minres": (_Report, [C.POINTER(_Csr), f64p, f64p, C.c_double, C.c_int64, f64p]),
+            "orc_minres_cheb": (_Report, [C.POINTER(_Csr), C.POINTER(_Csr), f64p, C.c_int64, C.c_int,
+                                          f64p, f64p, C.c_double, C.c_int64, f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -601,7 +603,27 @@ def chebyshev_jacobi(c: WhitneyComplex, k, nh, m):
         return out
 
     prec.lambda_max = lam
+    prec.P = Csr(P.shape, P.indptr.astype(np.int64), P.indices.astype(np.int64), P.data)
+    prec.dinv = dinv
+    prec.split = split
+    prec.m = m
+    flat = [coefs[0][0], coefs[1][0]]
+    for k in range(m - 1):
+        for b in range(2):
+            flat += list(coefs[b][1][k])
+    prec.coef = np.array(flat, dtype=np.float64)
     return prec
+
+
+def minres_cheb(a, b, prec, rtol=1e-10, max_iters=10_000):
+    """The C oracle's MINRES with the Chebyshev block preconditioner built by
+    chebyshev_jacobi (the same operator as its Python closure)."""
+    s, ps = a._struct(), prec.P._struct()
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(a.shape[0])
+    rep = lib().orc_minres_cheb(C.byref(s), C.byref(ps), _p(prec.dinv, f64p), prec.split, prec.m,
+                                _p(prec.coef, f64p), _p(b, f64p), rtol, max_iters, _p(x, f64p))
+    return x, Report(int(rep.iters), float(rep.residual), bool(rep.converged))
 
 
 def minres_py(op, prec, b, rtol=1e-10, max_iters=10_000):

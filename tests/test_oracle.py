@@ -427,6 +427,10 @@ def test_chebyshev_jacobi_preconditioner():
     assert rep.iters * 4 < repj.iters, (rep.iters, repj.iters)
     x_lu = spla.spsolve(A.tocsc(), rhs)
     assert np.linalg.norm(x - x_lu) / np.linalg.norm(x_lu) < 1e-7
+    # the C restatement of the same preconditioned MINRES
+    xc, repc = o.minres_cheb(kkt, rhs, prec, 1e-10, 20000)
+    assert repc.converged and abs(repc.iters - rep.iters) <= 2, (repc.iters, rep.iters)
+    assert np.linalg.norm(xc - x) / np.linalg.norm(x) < 1e-8
 
 
 def test_minres_finite_termination():
