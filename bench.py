@@ -18,6 +18,7 @@ MINRES, same preconditioner, same 1e-10 certificate) is timed beside it.
 """
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -90,6 +91,160 @@ def cpu_baseline(n_sample, gpu_iters, c2_cells, c2_nnz, cheb_degree):
     }
 
 
+def kuhn_n_for(d, cells):
+    """Cells per axis of a d-dim Kuhn box with about `cells` simplices."""
+    return max(1, int(round((cells / math.factorial(d)) ** (1.0 / d))))
+
+
+def run_c3(args):
+    """4D Lorentzian Hodge-Dirac element matrices on the Kuhn 16^4 box
+    (1,572,864 4-simplices): time axis 0 unscaled, axes 1..3 jittered by
+    U(+-0.02 h) (seed 3), squared lengths in the Minkowski signature
+    (-,+,+,+). Throughput of the all-grade element kernel, and the error of
+    every block on a sample of cells against the FP64 oracle."""
+    import formoniq_amd as fa
+    from oracle import oracle as o
+    n = args.n if args.n != 64 else 16
+    g = fa.CartesianGrid(4, n)
+    cells, coords = g.triangulate()
+    coords = g.jitter(coords, 0.02, 3, axes=[1, 2, 3])
+    topo = fa.Complex(4, cells, nverts=len(coords))
+    geo = fa.MeshLengthsSq.from_coords(topo, coords, minkowski=True)
+    nc = topo.nsimplices(4)
+    census, _ = geo.causal_census()
+    sec, per = geo.dirac_element_time(reps=args.steps)
+    blocks = [(fa.MASS, k) for k in range(5)] + [(fa.DIF_TRIAL, k) for k in range(1, 5)]
+    rng = np.random.default_rng(3)
+    sample = np.sort(rng.choice(nc, size=min(nc, 2048), replace=False))
+    L = geo.lengths()
+    edges = topo.cell_faces(1)
+    worst = 0.0
+    for kind, k in blocks:
+        el = fa.WhitneyComplex(geo).element(kind, k)
+        for c in sample:
+            want = o.element_matrix(4, kind, k, L[edges[c]])
+            got = el[c]
+            scale = max(np.abs(want).max(), 1e-300)
+            worst = max(worst, float(np.abs(got - want).max() / scale))
+        del el
+    bytes_per_cell = 8 * 10 + 4 * 10 + 8 * per
+    line = {
+        "config": "C3", "workload": f"4D Lorentzian Hodge-Dirac element matrices, Kuhn {n}^4 ({nc} cells), "
+                                    "axes 1..3 jittered 0.02h seed 3, Minkowski (-,+,+,+)",
+        "edges_timelike_null_spacelike": census.tolist(),
+        "seconds_per_launch": sec, "values_per_cell": per,
+        "element_matrices_per_s": 9 * nc / sec, "cells_per_s": nc / sec,
+        "hbm_gbs": bytes_per_cell * nc / sec / 1e9,
+        "max_rel_err_vs_oracle": worst, "sampled_cells": int(sample.size), "dtype": "f64",
+    }
+    print(json.dumps(line), flush=True)
+
+
+def _requests(fa, count, seed, dims, cells_lo, cells_hi):
+    rng = np.random.default_rng(seed)
+    amp = {2: 0.1, 3: 0.05, 4: 0.02}
+    out = []
+    for i in range(count):
+        d = int(rng.choice(dims))
+        k = int(rng.integers(0, d))
+        cells = math.exp(rng.uniform(math.log(cells_lo), math.log(cells_hi)))
+        r = fa.Request()
+        r.dim, r.grade, r.n, r.bc = d, k, kuhn_n_for(d, cells), fa.FULL
+        r.jitter = amp[d]
+        r.mesh_seed, r.source_seed = seed * 100000 + i, seed * 100000 + 50000 + i
+        r.precision = fa.MIXED
+        out.append(r)
+    return out
+
+
+def run_c4(args):
+    """512 independent jittered Kuhn meshes (seed 4): d in {2,3,4}, k in
+    0..d-1, cells log-uniform in 1e3..1e5; all element matrices in one
+    persistent launch, then mixed-precision solves to 1e-10, overlapped on
+    worker streams. The same batch in FP64 beside it."""
+    import formoniq_amd as fa
+    reqs = _requests(fa, args.batch, 4, [2, 3, 4], 1e3, 1e5)
+    out = {}
+    for prec in (fa.MIXED, fa.FP64):
+        for r in reqs:
+            r.precision = prec
+        fa.batch_run(reqs[:8])  # warm up tables and the autotuner
+        t0 = time.perf_counter()
+        res = fa.batch_run(reqs)
+        dt = time.perf_counter() - t0
+        ok = all(x.status == 0 and x.residual <= 1e-10 for x in res)
+        out["mixed" if prec == fa.MIXED else "fp64"] = {
+            "seconds": dt, "solves_per_s": len(res) / dt, "all_converged_1e-10": ok,
+            "cells": int(sum(x.ncells for x in res)), "dofs": int(sum(x.ndofs for x in res)),
+            "iters": int(sum(x.iters for x in res)),
+        }
+    line = {"config": "C4", "workload": f"{len(reqs)} jittered Kuhn meshes, d in 2..4, k in 0..d-1, "
+                                        "cells logU[1e3,1e5], seed 4", **out,
+            "mixed_over_fp64": out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]}
+    print(json.dumps(line), flush=True)
+
+
+def run_c5(args):
+    """Serving stream (seed 5): Poisson arrivals at a fixed rate of
+    Hodge-Laplace solve requests, d in {2,3}, k in 0..d-1, cells log-uniform
+    in 1e3..2e6; admitted by HBM budget, solved mixed precision to 1e-10.
+    Throughput and p50/p99 latency, and the same stream on the FP64 path."""
+    import formoniq_amd as fa
+    reqs = _requests(fa, args.requests, 5, [2, 3], 1e3, 2e6)
+    gaps = np.random.default_rng(5).exponential(1.0 / args.rate, size=len(reqs))
+    out = {}
+    for prec in (fa.MIXED, fa.FP64):
+        for r in reqs:
+            r.precision = prec
+        opts = fa.ServerOpts()
+        opts.hbm_budget_bytes = 200 << 30
+        opts.max_inflight = args.slots
+        opts.solve = fa.solve_opts(precision=prec)
+        srv = fa.Server(opts)
+        t0 = time.perf_counter()
+        tickets = []
+        for r, gap in zip(reqs, gaps):
+            time.sleep(gap)
+            tickets.append(srv.submit(r))
+        srv.drain()
+        dt = time.perf_counter() - t0
+        res = [srv.poll(t) for t in tickets]
+        srv.close()
+        lat = np.array([x.latency for x in res])
+        ok = all(x.status == 0 and x.residual <= 1e-10 for x in res)
+        out["mixed" if prec == fa.MIXED else "fp64"] = {
+            "seconds": dt, "solves_per_s": len(res) / dt, "all_converged_1e-10": ok,
+            "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "cells": int(sum(x.ncells for x in res)),
+        }
+    line = {"config": "C5", "workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, d in 2..3, "
+                                        f"k in 0..d-1, cells logU[1e3,2e6], seed 5, {args.slots} slots", **out}
+    print(json.dumps(line), flush=True)
+
+
+def run_kernels(args):
+    """Short run for hardware-counter passes: the C2 mixed problem's element
+    kernel, KKT scatter and fused MINRES SpMV (FP32, FP64) launched a few
+    times each, then a streaming kernel of known size (torch multiply of
+    256 Mi floats: 1 GiB read, 1 GiB written) to calibrate byte counters."""
+    import torch
+    import formoniq_amd as fa
+    ctx = fa.Context(0)
+    topo, geo, wc = build_problem(fa, args.n, fa.MIXED, ctx)
+    prob = fa.Problem(wc, 1, fa.solve_opts(precision=fa.MIXED))
+    prob.assemble()
+    times = prob.kernel_times(reps=args.steps)
+    a = torch.ones(256 << 20, device="cuda:0", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"config": "kernels", "n_kkt": prob.sizes[0] + prob.sizes[1], "nnz": prob.nnz,
+                      "kernel_seconds": times, "calibration_bytes_read": a.numel() * 4,
+                      "calibration_bytes_written": b.numel() * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,14 +252,29 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64, help="cells per axis (the workload is 64)")
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp64-cheb", type=int, default=24, help="Chebyshev degree of the FP64 path")
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "mr_spmv_f32_traffic.json"))
+    ap.add_argument("--no-jacobi", action="store_true", help="skip the FP64 Jacobi-block solve")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "c2_kernel_traffic.json"))
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the Krylov kernels one by one instead of replaying graph segments "
+                         "(same kernels; for tracing tools)")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "kernels"],
+                    help="c2 (default): the headline line; c3/c4/c5: the 4D Dirac element, batched "
+                         "and serving workloads, each printed as its own JSON line")
+    ap.add_argument("--batch", type=int, default=512, help="c4: number of meshes")
+    ap.add_argument("--requests", type=int, default=48, help="c5: number of requests")
+    ap.add_argument("--rate", type=float, default=2.0, help="c5: Poisson arrival rate per second")
+    ap.add_argument("--slots", type=int, default=4, help="c5: requests resident at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported in this build "
               "(the path shards by design only; see the multi-GPU section of DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.config != "c2":
+        {"c3": run_c3, "c4": run_c4, "c5": run_c5, "kernels": run_kernels}[args.config](args)
+        return
 
     import formoniq_amd as fa
     ctx = fa.Context(0)
@@ -116,7 +286,8 @@ def main():
     f = fa.normal_vector(topo.nsimplices(1), 2)
 
     t0 = time.perf_counter()
-    mixed = fa.Problem(wc, 1, fa.solve_opts(precision=fa.MIXED))
+    use_graph = 0 if args.no_graph else 1
+    mixed = fa.Problem(wc, 1, fa.solve_opts(precision=fa.MIXED, use_graph=use_graph))
     log(f"mixed problem: sizes {mixed.sizes}, nnz {mixed.nnz}, setup {time.perf_counter() - t0:.2f}s")
     n_kkt = mixed.sizes[0] + mixed.sizes[1]
 
@@ -148,7 +319,10 @@ def main():
     log(f"kernel times: {times}")
 
     # plain FP64 GPU path
-    fp64 = fa.Problem(wc, 1, fa.solve_opts(precision=fa.FP64))
+    # each path at the Chebyshev degree that is fastest for it on this
+    # workload (FP64: 24; the mixed path runs the default, 32)
+    fp64 = fa.Problem(wc, 1, fa.solve_opts(precision=fa.FP64, cheb_degree=args.fp64_cheb,
+                                           use_graph=use_graph))
     fp64.upload_source(f)
     fp64.assemble()
     r64 = fp64.solve()
@@ -163,18 +337,42 @@ def main():
         log(f"fp64 step {i}: {t64[-1]:.3f}s iters {r64.iters} residual {r64.residual:.2e}")
     assert r64.converged and r64.residual <= 1e-10
     fp64_rate = 1.0 / statistics.mean(t64)
+    fp64_cheb = int(fp64.opts.cheb_degree)
+    del fp64
+
+    # the reference's block structure with Jacobi blocks (no polynomial): the
+    # plainest FP64 GPU path
+    jac_rate, r_jac = None, None
+    if not args.no_jacobi:
+        jac = fa.Problem(wc, 1, fa.solve_opts(precision=fa.FP64, cheb_degree=0, use_graph=use_graph))
+        jac.upload_source(f)
+        ts = time.perf_counter()
+        jac.assemble()
+        r_jac = jac.solve()
+        ctx.synchronize()
+        t_jac = time.perf_counter() - ts
+        log(f"fp64 jacobi: {t_jac:.3f}s iters {r_jac.iters} residual {r_jac.residual:.2e}")
+        assert r_jac.converged and r_jac.residual <= 1e-10
+        jac_rate = 1.0 / t_jac
+        del jac
     value = args.steps / elapsed
 
-    # roofline of the dominant kernel: the fused FP32 MINRES SpMV
+    # roofline of the dominant kernel of the mixed solve: the FP32 Chebyshev
+    # step (one preconditioner SpMV fused with the polynomial recurrence);
+    # the fused MINRES SpMV is reported beside it
     nnz = mixed.nnz
+    p_nnz, cheb_deg, _ = mixed.preconditioner()
     spmv_bytes = 8 * nnz + 8 * (n_kkt + 1) + 12 * n_kkt
-    achieved = spmv_bytes / times["spmv32"] / 1e9
+    # P values + columns, row pointers, dinv and d (4n each), res and z read
+    # and written (16n), the next direction written (4n)
+    cheb_bytes = 8 * p_nnz + 8 * (n_kkt + 1) + 28 * n_kkt
+    achieved = cheb_bytes / times["cheb32"] / 1e9
     traffic = None
     if os.path.exists(args.traffic):
         with open(args.traffic) as fh:
             tr = json.load(fh)
         if tr.get("n_kkt") == n_kkt:
-            traffic = tr.get("hbm_bytes_per_launch")
+            traffic = tr.get("cheb_step_f32_hbm_bytes_per_launch")
     # per tet: 6 edge lengths + 4 vertex indices in; M_0, dif_test(1), dif_both(2),
     # two diagonals, M_1 and dif_both(1) out (138 doubles)
     elmat_bytes = ncells * (48 + 8 * (16 + 24 + 36 + 6 + 4 + 36 + 16))
@@ -199,6 +397,9 @@ def main():
         },
         "solves_per_s_fp64": fp64_rate,
         "mixed_over_fp64": value / fp64_rate,
+        "solves_per_s_fp64_jacobi": jac_rate,
+        "mixed_over_fp64_jacobi": value / jac_rate if jac_rate else None,
+        "iters_fp64_jacobi": int(r_jac.iters) if r_jac is not None else None,
         "latency_p50_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
         "latency_p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
         "iters_mixed": int(reps[-1].iters), "refinements_mixed": int(reps[-1].refinements),
@@ -209,16 +410,19 @@ def main():
         "elmat_hbm_gbs": elmat_bytes / times["elmat"] / 1e9,
         "kernel_seconds": times,
         "roofline": {
-            "kernel": "k_mr_spmv<float> (fused FP32 MINRES SpMV)",
+            "kernel": "k_cheb_step<float> (FP32 Chebyshev step: preconditioner SpMV + recurrence)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "algorithmic_bytes_per_launch": spmv_bytes,
+            "algorithmic_bytes_per_launch": cheb_bytes,
+            "launch_seconds": times["cheb32"], "precond_nnz": p_nnz, "cheb_degree": cheb_deg,
         },
+        "mr_spmv_f32": {"launch_seconds": times["spmv32"], "algorithmic_bytes_per_launch": spmv_bytes,
+                        "achieved_gbs": spmv_bytes / times["spmv32"] / 1e9},
     }
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(args.cpu_n, int(r64.iters), ncells, nnz,
-                                                int(fp64.opts.cheb_degree))
+                                                fp64_cheb)
         except Exception as e:  # reported, never fatal for the GPU line
             line["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(line), flush=True)

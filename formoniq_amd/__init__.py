@@ -274,6 +274,12 @@ class MeshLengthsSq:
                                        _ptr(det, _lib.f64p)))
         return g, gi, det
 
+    def dirac_element_time(self, reps=10):
+        """(seconds per launch, values per cell) of the all-grade Hodge-Dirac element kernel."""
+        sec, per = C.c_double(), C.c_int64()
+        check(load().feec_dirac_element_time(self._h, reps, C.byref(sec), C.byref(per)))
+        return sec.value, per.value
+
     def causal_census(self):
         counts = np.zeros(3, np.int64)
         riem = np.zeros(self.complex.nsimplices(self.complex.dim), np.uint8)
@@ -395,10 +401,19 @@ class Problem:
         check(load().feec_problem_assemble(self._h))
 
     def kernel_times(self, reps=20):
-        """Seconds per launch: element kernel, KKT scatter, FP32 and FP64 MINRES SpMV."""
-        out = np.zeros(4)
+        """Seconds per launch: element kernel, KKT scatter, FP32 and FP64 MINRES
+        SpMV, FP32 and FP64 Chebyshev step."""
+        out = np.zeros(6)
         check(load().feec_problem_kernel_times(self._h, reps, _ptr(out, _lib.f64p)))
-        return dict(elmat=out[0], scatter=out[1], spmv32=out[2], spmv64=out[3])
+        return dict(elmat=float(out[0]), scatter=float(out[1]), spmv32=float(out[2]),
+                    spmv64=float(out[3]), cheb32=float(out[4]), cheb64=float(out[5]))
+
+    def preconditioner(self):
+        """(nonzeros, Chebyshev degree, lambda_max per block) of the block preconditioner."""
+        nnz, deg, lam = C.c_int64(), C.c_int32(), np.zeros(2)
+        check(load().feec_problem_preconditioner(self._h, C.byref(nnz), C.byref(deg),
+                                                 _ptr(lam, _lib.f64p)))
+        return nnz.value, deg.value, lam
 
     def upload_source(self, source):
         """Make `source` (N_k) resident on the device for solve()."""

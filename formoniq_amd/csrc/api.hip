@@ -461,10 +461,10 @@ void feec_solve_opts_default(feec_solve_opts* o) {
   o->max_iters = 200000;
   o->precision = FEEC_PREC_MIXED;
   o->max_refinements = 12;
-  o->inner_rtol = 1e-5;
+  o->inner_rtol = 1e-4;
   o->use_graph = 1;
   o->graph_iters = 32;
-  o->cheb_degree = 8;
+  o->cheb_degree = 32;
   o->reserved = 0;
 }
 
@@ -1022,6 +1022,19 @@ feec_status feec_problem_sizes(const feec_problem* p, int64_t sizes[3], int64_t*
   });
 }
 
+feec_status feec_problem_preconditioner(const feec_problem* p, int64_t* nnz, int32_t* cheb_degree,
+                                        double lambda_max[2]) {
+  return guard([&] {
+    FEEC_REQUIRE(p, FEEC_E_ARG, "null problem");
+    if (nnz) *nnz = p->sys.P.nnz;
+    if (cheb_degree) *cheb_degree = p->sys.cheb_m;
+    if (lambda_max) {
+      lambda_max[0] = p->sys.lambda_max[0];
+      lambda_max[1] = p->sys.lambda_max[1];
+    }
+  });
+}
+
 feec_status feec_problem_solve_device(feec_problem* p, const double* src, feec_report* report) {
   return guard([&] {
     Mesh& m = *p->g->mesh;
@@ -1073,7 +1086,7 @@ feec_status feec_problem_solution(feec_problem* p, double* sigma, double* u, dou
   });
 }
 
-feec_status feec_problem_kernel_times(feec_problem* p, int reps, double out[4]) {
+feec_status feec_problem_kernel_times(feec_problem* p, int reps, double out[6]) {
   return guard([&] {
     Mesh& m = *p->g->mesh;
     Ctx* ctx = m.ctx;
@@ -1103,6 +1116,34 @@ feec_status feec_problem_kernel_times(feec_problem* p, int reps, double out[4]) 
     out[1] = b * 1e-3 / reps;
     out[2] = time_minres_spmv(p->sys, true, reps);
     out[3] = time_minres_spmv(p->sys, false, reps);
+    out[4] = time_cheb_step(p->sys, true, reps);
+    out[5] = time_cheb_step(p->sys, false, reps);
+  });
+}
+
+feec_status feec_dirac_element_time(feec_geom* g, int reps, double* seconds,
+                                   int64_t* values_per_cell) {
+  return guard([&] {
+    FEEC_REQUIRE(g && seconds, FEEC_E_ARG, "null argument");
+    Mesh& m = *g->mesh;
+    Ctx* ctx = m.ctx;
+    FEEC_HIP(hipSetDevice(ctx->device));
+    const int64_t per = dirac_element_count(m.d);
+    if (values_per_cell) *values_per_cell = per;
+    DBuf<double> out(ctx, std::max<int64_t>(per * m.n[m.d], 1));
+    dirac_element_matrices(*g, out.p, true);
+    hipEvent_t e0, e1;
+    FEEC_HIP(hipEventCreate(&e0));
+    FEEC_HIP(hipEventCreate(&e1));
+    FEEC_HIP(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < reps; ++i) dirac_element_matrices(*g, out.p, false);
+    FEEC_HIP(hipEventRecord(e1, ctx->stream));
+    FEEC_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *seconds = ms * 1e-3 / std::max(reps, 1);
   });
 }
 

@@ -9,27 +9,25 @@
 //   WhitneyExpansion::pullback     derham/src/interpolate/form.rs:195-232
 //   WhitneyPairing::element        formoniq/src/operators.rs:201-211
 //
-// Formulation. The reference forms H = P (Lambda^k g^-1) P^T with P the k-th
-// compound of the barycentric differentials. By Cauchy-Binet this is the k-th
-// compound of the barycentric Gram matrix G = Dlambda g^-1 Dlambda^T, so each
-// H entry is one k x k minor of the (d+1) x (d+1) matrix G, which is g^-1
-// bordered by its negated row/column sums. The pullback sums (k+1)^2 signed H
-// entries per mass entry with weights 1 or 2 (the barycentric Gramian
+// Formulation. As the reference, H = P (Lambda^K g^-1) P^T with P the K-th
+// compound of the barycentric differentials Dlambda (row 0 = -1, rows 1..d =
+// I); P has entries 0 and +-1 in closed form. The pullback sums (K+1)^2 signed
+// H entries per mass entry with weights 1 or 2 (the barycentric Gramian
 // Q = (1 + delta)/((d+1)(d+2))), and the exterior derivative sides are +-1
 // incidences, so every element block -- mass, dif_trial, dif_test, dif_both --
-// is a fixed integer-coefficient linear map of the symmetric H, times
-// vol (k!)^2/((d+1)(d+2)). Those maps are built once on the host in exact
-// integer arithmetic.
+// is a fixed integer-coefficient linear map of the NU = n(n+1)/2 unique
+// entries of the symmetric Lambda^K g^-1 (n = C(d,K), NU <= 21 for d <= 4),
+// times vol (K!)^2/((d+1)(d+2)). The integer maps are built once on the host.
 //
 // Kernel shape: one wavefront per group of 64 cells, one lane per cell. The
-// lane computes g, det g, g^-1 and G in registers, then for each grade K the
-// unique K x K minors H of G (symmetric, so NU = n(n+1)/2 of them) in
-// registers, and emits every block of that grade as a dense contraction
-// out_e = scale * sum_u C[e][u] H[u]. C is wave-uniform: it is read with
-// scalar loads and enters the FP64 FMAs as an SGPR operand, so the inner loop
-// touches neither LDS nor vector memory. Element matrices are written
-// structure-of-arrays, out[entry * ncells + cell], so each store instruction
-// of a wavefront covers 64 consecutive doubles.
+// lane computes g, det g and g^-1 in registers, then for each grade K the
+// unique K x K minors L of g^-1 in registers, and emits every block of that
+// grade as a dense contraction out_e = scale * sum_u C[e][u] L[u]. C is
+// wave-uniform: it is read with scalar loads and enters the FP64 FMAs as an
+// SGPR operand, so the inner loop touches neither LDS nor vector memory.
+// Element matrices are written structure-of-arrays,
+// out[entry * ncells + cell], so each store instruction of a wavefront covers
+// 64 consecutive doubles.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -136,30 +134,6 @@ __device__ __forceinline__ void cell_geometry(const double* s, CellGeom<D>& cg) 
   }
 }
 
-// Barycentric Gram G = Dlambda g^-1 Dlambda^T, Dlambda row 0 = -1, rows 1.. = I.
-template <int D>
-__device__ __forceinline__ void bary_gram(const double (&gi)[D][D], double (&G)[D + 1][D + 1]) {
-  double colsum[D];
-  double total = 0.0;
-#pragma unroll
-  for (int j = 0; j < D; ++j) {
-    double c = 0.0;
-#pragma unroll
-    for (int i = 0; i < D; ++i) c += gi[i][j];
-    colsum[j] = c;
-    total += c;
-  }
-  G[0][0] = total;
-#pragma unroll
-  for (int j = 0; j < D; ++j) {
-    G[0][j + 1] = -colsum[j];
-    G[j + 1][0] = -colsum[j];  // g^-1 is symmetric
-#pragma unroll
-    for (int i = 0; i < D; ++i) G[i + 1][j + 1] = gi[i][j];
-  }
-}
-
-
 // ---------------------------------------------------------------------------
 // coefficient tables (host)
 // ---------------------------------------------------------------------------
@@ -194,19 +168,43 @@ static int sym_index(int a, int b, int nb) {
   return a * nb - a * (a - 1) / 2 + (b - a);
 }
 
-using LinH = std::map<int, long>;  // H index -> integer coefficient
+using LinH = std::map<int, long>;  // unique Lambda^K g^-1 index -> integer coefficient
+
+// Entry (a, S) of P = Lambda^K(Dlambda), Dlambda row 0 = -1 and rows v >= 1 =
+// e_{v-1}: a is a vertex subset (bit mask), S a subset of the d axes.
+static int compound_dlambda(unsigned a, unsigned S) {
+  if (!(a & 1u)) return S == (a >> 1) ? 1 : 0;
+  const unsigned rest = (a & ~1u) >> 1;
+  if ((S & rest) != rest || popc(S) != popc(a)) return 0;
+  const unsigned s = S & ~rest;
+  if (popc(s) != 1) return 0;
+  return (popc(S & (s - 1u)) & 1) ? 1 : -1;
+}
 
 static HostBlock make_block(int D, int K, int kind) {
   const int nv = D + 1;
   auto blades = masks_of(nv, K);
   auto dofs = masks_of(nv, K + 1);
-  const int nb = (int)blades.size(), nd = (int)dofs.size();
+  auto axes = masks_of(D, K);
+  const int nb = (int)blades.size(), nd = (int)dofs.size(), na = (int)axes.size();
   auto blade_idx = [&](unsigned m) {
     for (int i = 0; i < nb; ++i)
       if (blades[i] == m) return i;
     return -1;
   };
-  // Whitney mass M[s][t] as a combination of H entries (form.rs:195-232)
+  // H[a][b] = sum_{S,T} P[a][S] P[b][T] Lambda[S][T] (operators.rs:76, 84-94)
+  std::vector<LinH> Hl((size_t)nb * nb);
+  for (int a = 0; a < nb; ++a)
+    for (int b = 0; b < nb; ++b)
+      for (int S = 0; S < na; ++S) {
+        int pa = compound_dlambda(blades[a], axes[S]);
+        if (!pa) continue;
+        for (int T = 0; T < na; ++T) {
+          int pb = compound_dlambda(blades[b], axes[T]);
+          if (pb) Hl[(size_t)a * nb + b][sym_index(S, T, na)] += (long)pa * pb;
+        }
+      }
+  // Whitney mass M[s][t] as a combination of Lambda entries (form.rs:195-232)
   std::vector<LinH> M((size_t)nd * nd);
   for (int i = 0; i < nd; ++i)
     for (int j = 0; j < nd; ++j) {
@@ -216,7 +214,7 @@ static HostBlock make_block(int D, int K, int kind) {
           int va = elem_of(dofs[i], p), vb = elem_of(dofs[j], q);
           int a = blade_idx(dofs[i] & ~(1u << va)), b = blade_idx(dofs[j] & ~(1u << vb));
           long w = (((p + q) & 1) ? -1 : 1) * (va == vb ? 2 : 1);
-          acc[sym_index(a, b, nb)] += w;
+          for (auto& kv : Hl[(size_t)a * nb + b]) acc[kv.first] += w * kv.second;
         }
       M[(size_t)i * nd + j] = acc;
     }
@@ -318,8 +316,8 @@ static DevTables& tables_for(Ctx* ctx, int D) {
   std::vector<double> coef;
   const int nv = D + 1;
   for (int K = 0; K <= D; ++K) {
-    const int nb = binom(nv, K);
-    const int nu = nb * (nb + 1) / 2;
+    const int na = binom(D, K);
+    const int nu = na * (na + 1) / 2;
     T->nU[K] = nu;
     for (int kind = 0; kind < 6; ++kind) {
       T->block_nout[K][kind] = 0;
@@ -382,14 +380,14 @@ struct Blades {
 // SGPR operands.
 typedef const __attribute__((address_space(4))) double* ConstDouble;
 
-// Every block of grade K: the unique K x K minors H of G in registers, then
-// out_e = scale * sum_u C[e][u] H[u] with the dense integer table C.
+// Every block of grade K: the unique K x K minors L of g^-1 in registers,
+// then out_e = scale * sum_u C[e][u] L[u] with the dense integer table C.
 template <int D, int K>
-__device__ __forceinline__ void grade_blocks(const double (&G)[D + 1][D + 1], double vol,
-                                             const Job& job, const double* __restrict__ coef,
-                                             int64_t nc, int64_t cell, bool active) {
-  constexpr Blades<D + 1, K> B{};
-  constexpr int NB = Blades<D + 1, K>::count;
+__device__ __forceinline__ void grade_blocks(const double (&gi)[D][D], double vol, const Job& job,
+                                             const double* __restrict__ coef, int64_t nc,
+                                             int64_t cell, bool active) {
+  constexpr Blades<D, K> B{};
+  constexpr int NB = Blades<D, K>::count;
   constexpr int NU = NB * (NB + 1) / 2;
   double H[NU];
 #pragma unroll
@@ -400,7 +398,7 @@ __device__ __forceinline__ void grade_blocks(const double (&G)[D + 1][D + 1], do
 #pragma unroll
       for (int i = 0; i < K; ++i)
 #pragma unroll
-        for (int j = 0; j < K; ++j) m[i][j] = G[B.e[a][i]][B.e[b][j]];
+        for (int j = 0; j < K; ++j) m[i][j] = gi[B.e[a][i]][B.e[b][j]];
       H[a * NB - a * (a - 1) / 2 + (b - a)] = det<K>(m);
     }
   constexpr double kf = factorial(K);
@@ -410,11 +408,27 @@ __device__ __forceinline__ void grade_blocks(const double (&G)[D + 1][D + 1], do
     ConstDouble c = (ConstDouble)(coef + job.b[bi].coef_off);
     double* __restrict__ out = job.b[bi].out;
     const int nout = job.b[bi].nout;
-    for (int e = 0; e < nout; ++e, c += NU) {
-      double acc = 0.0;
+    // small tables: two outputs per pass (independent FMA chains on the same
+    // H operands); large ones keep one row of coefficients in SGPRs
+    constexpr int PAIR = NU <= 28 ? 2 : 1;
+    int e = 0;
+    for (; PAIR == 2 && e + 1 < nout; e += 2, c += 2 * NU) {
+      double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-      for (int u = 0; u < NU; ++u) acc += c[u] * H[u];
-      if (active) out[(int64_t)e * nc + cell] = scale * acc;
+      for (int u = 0; u < NU; ++u) {
+        a0 += c[u] * H[u];
+        a1 += c[NU + u] * H[u];
+      }
+      if (active) {
+        out[(int64_t)e * nc + cell] = scale * a0;
+        out[(int64_t)(e + 1) * nc + cell] = scale * a1;
+      }
+    }
+    for (; e < nout; ++e, c += NU) {
+      double a0 = 0.0;
+#pragma unroll
+      for (int u = 0; u < NU; ++u) a0 += c[u] * H[u];
+      if (active) out[(int64_t)e * nc + cell] = scale * a0;
     }
   }
 }
@@ -435,15 +449,13 @@ __device__ __forceinline__ void elmat_group(const int32_t* __restrict__ edges,
   cell_geometry<D>(s, cg);
   if (active && cg.det == 0.0) atomicOr(degenerate, 1);
   const double vol = (1.0 / factorial(D)) * sqrt(fabs(cg.det));
-  double G[D + 1][D + 1];
-  bary_gram<D>(cg.ginv, G);
-  for (int gi = 0; gi < job.ngrades; ++gi) {
-    switch (job.grades[gi]) {
-      case 0: grade_blocks<D, 0>(G, vol, job, coef, nc, cell, active); break;
-      case 1: grade_blocks<D, 1>(G, vol, job, coef, nc, cell, active); break;
-      case 2: if constexpr (D >= 2) grade_blocks<D, 2>(G, vol, job, coef, nc, cell, active); break;
-      case 3: if constexpr (D >= 3) grade_blocks<D, 3>(G, vol, job, coef, nc, cell, active); break;
-      case 4: if constexpr (D >= 4) grade_blocks<D, 4>(G, vol, job, coef, nc, cell, active); break;
+  for (int gk = 0; gk < job.ngrades; ++gk) {
+    switch (job.grades[gk]) {
+      case 0: grade_blocks<D, 0>(cg.ginv, vol, job, coef, nc, cell, active); break;
+      case 1: grade_blocks<D, 1>(cg.ginv, vol, job, coef, nc, cell, active); break;
+      case 2: if constexpr (D >= 2) grade_blocks<D, 2>(cg.ginv, vol, job, coef, nc, cell, active); break;
+      case 3: if constexpr (D >= 3) grade_blocks<D, 3>(cg.ginv, vol, job, coef, nc, cell, active); break;
+      case 4: if constexpr (D >= 4) grade_blocks<D, 4>(cg.ginv, vol, job, coef, nc, cell, active); break;
       default: break;
     }
   }

@@ -294,6 +294,7 @@ feec_report kkt_solve(KktSystem& s, const double* b_dev, double* x_dev, const fe
 void solver_state_free(SolverState* ws);
 void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y);
 double time_minres_spmv(KktSystem& s, bool fp32, int reps);
+double time_cheb_step(KktSystem& s, bool fp32, int reps);
 KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
                      double rtol, int64_t max_iters);
 

@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <memory>
 #include <tuple>
 #include <type_traits>
 #include <vector>
@@ -45,37 +46,50 @@ namespace feec {
 namespace {
 
 constexpr int TPB = 256;
-constexpr int NBLK = 1024;  // fixed reduction grid: 4 blocks per CU on 256 CUs
+constexpr int NBLK = 4096;  // fixed reduction grid: 16 blocks per CU on 256 CUs
 constexpr int LANES = 8;    // lanes per CSR row in the SpMV
 
 // Enqueues kernels on a stream, or appends them as a linear chain of kernel
-// nodes to a graph under construction.
-struct Enq {
-  hipStream_t st = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphNode_t last = nullptr;
-
-  template <class... P, class... A>
-  void operator()(void (*kernel)(P...), dim3 grid, dim3 block, A... a) {
-    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
-    std::tuple<std::decay_t<P>...> args(static_cast<std::decay_t<P>>(a)...);
-    void* ptrs[sizeof...(P) > 0 ? sizeof...(P) : 1];
+// nodes to a graph under construction. Node arguments are kept alive in
+// `keep` for as long as the graph exists, so nothing that inspects a node's
+// parameters after its creation can see freed memory.
+template <class... P>
+struct KernelArgs {
+  std::tuple<P...> values;
+  void* ptrs[sizeof...(P) > 0 ? sizeof...(P) : 1];
+  template <class... A>
+  explicit KernelArgs(A... a) : values(static_cast<P>(a)...) {
     std::apply(
         [&](auto&... x) {
           int i = 0;
           ((ptrs[i++] = static_cast<void*>(&x)), ...);
         },
-        args);
+        values);
+  }
+};
+
+struct Enq {
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<std::shared_ptr<void>>* keep = nullptr;
+  hipGraphNode_t last = nullptr;
+
+  template <class... P, class... A>
+  void operator()(void (*kernel)(P...), dim3 grid, dim3 block, A... a) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
     if (!graph) {
-      FEEC_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, 0, st));
+      KernelArgs<std::decay_t<P>...> args(a...);
+      FEEC_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, args.ptrs, 0, st));
       return;
     }
+    auto args = std::make_shared<KernelArgs<std::decay_t<P>...>>(a...);
+    keep->push_back(args);
     hipKernelNodeParams p{};
     p.func = reinterpret_cast<void*>(kernel);
     p.gridDim = grid;
     p.blockDim = block;
     p.sharedMemBytes = 0;
-    p.kernelParams = ptrs;
+    p.kernelParams = args->ptrs;
     p.extra = nullptr;
     hipGraphNode_t node = nullptr;
     FEEC_HIP(hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &p));
@@ -97,6 +111,42 @@ struct MinresS {
 template <class T>
 __device__ __forceinline__ double to_d(T v) {
   return (double)v;
+}
+
+// Dot product of CSR row `row` with x by LANES lanes of a wavefront, returned
+// on every lane. Each lane keeps SPMV_UNROLL independent loads in flight; the
+// summation order is fixed by the row pattern alone.
+constexpr int SPMV_UNROLL = 2;
+template <class T>
+__device__ __forceinline__ T row_dot(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                     const T* __restrict__ val, const T* __restrict__ x, int64_t row,
+                                     int lane) {
+  constexpr int U = SPMV_UNROLL;
+  const int64_t end = rp[row + 1];
+  T a[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) a[u] = 0;
+  int64_t p = rp[row] + lane;
+  for (; p + (U - 1) * LANES < end; p += U * LANES) {
+    int c[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      c[u] = col[p + u * LANES];
+      v[u] = val[p + u * LANES];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] += v[u] * x[c[u]];
+  }
+#pragma unroll
+  for (int u = 0; u + 1 < U; ++u)
+    if (p + u * LANES < end) a[u] += val[p + u * LANES] * x[col[p + u * LANES]];
+  T acc = a[0];
+#pragma unroll
+  for (int u = 1; u < U; ++u) acc += a[u];
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+  return acc;
 }
 
 __device__ __forceinline__ double block_reduce(double v, double* sh) {
@@ -123,10 +173,7 @@ __global__ void __launch_bounds__(TPB) k_spmv(const int64_t* __restrict__ rp,
   const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
   for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
        row += groups) {
-    T acc = 0;
-    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LANES) acc += val[p] * x[col[p]];
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    T acc = row_dot<T>(rp, col, val, x, row, lane);
     if (lane == 0) y[row] = acc;
   }
 }
@@ -144,10 +191,7 @@ __global__ void __launch_bounds__(TPB) k_residual(const int64_t* __restrict__ rp
   double loc = 0.0;
   for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
        row += groups) {
-    double acc = 0.0;
-    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LANES) acc += val[p] * x[col[p]];
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    double acc = row_dot<double>(rp, col, val, x, row, lane);
     if (lane == 0) {
       double ri = b[row] - acc;
       r[row] = ri;
@@ -248,10 +292,7 @@ __global__ void __launch_bounds__(TPB) k_mr_spmv(const int64_t* __restrict__ rp,
   double loc = 0.0;
   for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
        row += groups) {
-    T acc = 0;
-    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LANES) acc += val[p] * y[col[p]];
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    T acc = row_dot<T>(rp, col, val, y, row, lane);
     if (lane == 0) {
       T yn = acc * invb - cr * r1[row];
       r1[row] = yn;
@@ -462,10 +503,7 @@ __global__ void __launch_bounds__(TPB) k_cheb_step(const int64_t* __restrict__ r
   const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
   for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
        row += groups) {
-    T acc = 0;
-    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LANES) acc += val[p] * d[col[p]];
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+    T acc = row_dot<T>(rp, col, val, d, row, lane);
     if (lane == 0) {
       T di = d[row];
       T ri = res[row] - dinv[row] * acc;
@@ -581,6 +619,8 @@ inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n +
 inline unsigned red_grid(int64_t n) {
   return (unsigned)std::min<int64_t>(NBLK, std::max<int64_t>(1, (n + TPB - 1) / TPB));
 }
+// Blocks of a CSR row kernel (grid-stride over rows): one per TPB/LANES rows,
+// at most NBLK.
 inline unsigned spmv_grid(int64_t n) {
   int64_t rows_per_block = TPB / LANES;
   return (unsigned)std::min<int64_t>(NBLK, std::max<int64_t>(1, (n + rows_per_block - 1) / rows_per_block));
@@ -598,6 +638,7 @@ struct MinresWork {
   DBuf<double> part, part2;
   DBuf<MinresS> S;
   hipGraphExec_t graph = nullptr;
+  std::vector<std::shared_ptr<void>> graph_args;  // node arguments of `graph`
   int graph_iters = 0, graph_cheb = -1;
   int64_t n = 0, nh = 0;
   // block preconditioner (cheb_m == 0: Jacobi with dinv)
@@ -741,7 +782,8 @@ static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const
     // Build the segment graph node by node; it is destroyed on any failure.
     hipGraph_t g = nullptr;
     FEEC_HIP(hipGraphCreate(&g, 0));
-    Enq gq{st, g};
+    w.graph_args.clear();
+    Enq gq{st, g, &w.graph_args};
     try {
       for (int i = 0; i < seg; ++i) minres_iteration<T>(gq, w, rp, col, val, i & 1);
     } catch (...) {
@@ -927,8 +969,12 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
     while (true) {
       int64_t budget = o.max_iters - rep.iters;
       if (budget <= 0) break;
+      // a correction only has to close the remaining gap (relative to its own
+      // right-hand side, whose norm is res * |b|), with a margin of 10 for the
+      // gap between the preconditioner norm MINRES monitors and the 2-norm
+      const double tol = steps == 0 ? o.rtol : std::min(0.5, 0.1 * o.rtol / res);
       KrylovStats ks = minres_run<double>(ctx, ss, w, s.A.row_ptr.p, s.A.col.p, s.A.val.p,
-                                          o.rtol, budget, o);
+                                          tol, budget, o);
       rep.iters += ks.iters;
       rep.precond_residual = ks.precond_residual;
       k_axpy<<<grid_for(N), TPB, 0, st>>>(w.x.p, N, 1.0, x_out);
@@ -955,8 +1001,11 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
       int64_t budget = o.max_iters - rep.iters;
       if (budget <= 0) break;
       k_to_f32_scaled<<<grid_for(N), TPB, 0, st>>>(ss.r.p, N, 1.0 / rnorm, w.b.p);
+      // the last refinement step only has to close the remaining gap (margin
+      // of 10, as for the FP64 corrections)
+      const double tol = std::max(o.inner_rtol, std::min(0.5, 0.1 * o.rtol / res));
       KrylovStats ks = minres_run<float>(ctx, ss, w, s.A.row_ptr.p, s.A.col.p, s.val32.p,
-                                         o.inner_rtol, budget, o);
+                                         tol, budget, o);
       rep.iters += ks.iters;
       rep.precond_residual = ks.precond_residual;
       k_axpy_from_f32<<<grid_for(N), TPB, 0, st>>>(w.x.p, N, rnorm, x_out);
@@ -1020,6 +1069,45 @@ double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
     FEEC_HIP(hipEventSynchronize(e1));
   }
   FEEC_HIP(hipGetLastError());
+  FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms * 1e-3 / std::max(reps, 1);
+}
+
+// Average duration of one Chebyshev step (the preconditioner's fused SpMV)
+// on this system, FP32 or FP64; 0 when the system has no polynomial.
+double time_cheb_step(KktSystem& s, bool fp32, int reps) {
+  if (s.P.nnz == 0 || s.cheb_m < 2 || !s.cheb.p) return 0.0;
+  Ctx* ctx = s.ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n;
+  if (fp32 && !s.P32.p) kkt_refresh_fp32(s);
+  hipEvent_t e0, e1;
+  FEEC_HIP(hipEventCreate(&e0));
+  FEEC_HIP(hipEventCreate(&e1));
+  auto run = [&](auto tag, const auto* pval, const auto* dinv) {
+    using T = decltype(tag);
+    DBuf<T> d(ctx, n), res(ctx, n), z(ctx, n), dn(ctx, n);
+    FEEC_HIP(hipMemsetAsync(d.p, 0, n * sizeof(T), st));
+    FEEC_HIP(hipMemsetAsync(res.p, 0, n * sizeof(T), st));
+    FEEC_HIP(hipMemsetAsync(z.p, 0, n * sizeof(T), st));
+    const MinresS* none = nullptr;
+    k_cheb_step<T><<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, pval, n, s.split, dinv,
+                                                  d.p, res.p, z.p, dn.p, s.cheb.p, 0, none);
+    FEEC_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < reps; ++i)
+      k_cheb_step<T><<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, pval, n, s.split, dinv,
+                                                    d.p, res.p, z.p, dn.p, s.cheb.p, 0, none);
+    FEEC_HIP(hipEventRecord(e1, st));
+    FEEC_HIP(hipEventSynchronize(e1));
+  };
+  if (fp32)
+    run(float{}, s.P32.p, s.inv_diag32.p);
+  else
+    run(double{}, s.P.val.p, s.inv_diag.p);
+  FEEC_HIP(hipGetLastError());
+  float ms = 0.f;
   FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

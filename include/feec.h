@@ -85,11 +85,11 @@ typedef struct feec_solve_opts {
   int64_t max_iters;     /* Krylov iteration budget over all refinements (default 200000) */
   int32_t precision;     /* feec_precision (default FEEC_PREC_MIXED) */
   int32_t max_refinements; /* mixed: outer FP64 refinement steps (default 12) */
-  double inner_rtol;     /* mixed: FP32 inner MINRES tolerance (default 1e-5) */
+  double inner_rtol;     /* mixed: FP32 inner MINRES tolerance (default 1e-4) */
   int32_t use_graph;     /* capture Krylov iterations in a hipGraph (default 1) */
-  int32_t graph_iters;   /* iterations per captured segment, even (default 32) */
+  int32_t graph_iters;   /* iterations per graph segment, even (default 32) */
   int32_t cheb_degree;   /* degree of the Chebyshev-Jacobi polynomial applied to each
-                            hdif_gram block of the preconditioner; 0 = Jacobi only (default 8) */
+                            hdif_gram block of the preconditioner; 0 = Jacobi only (default 32) */
   int32_t reserved;
 } feec_solve_opts;
 
@@ -203,14 +203,27 @@ feec_status feec_problem_solve_device(feec_problem *prob, const double *source_d
 feec_status feec_problem_solve(feec_problem *prob, const double *source_host, feec_report *report);
 feec_status feec_problem_solution(feec_problem *prob, double *sigma, double *u, double *p);
 feec_status feec_problem_sizes(const feec_problem *prob, int64_t sizes[3], int64_t *nnz);
+/* The block preconditioner blockdiag(hdif_gram(k-1), hdif_gram(k)) as built:
+ * its nonzeros, the Chebyshev degree in use (0 = Jacobi) and the estimated
+ * largest eigenvalue of D^-1 hdif_gram per block (any pointer may be NULL). */
+feec_status feec_problem_preconditioner(const feec_problem *prob, int64_t *nnz,
+                                        int32_t *cheb_degree, double lambda_max[2]);
 /* Device pointer to an N_k scratch vector for feec_problem_solve_device. */
 feec_status feec_problem_source_buffer(feec_problem *prob, double **device_ptr);
 /* Copy an N_k host source into that device buffer. */
 feec_status feec_problem_upload_source(feec_problem *prob, const double *source_host);
 /* Diagnostics (no counterpart in the reference): average seconds per launch,
  * from HIP events on the context stream, of {element kernel, KKT value
- * scatter, fused MINRES SpMV in FP32, the same in FP64}. */
-feec_status feec_problem_kernel_times(feec_problem *prob, int reps, double out[4]);
+ * scatter, fused MINRES SpMV in FP32, the same in FP64, Chebyshev step (the
+ * preconditioner's fused SpMV) in FP32, the same in FP64}. */
+feec_status feec_problem_kernel_times(feec_problem *prob, int reps, double out[6]);
+
+/* Diagnostics (no counterpart in the reference): average seconds per launch of
+ * the all-grade Hodge-Dirac element kernel -- M_k for k = 0..dim and
+ * dif_trial(k) for k = 1..dim, every cell, one launch -- and the element
+ * values it writes per cell. */
+feec_status feec_dirac_element_time(feec_geom *geom, int reps, double *seconds,
+                                    int64_t *values_per_cell);
 
 /* ---- batching and serving -------------------------------------------------- */
 /* One request: a jittered Kuhn box and a Hodge-Laplace source solve. */
