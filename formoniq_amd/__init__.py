@@ -409,11 +409,12 @@ class Problem:
                     spmv64=float(out[3]), cheb32=float(out[4]), cheb64=float(out[5]))
 
     def preconditioner(self):
-        """(nonzeros, Chebyshev degree, lambda_max per block) of the block preconditioner."""
-        nnz, deg, lam = C.c_int64(), C.c_int32(), np.zeros(2)
+        """(nonzeros, Chebyshev degree, lambda_max per block, bytes per stored column
+        index) of the block preconditioner."""
+        nnz, deg, lam, cb = C.c_int64(), C.c_int32(), np.zeros(2), C.c_int32()
         check(load().feec_problem_preconditioner(self._h, C.byref(nnz), C.byref(deg),
-                                                 _ptr(lam, _lib.f64p)))
-        return nnz.value, deg.value, lam
+                                                 _ptr(lam, _lib.f64p), C.byref(cb)))
+        return nnz.value, deg.value, lam, cb.value
 
     def upload_source(self, source):
         """Make `source` (N_k) resident on the device for solve()."""

@@ -109,7 +109,7 @@ SIGNATURES = {
     "feec_problem_solution": (S, [vp, f64p, f64p, f64p]),
     "feec_problem_sizes": (S, [vp, i64p, i64p]),
     "feec_problem_source_buffer": (S, [vp, C.POINTER(vp)]),
-    "feec_problem_preconditioner": (S, [vp, i64p, C.POINTER(C.c_int32), f64p]),
+    "feec_problem_preconditioner": (S, [vp, i64p, C.POINTER(C.c_int32), f64p, C.POINTER(C.c_int32)]),
     "feec_problem_kernel_times": (S, [vp, C.c_int, f64p]),
     "feec_problem_upload_source": (S, [vp, f64p]),
     "feec_dirac_element_time": (S, [vp, C.c_int, f64p, i64p]),

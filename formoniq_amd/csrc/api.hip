@@ -1023,10 +1023,11 @@ feec_status feec_problem_sizes(const feec_problem* p, int64_t sizes[3], int64_t*
 }
 
 feec_status feec_problem_preconditioner(const feec_problem* p, int64_t* nnz, int32_t* cheb_degree,
-                                        double lambda_max[2]) {
+                                        double lambda_max[2], int32_t* column_bytes) {
   return guard([&] {
     FEEC_REQUIRE(p, FEEC_E_ARG, "null problem");
     if (nnz) *nnz = p->sys.P.nnz;
+    if (column_bytes) *column_bytes = p->sys.Pcol16.p ? 2 : 4;
     if (cheb_degree) *cheb_degree = p->sys.cheb_m;
     if (lambda_max) {
       lambda_max[0] = p->sys.lambda_max[0];

@@ -39,7 +39,9 @@ __global__ void k_keys(const int32_t* rows, const int32_t* cols, int64_t n, int6
   int32_t r = rows[i], c = cols[i];
   bool ok = r >= 0 && c >= 0;
   keys[i] = ok ? (uint64_t)r * (uint64_t)ncols + (uint64_t)c : sentinel;
-  if (ok) atomicAdd(nvalid, 1ull);
+  // one atomic per wavefront (lane 0 is active whenever any lane is)
+  const unsigned long long valid = __ballot(ok);
+  if ((threadIdx.x & 63) == 0 && valid) atomicAdd(nvalid, (unsigned long long)__popcll(valid));
 }
 
 __global__ void k_heads(const uint64_t* keys, int64_t n, int32_t* head) {

@@ -567,7 +567,14 @@ __global__ void k_census(const double* len, int64_t ne, unsigned long long* coun
   int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= ne) return;
   double s = len[e];
-  atomicAdd(&counts[s < 0.0 ? 0 : (s == 0.0 ? 1 : 2)], 1ull);
+  // one atomic per wavefront and category (lane 0 is active whenever any lane is)
+  const unsigned long long neg = __ballot(s < 0.0), zero = __ballot(s == 0.0),
+                           pos = __ballot(!(s < 0.0) && !(s == 0.0));
+  if ((threadIdx.x & 63) == 0) {
+    if (neg) atomicAdd(&counts[0], (unsigned long long)__popcll(neg));
+    if (zero) atomicAdd(&counts[1], (unsigned long long)__popcll(zero));
+    if (pos) atomicAdd(&counts[2], (unsigned long long)__popcll(pos));
+  }
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }

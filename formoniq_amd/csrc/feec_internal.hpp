@@ -278,6 +278,8 @@ struct KktSystem {
   // (nnz == 0: Jacobi with inv_diag), its FP32 values and the Chebyshev
   // coefficients of its two blocks (rows < split: block 0).
   Csr P;
+  DBuf<int16_t> Pcol16;  // P's columns as col - row, when every |col - row| <= 32767
+  bool band_checked = false;
   DBuf<float> P32;
   int64_t split = 0;
   int cheb_m = 0;

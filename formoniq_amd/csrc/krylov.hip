@@ -117,8 +117,17 @@ __device__ __forceinline__ double to_d(T v) {
 // on every lane. Each lane keeps SPMV_UNROLL independent loads in flight; the
 // summation order is fixed by the row pattern alone.
 constexpr int SPMV_UNROLL = 2;
-template <class T>
-__device__ __forceinline__ T row_dot(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+// Column of entry p of row `row`: stored directly (int32) or as a signed
+// 16-bit offset from the row (a square matrix whose band fits in +-32767).
+__device__ __forceinline__ int64_t col_at(const int32_t* __restrict__ col, int64_t p, int64_t) {
+  return col[p];
+}
+__device__ __forceinline__ int64_t col_at(const int16_t* __restrict__ col, int64_t p, int64_t row) {
+  return row + col[p];
+}
+
+template <class T, class I = int32_t>
+__device__ __forceinline__ T row_dot(const int64_t* __restrict__ rp, const I* __restrict__ col,
                                      const T* __restrict__ val, const T* __restrict__ x, int64_t row,
                                      int lane) {
   constexpr int U = SPMV_UNROLL;
@@ -128,11 +137,11 @@ __device__ __forceinline__ T row_dot(const int64_t* __restrict__ rp, const int32
   for (int u = 0; u < U; ++u) a[u] = 0;
   int64_t p = rp[row] + lane;
   for (; p + (U - 1) * LANES < end; p += U * LANES) {
-    int c[U];
+    int64_t c[U];
     T v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      c[u] = col[p + u * LANES];
+      c[u] = col_at(col, p + u * LANES, row);
       v[u] = val[p + u * LANES];
     }
 #pragma unroll
@@ -140,7 +149,7 @@ __device__ __forceinline__ T row_dot(const int64_t* __restrict__ rp, const int32
   }
 #pragma unroll
   for (int u = 0; u + 1 < U; ++u)
-    if (p + u * LANES < end) a[u] += val[p + u * LANES] * x[col[p + u * LANES]];
+    if (p + u * LANES < end) a[u] += val[p + u * LANES] * x[col_at(col, p + u * LANES, row)];
   T acc = a[0];
 #pragma unroll
   for (int u = 1; u < U; ++u) acc += a[u];
@@ -488,9 +497,9 @@ __global__ void __launch_bounds__(TPB) k_cheb_init(const T* __restrict__ r, cons
 }
 
 // one step: t = (A d)_i; res_i -= dinv_i t; z_i += d_i; dn_i = c1 d_i + c2 res_i
-template <class T>
+template <class T, class I>
 __global__ void __launch_bounds__(TPB) k_cheb_step(const int64_t* __restrict__ rp,
-                                                   const int32_t* __restrict__ col,
+                                                   const I* __restrict__ col,
                                                    const T* __restrict__ val, int64_t n, int64_t split,
                                                    const T* __restrict__ dinv, const T* __restrict__ d,
                                                    T* __restrict__ res, T* __restrict__ z,
@@ -503,7 +512,7 @@ __global__ void __launch_bounds__(TPB) k_cheb_step(const int64_t* __restrict__ r
   const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
   for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
        row += groups) {
-    T acc = row_dot<T>(rp, col, val, d, row, lane);
+    T acc = row_dot<T, I>(rp, col, val, d, row, lane);
     if (lane == 0) {
       T di = d[row];
       T ri = res[row] - dinv[row] * acc;
@@ -646,6 +655,7 @@ struct MinresWork {
   int64_t split = 0;
   const int64_t* prp = nullptr;
   const int32_t* pcol = nullptr;
+  const int16_t* pcol16 = nullptr;  // band-offset columns of P, when they fit
   const T* pval = nullptr;
   const double* coef = nullptr;
   ~MinresWork() {
@@ -699,7 +709,7 @@ template <class T>
 static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval) {
   int m = (s.P.nnz > 0 && s.cheb_m > 0 && s.cheb.p) ? s.cheb_m : 0;
   if (w.graph && (w.pval != pval || w.coef != s.cheb.p || w.prp != s.P.row_ptr.p ||
-                  w.pcol != s.P.col.p || w.cheb_m != m)) {
+                  w.pcol != s.P.col.p || w.pcol16 != s.Pcol16.p || w.cheb_m != m)) {
     (void)hipGraphExecDestroy(w.graph);
     w.graph = nullptr;
   }
@@ -707,6 +717,7 @@ static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval) {
   w.split = s.split;
   w.prp = s.P.row_ptr.p;
   w.pcol = s.P.col.p;
+  w.pcol16 = s.Pcol16.p;
   w.pval = pval;
   w.coef = s.cheb.p;
 }
@@ -721,8 +732,12 @@ static void cheb_apply(Enq& q, MinresWork<T>& w, const T* r, T* z, const MinresS
   T* d = w.d0.p;
   T* dn = w.d1.p;
   for (int k = 0; k + 1 < w.cheb_m; ++k) {
-    q(k_cheb_step<T>, spmv_grid(n), TPB, w.prp, w.pcol, w.pval, n, w.split, w.dinv.p, d, w.res.p,
-      z, dn, w.coef, k, S);
+    if (w.pcol16)
+      q(k_cheb_step<T, int16_t>, spmv_grid(n), TPB, w.prp, w.pcol16, w.pval, n, w.split, w.dinv.p, d,
+        w.res.p, z, dn, w.coef, k, S);
+    else
+      q(k_cheb_step<T, int32_t>, spmv_grid(n), TPB, w.prp, w.pcol, w.pval, n, w.split, w.dinv.p, d,
+        w.res.p, z, dn, w.coef, k, S);
     std::swap(d, dn);
   }
   q(k_cheb_finish<T>, red_grid(N), TPB, r, d, z, n, N, S, part);
@@ -830,11 +845,37 @@ void kkt_refresh_fp32(KktSystem& s) {
 
 // Power iteration on D^-1 P, one estimate per diagonal block; then the
 // Chebyshev interval [b lo, b] with b = 1.05 lambda_max and lo chosen by degree.
+// Band-offset copy of P's columns: col - row as int16 (one pass), and the
+// largest |col - row| so the caller keeps it only when it fits.
+__global__ void k_band16(const int64_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
+                         int16_t* __restrict__ off, unsigned long long* maxdist) {
+  unsigned long long md = 0;
+  for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < n;
+       row += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t p = rp[row]; p < rp[row + 1]; ++p) {
+      const int64_t dlt = (int64_t)col[p] - row;
+      const unsigned long long a = (unsigned long long)(dlt < 0 ? -dlt : dlt);
+      md = a > md ? a : md;
+      off[p] = (int16_t)(a <= 32767 ? dlt : 0);
+    }
+  if (md) atomicMax(maxdist, md);
+}
+
 void kkt_set_chebyshev(KktSystem& s, int m) {
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
   s.cheb_m = (s.P.nnz > 0) ? std::max(m, 0) : 0;
   if (s.cheb_m == 0) return;
+  if (!s.band_checked) {
+    s.band_checked = true;
+    s.Pcol16.alloc(ctx, s.P.nnz);
+    FEEC_HIP(hipMemsetAsync(ctx->dcount, 0, sizeof(unsigned long long), st));
+    k_band16<<<grid_for(s.n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.n, s.Pcol16.p, ctx->dcount);
+    unsigned long long md = 0;
+    FEEC_HIP(hipMemcpyAsync(&md, ctx->dcount, sizeof(md), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    if (md > 32767) s.Pcol16.reset();
+  }
   const int64_t n = s.n;
   DBuf<double> x(ctx, n), y(ctx, n), part(ctx, NBLK + 1);
   std::vector<double> hx(n);
@@ -1093,12 +1134,19 @@ double time_cheb_step(KktSystem& s, bool fp32, int reps) {
     FEEC_HIP(hipMemsetAsync(res.p, 0, n * sizeof(T), st));
     FEEC_HIP(hipMemsetAsync(z.p, 0, n * sizeof(T), st));
     const MinresS* none = nullptr;
-    k_cheb_step<T><<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, pval, n, s.split, dinv,
-                                                  d.p, res.p, z.p, dn.p, s.cheb.p, 0, none);
+    auto step = [&] {
+      if (s.Pcol16.p)
+        k_cheb_step<T, int16_t><<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.Pcol16.p, pval, n,
+                                                             s.split, dinv, d.p, res.p, z.p, dn.p,
+                                                             s.cheb.p, 0, none);
+      else
+        k_cheb_step<T, int32_t><<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, pval, n,
+                                                             s.split, dinv, d.p, res.p, z.p, dn.p,
+                                                             s.cheb.p, 0, none);
+    };
+    step();
     FEEC_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < reps; ++i)
-      k_cheb_step<T><<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, pval, n, s.split, dinv,
-                                                    d.p, res.p, z.p, dn.p, s.cheb.p, 0, none);
+    for (int i = 0; i < reps; ++i) step();
     FEEC_HIP(hipEventRecord(e1, st));
     FEEC_HIP(hipEventSynchronize(e1));
   };

@@ -279,6 +279,7 @@ feec_status feec_batch_run(feec_ctx* ctx, int nreq, const feec_request* reqs,
     for (auto* c : wctx) c->sync();
     kkt_element_blocks_batched(ctx, br);
     ctx->sync();
+    const double t_elements = seconds_since(t0);
     // 3. values, preconditioner and solve, overlapped across worker streams
     std::vector<double> t_done(nreq, 0.0);
     for_workers([&](int i, feec_ctx*) {
@@ -293,6 +294,9 @@ feec_status feec_batch_run(feec_ctx* ctx, int nreq, const feec_request* reqs,
         results[i].status = status[i];
       }
       results[i].ticket = i;
+      // batch phases: meshes, plans and all element blocks done by admit_time,
+      // this request's values and solve by finish_time
+      results[i].admit_time = t_elements;
       results[i].finish_time = t_done[i];
       results[i].latency = t_done[i];
     }

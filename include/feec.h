@@ -204,10 +204,13 @@ feec_status feec_problem_solve(feec_problem *prob, const double *source_host, fe
 feec_status feec_problem_solution(feec_problem *prob, double *sigma, double *u, double *p);
 feec_status feec_problem_sizes(const feec_problem *prob, int64_t sizes[3], int64_t *nnz);
 /* The block preconditioner blockdiag(hdif_gram(k-1), hdif_gram(k)) as built:
- * its nonzeros, the Chebyshev degree in use (0 = Jacobi) and the estimated
- * largest eigenvalue of D^-1 hdif_gram per block (any pointer may be NULL). */
+ * its nonzeros, the Chebyshev degree in use (0 = Jacobi), the estimated
+ * largest eigenvalue of D^-1 hdif_gram per block, and the bytes per stored
+ * column index (2 when every |col - row| fits a 16-bit offset, else 4); any
+ * pointer may be NULL. */
 feec_status feec_problem_preconditioner(const feec_problem *prob, int64_t *nnz,
-                                        int32_t *cheb_degree, double lambda_max[2]);
+                                        int32_t *cheb_degree, double lambda_max[2],
+                                        int32_t *column_bytes);
 /* Device pointer to an N_k scratch vector for feec_problem_solve_device. */
 feec_status feec_problem_source_buffer(feec_problem *prob, double **device_ptr);
 /* Copy an N_k host source into that device buffer. */
@@ -249,7 +252,8 @@ typedef struct feec_result {
   double residual;      /* true relative residual */
   double u_norm;        /* |u|_2 of the solution (for alone-vs-mixed checks) */
   double u_checksum;    /* sum_i u_i * ((i % 7) - 3) */
-  double admit_time;    /* seconds since server start */
+  double admit_time;    /* seconds since server start (batch: since the call, when every
+                           mesh, plan and element block was done) */
   double finish_time;
   double latency;       /* finish - submit, seconds */
 } feec_result;
