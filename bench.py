@@ -164,13 +164,14 @@ def run_c4(args):
     worker streams. The same batch in FP64 beside it."""
     import formoniq_amd as fa
     reqs = _requests(fa, args.batch, 4, [2, 3, 4], 1e3, 1e5)
+    opts = fa.solve_opts(use_graph=0 if args.no_graph else 1)
     out = {}
     for prec in (fa.MIXED, fa.FP64):
         for r in reqs:
             r.precision = prec
-        fa.batch_run(reqs[:8])  # warm up tables and the autotuner
+        fa.batch_run(reqs[:8], opts)  # warm up tables and the autotuner
         t0 = time.perf_counter()
-        res = fa.batch_run(reqs)
+        res = fa.batch_run(reqs, opts)
         dt = time.perf_counter() - t0
         log(f"c4 {'mixed' if prec == fa.MIXED else 'fp64'}: {dt:.2f}s")
         ok = all(x.status == 0 and x.residual <= 1e-10 for x in res)
@@ -366,12 +367,13 @@ def main():
     # step (one preconditioner SpMV fused with the polynomial recurrence);
     # the fused MINRES SpMV is reported beside it
     nnz = mixed.nnz
-    p_nnz, cheb_deg, _, p_col_bytes = mixed.preconditioner()
+    p_nnz, cheb_deg, _, p_col_bytes, p_val_bytes = mixed.preconditioner()
     spmv_bytes = 8 * nnz + 8 * (n_kkt + 1) + 12 * n_kkt
-    # P values (4 B) + columns (2 B as band offsets, else 4 B), row pointers,
-    # dinv and d (4n each), res and z read and written (16n), the next
-    # direction written (4n)
-    cheb_bytes = (4 + p_col_bytes) * p_nnz + 8 * (n_kkt + 1) + 28 * n_kkt
+    # P values (2 B as scaled FP16, else 4 B) + columns (2 B as band offsets,
+    # else 4 B), row pointers, d (4n), res and z read and written (16n), the
+    # next direction written (4n); dinv (4n) only when unscaled
+    cheb_bytes = ((p_val_bytes + p_col_bytes) * p_nnz + 8 * (n_kkt + 1) + 24 * n_kkt
+                  + (4 * n_kkt if p_val_bytes == 4 else 0))
     achieved = cheb_bytes / times["cheb32"] / 1e9
     traffic = None
     if os.path.exists(args.traffic):
@@ -416,12 +418,12 @@ def main():
         "elmat_hbm_gbs": elmat_bytes / times["elmat"] / 1e9,
         "kernel_seconds": times,
         "roofline": {
-            "kernel": "k_cheb_step<float> (FP32 Chebyshev step: preconditioner SpMV + recurrence)",
+            "kernel": "k_cheb_step<float> (Chebyshev step: preconditioner SpMV + recurrence, FP32 arithmetic)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "algorithmic_bytes_per_launch": cheb_bytes,
             "launch_seconds": times["cheb32"], "precond_nnz": p_nnz, "cheb_degree": cheb_deg,
-            "precond_column_bytes": p_col_bytes,
+            "precond_column_bytes": p_col_bytes, "precond_value_bytes": p_val_bytes,
         },
         "mr_spmv_f32": {"launch_seconds": times["spmv32"], "algorithmic_bytes_per_launch": spmv_bytes,
                         "achieved_gbs": spmv_bytes / times["spmv32"] / 1e9},

@@ -410,11 +410,11 @@ class Problem:
 
     def preconditioner(self):
         """(nonzeros, Chebyshev degree, lambda_max per block, bytes per stored column
-        index) of the block preconditioner."""
-        nnz, deg, lam, cb = C.c_int64(), C.c_int32(), np.zeros(2), C.c_int32()
+        index, bytes per stored value on the mixed path) of the block preconditioner."""
+        nnz, deg, lam, cb, vb = C.c_int64(), C.c_int32(), np.zeros(2), C.c_int32(), C.c_int32()
         check(load().feec_problem_preconditioner(self._h, C.byref(nnz), C.byref(deg),
-                                                 _ptr(lam, _lib.f64p), C.byref(cb)))
-        return nnz.value, deg.value, lam, cb.value
+                                                 _ptr(lam, _lib.f64p), C.byref(cb), C.byref(vb)))
+        return nnz.value, deg.value, lam, cb.value, vb.value
 
     def upload_source(self, source):
         """Make `source` (N_k) resident on the device for solve()."""

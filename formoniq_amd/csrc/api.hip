@@ -401,8 +401,8 @@ void problem_delete(Problem* P) { delete static_cast<feec_problem_s*>(P); }
 
 int64_t problem_ndofs(Problem* P) { return P->sys.n + P->nh; }
 
-// Upload a host source, solve; the report says whether it converged.
-feec_report problem_solve_host(Problem* P, const double* src) {
+// Upload a host source and form the KKT right-hand side (rhs = [0; E^T f; 0]).
+void problem_prepare_rhs(Problem* P, const double* src) {
   Mesh& m = *P->g->mesh;
   Ctx* ctx = m.ctx;
   FEEC_HIP(hipMemcpy(P->source.p, src, m.n[P->k] * 8, hipMemcpyHostToDevice));
@@ -413,10 +413,17 @@ feec_report problem_solve_host(Problem* P, const double* src) {
   k_rhs<<<grid_for(m.n[P->k]), TPB, 0, ctx->stream>>>(P->source.p, map, m.n[P->k], P->ns, N,
                                                       P->rhs.p);
   FEEC_HIP(hipGetLastError());
+}
+
+// Upload a host source, solve; the report says whether it converged.
+feec_report problem_solve_host(Problem* P, const double* src) {
+  problem_prepare_rhs(P, src);
   feec_report r = kkt_solve(P->sys, P->rhs.p, P->x.p, P->opts, &P->ws);
   P->last = r;
   return r;
 }
+
+void problem_set_report(Problem* P, const feec_report& r) { P->last = r; }
 
 void problem_u(Problem* P, double* u) {
   Mesh& m = *P->g->mesh;
@@ -465,7 +472,7 @@ void feec_solve_opts_default(feec_solve_opts* o) {
   o->use_graph = 1;
   o->graph_iters = 32;
   o->cheb_degree = 32;
-  o->reserved = 0;
+  o->precond_half = 1;
 }
 
 feec_status feec_ctx_create(int device, feec_ctx** out) {
@@ -1023,11 +1030,13 @@ feec_status feec_problem_sizes(const feec_problem* p, int64_t sizes[3], int64_t*
 }
 
 feec_status feec_problem_preconditioner(const feec_problem* p, int64_t* nnz, int32_t* cheb_degree,
-                                        double lambda_max[2], int32_t* column_bytes) {
+                                        double lambda_max[2], int32_t* column_bytes,
+                                        int32_t* value_bytes_mixed) {
   return guard([&] {
     FEEC_REQUIRE(p, FEEC_E_ARG, "null problem");
     if (nnz) *nnz = p->sys.P.nnz;
     if (column_bytes) *column_bytes = p->sys.Pcol16.p ? 2 : 4;
+    if (value_bytes_mixed) *value_bytes_mixed = (p->opts.precond_half && p->sys.P16.p) ? 2 : 4;
     if (cheb_degree) *cheb_degree = p->sys.cheb_m;
     if (lambda_max) {
       lambda_max[0] = p->sys.lambda_max[0];
@@ -1117,8 +1126,8 @@ feec_status feec_problem_kernel_times(feec_problem* p, int reps, double out[6]) 
     out[1] = b * 1e-3 / reps;
     out[2] = time_minres_spmv(p->sys, true, reps);
     out[3] = time_minres_spmv(p->sys, false, reps);
-    out[4] = time_cheb_step(p->sys, true, reps);
-    out[5] = time_cheb_step(p->sys, false, reps);
+    out[4] = time_cheb_step(p->sys, true, reps, p->opts.precond_half != 0);
+    out[5] = time_cheb_step(p->sys, false, reps, false);
   });
 }
 

@@ -281,6 +281,8 @@ struct KktSystem {
   DBuf<int16_t> Pcol16;  // P's columns as col - row, when every |col - row| <= 32767
   bool band_checked = false;
   DBuf<float> P32;
+  DBuf<_Float16> P16;    // D^-1/2 P D^-1/2 in FP16 (the mixed path's polynomial)
+  DBuf<float> sdinv32;   // D^-1/2, n entries
   int64_t split = 0;
   int cheb_m = 0;
   DBuf<double> cheb;
@@ -296,7 +298,7 @@ feec_report kkt_solve(KktSystem& s, const double* b_dev, double* x_dev, const fe
 void solver_state_free(SolverState* ws);
 void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y);
 double time_minres_spmv(KktSystem& s, bool fp32, int reps);
-double time_cheb_step(KktSystem& s, bool fp32, int reps);
+double time_cheb_step(KktSystem& s, bool fp32, int reps, bool half);
 KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
                      double rtol, int64_t max_iters);
 

@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
 #include <tuple>
 #include <type_traits>
@@ -126,9 +127,9 @@ __device__ __forceinline__ int64_t col_at(const int16_t* __restrict__ col, int64
   return row + col[p];
 }
 
-template <class T, class I = int32_t>
+template <class T, class I = int32_t, class V = T>
 __device__ __forceinline__ T row_dot(const int64_t* __restrict__ rp, const I* __restrict__ col,
-                                     const T* __restrict__ val, const T* __restrict__ x, int64_t row,
+                                     const V* __restrict__ val, const T* __restrict__ x, int64_t row,
                                      int lane) {
   constexpr int U = SPMV_UNROLL;
   const int64_t end = rp[row + 1];
@@ -138,18 +139,18 @@ __device__ __forceinline__ T row_dot(const int64_t* __restrict__ rp, const I* __
   int64_t p = rp[row] + lane;
   for (; p + (U - 1) * LANES < end; p += U * LANES) {
     int64_t c[U];
-    T v[U];
+    V v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       c[u] = col_at(col, p + u * LANES, row);
       v[u] = val[p + u * LANES];
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) a[u] += v[u] * x[c[u]];
+    for (int u = 0; u < U; ++u) a[u] += (T)v[u] * x[c[u]];
   }
 #pragma unroll
   for (int u = 0; u + 1 < U; ++u)
-    if (p + u * LANES < end) a[u] += val[p + u * LANES] * x[col_at(col, p + u * LANES, row)];
+    if (p + u * LANES < end) a[u] += (T)val[p + u * LANES] * x[col_at(col, p + u * LANES, row)];
   T acc = a[0];
 #pragma unroll
   for (int u = 1; u < U; ++u) acc += a[u];
@@ -496,11 +497,13 @@ __global__ void __launch_bounds__(TPB) k_cheb_init(const T* __restrict__ r, cons
   }
 }
 
-// one step: t = (A d)_i; res_i -= dinv_i t; z_i += d_i; dn_i = c1 d_i + c2 res_i
-template <class T, class I>
+// one step: t = (A d)_i; res_i -= dinv_i t; z_i += d_i; dn_i = c1 d_i + c2 res_i.
+// UNIT: A is the diagonally scaled D^-1/2 P D^-1/2 (unit diagonal), so the
+// Jacobi factor is 1 and dinv is not read.
+template <class T, class I, class V = T, bool UNIT = false>
 __global__ void __launch_bounds__(TPB) k_cheb_step(const int64_t* __restrict__ rp,
                                                    const I* __restrict__ col,
-                                                   const T* __restrict__ val, int64_t n, int64_t split,
+                                                   const V* __restrict__ val, int64_t n, int64_t split,
                                                    const T* __restrict__ dinv, const T* __restrict__ d,
                                                    T* __restrict__ res, T* __restrict__ z,
                                                    T* __restrict__ dn, const double* __restrict__ coef,
@@ -512,10 +515,10 @@ __global__ void __launch_bounds__(TPB) k_cheb_step(const int64_t* __restrict__ r
   const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
   for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
        row += groups) {
-    T acc = row_dot<T, I>(rp, col, val, d, row, lane);
+    T acc = row_dot<T, I, V>(rp, col, val, d, row, lane);
     if (lane == 0) {
       T di = d[row];
-      T ri = res[row] - dinv[row] * acc;
+      T ri = UNIT ? res[row] - acc : res[row] - dinv[row] * acc;
       res[row] = ri;
       z[row] += di;
       dn[row] = (row < split) ? (a0 * di + b0 * ri) : (a1 * di + b1 * ri);
@@ -523,10 +526,12 @@ __global__ void __launch_bounds__(TPB) k_cheb_step(const int64_t* __restrict__ r
   }
 }
 
-// z_i += d_i (the last step's direction); partial dot r.z over all N rows
+// z_i += d_i (the last step's direction), times sd_i when the polynomial ran
+// on the scaled matrix; partial dot r.z over all N rows
 template <class T>
 __global__ void __launch_bounds__(TPB) k_cheb_finish(const T* __restrict__ r, const T* __restrict__ d,
-                                                     T* __restrict__ z, int64_t n, int64_t N,
+                                                     T* __restrict__ z, const T* __restrict__ sd,
+                                                     int64_t n, int64_t N,
                                                      const MinresS* __restrict__ S, double* part) {
   if (S && S->done) return;
   __shared__ double sh[TPB / 64];
@@ -536,6 +541,7 @@ __global__ void __launch_bounds__(TPB) k_cheb_finish(const T* __restrict__ r, co
     T zi = z[i];
     if (i < n) {
       zi += d[i];
+      if (sd) zi *= sd[i];
       z[i] = zi;
     }
     v += to_d(r[i]) * to_d(zi);
@@ -656,6 +662,8 @@ struct MinresWork {
   const int64_t* prp = nullptr;
   const int32_t* pcol = nullptr;
   const int16_t* pcol16 = nullptr;  // band-offset columns of P, when they fit
+  const _Float16* pval16 = nullptr;  // scaled P in FP16 (FP32 path), or null
+  const T* sd = nullptr;             // D^-1/2 for the scaled polynomial
   const T* pval = nullptr;
   const double* coef = nullptr;
   ~MinresWork() {
@@ -706,10 +714,12 @@ static void minres_alloc(Ctx* ctx, MinresWork<T>& w, int64_t n, int64_t nh) {
 // Point the workspace at the system's preconditioner (matrix values of
 // precision T); a captured graph holding other pointers is dropped.
 template <class T>
-static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval) {
+static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval,
+                                const _Float16* pval16 = nullptr, const T* sd = nullptr) {
   int m = (s.P.nnz > 0 && s.cheb_m > 0 && s.cheb.p) ? s.cheb_m : 0;
   if (w.graph && (w.pval != pval || w.coef != s.cheb.p || w.prp != s.P.row_ptr.p ||
-                  w.pcol != s.P.col.p || w.pcol16 != s.Pcol16.p || w.cheb_m != m)) {
+                  w.pcol != s.P.col.p || w.pcol16 != s.Pcol16.p || w.pval16 != pval16 ||
+                  w.sd != sd || w.cheb_m != m)) {
     (void)hipGraphExecDestroy(w.graph);
     w.graph = nullptr;
   }
@@ -719,6 +729,8 @@ static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval) {
   w.pcol = s.P.col.p;
   w.pcol16 = s.Pcol16.p;
   w.pval = pval;
+  w.pval16 = pval16;
+  w.sd = sd;
   w.coef = s.cheb.p;
 }
 
@@ -728,11 +740,21 @@ template <class T>
 static void cheb_apply(Enq& q, MinresWork<T>& w, const T* r, T* z, const MinresS* S,
                        double* part) {
   const int64_t n = w.n, N = w.n + w.nh;
-  q(k_cheb_init<T>, red_grid(N), TPB, r, w.dinv.p, w.res.p, w.d0.p, z, n, N, w.split, w.coef, S);
+  // scaled mode: the polynomial runs on D^-1/2 P D^-1/2 (FP16 values) with
+  // input and output scaled by D^-1/2; otherwise on D^-1 P
+  const bool scaled = w.pval16 && w.sd;
+  q(k_cheb_init<T>, red_grid(N), TPB, r, scaled ? w.sd : w.dinv.p, w.res.p, w.d0.p, z, n, N,
+    w.split, w.coef, S);
   T* d = w.d0.p;
   T* dn = w.d1.p;
   for (int k = 0; k + 1 < w.cheb_m; ++k) {
-    if (w.pcol16)
+    if (scaled && w.pcol16)
+      q(k_cheb_step<T, int16_t, _Float16, true>, spmv_grid(n), TPB, w.prp, w.pcol16, w.pval16, n,
+        w.split, w.dinv.p, d, w.res.p, z, dn, w.coef, k, S);
+    else if (scaled)
+      q(k_cheb_step<T, int32_t, _Float16, true>, spmv_grid(n), TPB, w.prp, w.pcol, w.pval16, n,
+        w.split, w.dinv.p, d, w.res.p, z, dn, w.coef, k, S);
+    else if (w.pcol16)
       q(k_cheb_step<T, int16_t>, spmv_grid(n), TPB, w.prp, w.pcol16, w.pval, n, w.split, w.dinv.p, d,
         w.res.p, z, dn, w.coef, k, S);
     else
@@ -740,7 +762,7 @@ static void cheb_apply(Enq& q, MinresWork<T>& w, const T* r, T* z, const MinresS
         w.res.p, z, dn, w.coef, k, S);
     std::swap(d, dn);
   }
-  q(k_cheb_finish<T>, red_grid(N), TPB, r, d, z, n, N, S, part);
+  q(k_cheb_finish<T>, red_grid(N), TPB, r, d, z, scaled ? w.sd : (const T*)nullptr, n, N, S, part);
 }
 
 // Enqueue one MINRES iteration; `par` selects the buffer roles.
@@ -829,6 +851,20 @@ static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const
   return ks;
 }
 
+// Scaled preconditioner matrix D^-1/2 P D^-1/2 in FP16 (unit diagonal,
+// entries in [-1, 1] for an SPD P) and D^-1/2 in FP32.
+__global__ void k_scale_half(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                             const double* __restrict__ val, const double* __restrict__ inv_diag,
+                             int64_t n, _Float16* __restrict__ out, float* __restrict__ sd) {
+  for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < n;
+       row += (int64_t)gridDim.x * blockDim.x) {
+    const double si = sqrt(fabs(inv_diag[row]));
+    sd[row] = (float)si;
+    for (int64_t p = rp[row]; p < rp[row + 1]; ++p)
+      out[p] = (_Float16)(val[p] * si * sqrt(fabs(inv_diag[col[p]])));
+  }
+}
+
 void kkt_refresh_fp32(KktSystem& s) {
   if (!s.val32.p || (int64_t)s.val32.n != s.A.nnz) s.val32.alloc(s.ctx, s.A.nnz);
   if (s.A.nnz)
@@ -839,6 +875,10 @@ void kkt_refresh_fp32(KktSystem& s) {
   if (s.P.nnz) {
     if (!s.P32.p || (int64_t)s.P32.n != s.P.nnz) s.P32.alloc(s.ctx, s.P.nnz);
     k_f64_to_f32<<<grid_for(s.P.nnz), TPB, 0, s.ctx->stream>>>(s.P.val.p, s.P.nnz, s.P32.p);
+    if (!s.P16.p || (int64_t)s.P16.n != s.P.nnz) s.P16.alloc(s.ctx, s.P.nnz);
+    if (!s.sdinv32.p || (int64_t)s.sdinv32.n != s.n) s.sdinv32.alloc(s.ctx, std::max<int64_t>(s.n, 1));
+    k_scale_half<<<grid_for(s.n), TPB, 0, s.ctx->stream>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p,
+                                                           s.inv_diag.p, s.n, s.P16.p, s.sdinv32.p);
   }
   FEEC_HIP(hipGetLastError());
 }
@@ -1031,7 +1071,10 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
     MinresWork<float>& w = ss.w32;
     minres_alloc(ctx, w, n, nh);
     if (!s.val32.p) kkt_refresh_fp32(s);
-    bind_preconditioner(s, w, s.P32.p);
+    if (o.precond_half && s.P16.p)
+      bind_preconditioner(s, w, s.P32.p, s.P16.p, s.sdinv32.p);
+    else
+      bind_preconditioner(s, w, s.P32.p);
     FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag32.p, N * sizeof(float), hipMemcpyDeviceToDevice, st));
     if (nh) k_f64_to_f32<<<grid_for(nh * n), TPB, 0, st>>>(s.border.p, nh * n, w.border.p);
     // r = b
@@ -1118,7 +1161,7 @@ double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
 
 // Average duration of one Chebyshev step (the preconditioner's fused SpMV)
 // on this system, FP32 or FP64; 0 when the system has no polynomial.
-double time_cheb_step(KktSystem& s, bool fp32, int reps) {
+double time_cheb_step(KktSystem& s, bool fp32, int reps, bool half) {
   if (s.P.nnz == 0 || s.cheb_m < 2 || !s.cheb.p) return 0.0;
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
@@ -1127,7 +1170,7 @@ double time_cheb_step(KktSystem& s, bool fp32, int reps) {
   hipEvent_t e0, e1;
   FEEC_HIP(hipEventCreate(&e0));
   FEEC_HIP(hipEventCreate(&e1));
-  auto run = [&](auto tag, const auto* pval, const auto* dinv) {
+  auto run = [&](auto tag, const auto* pval, const auto* dinv, const _Float16* p16) {
     using T = decltype(tag);
     DBuf<T> d(ctx, n), res(ctx, n), z(ctx, n), dn(ctx, n);
     FEEC_HIP(hipMemsetAsync(d.p, 0, n * sizeof(T), st));
@@ -1135,6 +1178,19 @@ double time_cheb_step(KktSystem& s, bool fp32, int reps) {
     FEEC_HIP(hipMemsetAsync(z.p, 0, n * sizeof(T), st));
     const MinresS* none = nullptr;
     auto step = [&] {
+      if constexpr (std::is_same_v<T, float>) {
+        if (p16) {
+          if (s.Pcol16.p)
+            k_cheb_step<T, int16_t, _Float16, true><<<spmv_grid(n), TPB, 0, st>>>(
+                s.P.row_ptr.p, s.Pcol16.p, p16, n, s.split, dinv, d.p, res.p, z.p, dn.p, s.cheb.p, 0,
+                none);
+          else
+            k_cheb_step<T, int32_t, _Float16, true><<<spmv_grid(n), TPB, 0, st>>>(
+                s.P.row_ptr.p, s.P.col.p, p16, n, s.split, dinv, d.p, res.p, z.p, dn.p, s.cheb.p, 0,
+                none);
+          return;
+        }
+      }
       if (s.Pcol16.p)
         k_cheb_step<T, int16_t><<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.Pcol16.p, pval, n,
                                                              s.split, dinv, d.p, res.p, z.p, dn.p,
@@ -1151,9 +1207,9 @@ double time_cheb_step(KktSystem& s, bool fp32, int reps) {
     FEEC_HIP(hipEventSynchronize(e1));
   };
   if (fp32)
-    run(float{}, s.P32.p, s.inv_diag32.p);
+    run(float{}, s.P32.p, s.inv_diag32.p, half ? s.P16.p : nullptr);
   else
-    run(double{}, s.P.val.p, s.inv_diag.p);
+    run(double{}, s.P.val.p, s.inv_diag.p, nullptr);
   FEEC_HIP(hipGetLastError());
   float ms = 0.f;
   FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -1220,5 +1276,6 @@ KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b
   ks.converged = res <= rtol;
   return ks;
 }
+
 
 }  // namespace feec

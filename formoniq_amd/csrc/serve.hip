@@ -40,6 +40,7 @@ void problem_element_outputs(Problem& P, double** outs);
 Problem* problem_new(Geom* g, int bc, int k, const feec_solve_opts& o);
 void problem_delete(Problem* P);
 feec_report problem_solve_host(Problem* P, const double* src);
+void problem_set_report(Problem* P, const feec_report& r);
 void problem_u(Problem* P, double* u);
 int64_t problem_ndofs(Problem* P);
 }  // namespace feec
@@ -117,10 +118,19 @@ void instance_build(Instance& I, feec_ctx* ctx, const feec_request& r, const fee
   if (assemble) problem_assemble(*I.prob);
 }
 
+void instance_result(Instance& I, const feec_report& rep, feec_result& res,
+                     std::vector<double>* u_out);
+
 void instance_solve(Instance& I, feec_result& res, std::vector<double>* u_out) {
   std::vector<double> src(I.nk);
   check_status(feec_normal_vector(I.nk, I.req.source_seed, src.data()));
   feec_report rep = problem_solve_host(I.prob, src.data());
+  instance_result(I, rep, res, u_out);
+}
+
+void instance_result(Instance& I, const feec_report& rep, feec_result& res,
+                     std::vector<double>* u_out) {
+  problem_set_report(I.prob, rep);
   std::vector<double> u(I.nk);
   problem_u(I.prob, u.data());
   double nrm = 0.0, chk = 0.0;

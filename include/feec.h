@@ -90,7 +90,8 @@ typedef struct feec_solve_opts {
   int32_t graph_iters;   /* iterations per graph segment, even (default 32) */
   int32_t cheb_degree;   /* degree of the Chebyshev-Jacobi polynomial applied to each
                             hdif_gram block of the preconditioner; 0 = Jacobi only (default 32) */
-  int32_t reserved;
+  int32_t precond_half;  /* mixed: run the polynomial on D^-1/2 hdif_gram D^-1/2 stored in FP16
+                            (unit diagonal, entries in [-1, 1]), FP32 arithmetic (default 1) */
 } feec_solve_opts;
 
 typedef struct feec_report {
@@ -205,12 +206,13 @@ feec_status feec_problem_solution(feec_problem *prob, double *sigma, double *u, 
 feec_status feec_problem_sizes(const feec_problem *prob, int64_t sizes[3], int64_t *nnz);
 /* The block preconditioner blockdiag(hdif_gram(k-1), hdif_gram(k)) as built:
  * its nonzeros, the Chebyshev degree in use (0 = Jacobi), the estimated
- * largest eigenvalue of D^-1 hdif_gram per block, and the bytes per stored
- * column index (2 when every |col - row| fits a 16-bit offset, else 4); any
+ * largest eigenvalue of D^-1 hdif_gram per block, the bytes per stored
+ * column index (2 when every |col - row| fits a 16-bit offset, else 4) and
+ * per stored value on the mixed path (2 with precond_half, else 4); any
  * pointer may be NULL. */
 feec_status feec_problem_preconditioner(const feec_problem *prob, int64_t *nnz,
                                         int32_t *cheb_degree, double lambda_max[2],
-                                        int32_t *column_bytes);
+                                        int32_t *column_bytes, int32_t *value_bytes_mixed);
 /* Device pointer to an N_k scratch vector for feec_problem_solve_device. */
 feec_status feec_problem_source_buffer(feec_problem *prob, double **device_ptr);
 /* Copy an N_k host source into that device buffer. */
