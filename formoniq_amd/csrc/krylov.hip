@@ -48,7 +48,8 @@ namespace {
 
 constexpr int TPB = 256;
 constexpr int NBLK = 4096;  // fixed reduction grid: 16 blocks per CU on 256 CUs
-constexpr int LANES = 8;    // lanes per CSR row in the SpMV
+constexpr int LANES = 8;    // lanes per CSR row in the KKT SpMV (about 28 nonzeros a row)
+constexpr int PLANES = 4;   // lanes per row in the preconditioner SpMV (about 16 a row)
 
 // Enqueues kernels on a stream, or appends them as a linear chain of kernel
 // nodes to a graph under construction. Node arguments are kept alive in
@@ -114,7 +115,7 @@ __device__ __forceinline__ double to_d(T v) {
   return (double)v;
 }
 
-// Dot product of CSR row `row` with x by LANES lanes of a wavefront, returned
+// Dot product of CSR row `row` with x by L lanes of a wavefront, returned
 // on every lane. Each lane keeps SPMV_UNROLL independent loads in flight; the
 // summation order is fixed by the row pattern alone.
 constexpr int SPMV_UNROLL = 2;
@@ -127,7 +128,7 @@ __device__ __forceinline__ int64_t col_at(const int16_t* __restrict__ col, int64
   return row + col[p];
 }
 
-template <class T, class I = int32_t, class V = T>
+template <class T, class I = int32_t, class V = T, int L = LANES>
 __device__ __forceinline__ T row_dot(const int64_t* __restrict__ rp, const I* __restrict__ col,
                                      const V* __restrict__ val, const T* __restrict__ x, int64_t row,
                                      int lane) {
@@ -137,25 +138,25 @@ __device__ __forceinline__ T row_dot(const int64_t* __restrict__ rp, const I* __
 #pragma unroll
   for (int u = 0; u < U; ++u) a[u] = 0;
   int64_t p = rp[row] + lane;
-  for (; p + (U - 1) * LANES < end; p += U * LANES) {
+  for (; p + (U - 1) * L < end; p += U * L) {
     int64_t c[U];
     V v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      c[u] = col_at(col, p + u * LANES, row);
-      v[u] = val[p + u * LANES];
+      c[u] = col_at(col, p + u * L, row);
+      v[u] = val[p + u * L];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) a[u] += (T)v[u] * x[c[u]];
   }
 #pragma unroll
   for (int u = 0; u + 1 < U; ++u)
-    if (p + u * LANES < end) a[u] += (T)val[p + u * LANES] * x[col_at(col, p + u * LANES, row)];
+    if (p + u * L < end) a[u] += (T)val[p + u * L] * x[col_at(col, p + u * L, row)];
   T acc = a[0];
 #pragma unroll
   for (int u = 1; u < U; ++u) acc += a[u];
 #pragma unroll
-  for (int off = LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LANES);
+  for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
   return acc;
 }
 
@@ -511,11 +512,11 @@ __global__ void __launch_bounds__(TPB) k_cheb_step(const int64_t* __restrict__ r
   if (S && S->done) return;
   const double* c = coef + 2 + 4 * step;
   const T a0 = (T)c[0], b0 = (T)c[1], a1 = (T)c[2], b1 = (T)c[3];
-  const int lane = threadIdx.x % LANES;
-  const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
-  for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
+  const int lane = threadIdx.x % PLANES;
+  const int64_t groups = (int64_t)gridDim.x * (TPB / PLANES);
+  for (int64_t row = blockIdx.x * (int64_t)(TPB / PLANES) + threadIdx.x / PLANES; row < n;
        row += groups) {
-    T acc = row_dot<T, I, V>(rp, col, val, d, row, lane);
+    T acc = row_dot<T, I, V, PLANES>(rp, col, val, d, row, lane);
     if (lane == 0) {
       T di = d[row];
       T ri = UNIT ? res[row] - acc : res[row] - dinv[row] * acc;
@@ -634,12 +635,13 @@ inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n +
 inline unsigned red_grid(int64_t n) {
   return (unsigned)std::min<int64_t>(NBLK, std::max<int64_t>(1, (n + TPB - 1) / TPB));
 }
-// Blocks of a CSR row kernel (grid-stride over rows): one per TPB/LANES rows,
+// Blocks of a CSR row kernel (grid-stride over rows): one per TPB/lanes rows,
 // at most NBLK.
-inline unsigned spmv_grid(int64_t n) {
-  int64_t rows_per_block = TPB / LANES;
+inline unsigned spmv_grid(int64_t n, int lanes = LANES) {
+  int64_t rows_per_block = TPB / lanes;
   return (unsigned)std::min<int64_t>(NBLK, std::max<int64_t>(1, (n + rows_per_block - 1) / rows_per_block));
 }
+inline unsigned cheb_grid(int64_t n) { return spmv_grid(n, PLANES); }
 
 }  // namespace
 
@@ -749,16 +751,16 @@ static void cheb_apply(Enq& q, MinresWork<T>& w, const T* r, T* z, const MinresS
   T* dn = w.d1.p;
   for (int k = 0; k + 1 < w.cheb_m; ++k) {
     if (scaled && w.pcol16)
-      q(k_cheb_step<T, int16_t, _Float16, true>, spmv_grid(n), TPB, w.prp, w.pcol16, w.pval16, n,
+      q(k_cheb_step<T, int16_t, _Float16, true>, cheb_grid(n), TPB, w.prp, w.pcol16, w.pval16, n,
         w.split, w.dinv.p, d, w.res.p, z, dn, w.coef, k, S);
     else if (scaled)
-      q(k_cheb_step<T, int32_t, _Float16, true>, spmv_grid(n), TPB, w.prp, w.pcol, w.pval16, n,
+      q(k_cheb_step<T, int32_t, _Float16, true>, cheb_grid(n), TPB, w.prp, w.pcol, w.pval16, n,
         w.split, w.dinv.p, d, w.res.p, z, dn, w.coef, k, S);
     else if (w.pcol16)
-      q(k_cheb_step<T, int16_t>, spmv_grid(n), TPB, w.prp, w.pcol16, w.pval, n, w.split, w.dinv.p, d,
+      q(k_cheb_step<T, int16_t>, cheb_grid(n), TPB, w.prp, w.pcol16, w.pval, n, w.split, w.dinv.p, d,
         w.res.p, z, dn, w.coef, k, S);
     else
-      q(k_cheb_step<T, int32_t>, spmv_grid(n), TPB, w.prp, w.pcol, w.pval, n, w.split, w.dinv.p, d,
+      q(k_cheb_step<T, int32_t>, cheb_grid(n), TPB, w.prp, w.pcol, w.pval, n, w.split, w.dinv.p, d,
         w.res.p, z, dn, w.coef, k, S);
     std::swap(d, dn);
   }
@@ -1181,22 +1183,22 @@ double time_cheb_step(KktSystem& s, bool fp32, int reps, bool half) {
       if constexpr (std::is_same_v<T, float>) {
         if (p16) {
           if (s.Pcol16.p)
-            k_cheb_step<T, int16_t, _Float16, true><<<spmv_grid(n), TPB, 0, st>>>(
+            k_cheb_step<T, int16_t, _Float16, true><<<cheb_grid(n), TPB, 0, st>>>(
                 s.P.row_ptr.p, s.Pcol16.p, p16, n, s.split, dinv, d.p, res.p, z.p, dn.p, s.cheb.p, 0,
                 none);
           else
-            k_cheb_step<T, int32_t, _Float16, true><<<spmv_grid(n), TPB, 0, st>>>(
+            k_cheb_step<T, int32_t, _Float16, true><<<cheb_grid(n), TPB, 0, st>>>(
                 s.P.row_ptr.p, s.P.col.p, p16, n, s.split, dinv, d.p, res.p, z.p, dn.p, s.cheb.p, 0,
                 none);
           return;
         }
       }
       if (s.Pcol16.p)
-        k_cheb_step<T, int16_t><<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.Pcol16.p, pval, n,
+        k_cheb_step<T, int16_t><<<cheb_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.Pcol16.p, pval, n,
                                                              s.split, dinv, d.p, res.p, z.p, dn.p,
                                                              s.cheb.p, 0, none);
       else
-        k_cheb_step<T, int32_t><<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, pval, n,
+        k_cheb_step<T, int32_t><<<cheb_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, pval, n,
                                                              s.split, dinv, d.p, res.p, z.p, dn.p,
                                                              s.cheb.p, 0, none);
     };

@@ -99,6 +99,21 @@ def test_c1_configuration_fp64_and_mixed():
         assert rep.precond_residual <= 1e-10
 
 
+def test_mixed_preconditioner_storage_variants_agree():
+    """The mixed path's polynomial on the FP16 scaled preconditioner
+    (precond_half=1, the default) and on the FP32 one (0) changes only the
+    Krylov path: both meet the FP64 certificate and agree with the FP64 solve."""
+    _, _, gcx, geo, _, _ = both(3, 5, 0.05, seed=11)
+    gc = fa.WhitneyComplex(geo)
+    f = fa.normal_vector(gcx.nsimplices(1), 12)
+    _, u64, _, r64 = fa.solve_source(gc, f, 1, fa.solve_opts(precision=fa.FP64))
+    assert r64.residual <= 1e-10
+    for half in (0, 1):
+        _, u, _, rep = fa.solve_source(gc, f, 1, fa.solve_opts(precision=fa.MIXED, precond_half=half))
+        assert rep.residual <= 1e-10, (half, rep.residual)
+        assert np.linalg.norm(u - u64) <= 1e-8 * np.linalg.norm(u64), half
+
+
 def test_resident_problem_reassembly_and_device_source():
     _, _, gcx, geo, _, _ = both(3, 4, 0.05, seed=2)
     gc = fa.WhitneyComplex(geo)
