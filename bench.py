@@ -218,11 +218,16 @@ def run_c5(args):
         srv.close()
         lat = np.array([x.latency for x in res])
         ok = all(x.status == 0 and x.residual <= 1e-10 for x in res)
+        slow = sorted(zip(lat, reqs, res), key=lambda t: -t[0])[:3]
         out["mixed" if prec == fa.MIXED else "fp64"] = {
             "seconds": dt, "solves_per_s": len(res) / dt, "all_converged_1e-10": ok,
             "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
             "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
             "cells": int(sum(x.ncells for x in res)),
+            "slowest": [{"dim": q.dim, "grade": q.grade, "n": q.n, "cells": int(x.ncells),
+                         "dofs": int(x.ndofs), "iters": int(x.iters), "latency_ms": float(l * 1e3),
+                         "service_ms": float((x.finish_time - x.admit_time) * 1e3)}
+                        for l, q, x in slow],
         }
     line = {"config": "C5", "workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, d in 2..3, "
                                         f"k in 0..d-1, cells logU[1e3,2e6], seed 5, {args.slots} slots", **out}
@@ -367,19 +372,30 @@ def main():
     # step (one preconditioner SpMV fused with the polynomial recurrence);
     # the fused MINRES SpMV is reported beside it
     nnz = mixed.nnz
-    p_nnz, cheb_deg, _, p_col_bytes, p_val_bytes = mixed.preconditioner()
+    pc = mixed.preconditioner()
+    p_nnz, cheb_deg, p_col_bytes = pc["nnz"], pc["degree"], pc["column_bytes"]
     spmv_bytes = 8 * nnz + 8 * (n_kkt + 1) + 12 * n_kkt
-    # P values (2 B as scaled FP16, else 4 B) + columns (2 B as band offsets,
-    # else 4 B), row pointers, d (4n), res and z read and written (16n), the
-    # next direction written (4n); dinv (4n) only when unscaled
-    cheb_bytes = ((p_val_bytes + p_col_bytes) * p_nnz + 8 * (n_kkt + 1) + 24 * n_kkt
-                  + (4 * n_kkt if p_val_bytes == 4 else 0))
+    if pc["mixed_format"] == "sell64_f16":
+        # three-term step on the SELL-64 copy: 4 B per off-diagonal nonzero
+        # (FP16 value + int16 offset; padding is not counted), 8 B per 64-row
+        # slice offset, x_{k+1} read (4n), x_k read and overwritten (8n), f read (4n)
+        p_val_bytes = 2
+        cheb_bytes = 4 * pc["offdiag_nnz"] + 8 * ((n_kkt + 63) // 64 + 1) + 16 * n_kkt
+    else:
+        # two-term CSR step: P values (2 B as scaled FP16, else 4 B) + columns
+        # (2 B as band offsets, else 4 B), row pointers, d (4n), res and z read
+        # and written (16n), the next direction written (4n); dinv (4n) only
+        # when unscaled
+        p_val_bytes = 2 if pc["mixed_format"] == "csr_f16" else 4
+        cheb_bytes = ((p_val_bytes + p_col_bytes) * p_nnz + 8 * (n_kkt + 1) + 24 * n_kkt
+                      + (4 * n_kkt if p_val_bytes == 4 else 0))
     achieved = cheb_bytes / times["cheb32"] / 1e9
     traffic = None
     if os.path.exists(args.traffic):
         with open(args.traffic) as fh:
             tr = json.load(fh)
-        if tr.get("n_kkt") == n_kkt:
+        # only a measurement of the same kernel on the same workload applies
+        if tr.get("n_kkt") == n_kkt and tr.get("cheb_step_f32_format") == pc["mixed_format"]:
             traffic = tr.get("cheb_step_f32_hbm_bytes_per_launch")
     # per tet: 6 edge lengths + 4 vertex indices in; M_0, dif_test(1), dif_both(2),
     # two diagonals, M_1 and dif_both(1) out (138 doubles)
@@ -418,12 +434,16 @@ def main():
         "elmat_hbm_gbs": elmat_bytes / times["elmat"] / 1e9,
         "kernel_seconds": times,
         "roofline": {
-            "kernel": "k_cheb_step<float> (Chebyshev step: preconditioner SpMV + recurrence, FP32 arithmetic)",
+            "kernel": ("k_cheb3_step (Chebyshev step on the SELL-64 FP16 preconditioner: SpMV + three-term "
+                       "recurrence, FP32 arithmetic)" if pc["mixed_format"] == "sell64_f16" else
+                       "k_cheb_step<float> (Chebyshev step: preconditioner SpMV + recurrence, FP32 arithmetic)"),
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "algorithmic_bytes_per_launch": cheb_bytes,
             "launch_seconds": times["cheb32"], "precond_nnz": p_nnz, "cheb_degree": cheb_deg,
             "precond_column_bytes": p_col_bytes, "precond_value_bytes": p_val_bytes,
+            "precond_format": pc["mixed_format"], "precond_offdiag_nnz": pc["offdiag_nnz"],
+            "precond_stored_entries": pc["mixed_stored"],
         },
         "mr_spmv_f32": {"launch_seconds": times["spmv32"], "algorithmic_bytes_per_launch": spmv_bytes,
                         "achieved_gbs": spmv_bytes / times["spmv32"] / 1e9},

@@ -63,9 +63,9 @@ class Context:
         self._h = h
         self.device = device
 
-    def close(self):
+    def close(self, _load=load):
         if getattr(self, "_h", None):
-            load().feec_ctx_destroy(self._h)
+            _load().feec_ctx_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -133,17 +133,19 @@ def normal_vector(n, seed):
 class Csr:
     """A device CSR (feec_csr) and its host copy."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, owner):
         self._h = handle
+        self._owner = owner  # keeps the context that owns the buffers alive
         nr, nc, nnz = C.c_int64(), C.c_int64(), C.c_int64()
         check(load().feec_csr_shape(handle, C.byref(nr), C.byref(nc), C.byref(nnz)))
         self.shape = (nr.value, nc.value)
         self.nnz = nnz.value
         self._host = None
 
-    def __del__(self):
+    def __del__(self, _load=load):
+        # _load is bound at definition: module globals are gone at interpreter exit
         if getattr(self, "_h", None):
-            load().feec_csr_destroy(self._h)
+            _load().feec_csr_destroy(self._h)
             self._h = None
 
     def host(self):
@@ -197,9 +199,10 @@ class Complex:
         check(load().feec_mesh_counts(h, _ptr(cnt, _lib.i64p)))
         self.counts = cnt
 
-    def __del__(self):
+    def __del__(self, _load=load):
+        # _load is bound at definition: module globals are gone at interpreter exit
         if getattr(self, "_h", None):
-            load().feec_mesh_destroy(self._h)
+            _load().feec_mesh_destroy(self._h)
             self._h = None
 
     def nsimplices(self, k):
@@ -232,7 +235,7 @@ class Complex:
     def coboundary(self, k, bc=FULL):
         h = C.c_void_p()
         check(load().feec_mesh_coboundary(self._h, bc, k, C.byref(h)))
-        return Csr(h)
+        return Csr(h, self)
 
 
 class MeshLengthsSq:
@@ -257,9 +260,10 @@ class MeshLengthsSq:
                                            MINKOWSKI if minkowski else EUCLIDEAN, C.byref(h)))
         return MeshLengthsSq(complex_, h)
 
-    def __del__(self):
+    def __del__(self, _load=load):
+        # _load is bound at definition: module globals are gone at interpreter exit
         if getattr(self, "_h", None):
-            load().feec_geom_destroy(self._h)
+            _load().feec_geom_destroy(self._h)
             self._h = None
 
     def lengths(self):
@@ -320,7 +324,7 @@ class WhitneyComplex:
     def assemble(self, kind, grade) -> Csr:
         h = C.c_void_p()
         check(load().feec_assemble(self.geometry._h, self.bc, kind, grade, C.byref(h)))
-        return Csr(h)
+        return Csr(h, self)
 
     def mass(self, k):
         return self.assemble(MASS, k)
@@ -353,7 +357,7 @@ class WhitneyComplex:
         h = C.c_void_p()
         sizes = np.zeros(3, np.int64)
         check(load().feec_mixed_kkt(self.geometry._h, self.bc, k, C.byref(h), _ptr(sizes, _lib.i64p)))
-        return Csr(h), tuple(int(s) for s in sizes)
+        return Csr(h, self), tuple(int(s) for s in sizes)
 
 
 def solve_source(complex_: WhitneyComplex, source_galvec, grade, opts: SolveOpts | None = None,
@@ -392,9 +396,10 @@ class Problem:
         self.sizes = tuple(int(s) for s in sizes)
         self.nnz = nnz.value
 
-    def __del__(self):
+    def __del__(self, _load=load):
+        # _load is bound at definition: module globals are gone at interpreter exit
         if getattr(self, "_h", None):
-            load().feec_problem_destroy(self._h)
+            _load().feec_problem_destroy(self._h)
             self._h = None
 
     def assemble(self):
@@ -408,13 +413,34 @@ class Problem:
         return dict(elmat=float(out[0]), scatter=float(out[1]), spmv32=float(out[2]),
                     spmv64=float(out[3]), cheb32=float(out[4]), cheb64=float(out[5]))
 
+    PC_FORMATS = {0: "csr_f32", 1: "csr_f16", 2: "sell64_f16"}
+
+    def apply_preconditioner(self, r, precision=None, precond_half=None):
+        """z = B r (KKT order: sigma, u, harmonic rows) with the block preconditioner
+        as a solve of `precision` applies it; `precond_half` picks the mixed path's
+        storage (defaults: this problem's options)."""
+        prec = self.opts.precision if precision is None else precision
+        half = self.opts.precond_half if precond_half is None else precond_half
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        assert r.shape == (sum(self.sizes),)
+        z = np.zeros_like(r)
+        check(load().feec_problem_apply_preconditioner(self._h, prec, half, _ptr(r, _lib.f64p),
+                                                       _ptr(z, _lib.f64p)))
+        return z
+
     def preconditioner(self):
-        """(nonzeros, Chebyshev degree, lambda_max per block, bytes per stored column
-        index, bytes per stored value on the mixed path) of the block preconditioner."""
-        nnz, deg, lam, cb, vb = C.c_int64(), C.c_int32(), np.zeros(2), C.c_int32(), C.c_int32()
+        """The block preconditioner as built: nonzeros, Chebyshev degree, lambda_max
+        per block, bytes per stored CSR column index, the mixed path's storage
+        format, the entries it stores (SELL padding included) and the
+        off-diagonal nonzeros."""
+        nnz, deg, lam, cb = C.c_int64(), C.c_int32(), np.zeros(2), C.c_int32()
+        fmt, stored, offd = C.c_int32(), C.c_int64(), C.c_int64()
         check(load().feec_problem_preconditioner(self._h, C.byref(nnz), C.byref(deg),
-                                                 _ptr(lam, _lib.f64p), C.byref(cb), C.byref(vb)))
-        return nnz.value, deg.value, lam, cb.value, vb.value
+                                                 _ptr(lam, _lib.f64p), C.byref(cb), C.byref(fmt),
+                                                 C.byref(stored), C.byref(offd)))
+        return dict(nnz=nnz.value, degree=deg.value, lambda_max=lam, column_bytes=cb.value,
+                    mixed_format=self.PC_FORMATS[fmt.value], mixed_stored=stored.value,
+                    offdiag_nnz=offd.value)
 
     def upload_source(self, source):
         """Make `source` (N_k) resident on the device for solve()."""
@@ -458,7 +484,7 @@ class HodgeDirac:
         offs = np.zeros(complex_.dim + 2, np.int64)
         check(load().feec_hodge_dirac_assemble(complex_.geometry._h, complex_.bc, sign, C.byref(mb),
                                                C.byref(op), _ptr(offs, _lib.i64p)))
-        return HodgeDirac(offs, Csr(mb), Csr(op))
+        return HodgeDirac(offs, Csr(mb, complex_), Csr(op, complex_))
 
     @staticmethod
     def assemble(complex_):
@@ -525,9 +551,9 @@ class Server:
     def drain(self):
         check(load().feec_server_drain(self._h))
 
-    def close(self):
+    def close(self, _load=load):
         if getattr(self, "_h", None):
-            load().feec_server_destroy(self._h)
+            _load().feec_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -24,7 +24,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("max_iters", C.c_int64), ("precision", C.c_int32),
                 ("max_refinements", C.c_int32), ("inner_rtol", C.c_double),
                 ("use_graph", C.c_int32), ("graph_iters", C.c_int32),
-                ("cheb_degree", C.c_int32), ("precond_half", C.c_int32)]
+                ("cheb_degree", C.c_int32), ("cheb_lower", C.c_double), ("precond_half", C.c_int32)]
 
 
 class Report(C.Structure):
@@ -110,8 +110,9 @@ SIGNATURES = {
     "feec_problem_sizes": (S, [vp, i64p, i64p]),
     "feec_problem_source_buffer": (S, [vp, C.POINTER(vp)]),
     "feec_problem_preconditioner": (S, [vp, i64p, C.POINTER(C.c_int32), f64p, C.POINTER(C.c_int32),
-                                        C.POINTER(C.c_int32)]),
+                                        C.POINTER(C.c_int32), i64p, i64p]),
     "feec_problem_kernel_times": (S, [vp, C.c_int, f64p]),
+    "feec_problem_apply_preconditioner": (S, [vp, C.c_int32, C.c_int32, f64p, f64p]),
     "feec_problem_upload_source": (S, [vp, f64p]),
     "feec_dirac_element_time": (S, [vp, C.c_int, f64p, i64p]),
     "feec_batch_run": (S, [vp, C.c_int, C.POINTER(Request), C.POINTER(SolveOpts), C.POINTER(Result)]),

@@ -281,8 +281,20 @@ struct KktSystem {
   DBuf<int16_t> Pcol16;  // P's columns as col - row, when every |col - row| <= 32767
   bool band_checked = false;
   DBuf<float> P32;
-  DBuf<_Float16> P16;    // D^-1/2 P D^-1/2 in FP16 (the mixed path's polynomial)
+  DBuf<_Float16> P16;    // D^-1/2 P D^-1/2 in FP16, CSR (when the SELL copy cannot be made)
   DBuf<float> sdinv32;   // D^-1/2, n entries
+  // SELL-64 copy of D^-1/2 P D^-1/2 for the mixed path (made when the band fits
+  // int16): rows in slices of 64, one wavefront lane per row; a slice's entries
+  // are stored column-major (entry j of its 64 rows side by side), each packed
+  // as {FP16 value | int16 (col' - row') << 16}, rows zero-padded to the slice's
+  // longest; only off-diagonal entries, the unit diagonal is implied. Rows and
+  // columns are renumbered (row' = sell_inv[row]) by sorting the rows of each
+  // window of 256 (windows restart at `split`) by off-diagonal count, longest
+  // first, so that a slice holds rows of equal length; sell_perm[row'] = row.
+  DBuf<uint32_t> Psell;
+  DBuf<int64_t> Psell_ptr;  // n_slices + 1 offsets into Psell (multiples of 64)
+  DBuf<int32_t> sell_perm, sell_inv;
+  int64_t sell_entries = 0, P_offdiag = 0;
   int64_t split = 0;
   int cheb_m = 0;
   DBuf<double> cheb;
@@ -290,7 +302,7 @@ struct KktSystem {
 };
 // Estimate the spectra of D^-1 P per block and set degree-m Chebyshev
 // coefficients (m = 0 disables the polynomial).
-void kkt_set_chebyshev(KktSystem& s, int m);
+void kkt_set_chebyshev(KktSystem& s, int m, double lower = 0.0);
 void kkt_refresh_fp32(KktSystem& s);
 // Solve s x = b (b, x device, FP64) to the options' tolerance.
 feec_report kkt_solve(KktSystem& s, const double* b_dev, double* x_dev, const feec_solve_opts& o,
@@ -298,7 +310,13 @@ feec_report kkt_solve(KktSystem& s, const double* b_dev, double* x_dev, const fe
 void solver_state_free(SolverState* ws);
 void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y);
 double time_minres_spmv(KktSystem& s, bool fp32, int reps);
-double time_cheb_step(KktSystem& s, bool fp32, int reps, bool half);
+// z = B r with the block preconditioner as a solve of `precision` applies it
+// (half: the mixed path's storage, as feec_solve_opts.precond_half); device
+// vectors of n + nh doubles.
+void kkt_apply_preconditioner(KktSystem& s, int precision, int half, const double* r, double* z);
+// half: 0 FP32 CSR, 1 the mixed path's default storage (SELL-64 when made),
+// 2 FP16 CSR
+double time_cheb_step(KktSystem& s, bool fp32, int reps, int half);
 KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
                      double rtol, int64_t max_iters);
 

@@ -551,6 +551,119 @@ __global__ void __launch_bounds__(TPB) k_cheb_finish(const T* __restrict__ r, co
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// ---- the mixed path's polynomial on the SELL-64 copy of D^-1/2 P D^-1/2 ----
+// Three-term form of the same Chebyshev iteration (x_0 = 0, x_1 = f / theta):
+//   x_{k+2} = x_{k+1} + a_k (x_{k+1} - x_k) + b_k (f - A x_{k+1}),
+// with d_{k+1} = x_{k+2} - x_{k+1} and r_{k+1} = f - A x_{k+1} it is the
+// two-term step above, so (a_k, b_k) are the same per-block coefficients. Per
+// row it reads x_{k+1} (gathered), x_k and f and writes x_{k+2} over x_k.
+
+// Packed SELL entry: FP16 value in the low half, int16 (col - row) in the high.
+__device__ __forceinline__ float sell_val(uint32_t e) {
+  const uint16_t b = (uint16_t)(e & 0xffffu);
+  _Float16 h;
+  __builtin_memcpy(&h, &b, 2);
+  return (float)h;
+}
+__device__ __forceinline__ int64_t sell_off(uint32_t e) { return (int64_t)(int16_t)(e >> 16); }
+
+constexpr int SELL_UNROLL = 8;
+constexpr int SELL_WINDOW = 256;  // rows sorted by length inside windows of this many
+
+// Off-diagonal part of row `row` (lane row % 64 of its slice) times x; the
+// summation order is fixed by the stored pattern.
+__device__ __forceinline__ float sell_row(const uint32_t* __restrict__ E,
+                                          const int64_t* __restrict__ sp, int64_t row,
+                                          const float* __restrict__ x) {
+  constexpr int U = SELL_UNROLL;
+  const int64_t s = row >> 6;
+  const int64_t base = sp[s];
+  const int w = (int)((sp[s + 1] - base) >> 6);
+  const uint32_t* p = E + base + (row & 63);
+  float a[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) a[u] = 0.f;
+  int j = 0;
+  for (; j + U <= w; j += U) {
+    uint32_t e[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) e[u] = p[(int64_t)(j + u) * 64];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] += sell_val(e[u]) * x[row + sell_off(e[u])];
+  }
+  for (; j < w; ++j) {
+    const uint32_t e = p[(int64_t)j * 64];
+    a[0] += sell_val(e) * x[row + sell_off(e)];
+  }
+  float acc = a[0];
+#pragma unroll
+  for (int u = 1; u < U; ++u) acc += a[u];
+  return acc;
+}
+
+// f = sd r, x_0 = 0, x_1 = f / theta_block in the SELL row order (f[i] from
+// row perm[i]; the renumbering keeps both blocks in place, so i < split tells
+// the block)
+__global__ void __launch_bounds__(TPB) k_cheb3_init(const float* __restrict__ r,
+                                                    const float* __restrict__ sd,
+                                                    const int32_t* __restrict__ perm,
+                                                    float* __restrict__ f, float* __restrict__ x0,
+                                                    float* __restrict__ x1, int64_t n, int64_t split,
+                                                    const double* __restrict__ coef,
+                                                    const MinresS* __restrict__ S) {
+  if (S && S->done) return;
+  const float s0 = (float)coef[0], s1 = (float)coef[1];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = perm[i];
+    const float fi = r[j] * sd[j];
+    f[i] = fi;
+    x0[i] = 0.f;
+    x1[i] = fi * (i < split ? s0 : s1);
+  }
+}
+
+// One lane per row, one block per four 64-row slices: x0 <- x_{k+2}.
+__global__ void __launch_bounds__(TPB) k_cheb3_step(const uint32_t* __restrict__ E,
+                                                    const int64_t* __restrict__ sp, int64_t n,
+                                                    int64_t split, const float* __restrict__ f,
+                                                    const float* __restrict__ x1,
+                                                    float* __restrict__ x0,
+                                                    const double* __restrict__ coef, int step,
+                                                    const MinresS* __restrict__ S) {
+  if (S && S->done) return;
+  const int64_t row = blockIdx.x * (int64_t)TPB + threadIdx.x;
+  if (row >= n) return;
+  const double* c = coef + 2 + 4 * step + (row < split ? 0 : 2);
+  const float a = (float)c[0], b = (float)c[1];
+  const float acc = sell_row(E, sp, row, x1);
+  const float xi = x1[row];
+  x0[row] = xi + a * (xi - x0[row]) + b * (f[row] - (xi + acc));
+}
+
+// z = sd x_m on rows < n (x in the SELL row order) and z = r on the harmonic
+// rows; partial dot r.z
+__global__ void __launch_bounds__(TPB) k_cheb3_finish(const float* __restrict__ r,
+                                                      const float* __restrict__ x,
+                                                      const float* __restrict__ sd,
+                                                      const int32_t* __restrict__ inv,
+                                                      float* __restrict__ z, int64_t n, int64_t N,
+                                                      const MinresS* __restrict__ S,
+                                                      double* part) {
+  if (S && S->done) return;
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float ri = r[i];
+    const float zi = i < n ? sd[i] * x[inv[i]] : ri;
+    z[i] = zi;
+    v += (double)ri * (double)zi;
+  }
+  double t = block_reduce(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 template <class T>
 __global__ void k_mr_zero(const T* __restrict__ b, int64_t n, T* r1, T* r2, T* x, T* w, T* w2) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -622,6 +735,10 @@ __global__ void k_jacobi(const double* dd, const double* ii, double* oo, int64_t
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < nn) oo[i] = dd ? ii[i] * dd[i] : ii[i];
 }
+__global__ void k_jacobi32(const float* r, const float* dinv, float* z, int64_t nn) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < nn) z[i] = dinv[i] * r[i];
+}
 __global__ void k_to_scale(const double* a, int64_t n, double s, double* out) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) out[i] = s * a[i];
@@ -642,6 +759,8 @@ inline unsigned spmv_grid(int64_t n, int lanes = LANES) {
   return (unsigned)std::min<int64_t>(NBLK, std::max<int64_t>(1, (n + rows_per_block - 1) / rows_per_block));
 }
 inline unsigned cheb_grid(int64_t n) { return spmv_grid(n, PLANES); }
+// One lane per row, no grid stride: every block covers four whole 64-row slices.
+inline unsigned sell_grid(int64_t n) { return grid_for(n); }
 
 }  // namespace
 
@@ -666,6 +785,10 @@ struct MinresWork {
   const int16_t* pcol16 = nullptr;  // band-offset columns of P, when they fit
   const _Float16* pval16 = nullptr;  // scaled P in FP16 (FP32 path), or null
   const T* sd = nullptr;             // D^-1/2 for the scaled polynomial
+  const uint32_t* psell = nullptr;   // SELL-64 scaled P (FP32 path), or null
+  const int64_t* psell_ptr = nullptr;
+  const int32_t* sell_perm = nullptr;
+  const int32_t* sell_inv = nullptr;
   const T* pval = nullptr;
   const double* coef = nullptr;
   ~MinresWork() {
@@ -717,11 +840,17 @@ static void minres_alloc(Ctx* ctx, MinresWork<T>& w, int64_t n, int64_t nh) {
 // precision T); a captured graph holding other pointers is dropped.
 template <class T>
 static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval,
-                                const _Float16* pval16 = nullptr, const T* sd = nullptr) {
+                                const _Float16* pval16 = nullptr, const T* sd = nullptr,
+                                bool sell = false) {
   int m = (s.P.nnz > 0 && s.cheb_m > 0 && s.cheb.p) ? s.cheb_m : 0;
+  const uint32_t* psell = sell ? s.Psell.p : nullptr;
+  const int64_t* psell_ptr = sell ? s.Psell_ptr.p : nullptr;
+  const int32_t* sell_perm = sell ? s.sell_perm.p : nullptr;
+  const int32_t* sell_inv = sell ? s.sell_inv.p : nullptr;
   if (w.graph && (w.pval != pval || w.coef != s.cheb.p || w.prp != s.P.row_ptr.p ||
                   w.pcol != s.P.col.p || w.pcol16 != s.Pcol16.p || w.pval16 != pval16 ||
-                  w.sd != sd || w.cheb_m != m)) {
+                  w.sd != sd || w.cheb_m != m || w.psell != psell || w.psell_ptr != psell_ptr ||
+                  w.sell_perm != sell_perm || w.sell_inv != sell_inv)) {
     (void)hipGraphExecDestroy(w.graph);
     w.graph = nullptr;
   }
@@ -733,6 +862,10 @@ static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval,
   w.pval = pval;
   w.pval16 = pval16;
   w.sd = sd;
+  w.psell = psell;
+  w.psell_ptr = psell_ptr;
+  w.sell_perm = sell_perm;
+  w.sell_inv = sell_inv;
   w.coef = s.cheb.p;
 }
 
@@ -742,6 +875,22 @@ template <class T>
 static void cheb_apply(Enq& q, MinresWork<T>& w, const T* r, T* z, const MinresS* S,
                        double* part) {
   const int64_t n = w.n, N = w.n + w.nh;
+  if constexpr (std::is_same_v<T, float>) {
+    if (w.psell && w.sd) {
+      // three-term polynomial on the SELL-64 copy; x_{k+2} overwrites x_k
+      q(k_cheb3_init, red_grid(n), TPB, r, w.sd, w.sell_perm, w.res.p, w.d0.p, w.d1.p, n, w.split,
+        w.coef, S);
+      float* cur = w.d1.p;
+      float* prev = w.d0.p;
+      for (int k = 0; k + 1 < w.cheb_m; ++k) {
+        q(k_cheb3_step, sell_grid(n), TPB, w.psell, w.psell_ptr, n, w.split, (const float*)w.res.p,
+          (const float*)cur, prev, w.coef, k, S);
+        std::swap(cur, prev);
+      }
+      q(k_cheb3_finish, red_grid(N), TPB, r, (const float*)cur, w.sd, w.sell_inv, z, n, N, S, part);
+      return;
+    }
+  }
   // scaled mode: the polynomial runs on D^-1/2 P D^-1/2 (FP16 values) with
   // input and output scaled by D^-1/2; otherwise on D^-1 P
   const bool scaled = w.pval16 && w.sd;
@@ -867,6 +1016,122 @@ __global__ void k_scale_half(const int64_t* __restrict__ rp, const int32_t* __re
   }
 }
 
+// Off-diagonal entries per row.
+__global__ void k_row_offdiag(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                              int64_t n, int32_t* __restrict__ cnt) {
+  for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < n;
+       row += (int64_t)gridDim.x * blockDim.x) {
+    int32_t c = 0;
+    for (int64_t p = rp[row]; p < rp[row + 1]; ++p) c += col[p] != row;
+    cnt[row] = c;
+  }
+}
+
+// One block per window of SELL_WINDOW rows (windows restart at `split`): the
+// window's rows sorted by off-diagonal count, longest first, ties by row.
+__global__ void __launch_bounds__(SELL_WINDOW) k_sell_rank(const int32_t* __restrict__ cnt,
+                                                           int64_t n, int64_t split,
+                                                           int32_t* __restrict__ perm,
+                                                           int32_t* __restrict__ inv) {
+  __shared__ int32_t sc[SELL_WINDOW];
+  const int64_t nw0 = (split + SELL_WINDOW - 1) / SELL_WINDOW;
+  const int64_t b = blockIdx.x;
+  const int64_t lo = b < nw0 ? b * SELL_WINDOW : split + (b - nw0) * SELL_WINDOW;
+  const int64_t hi = b < nw0 ? (lo + SELL_WINDOW < split ? lo + SELL_WINDOW : split)
+                             : (lo + SELL_WINDOW < n ? lo + SELL_WINDOW : n);
+  const int t = threadIdx.x;
+  const int m = (int)(hi - lo);
+  const int32_t c = t < m ? cnt[lo + t] : 0;
+  sc[t] = c;
+  __syncthreads();
+  if (t >= m) return;
+  int rank = 0;
+  for (int u = 0; u < m; ++u) rank += (sc[u] > c) || (sc[u] == c && u < t);
+  perm[lo + rank] = (int32_t)(lo + t);
+  inv[lo + t] = (int32_t)(lo + rank);
+}
+
+// Longest row of each 64-row slice in the SELL order, and the sum of all
+// counts into *total.
+__global__ void k_sell_width(const int32_t* __restrict__ cnt, const int32_t* __restrict__ perm,
+                             int64_t n, int64_t nslices, int64_t* __restrict__ width,
+                             unsigned long long* total) {
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= nslices) return;
+  int64_t w = 0, sum = 0;
+  const int64_t r1 = (s + 1) * 64 < n ? (s + 1) * 64 : n;
+  for (int64_t i = s * 64; i < r1; ++i) {
+    const int64_t c = cnt[perm[i]];
+    w = c > w ? c : w;
+    sum += c;
+  }
+  width[s] = w;
+  if (sum) atomicAdd(total, (unsigned long long)sum);
+}
+
+// Largest |inv[col] - inv[row]| over the off-diagonal entries.
+__global__ void k_sell_band(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const int32_t* __restrict__ inv, int64_t n,
+                            unsigned long long* maxdist) {
+  unsigned long long md = 0;
+  for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < n;
+       row += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ir = inv[row];
+    for (int64_t p = rp[row]; p < rp[row + 1]; ++p) {
+      const int64_t dlt = (int64_t)inv[col[p]] - ir;
+      const unsigned long long a = (unsigned long long)(dlt < 0 ? -dlt : dlt);
+      md = a > md ? a : md;
+    }
+  }
+  if (md) atomicMax(maxdist, md);
+}
+
+// Scaled off-diagonal values of row perm[i] into lane i of the SELL copy, in
+// P's column order, with zero entries (value 0, offset 0) after the last one.
+__global__ void k_sell_fill(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const double* __restrict__ val, const double* __restrict__ inv_diag,
+                            const int64_t* __restrict__ sp, const int32_t* __restrict__ perm,
+                            const int32_t* __restrict__ inv, int64_t n, int64_t nrows_padded,
+                            uint32_t* __restrict__ E) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrows_padded;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = i >> 6, base = sp[s] + (i & 63);
+    const int64_t w = (sp[s + 1] - sp[s]) >> 6;
+    int64_t j = 0;
+    if (i < n) {
+      const int64_t row = perm[i];
+      const double si = sqrt(fabs(inv_diag[row]));
+      for (int64_t p = rp[row]; p < rp[row + 1]; ++p) {
+        const int64_t c = col[p];
+        if (c == row) continue;
+        const _Float16 h = (_Float16)(val[p] * si * sqrt(fabs(inv_diag[c])));
+        uint16_t hb;
+        __builtin_memcpy(&hb, &h, 2);
+        const uint16_t ob = (uint16_t)(int16_t)(inv[c] - i);
+        E[base + j * 64] = (uint32_t)hb | ((uint32_t)ob << 16);
+        ++j;
+      }
+    }
+    for (; j < w; ++j) E[base + j * 64] = 0u;
+  }
+}
+
+// D^-1/2 in FP32
+__global__ void k_scale_sd(const double* __restrict__ inv_diag, int64_t n, float* __restrict__ sd) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    sd[i] = (float)sqrt(fabs(inv_diag[i]));
+}
+
+static void ensure_band(KktSystem& s);
+
+// FP16 CSR copy of D^-1/2 P D^-1/2 (and D^-1/2) from P's FP64 values.
+static void kkt_refresh_half_csr(KktSystem& s) {
+  if (!s.P16.p || (int64_t)s.P16.n != s.P.nnz) s.P16.alloc(s.ctx, s.P.nnz);
+  k_scale_half<<<grid_for(s.n), TPB, 0, s.ctx->stream>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p,
+                                                         s.inv_diag.p, s.n, s.P16.p, s.sdinv32.p);
+}
+
 void kkt_refresh_fp32(KktSystem& s) {
   if (!s.val32.p || (int64_t)s.val32.n != s.A.nnz) s.val32.alloc(s.ctx, s.A.nnz);
   if (s.A.nnz)
@@ -875,12 +1140,23 @@ void kkt_refresh_fp32(KktSystem& s) {
   if (!s.inv_diag32.p || (int64_t)s.inv_diag32.n != N) s.inv_diag32.alloc(s.ctx, N);
   k_f64_to_f32<<<grid_for(N), TPB, 0, s.ctx->stream>>>(s.inv_diag.p, N, s.inv_diag32.p);
   if (s.P.nnz) {
+    hipStream_t st = s.ctx->stream;
+    ensure_band(s);
     if (!s.P32.p || (int64_t)s.P32.n != s.P.nnz) s.P32.alloc(s.ctx, s.P.nnz);
-    k_f64_to_f32<<<grid_for(s.P.nnz), TPB, 0, s.ctx->stream>>>(s.P.val.p, s.P.nnz, s.P32.p);
-    if (!s.P16.p || (int64_t)s.P16.n != s.P.nnz) s.P16.alloc(s.ctx, s.P.nnz);
+    k_f64_to_f32<<<grid_for(s.P.nnz), TPB, 0, st>>>(s.P.val.p, s.P.nnz, s.P32.p);
     if (!s.sdinv32.p || (int64_t)s.sdinv32.n != s.n) s.sdinv32.alloc(s.ctx, std::max<int64_t>(s.n, 1));
-    k_scale_half<<<grid_for(s.n), TPB, 0, s.ctx->stream>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p,
-                                                           s.inv_diag.p, s.n, s.P16.p, s.sdinv32.p);
+    k_scale_sd<<<grid_for(s.n), TPB, 0, st>>>(s.inv_diag.p, s.n, s.sdinv32.p);
+    // the FP16 CSR copy is the fallback when there is no SELL copy, or kept
+    // up to date once a solve has asked for it
+    if (!s.Psell_ptr.p || s.P16.p) kkt_refresh_half_csr(s);
+    if (s.Psell_ptr.p) {
+      if (!s.Psell.p || (int64_t)s.Psell.n != s.sell_entries)
+        s.Psell.alloc(s.ctx, std::max<int64_t>(s.sell_entries, 1));
+      const int64_t padded = (s.n + 63) / 64 * 64;
+      k_sell_fill<<<grid_for(padded), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p,
+                                                    s.inv_diag.p, s.Psell_ptr.p, s.sell_perm.p,
+                                                    s.sell_inv.p, s.n, padded, s.Psell.p);
+    }
   }
   FEEC_HIP(hipGetLastError());
 }
@@ -903,21 +1179,67 @@ __global__ void k_band16(const int64_t* __restrict__ rp, const int32_t* __restri
   if (md) atomicMax(maxdist, md);
 }
 
-void kkt_set_chebyshev(KktSystem& s, int m) {
+// Pattern-level set-up of P, once: int16 band-offset columns when every
+// |col - row| <= 32767, and then the SELL-64 row order and slice offsets when
+// its band fits int16 too.
+static void ensure_band(KktSystem& s) {
+  if (s.band_checked || s.P.nnz == 0) return;
+  Ctx* ctx = s.ctx;
+  hipStream_t st = ctx->stream;
+  s.band_checked = true;
+  s.Pcol16.alloc(ctx, s.P.nnz);
+  FEEC_HIP(hipMemsetAsync(ctx->dcount, 0, sizeof(unsigned long long), st));
+  k_band16<<<grid_for(s.n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.n, s.Pcol16.p, ctx->dcount);
+  unsigned long long md = 0;
+  FEEC_HIP(hipMemcpyAsync(&md, ctx->dcount, sizeof(md), hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  if (md > 32767) {
+    s.Pcol16.reset();
+    return;
+  }
+  // SELL row order: rows sorted by length inside windows, then its band
+  const int64_t n = s.n, ns = (n + 63) / 64;
+  DBuf<int32_t> cnt(ctx, n);
+  s.sell_perm.alloc(ctx, n);
+  s.sell_inv.alloc(ctx, n);
+  k_row_offdiag<<<grid_for(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, n, cnt.p);
+  const int64_t nwin = (s.split + SELL_WINDOW - 1) / SELL_WINDOW +
+                       (n - s.split + SELL_WINDOW - 1) / SELL_WINDOW;
+  if (nwin > 0)
+    k_sell_rank<<<(unsigned)nwin, SELL_WINDOW, 0, st>>>(cnt.p, n, s.split, s.sell_perm.p,
+                                                        s.sell_inv.p);
+  FEEC_HIP(hipMemsetAsync(ctx->dcount, 0, sizeof(unsigned long long), st));
+  k_sell_band<<<grid_for(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.sell_inv.p, n, ctx->dcount);
+  FEEC_HIP(hipMemcpyAsync(&md, ctx->dcount, sizeof(md), hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  if (md > 32767) {
+    s.sell_perm.reset();
+    s.sell_inv.reset();
+    return;
+  }
+  DBuf<int64_t> width(ctx, ns);
+  FEEC_HIP(hipMemsetAsync(ctx->dcount, 0, sizeof(unsigned long long), st));
+  k_sell_width<<<grid_for(ns), TPB, 0, st>>>(cnt.p, s.sell_perm.p, n, ns, width.p, ctx->dcount);
+  std::vector<int64_t> w(ns), ptr(ns + 1, 0);
+  unsigned long long offdiag = 0;
+  FEEC_HIP(hipMemcpyAsync(w.data(), width.p, ns * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  FEEC_HIP(hipMemcpyAsync(&offdiag, ctx->dcount, sizeof(offdiag), hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  for (int64_t i = 0; i < ns; ++i) ptr[i + 1] = ptr[i] + 64 * w[i];
+  s.Psell_ptr.alloc(ctx, ns + 1);
+  FEEC_HIP(hipMemcpyAsync(s.Psell_ptr.p, ptr.data(), (ns + 1) * sizeof(int64_t),
+                          hipMemcpyHostToDevice, st));
+  ctx->sync();
+  s.sell_entries = ptr[ns];
+  s.P_offdiag = (int64_t)offdiag;
+}
+
+void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
   s.cheb_m = (s.P.nnz > 0) ? std::max(m, 0) : 0;
   if (s.cheb_m == 0) return;
-  if (!s.band_checked) {
-    s.band_checked = true;
-    s.Pcol16.alloc(ctx, s.P.nnz);
-    FEEC_HIP(hipMemsetAsync(ctx->dcount, 0, sizeof(unsigned long long), st));
-    k_band16<<<grid_for(s.n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.n, s.Pcol16.p, ctx->dcount);
-    unsigned long long md = 0;
-    FEEC_HIP(hipMemcpyAsync(&md, ctx->dcount, sizeof(md), hipMemcpyDeviceToHost, st));
-    ctx->sync();
-    if (md > 32767) s.Pcol16.reset();
-  }
+  ensure_band(s);
   const int64_t n = s.n;
   DBuf<double> x(ctx, n), y(ctx, n), part(ctx, NBLK + 1);
   std::vector<double> hx(n);
@@ -954,7 +1276,8 @@ void kkt_set_chebyshev(KktSystem& s, int m) {
     }
   }
   // interval lower ends (fraction of b) tuned on 3D k=1 Kuhn meshes
-  const double frac = m <= 4 ? 1e-2 : m <= 8 ? 3e-3 : m <= 12 ? 1.5e-3 : 1e-3;
+  const double frac =
+      lower > 0.0 ? lower : m <= 4 ? 1e-2 : m <= 8 ? 3e-3 : m <= 12 ? 1.5e-3 : m <= 32 ? 1e-3 : 5e-4;
   std::vector<double> coef(2 + 4 * std::max(m - 1, 0), 0.0);
   double rho[2], sigma[2], delta[2];
   for (int b = 0; b < 2; ++b) {
@@ -1073,7 +1396,12 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
     MinresWork<float>& w = ss.w32;
     minres_alloc(ctx, w, n, nh);
     if (!s.val32.p) kkt_refresh_fp32(s);
-    if (o.precond_half && s.P16.p)
+    // precond_half: 1 = FP16 SELL-64 copy (FP16 CSR when there is none),
+    // 2 = FP16 CSR, 0 = FP32 CSR
+    if (o.precond_half == 2 && s.P.nnz && !s.P16.p) kkt_refresh_half_csr(s);
+    if (o.precond_half == 1 && s.Psell.p)
+      bind_preconditioner(s, w, s.P32.p, nullptr, s.sdinv32.p, true);
+    else if (o.precond_half && s.P16.p)
       bind_preconditioner(s, w, s.P32.p, s.P16.p, s.sdinv32.p);
     else
       bind_preconditioner(s, w, s.P32.p);
@@ -1161,14 +1489,62 @@ double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
   return ms * 1e-3 / std::max(reps, 1);
 }
 
+// z = B r, B the block preconditioner exactly as a solve of the given
+// precision applies it (`half` selects the mixed path's storage as
+// feec_solve_opts.precond_half does); r and z are N device doubles.
+void kkt_apply_preconditioner(KktSystem& s, int precision, int half, const double* r,
+                              double* z) {
+  Ctx* ctx = s.ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n, N = s.n + s.nh;
+  if (precision == FEEC_PREC_FP64) {
+    MinresWork<double> w;
+    minres_alloc(ctx, w, n, s.nh);
+    bind_preconditioner(s, w, s.P.val.p);
+    FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    FEEC_HIP(hipMemcpyAsync(w.b.p, r, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    Enq q{st};
+    if (w.cheb_m > 0)
+      cheb_apply<double>(q, w, w.b.p, w.y.p, nullptr, w.part.p);
+    else
+      k_jacobi<<<grid_for(N), TPB, 0, st>>>(w.b.p, w.dinv.p, w.y.p, N);
+    FEEC_HIP(hipMemcpyAsync(z, w.y.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    ctx->sync();
+    return;
+  }
+  if (!s.val32.p) kkt_refresh_fp32(s);
+  if (half == 2 && s.P.nnz && !s.P16.p) kkt_refresh_half_csr(s);
+  MinresWork<float> w;
+  minres_alloc(ctx, w, n, s.nh);
+  if (half == 1 && s.Psell.p)
+    bind_preconditioner(s, w, s.P32.p, nullptr, s.sdinv32.p, true);
+  else if (half && s.P16.p)
+    bind_preconditioner(s, w, s.P32.p, s.P16.p, s.sdinv32.p);
+  else
+    bind_preconditioner(s, w, s.P32.p);
+  FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag32.p, N * sizeof(float), hipMemcpyDeviceToDevice, st));
+  k_f64_to_f32<<<grid_for(N), TPB, 0, st>>>(r, N, w.b.p);
+  Enq q{st};
+  if (w.cheb_m > 0)
+    cheb_apply<float>(q, w, w.b.p, w.y.p, nullptr, w.part.p);
+  else
+    k_jacobi32<<<grid_for(N), TPB, 0, st>>>(w.b.p, w.dinv.p, w.y.p, N);
+  FEEC_HIP(hipMemsetAsync(z, 0, N * sizeof(double), st));
+  k_axpy_from_f32<<<grid_for(N), TPB, 0, st>>>(w.y.p, N, 1.0, z);
+  FEEC_HIP(hipGetLastError());
+  ctx->sync();
+}
+
 // Average duration of one Chebyshev step (the preconditioner's fused SpMV)
 // on this system, FP32 or FP64; 0 when the system has no polynomial.
-double time_cheb_step(KktSystem& s, bool fp32, int reps, bool half) {
+double time_cheb_step(KktSystem& s, bool fp32, int reps, int half) {
   if (s.P.nnz == 0 || s.cheb_m < 2 || !s.cheb.p) return 0.0;
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = s.n;
   if (fp32 && !s.P32.p) kkt_refresh_fp32(s);
+  if (fp32 && half == 2 && !s.P16.p) kkt_refresh_half_csr(s);
+  const bool sell = fp32 && half == 1 && s.Psell.p;
   hipEvent_t e0, e1;
   FEEC_HIP(hipEventCreate(&e0));
   FEEC_HIP(hipEventCreate(&e1));
@@ -1181,6 +1557,11 @@ double time_cheb_step(KktSystem& s, bool fp32, int reps, bool half) {
     const MinresS* none = nullptr;
     auto step = [&] {
       if constexpr (std::is_same_v<T, float>) {
+        if (sell) {
+          k_cheb3_step<<<sell_grid(n), TPB, 0, st>>>(s.Psell.p, s.Psell_ptr.p, n, s.split, res.p,
+                                                     d.p, z.p, s.cheb.p, 0, none);
+          return;
+        }
         if (p16) {
           if (s.Pcol16.p)
             k_cheb_step<T, int16_t, _Float16, true><<<cheb_grid(n), TPB, 0, st>>>(

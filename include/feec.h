@@ -89,9 +89,15 @@ typedef struct feec_solve_opts {
   int32_t use_graph;     /* capture Krylov iterations in a hipGraph (default 1) */
   int32_t graph_iters;   /* iterations per graph segment, even (default 32) */
   int32_t cheb_degree;   /* degree of the Chebyshev-Jacobi polynomial applied to each
-                            hdif_gram block of the preconditioner; 0 = Jacobi only (default 32) */
-  int32_t precond_half;  /* mixed: run the polynomial on D^-1/2 hdif_gram D^-1/2 stored in FP16
-                            (unit diagonal, entries in [-1, 1]), FP32 arithmetic (default 1) */
+                            hdif_gram block of the preconditioner; 0 = Jacobi only (default 48) */
+  double cheb_lower;     /* lower end of the Chebyshev interval as a fraction of its upper end
+                            (1.05 lambda_max); 0 = by degree: 1e-2 up to degree 4, 3e-3 up to 8,
+                            1.5e-3 up to 12, 1e-3 up to 32, else 5e-4 (default 0) */
+  int32_t precond_half;  /* mixed: storage of the matrix the polynomial runs on, FP32 arithmetic
+                            throughout. 1: D^-1/2 hdif_gram D^-1/2 in FP16 as SELL-64 slices
+                            with the unit diagonal implied (FP16 CSR when a column offset
+                            exceeds +-32767) (default); 2: the same in FP16 CSR; 0: D^-1
+                            hdif_gram with FP32 values in CSR */
 } feec_solve_opts;
 
 typedef struct feec_report {
@@ -204,15 +210,31 @@ feec_status feec_problem_solve_device(feec_problem *prob, const double *source_d
 feec_status feec_problem_solve(feec_problem *prob, const double *source_host, feec_report *report);
 feec_status feec_problem_solution(feec_problem *prob, double *sigma, double *u, double *p);
 feec_status feec_problem_sizes(const feec_problem *prob, int64_t sizes[3], int64_t *nnz);
+/* Storage of the mixed path's preconditioner matrix (feec_problem_preconditioner). */
+typedef enum {
+  FEEC_PC_CSR_F32 = 0,    /* D^-1 hdif_gram, FP32 values, CSR */
+  FEEC_PC_CSR_F16 = 1,    /* D^-1/2 hdif_gram D^-1/2, FP16 values, CSR */
+  FEEC_PC_SELL64_F16 = 2  /* the same, off-diagonal only, packed {FP16, int16 offset} in 64-row
+                             SELL slices */
+} feec_pc_format;
+
 /* The block preconditioner blockdiag(hdif_gram(k-1), hdif_gram(k)) as built:
  * its nonzeros, the Chebyshev degree in use (0 = Jacobi), the estimated
- * largest eigenvalue of D^-1 hdif_gram per block, the bytes per stored
- * column index (2 when every |col - row| fits a 16-bit offset, else 4) and
- * per stored value on the mixed path (2 with precond_half, else 4); any
- * pointer may be NULL. */
+ * largest eigenvalue of D^-1 hdif_gram per block, the bytes per stored CSR
+ * column index (2 when every |col - row| fits a 16-bit offset, else 4), the
+ * mixed path's storage (feec_pc_format), the entries that storage holds
+ * (with SELL padding) and the off-diagonal nonzeros; any pointer may be NULL. */
 feec_status feec_problem_preconditioner(const feec_problem *prob, int64_t *nnz,
                                         int32_t *cheb_degree, double lambda_max[2],
-                                        int32_t *column_bytes, int32_t *value_bytes_mixed);
+                                        int32_t *column_bytes, int32_t *mixed_format,
+                                        int64_t *mixed_stored, int64_t *offdiag_nnz);
+/* Diagnostics (no counterpart in the reference; the reference applies its
+ * preconditioner inside MINRES, krylov.rs): z = B r for host vectors r, z of
+ * the KKT order (sigma, u, harmonic rows; sizes[0] + sizes[1] + sizes[2]
+ * doubles), with B the block preconditioner exactly as a solve of `precision`
+ * applies it and `precond_half` selecting the mixed path's storage. */
+feec_status feec_problem_apply_preconditioner(feec_problem *prob, int32_t precision,
+                                              int32_t precond_half, const double *r, double *z);
 /* Device pointer to an N_k scratch vector for feec_problem_solve_device. */
 feec_status feec_problem_source_buffer(feec_problem *prob, double **device_ptr);
 /* Copy an N_k host source into that device buffer. */

@@ -521,7 +521,7 @@ CHEB_LMAX_MARGIN = 1.05
 
 def cheb_interval_fraction(m):
     """Lower end of the Chebyshev interval as a fraction of its upper end."""
-    return 1e-2 if m <= 4 else 3e-3 if m <= 8 else 1.5e-3 if m <= 12 else 1e-3
+    return 1e-2 if m <= 4 else 3e-3 if m <= 8 else 1.5e-3 if m <= 12 else 1e-3 if m <= 32 else 5e-4
 
 
 def _xorshift_start(n):
@@ -565,6 +565,24 @@ def chebyshev_apply(P, dinv, r, c0, steps):
         z = z + d
         d = a * d + b * res
     return z + d
+
+
+def chebyshev_apply_three_term(P, dinv, r, c0, steps):
+    """The same z by the form the mixed GPU path evaluates: on the similar
+    matrix S = D^-1/2 P D^-1/2 (unit diagonal) with f = D^-1/2 r, x_0 = 0,
+    x_1 = c0 f, x_{k+2} = x_{k+1} + a_k (x_{k+1} - x_k) + b_k (f - S x_{k+1})
+    (d_{k+1} = x_{k+2} - x_{k+1}, r_{k+1} = f - S x_{k+1}), z = D^-1/2 x_m,
+    since p(D^-1 P) D^-1 = D^-1/2 p(S) D^-1/2. The diagonal of S is taken
+    as exactly 1 and only P's off-diagonal part is applied."""
+    import scipy.sparse as sp
+    sd = np.sqrt(np.abs(dinv))
+    S = sp.diags(sd) @ P @ sp.diags(sd)
+    S = S - sp.diags(S.diagonal())
+    f = sd * r
+    x0, x1 = np.zeros_like(f), c0 * f
+    for a, b in steps:
+        x0, x1 = x1, x1 + a * (x1 - x0) + b * (f - (x1 + S @ x1))
+    return sd * x1
 
 
 def chebyshev_jacobi(c: WhitneyComplex, k, nh, m):

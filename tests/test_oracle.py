@@ -405,6 +405,24 @@ def test_chebyshev_recurrence_is_the_chebyshev_residual_polynomial(m):
     np.testing.assert_allclose(1 - lam * z, want, rtol=1e-10, atol=1e-13)
 
 
+@pytest.mark.parametrize("m", [1, 2, 8, 32])
+def test_chebyshev_three_term_form_matches_two_term(m):
+    """The three-term recurrence on D^-1/2 P D^-1/2 with the unit diagonal
+    implied (the mixed GPU path's form) gives the two-term recurrence's
+    z = p(D^-1 P) D^-1 r on a real hdif_gram(1) block."""
+    cells, coords = jittered(3, 3, 0.1, 4)
+    cx = o.Complex(3, cells)
+    c = o.WhitneyComplex(cx, cx.lengths_from_coords(coords))
+    P = c.hdif_gram(1).to_scipy().tocsr()
+    dinv = 1.0 / P.diagonal()
+    lmax = np.abs(np.linalg.eigvals((dinv[:, None] * P.toarray()))).max()
+    c0, steps = o.chebyshev_coefficients(lmax, m)
+    r = np.cos(np.arange(P.shape[0]) * 1.3)
+    z2 = o.chebyshev_apply(P, dinv, r, c0, steps)
+    z3 = o.chebyshev_apply_three_term(P, dinv, r, c0, steps)
+    np.testing.assert_allclose(z3, z2, rtol=1e-9, atol=1e-12 * np.abs(z2).max())
+
+
 def test_chebyshev_jacobi_preconditioner():
     """lambda_max of D^-1 hdif_gram per block against a dense eigensolve, and
     a preconditioned MINRES that needs far fewer iterations than Jacobi."""
