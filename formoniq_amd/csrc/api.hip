@@ -16,21 +16,38 @@ namespace feec {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
+// Device memory comes from the device's default memory pool, stream-ordered on
+// the context's stream: a free does not wait for other contexts' work (a plain
+// hipFree synchronizes the whole device), and freed blocks stay in the pool
+// for the next allocation (its release threshold is raised when a context is
+// created).
 void* device_alloc(Ctx* ctx, size_t bytes) {
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, bytes);
+  hipError_t e = hipMallocAsync(&p, bytes, ctx->stream);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     fail(e == hipErrorOutOfMemory ? FEEC_E_OOM : FEEC_E_DEVICE,
-         "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+         "hipMallocAsync(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
   }
   ctx->bytes += (int64_t)bytes;
   return p;
 }
 
 void device_free(Ctx* ctx, void* p, size_t bytes) {
-  (void)hipFree(p);
+  (void)hipFreeAsync(p, ctx->stream);
   ctx->bytes -= (int64_t)bytes;
+}
+
+void copy_to_device(Ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  FEEC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  c->sync();
+}
+
+void copy_to_host(Ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  FEEC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  c->sync();
 }
 
 void* Ctx::ensure_temp(size_t b) {
@@ -169,9 +186,9 @@ void harmonic_basis(Geom& g, int bc, int k, std::vector<double>& basis, int64_t&
     std::vector<int64_t> rp(n + 1);
     std::vector<int32_t> col(M.nnz);
     std::vector<double> val(M.nnz);
-    FEEC_HIP(hipMemcpy(rp.data(), M.row_ptr.p, (n + 1) * 8, hipMemcpyDeviceToHost));
-    FEEC_HIP(hipMemcpy(col.data(), M.col.p, M.nnz * 4, hipMemcpyDeviceToHost));
-    FEEC_HIP(hipMemcpy(val.data(), M.val.p, M.nnz * 8, hipMemcpyDeviceToHost));
+    copy_to_host(m.ctx, rp.data(), M.row_ptr.p, (n + 1) * 8);
+    copy_to_host(m.ctx, col.data(), M.col.p, M.nnz * 4);
+    copy_to_host(m.ctx, val.data(), M.val.p, M.nnz * 8);
     for (int64_t i = 0; i < n; ++i) {
       double dgl = 0.0;
       for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
@@ -181,11 +198,11 @@ void harmonic_basis(Geom& g, int bc, int k, std::vector<double>& basis, int64_t&
   }
   // normalize in the M_k inner product
   DBuf<double> dv(m.ctx, n), mv(m.ctx, n);
-  FEEC_HIP(hipMemcpy(dv.p, v.data(), n * 8, hipMemcpyHostToDevice));
+  copy_to_device(m.ctx, dv.p, v.data(), n * 8);
   spmv64(m.ctx, M, dv.p, mv.p);
   m.ctx->sync();
   std::vector<double> hmv(n);
-  FEEC_HIP(hipMemcpy(hmv.data(), mv.p, n * 8, hipMemcpyDeviceToHost));
+  copy_to_host(m.ctx, hmv.data(), mv.p, n * 8);
   double q = 0.0;
   for (int64_t i = 0; i < n; ++i) q += v[i] * hmv[i];
   double s = 1.0 / std::sqrt(q);
@@ -332,7 +349,7 @@ void problem_build(Problem& P) {
     Csr M;
     assemble_form(g, bc, FEEC_MASS, k, M);
     DBuf<double> h(ctx, P.nu), mh(ctx, P.nu);
-    FEEC_HIP(hipMemcpy(h.p, P.harmonics_host.data(), P.nu * 8, hipMemcpyHostToDevice));
+    copy_to_device(ctx, h.p, P.harmonics_host.data(), P.nu * 8);
     spmv64(ctx, M, h.p, mh.p);
     P.sys.border.alloc(ctx, P.nh * n);
     k_fill_border<<<grid_for(n), TPB, 0, ctx->stream>>>(mh.p, P.nu, P.ns, n, (int)P.nh,
@@ -411,7 +428,7 @@ int64_t problem_ndofs(Problem* P) { return P->sys.n + P->nh; }
 void problem_prepare_rhs(Problem* P, const double* src) {
   Mesh& m = *P->g->mesh;
   Ctx* ctx = m.ctx;
-  FEEC_HIP(hipMemcpy(P->source.p, src, m.n[P->k] * 8, hipMemcpyHostToDevice));
+  copy_to_device(ctx, P->source.p, src, m.n[P->k] * 8);
   const int64_t N = P->sys.n + P->nh;
   FEEC_HIP(hipMemsetAsync(P->rhs.p, 0, N * 8, ctx->stream));
   if (P->bc == FEEC_RELATIVE) m.ensure_interior();
@@ -439,7 +456,7 @@ void problem_u(Problem* P, double* u) {
   const int32_t* map = (P->bc == FEEC_RELATIVE) ? m.interior[P->k].p : nullptr;
   k_extend<<<grid_for(m.n[P->k]), TPB, 0, ctx->stream>>>(P->x.p, map, m.n[P->k], P->ns, o.p);
   ctx->sync();
-  FEEC_HIP(hipMemcpy(u, o.p, m.n[P->k] * 8, hipMemcpyDeviceToHost));
+  copy_to_host(ctx, u, o.p, m.n[P->k] * 8);
 }
 
 }  // namespace feec
@@ -477,7 +494,7 @@ void feec_solve_opts_default(feec_solve_opts* o) {
   o->inner_rtol = 1e-4;
   o->use_graph = 1;
   o->graph_iters = 32;
-  o->cheb_degree = 48;
+  o->cheb_degree = 64;
   o->cheb_lower = 0.0;
   o->precond_half = 1;
 }
@@ -491,13 +508,26 @@ feec_status feec_ctx_create(int device, feec_ctx** out) {
     FEEC_HIP(hipSetDevice(device));
     auto* c = new feec_ctx_s();
     c->device = device;
+    int pools = 0;
+    FEEC_HIP(hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device));
+    FEEC_REQUIRE(pools, FEEC_E_DEVICE, "device has no stream-ordered memory pools");
+    hipMemPool_t pool = nullptr;
+    FEEC_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t keep = UINT64_MAX;  // freed blocks stay cached in the pool
+    FEEC_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&c->dflag, sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&c->dcount, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc(&c->pinned, Ctx::kPinnedBytes, hipHostMallocDefault);
     if (e != hipSuccess) {
       if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
       fail(FEEC_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    try {
+      c->dflag = static_cast<int*>(device_alloc(c, sizeof(int)));
+      c->dcount = static_cast<unsigned long long*>(device_alloc(c, sizeof(unsigned long long)));
+    } catch (...) {
+      feec_ctx_destroy(c);
+      throw;
     }
     *out = c;
   });
@@ -509,9 +539,11 @@ void feec_ctx_destroy(feec_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   elmat_release_tables(ctx);
   if (ctx->temp) device_free(ctx, ctx->temp, ctx->temp_bytes);
-  (void)hipFree(ctx->dflag);
-  (void)hipFree(ctx->dcount);
+  if (ctx->dflag) device_free(ctx, ctx->dflag, sizeof(int));
+  if (ctx->dcount) device_free(ctx, ctx->dcount, sizeof(unsigned long long));
+  (void)hipStreamSynchronize(ctx->stream);
   (void)hipStreamDestroy(ctx->stream);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   delete ctx;
 }
 
@@ -658,7 +690,7 @@ feec_status feec_mesh_skeleton(const feec_mesh* mc, int k, int64_t* verts) {
     const int d = m->d, nv = d + 1;
     const int64_t nc = m->n[d];
     std::vector<int32_t> cells((size_t)nc * nv);
-    FEEC_HIP(hipMemcpy(cells.data(), m->faces[0].p, cells.size() * 4, hipMemcpyDeviceToHost));
+    copy_to_host(m->ctx, cells.data(), m->faces[0].p, cells.size() * 4);
     if (k == 0) {
       for (int64_t v = 0; v < m->n[0]; ++v) verts[v] = v;
       return;
@@ -668,7 +700,7 @@ feec_status feec_mesh_skeleton(const feec_mesh* mc, int k, int64_t* verts) {
       return;
     }
     std::vector<int64_t> rep(m->n[k]);
-    FEEC_HIP(hipMemcpy(rep.data(), m->rep[k].p, rep.size() * 8, hipMemcpyDeviceToHost));
+    copy_to_host(m->ctx, rep.data(), m->rep[k].p, rep.size() * 8);
     std::vector<std::vector<int>> subs;
     for (unsigned msk = 0; msk < (1u << nv); ++msk) {
       if (__builtin_popcount(msk) != k + 1) continue;
@@ -691,7 +723,7 @@ feec_status feec_mesh_cell_faces(const feec_mesh* m, int k, int64_t* out) {
     FEEC_REQUIRE(k >= 0 && k <= m->d && out, FEEC_E_ARG, "bad grade");
     int64_t n = m->n[m->d] * (int64_t)binom(m->d + 1, k + 1);
     std::vector<int32_t> h(n);
-    FEEC_HIP(hipMemcpy(h.data(), m->faces[k].p, n * 4, hipMemcpyDeviceToHost));
+    copy_to_host(m->ctx, h.data(), m->faces[k].p, n * 4);
     for (int64_t i = 0; i < n; ++i) out[i] = h[i];
   });
 }
@@ -699,7 +731,7 @@ feec_status feec_mesh_cell_faces(const feec_mesh* m, int k, int64_t* out) {
 feec_status feec_mesh_boundary(const feec_mesh* m, int k, uint8_t* flags) {
   return guard([&] {
     FEEC_REQUIRE(k >= 0 && k <= m->d && flags, FEEC_E_ARG, "bad grade");
-    FEEC_HIP(hipMemcpy(flags, m->bnd[k].p, m->n[k], hipMemcpyDeviceToHost));
+    copy_to_host(m->ctx, flags, m->bnd[k].p, m->n[k]);
   });
 }
 
@@ -749,7 +781,7 @@ feec_status feec_geom_from_lengths(feec_mesh* m, const double* lengths, feec_geo
     auto* g = new feec_geom_s();
     g->mesh = m;
     g->lengths.alloc(m->ctx, m->n[1]);
-    FEEC_HIP(hipMemcpy(g->lengths.p, lengths, m->n[1] * 8, hipMemcpyHostToDevice));
+    copy_to_device(m->ctx, g->lengths.p, lengths, m->n[1] * 8);
     *out = g;
   });
 }
@@ -763,7 +795,7 @@ feec_status feec_geom_from_coords(feec_mesh* m, const double* coords, feec_ambie
     g->mesh = m;
     g->lengths.alloc(m->ctx, m->n[1]);
     DBuf<double> c(m->ctx, m->n[0] * m->d);
-    FEEC_HIP(hipMemcpy(c.p, coords, m->n[0] * m->d * 8, hipMemcpyHostToDevice));
+    copy_to_device(m->ctx, c.p, coords, m->n[0] * m->d * 8);
     lengths_from_coords(*m, c.p, amb == FEEC_MINKOWSKI, g->lengths.p);
     m->ctx->sync();
     *out = g;
@@ -774,7 +806,7 @@ void feec_geom_destroy(feec_geom* g) { delete g; }
 
 feec_status feec_geom_lengths(const feec_geom* g, double* out) {
   return guard([&] {
-    FEEC_HIP(hipMemcpy(out, g->lengths.p, g->mesh->n[1] * 8, hipMemcpyDeviceToHost));
+    copy_to_host(g->mesh->ctx, out, g->lengths.p, g->mesh->n[1] * 8);
   });
 }
 
@@ -785,9 +817,9 @@ feec_status feec_cell_metrics(feec_geom* g, double* gm, double* gi, double* det)
     DBuf<double> a(m.ctx, gm ? nc * dd : 1), b(m.ctx, gi ? nc * dd : 1), c(m.ctx, det ? nc : 1);
     cell_metrics(*g, gm ? a.p : nullptr, gi ? b.p : nullptr, det ? c.p : nullptr);
     m.ctx->sync();
-    if (gm) FEEC_HIP(hipMemcpy(gm, a.p, nc * dd * 8, hipMemcpyDeviceToHost));
-    if (gi) FEEC_HIP(hipMemcpy(gi, b.p, nc * dd * 8, hipMemcpyDeviceToHost));
-    if (det) FEEC_HIP(hipMemcpy(det, c.p, nc * 8, hipMemcpyDeviceToHost));
+    if (gm) copy_to_host(m.ctx, gm, a.p, nc * dd * 8);
+    if (gi) copy_to_host(m.ctx, gi, b.p, nc * dd * 8);
+    if (det) copy_to_host(m.ctx, det, c.p, nc * 8);
   });
 }
 
@@ -796,7 +828,7 @@ feec_status feec_causal_census(feec_geom* g, int64_t counts[3], uint8_t* riem) {
     Mesh& m = *g->mesh;
     DBuf<uint8_t> r(m.ctx, riem ? m.n[m.d] : 1);
     causal_census(*g, counts, riem ? r.p : nullptr);
-    if (riem) FEEC_HIP(hipMemcpy(riem, r.p, m.n[m.d], hipMemcpyDeviceToHost));
+    if (riem) copy_to_host(m.ctx, riem, r.p, m.n[m.d]);
   });
 }
 
@@ -822,7 +854,7 @@ feec_status feec_element_matrices(feec_geom* g, feec_form_kind kind, int grade, 
     k_soa_to_aos<<<grid_for(total), TPB, 0, m.ctx->stream>>>(soa.p, nc, r * c, aos.p);
     FEEC_HIP(hipGetLastError());
     m.ctx->sync();
-    FEEC_HIP(hipMemcpy(out, aos.p, total * 8, hipMemcpyDeviceToHost));
+    copy_to_host(m.ctx, out, aos.p, total * 8);
   });
 }
 
@@ -853,19 +885,19 @@ feec_status feec_csr_shape(const feec_csr* a, int64_t* nrows, int64_t* ncols, in
 
 feec_status feec_csr_copy(const feec_csr* a, int64_t* row_ptr, int32_t* col, double* val) {
   return guard([&] {
-    if (row_ptr) FEEC_HIP(hipMemcpy(row_ptr, a->row_ptr.p, (a->nrows + 1) * 8, hipMemcpyDeviceToHost));
-    if (col && a->nnz) FEEC_HIP(hipMemcpy(col, a->col.p, a->nnz * 4, hipMemcpyDeviceToHost));
-    if (val && a->nnz) FEEC_HIP(hipMemcpy(val, a->val.p, a->nnz * 8, hipMemcpyDeviceToHost));
+    if (row_ptr) copy_to_host(a->ctx, row_ptr, a->row_ptr.p, (a->nrows + 1) * 8);
+    if (col && a->nnz) copy_to_host(a->ctx, col, a->col.p, a->nnz * 4);
+    if (val && a->nnz) copy_to_host(a->ctx, val, a->val.p, a->nnz * 8);
   });
 }
 
 feec_status feec_csr_spmv(feec_csr* a, const double* x, double* y) {
   return guard([&] {
     DBuf<double> dx(a->ctx, std::max<int64_t>(a->ncols, 1)), dy(a->ctx, std::max<int64_t>(a->nrows, 1));
-    FEEC_HIP(hipMemcpy(dx.p, x, a->ncols * 8, hipMemcpyHostToDevice));
+    copy_to_device(a->ctx, dx.p, x, a->ncols * 8);
     spmv64(a->ctx, *a, dx.p, dy.p);
     a->ctx->sync();
-    FEEC_HIP(hipMemcpy(y, dy.p, a->nrows * 8, hipMemcpyDeviceToHost));
+    copy_to_host(a->ctx, y, dy.p, a->nrows * 8);
   });
 }
 
@@ -905,7 +937,7 @@ feec_status feec_mixed_kkt(feec_geom* g, feec_bc bc, int grade, feec_csr** out, 
       // the border as dense CSR blocks, every entry kept (elliptic.rs:62-84)
       int64_t n = P.sys.n, nh = P.nh, nu = P.nu;
       std::vector<double> bh(nh * n);
-      FEEC_HIP(hipMemcpy(bh.data(), P.sys.border.p, nh * n * 8, hipMemcpyDeviceToHost));
+      copy_to_host(ctx, bh.data(), P.sys.border.p, nh * n * 8);
       Csr col_blk, row_blk;
       std::vector<int64_t> rp(nu + 1), rp2(nh + 1);
       std::vector<int32_t> c1(nu * nh), c2(nh * nu);
@@ -935,9 +967,9 @@ feec_status feec_mixed_kkt(feec_geom* g, feec_bc bc, int grade, feec_csr** out, 
         c.row_ptr.alloc(ctx, r + 1);
         c.col.alloc(ctx, c.nnz);
         c.val.alloc(ctx, c.nnz);
-        FEEC_HIP(hipMemcpy(c.row_ptr.p, rpv.data(), (r + 1) * 8, hipMemcpyHostToDevice));
-        FEEC_HIP(hipMemcpy(c.col.p, cv.data(), c.nnz * 4, hipMemcpyHostToDevice));
-        FEEC_HIP(hipMemcpy(c.val.p, vv.data(), c.nnz * 8, hipMemcpyHostToDevice));
+        copy_to_device(ctx, c.row_ptr.p, rpv.data(), (r + 1) * 8);
+        copy_to_device(ctx, c.col.p, cv.data(), c.nnz * 4);
+        copy_to_device(ctx, c.val.p, vv.data(), c.nnz * 8);
       };
       up(col_blk, nu, nh, rp, c1, v1);
       up(row_blk, nh, nu, rp2, c2, v2);
@@ -1023,7 +1055,7 @@ feec_status feec_problem_upload_source(feec_problem* p, const double* src) {
   return guard([&] {
     Mesh& m = *p->g->mesh;
     m.ctx->sync();
-    FEEC_HIP(hipMemcpy(p->source.p, src, m.n[p->k] * 8, hipMemcpyHostToDevice));
+    copy_to_device(m.ctx, p->source.p, src, m.n[p->k] * 8);
   });
 }
 
@@ -1071,9 +1103,9 @@ feec_status feec_problem_apply_preconditioner(feec_problem* p, int32_t precision
     Ctx* ctx = p->g->mesh->ctx;
     const int64_t N = p->sys.n + p->nh;
     DBuf<double> rd(ctx, N), zd(ctx, N);
-    FEEC_HIP(hipMemcpy(rd.p, r, N * sizeof(double), hipMemcpyHostToDevice));
+    copy_to_device(ctx, rd.p, r, N * sizeof(double));
     kkt_apply_preconditioner(p->sys, precision, precond_half, rd.p, zd.p);
-    FEEC_HIP(hipMemcpy(z, zd.p, N * sizeof(double), hipMemcpyDeviceToHost));
+    copy_to_host(ctx, z, zd.p, N * sizeof(double));
   });
 }
 
@@ -1098,7 +1130,7 @@ feec_status feec_problem_solve_device(feec_problem* p, const double* src, feec_r
 feec_status feec_problem_solve(feec_problem* p, const double* src, feec_report* report) {
   return guard([&] {
     Mesh& m = *p->g->mesh;
-    FEEC_HIP(hipMemcpy(p->source.p, src, m.n[p->k] * 8, hipMemcpyHostToDevice));
+    copy_to_device(m.ctx, p->source.p, src, m.n[p->k] * 8);
     feec_status s = feec_problem_solve_device(p, p->source.p, report);
     if (s != FEEC_OK) fail(s, g_last_error);
   });
@@ -1115,16 +1147,16 @@ feec_status feec_problem_solution(feec_problem* p, double* sigma, double* u, dou
       const int32_t* map = (p->bc == FEEC_RELATIVE) ? m.interior[k - 1].p : nullptr;
       k_extend<<<grid_for(m.n[k - 1]), TPB, 0, ctx->stream>>>(p->x.p, map, m.n[k - 1], 0, o.p);
       ctx->sync();
-      FEEC_HIP(hipMemcpy(sigma, o.p, m.n[k - 1] * 8, hipMemcpyDeviceToHost));
+      copy_to_host(ctx, sigma, o.p, m.n[k - 1] * 8);
     }
     if (u) {
       DBuf<double> o(ctx, m.n[k]);
       const int32_t* map = (p->bc == FEEC_RELATIVE) ? m.interior[k].p : nullptr;
       k_extend<<<grid_for(m.n[k]), TPB, 0, ctx->stream>>>(p->x.p, map, m.n[k], p->ns, o.p);
       ctx->sync();
-      FEEC_HIP(hipMemcpy(u, o.p, m.n[k] * 8, hipMemcpyDeviceToHost));
+      copy_to_host(ctx, u, o.p, m.n[k] * 8);
     }
-    if (ph && p->nh) FEEC_HIP(hipMemcpy(ph, p->x.p + p->sys.n, p->nh * 8, hipMemcpyDeviceToHost));
+    if (ph && p->nh) copy_to_host(ctx, ph, p->x.p + p->sys.n, p->nh * 8);
   });
 }
 

@@ -145,20 +145,20 @@ feec_status feec_solve_dirac_source(feec_geom* g, double mass_term, const double
     int64_t nrel = 0;
     for (int k = 0; k <= d; ++k) {
       std::vector<int32_t> ik(m.n[k]);
-      FEEC_HIP(hipMemcpy(ik.data(), m.interior[k].p, m.n[k] * 4, hipMemcpyDeviceToHost));
+      copy_to_host(ctx, ik.data(), m.interior[k].p, m.n[k] * 4);
       for (int64_t i = 0; i < m.n[k]; ++i)
         map[D.offsets[k] + i] = ik[i] >= 0 ? (int32_t)(nrel + ik[i]) : -1;
       nrel += m.n_interior[k];
     }
     DBuf<int32_t> dmap(ctx, nfull);
-    FEEC_HIP(hipMemcpy(dmap.p, map.data(), nfull * 4, hipMemcpyHostToDevice));
+    copy_to_device(ctx, dmap.p, map.data(), nfull * 4);
     // rhs = E^T (f - S lift)
     std::vector<double> f(load, load + nfull), l(lift, lift + nfull), sl(nfull);
     DBuf<double> dl(ctx, nfull), dsl(ctx, nfull);
-    FEEC_HIP(hipMemcpy(dl.p, l.data(), nfull * 8, hipMemcpyHostToDevice));
+    copy_to_device(ctx, dl.p, l.data(), nfull * 8);
     spmv64(ctx, S, dl.p, dsl.p);
     ctx->sync();
-    FEEC_HIP(hipMemcpy(sl.data(), dsl.p, nfull * 8, hipMemcpyDeviceToHost));
+    copy_to_host(ctx, sl.data(), dsl.p, nfull * 8);
     Csr Srel;
     csr_select(S, dmap.p, nrel, dmap.p, nrel, Srel);
     std::vector<double> rhs(nrel, 0.0);
@@ -176,16 +176,16 @@ feec_status feec_solve_dirac_source(feec_geom* g, double mass_term, const double
         DBuf<double> dd(ctx, Mn.nrows);
         csr_diagonal(Mn, dd.p);
         ctx->sync();
-        FEEC_HIP(hipMemcpy(dg.data(), dd.p, Mn.nrows * 8, hipMemcpyDeviceToHost));
+        copy_to_host(ctx, dg.data(), dd.p, Mn.nrows * 8);
         h.assign(nrel, 0.0);
         int64_t off = nrel - m.n_interior[d];
         for (int64_t c = 0; c < m.n_interior[d]; ++c) h[off + c] = m.orientation[c] / dg[c];
         DBuf<double> dh(ctx, nrel), sh(ctx, nrel);
-        FEEC_HIP(hipMemcpy(dh.p, h.data(), nrel * 8, hipMemcpyHostToDevice));
+        copy_to_device(ctx, dh.p, h.data(), nrel * 8);
         spmv64(ctx, Srel, dh.p, sh.p);
         ctx->sync();
         std::vector<double> shh(nrel);
-        FEEC_HIP(hipMemcpy(shh.data(), sh.p, nrel * 8, hipMemcpyDeviceToHost));
+        copy_to_host(ctx, shh.data(), sh.p, nrel * 8);
         double a = 0.0, b = 0.0;
         for (int64_t i = 0; i < nrel; ++i) {
           a += shh[i] * shh[i];
@@ -225,9 +225,9 @@ feec_status feec_solve_dirac_source(feec_geom* g, double mass_term, const double
         c.row_ptr.alloc(ctx, r + 1);
         c.col.alloc(ctx, c.nnz);
         c.val.alloc(ctx, c.nnz);
-        FEEC_HIP(hipMemcpy(c.row_ptr.p, rpv.data(), (r + 1) * 8, hipMemcpyHostToDevice));
-        FEEC_HIP(hipMemcpy(c.col.p, cv.data(), c.nnz * 4, hipMemcpyHostToDevice));
-        FEEC_HIP(hipMemcpy(c.val.p, vv.data(), c.nnz * 8, hipMemcpyHostToDevice));
+        copy_to_device(ctx, c.row_ptr.p, rpv.data(), (r + 1) * 8);
+        copy_to_device(ctx, c.col.p, cv.data(), c.nnz * 4);
+        copy_to_device(ctx, c.val.p, vv.data(), c.nnz * 8);
       };
       Csr bc_col, bc_row;
       up(bc_col, nrel, 1, rp, c1, v1);
@@ -251,13 +251,13 @@ feec_status feec_solve_dirac_source(feec_geom* g, double mass_term, const double
     FEEC_HIP(hipGetLastError());
     DBuf<double> b(ctx, n), x(ctx, n);
     ctx->sync();
-    FEEC_HIP(hipMemcpy(b.p, rhs.data(), n * 8, hipMemcpyHostToDevice));
+    copy_to_device(ctx, b.p, rhs.data(), n * 8);
     SolverState* ws = nullptr;
     feec_report r = kkt_solve(sys, b.p, x.p, o, &ws);
     solver_state_free(ws);
     if (report) *report = r;
     std::vector<double> xs(n);
-    FEEC_HIP(hipMemcpy(xs.data(), x.p, n * 8, hipMemcpyDeviceToHost));
+    copy_to_host(ctx, xs.data(), x.p, n * 8);
     for (int64_t i = 0; i < nfull; ++i) out[i] = l[i] + (map[i] >= 0 ? xs[map[i]] : 0.0);
     FEEC_REQUIRE(r.converged, FEEC_E_NOT_CONVERGED, "Dirac source solve did not converge");
   });

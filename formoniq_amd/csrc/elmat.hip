@@ -334,7 +334,7 @@ static DevTables& tables_for(Ctx* ctx, int D) {
     }
   }
   T->coef.alloc(ctx, std::max<size_t>(coef.size(), 1));
-  FEEC_HIP(hipMemcpy(T->coef.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
+  copy_to_device(ctx, T->coef.p, coef.data(), coef.size() * sizeof(double));
   auto& slot = g_tabs[key];
   slot = std::move(T);
   return *slot;

@@ -138,8 +138,17 @@ struct Ctx {
   // small device scalars owned by the context: an int flag and a counter
   int* dflag = nullptr;
   unsigned long long* dcount = nullptr;
+  // pinned host words for reading solver scalars back, owned by the context
+  void* pinned = nullptr;
+  static constexpr size_t kPinnedBytes = 4096;
   void sync() { FEEC_HIP(hipStreamSynchronize(stream)); }
 };
+
+// Copies ordered on the context's stream; the host buffer may be reused on
+// return (the stream is synchronized). Device memory is allocated and freed
+// stream-ordered on the same stream, so no copy may go through another stream.
+void copy_to_device(Ctx* c, void* dst, const void* src, size_t bytes);
+void copy_to_host(Ctx* c, void* dst, const void* src, size_t bytes);
 
 // Device COO -> CSR plan: the sorted order of (row, col) keys, kept so that a
 // later value pass is a segmented sum (duplicates summed in input order).

@@ -800,11 +800,9 @@ struct SolverState {
   MinresWork<double> w64;
   MinresWork<float> w32;
   DBuf<double> r, x, part;
-  MinresS* host_s = nullptr;  // pinned
-  ~SolverState() {
-    if (host_s) (void)hipHostFree(host_s);
-  }
+  MinresS* host_s = nullptr;  // the context's pinned words
 };
+static_assert(sizeof(MinresS) <= Ctx::kPinnedBytes, "solver scalars fit the pinned words");
 
 void solver_state_free(SolverState* ws) { delete ws; }
 
@@ -1250,7 +1248,7 @@ void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
     st_rng ^= st_rng << 17;
     hx[i] = 0.5 + (double)(st_rng >> 11) * (1.0 / 9007199254740992.0);
   }
-  FEEC_HIP(hipMemcpy(x.p, hx.data(), n * 8, hipMemcpyHostToDevice));
+  copy_to_device(ctx, x.p, hx.data(), n * 8);
   auto block_norm = [&](const double* v, int64_t lo, int64_t hi) {
     if (hi <= lo) return 0.0;
     k_dot<double><<<red_grid(hi - lo), TPB, 0, st>>>(v + lo, v + lo, hi - lo, part.p);
@@ -1277,7 +1275,13 @@ void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
   }
   // interval lower ends (fraction of b) tuned on 3D k=1 Kuhn meshes
   const double frac =
-      lower > 0.0 ? lower : m <= 4 ? 1e-2 : m <= 8 ? 3e-3 : m <= 12 ? 1.5e-3 : m <= 32 ? 1e-3 : 5e-4;
+      lower > 0.0 ? lower
+      : m <= 4    ? 1e-2
+      : m <= 8    ? 3e-3
+      : m <= 12   ? 1.5e-3
+      : m <= 32   ? 1e-3
+      : m <= 48   ? 5e-4
+                  : 2.5e-4;
   std::vector<double> coef(2 + 4 * std::max(m - 1, 0), 0.0);
   double rho[2], sigma[2], delta[2];
   for (int b = 0; b < 2; ++b) {
@@ -1340,7 +1344,7 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
   hipStream_t st = ctx->stream;
   if (!*wsp) {
     *wsp = new SolverState();
-    FEEC_HIP(hipHostMalloc((void**)&(*wsp)->host_s, sizeof(MinresS), hipHostMallocDefault));
+    (*wsp)->host_s = static_cast<MinresS*>(ctx->pinned);
   }
   SolverState& ss = **wsp;
   const int64_t n = s.n, nh = s.nh, N = n + nh;
@@ -1455,7 +1459,7 @@ double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
   h.oldb = 1.0;
   h.iter = 1;
   h.done = 0;
-  FEEC_HIP(hipMemcpy(S.p, &h, sizeof(h), hipMemcpyHostToDevice));
+  copy_to_device(ctx, S.p, &h, sizeof(h));
   DBuf<double> part(ctx, NBLK);
   hipEvent_t e0, e1;
   FEEC_HIP(hipEventCreate(&e0));

@@ -89,10 +89,10 @@ typedef struct feec_solve_opts {
   int32_t use_graph;     /* capture Krylov iterations in a hipGraph (default 1) */
   int32_t graph_iters;   /* iterations per graph segment, even (default 32) */
   int32_t cheb_degree;   /* degree of the Chebyshev-Jacobi polynomial applied to each
-                            hdif_gram block of the preconditioner; 0 = Jacobi only (default 48) */
+                            hdif_gram block of the preconditioner; 0 = Jacobi only (default 64) */
   double cheb_lower;     /* lower end of the Chebyshev interval as a fraction of its upper end
                             (1.05 lambda_max); 0 = by degree: 1e-2 up to degree 4, 3e-3 up to 8,
-                            1.5e-3 up to 12, 1e-3 up to 32, else 5e-4 (default 0) */
+                            1.5e-3 up to 12, 1e-3 up to 32, 5e-4 up to 48, else 2.5e-4 (default 0) */
   int32_t precond_half;  /* mixed: storage of the matrix the polynomial runs on, FP32 arithmetic
                             throughout. 1: D^-1/2 hdif_gram D^-1/2 in FP16 as SELL-64 slices
                             with the unit diagonal implied (FP16 CSR when a column offset
