@@ -375,7 +375,7 @@ def main():
     pc = mixed.preconditioner()
     p_nnz, cheb_deg, p_col_bytes = pc["nnz"], pc["degree"], pc["column_bytes"]
     spmv_bytes = 8 * nnz + 8 * (n_kkt + 1) + 12 * n_kkt
-    if pc["mixed_format"] == "sell64_f16":
+    if pc["format"] == "sell64_f16":
         # three-term step on the SELL-64 copy: 4 B per off-diagonal nonzero
         # (FP16 value + int16 offset; padding is not counted), 8 B per 64-row
         # slice offset, x_{k+1} read (4n), x_k read and overwritten (8n), f read (4n)
@@ -386,7 +386,7 @@ def main():
         # (2 B as band offsets, else 4 B), row pointers, d (4n), res and z read
         # and written (16n), the next direction written (4n); dinv (4n) only
         # when unscaled
-        p_val_bytes = 2 if pc["mixed_format"] == "csr_f16" else 4
+        p_val_bytes = 2 if pc["format"] == "csr_f16" else 4
         cheb_bytes = ((p_val_bytes + p_col_bytes) * p_nnz + 8 * (n_kkt + 1) + 24 * n_kkt
                       + (4 * n_kkt if p_val_bytes == 4 else 0))
     achieved = cheb_bytes / times["cheb32"] / 1e9
@@ -395,7 +395,7 @@ def main():
         with open(args.traffic) as fh:
             tr = json.load(fh)
         # only a measurement of the same kernel on the same workload applies
-        if tr.get("n_kkt") == n_kkt and tr.get("cheb_step_f32_format") == pc["mixed_format"]:
+        if tr.get("n_kkt") == n_kkt and tr.get("cheb_step_f32_format") == pc["format"]:
             traffic = tr.get("cheb_step_f32_hbm_bytes_per_launch")
     # per tet: 6 edge lengths + 4 vertex indices in; M_0, dif_test(1), dif_both(2),
     # two diagonals, M_1 and dif_both(1) out (138 doubles)
@@ -435,15 +435,15 @@ def main():
         "kernel_seconds": times,
         "roofline": {
             "kernel": ("k_cheb3_step (Chebyshev step on the SELL-64 FP16 preconditioner: SpMV + three-term "
-                       "recurrence, FP32 arithmetic)" if pc["mixed_format"] == "sell64_f16" else
+                       "recurrence, FP32 arithmetic)" if pc["format"] == "sell64_f16" else
                        "k_cheb_step<float> (Chebyshev step: preconditioner SpMV + recurrence, FP32 arithmetic)"),
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "algorithmic_bytes_per_launch": cheb_bytes,
             "launch_seconds": times["cheb32"], "precond_nnz": p_nnz, "cheb_degree": cheb_deg,
             "precond_column_bytes": p_col_bytes, "precond_value_bytes": p_val_bytes,
-            "precond_format": pc["mixed_format"], "precond_offdiag_nnz": pc["offdiag_nnz"],
-            "precond_stored_entries": pc["mixed_stored"],
+            "precond_format": pc["format"], "precond_offdiag_nnz": pc["offdiag_nnz"],
+            "precond_stored_entries": pc["stored"],
         },
         "mr_spmv_f32": {"launch_seconds": times["spmv32"], "algorithmic_bytes_per_launch": spmv_bytes,
                         "achieved_gbs": spmv_bytes / times["spmv32"] / 1e9},

@@ -30,6 +30,8 @@ MASS, DIF_TRIAL, DIF_TEST, DIF_BOTH, HDIF_GRAM = 0, 1, 2, 3, 4
 FULL, RELATIVE = 0, 1
 EUCLIDEAN, MINKOWSKI = 0, 1
 FP64, MIXED = 0, 1
+# feec_pc_storage: how the preconditioner matrix the polynomial runs on is stored
+PC_SELL, PC_CSR_HALF, PC_CSR = 0, 1, 2
 
 __all__ = [
     "Context", "CartesianGrid", "Complex", "MeshLengthsSq", "WhitneyComplex", "Csr",
@@ -413,33 +415,33 @@ class Problem:
         return dict(elmat=float(out[0]), scatter=float(out[1]), spmv32=float(out[2]),
                     spmv64=float(out[3]), cheb32=float(out[4]), cheb64=float(out[5]))
 
-    PC_FORMATS = {0: "csr_f32", 1: "csr_f16", 2: "sell64_f16"}
+    PC_FORMATS = {0: "csr_f32", 1: "csr_f16", 2: "sell64_f16", 3: "csr_f64", 4: "sell64_f64"}
 
-    def apply_preconditioner(self, r, precision=None, precond_half=None):
+    def apply_preconditioner(self, r, precision=None, storage=None):
         """z = B r (KKT order: sigma, u, harmonic rows) with the block preconditioner
-        as a solve of `precision` applies it; `precond_half` picks the mixed path's
-        storage (defaults: this problem's options)."""
+        as a solve of `precision` with `storage` (PC_SELL, PC_CSR_HALF, PC_CSR)
+        applies it (defaults: this problem's options)."""
         prec = self.opts.precision if precision is None else precision
-        half = self.opts.precond_half if precond_half is None else precond_half
+        store = self.opts.precond_storage if storage is None else storage
         r = np.ascontiguousarray(r, dtype=np.float64)
         assert r.shape == (sum(self.sizes),)
         z = np.zeros_like(r)
-        check(load().feec_problem_apply_preconditioner(self._h, prec, half, _ptr(r, _lib.f64p),
+        check(load().feec_problem_apply_preconditioner(self._h, prec, store, _ptr(r, _lib.f64p),
                                                        _ptr(z, _lib.f64p)))
         return z
 
     def preconditioner(self):
         """The block preconditioner as built: nonzeros, Chebyshev degree, lambda_max
-        per block, bytes per stored CSR column index, the mixed path's storage
-        format, the entries it stores (SELL padding included) and the
-        off-diagonal nonzeros."""
+        per block, bytes per stored CSR column index, the format a solve with this
+        problem's options applies, the entries that format stores (SELL padding
+        included) and the off-diagonal nonzeros."""
         nnz, deg, lam, cb = C.c_int64(), C.c_int32(), np.zeros(2), C.c_int32()
         fmt, stored, offd = C.c_int32(), C.c_int64(), C.c_int64()
         check(load().feec_problem_preconditioner(self._h, C.byref(nnz), C.byref(deg),
                                                  _ptr(lam, _lib.f64p), C.byref(cb), C.byref(fmt),
                                                  C.byref(stored), C.byref(offd)))
         return dict(nnz=nnz.value, degree=deg.value, lambda_max=lam, column_bytes=cb.value,
-                    mixed_format=self.PC_FORMATS[fmt.value], mixed_stored=stored.value,
+                    format=self.PC_FORMATS[fmt.value], stored=stored.value,
                     offdiag_nnz=offd.value)
 
     def upload_source(self, source):

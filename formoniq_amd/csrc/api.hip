@@ -379,8 +379,12 @@ void problem_assemble_from_elements(Problem& P) {
     P.ws = nullptr;
   }
   if (P.sys.P.nnz) coo_plan_values(ctx, P.pc_plan, P.el.p, P.sys.P.val.p, nullptr);
-  // the FP32/FP16 copies are made for mixed problems, or refreshed once made
+  // the FP32/FP16 copies are made for mixed problems, the FP64 SELL copy for
+  // FP64 ones; either is refreshed once made
   if (P.opts.precision == FEEC_PREC_MIXED || P.sys.val32.p) kkt_refresh_fp32(P.sys);
+  if ((P.opts.precision == FEEC_PREC_FP64 && P.opts.precond_storage == FEEC_PC_STORE_SELL) ||
+      P.sys.Psell64.p)
+    kkt_refresh_sell64(P.sys);
   kkt_set_chebyshev(P.sys, P.opts.cheb_degree, P.opts.cheb_lower);
   ctx->sync();
 }
@@ -403,7 +407,8 @@ namespace feec {
 Problem* problem_new(Geom* g, int bc, int k, const feec_solve_opts& o) {
   FEEC_REQUIRE(o.precision == FEEC_PREC_FP64 || o.precision == FEEC_PREC_MIXED, FEEC_E_ARG,
                "precision must be FEEC_PREC_FP64 or FEEC_PREC_MIXED");
-  FEEC_REQUIRE(o.precond_half >= 0 && o.precond_half <= 2, FEEC_E_ARG, "precond_half must be 0, 1 or 2");
+  FEEC_REQUIRE(o.precond_storage >= FEEC_PC_STORE_SELL && o.precond_storage <= FEEC_PC_STORE_CSR,
+               FEEC_E_ARG, "precond_storage must be a feec_pc_storage value");
   FEEC_REQUIRE(o.cheb_degree >= 0, FEEC_E_ARG, "cheb_degree must be >= 0");
   FEEC_REQUIRE(o.cheb_lower >= 0.0 && o.cheb_lower < 1.0, FEEC_E_ARG, "cheb_lower must be in [0, 1)");
   auto* P = new feec_problem_s();
@@ -496,7 +501,7 @@ void feec_solve_opts_default(feec_solve_opts* o) {
   o->graph_iters = 32;
   o->cheb_degree = 64;
   o->cheb_lower = 0.0;
-  o->precond_half = 1;
+  o->precond_storage = FEEC_PC_STORE_SELL;
 }
 
 feec_status feec_ctx_create(int device, feec_ctx** out) {
@@ -1070,19 +1075,16 @@ feec_status feec_problem_sizes(const feec_problem* p, int64_t sizes[3], int64_t*
 
 feec_status feec_problem_preconditioner(const feec_problem* p, int64_t* nnz, int32_t* cheb_degree,
                                         double lambda_max[2], int32_t* column_bytes,
-                                        int32_t* mixed_format, int64_t* mixed_stored,
-                                        int64_t* offdiag_nnz) {
+                                        int32_t* format, int64_t* stored, int64_t* offdiag_nnz) {
   return guard([&] {
     FEEC_REQUIRE(p, FEEC_E_ARG, "null problem");
     const KktSystem& s = p->sys;
-    const int half = p->opts.precond_half;
-    const int fmt = (half == 1 && s.Psell_ptr.p) ? FEEC_PC_SELL64_F16
-                    : (half && s.P.nnz)          ? FEEC_PC_CSR_F16
-                                                 : FEEC_PC_CSR_F32;
+    const int fmt = kkt_preconditioner_format(s, p->opts.precision, p->opts.precond_storage);
     if (nnz) *nnz = s.P.nnz;
     if (column_bytes) *column_bytes = s.Pcol16.p ? 2 : 4;
-    if (mixed_format) *mixed_format = fmt;
-    if (mixed_stored) *mixed_stored = fmt == FEEC_PC_SELL64_F16 ? s.sell_entries : s.P.nnz;
+    if (format) *format = fmt;
+    if (stored)
+      *stored = (fmt == FEEC_PC_SELL64_F16 || fmt == FEEC_PC_SELL64_F64) ? s.sell_entries : s.P.nnz;
     if (offdiag_nnz) *offdiag_nnz = s.P_offdiag;
     if (cheb_degree) *cheb_degree = s.cheb_m;
     if (lambda_max) {
@@ -1093,18 +1095,18 @@ feec_status feec_problem_preconditioner(const feec_problem* p, int64_t* nnz, int
 }
 
 feec_status feec_problem_apply_preconditioner(feec_problem* p, int32_t precision,
-                                              int32_t precond_half, const double* r,
-                                              double* z) {
+                                              int32_t storage, const double* r, double* z) {
   return guard([&] {
     FEEC_REQUIRE(p, FEEC_E_ARG, "null problem");
     FEEC_REQUIRE(precision == FEEC_PREC_FP64 || precision == FEEC_PREC_MIXED, FEEC_E_ARG,
                  "precision must be FEEC_PREC_FP64 or FEEC_PREC_MIXED");
-    FEEC_REQUIRE(precond_half >= 0 && precond_half <= 2, FEEC_E_ARG, "precond_half must be 0, 1 or 2");
+    FEEC_REQUIRE(storage >= FEEC_PC_STORE_SELL && storage <= FEEC_PC_STORE_CSR, FEEC_E_ARG,
+                 "storage must be a feec_pc_storage value");
     Ctx* ctx = p->g->mesh->ctx;
     const int64_t N = p->sys.n + p->nh;
     DBuf<double> rd(ctx, N), zd(ctx, N);
     copy_to_device(ctx, rd.p, r, N * sizeof(double));
-    kkt_apply_preconditioner(p->sys, precision, precond_half, rd.p, zd.p);
+    kkt_apply_preconditioner(p->sys, precision, storage, rd.p, zd.p);
     copy_to_host(ctx, z, zd.p, N * sizeof(double));
   });
 }
@@ -1190,8 +1192,8 @@ feec_status feec_problem_kernel_times(feec_problem* p, int reps, double out[6]) 
     out[1] = b * 1e-3 / reps;
     out[2] = time_minres_spmv(p->sys, true, reps);
     out[3] = time_minres_spmv(p->sys, false, reps);
-    out[4] = time_cheb_step(p->sys, true, reps, p->opts.precond_half);
-    out[5] = time_cheb_step(p->sys, false, reps, 0);
+    out[4] = time_cheb_step(p->sys, true, reps, p->opts.precond_storage);
+    out[5] = time_cheb_step(p->sys, false, reps, p->opts.precond_storage);
   });
 }
 

@@ -304,6 +304,10 @@ struct KktSystem {
   DBuf<int64_t> Psell_ptr;  // n_slices + 1 offsets into Psell (multiples of 64)
   DBuf<int32_t> sell_perm, sell_inv;
   int64_t sell_entries = 0, P_offdiag = 0;
+  // the same slices for the FP64 path: FP64 values and int16 offsets apart
+  DBuf<double> Psell64;
+  DBuf<int16_t> Psell_off;
+  DBuf<double> sdinv64;  // D^-1/2, n entries
   int64_t split = 0;
   int cheb_m = 0;
   DBuf<double> cheb;
@@ -313,19 +317,21 @@ struct KktSystem {
 // coefficients (m = 0 disables the polynomial).
 void kkt_set_chebyshev(KktSystem& s, int m, double lower = 0.0);
 void kkt_refresh_fp32(KktSystem& s);
+// FP64 SELL-64 copy of the scaled preconditioner from P's values (when the
+// pattern allows one).
+void kkt_refresh_sell64(KktSystem& s);
 // Solve s x = b (b, x device, FP64) to the options' tolerance.
 feec_report kkt_solve(KktSystem& s, const double* b_dev, double* x_dev, const feec_solve_opts& o,
                       SolverState** ws);
 void solver_state_free(SolverState* ws);
 void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y);
 double time_minres_spmv(KktSystem& s, bool fp32, int reps);
-// z = B r with the block preconditioner as a solve of `precision` applies it
-// (half: the mixed path's storage, as feec_solve_opts.precond_half); device
-// vectors of n + nh doubles.
-void kkt_apply_preconditioner(KktSystem& s, int precision, int half, const double* r, double* z);
-// half: 0 FP32 CSR, 1 the mixed path's default storage (SELL-64 when made),
-// 2 FP16 CSR
-double time_cheb_step(KktSystem& s, bool fp32, int reps, int half);
+// z = B r with the block preconditioner as a solve of `precision` with
+// `storage` (feec_pc_storage) applies it; device vectors of n + nh doubles.
+void kkt_apply_preconditioner(KktSystem& s, int precision, int storage, const double* r, double* z);
+// The feec_pc_format a solve of `precision` with `storage` applies.
+int kkt_preconditioner_format(const KktSystem& s, int precision, int storage);
+double time_cheb_step(KktSystem& s, bool fp32, int reps, int storage);
 KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
                      double rtol, int64_t max_iters);
 

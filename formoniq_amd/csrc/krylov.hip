@@ -551,12 +551,14 @@ __global__ void __launch_bounds__(TPB) k_cheb_finish(const T* __restrict__ r, co
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// ---- the mixed path's polynomial on the SELL-64 copy of D^-1/2 P D^-1/2 ----
+// ---- the polynomial on the SELL-64 copy of D^-1/2 P D^-1/2 ----
 // Three-term form of the same Chebyshev iteration (x_0 = 0, x_1 = f / theta):
 //   x_{k+2} = x_{k+1} + a_k (x_{k+1} - x_k) + b_k (f - A x_{k+1}),
 // with d_{k+1} = x_{k+2} - x_{k+1} and r_{k+1} = f - A x_{k+1} it is the
 // two-term step above, so (a_k, b_k) are the same per-block coefficients. Per
 // row it reads x_{k+1} (gathered), x_k and f and writes x_{k+2} over x_k.
+// Mixed path: FP32 vectors, entries packed {FP16 value | int16 offset << 16};
+// FP64 path: FP64 vectors, FP64 values and int16 offsets in two arrays.
 
 // Packed SELL entry: FP16 value in the low half, int16 (col - row) in the high.
 __device__ __forceinline__ float sell_val(uint32_t e) {
@@ -570,32 +572,53 @@ __device__ __forceinline__ int64_t sell_off(uint32_t e) { return (int64_t)(int16
 constexpr int SELL_UNROLL = 8;
 constexpr int SELL_WINDOW = 256;  // rows sorted by length inside windows of this many
 
+// Entry q of the slice storage as (value, column offset).
+struct SellF16 {
+  const uint32_t* __restrict__ e;
+  __device__ __forceinline__ void get(int64_t q, float& v, int64_t& o) const {
+    const uint32_t w = e[q];
+    v = sell_val(w);
+    o = sell_off(w);
+  }
+};
+struct SellF64 {
+  const double* __restrict__ v;
+  const int16_t* __restrict__ o;
+  __device__ __forceinline__ void get(int64_t q, double& val, int64_t& off) const {
+    val = v[q];
+    off = o[q];
+  }
+};
+
 // Off-diagonal part of row `row` (lane row % 64 of its slice) times x; the
 // summation order is fixed by the stored pattern.
-__device__ __forceinline__ float sell_row(const uint32_t* __restrict__ E,
-                                          const int64_t* __restrict__ sp, int64_t row,
-                                          const float* __restrict__ x) {
+template <class T, class E>
+__device__ __forceinline__ T sell_row(E E_, const int64_t* __restrict__ sp, int64_t row,
+                                      const T* __restrict__ x) {
   constexpr int U = SELL_UNROLL;
   const int64_t s = row >> 6;
   const int64_t base = sp[s];
   const int w = (int)((sp[s + 1] - base) >> 6);
-  const uint32_t* p = E + base + (row & 63);
-  float a[U];
+  const int64_t p = base + (row & 63);
+  T a[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) a[u] = 0.f;
+  for (int u = 0; u < U; ++u) a[u] = 0;
   int j = 0;
   for (; j + U <= w; j += U) {
-    uint32_t e[U];
+    T v[U];
+    int64_t o[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) e[u] = p[(int64_t)(j + u) * 64];
+    for (int u = 0; u < U; ++u) E_.get(p + (int64_t)(j + u) * 64, v[u], o[u]);
 #pragma unroll
-    for (int u = 0; u < U; ++u) a[u] += sell_val(e[u]) * x[row + sell_off(e[u])];
+    for (int u = 0; u < U; ++u) a[u] += v[u] * x[row + o[u]];
   }
   for (; j < w; ++j) {
-    const uint32_t e = p[(int64_t)j * 64];
-    a[0] += sell_val(e) * x[row + sell_off(e)];
+    T v;
+    int64_t o;
+    E_.get(p + (int64_t)j * 64, v, o);
+    a[0] += v * x[row + o];
   }
-  float acc = a[0];
+  T acc = a[0];
 #pragma unroll
   for (int u = 1; u < U; ++u) acc += a[u];
   return acc;
@@ -604,50 +627,62 @@ __device__ __forceinline__ float sell_row(const uint32_t* __restrict__ E,
 // f = sd r, x_0 = 0, x_1 = f / theta_block in the SELL row order (f[i] from
 // row perm[i]; the renumbering keeps both blocks in place, so i < split tells
 // the block)
-__global__ void __launch_bounds__(TPB) k_cheb3_init(const float* __restrict__ r,
-                                                    const float* __restrict__ sd,
+template <class T>
+__global__ void __launch_bounds__(TPB) k_cheb3_init(const T* __restrict__ r,
+                                                    const T* __restrict__ sd,
                                                     const int32_t* __restrict__ perm,
-                                                    float* __restrict__ f, float* __restrict__ x0,
-                                                    float* __restrict__ x1, int64_t n, int64_t split,
+                                                    T* __restrict__ f, T* __restrict__ x0,
+                                                    T* __restrict__ x1, int64_t n, int64_t split,
                                                     const double* __restrict__ coef,
                                                     const MinresS* __restrict__ S) {
   if (S && S->done) return;
-  const float s0 = (float)coef[0], s1 = (float)coef[1];
+  const T s0 = (T)coef[0], s1 = (T)coef[1];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t j = perm[i];
-    const float fi = r[j] * sd[j];
+    const T fi = r[j] * sd[j];
     f[i] = fi;
-    x0[i] = 0.f;
+    x0[i] = 0;
     x1[i] = fi * (i < split ? s0 : s1);
   }
 }
 
+// Workgroups are dispatched to the 8 XCDs of the chip round-robin (workgroup b
+// to XCD b % 8), each XCD with its own L2. xcd_block() maps them so that XCD x
+// runs the x-th contiguous eighth of the logical blocks: its L2 then fetches
+// only the part of the gathered vector its rows touch, not all of it. The grid
+// is a multiple of 8; logical blocks >= nblocks exit.
+constexpr unsigned XCDS = 8;
+__device__ __forceinline__ int64_t xcd_block() {
+  return (int64_t)(blockIdx.x % XCDS) * (gridDim.x / XCDS) + blockIdx.x / XCDS;
+}
+
 // One lane per row, one block per four 64-row slices: x0 <- x_{k+2}.
-__global__ void __launch_bounds__(TPB) k_cheb3_step(const uint32_t* __restrict__ E,
-                                                    const int64_t* __restrict__ sp, int64_t n,
-                                                    int64_t split, const float* __restrict__ f,
-                                                    const float* __restrict__ x1,
-                                                    float* __restrict__ x0,
+template <class T, class E>
+__global__ void __launch_bounds__(TPB) k_cheb3_step(E entries, const int64_t* __restrict__ sp,
+                                                    int64_t n, int64_t split,
+                                                    const T* __restrict__ f,
+                                                    const T* __restrict__ x1, T* __restrict__ x0,
                                                     const double* __restrict__ coef, int step,
                                                     const MinresS* __restrict__ S) {
   if (S && S->done) return;
-  const int64_t row = blockIdx.x * (int64_t)TPB + threadIdx.x;
+  const int64_t row = xcd_block() * TPB + threadIdx.x;
   if (row >= n) return;
   const double* c = coef + 2 + 4 * step + (row < split ? 0 : 2);
-  const float a = (float)c[0], b = (float)c[1];
-  const float acc = sell_row(E, sp, row, x1);
-  const float xi = x1[row];
+  const T a = (T)c[0], b = (T)c[1];
+  const T acc = sell_row<T>(entries, sp, row, x1);
+  const T xi = x1[row];
   x0[row] = xi + a * (xi - x0[row]) + b * (f[row] - (xi + acc));
 }
 
 // z = sd x_m on rows < n (x in the SELL row order) and z = r on the harmonic
 // rows; partial dot r.z
-__global__ void __launch_bounds__(TPB) k_cheb3_finish(const float* __restrict__ r,
-                                                      const float* __restrict__ x,
-                                                      const float* __restrict__ sd,
+template <class T>
+__global__ void __launch_bounds__(TPB) k_cheb3_finish(const T* __restrict__ r,
+                                                      const T* __restrict__ x,
+                                                      const T* __restrict__ sd,
                                                       const int32_t* __restrict__ inv,
-                                                      float* __restrict__ z, int64_t n, int64_t N,
+                                                      T* __restrict__ z, int64_t n, int64_t N,
                                                       const MinresS* __restrict__ S,
                                                       double* part) {
   if (S && S->done) return;
@@ -655,10 +690,10 @@ __global__ void __launch_bounds__(TPB) k_cheb3_finish(const float* __restrict__ 
   double v = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float ri = r[i];
-    const float zi = i < n ? sd[i] * x[inv[i]] : ri;
+    const T ri = r[i];
+    const T zi = i < n ? sd[i] * x[inv[i]] : ri;
     z[i] = zi;
-    v += (double)ri * (double)zi;
+    v += to_d(ri) * to_d(zi);
   }
   double t = block_reduce(v, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -759,8 +794,9 @@ inline unsigned spmv_grid(int64_t n, int lanes = LANES) {
   return (unsigned)std::min<int64_t>(NBLK, std::max<int64_t>(1, (n + rows_per_block - 1) / rows_per_block));
 }
 inline unsigned cheb_grid(int64_t n) { return spmv_grid(n, PLANES); }
-// One lane per row, no grid stride: every block covers four whole 64-row slices.
-inline unsigned sell_grid(int64_t n) { return grid_for(n); }
+// One lane per row, no grid stride: every block covers four whole 64-row
+// slices; rounded up to a multiple of the XCD count for xcd_block().
+inline unsigned sell_grid(int64_t n) { return (grid_for(n) + XCDS - 1) / XCDS * XCDS; }
 
 }  // namespace
 
@@ -785,7 +821,10 @@ struct MinresWork {
   const int16_t* pcol16 = nullptr;  // band-offset columns of P, when they fit
   const _Float16* pval16 = nullptr;  // scaled P in FP16 (FP32 path), or null
   const T* sd = nullptr;             // D^-1/2 for the scaled polynomial
-  const uint32_t* psell = nullptr;   // SELL-64 scaled P (FP32 path), or null
+  // SELL-64 copy of the scaled P (FP16 entries on the FP32 path, FP64 on the FP64 path)
+  using SellE = std::conditional_t<std::is_same_v<T, float>, SellF16, SellF64>;
+  bool has_sell = false;
+  SellE sell{};
   const int64_t* psell_ptr = nullptr;
   const int32_t* sell_perm = nullptr;
   const int32_t* sell_inv = nullptr;
@@ -841,13 +880,20 @@ static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval,
                                 const _Float16* pval16 = nullptr, const T* sd = nullptr,
                                 bool sell = false) {
   int m = (s.P.nnz > 0 && s.cheb_m > 0 && s.cheb.p) ? s.cheb_m : 0;
-  const uint32_t* psell = sell ? s.Psell.p : nullptr;
+  typename MinresWork<T>::SellE sel{};
+  if (sell) {
+    if constexpr (std::is_same_v<T, float>)
+      sel = SellF16{s.Psell.p};
+    else
+      sel = SellF64{s.Psell64.p, s.Psell_off.p};
+  }
   const int64_t* psell_ptr = sell ? s.Psell_ptr.p : nullptr;
   const int32_t* sell_perm = sell ? s.sell_perm.p : nullptr;
   const int32_t* sell_inv = sell ? s.sell_inv.p : nullptr;
   if (w.graph && (w.pval != pval || w.coef != s.cheb.p || w.prp != s.P.row_ptr.p ||
                   w.pcol != s.P.col.p || w.pcol16 != s.Pcol16.p || w.pval16 != pval16 ||
-                  w.sd != sd || w.cheb_m != m || w.psell != psell || w.psell_ptr != psell_ptr ||
+                  w.sd != sd || w.cheb_m != m || w.has_sell != sell ||
+                  std::memcmp(&w.sell, &sel, sizeof(sel)) != 0 || w.psell_ptr != psell_ptr ||
                   w.sell_perm != sell_perm || w.sell_inv != sell_inv)) {
     (void)hipGraphExecDestroy(w.graph);
     w.graph = nullptr;
@@ -860,11 +906,39 @@ static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval,
   w.pval = pval;
   w.pval16 = pval16;
   w.sd = sd;
-  w.psell = psell;
+  w.has_sell = sell;
+  w.sell = sel;
   w.psell_ptr = psell_ptr;
   w.sell_perm = sell_perm;
   w.sell_inv = sell_inv;
   w.coef = s.cheb.p;
+}
+
+// One Chebyshev step on the bound storage. SELL copy (three-term): b <- x_{k+2}
+// from a = x_{k+1} and b = x_k (z unused). CSR (two-term): direction a, next
+// direction into b, z accumulated; scaled mode runs on D^-1/2 P D^-1/2 with
+// FP16 values, otherwise on D^-1 P.
+template <class T>
+static void cheb_step(Enq& q, MinresWork<T>& w, const T* a, T* b, T* z, int k, const MinresS* S) {
+  const int64_t n = w.n;
+  if (w.has_sell && w.sd) {
+    q(k_cheb3_step<T, typename MinresWork<T>::SellE>, sell_grid(n), TPB, w.sell, w.psell_ptr, n,
+      w.split, (const T*)w.res.p, a, b, w.coef, k, S);
+    return;
+  }
+  const bool scaled = w.pval16 && w.sd;
+  if (scaled && w.pcol16)
+    q(k_cheb_step<T, int16_t, _Float16, true>, cheb_grid(n), TPB, w.prp, w.pcol16, w.pval16, n,
+      w.split, w.dinv.p, a, w.res.p, z, b, w.coef, k, S);
+  else if (scaled)
+    q(k_cheb_step<T, int32_t, _Float16, true>, cheb_grid(n), TPB, w.prp, w.pcol, w.pval16, n,
+      w.split, w.dinv.p, a, w.res.p, z, b, w.coef, k, S);
+  else if (w.pcol16)
+    q(k_cheb_step<T, int16_t>, cheb_grid(n), TPB, w.prp, w.pcol16, w.pval, n, w.split, w.dinv.p, a,
+      w.res.p, z, b, w.coef, k, S);
+  else
+    q(k_cheb_step<T, int32_t>, cheb_grid(n), TPB, w.prp, w.pcol, w.pval, n, w.split, w.dinv.p, a,
+      w.res.p, z, b, w.coef, k, S);
 }
 
 // z = P r with the Chebyshev block preconditioner, plus partial sums of r.z
@@ -873,42 +947,26 @@ template <class T>
 static void cheb_apply(Enq& q, MinresWork<T>& w, const T* r, T* z, const MinresS* S,
                        double* part) {
   const int64_t n = w.n, N = w.n + w.nh;
-  if constexpr (std::is_same_v<T, float>) {
-    if (w.psell && w.sd) {
-      // three-term polynomial on the SELL-64 copy; x_{k+2} overwrites x_k
-      q(k_cheb3_init, red_grid(n), TPB, r, w.sd, w.sell_perm, w.res.p, w.d0.p, w.d1.p, n, w.split,
-        w.coef, S);
-      float* cur = w.d1.p;
-      float* prev = w.d0.p;
-      for (int k = 0; k + 1 < w.cheb_m; ++k) {
-        q(k_cheb3_step, sell_grid(n), TPB, w.psell, w.psell_ptr, n, w.split, (const float*)w.res.p,
-          (const float*)cur, prev, w.coef, k, S);
-        std::swap(cur, prev);
-      }
-      q(k_cheb3_finish, red_grid(N), TPB, r, (const float*)cur, w.sd, w.sell_inv, z, n, N, S, part);
-      return;
+  if (w.has_sell && w.sd) {
+    // three-term polynomial on the SELL-64 copy; x_{k+2} overwrites x_k
+    q(k_cheb3_init<T>, red_grid(n), TPB, r, w.sd, w.sell_perm, w.res.p, w.d0.p, w.d1.p, n, w.split,
+      w.coef, S);
+    T* cur = w.d1.p;
+    T* prev = w.d0.p;
+    for (int k = 0; k + 1 < w.cheb_m; ++k) {
+      cheb_step<T>(q, w, cur, prev, nullptr, k, S);
+      std::swap(cur, prev);
     }
+    q(k_cheb3_finish<T>, red_grid(N), TPB, r, (const T*)cur, w.sd, w.sell_inv, z, n, N, S, part);
+    return;
   }
-  // scaled mode: the polynomial runs on D^-1/2 P D^-1/2 (FP16 values) with
-  // input and output scaled by D^-1/2; otherwise on D^-1 P
   const bool scaled = w.pval16 && w.sd;
   q(k_cheb_init<T>, red_grid(N), TPB, r, scaled ? w.sd : w.dinv.p, w.res.p, w.d0.p, z, n, N,
     w.split, w.coef, S);
   T* d = w.d0.p;
   T* dn = w.d1.p;
   for (int k = 0; k + 1 < w.cheb_m; ++k) {
-    if (scaled && w.pcol16)
-      q(k_cheb_step<T, int16_t, _Float16, true>, cheb_grid(n), TPB, w.prp, w.pcol16, w.pval16, n,
-        w.split, w.dinv.p, d, w.res.p, z, dn, w.coef, k, S);
-    else if (scaled)
-      q(k_cheb_step<T, int32_t, _Float16, true>, cheb_grid(n), TPB, w.prp, w.pcol, w.pval16, n,
-        w.split, w.dinv.p, d, w.res.p, z, dn, w.coef, k, S);
-    else if (w.pcol16)
-      q(k_cheb_step<T, int16_t>, cheb_grid(n), TPB, w.prp, w.pcol16, w.pval, n, w.split, w.dinv.p, d,
-        w.res.p, z, dn, w.coef, k, S);
-    else
-      q(k_cheb_step<T, int32_t>, cheb_grid(n), TPB, w.prp, w.pcol, w.pval, n, w.split, w.dinv.p, d,
-        w.res.p, z, dn, w.coef, k, S);
+    cheb_step<T>(q, w, d, dn, z, k, S);
     std::swap(d, dn);
   }
   q(k_cheb_finish<T>, red_grid(N), TPB, r, d, z, scaled ? w.sd : (const T*)nullptr, n, N, S, part);
@@ -1114,6 +1172,41 @@ __global__ void k_sell_fill(const int64_t* __restrict__ rp, const int32_t* __res
   }
 }
 
+// FP64 values and int16 offsets of the same slices (the FP64 path).
+__global__ void k_sell_fill64(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                              const double* __restrict__ val, const double* __restrict__ inv_diag,
+                              const int64_t* __restrict__ sp, const int32_t* __restrict__ perm,
+                              const int32_t* __restrict__ inv, int64_t n, int64_t nrows_padded,
+                              double* __restrict__ V, int16_t* __restrict__ O) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrows_padded;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = i >> 6, base = sp[s] + (i & 63);
+    const int64_t w = (sp[s + 1] - sp[s]) >> 6;
+    int64_t j = 0;
+    if (i < n) {
+      const int64_t row = perm[i];
+      const double si = sqrt(fabs(inv_diag[row]));
+      for (int64_t p = rp[row]; p < rp[row + 1]; ++p) {
+        const int64_t c = col[p];
+        if (c == row) continue;
+        V[base + j * 64] = val[p] * si * sqrt(fabs(inv_diag[c]));
+        O[base + j * 64] = (int16_t)(inv[c] - i);
+        ++j;
+      }
+    }
+    for (; j < w; ++j) {
+      V[base + j * 64] = 0.0;
+      O[base + j * 64] = 0;
+    }
+  }
+}
+
+__global__ void k_scale_sd64(const double* __restrict__ inv_diag, int64_t n, double* __restrict__ sd) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    sd[i] = sqrt(fabs(inv_diag[i]));
+}
+
 // D^-1/2 in FP32
 __global__ void k_scale_sd(const double* __restrict__ inv_diag, int64_t n, float* __restrict__ sd) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -1156,6 +1249,26 @@ void kkt_refresh_fp32(KktSystem& s) {
                                                     s.sell_inv.p, s.n, padded, s.Psell.p);
     }
   }
+  FEEC_HIP(hipGetLastError());
+}
+
+void kkt_refresh_sell64(KktSystem& s) {
+  if (!s.P.nnz) return;
+  ensure_band(s);
+  if (!s.Psell_ptr.p) return;
+  hipStream_t st = s.ctx->stream;
+  const int64_t n = s.n, E = std::max<int64_t>(s.sell_entries, 1);
+  if (!s.Psell64.p || (int64_t)s.Psell64.n != E) {
+    s.Psell64.alloc(s.ctx, E);
+    s.Psell_off.alloc(s.ctx, E);
+  }
+  if (!s.sdinv64.p || (int64_t)s.sdinv64.n != std::max<int64_t>(n, 1))
+    s.sdinv64.alloc(s.ctx, std::max<int64_t>(n, 1));
+  k_scale_sd64<<<grid_for(n), TPB, 0, st>>>(s.inv_diag.p, n, s.sdinv64.p);
+  const int64_t padded = (n + 63) / 64 * 64;
+  k_sell_fill64<<<grid_for(padded), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p, s.inv_diag.p,
+                                                  s.Psell_ptr.p, s.sell_perm.p, s.sell_inv.p, n,
+                                                  padded, s.Psell64.p, s.Psell_off.p);
   FEEC_HIP(hipGetLastError());
 }
 
@@ -1338,6 +1451,35 @@ static double device_norm(Ctx* ctx, SolverState& ss, const double* a, int64_t n)
   return std::sqrt(h);
 }
 
+// The polynomial's matrix for each path and feec_pc_storage (the SELL copy
+// falls back to CSR where the pattern allows none), made on first use.
+static void bind_fp64(KktSystem& s, MinresWork<double>& w, int storage) {
+  if (storage == FEEC_PC_STORE_SELL && s.P.nnz && !s.Psell64.p) kkt_refresh_sell64(s);
+  if (storage == FEEC_PC_STORE_SELL && s.Psell64.p)
+    bind_preconditioner(s, w, s.P.val.p, nullptr, s.sdinv64.p, true);
+  else
+    bind_preconditioner(s, w, s.P.val.p);
+}
+
+static void bind_fp32(KktSystem& s, MinresWork<float>& w, int storage) {
+  if (!s.val32.p) kkt_refresh_fp32(s);
+  if (storage == FEEC_PC_STORE_CSR_HALF && s.P.nnz && !s.P16.p) kkt_refresh_half_csr(s);
+  if (storage == FEEC_PC_STORE_SELL && s.Psell.p)
+    bind_preconditioner(s, w, s.P32.p, nullptr, s.sdinv32.p, true);
+  else if (storage != FEEC_PC_STORE_CSR && s.P16.p)
+    bind_preconditioner(s, w, s.P32.p, s.P16.p, s.sdinv32.p);
+  else
+    bind_preconditioner(s, w, s.P32.p);
+}
+
+int kkt_preconditioner_format(const KktSystem& s, int precision, int storage) {
+  if (precision == FEEC_PREC_FP64)
+    return (storage == FEEC_PC_STORE_SELL && s.Psell_ptr.p) ? FEEC_PC_SELL64_F64 : FEEC_PC_CSR_F64;
+  if (storage == FEEC_PC_STORE_SELL && s.Psell_ptr.p) return FEEC_PC_SELL64_F16;
+  if (storage != FEEC_PC_STORE_CSR && s.P.nnz) return FEEC_PC_CSR_F16;
+  return FEEC_PC_CSR_F32;
+}
+
 feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_solve_opts& o,
                       SolverState** wsp) {
   Ctx* ctx = s.ctx;
@@ -1370,7 +1512,7 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
     // correction steps until the true residual meets rtol.
     MinresWork<double>& w = ss.w64;
     minres_alloc(ctx, w, n, nh);
-    bind_preconditioner(s, w, s.P.val.p);
+    bind_fp64(s, w, o.precond_storage);
     FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
     if (nh) FEEC_HIP(hipMemcpyAsync(w.border.p, s.border.p, nh * n * sizeof(double), hipMemcpyDeviceToDevice, st));
     FEEC_HIP(hipMemcpyAsync(w.b.p, b, N * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1399,16 +1541,7 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
   } else {
     MinresWork<float>& w = ss.w32;
     minres_alloc(ctx, w, n, nh);
-    if (!s.val32.p) kkt_refresh_fp32(s);
-    // precond_half: 1 = FP16 SELL-64 copy (FP16 CSR when there is none),
-    // 2 = FP16 CSR, 0 = FP32 CSR
-    if (o.precond_half == 2 && s.P.nnz && !s.P16.p) kkt_refresh_half_csr(s);
-    if (o.precond_half == 1 && s.Psell.p)
-      bind_preconditioner(s, w, s.P32.p, nullptr, s.sdinv32.p, true);
-    else if (o.precond_half && s.P16.p)
-      bind_preconditioner(s, w, s.P32.p, s.P16.p, s.sdinv32.p);
-    else
-      bind_preconditioner(s, w, s.P32.p);
+    bind_fp32(s, w, o.precond_storage);
     FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag32.p, N * sizeof(float), hipMemcpyDeviceToDevice, st));
     if (nh) k_f64_to_f32<<<grid_for(nh * n), TPB, 0, st>>>(s.border.p, nh * n, w.border.p);
     // r = b
@@ -1494,9 +1627,9 @@ double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
 }
 
 // z = B r, B the block preconditioner exactly as a solve of the given
-// precision applies it (`half` selects the mixed path's storage as
-// feec_solve_opts.precond_half does); r and z are N device doubles.
-void kkt_apply_preconditioner(KktSystem& s, int precision, int half, const double* r,
+// precision with `storage` (feec_pc_storage) applies it; r and z are N device
+// doubles.
+void kkt_apply_preconditioner(KktSystem& s, int precision, int storage, const double* r,
                               double* z) {
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
@@ -1504,7 +1637,7 @@ void kkt_apply_preconditioner(KktSystem& s, int precision, int half, const doubl
   if (precision == FEEC_PREC_FP64) {
     MinresWork<double> w;
     minres_alloc(ctx, w, n, s.nh);
-    bind_preconditioner(s, w, s.P.val.p);
+    bind_fp64(s, w, storage);
     FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
     FEEC_HIP(hipMemcpyAsync(w.b.p, r, N * sizeof(double), hipMemcpyDeviceToDevice, st));
     Enq q{st};
@@ -1516,16 +1649,9 @@ void kkt_apply_preconditioner(KktSystem& s, int precision, int half, const doubl
     ctx->sync();
     return;
   }
-  if (!s.val32.p) kkt_refresh_fp32(s);
-  if (half == 2 && s.P.nnz && !s.P16.p) kkt_refresh_half_csr(s);
   MinresWork<float> w;
   minres_alloc(ctx, w, n, s.nh);
-  if (half == 1 && s.Psell.p)
-    bind_preconditioner(s, w, s.P32.p, nullptr, s.sdinv32.p, true);
-  else if (half && s.P16.p)
-    bind_preconditioner(s, w, s.P32.p, s.P16.p, s.sdinv32.p);
-  else
-    bind_preconditioner(s, w, s.P32.p);
+  bind_fp32(s, w, storage);
   FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag32.p, N * sizeof(float), hipMemcpyDeviceToDevice, st));
   k_f64_to_f32<<<grid_for(N), TPB, 0, st>>>(r, N, w.b.p);
   Enq q{st};
@@ -1541,62 +1667,37 @@ void kkt_apply_preconditioner(KktSystem& s, int precision, int half, const doubl
 
 // Average duration of one Chebyshev step (the preconditioner's fused SpMV)
 // on this system, FP32 or FP64; 0 when the system has no polynomial.
-double time_cheb_step(KktSystem& s, bool fp32, int reps, int half) {
+double time_cheb_step(KktSystem& s, bool fp32, int reps, int storage) {
   if (s.P.nnz == 0 || s.cheb_m < 2 || !s.cheb.p) return 0.0;
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
-  const int64_t n = s.n;
-  if (fp32 && !s.P32.p) kkt_refresh_fp32(s);
-  if (fp32 && half == 2 && !s.P16.p) kkt_refresh_half_csr(s);
-  const bool sell = fp32 && half == 1 && s.Psell.p;
+  const int64_t n = s.n, N = s.n + s.nh;
   hipEvent_t e0, e1;
   FEEC_HIP(hipEventCreate(&e0));
   FEEC_HIP(hipEventCreate(&e1));
-  auto run = [&](auto tag, const auto* pval, const auto* dinv, const _Float16* p16) {
-    using T = decltype(tag);
-    DBuf<T> d(ctx, n), res(ctx, n), z(ctx, n), dn(ctx, n);
-    FEEC_HIP(hipMemsetAsync(d.p, 0, n * sizeof(T), st));
-    FEEC_HIP(hipMemsetAsync(res.p, 0, n * sizeof(T), st));
-    FEEC_HIP(hipMemsetAsync(z.p, 0, n * sizeof(T), st));
+  auto run = [&](auto& w) {
+    using T = std::remove_reference_t<decltype(*w.res.p)>;
+    for (T* v : {w.res.p, w.d0.p, w.d1.p, w.z.p})
+      FEEC_HIP(hipMemsetAsync(v, 0, N * sizeof(T), st));
+    Enq q{st};
     const MinresS* none = nullptr;
-    auto step = [&] {
-      if constexpr (std::is_same_v<T, float>) {
-        if (sell) {
-          k_cheb3_step<<<sell_grid(n), TPB, 0, st>>>(s.Psell.p, s.Psell_ptr.p, n, s.split, res.p,
-                                                     d.p, z.p, s.cheb.p, 0, none);
-          return;
-        }
-        if (p16) {
-          if (s.Pcol16.p)
-            k_cheb_step<T, int16_t, _Float16, true><<<cheb_grid(n), TPB, 0, st>>>(
-                s.P.row_ptr.p, s.Pcol16.p, p16, n, s.split, dinv, d.p, res.p, z.p, dn.p, s.cheb.p, 0,
-                none);
-          else
-            k_cheb_step<T, int32_t, _Float16, true><<<cheb_grid(n), TPB, 0, st>>>(
-                s.P.row_ptr.p, s.P.col.p, p16, n, s.split, dinv, d.p, res.p, z.p, dn.p, s.cheb.p, 0,
-                none);
-          return;
-        }
-      }
-      if (s.Pcol16.p)
-        k_cheb_step<T, int16_t><<<cheb_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.Pcol16.p, pval, n,
-                                                             s.split, dinv, d.p, res.p, z.p, dn.p,
-                                                             s.cheb.p, 0, none);
-      else
-        k_cheb_step<T, int32_t><<<cheb_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, pval, n,
-                                                             s.split, dinv, d.p, res.p, z.p, dn.p,
-                                                             s.cheb.p, 0, none);
-    };
-    step();
+    cheb_step<T>(q, w, w.d0.p, w.d1.p, w.z.p, 0, none);
     FEEC_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < reps; ++i) step();
+    for (int i = 0; i < reps; ++i) cheb_step<T>(q, w, w.d0.p, w.d1.p, w.z.p, 0, none);
     FEEC_HIP(hipEventRecord(e1, st));
     FEEC_HIP(hipEventSynchronize(e1));
   };
-  if (fp32)
-    run(float{}, s.P32.p, s.inv_diag32.p, half ? s.P16.p : nullptr);
-  else
-    run(double{}, s.P.val.p, s.inv_diag.p, nullptr);
+  if (fp32) {
+    MinresWork<float> w;
+    minres_alloc(ctx, w, n, s.nh);
+    bind_fp32(s, w, storage);
+    run(w);
+  } else {
+    MinresWork<double> w;
+    minres_alloc(ctx, w, n, s.nh);
+    bind_fp64(s, w, storage);
+    run(w);
+  }
   FEEC_HIP(hipGetLastError());
   float ms = 0.f;
   FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));

@@ -80,6 +80,17 @@ typedef enum feec_precision {
   FEEC_PREC_MIXED = 1  /* FP32 SpMV/Krylov inside FP64 iterative refinement */
 } feec_precision;
 
+/* Storage of the preconditioner matrix the Chebyshev polynomial runs on. */
+typedef enum {
+  FEEC_PC_STORE_SELL = 0,     /* D^-1/2 hdif_gram D^-1/2 as SELL-64 slices, off-diagonal entries
+                                 with int16 column offsets, unit diagonal implied: FP16 values on
+                                 the mixed path, FP64 on the FP64 path (CSR when an offset exceeds
+                                 +-32767) */
+  FEEC_PC_STORE_CSR_HALF = 1, /* mixed path: D^-1/2 hdif_gram D^-1/2 in CSR with FP16 values;
+                                 FP64 path: as FEEC_PC_STORE_CSR */
+  FEEC_PC_STORE_CSR = 2       /* D^-1 hdif_gram in CSR with FP32 (mixed) or FP64 values */
+} feec_pc_storage;
+
 typedef struct feec_solve_opts {
   double rtol;           /* target true relative residual |b-Ax|/|b| (default 1e-10) */
   int64_t max_iters;     /* Krylov iteration budget over all refinements (default 200000) */
@@ -93,11 +104,8 @@ typedef struct feec_solve_opts {
   double cheb_lower;     /* lower end of the Chebyshev interval as a fraction of its upper end
                             (1.05 lambda_max); 0 = by degree: 1e-2 up to degree 4, 3e-3 up to 8,
                             1.5e-3 up to 12, 1e-3 up to 32, 5e-4 up to 48, else 2.5e-4 (default 0) */
-  int32_t precond_half;  /* mixed: storage of the matrix the polynomial runs on, FP32 arithmetic
-                            throughout. 1: D^-1/2 hdif_gram D^-1/2 in FP16 as SELL-64 slices
-                            with the unit diagonal implied (FP16 CSR when a column offset
-                            exceeds +-32767) (default); 2: the same in FP16 CSR; 0: D^-1
-                            hdif_gram with FP32 values in CSR */
+  int32_t precond_storage; /* feec_pc_storage: how the matrix the polynomial runs on is
+                              stored (default FEEC_PC_STORE_SELL) */
 } feec_solve_opts;
 
 typedef struct feec_report {
@@ -210,31 +218,35 @@ feec_status feec_problem_solve_device(feec_problem *prob, const double *source_d
 feec_status feec_problem_solve(feec_problem *prob, const double *source_host, feec_report *report);
 feec_status feec_problem_solution(feec_problem *prob, double *sigma, double *u, double *p);
 feec_status feec_problem_sizes(const feec_problem *prob, int64_t sizes[3], int64_t *nnz);
-/* Storage of the mixed path's preconditioner matrix (feec_problem_preconditioner). */
+/* Preconditioner matrix as a solve uses it (feec_problem_preconditioner). */
 typedef enum {
   FEEC_PC_CSR_F32 = 0,    /* D^-1 hdif_gram, FP32 values, CSR */
   FEEC_PC_CSR_F16 = 1,    /* D^-1/2 hdif_gram D^-1/2, FP16 values, CSR */
-  FEEC_PC_SELL64_F16 = 2  /* the same, off-diagonal only, packed {FP16, int16 offset} in 64-row
+  FEEC_PC_SELL64_F16 = 2, /* the same, off-diagonal only, packed {FP16, int16 offset} in 64-row
                              SELL slices */
+  FEEC_PC_CSR_F64 = 3,    /* D^-1 hdif_gram, FP64 values, CSR */
+  FEEC_PC_SELL64_F64 = 4  /* D^-1/2 hdif_gram D^-1/2, off-diagonal only, FP64 values and int16
+                             offsets in 64-row SELL slices */
 } feec_pc_format;
 
 /* The block preconditioner blockdiag(hdif_gram(k-1), hdif_gram(k)) as built:
  * its nonzeros, the Chebyshev degree in use (0 = Jacobi), the estimated
  * largest eigenvalue of D^-1 hdif_gram per block, the bytes per stored CSR
  * column index (2 when every |col - row| fits a 16-bit offset, else 4), the
- * mixed path's storage (feec_pc_format), the entries that storage holds
- * (with SELL padding) and the off-diagonal nonzeros; any pointer may be NULL. */
+ * format a solve with the problem's options applies (feec_pc_format), the
+ * entries that format stores (with SELL padding) and the off-diagonal
+ * nonzeros; any pointer may be NULL. */
 feec_status feec_problem_preconditioner(const feec_problem *prob, int64_t *nnz,
                                         int32_t *cheb_degree, double lambda_max[2],
-                                        int32_t *column_bytes, int32_t *mixed_format,
-                                        int64_t *mixed_stored, int64_t *offdiag_nnz);
+                                        int32_t *column_bytes, int32_t *format, int64_t *stored,
+                                        int64_t *offdiag_nnz);
 /* Diagnostics (no counterpart in the reference; the reference applies its
  * preconditioner inside MINRES, krylov.rs): z = B r for host vectors r, z of
  * the KKT order (sigma, u, harmonic rows; sizes[0] + sizes[1] + sizes[2]
  * doubles), with B the block preconditioner exactly as a solve of `precision`
- * applies it and `precond_half` selecting the mixed path's storage. */
+ * with `storage` (feec_pc_storage) applies it. */
 feec_status feec_problem_apply_preconditioner(feec_problem *prob, int32_t precision,
-                                              int32_t precond_half, const double *r, double *z);
+                                              int32_t storage, const double *r, double *z);
 /* Device pointer to an N_k scratch vector for feec_problem_solve_device. */
 feec_status feec_problem_source_buffer(feec_problem *prob, double **device_ptr);
 /* Copy an N_k host source into that device buffer. */

@@ -99,20 +99,21 @@ def test_c1_configuration_fp64_and_mixed():
         assert rep.precond_residual <= 1e-10
 
 
-def test_mixed_preconditioner_storage_variants_agree():
-    """The mixed path's polynomial on the FP16 scaled preconditioner as SELL-64
-    slices (precond_half=1, the default), as FP16 CSR (2) and on the FP32 CSR
-    one (0) changes only the Krylov path: all meet the FP64 certificate and
-    agree with the FP64 solve."""
+@pytest.mark.parametrize("precision", [fa.MIXED, fa.FP64])
+def test_preconditioner_storage_variants_agree(precision):
+    """The polynomial on the SELL-64 copy of D^-1/2 P D^-1/2 (PC_SELL, the
+    default; FP16 values on the mixed path, FP64 on the FP64 path), on the
+    FP16 CSR copy (PC_CSR_HALF) and on D^-1 P in CSR (PC_CSR) changes only the
+    Krylov path: all meet the FP64 certificate and agree with the FP64 solve."""
     _, _, gcx, geo, _, _ = both(3, 5, 0.05, seed=11)
     gc = fa.WhitneyComplex(geo)
     f = fa.normal_vector(gcx.nsimplices(1), 12)
-    _, u64, _, r64 = fa.solve_source(gc, f, 1, fa.solve_opts(precision=fa.FP64))
+    _, u64, _, r64 = fa.solve_source(gc, f, 1, fa.solve_opts(precision=fa.FP64, precond_storage=fa.PC_CSR))
     assert r64.residual <= 1e-10
-    for half in (0, 1, 2):
-        _, u, _, rep = fa.solve_source(gc, f, 1, fa.solve_opts(precision=fa.MIXED, precond_half=half))
-        assert rep.residual <= 1e-10, (half, rep.residual)
-        assert np.linalg.norm(u - u64) <= 1e-8 * np.linalg.norm(u64), half
+    for store in (fa.PC_SELL, fa.PC_CSR_HALF, fa.PC_CSR):
+        _, u, _, rep = fa.solve_source(gc, f, 1, fa.solve_opts(precision=precision, precond_storage=store))
+        assert rep.residual <= 1e-10, (store, rep.residual)
+        assert np.linalg.norm(u - u64) <= 1e-8 * np.linalg.norm(u64), store
 
 
 @pytest.mark.parametrize("d,k,n", [(1, 0, 12), (1, 1, 12), (2, 1, 7), (2, 2, 7), (3, 0, 4), (3, 1, 4),
@@ -120,9 +121,10 @@ def test_mixed_preconditioner_storage_variants_agree():
 def test_preconditioner_application_per_storage(d, k, n):
     """z = B r as each path applies it, against the oracle's FP64 Chebyshev
     block preconditioner (same power-iteration start, same coefficients):
-    FP64 to roundoff; the SELL-64 FP16 copy equal to the FP16 CSR one up to
-    FP32 summation order (a wrong slice offset, lane, packed offset or padding
-    entry would be an O(1) difference); FP16/FP32 storage within its rounding."""
+    FP64 on CSR and on the SELL-64 copy to roundoff; the FP16 SELL-64 copy
+    equal to the FP16 CSR one up to FP32 summation order (a wrong slice offset,
+    lane, renumbering, packed offset or padding entry would be an O(1)
+    difference); FP16/FP32 storage within its rounding."""
     m = 8
     ocx, oL, gcx, geo, _, _ = both(d, n, 0.1, seed=20 + d + k)
     oc = o.WhitneyComplex(ocx, oL)
@@ -131,19 +133,20 @@ def test_preconditioner_application_per_storage(d, k, n):
     info = prob.preconditioner()
     ns, nu, nh = prob.sizes
     n_p = ns + nu
-    assert info["mixed_format"] == "sell64_f16" and info["degree"] == m
+    assert info["format"] == "sell64_f16" and info["degree"] == m
     assert info["offdiag_nnz"] == info["nnz"] - n_p  # hdif_gram keeps every diagonal entry
-    assert info["offdiag_nnz"] <= info["mixed_stored"] and info["mixed_stored"] % 64 == 0
+    assert info["offdiag_nnz"] <= info["stored"] and info["stored"] % 64 == 0
     prec = o.chebyshev_jacobi(oc, k, nh, m)
     np.testing.assert_allclose(info["lambda_max"][1], prec.lambda_max[1], rtol=1e-10)
     r = np.sin(np.arange(n_p + nh) * 0.61) + 0.3
     want = prec(r)
     scale = np.abs(want).max()
-    z64 = prob.apply_preconditioner(r, precision=fa.FP64)
-    assert np.abs(z64 - want).max() <= 1e-11 * scale
-    z_sell = prob.apply_preconditioner(r, precond_half=1)
-    z_h = prob.apply_preconditioner(r, precond_half=2)
-    z_32 = prob.apply_preconditioner(r, precond_half=0)
+    for store in (fa.PC_SELL, fa.PC_CSR):
+        z64 = prob.apply_preconditioner(r, precision=fa.FP64, storage=store)
+        assert np.abs(z64 - want).max() <= 1e-11 * scale, store
+    z_sell = prob.apply_preconditioner(r, storage=fa.PC_SELL)
+    z_h = prob.apply_preconditioner(r, storage=fa.PC_CSR_HALF)
+    z_32 = prob.apply_preconditioner(r, storage=fa.PC_CSR)
     e_sell_h = np.abs(z_sell - z_h).max() / scale
     e_h = np.abs(z_h - want).max() / scale
     e_32 = np.abs(z_32 - want).max() / scale
@@ -153,6 +156,8 @@ def test_preconditioner_application_per_storage(d, k, n):
     # harmonic rows pass through (rounded to FP32 on the mixed path)
     np.testing.assert_array_equal(z64[n_p:], r[n_p:])
     np.testing.assert_array_equal(z_sell[n_p:], r[n_p:].astype(np.float32).astype(np.float64))
+    f64 = fa.Problem(gc, k, fa.solve_opts(precision=fa.FP64, cheb_degree=m))
+    assert f64.preconditioner()["format"] == "sell64_f64"
 
 
 def test_resident_problem_reassembly_and_device_source():
