@@ -226,7 +226,8 @@ def run_c5(args):
             "cells": int(sum(x.ncells for x in res)),
             "slowest": [{"dim": q.dim, "grade": q.grade, "n": q.n, "cells": int(x.ncells),
                          "dofs": int(x.ndofs), "iters": int(x.iters), "latency_ms": float(l * 1e3),
-                         "service_ms": float((x.finish_time - x.admit_time) * 1e3)}
+                         "service_ms": float((x.finish_time - x.admit_time) * 1e3),
+                         "build_ms": float(x.build_seconds * 1e3), "solve_ms": float(x.solve_seconds * 1e3)}
                         for l, q, x in slow],
         }
     line = {"config": "C5", "workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, d in 2..3, "

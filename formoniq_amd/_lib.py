@@ -46,7 +46,8 @@ class Result(C.Structure):
     _fields_ = [("ticket", C.c_int64), ("ndofs", C.c_int64), ("ncells", C.c_int64),
                 ("iters", C.c_int64), ("refinements", C.c_int32), ("status", C.c_int32),
                 ("residual", C.c_double), ("u_norm", C.c_double), ("u_checksum", C.c_double),
-                ("admit_time", C.c_double), ("finish_time", C.c_double), ("latency", C.c_double)]
+                ("admit_time", C.c_double), ("finish_time", C.c_double), ("latency", C.c_double),
+                ("build_seconds", C.c_double), ("solve_seconds", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

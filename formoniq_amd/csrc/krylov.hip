@@ -1581,6 +1581,29 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
 
 // Average duration of the fused MINRES SpMV launch (the kernel that moves the
 // matrix) on this system, FP32 or FP64, from HIP events on the context stream.
+// Mean seconds of one launch over `reps` back-to-back launches between two
+// events (after one warm-up launch). An event pair around every launch adds
+// the events' own cost to each; back to back, only the dispatch gap between
+// launches is counted beyond the kernels themselves.
+template <class F>
+static double time_launches(hipStream_t st, int reps, F&& launch) {
+  reps = std::max(reps, 1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  FEEC_HIP(hipEventCreate(&e0));
+  FEEC_HIP(hipEventCreate(&e1));
+  launch();
+  FEEC_HIP(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; ++i) launch();
+  FEEC_HIP(hipEventRecord(e1, st));
+  FEEC_HIP(hipEventSynchronize(e1));
+  FEEC_HIP(hipGetLastError());
+  float ms = 0.f;
+  FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms * 1e-3 / reps;
+}
+
 double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
@@ -1594,36 +1617,22 @@ double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
   h.done = 0;
   copy_to_device(ctx, S.p, &h, sizeof(h));
   DBuf<double> part(ctx, NBLK);
-  hipEvent_t e0, e1;
-  FEEC_HIP(hipEventCreate(&e0));
-  FEEC_HIP(hipEventCreate(&e1));
-  float ms = 0.f;
   if (fp32) {
     DBuf<float> y(ctx, N), r(ctx, N);
     FEEC_HIP(hipMemsetAsync(y.p, 0, N * 4, st));
     FEEC_HIP(hipMemsetAsync(r.p, 0, N * 4, st));
-    k_mr_spmv<float><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.val32.p, n, y.p, r.p, S.p, part.p);
-    FEEC_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < reps; ++i)
-      k_mr_spmv<float><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.val32.p, n, y.p, r.p, S.p, part.p);
-    FEEC_HIP(hipEventRecord(e1, st));
-    FEEC_HIP(hipEventSynchronize(e1));
-  } else {
-    DBuf<double> y(ctx, N), r(ctx, N);
-    FEEC_HIP(hipMemsetAsync(y.p, 0, N * 8, st));
-    FEEC_HIP(hipMemsetAsync(r.p, 0, N * 8, st));
-    k_mr_spmv<double><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.A.val.p, n, y.p, r.p, S.p, part.p);
-    FEEC_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < reps; ++i)
-      k_mr_spmv<double><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.A.val.p, n, y.p, r.p, S.p, part.p);
-    FEEC_HIP(hipEventRecord(e1, st));
-    FEEC_HIP(hipEventSynchronize(e1));
+    return time_launches(st, reps, [&] {
+      k_mr_spmv<float><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.val32.p, n, y.p,
+                                                    r.p, S.p, part.p);
+    });
   }
-  FEEC_HIP(hipGetLastError());
-  FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return ms * 1e-3 / std::max(reps, 1);
+  DBuf<double> y(ctx, N), r(ctx, N);
+  FEEC_HIP(hipMemsetAsync(y.p, 0, N * 8, st));
+  FEEC_HIP(hipMemsetAsync(r.p, 0, N * 8, st));
+  return time_launches(st, reps, [&] {
+    k_mr_spmv<double><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.A.val.p, n, y.p,
+                                                   r.p, S.p, part.p);
+  });
 }
 
 // z = B r, B the block preconditioner exactly as a solve of the given
@@ -1665,45 +1674,32 @@ void kkt_apply_preconditioner(KktSystem& s, int precision, int storage, const do
   ctx->sync();
 }
 
-// Average duration of one Chebyshev step (the preconditioner's fused SpMV)
-// on this system, FP32 or FP64; 0 when the system has no polynomial.
+// Mean duration of one Chebyshev step (the preconditioner's fused SpMV) on
+// this system with the given storage, FP32 or FP64, over back-to-back
+// launches; 0 when the system has no polynomial.
 double time_cheb_step(KktSystem& s, bool fp32, int reps, int storage) {
   if (s.P.nnz == 0 || s.cheb_m < 2 || !s.cheb.p) return 0.0;
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = s.n, N = s.n + s.nh;
-  hipEvent_t e0, e1;
-  FEEC_HIP(hipEventCreate(&e0));
-  FEEC_HIP(hipEventCreate(&e1));
   auto run = [&](auto& w) {
     using T = std::remove_reference_t<decltype(*w.res.p)>;
     for (T* v : {w.res.p, w.d0.p, w.d1.p, w.z.p})
       FEEC_HIP(hipMemsetAsync(v, 0, N * sizeof(T), st));
     Enq q{st};
     const MinresS* none = nullptr;
-    cheb_step<T>(q, w, w.d0.p, w.d1.p, w.z.p, 0, none);
-    FEEC_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < reps; ++i) cheb_step<T>(q, w, w.d0.p, w.d1.p, w.z.p, 0, none);
-    FEEC_HIP(hipEventRecord(e1, st));
-    FEEC_HIP(hipEventSynchronize(e1));
+    return time_launches(st, reps, [&] { cheb_step<T>(q, w, w.d0.p, w.d1.p, w.z.p, 0, none); });
   };
   if (fp32) {
     MinresWork<float> w;
     minres_alloc(ctx, w, n, s.nh);
     bind_fp32(s, w, storage);
-    run(w);
-  } else {
-    MinresWork<double> w;
-    minres_alloc(ctx, w, n, s.nh);
-    bind_fp64(s, w, storage);
-    run(w);
+    return run(w);
   }
-  FEEC_HIP(hipGetLastError());
-  float ms = 0.f;
-  FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return ms * 1e-3 / std::max(reps, 1);
+  MinresWork<double> w;
+  minres_alloc(ctx, w, n, s.nh);
+  bind_fp64(s, w, storage);
+  return run(w);
 }
 
 void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y) {

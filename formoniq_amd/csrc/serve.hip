@@ -124,8 +124,11 @@ void instance_result(Instance& I, const feec_report& rep, feec_result& res,
 void instance_solve(Instance& I, feec_result& res, std::vector<double>* u_out) {
   std::vector<double> src(I.nk);
   check_status(feec_normal_vector(I.nk, I.req.source_seed, src.data()));
+  const auto t0 = clk::now();
   feec_report rep = problem_solve_host(I.prob, src.data());
+  const double solve = seconds_since(t0);
   instance_result(I, rep, res, u_out);
+  res.solve_seconds = solve;
 }
 
 void instance_result(Instance& I, const feec_report& rep, feec_result& res,
@@ -212,14 +215,17 @@ struct feec_server_s {
       std::vector<double> u;
       feec_status st = guard([&] {
         Instance I;
+        const auto tb = clk::now();
         instance_build(I, wctx, req, opts.solve, true);
+        const double build = seconds_since(tb);
         instance_solve(I, res, &u);
+        res.build_seconds = build;
       });
       if (st != FEEC_OK) res.status = st;
       {
         std::lock_guard<std::mutex> lock(mu);
         Entry& e = entries[ticket];
-        double admit = e.result.admit_time;
+        const double admit = e.result.admit_time;
         e.result = res;
         e.result.ticket = ticket;
         e.result.admit_time = admit;
@@ -272,9 +278,12 @@ feec_status feec_batch_run(feec_ctx* ctx, int nreq, const feec_request* reqs,
     };
     auto t0 = clk::now();
     // 1. meshes, numbering and plans
+    std::vector<double> t_build(nreq, 0.0), t_solve(nreq, 0.0);
     for_workers([&](int i, feec_ctx* c) {
+      const auto tb = clk::now();
       inst[i] = std::make_unique<Instance>();
       instance_build(*inst[i], c, reqs[i], base, false);
+      t_build[i] = seconds_since(tb);
     });
     // 2. one persistent launch for every element block of every request
     std::vector<BatchElementRequest> br;
@@ -293,10 +302,12 @@ feec_status feec_batch_run(feec_ctx* ctx, int nreq, const feec_request* reqs,
     // 3. values, preconditioner and solve, overlapped across worker streams
     std::vector<double> t_done(nreq, 0.0);
     for_workers([&](int i, feec_ctx*) {
+      const auto ts = clk::now();
       problem_assemble_from_elements(*inst[i]->prob);
       results[i] = feec_result{};
       instance_solve(*inst[i], results[i], nullptr);
       t_done[i] = seconds_since(t0);
+      t_solve[i] = seconds_since(ts);
     });
     for (int i = 0; i < nreq; ++i) {
       if (status[i] != FEEC_OK) {
@@ -309,6 +320,8 @@ feec_status feec_batch_run(feec_ctx* ctx, int nreq, const feec_request* reqs,
       results[i].admit_time = t_elements;
       results[i].finish_time = t_done[i];
       results[i].latency = t_done[i];
+      results[i].build_seconds = t_build[i];
+      results[i].solve_seconds = t_solve[i];
     }
     inst.clear();
     for (auto* c : wctx) feec_ctx_destroy(c);

@@ -254,7 +254,8 @@ feec_status feec_problem_upload_source(feec_problem *prob, const double *source_
 /* Diagnostics (no counterpart in the reference): average seconds per launch,
  * from HIP events on the context stream, of {element kernel, KKT value
  * scatter, fused MINRES SpMV in FP32, the same in FP64, Chebyshev step (the
- * preconditioner's fused SpMV) in FP32, the same in FP64}. */
+ * preconditioner's fused SpMV, with the problem's precond_storage) in FP32,
+ * the same in FP64}, each over back-to-back launches between two events. */
 feec_status feec_problem_kernel_times(feec_problem *prob, int reps, double out[6]);
 
 /* Diagnostics (no counterpart in the reference): average seconds per launch of
@@ -292,6 +293,9 @@ typedef struct feec_result {
                            mesh, plan and element block was done) */
   double finish_time;
   double latency;       /* finish - submit, seconds */
+  double build_seconds; /* this request's mesh, numbering, plans and assembly (server) or
+                           mesh, numbering and plans (batch) */
+  double solve_seconds; /* its Krylov solve with refinement (batch: values and solve) */
 } feec_result;
 
 /* Batched: all element matrices of all requests in one persistent launch over
