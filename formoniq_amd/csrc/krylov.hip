@@ -774,9 +774,14 @@ __global__ void k_jacobi32(const float* r, const float* dinv, float* z, int64_t 
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < nn) z[i] = dinv[i] * r[i];
 }
-__global__ void k_to_scale(const double* a, int64_t n, double s, double* out) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) out[i] = s * a[i];
+// Power-iteration step of one block: with ny = sqrt(sq[0]), nx = sqrt(sq[1]),
+// lambda = ny / nx (when nx > 0) and x = y / ny (when ny > 0).
+__global__ void k_power_normalize(const double* y, int64_t n, const double* sq, double* x,
+                                  double* lam) {
+  const double ny = sqrt(sq[0]), nx = sqrt(sq[1]);
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i == 0 && nx > 0) *lam = ny / nx;
+  if (ny > 0 && i < n) x[i] = (1.0 / ny) * y[i];
 }
 __global__ void k_cg_p(double* pp, const double* zz, double bb, int64_t nn) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1362,30 +1367,30 @@ void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
     hx[i] = 0.5 + (double)(st_rng >> 11) * (1.0 / 9007199254740992.0);
   }
   copy_to_device(ctx, x.p, hx.data(), n * 8);
-  auto block_norm = [&](const double* v, int64_t lo, int64_t hi) {
-    if (hi <= lo) return 0.0;
+  // per block: |y|^2 and |x|^2 (fixed-order sums) and lambda, all on the device
+  // so the 30 iterations run without a host round trip
+  DBuf<double> sq(ctx, 4), dlam(ctx, 2);
+  double lam[2] = {1.0, 1.0};
+  copy_to_device(ctx, dlam.p, lam, sizeof(lam));
+  auto block_sumsq = [&](const double* v, int64_t lo, int64_t hi, double* out) {
     k_dot<double><<<red_grid(hi - lo), TPB, 0, st>>>(v + lo, v + lo, hi - lo, part.p);
-    k_finish_sum<<<1, TPB, 0, st>>>(part.p, red_grid(hi - lo), part.p + NBLK);
-    double h = 0.0;
-    FEEC_HIP(hipMemcpyAsync(&h, part.p + NBLK, sizeof(double), hipMemcpyDeviceToHost, st));
-    FEEC_HIP(hipStreamSynchronize(st));
-    return std::sqrt(h);
+    k_finish_sum<<<1, TPB, 0, st>>>(part.p, red_grid(hi - lo), out);
   };
   const int64_t lo[2] = {0, s.split}, hi[2] = {s.split, n};
-  double lam[2] = {1.0, 1.0};
   for (int it = 0; it < 30; ++it) {
     spmv64(ctx, s.P, x.p, y.p);
     k_jacobi<<<grid_for(n), TPB, 0, st>>>(s.inv_diag.p, y.p, y.p, n);
     // lambda per block from |y|/|x|, then x = y normalized per block
     for (int b = 0; b < 2; ++b) {
       if (hi[b] <= lo[b]) continue;
-      double ny = block_norm(y.p, lo[b], hi[b]), nx = block_norm(x.p, lo[b], hi[b]);
-      if (nx > 0) lam[b] = ny / nx;
-      if (ny > 0)
-        k_to_scale<<<grid_for(hi[b] - lo[b]), TPB, 0, st>>>(y.p + lo[b], hi[b] - lo[b], 1.0 / ny,
-                                                            x.p + lo[b]);
+      block_sumsq(y.p, lo[b], hi[b], sq.p + 2 * b);
+      block_sumsq(x.p, lo[b], hi[b], sq.p + 2 * b + 1);
+      k_power_normalize<<<grid_for(hi[b] - lo[b]), TPB, 0, st>>>(y.p + lo[b], hi[b] - lo[b],
+                                                                sq.p + 2 * b, x.p + lo[b],
+                                                                dlam.p + b);
     }
   }
+  copy_to_host(ctx, lam, dlam.p, sizeof(lam));
   // interval lower ends (fraction of b) tuned on 3D k=1 Kuhn meshes
   const double frac =
       lower > 0.0 ? lower
