@@ -363,6 +363,22 @@ void problem_element_outputs(Problem& P, double** outs) {
   for (int i = 0; i < 7; ++i) outs[i] = P.el.p + P.off[i];
 }
 
+// cheb_degree < 0: by mesh resolution. A quasi-uniform mesh of N_0 vertices
+// in d dimensions has about r = N_0^(1/d) vertices per axis and hdif_gram's
+// condition number grows like r^2. The degree that balanced polynomial cost
+// against Krylov iterations in sweeps on the mixed path (3D 64^3: 64; 2D 400^2
+// and 800^2: 128; batches of 1e3-1e5-cell meshes: 24-32) is about 8 sqrt(r):
+// taken in multiples of 8 within [16, 128]. The FP64 path, whose steps move
+// about 2.5x the bytes, takes half of it within [24, 64].
+static int resolve_cheb_degree(const Problem& P) {
+  if (P.opts.cheb_degree >= 0) return P.opts.cheb_degree;
+  const Mesh& m = *P.g->mesh;
+  const double r = std::pow((double)std::max<int64_t>(m.n[0], 1), 1.0 / m.d);
+  int deg = 8 * (int)std::lround(std::sqrt(r));
+  if (P.opts.precision == FEEC_PREC_FP64) return std::clamp(deg / 16 * 8, 24, 64);
+  return std::clamp(deg, 16, 128);
+}
+
 // CSR values, preconditioner diagonal and FP32 copy from the element blocks
 // already in P.el.
 void problem_assemble_from_elements(Problem& P) {
@@ -385,7 +401,7 @@ void problem_assemble_from_elements(Problem& P) {
   if ((P.opts.precision == FEEC_PREC_FP64 && P.opts.precond_storage == FEEC_PC_STORE_SELL) ||
       P.sys.Psell64.p)
     kkt_refresh_sell64(P.sys);
-  kkt_set_chebyshev(P.sys, P.opts.cheb_degree, P.opts.cheb_lower);
+  kkt_set_chebyshev(P.sys, resolve_cheb_degree(P), P.opts.cheb_lower);
   ctx->sync();
 }
 
@@ -409,7 +425,7 @@ Problem* problem_new(Geom* g, int bc, int k, const feec_solve_opts& o) {
                "precision must be FEEC_PREC_FP64 or FEEC_PREC_MIXED");
   FEEC_REQUIRE(o.precond_storage >= FEEC_PC_STORE_SELL && o.precond_storage <= FEEC_PC_STORE_CSR,
                FEEC_E_ARG, "precond_storage must be a feec_pc_storage value");
-  FEEC_REQUIRE(o.cheb_degree >= 0, FEEC_E_ARG, "cheb_degree must be >= 0");
+  FEEC_REQUIRE(o.cheb_degree >= -1, FEEC_E_ARG, "cheb_degree must be >= -1");
   FEEC_REQUIRE(o.cheb_lower >= 0.0 && o.cheb_lower < 1.0, FEEC_E_ARG, "cheb_lower must be in [0, 1)");
   auto* P = new feec_problem_s();
   P->g = g;
@@ -499,7 +515,7 @@ void feec_solve_opts_default(feec_solve_opts* o) {
   o->inner_rtol = 1e-4;
   o->use_graph = 1;
   o->graph_iters = 32;
-  o->cheb_degree = 64;
+  o->cheb_degree = -1;
   o->cheb_lower = 0.0;
   o->precond_storage = FEEC_PC_STORE_SELL;
 }

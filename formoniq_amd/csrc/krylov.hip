@@ -818,6 +818,8 @@ struct MinresWork {
   std::vector<std::shared_ptr<void>> graph_args;  // node arguments of `graph`
   int graph_iters = 0, graph_cheb = -1;
   int64_t n = 0, nh = 0;
+  int64_t a_nnz = 0;      // KKT nonzeros (for the segment-length estimate)
+  int64_t p_entries = 0;  // stored entries of the preconditioner's matrix
   // block preconditioner (cheb_m == 0: Jacobi with dinv)
   int cheb_m = 0;
   int64_t split = 0;
@@ -913,6 +915,8 @@ static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval,
   w.sd = sd;
   w.has_sell = sell;
   w.sell = sel;
+  w.a_nnz = s.A.nnz;
+  w.p_entries = sell ? s.sell_entries : s.P.nnz;
   w.psell_ptr = psell_ptr;
   w.sell_perm = sell_perm;
   w.sell_inv = sell_inv;
@@ -1006,6 +1010,27 @@ static void minres_iteration(Enq& q, MinresWork<T>& w, const int64_t* rp, const 
   q(k_mr_check, 1, 1, w.S.p);
 }
 
+// Iterations per segment. A segment ends with a host read of the MINRES
+// scalars, and work that other contexts queue meanwhile waits behind the
+// segments already on the device (one request's set-up syncs wait for another
+// request's segment), so a segment holds about 16 ms of estimated work: bytes
+// per iteration at 4 TB/s plus 4 us per kernel; even, 2..graph_iters.
+template <class T>
+static int segment_iters(const MinresWork<T>& w, const feec_solve_opts& o) {
+  const double vb = sizeof(T), N = (double)(w.n + w.nh);
+  double bytes = (vb + 4) * (double)w.a_nnz + 8.0 * (double)(w.n + 1) + 12 * vb * N;
+  double launches = 10;
+  if (w.cheb_m > 1) {
+    const double eb = w.has_sell ? (sizeof(T) == 4 ? 4.0 : 10.0) : (w.pval16 ? 4.0 : vb + 4.0);
+    bytes += (w.cheb_m - 1) * (eb * (double)w.p_entries + 4 * vb * (double)w.n);
+    launches += w.cheb_m + 1;
+  }
+  const double t = bytes / 4e12 + launches * 4e-6;
+  const int seg = (int)std::min(1e6, 0.016 / t);
+  const int cap = std::max(2, o.graph_iters - (o.graph_iters & 1));
+  return std::clamp(seg - (seg & 1), 2, cap);
+}
+
 // Run MINRES on w.b (already filled) into w.x. Returns stats.
 template <class T>
 static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const int64_t* rp,
@@ -1022,7 +1047,7 @@ static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const
       w.w2.p, w.part.p);
   }
   q(k_mr_start, 1, TPB, w.S.p, w.part.p, red_grid(N), rtol, (long long)max_iters);
-  int seg = std::max(2, o.graph_iters - (o.graph_iters & 1));
+  const int seg = segment_iters(w, o);
   if (o.use_graph && (!w.graph || w.graph_iters != seg || w.graph_cheb != w.cheb_m)) {
     if (w.graph) {
       (void)hipGraphExecDestroy(w.graph);
@@ -1399,7 +1424,9 @@ void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
       : m <= 12   ? 1.5e-3
       : m <= 32   ? 1e-3
       : m <= 48   ? 5e-4
-                  : 2.5e-4;
+      : m <= 64   ? 2.5e-4
+      : m <= 96   ? 1.2e-4
+                  : 6e-5;
   std::vector<double> coef(2 + 4 * std::max(m - 1, 0), 0.0);
   double rho[2], sigma[2], delta[2];
   for (int b = 0; b < 2; ++b) {

@@ -98,12 +98,17 @@ typedef struct feec_solve_opts {
   int32_t max_refinements; /* mixed: outer FP64 refinement steps (default 12) */
   double inner_rtol;     /* mixed: FP32 inner MINRES tolerance (default 1e-4) */
   int32_t use_graph;     /* capture Krylov iterations in a hipGraph (default 1) */
-  int32_t graph_iters;   /* iterations per graph segment, even (default 32) */
+  int32_t graph_iters;   /* most iterations per graph segment, even (default 32); a segment is
+                            cut to about 16 ms of estimated device work, so that other
+                            contexts' work does not queue long behind it */
   int32_t cheb_degree;   /* degree of the Chebyshev-Jacobi polynomial applied to each
-                            hdif_gram block of the preconditioner; 0 = Jacobi only (default 64) */
+                            hdif_gram block of the preconditioner; 0 = Jacobi only; -1 = by
+                            mesh resolution r = N_0^(1/d): 8 sqrt(r) in multiples of 8 within
+                            [16, 128] (mixed) or half that within [24, 64] (FP64) (default -1) */
   double cheb_lower;     /* lower end of the Chebyshev interval as a fraction of its upper end
                             (1.05 lambda_max); 0 = by degree: 1e-2 up to degree 4, 3e-3 up to 8,
-                            1.5e-3 up to 12, 1e-3 up to 32, 5e-4 up to 48, else 2.5e-4 (default 0) */
+                            1.5e-3 up to 12, 1e-3 up to 32, 5e-4 up to 48, 2.5e-4 up to 64,
+                            1.2e-4 up to 96, else 6e-5 (default 0) */
   int32_t precond_storage; /* feec_pc_storage: how the matrix the polynomial runs on is
                               stored (default FEEC_PC_STORE_SELL) */
 } feec_solve_opts;

@@ -522,7 +522,7 @@ CHEB_LMAX_MARGIN = 1.05
 def cheb_interval_fraction(m):
     """Lower end of the Chebyshev interval as a fraction of its upper end."""
     return (1e-2 if m <= 4 else 3e-3 if m <= 8 else 1.5e-3 if m <= 12 else 1e-3 if m <= 32
-            else 5e-4 if m <= 48 else 2.5e-4)
+            else 5e-4 if m <= 48 else 2.5e-4 if m <= 64 else 1.2e-4 if m <= 96 else 6e-5)
 
 
 def _xorshift_start(n):
