@@ -647,6 +647,32 @@ __global__ void __launch_bounds__(TPB) k_cheb3_init(const T* __restrict__ r,
   }
 }
 
+// The Lanczos update yn += c r2 (c = -alpha/beta) fused with k_cheb3_init:
+// lane i updates row perm[i] (a bijection on rows < n), so each row is read
+// and written by one lane; the harmonic rows (>= n) are updated in place.
+template <class T>
+__global__ void __launch_bounds__(TPB) k_mr_update_cheb3_init(
+    T* __restrict__ yn, const T* __restrict__ r2, const T* __restrict__ sd,
+    const int32_t* __restrict__ perm, T* __restrict__ f, T* __restrict__ x0, T* __restrict__ x1,
+    int64_t n, int64_t N, int64_t split, const double* __restrict__ coef,
+    const MinresS* __restrict__ S) {
+  if (S->done) return;
+  const T c = (T)(-S->alfa / S->beta);
+  const T s0 = (T)coef[0], s1 = (T)coef[1];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = i < n ? perm[i] : i;
+    const T a = yn[j] + c * r2[j];
+    yn[j] = a;
+    if (i < n) {
+      const T fi = a * sd[j];
+      f[i] = fi;
+      x0[i] = 0;
+      x1[i] = fi * (i < split ? s0 : s1);
+    }
+  }
+}
+
 // Workgroups are dispatched to the 8 XCDs of the chip round-robin (workgroup b
 // to XCD b % 8), each XCD with its own L2. xcd_block() maps them so that XCD x
 // runs the x-th contiguous eighth of the logical blocks: its L2 then fetches
@@ -952,14 +978,16 @@ static void cheb_step(Enq& q, MinresWork<T>& w, const T* a, T* b, T* z, int k, c
 
 // z = P r with the Chebyshev block preconditioner, plus partial sums of r.z
 // into `part` (red_grid(N) blocks).
+// With init_done, k_mr_update_cheb3_init has already started the SELL form.
 template <class T>
 static void cheb_apply(Enq& q, MinresWork<T>& w, const T* r, T* z, const MinresS* S,
-                       double* part) {
+                       double* part, bool init_done = false) {
   const int64_t n = w.n, N = w.n + w.nh;
   if (w.has_sell && w.sd) {
     // three-term polynomial on the SELL-64 copy; x_{k+2} overwrites x_k
-    q(k_cheb3_init<T>, red_grid(n), TPB, r, w.sd, w.sell_perm, w.res.p, w.d0.p, w.d1.p, n, w.split,
-      w.coef, S);
+    if (!init_done)
+      q(k_cheb3_init<T>, red_grid(n), TPB, r, w.sd, w.sell_perm, w.res.p, w.d0.p, w.d1.p, n,
+        w.split, w.coef, S);
     T* cur = w.d1.p;
     T* prev = w.d0.p;
     for (int k = 0; k + 1 < w.cheb_m; ++k) {
@@ -999,7 +1027,11 @@ static void minres_iteration(Enq& q, MinresWork<T>& w, const int64_t* rp, const 
     q(k_mr_border_axpy<T>, red_grid(n), TPB, w.border.p, (int)w.nh, n, y, r1, w.S.p);
   }
   q(k_mr_alpha, 1, TPB, w.S.p, w.part.p, spmv_grid(n), w.nh ? 1 : 0);
-  if (w.cheb_m > 0) {
+  if (w.cheb_m > 0 && w.has_sell && w.sd) {
+    q(k_mr_update_cheb3_init<T>, red_grid(N), TPB, r1, r2, w.sd, w.sell_perm, w.res.p, w.d0.p,
+      w.d1.p, n, N, w.split, w.coef, w.S.p);
+    cheb_apply<T>(q, w, r1, z, w.S.p, w.part2.p, true);
+  } else if (w.cheb_m > 0) {
     q(k_mr_update_only<T>, red_grid(N), TPB, r1, r2, N, w.S.p);
     cheb_apply<T>(q, w, r1, z, w.S.p, w.part2.p);
   } else {

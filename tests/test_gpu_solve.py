@@ -13,7 +13,7 @@ import scipy.sparse.linalg as spla
 import formoniq_amd as fa
 from oracle import oracle as o
 
-from _common import assert_csr_parity, both
+from _common import assert_csr_parity, both, kuhn_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -158,6 +158,42 @@ def test_preconditioner_application_per_storage(d, k, n):
     np.testing.assert_array_equal(z_sell[n_p:], r[n_p:].astype(np.float32).astype(np.float64))
     f64 = fa.Problem(gc, k, fa.solve_opts(precision=fa.FP64, cheb_degree=m))
     assert f64.preconditioner()["format"] == "sell64_f64"
+
+
+@pytest.mark.parametrize("precision", [fa.MIXED, fa.FP64])
+def test_wide_band_falls_back_to_csr(precision):
+    """Randomly numbered vertices spread the rows' columns over the whole
+    matrix, so no 16-bit column offset fits: the preconditioner runs from CSR
+    with 32-bit columns, and the solve meets the certificate and agrees with
+    the same mesh numbered in grid order (edges matched through their
+    vertices, orientation signs included)."""
+    cells, coords = kuhn_mesh(3, 20, 0.05, seed=8)
+    nv = coords.shape[0]
+    perm = np.random.default_rng(9).permutation(nv)  # old vertex id -> new id
+    meshes = [(cells, coords), (perm[cells], coords[np.argsort(perm)])]
+    out = []
+    for c, x in meshes:
+        cx = fa.Complex(3, c, nverts=nv)
+        wc = fa.WhitneyComplex(fa.MeshLengthsSq.from_coords(cx, x))
+        out.append((cx, wc, fa.Problem(wc, 1, fa.solve_opts(precision=precision))))
+    ea = out[0][0].skeleton(1)  # sorted vertex pairs, colex order
+    eb = out[1][0].skeleton(1)
+    index_b = {(int(a), int(b)): j for j, (a, b) in enumerate(eb)}
+    pa = perm[ea]
+    match = np.array([index_b[(min(a, b), max(a, b))] for a, b in pa])
+    sign = np.where(pa[:, 0] < pa[:, 1], 1.0, -1.0)
+    fa_src = np.cos(np.arange(ea.shape[0]) * 0.37)
+    fb_src = np.zeros_like(fa_src)
+    fb_src[match] = sign * fa_src
+    sols = []
+    for (cx, wc, prob), f in zip(out, (fa_src, fb_src)):
+        rep = prob.solve(f)
+        assert rep.converged and rep.residual <= 1e-10, rep.as_dict()
+        sols.append((prob.preconditioner()["format"], prob.solution()[1]))
+    assert sols[0][0].startswith("sell64")
+    assert sols[1][0] == ("csr_f16" if precision == fa.MIXED else "csr_f64")
+    u0, u1 = sols[0][1], sign * sols[1][1][match]
+    assert np.linalg.norm(u1 - u0) <= 1e-8 * np.linalg.norm(u0)
 
 
 def test_resident_problem_reassembly_and_device_source():
