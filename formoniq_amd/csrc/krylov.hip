@@ -1645,26 +1645,38 @@ feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_s
 
 // Average duration of the fused MINRES SpMV launch (the kernel that moves the
 // matrix) on this system, FP32 or FP64, from HIP events on the context stream.
-// Mean seconds of one launch over `reps` back-to-back launches between two
-// events (after one warm-up launch). An event pair around every launch adds
-// the events' own cost to each; back to back, only the dispatch gap between
-// launches is counted beyond the kernels themselves.
-template <class F>
-static double time_launches(hipStream_t st, int reps, F&& launch) {
+// Mean seconds per launch of `reps` launches replayed from one graph, as the
+// solve replays its kernels (after one warm-up replay).
+template <class Build>
+static double time_graph(hipStream_t st, int reps, Build&& build) {
   reps = std::max(reps, 1);
+  hipGraph_t g = nullptr;
+  FEEC_HIP(hipGraphCreate(&g, 0));
+  std::vector<std::shared_ptr<void>> keep;
+  hipGraphExec_t ex = nullptr;
+  try {
+    Enq gq{st, g, &keep};
+    for (int i = 0; i < reps; ++i) build(gq);
+    FEEC_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+  } catch (...) {
+    (void)hipGraphDestroy(g);
+    throw;
+  }
+  (void)hipGraphDestroy(g);
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  FEEC_HIP(hipEventCreate(&e0));
-  FEEC_HIP(hipEventCreate(&e1));
-  launch();
-  FEEC_HIP(hipEventRecord(e0, st));
-  for (int i = 0; i < reps; ++i) launch();
-  FEEC_HIP(hipEventRecord(e1, st));
-  FEEC_HIP(hipEventSynchronize(e1));
-  FEEC_HIP(hipGetLastError());
   float ms = 0.f;
-  FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  hipError_t e = hipGraphLaunch(ex, st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  if (e == hipSuccess) e = hipGraphLaunch(ex, st);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipGraphExecDestroy(ex);
+  FEEC_HIP(e);
   return ms * 1e-3 / reps;
 }
 
@@ -1685,17 +1697,17 @@ double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
     DBuf<float> y(ctx, N), r(ctx, N);
     FEEC_HIP(hipMemsetAsync(y.p, 0, N * 4, st));
     FEEC_HIP(hipMemsetAsync(r.p, 0, N * 4, st));
-    return time_launches(st, reps, [&] {
-      k_mr_spmv<float><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.val32.p, n, y.p,
-                                                    r.p, S.p, part.p);
+    return time_graph(st, reps, [&](Enq& q) {
+      q(k_mr_spmv<float>, spmv_grid(n), TPB, s.A.row_ptr.p, s.A.col.p, s.val32.p, n, y.p, r.p, S.p,
+        part.p);
     });
   }
   DBuf<double> y(ctx, N), r(ctx, N);
   FEEC_HIP(hipMemsetAsync(y.p, 0, N * 8, st));
   FEEC_HIP(hipMemsetAsync(r.p, 0, N * 8, st));
-  return time_launches(st, reps, [&] {
-    k_mr_spmv<double><<<spmv_grid(n), TPB, 0, st>>>(s.A.row_ptr.p, s.A.col.p, s.A.val.p, n, y.p,
-                                                   r.p, S.p, part.p);
+  return time_graph(st, reps, [&](Enq& q) {
+    q(k_mr_spmv<double>, spmv_grid(n), TPB, s.A.row_ptr.p, s.A.col.p, s.A.val.p, n, y.p, r.p, S.p,
+      part.p);
   });
 }
 
@@ -1739,8 +1751,8 @@ void kkt_apply_preconditioner(KktSystem& s, int precision, int storage, const do
 }
 
 // Mean duration of one Chebyshev step (the preconditioner's fused SpMV) on
-// this system with the given storage, FP32 or FP64, over back-to-back
-// launches; 0 when the system has no polynomial.
+// this system with the given storage, FP32 or FP64, replayed from a graph as
+// in the solve; 0 when the system has no polynomial.
 double time_cheb_step(KktSystem& s, bool fp32, int reps, int storage) {
   if (s.P.nnz == 0 || s.cheb_m < 2 || !s.cheb.p) return 0.0;
   Ctx* ctx = s.ctx;
@@ -1750,9 +1762,8 @@ double time_cheb_step(KktSystem& s, bool fp32, int reps, int storage) {
     using T = std::remove_reference_t<decltype(*w.res.p)>;
     for (T* v : {w.res.p, w.d0.p, w.d1.p, w.z.p})
       FEEC_HIP(hipMemsetAsync(v, 0, N * sizeof(T), st));
-    Enq q{st};
     const MinresS* none = nullptr;
-    return time_launches(st, reps, [&] { cheb_step<T>(q, w, w.d0.p, w.d1.p, w.z.p, 0, none); });
+    return time_graph(st, reps, [&](Enq& q) { cheb_step<T>(q, w, w.d0.p, w.d1.p, w.z.p, 0, none); });
   };
   if (fp32) {
     MinresWork<float> w;

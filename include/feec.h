@@ -260,7 +260,8 @@ feec_status feec_problem_upload_source(feec_problem *prob, const double *source_
  * from HIP events on the context stream, of {element kernel, KKT value
  * scatter, fused MINRES SpMV in FP32, the same in FP64, Chebyshev step (the
  * preconditioner's fused SpMV, with the problem's precond_storage) in FP32,
- * the same in FP64}, each over back-to-back launches between two events. */
+ * the same in FP64}; the last four replayed from a graph of `reps` launches,
+ * as a solve replays them. */
 feec_status feec_problem_kernel_times(feec_problem *prob, int reps, double out[6]);
 
 /* Diagnostics (no counterpart in the reference): average seconds per launch of
