@@ -391,12 +391,15 @@ def main():
     pc = mixed.preconditioner()
     p_nnz, cheb_deg, p_col_bytes = pc["nnz"], pc["degree"], pc["column_bytes"]
     spmv_bytes = 8 * nnz + 8 * (n_kkt + 1) + 12 * n_kkt
-    if pc["format"] == "sell64_f16":
+    if pc["format"] in ("sell64_f16", "sell64_f16_c32"):
         # three-term step on the SELL-64 copy: 4 B per off-diagonal nonzero
-        # (FP16 value + int16 offset; padding is not counted), 8 B per 64-row
-        # slice offset, x_{k+1} read (4n), x_k read and overwritten (8n), f read (4n)
+        # (FP16 value + int16 offset; 6 B with int32 columns; padding is not
+        # counted), 8 B per 64-row slice offset, x_{k+1} read (4n), x_k read
+        # and overwritten (8n), f read (4n)
         p_val_bytes = 2
-        cheb_bytes = 4 * pc["offdiag_nnz"] + 8 * ((n_kkt + 63) // 64 + 1) + 16 * n_kkt
+        p_col_bytes = 2 if pc["format"] == "sell64_f16" else 4
+        cheb_bytes = ((2 + p_col_bytes) * pc["offdiag_nnz"] + 8 * ((n_kkt + 63) // 64 + 1)
+                      + 16 * n_kkt)
     else:
         # two-term CSR step: P values (2 B as scaled FP16, else 4 B) + columns
         # (2 B as band offsets, else 4 B), row pointers, d (4n), res and z read
@@ -451,7 +454,7 @@ def main():
         "kernel_seconds": times,
         "roofline": {
             "kernel": ("k_cheb3_step (Chebyshev step on the SELL-64 FP16 preconditioner: SpMV + three-term "
-                       "recurrence, FP32 arithmetic)" if pc["format"] == "sell64_f16" else
+                       "recurrence, FP32 arithmetic)" if pc["format"].startswith("sell64") else
                        "k_cheb_step<float> (Chebyshev step: preconditioner SpMV + recurrence, FP32 arithmetic)"),
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

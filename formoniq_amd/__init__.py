@@ -415,7 +415,8 @@ class Problem:
         return dict(elmat=float(out[0]), scatter=float(out[1]), spmv32=float(out[2]),
                     spmv64=float(out[3]), cheb32=float(out[4]), cheb64=float(out[5]))
 
-    PC_FORMATS = {0: "csr_f32", 1: "csr_f16", 2: "sell64_f16", 3: "csr_f64", 4: "sell64_f64"}
+    PC_FORMATS = {0: "csr_f32", 1: "csr_f16", 2: "sell64_f16", 3: "csr_f64", 4: "sell64_f64",
+                  5: "sell64_f16_c32", 6: "sell64_f64_c32"}
 
     def apply_preconditioner(self, r, precision=None, storage=None):
         """z = B r (KKT order: sigma, u, harmonic rows) with the block preconditioner

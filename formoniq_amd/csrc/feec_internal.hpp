@@ -300,11 +300,17 @@ struct KktSystem {
   // columns are renumbered (row' = sell_inv[row]) by sorting the rows of each
   // window of 256 (windows restart at `split`) by off-diagonal count, longest
   // first, so that a slice holds rows of equal length; sell_perm[row'] = row.
+  // When some renumbered offset exceeds +-32767 (sell_wide), the slices keep
+  // int32 columns in Psell_col instead and FP16 values in Psell_v16.
   DBuf<uint32_t> Psell;
-  DBuf<int64_t> Psell_ptr;  // n_slices + 1 offsets into Psell (multiples of 64)
+  DBuf<int64_t> Psell_ptr;  // n_slices + 1 offsets into the slices (multiples of 64)
   DBuf<int32_t> sell_perm, sell_inv;
   int64_t sell_entries = 0, P_offdiag = 0;
-  // the same slices for the FP64 path: FP64 values and int16 offsets apart
+  bool sell_wide = false;
+  DBuf<int32_t> Psell_col;
+  DBuf<_Float16> Psell_v16;
+  // the same slices for the FP64 path: FP64 values, and int16 offsets apart
+  // (or Psell_col when wide)
   DBuf<double> Psell64;
   DBuf<int16_t> Psell_off;
   DBuf<double> sdinv64;  // D^-1/2, n entries

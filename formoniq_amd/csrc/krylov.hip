@@ -572,21 +572,39 @@ __device__ __forceinline__ int64_t sell_off(uint32_t e) { return (int64_t)(int16
 constexpr int SELL_UNROLL = 8;
 constexpr int SELL_WINDOW = 256;  // rows sorted by length inside windows of this many
 
-// Entry q of the slice storage as (value, column offset).
+// Entry q of the slice storage of row `row` as (value, column). Narrow forms
+// keep the column as an int16 offset from the row; the wide forms (when some
+// offset does not fit) keep int32 columns in an array of their own.
 struct SellF16 {
   const uint32_t* __restrict__ e;
-  __device__ __forceinline__ void get(int64_t q, float& v, int64_t& o) const {
+  __device__ __forceinline__ void get(int64_t q, int64_t row, float& v, int64_t& c) const {
     const uint32_t w = e[q];
     v = sell_val(w);
-    o = sell_off(w);
+    c = row + sell_off(w);
   }
 };
 struct SellF64 {
   const double* __restrict__ v;
   const int16_t* __restrict__ o;
-  __device__ __forceinline__ void get(int64_t q, double& val, int64_t& off) const {
+  __device__ __forceinline__ void get(int64_t q, int64_t row, double& val, int64_t& c) const {
     val = v[q];
-    off = o[q];
+    c = row + o[q];
+  }
+};
+struct SellF16W {
+  const _Float16* __restrict__ v;
+  const int32_t* __restrict__ col;
+  __device__ __forceinline__ void get(int64_t q, int64_t, float& val, int64_t& c) const {
+    val = (float)v[q];
+    c = col[q];
+  }
+};
+struct SellF64W {
+  const double* __restrict__ v;
+  const int32_t* __restrict__ col;
+  __device__ __forceinline__ void get(int64_t q, int64_t, double& val, int64_t& c) const {
+    val = v[q];
+    c = col[q];
   }
 };
 
@@ -606,17 +624,17 @@ __device__ __forceinline__ T sell_row(E E_, const int64_t* __restrict__ sp, int6
   int j = 0;
   for (; j + U <= w; j += U) {
     T v[U];
-    int64_t o[U];
+    int64_t c[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) E_.get(p + (int64_t)(j + u) * 64, v[u], o[u]);
+    for (int u = 0; u < U; ++u) E_.get(p + (int64_t)(j + u) * 64, row, v[u], c[u]);
 #pragma unroll
-    for (int u = 0; u < U; ++u) a[u] += v[u] * x[row + o[u]];
+    for (int u = 0; u < U; ++u) a[u] += v[u] * x[c[u]];
   }
   for (; j < w; ++j) {
     T v;
-    int64_t o;
-    E_.get(p + (int64_t)j * 64, v, o);
-    a[0] += v * x[row + o];
+    int64_t c;
+    E_.get(p + (int64_t)j * 64, row, v, c);
+    a[0] += v * x[c];
   }
   T acc = a[0];
 #pragma unroll
@@ -855,9 +873,11 @@ struct MinresWork {
   const _Float16* pval16 = nullptr;  // scaled P in FP16 (FP32 path), or null
   const T* sd = nullptr;             // D^-1/2 for the scaled polynomial
   // SELL-64 copy of the scaled P (FP16 entries on the FP32 path, FP64 on the FP64 path)
-  using SellE = std::conditional_t<std::is_same_v<T, float>, SellF16, SellF64>;
-  bool has_sell = false;
-  SellE sell{};
+  using SellN = std::conditional_t<std::is_same_v<T, float>, SellF16, SellF64>;
+  using SellW = std::conditional_t<std::is_same_v<T, float>, SellF16W, SellF64W>;
+  bool has_sell = false, sell_wide = false;
+  SellN sell{};   // int16 column offsets
+  SellW sellw{};  // int32 columns
   const int64_t* psell_ptr = nullptr;
   const int32_t* sell_perm = nullptr;
   const int32_t* sell_inv = nullptr;
@@ -913,12 +933,21 @@ static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval,
                                 const _Float16* pval16 = nullptr, const T* sd = nullptr,
                                 bool sell = false) {
   int m = (s.P.nnz > 0 && s.cheb_m > 0 && s.cheb.p) ? s.cheb_m : 0;
-  typename MinresWork<T>::SellE sel{};
+  typename MinresWork<T>::SellN sel{};
+  typename MinresWork<T>::SellW selw{};
+  const bool wide = sell && s.sell_wide;
   if (sell) {
-    if constexpr (std::is_same_v<T, float>)
-      sel = SellF16{s.Psell.p};
-    else
-      sel = SellF64{s.Psell64.p, s.Psell_off.p};
+    if constexpr (std::is_same_v<T, float>) {
+      if (wide)
+        selw = SellF16W{s.Psell_v16.p, s.Psell_col.p};
+      else
+        sel = SellF16{s.Psell.p};
+    } else {
+      if (wide)
+        selw = SellF64W{s.Psell64.p, s.Psell_col.p};
+      else
+        sel = SellF64{s.Psell64.p, s.Psell_off.p};
+    }
   }
   const int64_t* psell_ptr = sell ? s.Psell_ptr.p : nullptr;
   const int32_t* sell_perm = sell ? s.sell_perm.p : nullptr;
@@ -926,7 +955,8 @@ static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval,
   if (w.graph && (w.pval != pval || w.coef != s.cheb.p || w.prp != s.P.row_ptr.p ||
                   w.pcol != s.P.col.p || w.pcol16 != s.Pcol16.p || w.pval16 != pval16 ||
                   w.sd != sd || w.cheb_m != m || w.has_sell != sell ||
-                  std::memcmp(&w.sell, &sel, sizeof(sel)) != 0 || w.psell_ptr != psell_ptr ||
+                  std::memcmp(&w.sell, &sel, sizeof(sel)) != 0 ||
+                  std::memcmp(&w.sellw, &selw, sizeof(selw)) != 0 || w.psell_ptr != psell_ptr ||
                   w.sell_perm != sell_perm || w.sell_inv != sell_inv)) {
     (void)hipGraphExecDestroy(w.graph);
     w.graph = nullptr;
@@ -940,7 +970,9 @@ static void bind_preconditioner(KktSystem& s, MinresWork<T>& w, const T* pval,
   w.pval16 = pval16;
   w.sd = sd;
   w.has_sell = sell;
+  w.sell_wide = wide;
   w.sell = sel;
+  w.sellw = selw;
   w.a_nnz = s.A.nnz;
   w.p_entries = sell ? s.sell_entries : s.P.nnz;
   w.psell_ptr = psell_ptr;
@@ -957,8 +989,12 @@ template <class T>
 static void cheb_step(Enq& q, MinresWork<T>& w, const T* a, T* b, T* z, int k, const MinresS* S) {
   const int64_t n = w.n;
   if (w.has_sell && w.sd) {
-    q(k_cheb3_step<T, typename MinresWork<T>::SellE>, sell_grid(n), TPB, w.sell, w.psell_ptr, n,
-      w.split, (const T*)w.res.p, a, b, w.coef, k, S);
+    if (w.sell_wide)
+      q(k_cheb3_step<T, typename MinresWork<T>::SellW>, sell_grid(n), TPB, w.sellw, w.psell_ptr, n,
+        w.split, (const T*)w.res.p, a, b, w.coef, k, S);
+    else
+      q(k_cheb3_step<T, typename MinresWork<T>::SellN>, sell_grid(n), TPB, w.sell, w.psell_ptr, n,
+        w.split, (const T*)w.res.p, a, b, w.coef, k, S);
     return;
   }
   const bool scaled = w.pval16 && w.sd;
@@ -1053,7 +1089,8 @@ static int segment_iters(const MinresWork<T>& w, const feec_solve_opts& o) {
   double bytes = (vb + 4) * (double)w.a_nnz + 8.0 * (double)(w.n + 1) + 12 * vb * N;
   double launches = 10;
   if (w.cheb_m > 1) {
-    const double eb = w.has_sell ? (sizeof(T) == 4 ? 4.0 : 10.0) : (w.pval16 ? 4.0 : vb + 4.0);
+    const double eb = w.has_sell ? (sizeof(T) == 4 ? 4.0 : 10.0) + (w.sell_wide ? 2.0 : 0.0)
+                                 : (w.pval16 ? 4.0 : vb + 4.0);
     bytes += (w.cheb_m - 1) * (eb * (double)w.p_entries + 4 * vb * (double)w.n);
     launches += w.cheb_m + 1;
   }
@@ -1234,12 +1271,13 @@ __global__ void k_sell_fill(const int64_t* __restrict__ rp, const int32_t* __res
   }
 }
 
-// FP64 values and int16 offsets of the same slices (the FP64 path).
-__global__ void k_sell_fill64(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
-                              const double* __restrict__ val, const double* __restrict__ inv_diag,
-                              const int64_t* __restrict__ sp, const int32_t* __restrict__ perm,
-                              const int32_t* __restrict__ inv, int64_t n, int64_t nrows_padded,
-                              double* __restrict__ V, int16_t* __restrict__ O) {
+// Scaled off-diagonal values of row perm[i] into lane i of the slices (FP16
+// or FP64), in P's column order, zero after the row's last entry.
+template <class V>
+__global__ void k_sell_vals(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const double* __restrict__ val, const double* __restrict__ inv_diag,
+                            const int64_t* __restrict__ sp, const int32_t* __restrict__ perm,
+                            int64_t n, int64_t nrows_padded, V* __restrict__ out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrows_padded;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t s = i >> 6, base = sp[s] + (i & 63);
@@ -1251,15 +1289,37 @@ __global__ void k_sell_fill64(const int64_t* __restrict__ rp, const int32_t* __r
       for (int64_t p = rp[row]; p < rp[row + 1]; ++p) {
         const int64_t c = col[p];
         if (c == row) continue;
-        V[base + j * 64] = val[p] * si * sqrt(fabs(inv_diag[c]));
-        O[base + j * 64] = (int16_t)(inv[c] - i);
+        out[base + j * 64] = (V)(val[p] * si * sqrt(fabs(inv_diag[c])));
         ++j;
       }
     }
-    for (; j < w; ++j) {
-      V[base + j * 64] = 0.0;
-      O[base + j * 64] = 0;
+    for (; j < w; ++j) out[base + j * 64] = (V)0;
+  }
+}
+
+// Columns of the same entries in the SELL numbering: int16 offsets
+// inv[col] - i (REL) or int32 columns inv[col]; padding 0 / the row itself.
+template <class I, bool REL>
+__global__ void k_sell_cols(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const int64_t* __restrict__ sp, const int32_t* __restrict__ perm,
+                            const int32_t* __restrict__ inv, int64_t n, int64_t nrows_padded,
+                            I* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrows_padded;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = i >> 6, base = sp[s] + (i & 63);
+    const int64_t w = (sp[s + 1] - sp[s]) >> 6;
+    int64_t j = 0;
+    if (i < n) {
+      const int64_t row = perm[i];
+      for (int64_t p = rp[row]; p < rp[row + 1]; ++p) {
+        const int64_t c = col[p];
+        if (c == row) continue;
+        out[base + j * 64] = REL ? (I)(inv[c] - i) : (I)inv[c];
+        ++j;
+      }
     }
+    const I pad = REL ? (I)0 : (I)(i < n ? i : 0);
+    for (; j < w; ++j) out[base + j * 64] = pad;
   }
 }
 
@@ -1303,12 +1363,19 @@ void kkt_refresh_fp32(KktSystem& s) {
     // up to date once a solve has asked for it
     if (!s.Psell_ptr.p || s.P16.p) kkt_refresh_half_csr(s);
     if (s.Psell_ptr.p) {
-      if (!s.Psell.p || (int64_t)s.Psell.n != s.sell_entries)
-        s.Psell.alloc(s.ctx, std::max<int64_t>(s.sell_entries, 1));
+      const int64_t E = std::max<int64_t>(s.sell_entries, 1);
       const int64_t padded = (s.n + 63) / 64 * 64;
-      k_sell_fill<<<grid_for(padded), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p,
-                                                    s.inv_diag.p, s.Psell_ptr.p, s.sell_perm.p,
-                                                    s.sell_inv.p, s.n, padded, s.Psell.p);
+      if (!s.sell_wide) {
+        if (!s.Psell.p || (int64_t)s.Psell.n != E) s.Psell.alloc(s.ctx, E);
+        k_sell_fill<<<grid_for(padded), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p,
+                                                      s.inv_diag.p, s.Psell_ptr.p, s.sell_perm.p,
+                                                      s.sell_inv.p, s.n, padded, s.Psell.p);
+      } else {
+        if (!s.Psell_v16.p || (int64_t)s.Psell_v16.n != E) s.Psell_v16.alloc(s.ctx, E);
+        k_sell_vals<_Float16><<<grid_for(padded), TPB, 0, st>>>(
+            s.P.row_ptr.p, s.P.col.p, s.P.val.p, s.inv_diag.p, s.Psell_ptr.p, s.sell_perm.p, s.n,
+            padded, s.Psell_v16.p);
+      }
     }
   }
   FEEC_HIP(hipGetLastError());
@@ -1320,17 +1387,20 @@ void kkt_refresh_sell64(KktSystem& s) {
   if (!s.Psell_ptr.p) return;
   hipStream_t st = s.ctx->stream;
   const int64_t n = s.n, E = std::max<int64_t>(s.sell_entries, 1);
-  if (!s.Psell64.p || (int64_t)s.Psell64.n != E) {
-    s.Psell64.alloc(s.ctx, E);
+  const int64_t padded = (n + 63) / 64 * 64;
+  if (!s.Psell64.p || (int64_t)s.Psell64.n != E) s.Psell64.alloc(s.ctx, E);
+  if (!s.sell_wide && !s.Psell_off.p) {  // pattern-level, once
     s.Psell_off.alloc(s.ctx, E);
+    k_sell_cols<int16_t, true><<<grid_for(padded), TPB, 0, st>>>(
+        s.P.row_ptr.p, s.P.col.p, s.Psell_ptr.p, s.sell_perm.p, s.sell_inv.p, n, padded,
+        s.Psell_off.p);
   }
   if (!s.sdinv64.p || (int64_t)s.sdinv64.n != std::max<int64_t>(n, 1))
     s.sdinv64.alloc(s.ctx, std::max<int64_t>(n, 1));
   k_scale_sd64<<<grid_for(n), TPB, 0, st>>>(s.inv_diag.p, n, s.sdinv64.p);
-  const int64_t padded = (n + 63) / 64 * 64;
-  k_sell_fill64<<<grid_for(padded), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p, s.inv_diag.p,
-                                                  s.Psell_ptr.p, s.sell_perm.p, s.sell_inv.p, n,
-                                                  padded, s.Psell64.p, s.Psell_off.p);
+  k_sell_vals<double><<<grid_for(padded), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p,
+                                                        s.inv_diag.p, s.Psell_ptr.p, s.sell_perm.p,
+                                                        n, padded, s.Psell64.p);
   FEEC_HIP(hipGetLastError());
 }
 
@@ -1353,8 +1423,8 @@ __global__ void k_band16(const int64_t* __restrict__ rp, const int32_t* __restri
 }
 
 // Pattern-level set-up of P, once: int16 band-offset columns when every
-// |col - row| <= 32767, and then the SELL-64 row order and slice offsets when
-// its band fits int16 too.
+// |col - row| <= 32767, the SELL-64 row order and slice offsets, and the
+// slices' int32 columns when a renumbered offset does not fit int16.
 static void ensure_band(KktSystem& s) {
   if (s.band_checked || s.P.nnz == 0) return;
   Ctx* ctx = s.ctx;
@@ -1366,10 +1436,7 @@ static void ensure_band(KktSystem& s) {
   unsigned long long md = 0;
   FEEC_HIP(hipMemcpyAsync(&md, ctx->dcount, sizeof(md), hipMemcpyDeviceToHost, st));
   ctx->sync();
-  if (md > 32767) {
-    s.Pcol16.reset();
-    return;
-  }
+  if (md > 32767) s.Pcol16.reset();
   // SELL row order: rows sorted by length inside windows, then its band
   const int64_t n = s.n, ns = (n + 63) / 64;
   DBuf<int32_t> cnt(ctx, n);
@@ -1385,11 +1452,7 @@ static void ensure_band(KktSystem& s) {
   k_sell_band<<<grid_for(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.sell_inv.p, n, ctx->dcount);
   FEEC_HIP(hipMemcpyAsync(&md, ctx->dcount, sizeof(md), hipMemcpyDeviceToHost, st));
   ctx->sync();
-  if (md > 32767) {
-    s.sell_perm.reset();
-    s.sell_inv.reset();
-    return;
-  }
+  s.sell_wide = md > 32767;
   DBuf<int64_t> width(ctx, ns);
   FEEC_HIP(hipMemsetAsync(ctx->dcount, 0, sizeof(unsigned long long), st));
   k_sell_width<<<grid_for(ns), TPB, 0, st>>>(cnt.p, s.sell_perm.p, n, ns, width.p, ctx->dcount);
@@ -1405,6 +1468,14 @@ static void ensure_band(KktSystem& s) {
   ctx->sync();
   s.sell_entries = ptr[ns];
   s.P_offdiag = (int64_t)offdiag;
+  if (s.sell_wide) {  // int32 columns, shared by both precisions
+    const int64_t padded = ns * 64;
+    s.Psell_col.alloc(ctx, std::max<int64_t>(s.sell_entries, 1));
+    k_sell_cols<int32_t, false><<<grid_for(padded), TPB, 0, st>>>(
+        s.P.row_ptr.p, s.P.col.p, s.Psell_ptr.p, s.sell_perm.p, s.sell_inv.p, n, padded,
+        s.Psell_col.p);
+    FEEC_HIP(hipGetLastError());
+  }
 }
 
 void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
@@ -1528,7 +1599,7 @@ static void bind_fp64(KktSystem& s, MinresWork<double>& w, int storage) {
 static void bind_fp32(KktSystem& s, MinresWork<float>& w, int storage) {
   if (!s.val32.p) kkt_refresh_fp32(s);
   if (storage == FEEC_PC_STORE_CSR_HALF && s.P.nnz && !s.P16.p) kkt_refresh_half_csr(s);
-  if (storage == FEEC_PC_STORE_SELL && s.Psell.p)
+  if (storage == FEEC_PC_STORE_SELL && (s.Psell.p || s.Psell_v16.p))
     bind_preconditioner(s, w, s.P32.p, nullptr, s.sdinv32.p, true);
   else if (storage != FEEC_PC_STORE_CSR && s.P16.p)
     bind_preconditioner(s, w, s.P32.p, s.P16.p, s.sdinv32.p);
@@ -1537,9 +1608,10 @@ static void bind_fp32(KktSystem& s, MinresWork<float>& w, int storage) {
 }
 
 int kkt_preconditioner_format(const KktSystem& s, int precision, int storage) {
+  const bool sell = storage == FEEC_PC_STORE_SELL && s.Psell_ptr.p;
   if (precision == FEEC_PREC_FP64)
-    return (storage == FEEC_PC_STORE_SELL && s.Psell_ptr.p) ? FEEC_PC_SELL64_F64 : FEEC_PC_CSR_F64;
-  if (storage == FEEC_PC_STORE_SELL && s.Psell_ptr.p) return FEEC_PC_SELL64_F16;
+    return !sell ? FEEC_PC_CSR_F64 : s.sell_wide ? FEEC_PC_SELL64_F64_C32 : FEEC_PC_SELL64_F64;
+  if (sell) return s.sell_wide ? FEEC_PC_SELL64_F16_C32 : FEEC_PC_SELL64_F16;
   if (storage != FEEC_PC_STORE_CSR && s.P.nnz) return FEEC_PC_CSR_F16;
   return FEEC_PC_CSR_F32;
 }

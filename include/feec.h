@@ -83,9 +83,9 @@ typedef enum feec_precision {
 /* Storage of the preconditioner matrix the Chebyshev polynomial runs on. */
 typedef enum {
   FEEC_PC_STORE_SELL = 0,     /* D^-1/2 hdif_gram D^-1/2 as SELL-64 slices, off-diagonal entries
-                                 with int16 column offsets, unit diagonal implied: FP16 values on
-                                 the mixed path, FP64 on the FP64 path (CSR when an offset exceeds
-                                 +-32767) */
+                                 with int16 column offsets (int32 columns when an offset exceeds
+                                 +-32767), unit diagonal implied: FP16 values on the mixed path,
+                                 FP64 on the FP64 path */
   FEEC_PC_STORE_CSR_HALF = 1, /* mixed path: D^-1/2 hdif_gram D^-1/2 in CSR with FP16 values;
                                  FP64 path: as FEEC_PC_STORE_CSR */
   FEEC_PC_STORE_CSR = 2       /* D^-1 hdif_gram in CSR with FP32 (mixed) or FP64 values */
@@ -230,8 +230,11 @@ typedef enum {
   FEEC_PC_SELL64_F16 = 2, /* the same, off-diagonal only, packed {FP16, int16 offset} in 64-row
                              SELL slices */
   FEEC_PC_CSR_F64 = 3,    /* D^-1 hdif_gram, FP64 values, CSR */
-  FEEC_PC_SELL64_F64 = 4  /* D^-1/2 hdif_gram D^-1/2, off-diagonal only, FP64 values and int16
+  FEEC_PC_SELL64_F64 = 4, /* D^-1/2 hdif_gram D^-1/2, off-diagonal only, FP64 values and int16
                              offsets in 64-row SELL slices */
+  FEEC_PC_SELL64_F16_C32 = 5, /* as FEEC_PC_SELL64_F16 with FP16 values and int32 columns apart
+                                 (some column offset exceeds +-32767) */
+  FEEC_PC_SELL64_F64_C32 = 6  /* as FEEC_PC_SELL64_F64 with int32 columns */
 } feec_pc_format;
 
 /* The block preconditioner blockdiag(hdif_gram(k-1), hdif_gram(k)) as built:

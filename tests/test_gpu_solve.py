@@ -163,10 +163,10 @@ def test_preconditioner_application_per_storage(d, k, n):
 @pytest.mark.parametrize("precision", [fa.MIXED, fa.FP64])
 def test_wide_band_falls_back_to_csr(precision):
     """Randomly numbered vertices spread the rows' columns over the whole
-    matrix, so no 16-bit column offset fits: the preconditioner runs from CSR
-    with 32-bit columns, and the solve meets the certificate and agrees with
-    the same mesh numbered in grid order (edges matched through their
-    vertices, orientation signs included)."""
+    matrix, so no 16-bit column offset fits: the SELL slices keep int32
+    columns (and the CSR storages 32-bit columns), and every storage meets the
+    certificate and agrees with the same mesh numbered in grid order (edges
+    matched through their vertices, orientation signs included)."""
     cells, coords = kuhn_mesh(3, 20, 0.05, seed=8)
     nv = coords.shape[0]
     perm = np.random.default_rng(9).permutation(nv)  # old vertex id -> new id
@@ -190,10 +190,25 @@ def test_wide_band_falls_back_to_csr(precision):
         rep = prob.solve(f)
         assert rep.converged and rep.residual <= 1e-10, rep.as_dict()
         sols.append((prob.preconditioner()["format"], prob.solution()[1]))
-    assert sols[0][0].startswith("sell64")
-    assert sols[1][0] == ("csr_f16" if precision == fa.MIXED else "csr_f64")
-    u0, u1 = sols[0][1], sign * sols[1][1][match]
-    assert np.linalg.norm(u1 - u0) <= 1e-8 * np.linalg.norm(u0)
+    wide = "sell64_f16_c32" if precision == fa.MIXED else "sell64_f64_c32"
+    assert sols[0][0] == ("sell64_f16" if precision == fa.MIXED else "sell64_f64")
+    assert sols[1][0] == wide
+    u0 = sols[0][1]
+    assert np.linalg.norm(sign * sols[1][1][match] - u0) <= 1e-8 * np.linalg.norm(u0)
+    # the wide slices apply the same polynomial as the 32-bit-column CSR storages
+    _, wc_b, prob_b = out[1]
+    for store in (fa.PC_CSR_HALF, fa.PC_CSR):
+        p2 = fa.Problem(wc_b, 1, fa.solve_opts(precision=precision, precond_storage=store))
+        rep = p2.solve(fb_src)
+        assert rep.converged and rep.residual <= 1e-10, (store, rep.as_dict())
+        assert np.linalg.norm(sign * p2.solution()[1][match] - u0) <= 1e-8 * np.linalg.norm(u0)
+    r = np.sin(np.arange(sum(prob_b.sizes)) * 0.3)
+    z_sell = prob_b.apply_preconditioner(r, precision=fa.FP64, storage=fa.PC_SELL)
+    z_csr = prob_b.apply_preconditioner(r, precision=fa.FP64, storage=fa.PC_CSR)
+    assert np.abs(z_sell - z_csr).max() <= 1e-11 * np.abs(z_csr).max()
+    z16 = prob_b.apply_preconditioner(r, precision=fa.MIXED, storage=fa.PC_SELL)
+    z16c = prob_b.apply_preconditioner(r, precision=fa.MIXED, storage=fa.PC_CSR_HALF)
+    assert np.abs(z16 - z16c).max() <= 1e-5 * np.abs(z16c).max()
 
 
 def test_resident_problem_reassembly_and_device_source():
