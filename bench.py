@@ -178,6 +178,8 @@ def run_c4(args):
         out["mixed" if prec == fa.MIXED else "fp64"] = {
             "seconds": dt, "solves_per_s": len(res) / dt, "all_converged_1e-10": ok,
             "setup_and_elements_seconds": float(res[0].admit_time),
+            "solve_phase_seconds": float(res[0].solve_seconds),
+            "results_done_seconds": float(max(x.finish_time for x in res)),
             "cells": int(sum(x.ncells for x in res)), "dofs": int(sum(x.ndofs for x in res)),
             "iters": int(sum(x.iters for x in res)),
         }

@@ -16,26 +16,85 @@ namespace feec {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
-// Device memory comes from the device's default memory pool, stream-ordered on
-// the context's stream: a free does not wait for other contexts' work (a plain
-// hipFree synchronizes the whole device), and freed blocks stay in the pool
-// for the next allocation (its release threshold is raised when a context is
-// created).
-void* device_alloc(Ctx* ctx, size_t bytes) {
-  void* p = nullptr;
-  hipError_t e = hipMallocAsync(&p, bytes, ctx->stream);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    fail(e == hipErrorOutOfMemory ? FEEC_E_OOM : FEEC_E_DEVICE,
-         "hipMallocAsync(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+// Device memory: a caching allocator over the device's default stream-ordered
+// pool. A freed block goes to its context's cache, by size class, and is
+// reused by that context's later allocations, in its stream's order. When a
+// context is destroyed (its stream drained) its blocks move to a device-wide
+// cache that any context may draw from. So a request's set-up and tear-down
+// make no HIP call per buffer: hipFreeAsync costs about 17 us and serializes
+// across threads, and a plain hipFree synchronizes the whole device. Blocks
+// are returned to the pool only when an allocation fails for lack of memory.
+namespace {
+std::mutex g_cache_mu;
+std::map<int, std::map<size_t, std::vector<void*>>> g_cache;  // device -> class -> blocks
+
+// Size classes: multiples of 512 B up to 64 KiB, then 8 steps per power of two.
+size_t size_class(size_t b) {
+  if (b <= 65536) return (b + 511) & ~size_t(511);
+  int e = 63 - __builtin_clzll((unsigned long long)(b - 1));  // 2^e < b <= 2^(e+1)
+  const size_t step = (size_t)1 << (e - 2);
+  return (b + step - 1) / step * step;
+}
+
+void* pop(std::map<size_t, std::vector<void*>>& c, size_t cls) {
+  auto it = c.find(cls);
+  if (it == c.end() || it->second.empty()) return nullptr;
+  void* p = it->second.back();
+  it->second.pop_back();
+  return p;
+}
+
+// Give the cached blocks back to the pool: the device-wide ones (idle) and
+// this context's own (freed in its stream's order).
+void trim_caches(Ctx* ctx) {
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  for (auto* c : {&g_cache[ctx->device], &ctx->cache}) {
+    for (auto& kv : *c)
+      for (void* p : kv.second) (void)hipFreeAsync(p, ctx->stream);
+    c->clear();
   }
-  ctx->bytes += (int64_t)bytes;
+  (void)hipStreamSynchronize(ctx->stream);
+}
+}  // namespace
+
+void* device_alloc(Ctx* ctx, size_t bytes) {
+  const size_t cls = size_class(std::max<size_t>(bytes, 1));
+  void* p = pop(ctx->cache, cls);
+  if (!p) {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    p = pop(g_cache[ctx->device], cls);
+  }
+  if (!p) {
+    hipError_t e = hipMallocAsync(&p, cls, ctx->stream);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      trim_caches(ctx);
+      e = hipMallocAsync(&p, cls, ctx->stream);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(e == hipErrorOutOfMemory ? FEEC_E_OOM : FEEC_E_DEVICE,
+           "hipMallocAsync(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+    }
+  }
+  ctx->bytes += (int64_t)cls;
   return p;
 }
 
 void device_free(Ctx* ctx, void* p, size_t bytes) {
-  (void)hipFreeAsync(p, ctx->stream);
-  ctx->bytes -= (int64_t)bytes;
+  const size_t cls = size_class(std::max<size_t>(bytes, 1));
+  ctx->cache[cls].push_back(p);
+  ctx->bytes -= (int64_t)cls;
+}
+
+void release_cache(Ctx* ctx) {
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  auto& g = g_cache[ctx->device];
+  for (auto& kv : ctx->cache) {
+    auto& v = g[kv.first];
+    v.insert(v.end(), kv.second.begin(), kv.second.end());
+  }
+  ctx->cache.clear();
 }
 
 void copy_to_device(Ctx* c, void* dst, const void* src, size_t bytes) {
@@ -467,6 +526,25 @@ feec_report problem_solve_host(Problem* P, const double* src) {
   return r;
 }
 
+// Several problems (sources already uploaded by problem_prepare_rhs) solved
+// with their MINRES runs in lockstep on lead's stream.
+void problem_solve_batch(std::vector<Problem*>& probs, Ctx* lead, std::vector<feec_report>& reps) {
+  std::vector<KktSystem*> sys;
+  std::vector<const double*> b;
+  std::vector<double*> x;
+  std::vector<feec_solve_opts> o;
+  std::vector<SolverState**> ws;
+  for (Problem* P : probs) {
+    sys.push_back(&P->sys);
+    b.push_back(P->rhs.p);
+    x.push_back(P->x.p);
+    o.push_back(P->opts);
+    ws.push_back(&P->ws);
+  }
+  kkt_solve_batch(sys, b, x, o, ws, reps, lead);
+  for (size_t i = 0; i < probs.size(); ++i) probs[i]->last = reps[i];
+}
+
 void problem_set_report(Problem* P, const feec_report& r) { P->last = r; }
 
 void problem_u(Problem* P, double* u) {
@@ -563,6 +641,7 @@ void feec_ctx_destroy(feec_ctx* ctx) {
   if (ctx->dflag) device_free(ctx, ctx->dflag, sizeof(int));
   if (ctx->dcount) device_free(ctx, ctx->dcount, sizeof(unsigned long long));
   (void)hipStreamSynchronize(ctx->stream);
+  release_cache(ctx);  // idle blocks, usable from any stream
   (void)hipStreamDestroy(ctx->stream);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   delete ctx;

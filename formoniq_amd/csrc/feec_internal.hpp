@@ -141,8 +141,15 @@ struct Ctx {
   // pinned host words for reading solver scalars back, owned by the context
   void* pinned = nullptr;
   static constexpr size_t kPinnedBytes = 4096;
+  // device blocks this context freed, by size class, for its own next
+  // allocations (reuse in the stream's order); handed to the device-wide
+  // cache when the context is destroyed
+  std::map<size_t, std::vector<void*>> cache;
   void sync() { FEEC_HIP(hipStreamSynchronize(stream)); }
 };
+// Return a destroyed context's cached blocks (its stream drained) to the
+// device-wide cache.
+void release_cache(Ctx* ctx);
 
 // Copies ordered on the context's stream; the host buffer may be reused on
 // return (the stream is synchronized). Device memory is allocated and freed
@@ -330,6 +337,12 @@ void kkt_refresh_sell64(KktSystem& s);
 feec_report kkt_solve(KktSystem& s, const double* b_dev, double* x_dev, const feec_solve_opts& o,
                       SolverState** ws);
 void solver_state_free(SolverState* ws);
+// Several solves whose MINRES runs advance in lockstep on lead's stream (one
+// launch per phase for all of them); each result is bit-identical to
+// kkt_solve's. Solves the lockstep path cannot take run one by one.
+void kkt_solve_batch(std::vector<KktSystem*>& systems, std::vector<const double*>& b,
+                     std::vector<double*>& x, const std::vector<feec_solve_opts>& opts,
+                     std::vector<SolverState**>& ws, std::vector<feec_report>& reps, Ctx* lead);
 void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y);
 double time_minres_spmv(KktSystem& s, bool fp32, int reps);
 // z = B r with the block preconditioner as a solve of `precision` with

@@ -174,6 +174,12 @@ __device__ __forceinline__ double block_reduce(double v, double* sh) {
   return r;  // valid on thread 0
 }
 
+// The (block, grid) a kernel body runs as: the launch's own for a standalone
+// launch; one system's share of the grid for a lockstep batch launch.
+struct Blk {
+  int64_t id, dim;
+};
+
 // y = A x, LANES lanes per row, accumulation in T
 template <class T>
 __global__ void __launch_bounds__(TPB) k_spmv(const int64_t* __restrict__ rp,
@@ -186,6 +192,70 @@ __global__ void __launch_bounds__(TPB) k_spmv(const int64_t* __restrict__ rp,
        row += groups) {
     T acc = row_dot<T>(rp, col, val, x, row, lane);
     if (lane == 0) y[row] = acc;
+  }
+}
+
+// y = diag(scale) (A x), FP64 (one power-iteration step of D^-1 A)
+__global__ void __launch_bounds__(TPB) k_spmv_scaled(const int64_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ val, int64_t n,
+                                                     const double* __restrict__ scale,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ y) {
+  const int lane = threadIdx.x % LANES;
+  const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
+  for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
+       row += groups) {
+    double acc = row_dot<double>(rp, col, val, x, row, lane);
+    if (lane == 0) y[row] = scale[row] * acc;
+  }
+}
+
+// Partial |y|^2 and |x|^2 of both diagonal blocks in one launch: blocks
+// [0, g0) cover rows [0, split) and blocks [g0, g0 + g1) rows [split, n), each
+// range strided over its own blocks (the partials of two separate dot
+// launches over the ranges).
+__global__ void __launch_bounds__(TPB) k_power_sumsq(const double* __restrict__ y,
+                                                     const double* __restrict__ x, int64_t split,
+                                                     int64_t n, int g0, double* __restrict__ py,
+                                                     double* __restrict__ px) {
+  __shared__ double sh[TPB / 64];
+  const bool b0 = (int)blockIdx.x < g0;
+  const int64_t lo = b0 ? 0 : split, hi = b0 ? split : n;
+  const int64_t lb = b0 ? blockIdx.x : blockIdx.x - g0;
+  const int64_t g = b0 ? g0 : gridDim.x - g0;
+  double vy = 0.0, vx = 0.0;
+  for (int64_t i = lo + lb * TPB + threadIdx.x; i < hi; i += g * TPB) {
+    vy += y[i] * y[i];
+    vx += x[i] * x[i];
+  }
+  const double ty = block_reduce(vy, sh);
+  __syncthreads();
+  const double tx = block_reduce(vx, sh);
+  if (threadIdx.x == 0) {
+    py[blockIdx.x] = ty;
+    px[blockIdx.x] = tx;
+  }
+}
+
+// The two blocks' sums in fixed order (block b of the launch: matrix block b):
+// sq[2b] = |y_b|^2, sq[2b + 1] = |x_b|^2.
+__global__ void __launch_bounds__(TPB) k_power_finish(const double* __restrict__ py,
+                                                      const double* __restrict__ px, int g0,
+                                                      int g1, double* __restrict__ sq) {
+  __shared__ double sh[TPB / 64];
+  const int off = blockIdx.x == 0 ? 0 : g0, np = blockIdx.x == 0 ? g0 : g1;
+  double vy = 0.0, vx = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    vy += py[off + i];
+    vx += px[off + i];
+  }
+  const double ty = block_reduce(vy, sh);
+  __syncthreads();
+  const double tx = block_reduce(vx, sh);
+  if (threadIdx.x == 0) {
+    sq[2 * blockIdx.x] = ty;
+    sq[2 * blockIdx.x + 1] = tx;
   }
 }
 
@@ -257,7 +327,7 @@ __global__ void k_mr_init(const T* __restrict__ b, const T* __restrict__ dinv, i
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-__global__ void k_mr_start(MinresS* S, const double* part, int np, double rtol, long long max_iters) {
+__device__ __forceinline__ void k_mr_start_body(MinresS* S, const double* part, int np, double rtol, long long max_iters, Blk bk) {
   __shared__ double sh[TPB / 64];
   double v = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
@@ -287,21 +357,25 @@ __global__ void k_mr_start(MinresS* S, const double* part, int np, double rtol, 
   }
 }
 
+__global__ void k_mr_start(MinresS* S, const double* part, int np, double rtol, long long max_iters) {
+  k_mr_start_body(S, part, np, rtol, max_iters, Blk{blockIdx.x, (int64_t)gridDim.x});
+}
+
 // yn = A (y/beta) - (beta/oldb) r1, written over r1; partial alpha = (y/beta).yn
 template <class T>
-__global__ void __launch_bounds__(TPB) k_mr_spmv(const int64_t* __restrict__ rp,
+__device__ __forceinline__ void k_mr_spmv_body(const int64_t* __restrict__ rp,
                                                  const int32_t* __restrict__ col,
                                                  const T* __restrict__ val, int64_t n,
                                                  const T* __restrict__ y, T* __restrict__ r1,
-                                                 const MinresS* __restrict__ S, double* part) {
+                                                 const MinresS* __restrict__ S, double* part, Blk bk) {
   if (S->done) return;
   __shared__ double sh[TPB / 64];
   const T invb = (T)(1.0 / S->beta);
   const T cr = (S->iter >= 1) ? (T)(S->beta / S->oldb) : (T)0;
   const int lane = threadIdx.x % LANES;
-  const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
+  const int64_t groups = (int64_t)bk.dim * (TPB / LANES);
   double loc = 0.0;
-  for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
+  for (int64_t row = bk.id * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
        row += groups) {
     T acc = row_dot<T>(rp, col, val, y, row, lane);
     if (lane == 0) {
@@ -311,37 +385,52 @@ __global__ void __launch_bounds__(TPB) k_mr_spmv(const int64_t* __restrict__ rp,
     }
   }
   double t = block_reduce(loc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  if (threadIdx.x == 0) part[bk.id] = t;
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_mr_spmv(const int64_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ col,
+                                                 const T* __restrict__ val, int64_t n,
+                                                 const T* __restrict__ y, T* __restrict__ r1,
+                                                 const MinresS* __restrict__ S, double* part) {
+  k_mr_spmv_body<T>(rp, col, val, n, y, r1, S, part, Blk{blockIdx.x, (int64_t)gridDim.x});
 }
 
 // Harmonic border (rank nh): rows/cols n..n+nh-1 of the KKT system.
 // partial dots y[0:n] . border_h
 template <class T>
-__global__ void k_mr_border_dot(const T* __restrict__ border, int nh, int64_t n,
-                                const T* __restrict__ y, const MinresS* S, double* part) {
+__device__ __forceinline__ void k_mr_border_dot_body(const T* __restrict__ border, int nh, int64_t n,
+                                const T* __restrict__ y, const MinresS* S, double* part, Blk bk) {
   if (S->done) return;
   __shared__ double sh[TPB / 64];
   for (int h = 0; h < nh; ++h) {
     double v = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = bk.id * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)bk.dim * blockDim.x)
       v += to_d(border[h * n + i]) * to_d(y[i]);
     double t = block_reduce(v, sh);
-    if (threadIdx.x == 0) part[h * NBLK + blockIdx.x] = t;
+    if (threadIdx.x == 0) part[h * NBLK + bk.id] = t;
     __syncthreads();
   }
 }
 
+template <class T>
+__global__ void k_mr_border_dot(const T* __restrict__ border, int nh, int64_t n,
+                                const T* __restrict__ y, const MinresS* S, double* part) {
+  k_mr_border_dot_body<T>(border, nh, n, y, S, part, Blk{blockIdx.x, (int64_t)gridDim.x});
+}
+
 // border rows of yn and the alpha correction (one block)
 template <class T>
-__global__ void k_mr_border_rows(int nh, int64_t n, const T* y, T* r1, MinresS* S,
-                                 const double* part) {
+__device__ __forceinline__ void k_mr_border_rows_body(int nh, int64_t n, const T* y, T* r1, MinresS* S,
+                                 const double* part, int np, Blk bk) {
   if (S->done) return;
   __shared__ double sh[TPB / 64];
   __shared__ double dh[8];
   for (int h = 0; h < nh; ++h) {
     double v = 0.0;
-    for (int i = threadIdx.x; i < NBLK; i += blockDim.x) v += part[h * NBLK + i];
+    for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[h * NBLK + i];
     double t = block_reduce(v, sh);
     if (threadIdx.x == 0) dh[h] = t;
     __syncthreads();
@@ -360,27 +449,43 @@ __global__ void k_mr_border_rows(int nh, int64_t n, const T* y, T* r1, MinresS* 
   }
 }
 
+template <class T>
+__global__ void k_mr_border_rows(int nh, int64_t n, const T* y, T* r1, MinresS* S,
+                                 const double* part, int np) {
+  k_mr_border_rows_body<T>(nh, n, y, r1, S, part, np, Blk{blockIdx.x, (int64_t)gridDim.x});
+}
+
 // yn[0:n] += (y[n+h]/beta) border_h
 template <class T>
-__global__ void k_mr_border_axpy(const T* __restrict__ border, int nh, int64_t n,
-                                 const T* __restrict__ y, T* __restrict__ r1, const MinresS* S) {
+__device__ __forceinline__ void k_mr_border_axpy_body(const T* __restrict__ border, int nh, int64_t n,
+                                 const T* __restrict__ y, T* __restrict__ r1, const MinresS* S, Blk bk) {
   if (S->done) return;
   const T invb = (T)(1.0 / S->beta);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = bk.id * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)bk.dim * blockDim.x) {
     T acc = r1[i];
     for (int h = 0; h < nh; ++h) acc += (y[n + h] * invb) * border[h * n + i];
     r1[i] = acc;
   }
 }
 
-__global__ void k_mr_alpha(MinresS* S, const double* part, int np, int with_extra) {
+template <class T>
+__global__ void k_mr_border_axpy(const T* __restrict__ border, int nh, int64_t n,
+                                 const T* __restrict__ y, T* __restrict__ r1, const MinresS* S) {
+  k_mr_border_axpy_body<T>(border, nh, n, y, r1, S, Blk{blockIdx.x, (int64_t)gridDim.x});
+}
+
+__device__ __forceinline__ void k_mr_alpha_body(MinresS* S, const double* part, int np, int with_extra, Blk bk) {
   if (S->done) return;
   __shared__ double sh[TPB / 64];
   double v = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
   double t = block_reduce(v, sh);
   if (threadIdx.x == 0) S->alfa = t + (with_extra ? S->extra_alpha : 0.0);
+}
+
+__global__ void k_mr_alpha(MinresS* S, const double* part, int np, int with_extra) {
+  k_mr_alpha_body(S, part, np, with_extra, Blk{blockIdx.x, (int64_t)gridDim.x});
 }
 
 // yn -= (alpha/beta) r2; z = dinv * yn; partial beta^2 = yn.z
@@ -406,7 +511,7 @@ __global__ void __launch_bounds__(TPB) k_mr_update(T* __restrict__ yn, const T* 
 }
 
 // Givens rotation (krylov.rs:168-186), one thread after the beta^2 reduction.
-__global__ void k_mr_rotate(MinresS* S, const double* part, int np) {
+__device__ __forceinline__ void k_mr_rotate_body(MinresS* S, const double* part, int np, Blk bk) {
   if (S->done) return;
   __shared__ double sh[TPB / 64];
   double v = 0.0;
@@ -439,21 +544,32 @@ __global__ void k_mr_rotate(MinresS* S, const double* part, int np) {
   *S = s;
 }
 
+__global__ void k_mr_rotate(MinresS* S, const double* part, int np) {
+  k_mr_rotate_body(S, part, np, Blk{blockIdx.x, (int64_t)gridDim.x});
+}
+
 // w_new = (y_old/beta_old - oldeps w2 - delta w)/gamma -> into w2's buffer; x += phi w_new.
 // Runs in the same iteration as the rotation; the done flag is raised after it.
 template <class T>
-__global__ void __launch_bounds__(TPB) k_mr_wx(const T* __restrict__ yold, T* __restrict__ w,
+__device__ __forceinline__ void k_mr_wx_body(const T* __restrict__ yold, T* __restrict__ w,
                                                T* __restrict__ w2, T* __restrict__ x, int64_t n,
-                                               MinresS* S) {
+                                               MinresS* S, Blk bk) {
   if (S->done) return;
   const T ib = (T)S->invb_prev, oe = (T)S->oldeps, de = (T)S->delta, ig = (T)S->invg,
           ph = (T)S->phi;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = bk.id * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)bk.dim * blockDim.x) {
     T wn = (yold[i] * ib - oe * w2[i] - de * w[i]) * ig;
     w2[i] = wn;
     x[i] += ph * wn;
   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_mr_wx(const T* __restrict__ yold, T* __restrict__ w,
+                                               T* __restrict__ w2, T* __restrict__ x, int64_t n,
+                                               MinresS* S) {
+  k_mr_wx_body<T>(yold, w, w2, x, n, S, Blk{blockIdx.x, (int64_t)gridDim.x});
 }
 
 // --- Chebyshev block preconditioner ---------------------------------------
@@ -646,17 +762,17 @@ __device__ __forceinline__ T sell_row(E E_, const int64_t* __restrict__ sp, int6
 // row perm[i]; the renumbering keeps both blocks in place, so i < split tells
 // the block)
 template <class T>
-__global__ void __launch_bounds__(TPB) k_cheb3_init(const T* __restrict__ r,
+__device__ __forceinline__ void k_cheb3_init_body(const T* __restrict__ r,
                                                     const T* __restrict__ sd,
                                                     const int32_t* __restrict__ perm,
                                                     T* __restrict__ f, T* __restrict__ x0,
                                                     T* __restrict__ x1, int64_t n, int64_t split,
                                                     const double* __restrict__ coef,
-                                                    const MinresS* __restrict__ S) {
+                                                    const MinresS* __restrict__ S, Blk bk) {
   if (S && S->done) return;
   const T s0 = (T)coef[0], s1 = (T)coef[1];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = bk.id * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)bk.dim * blockDim.x) {
     const int64_t j = perm[i];
     const T fi = r[j] * sd[j];
     f[i] = fi;
@@ -665,20 +781,31 @@ __global__ void __launch_bounds__(TPB) k_cheb3_init(const T* __restrict__ r,
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(TPB) k_cheb3_init(const T* __restrict__ r,
+                                                    const T* __restrict__ sd,
+                                                    const int32_t* __restrict__ perm,
+                                                    T* __restrict__ f, T* __restrict__ x0,
+                                                    T* __restrict__ x1, int64_t n, int64_t split,
+                                                    const double* __restrict__ coef,
+                                                    const MinresS* __restrict__ S) {
+  k_cheb3_init_body<T>(r, sd, perm, f, x0, x1, n, split, coef, S, Blk{blockIdx.x, (int64_t)gridDim.x});
+}
+
 // The Lanczos update yn += c r2 (c = -alpha/beta) fused with k_cheb3_init:
 // lane i updates row perm[i] (a bijection on rows < n), so each row is read
 // and written by one lane; the harmonic rows (>= n) are updated in place.
 template <class T>
-__global__ void __launch_bounds__(TPB) k_mr_update_cheb3_init(
+__device__ __forceinline__ void k_mr_update_cheb3_init_body(
     T* __restrict__ yn, const T* __restrict__ r2, const T* __restrict__ sd,
     const int32_t* __restrict__ perm, T* __restrict__ f, T* __restrict__ x0, T* __restrict__ x1,
     int64_t n, int64_t N, int64_t split, const double* __restrict__ coef,
-    const MinresS* __restrict__ S) {
+    const MinresS* __restrict__ S, Blk bk) {
   if (S->done) return;
   const T c = (T)(-S->alfa / S->beta);
   const T s0 = (T)coef[0], s1 = (T)coef[1];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = bk.id * (int64_t)blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)bk.dim * blockDim.x) {
     const int64_t j = i < n ? perm[i] : i;
     const T a = yn[j] + c * r2[j];
     yn[j] = a;
@@ -689,6 +816,15 @@ __global__ void __launch_bounds__(TPB) k_mr_update_cheb3_init(
       x1[i] = fi * (i < split ? s0 : s1);
     }
   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_mr_update_cheb3_init(
+    T* __restrict__ yn, const T* __restrict__ r2, const T* __restrict__ sd,
+    const int32_t* __restrict__ perm, T* __restrict__ f, T* __restrict__ x0, T* __restrict__ x1,
+    int64_t n, int64_t N, int64_t split, const double* __restrict__ coef,
+    const MinresS* __restrict__ S) {
+  k_mr_update_cheb3_init_body<T>(yn, r2, sd, perm, f, x0, x1, n, N, split, coef, S, Blk{blockIdx.x, (int64_t)gridDim.x});
 }
 
 // Workgroups are dispatched to the 8 XCDs of the chip round-robin (workgroup b
@@ -703,14 +839,14 @@ __device__ __forceinline__ int64_t xcd_block() {
 
 // One lane per row, one block per four 64-row slices: x0 <- x_{k+2}.
 template <class T, class E>
-__global__ void __launch_bounds__(TPB) k_cheb3_step(E entries, const int64_t* __restrict__ sp,
+__device__ __forceinline__ void k_cheb3_step_body(E entries, const int64_t* __restrict__ sp,
                                                     int64_t n, int64_t split,
                                                     const T* __restrict__ f,
                                                     const T* __restrict__ x1, T* __restrict__ x0,
                                                     const double* __restrict__ coef, int step,
-                                                    const MinresS* __restrict__ S) {
+                                                    const MinresS* __restrict__ S, Blk bk) {
   if (S && S->done) return;
-  const int64_t row = xcd_block() * TPB + threadIdx.x;
+  const int64_t row = bk.id * TPB + threadIdx.x;
   if (row >= n) return;
   const double* c = coef + 2 + 4 * step + (row < split ? 0 : 2);
   const T a = (T)c[0], b = (T)c[1];
@@ -719,8 +855,40 @@ __global__ void __launch_bounds__(TPB) k_cheb3_step(E entries, const int64_t* __
   x0[row] = xi + a * (xi - x0[row]) + b * (f[row] - (xi + acc));
 }
 
+template <class T, class E>
+__global__ void __launch_bounds__(TPB) k_cheb3_step(E entries, const int64_t* __restrict__ sp,
+                                                    int64_t n, int64_t split,
+                                                    const T* __restrict__ f,
+                                                    const T* __restrict__ x1, T* __restrict__ x0,
+                                                    const double* __restrict__ coef, int step,
+                                                    const MinresS* __restrict__ S) {
+  k_cheb3_step_body<T, E>(entries, sp, n, split, f, x1, x0, coef, step, S, Blk{xcd_block(), (int64_t)gridDim.x});
+}
+
 // z = sd x_m on rows < n (x in the SELL row order) and z = r on the harmonic
 // rows; partial dot r.z
+template <class T>
+__device__ __forceinline__ void k_cheb3_finish_body(const T* __restrict__ r,
+                                                      const T* __restrict__ x,
+                                                      const T* __restrict__ sd,
+                                                      const int32_t* __restrict__ inv,
+                                                      T* __restrict__ z, int64_t n, int64_t N,
+                                                      const MinresS* __restrict__ S,
+                                                      double* part, Blk bk) {
+  if (S && S->done) return;
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int64_t i = bk.id * (int64_t)blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)bk.dim * blockDim.x) {
+    const T ri = r[i];
+    const T zi = i < n ? sd[i] * x[inv[i]] : ri;
+    z[i] = zi;
+    v += to_d(ri) * to_d(zi);
+  }
+  double t = block_reduce(v, sh);
+  if (threadIdx.x == 0) part[bk.id] = t;
+}
+
 template <class T>
 __global__ void __launch_bounds__(TPB) k_cheb3_finish(const T* __restrict__ r,
                                                       const T* __restrict__ x,
@@ -729,24 +897,13 @@ __global__ void __launch_bounds__(TPB) k_cheb3_finish(const T* __restrict__ r,
                                                       T* __restrict__ z, int64_t n, int64_t N,
                                                       const MinresS* __restrict__ S,
                                                       double* part) {
-  if (S && S->done) return;
-  __shared__ double sh[TPB / 64];
-  double v = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const T ri = r[i];
-    const T zi = i < n ? sd[i] * x[inv[i]] : ri;
-    z[i] = zi;
-    v += to_d(ri) * to_d(zi);
-  }
-  double t = block_reduce(v, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  k_cheb3_finish_body<T>(r, x, sd, inv, z, n, N, S, part, Blk{blockIdx.x, (int64_t)gridDim.x});
 }
 
 template <class T>
-__global__ void k_mr_zero(const T* __restrict__ b, int64_t n, T* r1, T* r2, T* x, T* w, T* w2) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void k_mr_zero_body(const T* __restrict__ b, int64_t n, T* r1, T* r2, T* x, T* w, T* w2, Blk bk) {
+  for (int64_t i = bk.id * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)bk.dim * blockDim.x) {
     T bi = b[i];
     r1[i] = bi;
     r2[i] = bi;
@@ -756,9 +913,18 @@ __global__ void k_mr_zero(const T* __restrict__ b, int64_t n, T* r1, T* r2, T* x
   }
 }
 
-__global__ void k_mr_check(MinresS* S) {
+template <class T>
+__global__ void k_mr_zero(const T* __restrict__ b, int64_t n, T* r1, T* r2, T* x, T* w, T* w2) {
+  k_mr_zero_body<T>(b, n, r1, r2, x, w, w2, Blk{blockIdx.x, (int64_t)gridDim.x});
+}
+
+__device__ __forceinline__ void k_mr_check_body(MinresS* S, Blk bk) {
   if (S->done) return;
   if (S->residual <= S->rtol || S->iter >= S->max_iters) S->done = 1;
+}
+
+__global__ void k_mr_check(MinresS* S) {
+  k_mr_check_body(S, Blk{blockIdx.x, (int64_t)gridDim.x});
 }
 
 // --- refinement helpers --------------------------------------------------------
@@ -818,14 +984,17 @@ __global__ void k_jacobi32(const float* r, const float* dinv, float* z, int64_t 
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < nn) z[i] = dinv[i] * r[i];
 }
-// Power-iteration step of one block: with ny = sqrt(sq[0]), nx = sqrt(sq[1]),
-// lambda = ny / nx (when nx > 0) and x = y / ny (when ny > 0).
-__global__ void k_power_normalize(const double* y, int64_t n, const double* sq, double* x,
-                                  double* lam) {
-  const double ny = sqrt(sq[0]), nx = sqrt(sq[1]);
+// Power-iteration step of both blocks: with ny = sqrt(sq[2b]) and nx =
+// sqrt(sq[2b + 1]) of row i's block b, lambda_b = ny / nx (when nx > 0) and
+// x = y / ny (when ny > 0).
+__global__ void k_power_normalize(const double* y, int64_t split, int64_t n, const double* sq,
+                                  double* x, double* lam) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i == 0 && nx > 0) *lam = ny / nx;
-  if (ny > 0 && i < n) x[i] = (1.0 / ny) * y[i];
+  if (i >= n) return;
+  const int b = i < split ? 0 : 1;
+  const double ny = sqrt(sq[2 * b]), nx = sqrt(sq[2 * b + 1]);
+  if (i == (b ? split : 0) && nx > 0) lam[b] = ny / nx;
+  if (ny > 0) x[i] = (1.0 / ny) * y[i];
 }
 __global__ void k_cg_p(double* pp, const double* zz, double bb, int64_t nn) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1058,8 +1227,8 @@ static void minres_iteration(Enq& q, MinresWork<T>& w, const int64_t* rp, const 
   T* w2 = par ? w.w.p : w.w2.p;
   q(k_mr_spmv<T>, spmv_grid(n), TPB, rp, col, val, n, y, r1, w.S.p, w.part.p);
   if (w.nh) {
-    q(k_mr_border_dot<T>, NBLK, TPB, w.border.p, (int)w.nh, n, y, w.S.p, w.part.p + NBLK);
-    q(k_mr_border_rows<T>, 1, TPB, (int)w.nh, n, y, r1, w.S.p, w.part.p + NBLK);
+    q(k_mr_border_dot<T>, red_grid(n), TPB, w.border.p, (int)w.nh, n, y, w.S.p, w.part.p + NBLK);
+    q(k_mr_border_rows<T>, 1, TPB, (int)w.nh, n, y, r1, w.S.p, w.part.p + NBLK, (int)red_grid(n));
     q(k_mr_border_axpy<T>, red_grid(n), TPB, w.border.p, (int)w.nh, n, y, r1, w.S.p);
   }
   q(k_mr_alpha, 1, TPB, w.S.p, w.part.p, spmv_grid(n), w.nh ? 1 : 0);
@@ -1497,27 +1666,21 @@ void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
   copy_to_device(ctx, x.p, hx.data(), n * 8);
   // per block: |y|^2 and |x|^2 (fixed-order sums) and lambda, all on the device
   // so the 30 iterations run without a host round trip
-  DBuf<double> sq(ctx, 4), dlam(ctx, 2);
+  // four launches per iteration: y = D^-1 P x; per block |y|^2, |x|^2 (fixed
+  // order); lambda_b and x = y / |y| per block
+  DBuf<double> sq(ctx, 4), dlam(ctx, 2), py(ctx, 2 * NBLK), px(ctx, 2 * NBLK);
   double lam[2] = {1.0, 1.0};
   copy_to_device(ctx, dlam.p, lam, sizeof(lam));
-  auto block_sumsq = [&](const double* v, int64_t lo, int64_t hi, double* out) {
-    k_dot<double><<<red_grid(hi - lo), TPB, 0, st>>>(v + lo, v + lo, hi - lo, part.p);
-    k_finish_sum<<<1, TPB, 0, st>>>(part.p, red_grid(hi - lo), out);
-  };
-  const int64_t lo[2] = {0, s.split}, hi[2] = {s.split, n};
+  const int g0 = s.split > 0 ? (int)red_grid(s.split) : 0;
+  const int g1 = n > s.split ? (int)red_grid(n - s.split) : 0;
   for (int it = 0; it < 30; ++it) {
-    spmv64(ctx, s.P, x.p, y.p);
-    k_jacobi<<<grid_for(n), TPB, 0, st>>>(s.inv_diag.p, y.p, y.p, n);
-    // lambda per block from |y|/|x|, then x = y normalized per block
-    for (int b = 0; b < 2; ++b) {
-      if (hi[b] <= lo[b]) continue;
-      block_sumsq(y.p, lo[b], hi[b], sq.p + 2 * b);
-      block_sumsq(x.p, lo[b], hi[b], sq.p + 2 * b + 1);
-      k_power_normalize<<<grid_for(hi[b] - lo[b]), TPB, 0, st>>>(y.p + lo[b], hi[b] - lo[b],
-                                                                sq.p + 2 * b, x.p + lo[b],
-                                                                dlam.p + b);
-    }
+    k_spmv_scaled<<<spmv_grid(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p, n,
+                                                s.inv_diag.p, x.p, y.p);
+    k_power_sumsq<<<g0 + g1, TPB, 0, st>>>(y.p, x.p, s.split, n, g0, py.p, px.p);
+    k_power_finish<<<2, TPB, 0, st>>>(py.p, px.p, g0, g1, sq.p);
+    k_power_normalize<<<grid_for(n), TPB, 0, st>>>(y.p, s.split, n, sq.p, x.p, dlam.p);
   }
+  FEEC_HIP(hipGetLastError());
   copy_to_host(ctx, lam, dlam.p, sizeof(lam));
   // interval lower ends (fraction of b) tuned on 3D k=1 Kuhn meshes
   const double frac =
@@ -1616,107 +1779,572 @@ int kkt_preconditioner_format(const KktSystem& s, int precision, int storage) {
   return FEEC_PC_CSR_F32;
 }
 
-feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_solve_opts& o,
-                      SolverState** wsp) {
+// One solve as a sequence of MINRES runs with residual checks in between, so
+// that a batch can run the MINRES part of many solves in lockstep. begin()
+// sets up, prepare() fills w.b and the run's tolerance and budget (false when
+// the solve is over), post() takes the run's result and checks the true
+// residual, end() fills the report.
+struct SolveJob {
+  KktSystem* s = nullptr;
+  const double* b = nullptr;
+  double* x = nullptr;
+  feec_solve_opts o{};
+  SolverState* ss = nullptr;
+  feec_report rep{};
+  double bnorm = 0.0, rnorm = 0.0, res = 1.0;
+  int steps = 0;
+  bool done = false;
+  double tol = 0.0;
+  int64_t budget = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool fp64() const { return o.precision == FEEC_PREC_FP64; }
+};
+
+static void job_begin(SolveJob& j, SolverState** wsp) {
+  KktSystem& s = *j.s;
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
   if (!*wsp) {
     *wsp = new SolverState();
     (*wsp)->host_s = static_cast<MinresS*>(ctx->pinned);
   }
-  SolverState& ss = **wsp;
+  j.ss = *wsp;
+  SolverState& ss = *j.ss;
   const int64_t n = s.n, nh = s.nh, N = n + nh;
   if ((int64_t)ss.r.n != N) {
     ss.r.alloc(ctx, N);
     ss.x.alloc(ctx, N);
     ss.part.alloc(ctx, NBLK * 8);
   }
-  feec_report rep{};
-  hipEvent_t e0, e1;
-  FEEC_HIP(hipEventCreate(&e0));
-  FEEC_HIP(hipEventCreate(&e1));
-  FEEC_HIP(hipEventRecord(e0, st));
-
-  const double bnorm = device_norm(ctx, ss, b, N);
-  FEEC_HIP(hipMemsetAsync(x_out, 0, N * sizeof(double), st));
-  if (bnorm == 0.0) {
-    rep.converged = 1;
-    rep.residual = 0.0;
-    rep.precond_residual = 0.0;
-  } else if (o.precision == FEEC_PREC_FP64) {
-    // FP64 MINRES on the reference's stopping quantity, then FP64 residual
-    // correction steps until the true residual meets rtol.
+  FEEC_HIP(hipEventCreate(&j.e0));
+  FEEC_HIP(hipEventCreate(&j.e1));
+  FEEC_HIP(hipEventRecord(j.e0, st));
+  j.bnorm = device_norm(ctx, ss, j.b, N);
+  FEEC_HIP(hipMemsetAsync(j.x, 0, N * sizeof(double), st));
+  if (j.bnorm == 0.0) {
+    j.done = true;
+    j.rep.converged = 1;
+    j.rep.residual = 0.0;
+    j.rep.precond_residual = 0.0;
+    j.res = 0.0;
+    return;
+  }
+  if (j.fp64()) {
     MinresWork<double>& w = ss.w64;
     minres_alloc(ctx, w, n, nh);
-    bind_fp64(s, w, o.precond_storage);
+    bind_fp64(s, w, j.o.precond_storage);
     FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
     if (nh) FEEC_HIP(hipMemcpyAsync(w.border.p, s.border.p, nh * n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    FEEC_HIP(hipMemcpyAsync(w.b.p, b, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-    double res = 1.0;
-    int steps = 0;
-    while (true) {
-      int64_t budget = o.max_iters - rep.iters;
-      if (budget <= 0) break;
-      // a correction only has to close the remaining gap (relative to its own
-      // right-hand side, whose norm is res * |b|), with a margin of 10 for the
-      // gap between the preconditioner norm MINRES monitors and the 2-norm
-      const double tol = steps == 0 ? o.rtol : std::min(0.5, 0.1 * o.rtol / res);
-      KrylovStats ks = minres_run<double>(ctx, ss, w, s.A.row_ptr.p, s.A.col.p, s.A.val.p,
-                                          tol, budget, o);
-      rep.iters += ks.iters;
-      rep.precond_residual = ks.precond_residual;
-      k_axpy<<<grid_for(N), TPB, 0, st>>>(w.x.p, N, 1.0, x_out);
-      res = true_residual(s, ss, b, x_out, bnorm);
-      if (res <= o.rtol || steps >= o.max_refinements) break;
-      ++steps;
-      FEEC_HIP(hipMemcpyAsync(w.b.p, ss.r.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-    }
-    rep.refinements = steps;
-    rep.residual = res;
-    rep.converged = res <= o.rtol;
+    FEEC_HIP(hipMemcpyAsync(w.b.p, j.b, N * sizeof(double), hipMemcpyDeviceToDevice, st));
   } else {
     MinresWork<float>& w = ss.w32;
     minres_alloc(ctx, w, n, nh);
-    bind_fp32(s, w, o.precond_storage);
+    bind_fp32(s, w, j.o.precond_storage);
     FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag32.p, N * sizeof(float), hipMemcpyDeviceToDevice, st));
     if (nh) k_f64_to_f32<<<grid_for(nh * n), TPB, 0, st>>>(s.border.p, nh * n, w.border.p);
     // r = b
-    FEEC_HIP(hipMemcpyAsync(ss.r.p, b, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-    double rnorm = bnorm, res = 1.0;
-    int steps = 0;
-    while (steps < o.max_refinements) {
-      int64_t budget = o.max_iters - rep.iters;
-      if (budget <= 0) break;
-      k_to_f32_scaled<<<grid_for(N), TPB, 0, st>>>(ss.r.p, N, 1.0 / rnorm, w.b.p);
-      // the last refinement step only has to close the remaining gap (margin
-      // of 10, as for the FP64 corrections)
-      const double tol = std::max(o.inner_rtol, std::min(0.5, 0.1 * o.rtol / res));
-      KrylovStats ks = minres_run<float>(ctx, ss, w, s.A.row_ptr.p, s.A.col.p, s.val32.p,
-                                         tol, budget, o);
-      rep.iters += ks.iters;
-      rep.precond_residual = ks.precond_residual;
-      k_axpy_from_f32<<<grid_for(N), TPB, 0, st>>>(w.x.p, N, rnorm, x_out);
-      ++steps;
-      res = true_residual(s, ss, b, x_out, bnorm);
-      if (res <= o.rtol) break;
-      rnorm = res * bnorm;
-    }
-    rep.refinements = steps;
-    rep.residual = res;
-    rep.converged = res <= o.rtol;
+    FEEC_HIP(hipMemcpyAsync(ss.r.p, j.b, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    j.rnorm = j.bnorm;
   }
-  FEEC_HIP(hipEventRecord(e1, st));
-  FEEC_HIP(hipEventSynchronize(e1));
-  float ms = 0.f;
-  FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
-  rep.seconds = ms * 1e-3;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return rep;
 }
 
-// Average duration of the fused MINRES SpMV launch (the kernel that moves the
-// matrix) on this system, FP32 or FP64, from HIP events on the context stream.
+static bool job_prepare(SolveJob& j) {
+  if (j.done) return false;
+  const feec_solve_opts& o = j.o;
+  if (!j.fp64() && j.steps >= o.max_refinements) {
+    j.done = true;
+    return false;
+  }
+  j.budget = o.max_iters - j.rep.iters;
+  if (j.budget <= 0) {
+    j.done = true;
+    return false;
+  }
+  if (j.fp64()) {
+    // FP64 MINRES on the reference's stopping quantity, then FP64 residual
+    // correction steps until the true residual meets rtol. A correction only
+    // has to close the remaining gap (relative to its own right-hand side,
+    // whose norm is res * |b|), with a margin of 10 for the gap between the
+    // preconditioner norm MINRES monitors and the 2-norm.
+    j.tol = j.steps == 0 ? o.rtol : std::min(0.5, 0.1 * o.rtol / j.res);
+  } else {
+    const int64_t N = j.s->n + j.s->nh;
+    k_to_f32_scaled<<<grid_for(N), TPB, 0, j.s->ctx->stream>>>(j.ss->r.p, N, 1.0 / j.rnorm,
+                                                                j.ss->w32.b.p);
+    // the last refinement step only has to close the remaining gap (margin
+    // of 10, as for the FP64 corrections)
+    j.tol = std::max(o.inner_rtol, std::min(0.5, 0.1 * o.rtol / j.res));
+  }
+  return true;
+}
+
+static void job_post(SolveJob& j, const KrylovStats& ks) {
+  KktSystem& s = *j.s;
+  SolverState& ss = *j.ss;
+  hipStream_t st = s.ctx->stream;
+  const int64_t N = s.n + s.nh;
+  j.rep.iters += ks.iters;
+  j.rep.precond_residual = ks.precond_residual;
+  if (j.fp64()) {
+    k_axpy<<<grid_for(N), TPB, 0, st>>>(ss.w64.x.p, N, 1.0, j.x);
+    j.res = true_residual(s, ss, j.b, j.x, j.bnorm);
+    if (j.res <= j.o.rtol || j.steps >= j.o.max_refinements) {
+      j.done = true;
+      return;
+    }
+    ++j.steps;
+    FEEC_HIP(hipMemcpyAsync(ss.w64.b.p, ss.r.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  } else {
+    k_axpy_from_f32<<<grid_for(N), TPB, 0, st>>>(ss.w32.x.p, N, j.rnorm, j.x);
+    ++j.steps;
+    j.res = true_residual(s, ss, j.b, j.x, j.bnorm);
+    if (j.res <= j.o.rtol) {
+      j.done = true;
+      return;
+    }
+    j.rnorm = j.res * j.bnorm;
+  }
+}
+
+static feec_report job_end(SolveJob& j) {
+  if (j.bnorm != 0.0) {
+    j.rep.refinements = j.steps;
+    j.rep.residual = j.res;
+    j.rep.converged = j.res <= j.o.rtol;
+  }
+  hipStream_t st = j.s->ctx->stream;
+  FEEC_HIP(hipEventRecord(j.e1, st));
+  FEEC_HIP(hipEventSynchronize(j.e1));
+  float ms = 0.f;
+  FEEC_HIP(hipEventElapsedTime(&ms, j.e0, j.e1));
+  j.rep.seconds = ms * 1e-3;
+  (void)hipEventDestroy(j.e0);
+  (void)hipEventDestroy(j.e1);
+  j.e0 = j.e1 = nullptr;
+  return j.rep;
+}
+
+static KrylovStats job_run(SolveJob& j) {
+  KktSystem& s = *j.s;
+  SolverState& ss = *j.ss;
+  if (j.fp64())
+    return minres_run<double>(s.ctx, ss, ss.w64, s.A.row_ptr.p, s.A.col.p, s.A.val.p, j.tol,
+                              j.budget, j.o);
+  return minres_run<float>(s.ctx, ss, ss.w32, s.A.row_ptr.p, s.A.col.p, s.val32.p, j.tol, j.budget,
+                           j.o);
+}
+
+// ---------------------------------------------------------------------------
+// lockstep batch: the MINRES runs of many solves advance together, one launch
+// per phase covering every active system. Each system gets exactly the blocks
+// a standalone launch would give it (its own grid, the same block -> row map
+// and the same partial sums), so its iterates are bit-identical to a
+// standalone solve's. Systems whose run has finished drop out of the launch
+// tables between segments.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int64_t red_blocks(int64_t n) {
+  const int64_t b = (n + TPB - 1) / TPB;
+  return b < 1 ? 1 : (b > NBLK ? NBLK : b);
+}
+__host__ __device__ inline int64_t spmv_blocks(int64_t n) {
+  const int64_t b = (n + TPB / LANES - 1) / (TPB / LANES);
+  return b < 1 ? 1 : (b > NBLK ? NBLK : b);
+}
+__host__ __device__ inline int64_t for_blocks(int64_t n) {
+  const int64_t b = (n + TPB - 1) / TPB;
+  return b < 1 ? 1 : b;
+}
+
+template <class T>
+struct BatchSys {
+  int64_t n, nh, N, split;
+  const int64_t* rp;
+  const int32_t* col;
+  const T* val;
+  T *r1, *r2, *y, *z, *w, *w2, *x, *b, *border, *res, *d0, *d1;
+  const T* sd;
+  const int32_t* perm;
+  const int32_t* inv;
+  typename MinresWork<T>::SellN sell;
+  typename MinresWork<T>::SellW sellw;
+  int wide, cheb_m;
+  const int64_t* psell_ptr;
+  const double* coef;
+  double *part, *part2;
+  MinresS* S;
+  double rtol;
+  long long max_iters;
+};
+
+// The system whose share of the launch holds this block (block starts are a
+// prefix sum over the active systems; empty shares are skipped).
+__device__ __forceinline__ int batch_find(const int64_t* starts, int ns) {
+  const int64_t bid = blockIdx.x;
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (starts[mid] <= bid) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define FEEC_BATCH_SYS                                                                 \
+  const int si_ = batch_find(starts, ns);                                              \
+  const BatchSys<T>& d = sys[si_];                                                     \
+  const Blk bk{(int64_t)blockIdx.x - starts[si_], starts[si_ + 1] - starts[si_]};      \
+  (void)bk;
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_zero(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+  FEEC_BATCH_SYS
+  k_mr_zero_body<T>(d.b, d.N, d.r1, d.r2, d.x, d.w, d.w2, bk);
+}
+
+// first polynomial application of a run (r = b into y; no MINRES scalars yet)
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_cheb3_init0(const BatchSys<T>* sys, const int64_t* starts,
+                                                      int ns) {
+  FEEC_BATCH_SYS
+  k_cheb3_init_body<T>(d.b, d.sd, d.perm, d.res, d.d0, d.d1, d.n, d.split, d.coef,
+                       (const MinresS*)nullptr, bk);
+}
+
+// step k of the polynomial; systems of lower degree sit it out
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_cheb3_step(const BatchSys<T>* sys, const int64_t* starts,
+                                                     int ns, int k, int with_s) {
+  FEEC_BATCH_SYS
+  if (k + 1 >= d.cheb_m) return;
+  const T* a = (k & 1) ? d.d0 : d.d1;
+  T* b = (k & 1) ? d.d1 : d.d0;
+  const MinresS* S = with_s ? d.S : nullptr;
+  if (d.wide)
+    k_cheb3_step_body<T, typename MinresWork<T>::SellW>(d.sellw, d.psell_ptr, d.n, d.split, d.res,
+                                                        a, b, d.coef, k, S, bk);
+  else
+    k_cheb3_step_body<T, typename MinresWork<T>::SellN>(d.sell, d.psell_ptr, d.n, d.split, d.res,
+                                                        a, b, d.coef, k, S, bk);
+}
+
+// finish: initial (r = b, z = y, partials into part) or in-iteration (par)
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_cheb3_finish(const BatchSys<T>* sys, const int64_t* starts,
+                                                       int ns, int initial, int par) {
+  FEEC_BATCH_SYS
+  const T* x = ((d.cheb_m - 1) & 1) ? d.d0 : d.d1;
+  if (initial)
+    k_cheb3_finish_body<T>(d.b, x, d.sd, d.inv, d.y, d.n, d.N, (const MinresS*)nullptr, d.part, bk);
+  else
+    k_cheb3_finish_body<T>(par ? d.r2 : d.r1, x, d.sd, d.inv, par ? d.y : d.z, d.n, d.N, d.S,
+                           d.part2, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_start(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+  FEEC_BATCH_SYS
+  k_mr_start_body(d.S, d.part, (int)red_blocks(d.N), d.rtol, d.max_iters, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_spmv(const BatchSys<T>* sys, const int64_t* starts, int ns,
+                                               int par) {
+  FEEC_BATCH_SYS
+  k_mr_spmv_body<T>(d.rp, d.col, d.val, d.n, par ? d.z : d.y, par ? d.r2 : d.r1, d.S, d.part, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_border_dot(const BatchSys<T>* sys, const int64_t* starts,
+                                                     int ns, int par) {
+  FEEC_BATCH_SYS
+  k_mr_border_dot_body<T>(d.border, (int)d.nh, d.n, par ? d.z : d.y, d.S, d.part + NBLK, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_border_rows(const BatchSys<T>* sys, const int64_t* starts,
+                                                      int ns, int par) {
+  FEEC_BATCH_SYS
+  k_mr_border_rows_body<T>((int)d.nh, d.n, par ? d.z : d.y, par ? d.r2 : d.r1, d.S, d.part + NBLK,
+                           (int)red_blocks(d.n), bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_border_axpy(const BatchSys<T>* sys, const int64_t* starts,
+                                                      int ns, int par) {
+  FEEC_BATCH_SYS
+  k_mr_border_axpy_body<T>(d.border, (int)d.nh, d.n, par ? d.z : d.y, par ? d.r2 : d.r1, d.S, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_alpha(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+  FEEC_BATCH_SYS
+  k_mr_alpha_body(d.S, d.part, (int)spmv_blocks(d.n), d.nh ? 1 : 0, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_update_cheb3_init(const BatchSys<T>* sys,
+                                                            const int64_t* starts, int ns, int par) {
+  FEEC_BATCH_SYS
+  k_mr_update_cheb3_init_body<T>(par ? d.r2 : d.r1, par ? d.r1 : d.r2, d.sd, d.perm, d.res, d.d0,
+                                 d.d1, d.n, d.N, d.split, d.coef, d.S, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_rotate(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+  FEEC_BATCH_SYS
+  k_mr_rotate_body(d.S, d.part2, (int)red_blocks(d.N), bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_wx(const BatchSys<T>* sys, const int64_t* starts, int ns,
+                                             int par) {
+  FEEC_BATCH_SYS
+  k_mr_wx_body<T>(par ? d.z : d.y, par ? d.w2 : d.w, par ? d.w : d.w2, d.x, d.N, d.S, bk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) kb_check(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+  FEEC_BATCH_SYS
+  if (threadIdx.x == 0) k_mr_check_body(d.S, bk);
+}
+
+template <class T>
+__global__ void kb_gather_s(const BatchSys<T>* sys, int ns, MinresS* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ns) out[i] = *sys[i].S;
+}
+
+template <class T>
+static BatchSys<T> batch_sys(MinresWork<T>& w, KktSystem& s, const T* val, double rtol,
+                             int64_t max_iters) {
+  BatchSys<T> d{};
+  d.n = w.n;
+  d.nh = w.nh;
+  d.N = w.n + w.nh;
+  d.split = w.split;
+  d.rp = s.A.row_ptr.p;
+  d.col = s.A.col.p;
+  d.val = val;
+  d.r1 = w.r1.p; d.r2 = w.r2.p; d.y = w.y.p; d.z = w.z.p; d.w = w.w.p; d.w2 = w.w2.p;
+  d.x = w.x.p; d.b = w.b.p; d.border = w.border.p; d.res = w.res.p; d.d0 = w.d0.p; d.d1 = w.d1.p;
+  d.sd = w.sd;
+  d.perm = w.sell_perm;
+  d.inv = w.sell_inv;
+  d.sell = w.sell;
+  d.sellw = w.sellw;
+  d.wide = w.sell_wide ? 1 : 0;
+  d.cheb_m = w.cheb_m;
+  d.psell_ptr = w.psell_ptr;
+  d.coef = w.coef;
+  d.part = w.part.p;
+  d.part2 = w.part2.p;
+  d.S = w.S.p;
+  d.rtol = rtol;
+  d.max_iters = (long long)max_iters;
+  return d;
+}
+
+// Launch tables of one set of systems: the BatchSys array and, per grid
+// shape, the prefix sums of the systems' block counts.
+template <class T>
+struct BatchTables {
+  enum { RED_N, RED_n, SPMV, FOR_n, FOR_N, ONE, BORDER_RED, BORDER_ONE, NSHAPE };
+  int ns = 0;
+  int max_m = 0;
+  bool any_border = false;
+  DBuf<BatchSys<T>> sys;
+  DBuf<int64_t> starts;  // [NSHAPE][ns + 1]
+  int64_t total[NSHAPE] = {};
+  std::vector<std::vector<int64_t>> hstart;  // host copy of the prefix sums per shape
+  std::vector<int> ns_step;                 // systems (a prefix) whose degree reaches step k
+  void build(Ctx* ctx, const std::vector<BatchSys<T>>& v) {
+    ns = (int)v.size();
+    max_m = 0;
+    any_border = false;
+    std::vector<int64_t> h((size_t)NSHAPE * (ns + 1), 0);
+    for (int i = 0; i < ns; ++i) {
+      const BatchSys<T>& d = v[i];
+      max_m = std::max(max_m, d.cheb_m);
+      any_border = any_border || d.nh > 0;
+      const int64_t g[NSHAPE] = {red_blocks(d.N), red_blocks(d.n), spmv_blocks(d.n), for_blocks(d.n),
+                                 for_blocks(d.N), 1, d.nh ? red_blocks(d.n) : 0, d.nh ? 1 : 0};
+      for (int k = 0; k < NSHAPE; ++k) h[(size_t)k * (ns + 1) + i + 1] = h[(size_t)k * (ns + 1) + i] + g[k];
+    }
+    for (int k = 0; k < NSHAPE; ++k) total[k] = h[(size_t)k * (ns + 1) + ns];
+    hstart.assign(NSHAPE, {});
+    for (int k = 0; k < NSHAPE; ++k)
+      hstart[k].assign(h.begin() + (size_t)k * (ns + 1), h.begin() + (size_t)(k + 1) * (ns + 1));
+    ns_step.assign(std::max(max_m, 1), 0);
+    for (int k = 0; k + 1 < max_m; ++k) {
+      int c = 0;
+      while (c < ns && v[c].cheb_m > k + 1) ++c;  // v is sorted by decreasing degree
+      ns_step[k] = c;
+    }
+    sys.alloc(ctx, std::max(ns, 1));
+    starts.alloc(ctx, h.size());
+    copy_to_device(ctx, sys.p, v.data(), v.size() * sizeof(BatchSys<T>));
+    copy_to_device(ctx, starts.p, h.data(), h.size() * sizeof(int64_t));
+  }
+  const int64_t* st(int k) const { return starts.p + (size_t)k * (ns + 1); }
+};
+
+// Step k over the leading systems (sorted by decreasing degree) whose degree
+// reaches it.
+template <class T>
+static void batch_steps(hipStream_t st, const BatchTables<T>& t, int k, int with_s) {
+  const int ns_k = t.ns_step[k];
+  if (ns_k <= 0) return;
+  const int64_t blocks = t.hstart[BatchTables<T>::FOR_n][ns_k];
+  if (blocks <= 0) return;
+  kb_cheb3_step<T><<<(unsigned)blocks, TPB, 0, st>>>(t.sys.p, t.st(BatchTables<T>::FOR_n), ns_k, k,
+                                                     with_s);
+}
+
+template <class T, class K, class... A>
+static void batch_launch(hipStream_t st, K kernel, const BatchTables<T>& t, int shape, A... a) {
+  const int64_t blocks = t.total[shape];
+  if (blocks <= 0 || t.ns == 0) return;
+  kernel<<<(unsigned)blocks, TPB, 0, st>>>(t.sys.p, t.st(shape), t.ns, a...);
+}
+
+// One MINRES run (w.b filled, preconditioner bound, SELL storage and a
+// polynomial of degree >= 2) of each system, in lockstep on ctx's stream.
+template <class T>
+static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T>>& all, std::vector<KrylovStats>& out) {
+  using Tab = BatchTables<T>;
+  hipStream_t st = ctx->stream;
+  out.assign(all.size(), KrylovStats{});
+  if (all.empty()) return;
+  // systems in order of decreasing degree: step k of the polynomial then
+  // launches only over the leading systems whose degree reaches it
+  std::vector<int> order(all.size());
+  for (size_t i = 0; i < all.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return all[a].cheb_m > all[b].cheb_m; });
+  std::vector<BatchSys<T>> sorted;
+  for (int i : order) sorted.push_back(all[i]);
+  Tab t;
+  t.build(ctx, sorted);
+  // start of the run: r1 = r2 = b, y = B b, beta1
+  batch_launch(st, kb_zero<T>, t, Tab::RED_N);
+  batch_launch(st, kb_cheb3_init0<T>, t, Tab::RED_n);
+  for (int k = 0; k + 1 < t.max_m; ++k) batch_steps(st, t, k, 0);
+  batch_launch(st, kb_cheb3_finish<T>, t, Tab::RED_N, 1, 0);
+  batch_launch(st, kb_start<T>, t, Tab::ONE);
+  FEEC_HIP(hipGetLastError());
+  std::vector<int> active = order;
+  DBuf<MinresS> hs_dev(ctx, all.size());
+  std::vector<MinresS> hs(all.size());
+  constexpr int SEG = 16;
+  while (!active.empty()) {
+    for (int it = 0; it < SEG; ++it) {
+      const int par = it & 1;
+      batch_launch(st, kb_spmv<T>, t, Tab::SPMV, par);
+      if (t.any_border) {
+        batch_launch(st, kb_border_dot<T>, t, Tab::BORDER_RED, par);
+        batch_launch(st, kb_border_rows<T>, t, Tab::BORDER_ONE, par);
+        batch_launch(st, kb_border_axpy<T>, t, Tab::BORDER_RED, par);
+      }
+      batch_launch(st, kb_alpha<T>, t, Tab::ONE);
+      batch_launch(st, kb_update_cheb3_init<T>, t, Tab::RED_N, par);
+      for (int k = 0; k + 1 < t.max_m; ++k) batch_steps(st, t, k, 1);
+      batch_launch(st, kb_cheb3_finish<T>, t, Tab::RED_N, 0, par);
+      batch_launch(st, kb_rotate<T>, t, Tab::ONE);
+      batch_launch(st, kb_wx<T>, t, Tab::FOR_N, par);
+      batch_launch(st, kb_check<T>, t, Tab::ONE);
+    }
+    FEEC_HIP(hipGetLastError());
+    kb_gather_s<T><<<(unsigned)((t.ns + TPB - 1) / TPB), TPB, 0, st>>>(t.sys.p, t.ns, hs_dev.p);
+    FEEC_HIP(hipGetLastError());
+    copy_to_host(ctx, hs.data(), hs_dev.p, t.ns * sizeof(MinresS));
+    std::vector<int> still;
+    std::vector<BatchSys<T>> v;
+    for (int i = 0; i < t.ns; ++i) {
+      const int id = active[i];
+      if (hs[i].done) {
+        out[id].iters = hs[i].iter;
+        out[id].precond_residual = hs[i].residual;
+        out[id].converged = hs[i].residual <= all[id].rtol;
+      } else {
+        still.push_back(id);
+        v.push_back(all[id]);
+      }
+    }
+    if (still.size() != active.size() && !still.empty()) t.build(ctx, v);
+    active.swap(still);
+  }
+}
+
+// Whether a solve's MINRES runs can go through the lockstep path.
+static bool lockstep_eligible(const SolveJob& j) {
+  if (!j.ss) return false;
+  if (j.fp64()) return j.ss->w64.has_sell && j.ss->w64.sd && j.ss->w64.cheb_m >= 2;
+  return j.ss->w32.has_sell && j.ss->w32.sd && j.ss->w32.cheb_m >= 2;
+}
+
+void kkt_solve_batch(std::vector<KktSystem*>& systems, std::vector<const double*>& b,
+                     std::vector<double*>& x, const std::vector<feec_solve_opts>& opts,
+                     std::vector<SolverState**>& ws, std::vector<feec_report>& reps, Ctx* lead) {
+  const size_t m = systems.size();
+  std::vector<SolveJob> jobs(m);
+  for (size_t i = 0; i < m; ++i) {
+    jobs[i].s = systems[i];
+    jobs[i].b = b[i];
+    jobs[i].x = x[i];
+    jobs[i].o = opts[i];
+    job_begin(jobs[i], ws[i]);
+  }
+  while (true) {
+    std::vector<size_t> run32, run64, solo;
+    for (size_t i = 0; i < m; ++i) {
+      if (!job_prepare(jobs[i])) continue;
+      if (!lockstep_eligible(jobs[i]))
+        solo.push_back(i);
+      else if (jobs[i].fp64())
+        run64.push_back(i);
+      else
+        run32.push_back(i);
+    }
+    if (run32.empty() && run64.empty() && solo.empty()) break;
+    for (size_t i : solo) job_post(jobs[i], job_run(jobs[i]));
+    // the systems' own streams have filled w.b; the lockstep runs on lead's
+    for (size_t i : run32) jobs[i].s->ctx->sync();
+    for (size_t i : run64) jobs[i].s->ctx->sync();
+    std::vector<KrylovStats> ks;
+    if (!run32.empty()) {
+      std::vector<BatchSys<float>> v;
+      for (size_t i : run32)
+        v.push_back(batch_sys<float>(jobs[i].ss->w32, *jobs[i].s, jobs[i].s->val32.p, jobs[i].tol,
+                                     jobs[i].budget));
+      minres_lockstep<float>(lead, v, ks);
+      lead->sync();
+      for (size_t a = 0; a < run32.size(); ++a) job_post(jobs[run32[a]], ks[a]);
+    }
+    if (!run64.empty()) {
+      std::vector<BatchSys<double>> v;
+      for (size_t i : run64)
+        v.push_back(batch_sys<double>(jobs[i].ss->w64, *jobs[i].s, jobs[i].s->A.val.p, jobs[i].tol,
+                                      jobs[i].budget));
+      minres_lockstep<double>(lead, v, ks);
+      lead->sync();
+      for (size_t a = 0; a < run64.size(); ++a) job_post(jobs[run64[a]], ks[a]);
+    }
+  }
+  reps.resize(m);
+  for (size_t i = 0; i < m; ++i) reps[i] = job_end(jobs[i]);
+}
+
+feec_report kkt_solve(KktSystem& s, const double* b, double* x_out, const feec_solve_opts& o,
+                      SolverState** wsp) {
+  SolveJob j;
+  j.s = &s;
+  j.b = b;
+  j.x = x_out;
+  j.o = o;
+  job_begin(j, wsp);
+  while (job_prepare(j)) job_post(j, job_run(j));
+  return job_end(j);
+}
+
 // Mean seconds per launch of `reps` launches replayed from one graph, as the
 // solve replays its kernels (after one warm-up replay).
 template <class Build>
@@ -1752,6 +2380,8 @@ static double time_graph(hipStream_t st, int reps, Build&& build) {
   return ms * 1e-3 / reps;
 }
 
+// Mean duration of the fused MINRES SpMV launch (the kernel that moves the
+// matrix) on this system, FP32 or FP64, replayed from a graph as in the solve.
 double time_minres_spmv(KktSystem& s, bool fp32, int reps) {
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;

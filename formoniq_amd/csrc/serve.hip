@@ -9,7 +9,10 @@
 // worker threads, each with its own context (HIP stream); then one persistent
 // launch computes the element blocks of all requests, of every dimension,
 // from a (mesh, 64-cell range) work queue; then the workers scatter values and
-// solve, so solves of different requests overlap on the device.
+// upload sources, and the MINRES runs of all requests advance in lockstep, one
+// launch per phase covering every request that is still running (each with
+// the blocks and partial sums of a solve on its own, so results equal single
+// solves bit for bit).
 //
 // Server: `max_inflight` slots, each a worker thread with its own stream. A
 // queued request is admitted when a slot is free and its estimated resident
@@ -40,6 +43,8 @@ void problem_element_outputs(Problem& P, double** outs);
 Problem* problem_new(Geom* g, int bc, int k, const feec_solve_opts& o);
 void problem_delete(Problem* P);
 feec_report problem_solve_host(Problem* P, const double* src);
+void problem_prepare_rhs(Problem* P, const double* src);
+void problem_solve_batch(std::vector<Problem*>& probs, Ctx* lead, std::vector<feec_report>& reps);
 void problem_set_report(Problem* P, const feec_report& r);
 void problem_u(Problem* P, double* u);
 int64_t problem_ndofs(Problem* P);
@@ -278,7 +283,7 @@ feec_status feec_batch_run(feec_ctx* ctx, int nreq, const feec_request* reqs,
     };
     auto t0 = clk::now();
     // 1. meshes, numbering and plans
-    std::vector<double> t_build(nreq, 0.0), t_solve(nreq, 0.0);
+    std::vector<double> t_build(nreq, 0.0);
     for_workers([&](int i, feec_ctx* c) {
       const auto tb = clk::now();
       inst[i] = std::make_unique<Instance>();
@@ -299,15 +304,44 @@ feec_status feec_batch_run(feec_ctx* ctx, int nreq, const feec_request* reqs,
     kkt_element_blocks_batched(ctx, br);
     ctx->sync();
     const double t_elements = seconds_since(t0);
-    // 3. values, preconditioner and solve, overlapped across worker streams
+    // 3. values, preconditioner and sources per request on the worker
+    //    streams; then every solve with its MINRES runs in lockstep (one
+    //    launch per phase for all requests); then the results
     std::vector<double> t_done(nreq, 0.0);
     for_workers([&](int i, feec_ctx*) {
-      const auto ts = clk::now();
       problem_assemble_from_elements(*inst[i]->prob);
+      std::vector<double> src(inst[i]->nk);
+      check_status(feec_normal_vector(inst[i]->nk, inst[i]->req.source_seed, src.data()));
+      problem_prepare_rhs(inst[i]->prob, src.data());
+    });
+    for (auto* c : wctx) c->sync();
+    const double t_assembled = seconds_since(t0);
+    std::vector<Problem*> probs;
+    std::vector<int> owner;
+    for (int i = 0; i < nreq; ++i)
+      if (status[i] == FEEC_OK) {
+        probs.push_back(inst[i]->prob);
+        owner.push_back(i);
+      }
+    std::vector<feec_report> reps;
+    if (!probs.empty()) {
+      feec_status bs = guard([&] { problem_solve_batch(probs, ctx, reps); });
+      if (bs != FEEC_OK) {
+        const std::string msg = feec_last_error();
+        for (int i : owner) {
+          status[i] = bs;
+          errors[i] = msg;
+        }
+      }
+    }
+    const double t_solved = seconds_since(t0);
+    std::vector<feec_report> rep_of(nreq);
+    for (size_t a = 0; a < owner.size() && a < reps.size(); ++a) rep_of[owner[a]] = reps[a];
+    for_workers([&](int i, feec_ctx*) {
       results[i] = feec_result{};
-      instance_solve(*inst[i], results[i], nullptr);
+      instance_result(*inst[i], rep_of[i], results[i], nullptr);
+      results[i].solve_seconds = t_solved - t_assembled;
       t_done[i] = seconds_since(t0);
-      t_solve[i] = seconds_since(ts);
     });
     for (int i = 0; i < nreq; ++i) {
       if (status[i] != FEEC_OK) {
@@ -321,10 +355,13 @@ feec_status feec_batch_run(feec_ctx* ctx, int nreq, const feec_request* reqs,
       results[i].finish_time = t_done[i];
       results[i].latency = t_done[i];
       results[i].build_seconds = t_build[i];
-      results[i].solve_seconds = t_solve[i];
     }
+    // tear down on the workers too (hundreds of stream-ordered frees per request)
+    for_workers([&](int i, feec_ctx*) { inst[i].reset(); });
     inst.clear();
-    for (auto* c : wctx) feec_ctx_destroy(c);
+    std::vector<std::thread> th;
+    for (auto* c : wctx) th.emplace_back([c] { feec_ctx_destroy(c); });
+    for (auto& t : th) t.join();
   });
 }
 

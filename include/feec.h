@@ -304,11 +304,14 @@ typedef struct feec_result {
   double latency;       /* finish - submit, seconds */
   double build_seconds; /* this request's mesh, numbering, plans and assembly (server) or
                            mesh, numbering and plans (batch) */
-  double solve_seconds; /* its Krylov solve with refinement (batch: values and solve) */
+  double solve_seconds; /* its Krylov solve with refinement (batch: the lockstep solve of the
+                           whole batch) */
 } feec_result;
 
 /* Batched: all element matrices of all requests in one persistent launch over
- * a (mesh, cell-range) work queue, then the solves. */
+ * a (mesh, cell-range) work queue, then the solves with their MINRES runs in
+ * lockstep (one launch per phase for every request still running); results
+ * equal single solves bit for bit. */
 feec_status feec_batch_run(feec_ctx *ctx, int nreq, const feec_request *reqs,
                            const feec_solve_opts *opts, feec_result *results);
 

@@ -52,6 +52,21 @@ def test_batch_run_matches_standalone():
         assert abs(r.u_checksum - checksum(u)) <= 1e-7 * np.abs(u).sum()
 
 
+def test_batch_lockstep_is_bit_identical_to_single_solves():
+    """feec_batch_run advances the MINRES runs of all its requests in lockstep
+    (one launch per phase); every request still gets the blocks, partial sums
+    and iterates of a solve on its own, so iteration counts, residuals and
+    the solution's norm and checksum equal the server's single solves bit for
+    bit (mixed and FP64, harmonic border rows included)."""
+    res = fa.batch_run(MIX)
+    single = run_server(MIX, 1)
+    for r, (s_, _), q in zip(res, single, MIX):
+        assert r.status == 0 and s_.status == 0
+        assert (r.iters, r.refinements) == (s_.iters, s_.refinements), (q.dim, q.grade)
+        assert r.residual == s_.residual, (q.dim, q.grade)
+        assert r.u_norm == s_.u_norm and r.u_checksum == s_.u_checksum, (q.dim, q.grade)
+
+
 def run_server(reqs, inflight, budget=32 << 30):
     opts = fa.ServerOpts()
     opts.hbm_budget_bytes = budget
