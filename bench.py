@@ -9,10 +9,14 @@ coordinates, source cochain N(0,1) on the edges (seed 2).
 
 One step = re-assemble every KKT element block and the CSR values from the
 resident geometry (pattern reused), then solve the KKT system to a true
-relative residual |b - A x|/|b| <= 1e-10 with FP32 MINRES inside FP64
-iterative refinement. `value` is completed steps per second (solves/s) with
-all inputs resident in HBM. The plain FP64 GPU path (FP64 matrix, vectors and
-MINRES, same preconditioner, same 1e-10 certificate) is timed beside it.
+relative residual |b - A x|/|b| <= 1e-10. The headline solve is mixed
+precision: FP64 MINRES and KKT SpMV with the Chebyshev block preconditioner
+run in FP32 arithmetic on its FP16 SELL-64 copy (FEEC_PREC_FP64_PC32;
+`--mixed-mode refine` selects FP32 MINRES inside FP64 iterative refinement,
+FEEC_PREC_MIXED, which is timed beside it either way). `value` is completed
+steps per second (solves/s) with all inputs resident in HBM. The plain FP64
+GPU path (FP64 matrix, vectors, MINRES and polynomial, same 1e-10
+certificate) is timed beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -276,7 +280,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64, help="cells per axis (the workload is 64)")
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=1,
+                    help="timed steps of the FP64 path and of the other mixed-precision mode")
+    ap.add_argument("--mixed-mode", default="pc32", choices=["pc32", "refine"],
+                    help="headline solve: FP64 MINRES with the FP32 polynomial on FP16 storage (pc32), "
+                         "or FP32 MINRES inside FP64 refinement (refine)")
     ap.add_argument("--fp64-cheb", type=int, default=24, help="Chebyshev degree of the FP64 path")
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -316,8 +324,12 @@ def main():
 
     t0 = time.perf_counter()
     use_graph = 0 if args.no_graph else 1
-    mixed = fa.Problem(wc, 1, fa.solve_opts(precision=fa.MIXED, use_graph=use_graph))
-    log(f"mixed problem: sizes {mixed.sizes}, nnz {mixed.nnz}, setup {time.perf_counter() - t0:.2f}s")
+    head_prec = fa.FP64_PC32 if args.mixed_mode == "pc32" else fa.MIXED
+    other_prec = fa.MIXED if head_prec == fa.FP64_PC32 else fa.FP64_PC32
+    mode_name = {fa.FP64_PC32: "fp64_pc32", fa.MIXED: "fp32_refine"}
+    mixed = fa.Problem(wc, 1, fa.solve_opts(precision=head_prec, use_graph=use_graph))
+    log(f"mixed problem ({mode_name[head_prec]}): sizes {mixed.sizes}, nnz {mixed.nnz}, "
+        f"setup {time.perf_counter() - t0:.2f}s")
     n_kkt = mixed.sizes[0] + mixed.sizes[1]
 
     # source resident on the device before timing
@@ -347,9 +359,27 @@ def main():
     times = mixed.kernel_times(reps=10)
     log(f"kernel times: {times}")
 
+    # the other mixed-precision mode on the same workload
+    alt = fa.Problem(wc, 1, fa.solve_opts(precision=other_prec, use_graph=use_graph))
+    alt.upload_source(f)
+    alt.assemble()
+    r_alt = alt.solve()
+    t_alt = []
+    for i in range(args.fp64_steps):
+        ts = time.perf_counter()
+        alt.assemble()
+        r_alt = alt.solve()
+        ctx.synchronize()
+        t_alt.append(time.perf_counter() - ts)
+        log(f"{mode_name[other_prec]} step {i}: {t_alt[-1]:.3f}s iters {r_alt.iters} "
+            f"refinements {r_alt.refinements} residual {r_alt.residual:.2e}")
+    assert r_alt.converged and r_alt.residual <= 1e-10
+    alt_rate = 1.0 / statistics.mean(t_alt)
+    del alt
+
     # plain FP64 GPU path
     # each path at the Chebyshev degree that is fastest for it on this
-    # workload (FP64: 24; the mixed path runs the default, 32)
+    # workload (FP64: 24; the mixed modes run the default, 64)
     fp64 = fa.Problem(wc, 1, fa.solve_opts(precision=fa.FP64, cheb_degree=args.fp64_cheb,
                                            use_graph=use_graph))
     fp64.upload_source(f)
@@ -388,11 +418,12 @@ def main():
 
     # roofline of the dominant kernel of the mixed solve: the FP32 Chebyshev
     # step (one preconditioner SpMV fused with the polynomial recurrence);
-    # the fused MINRES SpMV is reported beside it
+    # the fused MINRES SpMVs are reported beside it
     nnz = mixed.nnz
     pc = mixed.preconditioner()
     p_nnz, cheb_deg, p_col_bytes = pc["nnz"], pc["degree"], pc["column_bytes"]
-    spmv_bytes = 8 * nnz + 8 * (n_kkt + 1) + 12 * n_kkt
+    spmv_bytes = 8 * nnz + 8 * (n_kkt + 1) + 12 * n_kkt  # FP32 values, int32 columns
+    spmv64_bytes = 12 * nnz + 8 * (n_kkt + 1) + 24 * n_kkt
     if pc["format"] in ("sell64_f16", "sell64_f16_c32"):
         # three-term step on the SELL-64 copy: 4 B per off-diagonal nonzero
         # (FP16 value + int16 offset; 6 B with int32 columns; padding is not
@@ -433,13 +464,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 assembly, f32 krylov + f64 refinement",
+        "dtype": ("f64 assembly, f64 minres, f32 preconditioner polynomial on f16 storage"
+                  if head_prec == fa.FP64_PC32 else "f64 assembly, f32 minres + f64 refinement"),
         "data": "synthetic",
         "config": {
             "workload": f"3D mixed Hodge-Laplace k=1, Kuhn {args.n}^3 ({ncells} tets), jitter 0.05h seed 2, "
                         "N(0,1) edge source seed 2, full complex, true residual <= 1e-10",
             "kkt_n": n_kkt, "kkt_nnz": nnz,
         },
+        "mixed_mode": mode_name[head_prec],
+        f"solves_per_s_{mode_name[other_prec]}": alt_rate,
+        f"iters_{mode_name[other_prec]}": int(r_alt.iters),
+        f"refinements_{mode_name[other_prec]}": int(r_alt.refinements),
         "solves_per_s_fp64": fp64_rate,
         "mixed_over_fp64": value / fp64_rate,
         "solves_per_s_fp64_jacobi": jac_rate,
@@ -468,6 +504,8 @@ def main():
         },
         "mr_spmv_f32": {"launch_seconds": times["spmv32"], "algorithmic_bytes_per_launch": spmv_bytes,
                         "achieved_gbs": spmv_bytes / times["spmv32"] / 1e9},
+        "mr_spmv_f64": {"launch_seconds": times["spmv64"], "algorithmic_bytes_per_launch": spmv64_bytes,
+                        "achieved_gbs": spmv64_bytes / times["spmv64"] / 1e9},
     }
     if not args.no_cpu_baseline:
         try:

@@ -29,7 +29,9 @@ from ._lib import FeecError, Report, Request, Result, ServerOpts, SolveOpts, che
 MASS, DIF_TRIAL, DIF_TEST, DIF_BOTH, HDIF_GRAM = 0, 1, 2, 3, 4
 FULL, RELATIVE = 0, 1
 EUCLIDEAN, MINKOWSKI = 0, 1
-FP64, MIXED = 0, 1
+# feec_precision: FP64; FP32 Krylov inside FP64 refinement; FP64 Krylov with the
+# polynomial in FP32 on the FP16 SELL copy
+FP64, MIXED, FP64_PC32 = 0, 1, 2
 # feec_pc_storage: how the preconditioner matrix the polynomial runs on is stored
 PC_SELL, PC_CSR_HALF, PC_CSR = 0, 1, 2
 
@@ -37,7 +39,7 @@ __all__ = [
     "Context", "CartesianGrid", "Complex", "MeshLengthsSq", "WhitneyComplex", "Csr",
     "HodgeDirac", "Problem", "solve_source", "solve_dirac_source", "solve_opts", "FeecError",
     "normal_vector", "MASS", "DIF_TRIAL", "DIF_TEST", "DIF_BOTH", "HDIF_GRAM", "FULL",
-    "RELATIVE", "EUCLIDEAN", "MINKOWSKI", "FP64", "MIXED", "Report", "Request", "Result",
+    "RELATIVE", "EUCLIDEAN", "MINKOWSKI", "FP64", "MIXED", "FP64_PC32", "Report", "Request", "Result",
     "ServerOpts", "SolveOpts", "Server", "batch_run",
 ]
 

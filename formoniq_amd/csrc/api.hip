@@ -427,8 +427,9 @@ void problem_element_outputs(Problem& P, double** outs) {
 // condition number grows like r^2. The degree that balanced polynomial cost
 // against Krylov iterations in sweeps on the mixed path (3D 64^3: 64; 2D 400^2
 // and 800^2: 128; batches of 1e3-1e5-cell meshes: 24-32) is about 8 sqrt(r):
-// taken in multiples of 8 within [16, 128]. The FP64 path, whose steps move
-// about 2.5x the bytes, takes half of it within [24, 64].
+// taken in multiples of 8 within [16, 128], also for FP64 solves that run the
+// polynomial in FP32. The FP64 path, whose steps move about 2.5x the bytes,
+// takes half of it within [24, 64].
 static int resolve_cheb_degree(const Problem& P) {
   if (P.opts.cheb_degree >= 0) return P.opts.cheb_degree;
   const Mesh& m = *P.g->mesh;
@@ -454,9 +455,9 @@ void problem_assemble_from_elements(Problem& P) {
     P.ws = nullptr;
   }
   if (P.sys.P.nnz) coo_plan_values(ctx, P.pc_plan, P.el.p, P.sys.P.val.p, nullptr);
-  // the FP32/FP16 copies are made for mixed problems, the FP64 SELL copy for
-  // FP64 ones; either is refreshed once made
-  if (P.opts.precision == FEEC_PREC_MIXED || P.sys.val32.p) kkt_refresh_fp32(P.sys);
+  // the FP32/FP16 copies are made for mixed and FP64_PC32 problems, the FP64
+  // SELL copy for FP64 ones; either is refreshed once made
+  if (P.opts.precision != FEEC_PREC_FP64 || P.sys.val32.p) kkt_refresh_fp32(P.sys);
   if ((P.opts.precision == FEEC_PREC_FP64 && P.opts.precond_storage == FEEC_PC_STORE_SELL) ||
       P.sys.Psell64.p)
     kkt_refresh_sell64(P.sys);
@@ -479,13 +480,20 @@ struct feec_problem_s : feec::Problem {};
 
 namespace feec {
 
-Problem* problem_new(Geom* g, int bc, int k, const feec_solve_opts& o) {
-  FEEC_REQUIRE(o.precision == FEEC_PREC_FP64 || o.precision == FEEC_PREC_MIXED, FEEC_E_ARG,
-               "precision must be FEEC_PREC_FP64 or FEEC_PREC_MIXED");
+// Rejects solver options outside their documented ranges.
+static void validate_opts(const feec_solve_opts& o) {
+  FEEC_REQUIRE(o.precision >= FEEC_PREC_FP64 && o.precision <= FEEC_PREC_FP64_PC32, FEEC_E_ARG,
+               "precision must be a feec_precision value");
   FEEC_REQUIRE(o.precond_storage >= FEEC_PC_STORE_SELL && o.precond_storage <= FEEC_PC_STORE_CSR,
                FEEC_E_ARG, "precond_storage must be a feec_pc_storage value");
+  FEEC_REQUIRE(o.precision != FEEC_PREC_FP64_PC32 || o.precond_storage == FEEC_PC_STORE_SELL,
+               FEEC_E_ARG, "FEEC_PREC_FP64_PC32 runs on the SELL storage only");
   FEEC_REQUIRE(o.cheb_degree >= -1, FEEC_E_ARG, "cheb_degree must be >= -1");
   FEEC_REQUIRE(o.cheb_lower >= 0.0 && o.cheb_lower < 1.0, FEEC_E_ARG, "cheb_lower must be in [0, 1)");
+}
+
+Problem* problem_new(Geom* g, int bc, int k, const feec_solve_opts& o) {
+  validate_opts(o);
   auto* P = new feec_problem_s();
   P->g = g;
   P->bc = bc;
@@ -1131,6 +1139,7 @@ feec_status feec_problem_create(feec_geom* g, feec_bc bc, int grade, const feec_
     else
       feec_solve_opts_default(&P->opts);
     try {
+      validate_opts(P->opts);
       problem_build(*P);
       problem_assemble(*P);
     } catch (...) {
@@ -1193,10 +1202,12 @@ feec_status feec_problem_apply_preconditioner(feec_problem* p, int32_t precision
                                               int32_t storage, const double* r, double* z) {
   return guard([&] {
     FEEC_REQUIRE(p, FEEC_E_ARG, "null problem");
-    FEEC_REQUIRE(precision == FEEC_PREC_FP64 || precision == FEEC_PREC_MIXED, FEEC_E_ARG,
-                 "precision must be FEEC_PREC_FP64 or FEEC_PREC_MIXED");
+    FEEC_REQUIRE(precision >= FEEC_PREC_FP64 && precision <= FEEC_PREC_FP64_PC32, FEEC_E_ARG,
+                 "precision must be a feec_precision value");
     FEEC_REQUIRE(storage >= FEEC_PC_STORE_SELL && storage <= FEEC_PC_STORE_CSR, FEEC_E_ARG,
                  "storage must be a feec_pc_storage value");
+    FEEC_REQUIRE(precision != FEEC_PREC_FP64_PC32 || storage == FEEC_PC_STORE_SELL, FEEC_E_ARG,
+                 "FEEC_PREC_FP64_PC32 runs on the SELL storage only");
     Ctx* ctx = p->g->mesh->ctx;
     const int64_t N = p->sys.n + p->nh;
     DBuf<double> rd(ctx, N), zd(ctx, N);

@@ -76,8 +76,11 @@ typedef enum feec_bc { FEEC_FULL = 0, FEEC_RELATIVE = 1 } feec_bc;
 typedef enum feec_ambient { FEEC_EUCLIDEAN = 0, FEEC_MINKOWSKI = 1 } feec_ambient;
 
 typedef enum feec_precision {
-  FEEC_PREC_FP64 = 0,  /* FP64 matrix, vectors and Krylov */
-  FEEC_PREC_MIXED = 1  /* FP32 SpMV/Krylov inside FP64 iterative refinement */
+  FEEC_PREC_FP64 = 0,      /* FP64 matrix, vectors and Krylov */
+  FEEC_PREC_MIXED = 1,     /* FP32 SpMV/Krylov inside FP64 iterative refinement */
+  FEEC_PREC_FP64_PC32 = 2  /* FP64 SpMV and Krylov; the Chebyshev polynomial runs in FP32 on the
+                              FP16 SELL copy (the mixed solve's preconditioner, without its
+                              restarts). Requires FEEC_PC_STORE_SELL */
 } feec_precision;
 
 /* Storage of the preconditioner matrix the Chebyshev polynomial runs on. */

@@ -35,7 +35,8 @@ def standalone_u(r):
 
 
 MIX = [req(2, 0, 12, 1), req(2, 1, 10, 2), req(3, 0, 5, 3), req(3, 1, 4, 4), req(3, 2, 4, 5),
-       req(4, 0, 2, 6), req(4, 1, 2, 7), req(4, 3, 2, 8), req(2, 1, 16, 9, fa.FP64)]
+       req(4, 0, 2, 6), req(4, 1, 2, 7), req(4, 3, 2, 8), req(2, 1, 16, 9, fa.FP64),
+       req(3, 1, 5, 10, fa.FP64_PC32), req(2, 2, 12, 11, fa.FP64_PC32)]
 
 
 def checksum(u):
@@ -57,7 +58,8 @@ def test_batch_lockstep_is_bit_identical_to_single_solves():
     (one launch per phase); every request still gets the blocks, partial sums
     and iterates of a solve on its own, so iteration counts, residuals and
     the solution's norm and checksum equal the server's single solves bit for
-    bit (mixed and FP64, harmonic border rows included)."""
+    bit (mixed and FP64, harmonic border rows included; FP64_PC32 requests run
+    one after another beside the lockstep runs)."""
     res = fa.batch_run(MIX)
     single = run_server(MIX, 1)
     for r, (s_, _), q in zip(res, single, MIX):

@@ -43,7 +43,7 @@ def test_kkt_parity(d, n, amp, rel):
             np.testing.assert_allclose(s * gh, h, rtol=1e-12, atol=1e-14)
 
 
-@pytest.mark.parametrize("precision", [fa.FP64, fa.MIXED])
+@pytest.mark.parametrize("precision", [fa.FP64, fa.MIXED, fa.FP64_PC32])
 @pytest.mark.parametrize("d,n,amp", [(1, 16, 0.2), (2, 8, 0.1), (3, 4, 0.05)])
 @pytest.mark.parametrize("rel", [False, True])
 def test_solve_source(d, n, amp, rel, precision):
@@ -156,8 +156,16 @@ def test_preconditioner_application_per_storage(d, k, n):
     # harmonic rows pass through (rounded to FP32 on the mixed path)
     np.testing.assert_array_equal(z64[n_p:], r[n_p:])
     np.testing.assert_array_equal(z_sell[n_p:], r[n_p:].astype(np.float32).astype(np.float64))
+    # FP64 solves with the FP32 polynomial apply the mixed solve's
+    # preconditioner bit for bit (r rounded to FP32 first); harmonic rows pass
+    # through unrounded
+    z_pc = prob.apply_preconditioner(r, precision=fa.FP64_PC32)
+    np.testing.assert_array_equal(z_pc[:n_p], z_sell[:n_p])
+    np.testing.assert_array_equal(z_pc[n_p:], r[n_p:])
     f64 = fa.Problem(gc, k, fa.solve_opts(precision=fa.FP64, cheb_degree=m))
     assert f64.preconditioner()["format"] == "sell64_f64"
+    pc32 = fa.Problem(gc, k, fa.solve_opts(precision=fa.FP64_PC32, cheb_degree=m))
+    assert pc32.preconditioner()["format"] == "sell64_f16"
 
 
 @pytest.mark.parametrize("precision", [fa.MIXED, fa.FP64])
@@ -209,6 +217,39 @@ def test_wide_band_falls_back_to_csr(precision):
     z16 = prob_b.apply_preconditioner(r, precision=fa.MIXED, storage=fa.PC_SELL)
     z16c = prob_b.apply_preconditioner(r, precision=fa.MIXED, storage=fa.PC_CSR_HALF)
     assert np.abs(z16 - z16c).max() <= 1e-5 * np.abs(z16c).max()
+    np.testing.assert_array_equal(prob_b.apply_preconditioner(r, precision=fa.FP64_PC32), z16)
+
+
+def test_fp64_pc32_solve():
+    """FP64 MINRES with the polynomial in FP32 on the FP16 SELL copy: one run
+    without restarts meets the FP64 certificate and agrees with the FP64
+    solve, on the narrow (int16 offsets) and the wide (int32 columns) slices;
+    other storages are rejected."""
+    _, _, gcx, geo, _, _ = both(3, 6, 0.05, seed=13)
+    gc = fa.WhitneyComplex(geo)
+    f = fa.normal_vector(gcx.nsimplices(1), 14)
+    _, u64, _, r64 = fa.solve_source(gc, f, 1, fa.solve_opts(precision=fa.FP64))
+    _, u, _, rep = fa.solve_source(gc, f, 1, fa.solve_opts(precision=fa.FP64_PC32))
+    assert rep.converged and rep.residual <= 1e-10, rep.as_dict()
+    assert np.linalg.norm(u - u64) <= 1e-8 * np.linalg.norm(u64)
+    _, _, _, rmix = fa.solve_source(gc, f, 1, fa.solve_opts(precision=fa.MIXED))
+    assert rep.iters <= rmix.iters, (rep.as_dict(), rmix.as_dict())
+    for store in (fa.PC_CSR_HALF, fa.PC_CSR):
+        with pytest.raises(fa.FeecError):
+            fa.Problem(gc, 1, fa.solve_opts(precision=fa.FP64_PC32, precond_storage=store))
+    # wide slices: randomly numbered vertices
+    cells, coords = kuhn_mesh(3, 20, 0.05, seed=15)
+    nv = coords.shape[0]
+    perm = np.random.default_rng(16).permutation(nv)
+    cx = fa.Complex(3, perm[cells], nverts=nv)
+    wc = fa.WhitneyComplex(fa.MeshLengthsSq.from_coords(cx, coords[np.argsort(perm)]))
+    f = fa.normal_vector(cx.nsimplices(1), 17)
+    _, u64, _, _ = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64))
+    prob = fa.Problem(wc, 1, fa.solve_opts(precision=fa.FP64_PC32))
+    assert prob.preconditioner()["format"] == "sell64_f16_c32"
+    rep = prob.solve(f)
+    assert rep.converged and rep.residual <= 1e-10, rep.as_dict()
+    assert np.linalg.norm(prob.solution()[1] - u64) <= 1e-8 * np.linalg.norm(u64)
 
 
 def test_resident_problem_reassembly_and_device_source():
