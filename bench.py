@@ -156,30 +156,37 @@ def _requests(fa, count, seed, dims, cells_lo, cells_hi):
         r.dim, r.grade, r.n, r.bc = d, k, kuhn_n_for(d, cells), fa.FULL
         r.jitter = amp[d]
         r.mesh_seed, r.source_seed = seed * 100000 + i, seed * 100000 + 50000 + i
-        r.precision = fa.MIXED
+        r.precision = fa.FP64_PC32
         out.append(r)
     return out
+
+
+def _mode_names(fa):
+    # "mixed": FP64 MINRES with the FP32 polynomial on FP16 storage
+    return {fa.FP64_PC32: "mixed", fa.MIXED: "fp32_refine", fa.FP64: "fp64"}
 
 
 def run_c4(args):
     """512 independent jittered Kuhn meshes (seed 4): d in {2,3,4}, k in
     0..d-1, cells log-uniform in 1e3..1e5; all element matrices in one
-    persistent launch, then mixed-precision solves to 1e-10, overlapped on
-    worker streams. The same batch in FP64 beside it."""
+    persistent launch, then mixed-precision solves to 1e-10 in lockstep.
+    The same batch with FP32 MINRES inside FP64 refinement and in FP64
+    beside it."""
     import formoniq_amd as fa
     reqs = _requests(fa, args.batch, 4, [2, 3, 4], 1e3, 1e5)
     opts = fa.solve_opts(use_graph=0 if args.no_graph else 1)
+    names = _mode_names(fa)
     out = {}
-    for prec in (fa.MIXED, fa.FP64):
+    for prec in (fa.FP64_PC32, fa.MIXED, fa.FP64):
         for r in reqs:
             r.precision = prec
         fa.batch_run(reqs[:8], opts)  # warm up tables and the autotuner
         t0 = time.perf_counter()
         res = fa.batch_run(reqs, opts)
         dt = time.perf_counter() - t0
-        log(f"c4 {'mixed' if prec == fa.MIXED else 'fp64'}: {dt:.2f}s")
+        log(f"c4 {names[prec]}: {dt:.2f}s")
         ok = all(x.status == 0 and x.residual <= 1e-10 for x in res)
-        out["mixed" if prec == fa.MIXED else "fp64"] = {
+        out[names[prec]] = {
             "seconds": dt, "solves_per_s": len(res) / dt, "all_converged_1e-10": ok,
             "setup_and_elements_seconds": float(res[0].admit_time),
             "solve_phase_seconds": float(res[0].solve_seconds),
@@ -196,13 +203,15 @@ def run_c4(args):
 def run_c5(args):
     """Serving stream (seed 5): Poisson arrivals at a fixed rate of
     Hodge-Laplace solve requests, d in {2,3}, k in 0..d-1, cells log-uniform
-    in 1e3..2e6; admitted by HBM budget, solved mixed precision to 1e-10.
-    Throughput and p50/p99 latency, and the same stream on the FP64 path."""
+    in 1e3..2e6; admitted by HBM budget, solved mixed precision (FP64 MINRES,
+    FP32 polynomial on FP16 storage) to 1e-10. Throughput and p50/p99
+    latency, and the same stream on the FP64 path."""
     import formoniq_amd as fa
     reqs = _requests(fa, args.requests, 5, [2, 3], 1e3, 2e6)
     gaps = np.random.default_rng(5).exponential(1.0 / args.rate, size=len(reqs))
+    names = _mode_names(fa)
     out = {}
-    for prec in (fa.MIXED, fa.FP64):
+    for prec in (fa.FP64_PC32, fa.FP64):
         for r in reqs:
             r.precision = prec
         opts = fa.ServerOpts()
@@ -226,7 +235,7 @@ def run_c5(args):
             time.sleep(gap)
             tickets.append(srv.submit(r))
             if i % 8 == 7:
-                log(f"c5 {'mixed' if prec == fa.MIXED else 'fp64'}: {i + 1} submitted at "
+                log(f"c5 {names[prec]}: {i + 1} submitted at "
                     f"{time.perf_counter() - t0:.1f}s")
         srv.drain()
         dt = time.perf_counter() - t0
@@ -235,7 +244,7 @@ def run_c5(args):
         lat = np.array([x.latency for x in res])
         ok = all(x.status == 0 and x.residual <= 1e-10 for x in res)
         slow = sorted(zip(lat, reqs, res), key=lambda t: -t[0])[:3]
-        out["mixed" if prec == fa.MIXED else "fp64"] = {
+        out[names[prec]] = {
             "seconds": dt, "solves_per_s": len(res) / dt, "all_converged_1e-10": ok,
             "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
             "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),

@@ -596,7 +596,7 @@ void feec_solve_opts_default(feec_solve_opts* o) {
   if (!o) return;
   o->rtol = 1e-10;
   o->max_iters = 200000;
-  o->precision = FEEC_PREC_MIXED;
+  o->precision = FEEC_PREC_FP64_PC32;
   o->max_refinements = 12;
   o->inner_rtol = 1e-4;
   o->use_graph = 1;

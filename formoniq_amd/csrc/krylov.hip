@@ -2074,18 +2074,21 @@ __host__ __device__ inline int64_t for_blocks(int64_t n) {
   return b < 1 ? 1 : b;
 }
 
-template <class T>
+// TP: precision of the polynomial (float for FP64 solves with the FP32
+// polynomial, else T)
+template <class T, class TP = T>
 struct BatchSys {
   int64_t n, nh, N, split;
   const int64_t* rp;
   const int32_t* col;
   const T* val;
-  T *r1, *r2, *y, *z, *w, *w2, *x, *b, *border, *res, *d0, *d1;
-  const T* sd;
+  T *r1, *r2, *y, *z, *w, *w2, *x, *b, *border;
+  TP *res, *d0, *d1;
+  const TP* sd;
   const int32_t* perm;
   const int32_t* inv;
-  typename MinresWork<T>::SellN sell;
-  typename MinresWork<T>::SellW sellw;
+  typename MinresWork<TP>::SellN sell;
+  typename MinresWork<TP>::SellW sellw;
   int wide, cheb_m;
   const int64_t* psell_ptr;
   const double* coef;
@@ -2109,133 +2112,134 @@ __device__ __forceinline__ int batch_find(const int64_t* starts, int ns) {
 
 #define FEEC_BATCH_SYS                                                                 \
   const int si_ = batch_find(starts, ns);                                              \
-  const BatchSys<T>& d = sys[si_];                                                     \
+  const BatchSys<T, TP>& d = sys[si_];                                                     \
   const Blk bk{(int64_t)blockIdx.x - starts[si_], starts[si_ + 1] - starts[si_]};      \
   (void)bk;
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_zero(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_zero(const BatchSys<T, TP>* sys, const int64_t* starts, int ns) {
   FEEC_BATCH_SYS
   k_mr_zero_body<T>(d.b, d.N, d.r1, d.r2, d.x, d.w, d.w2, bk);
 }
 
 // first polynomial application of a run (r = b into y; no MINRES scalars yet)
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_cheb3_init0(const BatchSys<T>* sys, const int64_t* starts,
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_cheb3_init0(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                       int ns) {
   FEEC_BATCH_SYS
-  k_cheb3_init_body<T>(d.b, d.sd, d.perm, d.res, d.d0, d.d1, d.n, d.split, d.coef,
+  k_cheb3_init_body<T, TP>(d.b, d.sd, d.perm, d.res, d.d0, d.d1, d.n, d.split, d.coef,
                        (const MinresS*)nullptr, bk);
 }
 
 // step k of the polynomial; systems of lower degree sit it out
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_cheb3_step(const BatchSys<T>* sys, const int64_t* starts,
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_cheb3_step(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                      int ns, int k, int with_s) {
   FEEC_BATCH_SYS
   if (k + 1 >= d.cheb_m) return;
-  const T* a = (k & 1) ? d.d0 : d.d1;
-  T* b = (k & 1) ? d.d1 : d.d0;
+  const TP* a = (k & 1) ? d.d0 : d.d1;
+  TP* b = (k & 1) ? d.d1 : d.d0;
   const MinresS* S = with_s ? d.S : nullptr;
   if (d.wide)
-    k_cheb3_step_body<T, typename MinresWork<T>::SellW>(d.sellw, d.psell_ptr, d.n, d.split, d.res,
-                                                        a, b, d.coef, k, S, bk);
+    k_cheb3_step_body<TP, typename MinresWork<TP>::SellW>(d.sellw, d.psell_ptr, d.n, d.split, d.res,
+                                                          a, b, d.coef, k, S, bk);
   else
-    k_cheb3_step_body<T, typename MinresWork<T>::SellN>(d.sell, d.psell_ptr, d.n, d.split, d.res,
-                                                        a, b, d.coef, k, S, bk);
+    k_cheb3_step_body<TP, typename MinresWork<TP>::SellN>(d.sell, d.psell_ptr, d.n, d.split, d.res,
+                                                          a, b, d.coef, k, S, bk);
 }
 
 // finish: initial (r = b, z = y, partials into part) or in-iteration (par)
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_cheb3_finish(const BatchSys<T>* sys, const int64_t* starts,
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_cheb3_finish(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                        int ns, int initial, int par) {
   FEEC_BATCH_SYS
-  const T* x = ((d.cheb_m - 1) & 1) ? d.d0 : d.d1;
+  const TP* x = ((d.cheb_m - 1) & 1) ? d.d0 : d.d1;
   if (initial)
-    k_cheb3_finish_body<T>(d.b, x, d.sd, d.inv, d.y, d.n, d.N, (const MinresS*)nullptr, d.part, bk);
+    k_cheb3_finish_body<T, TP>(d.b, x, d.sd, d.inv, d.y, d.n, d.N, (const MinresS*)nullptr, d.part,
+                               bk);
   else
-    k_cheb3_finish_body<T>(par ? d.r2 : d.r1, x, d.sd, d.inv, par ? d.y : d.z, d.n, d.N, d.S,
-                           d.part2, bk);
+    k_cheb3_finish_body<T, TP>(par ? d.r2 : d.r1, x, d.sd, d.inv, par ? d.y : d.z, d.n, d.N, d.S,
+                               d.part2, bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_start(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_start(const BatchSys<T, TP>* sys, const int64_t* starts, int ns) {
   FEEC_BATCH_SYS
   k_mr_start_body(d.S, d.part, (int)red_blocks(d.N), d.rtol, d.max_iters, bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_spmv(const BatchSys<T>* sys, const int64_t* starts, int ns,
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_spmv(const BatchSys<T, TP>* sys, const int64_t* starts, int ns,
                                                int par) {
   FEEC_BATCH_SYS
   k_mr_spmv_body<T>(d.rp, d.col, d.val, d.n, par ? d.z : d.y, par ? d.r2 : d.r1, d.S, d.part, bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_border_dot(const BatchSys<T>* sys, const int64_t* starts,
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_border_dot(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                      int ns, int par) {
   FEEC_BATCH_SYS
   k_mr_border_dot_body<T>(d.border, (int)d.nh, d.n, par ? d.z : d.y, d.S, d.part + NBLK, bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_border_rows(const BatchSys<T>* sys, const int64_t* starts,
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_border_rows(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                       int ns, int par) {
   FEEC_BATCH_SYS
   k_mr_border_rows_body<T>((int)d.nh, d.n, par ? d.z : d.y, par ? d.r2 : d.r1, d.S, d.part + NBLK,
                            (int)red_blocks(d.n), bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_border_axpy(const BatchSys<T>* sys, const int64_t* starts,
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_border_axpy(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                       int ns, int par) {
   FEEC_BATCH_SYS
   k_mr_border_axpy_body<T>(d.border, (int)d.nh, d.n, par ? d.z : d.y, par ? d.r2 : d.r1, d.S, bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_alpha(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_alpha(const BatchSys<T, TP>* sys, const int64_t* starts, int ns) {
   FEEC_BATCH_SYS
   k_mr_alpha_body(d.S, d.part, (int)spmv_blocks(d.n), d.nh ? 1 : 0, bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_update_cheb3_init(const BatchSys<T>* sys,
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_update_cheb3_init(const BatchSys<T, TP>* sys,
                                                             const int64_t* starts, int ns, int par) {
   FEEC_BATCH_SYS
-  k_mr_update_cheb3_init_body<T>(par ? d.r2 : d.r1, par ? d.r1 : d.r2, d.sd, d.perm, d.res, d.d0,
+  k_mr_update_cheb3_init_body<T, TP>(par ? d.r2 : d.r1, par ? d.r1 : d.r2, d.sd, d.perm, d.res, d.d0,
                                  d.d1, d.n, d.N, d.split, d.coef, d.S, bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_rotate(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_rotate(const BatchSys<T, TP>* sys, const int64_t* starts, int ns) {
   FEEC_BATCH_SYS
   k_mr_rotate_body(d.S, d.part2, (int)red_blocks(d.N), bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_wx(const BatchSys<T>* sys, const int64_t* starts, int ns,
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_wx(const BatchSys<T, TP>* sys, const int64_t* starts, int ns,
                                              int par) {
   FEEC_BATCH_SYS
   k_mr_wx_body<T>(par ? d.z : d.y, par ? d.w2 : d.w, par ? d.w : d.w2, d.x, d.N, d.S, bk);
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) kb_check(const BatchSys<T>* sys, const int64_t* starts, int ns) {
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_check(const BatchSys<T, TP>* sys, const int64_t* starts, int ns) {
   FEEC_BATCH_SYS
   if (threadIdx.x == 0) k_mr_check_body(d.S, bk);
 }
 
-template <class T>
-__global__ void kb_gather_s(const BatchSys<T>* sys, int ns, MinresS* out) {
+template <class T, class TP = T>
+__global__ void kb_gather_s(const BatchSys<T, TP>* sys, int ns, MinresS* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ns) out[i] = *sys[i].S;
 }
 
-template <class T>
-static BatchSys<T> batch_sys(MinresWork<T>& w, KktSystem& s, const T* val, double rtol,
-                             int64_t max_iters) {
-  BatchSys<T> d{};
+template <class T, class TP = T>
+static BatchSys<T, TP> batch_sys(MinresWork<T>& w, KktSystem& s, const T* val, double rtol,
+                                 int64_t max_iters) {
+  BatchSys<T, TP> d{};
   d.n = w.n;
   d.nh = w.nh;
   d.N = w.n + w.nh;
@@ -2244,12 +2248,20 @@ static BatchSys<T> batch_sys(MinresWork<T>& w, KktSystem& s, const T* val, doubl
   d.col = s.A.col.p;
   d.val = val;
   d.r1 = w.r1.p; d.r2 = w.r2.p; d.y = w.y.p; d.z = w.z.p; d.w = w.w.p; d.w2 = w.w2.p;
-  d.x = w.x.p; d.b = w.b.p; d.border = w.border.p; d.res = w.res.p; d.d0 = w.d0.p; d.d1 = w.d1.p;
-  d.sd = w.sd;
+  d.x = w.x.p; d.b = w.b.p; d.border = w.border.p;
+  if constexpr (std::is_same_v<T, TP>) {
+    d.res = w.res.p; d.d0 = w.d0.p; d.d1 = w.d1.p;
+    d.sd = w.sd;
+    d.sell = w.sell;
+    d.sellw = w.sellw;
+  } else {  // FP64 solve, FP32 polynomial on the FP16 copy
+    d.res = w.res32.p; d.d0 = w.d0_32.p; d.d1 = w.d1_32.p;
+    d.sd = w.sd32;
+    d.sell = w.sell16;
+    d.sellw = w.sell16w;
+  }
   d.perm = w.sell_perm;
   d.inv = w.sell_inv;
-  d.sell = w.sell;
-  d.sellw = w.sellw;
   d.wide = w.sell_wide ? 1 : 0;
   d.cheb_m = w.cheb_m;
   d.psell_ptr = w.psell_ptr;
@@ -2264,24 +2276,24 @@ static BatchSys<T> batch_sys(MinresWork<T>& w, KktSystem& s, const T* val, doubl
 
 // Launch tables of one set of systems: the BatchSys array and, per grid
 // shape, the prefix sums of the systems' block counts.
-template <class T>
+template <class T, class TP = T>
 struct BatchTables {
   enum { RED_N, RED_n, SPMV, FOR_n, FOR_N, ONE, BORDER_RED, BORDER_ONE, NSHAPE };
   int ns = 0;
   int max_m = 0;
   bool any_border = false;
-  DBuf<BatchSys<T>> sys;
+  DBuf<BatchSys<T, TP>> sys;
   DBuf<int64_t> starts;  // [NSHAPE][ns + 1]
   int64_t total[NSHAPE] = {};
   std::vector<std::vector<int64_t>> hstart;  // host copy of the prefix sums per shape
   std::vector<int> ns_step;                 // systems (a prefix) whose degree reaches step k
-  void build(Ctx* ctx, const std::vector<BatchSys<T>>& v) {
+  void build(Ctx* ctx, const std::vector<BatchSys<T, TP>>& v) {
     ns = (int)v.size();
     max_m = 0;
     any_border = false;
     std::vector<int64_t> h((size_t)NSHAPE * (ns + 1), 0);
     for (int i = 0; i < ns; ++i) {
-      const BatchSys<T>& d = v[i];
+      const BatchSys<T, TP>& d = v[i];
       max_m = std::max(max_m, d.cheb_m);
       any_border = any_border || d.nh > 0;
       const int64_t g[NSHAPE] = {red_blocks(d.N), red_blocks(d.n), spmv_blocks(d.n), for_blocks(d.n),
@@ -2300,7 +2312,7 @@ struct BatchTables {
     }
     sys.alloc(ctx, std::max(ns, 1));
     starts.alloc(ctx, h.size());
-    copy_to_device(ctx, sys.p, v.data(), v.size() * sizeof(BatchSys<T>));
+    copy_to_device(ctx, sys.p, v.data(), v.size() * sizeof(BatchSys<T, TP>));
     copy_to_device(ctx, starts.p, h.data(), h.size() * sizeof(int64_t));
   }
   const int64_t* st(int k) const { return starts.p + (size_t)k * (ns + 1); }
@@ -2308,18 +2320,19 @@ struct BatchTables {
 
 // Step k over the leading systems (sorted by decreasing degree) whose degree
 // reaches it.
-template <class T>
-static void batch_steps(hipStream_t st, const BatchTables<T>& t, int k, int with_s) {
+template <class T, class TP>
+static void batch_steps(hipStream_t st, const BatchTables<T, TP>& t, int k, int with_s) {
+  using Tab = BatchTables<T, TP>;
   const int ns_k = t.ns_step[k];
   if (ns_k <= 0) return;
-  const int64_t blocks = t.hstart[BatchTables<T>::FOR_n][ns_k];
+  const int64_t blocks = t.hstart[Tab::FOR_n][ns_k];
   if (blocks <= 0) return;
-  kb_cheb3_step<T><<<(unsigned)blocks, TPB, 0, st>>>(t.sys.p, t.st(BatchTables<T>::FOR_n), ns_k, k,
-                                                     with_s);
+  kb_cheb3_step<T, TP><<<(unsigned)blocks, TPB, 0, st>>>(t.sys.p, t.st(Tab::FOR_n), ns_k, k,
+                                                         with_s);
 }
 
-template <class T, class K, class... A>
-static void batch_launch(hipStream_t st, K kernel, const BatchTables<T>& t, int shape, A... a) {
+template <class T, class TP, class K, class... A>
+static void batch_launch(hipStream_t st, K kernel, const BatchTables<T, TP>& t, int shape, A... a) {
   const int64_t blocks = t.total[shape];
   if (blocks <= 0 || t.ns == 0) return;
   kernel<<<(unsigned)blocks, TPB, 0, st>>>(t.sys.p, t.st(shape), t.ns, a...);
@@ -2327,9 +2340,10 @@ static void batch_launch(hipStream_t st, K kernel, const BatchTables<T>& t, int 
 
 // One MINRES run (w.b filled, preconditioner bound, SELL storage and a
 // polynomial of degree >= 2) of each system, in lockstep on ctx's stream.
-template <class T>
-static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T>>& all, std::vector<KrylovStats>& out) {
-  using Tab = BatchTables<T>;
+template <class T, class TP = T>
+static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T, TP>>& all,
+                            std::vector<KrylovStats>& out) {
+  using Tab = BatchTables<T, TP>;
   hipStream_t st = ctx->stream;
   out.assign(all.size(), KrylovStats{});
   if (all.empty()) return;
@@ -2339,16 +2353,16 @@ static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T>>& all, std::vector
   for (size_t i = 0; i < all.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return all[a].cheb_m > all[b].cheb_m; });
-  std::vector<BatchSys<T>> sorted;
+  std::vector<BatchSys<T, TP>> sorted;
   for (int i : order) sorted.push_back(all[i]);
   Tab t;
   t.build(ctx, sorted);
   // start of the run: r1 = r2 = b, y = B b, beta1
-  batch_launch(st, kb_zero<T>, t, Tab::RED_N);
-  batch_launch(st, kb_cheb3_init0<T>, t, Tab::RED_n);
+  batch_launch(st, kb_zero<T, TP>, t, Tab::RED_N);
+  batch_launch(st, kb_cheb3_init0<T, TP>, t, Tab::RED_n);
   for (int k = 0; k + 1 < t.max_m; ++k) batch_steps(st, t, k, 0);
-  batch_launch(st, kb_cheb3_finish<T>, t, Tab::RED_N, 1, 0);
-  batch_launch(st, kb_start<T>, t, Tab::ONE);
+  batch_launch(st, kb_cheb3_finish<T, TP>, t, Tab::RED_N, 1, 0);
+  batch_launch(st, kb_start<T, TP>, t, Tab::ONE);
   FEEC_HIP(hipGetLastError());
   std::vector<int> active = order;
   DBuf<MinresS> hs_dev(ctx, all.size());
@@ -2357,26 +2371,26 @@ static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T>>& all, std::vector
   while (!active.empty()) {
     for (int it = 0; it < SEG; ++it) {
       const int par = it & 1;
-      batch_launch(st, kb_spmv<T>, t, Tab::SPMV, par);
+      batch_launch(st, kb_spmv<T, TP>, t, Tab::SPMV, par);
       if (t.any_border) {
-        batch_launch(st, kb_border_dot<T>, t, Tab::BORDER_RED, par);
-        batch_launch(st, kb_border_rows<T>, t, Tab::BORDER_ONE, par);
-        batch_launch(st, kb_border_axpy<T>, t, Tab::BORDER_RED, par);
+        batch_launch(st, kb_border_dot<T, TP>, t, Tab::BORDER_RED, par);
+        batch_launch(st, kb_border_rows<T, TP>, t, Tab::BORDER_ONE, par);
+        batch_launch(st, kb_border_axpy<T, TP>, t, Tab::BORDER_RED, par);
       }
-      batch_launch(st, kb_alpha<T>, t, Tab::ONE);
-      batch_launch(st, kb_update_cheb3_init<T>, t, Tab::RED_N, par);
+      batch_launch(st, kb_alpha<T, TP>, t, Tab::ONE);
+      batch_launch(st, kb_update_cheb3_init<T, TP>, t, Tab::RED_N, par);
       for (int k = 0; k + 1 < t.max_m; ++k) batch_steps(st, t, k, 1);
-      batch_launch(st, kb_cheb3_finish<T>, t, Tab::RED_N, 0, par);
-      batch_launch(st, kb_rotate<T>, t, Tab::ONE);
-      batch_launch(st, kb_wx<T>, t, Tab::FOR_N, par);
-      batch_launch(st, kb_check<T>, t, Tab::ONE);
+      batch_launch(st, kb_cheb3_finish<T, TP>, t, Tab::RED_N, 0, par);
+      batch_launch(st, kb_rotate<T, TP>, t, Tab::ONE);
+      batch_launch(st, kb_wx<T, TP>, t, Tab::FOR_N, par);
+      batch_launch(st, kb_check<T, TP>, t, Tab::ONE);
     }
     FEEC_HIP(hipGetLastError());
-    kb_gather_s<T><<<(unsigned)((t.ns + TPB - 1) / TPB), TPB, 0, st>>>(t.sys.p, t.ns, hs_dev.p);
+    kb_gather_s<T, TP><<<(unsigned)((t.ns + TPB - 1) / TPB), TPB, 0, st>>>(t.sys.p, t.ns, hs_dev.p);
     FEEC_HIP(hipGetLastError());
     copy_to_host(ctx, hs.data(), hs_dev.p, t.ns * sizeof(MinresS));
     std::vector<int> still;
-    std::vector<BatchSys<T>> v;
+    std::vector<BatchSys<T, TP>> v;
     for (int i = 0; i < t.ns; ++i) {
       const int id = active[i];
       if (hs[i].done) {
@@ -2396,7 +2410,10 @@ static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T>>& all, std::vector
 // Whether a solve's MINRES runs can go through the lockstep path.
 static bool lockstep_eligible(const SolveJob& j) {
   if (!j.ss) return false;
-  if (j.fp64()) return j.ss->w64.has_sell && j.ss->w64.sd && j.ss->w64.cheb_m >= 2;
+  if (j.fp64()) {
+    const MinresWork<double>& w = j.ss->w64;
+    return ((w.has_sell && w.sd) || w.pc32) && w.cheb_m >= 2;
+  }
   return j.ss->w32.has_sell && j.ss->w32.sd && j.ss->w32.cheb_m >= 2;
 }
 
@@ -2413,21 +2430,22 @@ void kkt_solve_batch(std::vector<KktSystem*>& systems, std::vector<const double*
     job_begin(jobs[i], ws[i]);
   }
   while (true) {
-    std::vector<size_t> run32, run64, solo;
+    std::vector<size_t> run32, run64, runpc, solo;
     for (size_t i = 0; i < m; ++i) {
       if (!job_prepare(jobs[i])) continue;
       if (!lockstep_eligible(jobs[i]))
         solo.push_back(i);
       else if (jobs[i].fp64())
-        run64.push_back(i);
+        (jobs[i].ss->w64.pc32 ? runpc : run64).push_back(i);
       else
         run32.push_back(i);
     }
-    if (run32.empty() && run64.empty() && solo.empty()) break;
+    if (run32.empty() && run64.empty() && runpc.empty() && solo.empty()) break;
     for (size_t i : solo) job_post(jobs[i], job_run(jobs[i]));
     // the systems' own streams have filled w.b; the lockstep runs on lead's
     for (size_t i : run32) jobs[i].s->ctx->sync();
     for (size_t i : run64) jobs[i].s->ctx->sync();
+    for (size_t i : runpc) jobs[i].s->ctx->sync();
     std::vector<KrylovStats> ks;
     if (!run32.empty()) {
       std::vector<BatchSys<float>> v;
@@ -2446,6 +2464,15 @@ void kkt_solve_batch(std::vector<KktSystem*>& systems, std::vector<const double*
       minres_lockstep<double>(lead, v, ks);
       lead->sync();
       for (size_t a = 0; a < run64.size(); ++a) job_post(jobs[run64[a]], ks[a]);
+    }
+    if (!runpc.empty()) {
+      std::vector<BatchSys<double, float>> v;
+      for (size_t i : runpc)
+        v.push_back(batch_sys<double, float>(jobs[i].ss->w64, *jobs[i].s, jobs[i].s->A.val.p,
+                                             jobs[i].tol, jobs[i].budget));
+      minres_lockstep<double, float>(lead, v, ks);
+      lead->sync();
+      for (size_t a = 0; a < runpc.size(); ++a) job_post(jobs[runpc[a]], ks[a]);
     }
   }
   reps.resize(m);

@@ -97,7 +97,7 @@ typedef enum {
 typedef struct feec_solve_opts {
   double rtol;           /* target true relative residual |b-Ax|/|b| (default 1e-10) */
   int64_t max_iters;     /* Krylov iteration budget over all refinements (default 200000) */
-  int32_t precision;     /* feec_precision (default FEEC_PREC_MIXED) */
+  int32_t precision;     /* feec_precision (default FEEC_PREC_FP64_PC32) */
   int32_t max_refinements; /* mixed: outer FP64 refinement steps (default 12) */
   double inner_rtol;     /* mixed: FP32 inner MINRES tolerance (default 1e-4) */
   int32_t use_graph;     /* capture Krylov iterations in a hipGraph (default 1) */

@@ -37,7 +37,7 @@ def test_status_strings_and_defaults():
     assert L.feec_status_string(0) == b"ok"
     assert L.feec_status_string(7) == b"not converged"
     opts = fa.solve_opts()
-    assert opts.rtol == 1e-10 and opts.precision == fa.MIXED and opts.graph_iters % 2 == 0
+    assert opts.rtol == 1e-10 and opts.precision == fa.FP64_PC32 and opts.graph_iters % 2 == 0
 
 
 def test_kuhn_grid_matches_oracle_and_reference(golden):
