@@ -174,6 +174,15 @@ __device__ __forceinline__ double block_reduce(double v, double* sh) {
   return r;  // valid on thread 0
 }
 
+// Workgroups are dispatched to the 8 XCDs of the chip round-robin (workgroup b
+// to XCD b % 8), each XCD with its own L2. xcd_block() maps them so that XCD x
+// runs the x-th contiguous eighth of the logical blocks: its L2 then fetches
+// only the part of the gathered vector its rows touch, not all of it. The grid
+// is a multiple of 8; logical blocks >= nblocks exit.
+constexpr unsigned XCDS = 8;
+__device__ __forceinline__ int64_t xcd_block() {
+  return (int64_t)(blockIdx.x % XCDS) * (gridDim.x / XCDS) + blockIdx.x / XCDS;
+}
 // The (block, grid) a kernel body runs as: the launch's own for a standalone
 // launch; one system's share of the grid for a lockstep batch launch.
 struct Blk {
@@ -827,15 +836,6 @@ __global__ void __launch_bounds__(TPB) k_mr_update_cheb3_init(
   k_mr_update_cheb3_init_body<T, TP>(yn, r2, sd, perm, f, x0, x1, n, N, split, coef, S, Blk{blockIdx.x, (int64_t)gridDim.x});
 }
 
-// Workgroups are dispatched to the 8 XCDs of the chip round-robin (workgroup b
-// to XCD b % 8), each XCD with its own L2. xcd_block() maps them so that XCD x
-// runs the x-th contiguous eighth of the logical blocks: its L2 then fetches
-// only the part of the gathered vector its rows touch, not all of it. The grid
-// is a multiple of 8; logical blocks >= nblocks exit.
-constexpr unsigned XCDS = 8;
-__device__ __forceinline__ int64_t xcd_block() {
-  return (int64_t)(blockIdx.x % XCDS) * (gridDim.x / XCDS) + blockIdx.x / XCDS;
-}
 
 // One lane per row, one block per four 64-row slices: x0 <- x_{k+2}.
 template <class T, class E>
