@@ -32,16 +32,31 @@ constexpr uint32_t NEG = 0x80000000u;
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
 
-__global__ void k_keys(const int32_t* rows, const int32_t* cols, int64_t n, int64_t ncols,
-                       uint64_t sentinel, uint64_t* keys, unsigned long long* nvalid) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int32_t r = rows[i], c = cols[i];
-  bool ok = r >= 0 && c >= 0;
-  keys[i] = ok ? (uint64_t)r * (uint64_t)ncols + (uint64_t)c : sentinel;
-  // one atomic per wavefront (lane 0 is active whenever any lane is)
-  const unsigned long long valid = __ballot(ok);
-  if ((threadIdx.x & 63) == 0 && valid) atomicAdd(nvalid, (unsigned long long)__popcll(valid));
+// Sort keys row * ncols + col (sentinel for dropped contributions) and the
+// number of valid ones. Grid-stride with a bounded grid and one atomic per
+// block: an atomic per wavefront on the one counter serialized (22 ms for the
+// 201 M contributions of a 64^3 KKT system).
+constexpr unsigned KEYS_MAX_BLOCKS = 2048;
+__global__ void __launch_bounds__(TPB) k_keys(const int32_t* rows, const int32_t* cols, int64_t n,
+                                              int64_t ncols, uint64_t sentinel, uint64_t* keys,
+                                              unsigned long long* nvalid) {
+  __shared__ unsigned long long sh[TPB / 64];
+  unsigned long long cnt = 0;  // valid keys of the wavefront, on lane 0
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t r = rows[i], c = cols[i];
+    const bool ok = r >= 0 && c >= 0;
+    keys[i] = ok ? (uint64_t)r * (uint64_t)ncols + (uint64_t)c : sentinel;
+    // lane 0 is active whenever any lane of its wavefront is
+    cnt += (unsigned long long)__popcll(__ballot(ok));
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (unsigned w = 0; w < blockDim.x / 64; ++w) t += sh[w];
+    if (t) atomicAdd(nvalid, t);
+  }
 }
 
 __global__ void k_heads(const uint64_t* keys, int64_t n, int32_t* head) {
@@ -169,8 +184,8 @@ void coo_plan_build(Ctx* ctx, CooPlan& plan, int64_t nrows, int64_t ncols, int64
   DBuf<uint64_t> k0(ctx, n), k1(ctx, n);
   DBuf<unsigned long long> nvalid(ctx, 1);
   FEEC_HIP(hipMemsetAsync(nvalid.p, 0, sizeof(unsigned long long), ctx->stream));
-  k_keys<<<grid_for(n), TPB, 0, ctx->stream>>>(rows, cols, n, std::max<int64_t>(ncols, 1),
-                                               sentinel, k0.p, nvalid.p);
+  k_keys<<<std::min(grid_for(n), KEYS_MAX_BLOCKS), TPB, 0, ctx->stream>>>(
+      rows, cols, n, std::max<int64_t>(ncols, 1), sentinel, k0.p, nvalid.p);
   DBuf<uint32_t> s0(ctx, n);
   FEEC_HIP(hipMemcpyAsync(s0.p, srcs, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
   plan.src.alloc(ctx, n);
