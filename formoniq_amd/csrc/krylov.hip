@@ -927,6 +927,16 @@ __global__ void k_mr_check(MinresS* S) {
   k_mr_check_body(S, Blk{blockIdx.x, (int64_t)gridDim.x});
 }
 
+// Continue a finished MINRES run to a new target: its recurrence and iterate
+// are kept, only the stopping test changes.
+__global__ void k_mr_resume(MinresS* S, double rtol, long long max_iters) {
+  if (threadIdx.x == 0) {
+    S->done = 0;
+    S->rtol = rtol;
+    S->max_iters = max_iters;
+  }
+}
+
 // --- refinement helpers --------------------------------------------------------
 __global__ void k_to_f32_scaled(const double* a, int64_t n, double s, float* b) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1318,21 +1328,32 @@ static int segment_iters(const MinresWork<T>& w, const feec_solve_opts& o) {
 }
 
 // Run MINRES on w.b (already filled) into w.x. Returns stats.
+// With `resume`, the previous run on w (its recurrence, iterate and iteration
+// count) continues to the new rtol for at most max_iters more iterations.
 template <class T>
 static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const int64_t* rp,
                               const int32_t* col, const T* val, double rtol, int64_t max_iters,
-                              const feec_solve_opts& o) {
+                              const feec_solve_opts& o, bool resume = false) {
   hipStream_t st = ctx->stream;
   const int64_t N = w.n + w.nh;
   Enq q{st};
-  if (w.cheb_m > 0) {
+  long long start_iter = 0;
+  if (resume) {
+    MinresS h{};
+    copy_to_host(ctx, &h, w.S.p, sizeof(h));
+    start_iter = h.iter;
+    q(k_mr_resume, 1, 64, w.S.p, rtol, start_iter + (long long)max_iters);
+    // the buffer roles rotate with the executed iterations and a segment
+    // starts at parity 0: an odd count takes one iteration outside the graph
+    if (start_iter & 1) minres_iteration<T>(q, w, rp, col, val, 1);
+  } else if (w.cheb_m > 0) {
     q(k_mr_zero<T>, red_grid(N), TPB, w.b.p, N, w.r1.p, w.r2.p, w.x.p, w.w.p, w.w2.p);
     cheb_apply<T>(q, w, w.b.p, w.y.p, nullptr, w.part.p);
   } else {
     q(k_mr_init<T>, red_grid(N), TPB, w.b.p, w.dinv.p, N, w.r1.p, w.r2.p, w.y.p, w.x.p, w.w.p,
       w.w2.p, w.part.p);
   }
-  q(k_mr_start, 1, TPB, w.S.p, w.part.p, red_grid(N), rtol, (long long)max_iters);
+  if (!resume) q(k_mr_start, 1, TPB, w.S.p, w.part.p, red_grid(N), rtol, (long long)max_iters);
   const int seg = segment_iters(w, o);
   if (o.use_graph && (!w.graph || w.graph_iters != seg || w.graph_cheb != w.cheb_m)) {
     if (w.graph) {
@@ -1368,7 +1389,7 @@ static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const
     FEEC_HIP(hipStreamSynchronize(st));
     if (ss.host_s->done) break;
   }
-  ks.iters = ss.host_s->iter;
+  ks.iters = ss.host_s->iter - start_iter;
   ks.precond_residual = ss.host_s->residual;
   ks.converged = ss.host_s->residual <= rtol;
   return ks;
@@ -1909,6 +1930,9 @@ struct SolveJob {
   bool done = false;
   double tol = 0.0;
   int64_t budget = 0;
+  // FP64_PC32: the next run continues the last one (resume_tol), once per solve
+  bool resume = false, resumed = false;
+  double resume_tol = 0.0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // FP64 Krylov (FEEC_PREC_FP64, FEEC_PREC_FP64_PC32) or FP32 refinement steps
   bool fp64() const { return o.precision != FEEC_PREC_MIXED; }
@@ -1985,7 +2009,9 @@ static bool job_prepare(SolveJob& j) {
     // whose norm is res * |b|), with a margin of 10 for the gap between the
     // preconditioner norm MINRES monitors and the 2-norm.
     const double first = o.precision == FEEC_PREC_FP64_PC32 ? kPc32FirstRun * o.rtol : o.rtol;
-    j.tol = j.steps == 0 ? first : std::min(0.5, 0.1 * o.rtol / j.res);
+    j.tol = j.resume ? j.resume_tol : j.steps == 0 ? first : std::min(0.5, 0.1 * o.rtol / j.res);
+    // a continued run gets at most half its previous length (plus 20) more
+    if (j.resume) j.budget = std::min<int64_t>(j.budget, j.rep.iters / 2 + 20);
   } else {
     const int64_t N = j.s->n + j.s->nh;
     k_to_f32_scaled<<<grid_for(N), TPB, 0, j.s->ctx->stream>>>(j.ss->r.p, N, 1.0 / j.rnorm,
@@ -2012,6 +2038,18 @@ static void job_post(SolveJob& j, const KrylovStats& ks) {
       return;
     }
     ++j.steps;
+    if (j.o.precision == FEEC_PREC_FP64_PC32 && !j.resumed && ks.converged && ks.iters > 0 &&
+        ks.precond_residual > 0.0) {
+      // The run met its target in the preconditioner norm but the true
+      // residual is short by res / rtol: continue the same run to a target
+      // lowered by that factor (margin 2) instead of restarting on the
+      // residual, which would discard its Krylov space. Its iterate replaces
+      // this one in x.
+      k_axpy<<<grid_for(N), TPB, 0, st>>>(ss.w64.x.p, N, -1.0, j.x);
+      j.resume = j.resumed = true;
+      j.resume_tol = ks.precond_residual * std::min(0.5, 0.5 * j.o.rtol / j.res);
+      return;
+    }
     FEEC_HIP(hipMemcpyAsync(ss.w64.b.p, ss.r.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
   } else {
     k_axpy_from_f32<<<grid_for(N), TPB, 0, st>>>(ss.w32.x.p, N, j.rnorm, j.x);
@@ -2046,9 +2084,12 @@ static feec_report job_end(SolveJob& j) {
 static KrylovStats job_run(SolveJob& j) {
   KktSystem& s = *j.s;
   SolverState& ss = *j.ss;
-  if (j.fp64())
+  if (j.fp64()) {
+    const bool resume = j.resume;
+    j.resume = false;
     return minres_run<double>(s.ctx, ss, ss.w64, s.A.row_ptr.p, s.A.col.p, s.A.val.p, j.tol,
-                              j.budget, j.o);
+                              j.budget, j.o, resume);
+  }
   return minres_run<float>(s.ctx, ss, ss.w32, s.A.row_ptr.p, s.A.col.p, s.val32.p, j.tol, j.budget,
                            j.o);
 }
@@ -2409,7 +2450,7 @@ static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T, TP>>& all,
 
 // Whether a solve's MINRES runs can go through the lockstep path.
 static bool lockstep_eligible(const SolveJob& j) {
-  if (!j.ss) return false;
+  if (!j.ss || j.resume) return false;  // a continued run goes on by itself
   if (j.fp64()) {
     const MinresWork<double>& w = j.ss->w64;
     return ((w.has_sell && w.sd) || w.pc32) && w.cheb_m >= 2;

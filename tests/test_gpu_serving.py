@@ -36,7 +36,8 @@ def standalone_u(r):
 
 MIX = [req(2, 0, 12, 1), req(2, 1, 10, 2), req(3, 0, 5, 3), req(3, 1, 4, 4), req(3, 2, 4, 5),
        req(4, 0, 2, 6), req(4, 1, 2, 7), req(4, 3, 2, 8), req(2, 1, 16, 9, fa.FP64),
-       req(3, 1, 5, 10, fa.FP64_PC32), req(2, 2, 12, 11, fa.FP64_PC32)]
+       req(3, 1, 5, 10, fa.FP64_PC32), req(2, 2, 12, 11, fa.FP64_PC32),
+       req(2, 1, 96, 12, fa.FP64_PC32, jitter=0.1)]
 
 
 def checksum(u):
