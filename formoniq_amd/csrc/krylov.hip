@@ -2137,6 +2137,7 @@ struct BatchSys {
   MinresS* S;
   double rtol;
   long long max_iters;
+  int par0;  // parity of the executed iterations when the run starts (a continued run may be odd)
 };
 
 // The system whose share of the launch holds this block (block starts are a
@@ -2194,6 +2195,7 @@ template <class T, class TP = T>
 __global__ void __launch_bounds__(TPB) kb_cheb3_finish(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                        int ns, int initial, int par) {
   FEEC_BATCH_SYS
+  par ^= d.par0;
   const TP* x = ((d.cheb_m - 1) & 1) ? d.d0 : d.d1;
   if (initial)
     k_cheb3_finish_body<T, TP>(d.b, x, d.sd, d.inv, d.y, d.n, d.N, (const MinresS*)nullptr, d.part,
@@ -2213,6 +2215,7 @@ template <class T, class TP = T>
 __global__ void __launch_bounds__(TPB) kb_spmv(const BatchSys<T, TP>* sys, const int64_t* starts, int ns,
                                                int par) {
   FEEC_BATCH_SYS
+  par ^= d.par0;
   k_mr_spmv_body<T>(d.rp, d.col, d.val, d.n, par ? d.z : d.y, par ? d.r2 : d.r1, d.S, d.part, bk);
 }
 
@@ -2220,6 +2223,7 @@ template <class T, class TP = T>
 __global__ void __launch_bounds__(TPB) kb_border_dot(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                      int ns, int par) {
   FEEC_BATCH_SYS
+  par ^= d.par0;
   k_mr_border_dot_body<T>(d.border, (int)d.nh, d.n, par ? d.z : d.y, d.S, d.part + NBLK, bk);
 }
 
@@ -2227,6 +2231,7 @@ template <class T, class TP = T>
 __global__ void __launch_bounds__(TPB) kb_border_rows(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                       int ns, int par) {
   FEEC_BATCH_SYS
+  par ^= d.par0;
   k_mr_border_rows_body<T>((int)d.nh, d.n, par ? d.z : d.y, par ? d.r2 : d.r1, d.S, d.part + NBLK,
                            (int)red_blocks(d.n), bk);
 }
@@ -2235,6 +2240,7 @@ template <class T, class TP = T>
 __global__ void __launch_bounds__(TPB) kb_border_axpy(const BatchSys<T, TP>* sys, const int64_t* starts,
                                                       int ns, int par) {
   FEEC_BATCH_SYS
+  par ^= d.par0;
   k_mr_border_axpy_body<T>(d.border, (int)d.nh, d.n, par ? d.z : d.y, par ? d.r2 : d.r1, d.S, bk);
 }
 
@@ -2248,6 +2254,7 @@ template <class T, class TP = T>
 __global__ void __launch_bounds__(TPB) kb_update_cheb3_init(const BatchSys<T, TP>* sys,
                                                             const int64_t* starts, int ns, int par) {
   FEEC_BATCH_SYS
+  par ^= d.par0;
   k_mr_update_cheb3_init_body<T, TP>(par ? d.r2 : d.r1, par ? d.r1 : d.r2, d.sd, d.perm, d.res, d.d0,
                                  d.d1, d.n, d.N, d.split, d.coef, d.S, bk);
 }
@@ -2262,6 +2269,7 @@ template <class T, class TP = T>
 __global__ void __launch_bounds__(TPB) kb_wx(const BatchSys<T, TP>* sys, const int64_t* starts, int ns,
                                              int par) {
   FEEC_BATCH_SYS
+  par ^= d.par0;
   k_mr_wx_body<T>(par ? d.z : d.y, par ? d.w2 : d.w, par ? d.w : d.w2, d.x, d.N, d.S, bk);
 }
 
@@ -2269,6 +2277,18 @@ template <class T, class TP = T>
 __global__ void __launch_bounds__(TPB) kb_check(const BatchSys<T, TP>* sys, const int64_t* starts, int ns) {
   FEEC_BATCH_SYS
   if (threadIdx.x == 0) k_mr_check_body(d.S, bk);
+}
+
+// continue each system's finished run to its new target (k_mr_resume)
+template <class T, class TP = T>
+__global__ void __launch_bounds__(TPB) kb_resume(const BatchSys<T, TP>* sys, const int64_t* starts,
+                                                 int ns) {
+  FEEC_BATCH_SYS
+  if (threadIdx.x == 0) {
+    d.S->done = 0;
+    d.S->rtol = d.rtol;
+    d.S->max_iters = d.max_iters;
+  }
 }
 
 template <class T, class TP = T>
@@ -2381,13 +2401,26 @@ static void batch_launch(hipStream_t st, K kernel, const BatchTables<T, TP>& t, 
 
 // One MINRES run (w.b filled, preconditioner bound, SELL storage and a
 // polynomial of degree >= 2) of each system, in lockstep on ctx's stream.
+// With `resume`, each system's finished run continues instead (rtol: its new
+// target; max_iters: its further iterations), with the buffer parity its
+// executed iterations left.
 template <class T, class TP = T>
 static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T, TP>>& all,
-                            std::vector<KrylovStats>& out) {
+                            std::vector<KrylovStats>& out, bool resume = false) {
   using Tab = BatchTables<T, TP>;
   hipStream_t st = ctx->stream;
   out.assign(all.size(), KrylovStats{});
   if (all.empty()) return;
+  std::vector<long long> start_iter(all.size(), 0);
+  if (resume) {
+    for (size_t i = 0; i < all.size(); ++i) {
+      MinresS h{};
+      copy_to_host(ctx, &h, all[i].S, sizeof(h));
+      start_iter[i] = h.iter;
+      all[i].par0 = (int)(h.iter & 1);
+      all[i].max_iters += h.iter;
+    }
+  }
   // systems in order of decreasing degree: step k of the polynomial then
   // launches only over the leading systems whose degree reaches it
   std::vector<int> order(all.size());
@@ -2398,12 +2431,16 @@ static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T, TP>>& all,
   for (int i : order) sorted.push_back(all[i]);
   Tab t;
   t.build(ctx, sorted);
-  // start of the run: r1 = r2 = b, y = B b, beta1
-  batch_launch(st, kb_zero<T, TP>, t, Tab::RED_N);
-  batch_launch(st, kb_cheb3_init0<T, TP>, t, Tab::RED_n);
-  for (int k = 0; k + 1 < t.max_m; ++k) batch_steps(st, t, k, 0);
-  batch_launch(st, kb_cheb3_finish<T, TP>, t, Tab::RED_N, 1, 0);
-  batch_launch(st, kb_start<T, TP>, t, Tab::ONE);
+  if (resume) {
+    batch_launch(st, kb_resume<T, TP>, t, Tab::ONE);
+  } else {
+    // start of the run: r1 = r2 = b, y = B b, beta1
+    batch_launch(st, kb_zero<T, TP>, t, Tab::RED_N);
+    batch_launch(st, kb_cheb3_init0<T, TP>, t, Tab::RED_n);
+    for (int k = 0; k + 1 < t.max_m; ++k) batch_steps(st, t, k, 0);
+    batch_launch(st, kb_cheb3_finish<T, TP>, t, Tab::RED_N, 1, 0);
+    batch_launch(st, kb_start<T, TP>, t, Tab::ONE);
+  }
   FEEC_HIP(hipGetLastError());
   std::vector<int> active = order;
   DBuf<MinresS> hs_dev(ctx, all.size());
@@ -2435,7 +2472,7 @@ static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T, TP>>& all,
     for (int i = 0; i < t.ns; ++i) {
       const int id = active[i];
       if (hs[i].done) {
-        out[id].iters = hs[i].iter;
+        out[id].iters = hs[i].iter - start_iter[id];
         out[id].precond_residual = hs[i].residual;
         out[id].converged = hs[i].residual <= all[id].rtol;
       } else {
@@ -2450,7 +2487,7 @@ static void minres_lockstep(Ctx* ctx, std::vector<BatchSys<T, TP>>& all,
 
 // Whether a solve's MINRES runs can go through the lockstep path.
 static bool lockstep_eligible(const SolveJob& j) {
-  if (!j.ss || j.resume) return false;  // a continued run goes on by itself
+  if (!j.ss) return false;
   if (j.fp64()) {
     const MinresWork<double>& w = j.ss->w64;
     return ((w.has_sell && w.sd) || w.pc32) && w.cheb_m >= 2;
@@ -2471,22 +2508,24 @@ void kkt_solve_batch(std::vector<KktSystem*>& systems, std::vector<const double*
     job_begin(jobs[i], ws[i]);
   }
   while (true) {
-    std::vector<size_t> run32, run64, runpc, solo;
+    std::vector<size_t> run32, run64, runpc, runpc_resume, solo;
     for (size_t i = 0; i < m; ++i) {
       if (!job_prepare(jobs[i])) continue;
       if (!lockstep_eligible(jobs[i]))
         solo.push_back(i);
       else if (jobs[i].fp64())
-        (jobs[i].ss->w64.pc32 ? runpc : run64).push_back(i);
+        (jobs[i].ss->w64.pc32 ? (jobs[i].resume ? runpc_resume : runpc) : run64).push_back(i);
       else
         run32.push_back(i);
     }
-    if (run32.empty() && run64.empty() && runpc.empty() && solo.empty()) break;
+    if (run32.empty() && run64.empty() && runpc.empty() && runpc_resume.empty() && solo.empty())
+      break;
     for (size_t i : solo) job_post(jobs[i], job_run(jobs[i]));
     // the systems' own streams have filled w.b; the lockstep runs on lead's
     for (size_t i : run32) jobs[i].s->ctx->sync();
     for (size_t i : run64) jobs[i].s->ctx->sync();
     for (size_t i : runpc) jobs[i].s->ctx->sync();
+    for (size_t i : runpc_resume) jobs[i].s->ctx->sync();
     std::vector<KrylovStats> ks;
     if (!run32.empty()) {
       std::vector<BatchSys<float>> v;
@@ -2514,6 +2553,17 @@ void kkt_solve_batch(std::vector<KktSystem*>& systems, std::vector<const double*
       minres_lockstep<double, float>(lead, v, ks);
       lead->sync();
       for (size_t a = 0; a < runpc.size(); ++a) job_post(jobs[runpc[a]], ks[a]);
+    }
+    if (!runpc_resume.empty()) {
+      std::vector<BatchSys<double, float>> v;
+      for (size_t i : runpc_resume) {
+        jobs[i].resume = false;
+        v.push_back(batch_sys<double, float>(jobs[i].ss->w64, *jobs[i].s, jobs[i].s->A.val.p,
+                                             jobs[i].tol, jobs[i].budget));
+      }
+      minres_lockstep<double, float>(lead, v, ks, true);
+      lead->sync();
+      for (size_t a = 0; a < runpc_resume.size(); ++a) job_post(jobs[runpc_resume[a]], ks[a]);
     }
   }
   reps.resize(m);
