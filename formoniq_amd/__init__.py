@@ -37,7 +37,8 @@ PC_SELL, PC_CSR_HALF, PC_CSR = 0, 1, 2
 
 __all__ = [
     "Context", "CartesianGrid", "Complex", "MeshLengthsSq", "WhitneyComplex", "Csr",
-    "HodgeDirac", "Problem", "solve_source", "solve_dirac_source", "solve_opts", "FeecError",
+    "HodgeDirac", "Problem", "solve_source", "solve_dirac_source", "dirac_leapfrog", "solve_opts",
+    "FeecError",
     "normal_vector", "MASS", "DIF_TRIAL", "DIF_TEST", "DIF_BOTH", "HDIF_GRAM", "FULL",
     "RELATIVE", "EUCLIDEAN", "MINKOWSKI", "FP64", "MIXED", "FP64_PC32", "Report", "Request", "Result",
     "ServerOpts", "SolveOpts", "Server", "batch_run",
@@ -501,6 +502,19 @@ class HodgeDirac:
 
     def ndofs_total(self):
         return int(self.offsets[-1])
+
+
+def dirac_leapfrog(complex_: WhitneyComplex, dt, nsteps, y0):
+    """solve_dirac_leapfrog's time loop (Leapfrog::step, time.rs:386-400) on the
+    device: the flat field y0 of the complex (HodgeDirac offsets) after nsteps
+    steps of dt, the staggered invariant of every state (nsteps + 1 values,
+    time.rs:409-420) and the report (CG iterations of the mass solves)."""
+    y = np.array(y0, dtype=np.float64, copy=True)
+    energy = np.zeros(int(nsteps) + 1)
+    rep = Report()
+    check(load().feec_dirac_leapfrog(complex_.geometry._h, complex_.bc, float(dt), int(nsteps),
+                                     _ptr(y, _lib.f64p), _ptr(energy, _lib.f64p), C.byref(rep)))
+    return y, energy, rep
 
 
 def solve_dirac_source(geometry: MeshLengthsSq, mass_term, load_field, lift, opts=None):

@@ -2726,6 +2726,23 @@ void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y) {
   FEEC_HIP(hipGetLastError());
 }
 
+// a . b over n device doubles (fixed-order partial sums), read on the host.
+double dot64(Ctx* ctx, const double* a, const double* b, int64_t n, double* part) {
+  hipStream_t st = ctx->stream;
+  k_dot<double><<<red_grid(n), TPB, 0, st>>>(a, b, n, part);
+  k_finish_sum<<<1, TPB, 0, st>>>(part, red_grid(n), part + NBLK);
+  double h = 0.0;
+  FEEC_HIP(hipMemcpyAsync(&h, part + NBLK, sizeof(double), hipMemcpyDeviceToHost, st));
+  FEEC_HIP(hipStreamSynchronize(st));
+  return h;
+}
+
+// x += s a on n device doubles.
+void axpy64(Ctx* ctx, const double* a, int64_t n, double s, double* x) {
+  if (n > 0) k_axpy<<<grid_for(n), TPB, 0, ctx->stream>>>(a, n, s, x);
+  FEEC_HIP(hipGetLastError());
+}
+
 // Jacobi-preconditioned CG (krylov.rs:48-95) on an FP64 CSR; host-checked
 // every iteration (used for mass solves and the harmonic normal equations).
 KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
