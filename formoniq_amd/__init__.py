@@ -504,17 +504,21 @@ class HodgeDirac:
         return int(self.offsets[-1])
 
 
-def dirac_leapfrog(complex_: WhitneyComplex, dt, nsteps, y0):
+def dirac_leapfrog(complex_: WhitneyComplex, dt, nsteps, y0, keep_states=False):
     """solve_dirac_leapfrog's time loop (Leapfrog::step, time.rs:386-400) on the
     device: the flat field y0 of the complex (HodgeDirac offsets) after nsteps
     steps of dt, the staggered invariant of every state (nsteps + 1 values,
-    time.rs:409-420) and the report (CG iterations of the mass solves)."""
+    time.rs:409-420) and the report (CG iterations of the mass solves); with
+    keep_states, also every state as an (nsteps + 1, ndofs) array."""
     y = np.array(y0, dtype=np.float64, copy=True)
     energy = np.zeros(int(nsteps) + 1)
+    states = np.zeros((int(nsteps) + 1, y.size)) if keep_states else None
     rep = Report()
     check(load().feec_dirac_leapfrog(complex_.geometry._h, complex_.bc, float(dt), int(nsteps),
-                                     _ptr(y, _lib.f64p), _ptr(energy, _lib.f64p), C.byref(rep)))
-    return y, energy, rep
+                                     _ptr(y, _lib.f64p),
+                                     _ptr(states, _lib.f64p) if keep_states else None,
+                                     _ptr(energy, _lib.f64p), C.byref(rep)))
+    return (y, energy, rep, states) if keep_states else (y, energy, rep)
 
 
 def solve_dirac_source(geometry: MeshLengthsSq, mass_term, load_field, lift, opts=None):

@@ -102,7 +102,8 @@ SIGNATURES = {
     "feec_hodge_dirac_assemble": (S, [vp, C.c_int, C.c_double, C.POINTER(vp), C.POINTER(vp), i64p]),
     "feec_solve_dirac_source": (S, [vp, C.c_double, f64p, f64p, C.POINTER(SolveOpts), f64p,
                                     C.POINTER(Report)]),
-    "feec_dirac_leapfrog": (S, [vp, C.c_int, C.c_double, C.c_int64, f64p, f64p, C.POINTER(Report)]),
+    "feec_dirac_leapfrog": (S, [vp, C.c_int, C.c_double, C.c_int64, f64p, f64p, f64p,
+                                C.POINTER(Report)]),
     "feec_problem_create": (S, [vp, C.c_int, C.c_int, C.POINTER(SolveOpts), C.POINTER(vp)]),
     "feec_problem_destroy": (None, [vp]),
     "feec_problem_assemble": (S, [vp]),

@@ -138,7 +138,7 @@ feec_status feec_hodge_dirac_assemble(feec_geom* g, feec_bc bc, double sign, fee
 // takes two solves. energy[s] is the staggered invariant
 // 1/2 y^T M y - dt^2/8 (A q)^T M^-1 (A q) of the state after s steps.
 static void dirac_leapfrog(Geom& g, int bc, double dt, int64_t nsteps, double* y_host,
-                           double* energy, feec_report* report, double cg_rtol) {
+                           double* states, double* energy, feec_report* report, double cg_rtol) {
   Mesh& m = *g.mesh;
   Ctx* ctx = m.ctx;
   hipStream_t st = ctx->stream;
@@ -176,6 +176,7 @@ static void dirac_leapfrog(Geom& g, int bc, double dt, int64_t nsteps, double* y
   };
   kick(0);
   if (energy) energy[0] = staggered_energy();
+  if (states) std::copy(y_host, y_host + N, states);
   for (int64_t s = 0; s < nsteps; ++s) {
     axpy64(ctx, z.p, N, 0.5 * dt, y.p);  // p half-kick (z lives on the odd grades)
     kick(1);
@@ -183,6 +184,7 @@ static void dirac_leapfrog(Geom& g, int bc, double dt, int64_t nsteps, double* y
     kick(0);
     axpy64(ctx, z.p, N, 0.5 * dt, y.p);  // p half-kick
     if (energy) energy[s + 1] = staggered_energy();
+    if (states) copy_to_host(ctx, states + (s + 1) * N, y.p, N * sizeof(double));
   }
   copy_to_host(ctx, y_host, y.p, N * sizeof(double));
   FEEC_HIP(hipEventRecord(e1, st));
@@ -201,13 +203,13 @@ static void dirac_leapfrog(Geom& g, int bc, double dt, int64_t nsteps, double* y
 }
 
 feec_status feec_dirac_leapfrog(feec_geom* g, feec_bc bc, double dt, int64_t nsteps, double* y,
-                                double* energy, feec_report* report) {
+                                double* states, double* energy, feec_report* report) {
   return guard([&] {
     FEEC_REQUIRE(g && y, FEEC_E_ARG, "null argument");
     FEEC_REQUIRE(bc == FEEC_FULL || bc == FEEC_RELATIVE, FEEC_E_ARG, "bc must be a feec_bc value");
     FEEC_REQUIRE(nsteps >= 0 && std::isfinite(dt), FEEC_E_ARG, "nsteps >= 0 and a finite dt");
     FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
-    dirac_leapfrog(*g, bc, dt, nsteps, y, energy, report, 1e-14);
+    dirac_leapfrog(*g, bc, dt, nsteps, y, states, energy, report, 1e-14);
   });
 }
 

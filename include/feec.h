@@ -215,11 +215,12 @@ feec_status feec_solve_dirac_source(feec_geom *geom, double mass_term, const dou
 /* Leapfrog::step / conserved_energy (time.rs:292-420) on HodgeDirac::assemble of the complex,
  * 2-colored by grade parity (dirac.rs:330-337), as solve_dirac_leapfrog's loop
  * (dirac.rs:415-434): y (the flat field of the complex, offsets as feec_hodge_dirac_assemble)
- * advances nsteps steps of dt in place; energy (nsteps + 1 values, or null) receives the
- * staggered invariant of the initial state and after every step. Mass solves by Jacobi CG to
+ * advances nsteps steps of dt in place; states ((nsteps + 1) * ndofs values, or null) receives
+ * every state from the initial one on, as the reference's returned vector of fields; energy
+ * (nsteps + 1 values, or null) the staggered invariant of each. Mass solves by Jacobi CG to
  * 1e-14 (the reference factors them); the report counts CG iterations and device seconds. */
 feec_status feec_dirac_leapfrog(feec_geom *geom, feec_bc bc, double dt, int64_t nsteps, double *y,
-                                double *energy, feec_report *report);
+                                double *states, double *energy, feec_report *report);
 
 /* ---- resident problems (device-resident state, pattern reused) ----------- */
 feec_status feec_problem_create(feec_geom *geom, feec_bc bc, int grade, const feec_solve_opts *opts,

@@ -319,6 +319,13 @@ def test_dirac_leapfrog_matches_oracle(d, n, rel):
     y1, e1, _ = fa.dirac_leapfrog(gc, dt, 0, y0)
     np.testing.assert_array_equal(y1, y0)
     assert abs(e1[0] - e_o[0]) <= 1e-10 * e_o[0]
+    # every state, as solve_dirac_leapfrog returns them: the same run
+    y5, _, _, states = fa.dirac_leapfrog(gc, dt, 5, y0, keep_states=True)
+    assert states.shape == (6, y0.size)
+    np.testing.assert_array_equal(states[0], y0)
+    np.testing.assert_array_equal(states[-1], y5)
+    y3, _, _ = fa.dirac_leapfrog(gc, dt, 3, y0)
+    np.testing.assert_array_equal(states[3], y3)
 
 
 def test_dirac_lorentzian_4d_assembly():
