@@ -132,8 +132,9 @@ feec_status feec_hodge_dirac_assemble(feec_geom* g, feec_bc bc, double sign, fee
 // grades of even parity are the drifted block q, odd ones the kicked block p;
 // one step is p += dt/2 M^-1 A q, q += dt M^-1 A p, p += dt/2 M^-1 A q. The
 // reference factors the two color-block masses; here each mass solve is a
-// Jacobi CG on the whole block-diagonal mass matrix with a right-hand side
-// supported on one color (the iterates stay on that color exactly). The
+// Jacobi CG (scalars on the device) on the whole block-diagonal mass matrix
+// with a right-hand side supported on one color (the iterates stay on that
+// color exactly). The
 // closing half-kick's M^-1 A q is the next step's opening one, so a step
 // takes two solves. energy[s] is the staggered invariant
 // 1/2 y^T M y - dt^2/8 (A q)^T M^-1 (A q) of the state after s steps.
@@ -165,7 +166,7 @@ static void dirac_leapfrog(Geom& g, int bc, double dt, int64_t nsteps, double* y
   auto kick = [&](int odd) {
     k_parity_part<<<grid_for(N), TPB, 0, st>>>(y.p, go, odd, x.p);
     spmv64(ctx, D.op, x.p, r.p);
-    KrylovStats ks = cg_solve(ctx, D.mass_block, dinv.p, r.p, z.p, cg_rtol, 100000);
+    KrylovStats ks = cg_solve_device(ctx, D.mass_block, dinv.p, r.p, z.p, cg_rtol, 100000);
     FEEC_REQUIRE(ks.converged, FEEC_E_NOT_CONVERGED, "leapfrog mass solve did not converge");
     cg_iters += ks.iters;
     worst = std::max(worst, ks.precond_residual);

@@ -353,6 +353,9 @@ int kkt_preconditioner_format(const KktSystem& s, int precision, int storage);
 double time_cheb_step(KktSystem& s, bool fp32, int reps, int storage);
 KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
                      double rtol, int64_t max_iters);
+// The same CG with its scalars on the device (host read every 8 iterations).
+KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
+                            double rtol, int64_t max_iters);
 // a . b (part: 2 * 4096 doubles of scratch) and x += s a, on device doubles
 double dot64(Ctx* ctx, const double* a, const double* b, int64_t n, double* part);
 void axpy64(Ctx* ctx, const double* a, int64_t n, double s, double* x);

@@ -2743,6 +2743,196 @@ void axpy64(Ctx* ctx, const double* a, int64_t n, double s, double* x) {
   FEEC_HIP(hipGetLastError());
 }
 
+// --- device-resident Jacobi CG -------------------------------------------------
+// The scalars live in device memory; the host reads them once per segment of
+// CG_SEG iterations, and iterations after convergence return at once.
+struct CgS {
+  double rz, bnorm2, rtol2, alpha, beta, res2;
+  long long iter, max_iters;
+  int done;
+  int pad;
+};
+constexpr int CG_SEG = 8;
+
+// x = 0, r = b, z = p = dinv b; partials r.z and b.b
+__global__ void __launch_bounds__(TPB) k_cg_init(const double* __restrict__ b,
+                                                 const double* __restrict__ dinv, int64_t n,
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 double* __restrict__ z, double* __restrict__ p,
+                                                 double* part) {
+  __shared__ double sh[TPB / 64];
+  double rz = 0.0, bb = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double bi = b[i], zi = dinv[i] * bi;
+    x[i] = 0.0;
+    r[i] = bi;
+    z[i] = zi;
+    p[i] = zi;
+    rz += bi * zi;
+    bb += bi * bi;
+  }
+  const double t0 = block_reduce(rz, sh);
+  __syncthreads();
+  const double t1 = block_reduce(bb, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = t0;
+    part[NBLK + blockIdx.x] = t1;
+  }
+}
+
+__global__ void k_cg_start(CgS* S, const double* part, int np, double rtol, long long max_iters) {
+  __shared__ double sh[TPB / 64];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    a += part[i];
+    b += part[NBLK + i];
+  }
+  const double rz = block_reduce(a, sh);
+  __syncthreads();
+  const double bb = block_reduce(b, sh);
+  if (threadIdx.x == 0) {
+    CgS s{};
+    s.rz = rz;
+    s.bnorm2 = bb;
+    s.rtol2 = rtol * rtol;
+    s.res2 = bb;
+    s.max_iters = max_iters;
+    s.done = (bb == 0.0 || max_iters <= 0) ? 1 : 0;
+    *S = s;
+  }
+}
+
+// ap = A p; partial p.ap
+__global__ void __launch_bounds__(TPB) k_cg_spmv(const int64_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ col,
+                                                 const double* __restrict__ val, int64_t n,
+                                                 const double* __restrict__ p,
+                                                 double* __restrict__ ap, const CgS* S,
+                                                 double* part) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  const int lane = threadIdx.x % LANES;
+  const int64_t groups = (int64_t)gridDim.x * (TPB / LANES);
+  double loc = 0.0;
+  for (int64_t row = blockIdx.x * (int64_t)(TPB / LANES) + threadIdx.x / LANES; row < n;
+       row += groups) {
+    const double acc = row_dot<double>(rp, col, val, p, row, lane);
+    if (lane == 0) {
+      ap[row] = acc;
+      loc += p[row] * acc;
+    }
+  }
+  const double t = block_reduce(loc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_cg_alpha(CgS* S, const double* part, int np) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+  const double pap = block_reduce(v, sh);
+  if (threadIdx.x == 0) S->alpha = S->rz / pap;
+}
+
+// x += alpha p, r -= alpha ap, z = dinv r; partials r.z and r.r
+__global__ void __launch_bounds__(TPB) k_cg_update(double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ z,
+                                                   const double* __restrict__ p,
+                                                   const double* __restrict__ ap,
+                                                   const double* __restrict__ dinv, int64_t n,
+                                                   const CgS* S, double* part) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  const double alpha = S->alpha;
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * ap[i], zi = dinv[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    rz += ri * zi;
+    rr += ri * ri;
+  }
+  const double t0 = block_reduce(rz, sh);
+  __syncthreads();
+  const double t1 = block_reduce(rr, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = t0;
+    part[NBLK + blockIdx.x] = t1;
+  }
+}
+
+__global__ void k_cg_beta(CgS* S, const double* part, int np) {
+  if (S->done) return;
+  __shared__ double sh[TPB / 64];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    a += part[i];
+    b += part[NBLK + i];
+  }
+  const double rz = block_reduce(a, sh);
+  __syncthreads();
+  const double rr = block_reduce(b, sh);
+  if (threadIdx.x == 0) {
+    CgS s = *S;
+    s.beta = rz / s.rz;
+    s.rz = rz;
+    s.res2 = rr;
+    s.iter += 1;
+    if (rr <= s.rtol2 * s.bnorm2 || s.iter >= s.max_iters) s.done = 1;
+    *S = s;
+  }
+}
+
+// p = z + beta p
+__global__ void k_cg_dir(double* __restrict__ p, const double* __restrict__ z, int64_t n,
+                         const CgS* S) {
+  if (S->done) return;
+  const double beta = S->beta;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+// Jacobi CG (krylov.rs:48-95: stop when |r| <= rtol |b|) with its scalars on
+// the device: five launches per iteration, one host read per CG_SEG iterations.
+KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
+                            double rtol, int64_t max_iters) {
+  const int64_t n = a.nrows;
+  hipStream_t st = ctx->stream;
+  KrylovStats ks;
+  if (n == 0) {
+    ks.converged = true;
+    ks.precond_residual = 0.0;
+    return ks;
+  }
+  DBuf<double> r(ctx, n), z(ctx, n), p(ctx, n), ap(ctx, n), part(ctx, 2 * NBLK);
+  DBuf<CgS> S(ctx, 1);
+  const unsigned gr = red_grid(n), gs = spmv_grid(n);
+  k_cg_init<<<gr, TPB, 0, st>>>(b, dinv, n, x, r.p, z.p, p.p, part.p);
+  k_cg_start<<<1, TPB, 0, st>>>(S.p, part.p, (int)gr, rtol, (long long)max_iters);
+  CgS h{};
+  while (true) {
+    for (int it = 0; it < CG_SEG; ++it) {
+      k_cg_spmv<<<gs, TPB, 0, st>>>(a.row_ptr.p, a.col.p, a.val.p, n, p.p, ap.p, S.p, part.p);
+      k_cg_alpha<<<1, TPB, 0, st>>>(S.p, part.p, (int)gs);
+      k_cg_update<<<gr, TPB, 0, st>>>(x, r.p, z.p, p.p, ap.p, dinv, n, S.p, part.p);
+      k_cg_beta<<<1, TPB, 0, st>>>(S.p, part.p, (int)gr);
+      k_cg_dir<<<gr, TPB, 0, st>>>(p.p, z.p, n, S.p);
+    }
+    FEEC_HIP(hipGetLastError());
+    copy_to_host(ctx, &h, S.p, sizeof(h));
+    if (h.done) break;
+  }
+  ks.iters = h.iter;
+  ks.precond_residual = h.bnorm2 > 0.0 ? std::sqrt(h.res2 / h.bnorm2) : 0.0;
+  ks.converged = h.bnorm2 == 0.0 || h.res2 <= h.rtol2 * h.bnorm2;
+  return ks;
+}
+
 // Jacobi-preconditioned CG (krylov.rs:48-95) on an FP64 CSR; host-checked
 // every iteration (used for mass solves and the harmonic normal equations).
 KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
