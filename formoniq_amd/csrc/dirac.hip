@@ -143,10 +143,16 @@ static void dirac_leapfrog(Geom& g, int bc, double dt, int64_t nsteps, double* y
   Mesh& m = *g.mesh;
   Ctx* ctx = m.ctx;
   hipStream_t st = ctx->stream;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  FEEC_HIP(hipEventCreate(&e0));
-  FEEC_HIP(hipEventCreate(&e1));
-  FEEC_HIP(hipEventRecord(e0, st));
+  struct Events {  // destroyed on every path out, errors included
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
+  FEEC_HIP(hipEventCreate(&ev.e0));
+  FEEC_HIP(hipEventCreate(&ev.e1));
+  FEEC_HIP(hipEventRecord(ev.e0, st));
   Dirac D;
   dirac_assemble(g, bc, -1.0, D);
   const int d = m.d;
@@ -188,12 +194,10 @@ static void dirac_leapfrog(Geom& g, int bc, double dt, int64_t nsteps, double* y
     if (states) copy_to_host(ctx, states + (s + 1) * N, y.p, N * sizeof(double));
   }
   copy_to_host(ctx, y_host, y.p, N * sizeof(double));
-  FEEC_HIP(hipEventRecord(e1, st));
-  FEEC_HIP(hipEventSynchronize(e1));
+  FEEC_HIP(hipEventRecord(ev.e1, st));
+  FEEC_HIP(hipEventSynchronize(ev.e1));
   float ms = 0.f;
-  FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  FEEC_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
   if (report) {
     *report = feec_report{};
     report->iters = cg_iters;
