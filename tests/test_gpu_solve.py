@@ -295,7 +295,7 @@ def test_dirac_assembly_parity(d):
 
 
 @pytest.mark.parametrize("d,n,rel", [(1, 8, False), (2, 4, False), (2, 4, True), (3, 3, False),
-                                     (3, 2, True)])
+                                     (3, 2, True), (4, 2, False)])
 def test_dirac_leapfrog_matches_oracle(d, n, rel):
     """The explicit Hodge-Dirac leapfrog (Leapfrog, time.rs:292-420) on the
     GPU against the oracle's (color-block masses factored by SuperLU): states
