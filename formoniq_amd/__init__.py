@@ -37,7 +37,8 @@ PC_SELL, PC_CSR_HALF, PC_CSR = 0, 1, 2
 
 __all__ = [
     "Context", "CartesianGrid", "Complex", "MeshLengthsSq", "WhitneyComplex", "Csr",
-    "HodgeDirac", "Problem", "solve_source", "solve_dirac_source", "dirac_leapfrog", "solve_opts",
+    "HodgeDirac", "Problem", "solve_source", "solve_dirac_source", "dirac_leapfrog", "dirac_irk",
+    "solve_opts",
     "FeecError",
     "normal_vector", "MASS", "DIF_TRIAL", "DIF_TEST", "DIF_BOTH", "HDIF_GRAM", "FULL",
     "RELATIVE", "EUCLIDEAN", "MINKOWSKI", "FP64", "MIXED", "FP64_PC32", "Report", "Request", "Result",
@@ -518,6 +519,21 @@ def dirac_leapfrog(complex_: WhitneyComplex, dt, nsteps, y0, keep_states=False):
                                      _ptr(y, _lib.f64p),
                                      _ptr(states, _lib.f64p) if keep_states else None,
                                      _ptr(energy, _lib.f64p), C.byref(rep)))
+    return (y, energy, rep, states) if keep_states else (y, energy, rep)
+
+
+def dirac_irk(complex_: WhitneyComplex, dt, nsteps, y0, stages=2, keep_states=False):
+    """solve_dirac's time loop (LinearIrk::step with Tableau::gauss_legendre,
+    time.rs:191-274) on the device: the flat field after nsteps steps, the
+    energy 1/2 y^T M y of every state and the report (GMRES iterations)."""
+    y = np.array(y0, dtype=np.float64, copy=True)
+    energy = np.zeros(int(nsteps) + 1)
+    states = np.zeros((int(nsteps) + 1, y.size)) if keep_states else None
+    rep = Report()
+    check(load().feec_dirac_irk(complex_.geometry._h, complex_.bc, int(stages), float(dt),
+                                int(nsteps), _ptr(y, _lib.f64p),
+                                _ptr(states, _lib.f64p) if keep_states else None,
+                                _ptr(energy, _lib.f64p), C.byref(rep)))
     return (y, energy, rep, states) if keep_states else (y, energy, rep)
 
 

@@ -104,6 +104,8 @@ SIGNATURES = {
                                     C.POINTER(Report)]),
     "feec_dirac_leapfrog": (S, [vp, C.c_int, C.c_double, C.c_int64, f64p, f64p, f64p,
                                 C.POINTER(Report)]),
+    "feec_dirac_irk": (S, [vp, C.c_int, C.c_int, C.c_double, C.c_int64, f64p, f64p, f64p,
+                           C.POINTER(Report)]),
     "feec_problem_create": (S, [vp, C.c_int, C.c_int, C.POINTER(SolveOpts), C.POINTER(vp)]),
     "feec_problem_destroy": (None, [vp]),
     "feec_problem_assemble": (S, [vp]),

@@ -6,6 +6,7 @@
 //   top_harmonic                  formoniq/src/problems/dirac.rs:553-568
 //   solve_bordered                formoniq/src/problems/dirac.rs:578-598
 //   Leapfrog::{new, step, conserved_energy}   formoniq/src/time.rs:292-420
+//   LinearIrk::{new, step}, stage_matrix      formoniq/src/time.rs:191-274 (Gauss-Legendre)
 //   solve_dirac_leapfrog, grade_parity_coloring  formoniq/src/problems/dirac.rs:330-337, 415-434
 //
 // mass_block is the block diagonal of the grade masses M_k; op carries
@@ -38,6 +39,19 @@ struct GradeOffsets {
   int64_t off[6];
   int ngrades;
 };
+
+// out = a * b (elementwise)
+__global__ void k_mul(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+                      double* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] * b[i];
+}
+
+// x = s a
+__global__ void k_scaled(const double* __restrict__ a, int64_t n, double s, double* __restrict__ x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = s * a[i];
+}
 
 // x = y on the rows whose grade has parity `odd`, 0 on the others (the
 // grade-parity coloring of the leapfrog split).
@@ -205,6 +219,188 @@ static void dirac_leapfrog(Geom& g, int bc, double dt, int64_t nsteps, double* y
     report->precond_residual = worst;
     report->seconds = ms * 1e-3;
   }
+}
+
+// Gauss-Legendre collocation tableau with s = 1 (implicit midpoint) or s = 2
+// stages: nodes 1/2 -+ sqrt(3)/6, A = [[1/4, 1/4 - sqrt(3)/6], [1/4 + sqrt(3)/6, 1/4]],
+// weights 1/2 (the textbook coefficients of the 4th-order Gauss method).
+struct GaussTableau {
+  int s;
+  double a[2][2], b[2];
+};
+static GaussTableau gauss_legendre(int s) {
+  GaussTableau t{};
+  t.s = s;
+  if (s == 1) {
+    t.a[0][0] = 0.5;
+    t.b[0] = 1.0;
+  } else {
+    const double r = std::sqrt(3.0) / 6.0;
+    t.a[0][0] = 0.25;
+    t.a[0][1] = 0.25 - r;
+    t.a[1][0] = 0.25 + r;
+    t.a[1][1] = 0.25;
+    t.b[0] = t.b[1] = 0.5;
+  }
+  return t;
+}
+
+// Implicit Gauss-Legendre steps of M y' = A y (LinearIrk::step with zero
+// forcing): the stage derivatives K solve (I_s (x) M - dt (A_tab (x) A)) K =
+// 1_s (x) A y0, then y1 = y0 + dt sum_i b_i K_i. The reference factors the
+// stage matrix once; here each step solves it by restarted GMRES (right
+// preconditioned by the inverse diagonal of M on every stage block), with the
+// Hessenberg least-squares problem on the host and every vector on the device.
+// energy[s] = 1/2 y^T M y of the state after s steps (conserved by the method).
+static void dirac_irk(Geom& g, int bc, int stages, double dt, int64_t nsteps, double* y_host,
+                      double* states, double* energy, feec_report* report, double rtol) {
+  Mesh& m = *g.mesh;
+  Ctx* ctx = m.ctx;
+  hipStream_t st = ctx->stream;
+  struct Events {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
+  FEEC_HIP(hipEventCreate(&ev.e0));
+  FEEC_HIP(hipEventCreate(&ev.e1));
+  FEEC_HIP(hipEventRecord(ev.e0, st));
+  Dirac D;
+  dirac_assemble(g, bc, -1.0, D);
+  const int64_t N = D.offsets[m.d + 1];
+  const GaussTableau tab = gauss_legendre(stages);
+  const int S = tab.s;
+  const int64_t SN = S * N;
+  constexpr int RESTART = 40;
+  DBuf<double> y(ctx, N), my(ctx, N), ay(ctx, N), dinv(ctx, N), rhs(ctx, SN), k(ctx, SN),
+      w(ctx, SN), t(ctx, SN), u(ctx, SN), part(ctx, 2 * 4096);
+  DBuf<double> V(ctx, (RESTART + 1) * SN);
+  copy_to_device(ctx, y.p, y_host, N * sizeof(double));
+  csr_diagonal(D.mass_block, dinv.p);
+  k_abs_inv<<<grid_for(N), TPB, 0, st>>>(dinv.p, N, dinv.p);
+  FEEC_HIP(hipGetLastError());
+  // out = (I (x) M - dt A_tab (x) A) in, on S stacked blocks of N
+  auto stage_apply = [&](const double* in, double* out) {
+    for (int j = 0; j < S; ++j) spmv64(ctx, D.op, in + j * N, t.p + j * N);
+    for (int i = 0; i < S; ++i) {
+      spmv64(ctx, D.mass_block, in + i * N, out + i * N);
+      for (int j = 0; j < S; ++j)
+        if (tab.a[i][j] != 0.0) axpy64(ctx, t.p + j * N, N, -dt * tab.a[i][j], out + i * N);
+    }
+  };
+  // u = P^-1 v: the inverse mass diagonal on every stage block
+  auto precond = [&](const double* v, double* out) {
+    for (int i = 0; i < S; ++i)
+      k_mul<<<grid_for(N), TPB, 0, st>>>(dinv.p, v + i * N, N, out + i * N);
+  };
+  auto dot = [&](const double* a, const double* b) { return dot64(ctx, a, b, SN, part.p); };
+  int64_t total_iters = 0;
+  double worst = 0.0;
+  auto gmres = [&]() {  // k = stage^-1 rhs from k = 0
+    FEEC_HIP(hipMemsetAsync(k.p, 0, SN * sizeof(double), st));
+    const double bnorm = std::sqrt(dot(rhs.p, rhs.p));
+    if (bnorm == 0.0) return;
+    double res = 1.0;
+    for (int cycle = 0; cycle < 200 && res > rtol; ++cycle) {
+      // r = rhs - S k into V_0
+      stage_apply(k.p, w.p);
+      FEEC_HIP(hipMemcpyAsync(V.p, rhs.p, SN * sizeof(double), hipMemcpyDeviceToDevice, st));
+      axpy64(ctx, w.p, SN, -1.0, V.p);
+      const double beta = std::sqrt(dot(V.p, V.p));
+      res = beta / bnorm;
+      if (res <= rtol) break;
+      k_scaled<<<grid_for(SN), TPB, 0, st>>>(V.p, SN, 1.0 / beta, V.p);
+      std::vector<double> H((RESTART + 1) * RESTART, 0.0), cs(RESTART), sn(RESTART),
+          gv(RESTART + 1, 0.0);
+      gv[0] = beta;
+      int jn = 0;
+      for (int j = 0; j < RESTART; ++j) {
+        double* vj = V.p + j * SN;
+        double* vn = V.p + (j + 1) * SN;
+        precond(vj, u.p);
+        stage_apply(u.p, vn);
+        for (int i = 0; i <= j; ++i) {  // modified Gram-Schmidt
+          const double h = dot(vn, V.p + i * SN);
+          H[i * RESTART + j] = h;
+          axpy64(ctx, V.p + i * SN, SN, -h, vn);
+        }
+        const double hn = std::sqrt(dot(vn, vn));
+        H[(j + 1) * RESTART + j] = hn;
+        if (hn > 0.0) k_scaled<<<grid_for(SN), TPB, 0, st>>>(vn, SN, 1.0 / hn, vn);
+        for (int i = 0; i < j; ++i) {  // earlier rotations on the new column
+          const double a0 = H[i * RESTART + j], a1 = H[(i + 1) * RESTART + j];
+          H[i * RESTART + j] = cs[i] * a0 + sn[i] * a1;
+          H[(i + 1) * RESTART + j] = -sn[i] * a0 + cs[i] * a1;
+        }
+        const double a0 = H[j * RESTART + j], a1 = H[(j + 1) * RESTART + j];
+        const double rr = std::hypot(a0, a1);
+        cs[j] = rr > 0.0 ? a0 / rr : 1.0;
+        sn[j] = rr > 0.0 ? a1 / rr : 0.0;
+        H[j * RESTART + j] = rr;
+        H[(j + 1) * RESTART + j] = 0.0;
+        gv[j + 1] = -sn[j] * gv[j];
+        gv[j] = cs[j] * gv[j];
+        jn = j + 1;
+        ++total_iters;
+        res = std::fabs(gv[j + 1]) / bnorm;
+        if (res <= rtol || hn == 0.0) break;
+      }
+      // z = H^-1 g (upper triangular), k += P^-1 sum_i z_i V_i
+      std::vector<double> z(jn, 0.0);
+      for (int i = jn - 1; i >= 0; --i) {
+        double v = gv[i];
+        for (int c = i + 1; c < jn; ++c) v -= H[i * RESTART + c] * z[c];
+        z[i] = v / H[i * RESTART + i];
+      }
+      FEEC_HIP(hipMemsetAsync(w.p, 0, SN * sizeof(double), st));
+      for (int i = 0; i < jn; ++i) axpy64(ctx, V.p + i * SN, SN, z[i], w.p);
+      precond(w.p, u.p);
+      axpy64(ctx, u.p, SN, 1.0, k.p);
+    }
+    FEEC_REQUIRE(res <= rtol, FEEC_E_NOT_CONVERGED, "Gauss-Legendre stage solve did not converge");
+    worst = std::max(worst, res);
+  };
+  auto energy_of = [&]() {
+    spmv64(ctx, D.mass_block, y.p, my.p);
+    return 0.5 * dot64(ctx, y.p, my.p, N, part.p);
+  };
+  if (energy) energy[0] = energy_of();
+  if (states) std::copy(y_host, y_host + N, states);
+  for (int64_t step = 0; step < nsteps; ++step) {
+    spmv64(ctx, D.op, y.p, ay.p);
+    for (int i = 0; i < S; ++i)
+      FEEC_HIP(hipMemcpyAsync(rhs.p + i * N, ay.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    gmres();
+    for (int i = 0; i < S; ++i) axpy64(ctx, k.p + i * N, N, dt * tab.b[i], y.p);
+    if (energy) energy[step + 1] = energy_of();
+    if (states) copy_to_host(ctx, states + (step + 1) * N, y.p, N * sizeof(double));
+  }
+  copy_to_host(ctx, y_host, y.p, N * sizeof(double));
+  FEEC_HIP(hipEventRecord(ev.e1, st));
+  FEEC_HIP(hipEventSynchronize(ev.e1));
+  float ms = 0.f;
+  FEEC_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
+  if (report) {
+    *report = feec_report{};
+    report->iters = total_iters;
+    report->converged = 1;
+    report->residual = worst;
+    report->seconds = ms * 1e-3;
+  }
+}
+
+feec_status feec_dirac_irk(feec_geom* g, feec_bc bc, int stages, double dt, int64_t nsteps,
+                           double* y, double* states, double* energy, feec_report* report) {
+  return guard([&] {
+    FEEC_REQUIRE(g && y, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(bc == FEEC_FULL || bc == FEEC_RELATIVE, FEEC_E_ARG, "bc must be a feec_bc value");
+    FEEC_REQUIRE(stages == 1 || stages == 2, FEEC_E_ARG, "stages must be 1 or 2");
+    FEEC_REQUIRE(nsteps >= 0 && std::isfinite(dt), FEEC_E_ARG, "nsteps >= 0 and a finite dt");
+    FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
+    dirac_irk(*g, bc, stages, dt, nsteps, y, states, energy, report, 1e-14);
+  });
 }
 
 feec_status feec_dirac_leapfrog(feec_geom* g, feec_bc bc, double dt, int64_t nsteps, double* y,

@@ -221,6 +221,13 @@ feec_status feec_solve_dirac_source(feec_geom *geom, double mass_term, const dou
  * 1e-14 (the reference factors them); the report counts CG iterations and device seconds. */
 feec_status feec_dirac_leapfrog(feec_geom *geom, feec_bc bc, double dt, int64_t nsteps, double *y,
                                 double *states, double *energy, feec_report *report);
+/* LinearIrk::step with zero forcing (time.rs:191-274) and Tableau::gauss_legendre(stages),
+ * stages 1 or 2 (solve_dirac, dirac.rs:383-403, uses 2), on HodgeDirac::assemble of the
+ * complex: y, states and energy as feec_dirac_leapfrog, energy being 1/2 y^T M y. The stage
+ * matrix I (x) M - dt A_tab (x) A is solved per step by restarted GMRES to 1e-14 (the reference
+ * factors it with LU); the report counts GMRES iterations and device seconds. */
+feec_status feec_dirac_irk(feec_geom *geom, feec_bc bc, int stages, double dt, int64_t nsteps,
+                           double *y, double *states, double *energy, feec_report *report);
 
 /* ---- resident problems (device-resident state, pattern reused) ----------- */
 feec_status feec_problem_create(feec_geom *geom, feec_bc bc, int grade, const feec_solve_opts *opts,

@@ -781,3 +781,50 @@ def dirac_leapfrog(c: WhitneyComplex, dt, nsteps, y0):
         y = lf.step(y)
         energy.append(lf.conserved_energy(y))
     return y, np.array(energy)
+
+
+def gauss_legendre(s):
+    """Tableau::gauss_legendre (time.rs:59-67) for s = 1, 2: (A, b, c) of the
+    Gauss collocation methods, from the textbook nodes 1/2 -+ sqrt(3)/6."""
+    if s == 1:
+        return np.array([[0.5]]), np.array([1.0]), np.array([0.5])
+    r = np.sqrt(3.0) / 6.0
+    return (np.array([[0.25, 0.25 - r], [0.25 + r, 0.25]]), np.array([0.5, 0.5]),
+            np.array([0.5 - r, 0.5 + r]))
+
+
+class LinearIrk:
+    """LinearIrk::{new, step} (time.rs:191-274) with zero forcing: the sparse
+    stage matrix I_s (x) M - dt (A_tab (x) A) factored once (SuperLU standing in
+    for faer's LU); a step solves it for the stage derivatives against
+    1_s (x) A y0 and returns y0 + dt sum_i b_i K_i."""
+
+    def __init__(self, stages, mass, op, dt):
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        self.a, self.b, _ = gauss_legendre(stages)
+        m, a = mass.to_scipy().tocsr(), op.to_scipy().tocsr()
+        s = self.a.shape[0]
+        stage = sp.kron(sp.identity(s), m) - dt * sp.kron(sp.csr_matrix(self.a), a)
+        self.lu = spla.splu(stage.tocsc())
+        self.op, self.m, self.dt, self.s, self.n = a, m, dt, s, m.shape[0]
+
+    def step(self, y0):
+        k = self.lu.solve(np.tile(self.op @ y0, self.s))
+        return y0 + self.dt * sum(self.b[i] * k[i * self.n:(i + 1) * self.n] for i in range(self.s))
+
+    def energy(self, y):
+        return 0.5 * y @ (self.m @ y)
+
+
+def dirac_irk(c: WhitneyComplex, dt, nsteps, y0, stages=2):
+    """solve_dirac's loop (dirac.rs:383-403) on the flat field of the complex:
+    the final state and the energy 1/2 y^T M y of every state."""
+    offsets, mass, op = hodge_dirac(c, -1.0)
+    irk = LinearIrk(stages, mass, op, dt)
+    y = np.array(y0, dtype=np.float64)
+    energy = [irk.energy(y)]
+    for _ in range(nsteps):
+        y = irk.step(y)
+        energy.append(irk.energy(y))
+    return y, np.array(energy)

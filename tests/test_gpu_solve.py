@@ -328,6 +328,32 @@ def test_dirac_leapfrog_matches_oracle(d, n, rel):
     np.testing.assert_array_equal(states[3], y3)
 
 
+@pytest.mark.parametrize("d,n,rel,stages", [(1, 8, False, 2), (2, 4, False, 2), (2, 4, True, 2),
+                                            (3, 2, False, 2), (3, 2, True, 1)])
+def test_dirac_gauss_legendre_matches_oracle(d, n, rel, stages):
+    """Gauss-Legendre steps of the Hodge-Dirac flow (LinearIrk, time.rs:191-274)
+    on the GPU, stage systems by GMRES to 1e-14, against the oracle's
+    (stage matrix factored by SuperLU): states after 20 steps agree to 1e-9,
+    and 1/2 y^T M y is conserved as the reference's own test demands
+    (dirac.rs:737-754)."""
+    ocx, oL, gcx, geo, _, _ = both(d, n, 0.1, seed=50 + d)
+    oc = o.WhitneyComplex(ocx, oL, relative=rel)
+    gc = fa.WhitneyComplex(geo, relative=rel)
+    offsets = fa.HodgeDirac.assemble(gc).offsets
+    y0 = np.sin(0.7 * np.arange(int(offsets[-1])) + 0.3) + 0.1
+    dt = 0.05
+    y_g, e_g, rep, states = fa.dirac_irk(gc, dt, 20, y0, stages=stages, keep_states=True)
+    y_o, e_o = o.dirac_irk(oc, dt, 20, y0, stages=stages)
+    assert rep.converged and rep.iters > 0
+    assert np.linalg.norm(y_g - y_o) <= 1e-9 * np.linalg.norm(y_o)
+    assert np.abs(e_g - e_o).max() <= 1e-10 * e_o[0]
+    assert np.abs(e_g - e_g[0]).max() <= 1e-9 * e_g[0]
+    np.testing.assert_array_equal(states[0], y0)
+    np.testing.assert_array_equal(states[-1], y_g)
+    with pytest.raises(fa.FeecError):
+        fa.dirac_irk(gc, dt, 1, y0, stages=3)
+
+
 def test_dirac_lorentzian_4d_assembly():
     ocx, oL, gcx, geo, _, _ = both(4, 2, 0.02, seed=3, minkowski=True, axes=[1, 2, 3])
     oc = o.WhitneyComplex(ocx, oL)

@@ -1,10 +1,16 @@
-"""The oracle's Hodge-Dirac leapfrog (Leapfrog, time.rs:292-420; the loop of
-solve_dirac_leapfrog, dirac.rs:415-434), pinned by what the mathematics fixes:
+"""The oracle's Hodge-Dirac time steppers, pinned by what the mathematics fixes.
+
+Leapfrog (time.rs:292-420; the loop of solve_dirac_leapfrog, dirac.rs:415-434):
 the reference's own conservation law for the staggered invariant
 (dirac.rs:765-784), time reversibility, second-order convergence to the exact
 flow exp(t M^-1 A) y0, and the Stormer-Verlet closed form of a scalar
 oscillator. A wrong kick coefficient, a swapped coloring or a dropped mass
-solve fails one of them."""
+solve fails one of them.
+
+Gauss-Legendre (LinearIrk, time.rs:191-274; solve_dirac, dirac.rs:383-403):
+the collocation and quadrature conditions that define the tableau, the
+reference's energy conservation law (dirac.rs:737-754), and orders 2 and 4
+against the exact flow."""
 import numpy as np
 import pytest
 import scipy.linalg as sla
@@ -80,3 +86,39 @@ def test_scalar_oscillator_closed_form():
         y = lf.step(y)
     theta = np.arccos(1.0 - (w * dt) ** 2 / 2.0)
     assert abs(y[0] - np.cos(n * theta)) <= 1e-12
+
+
+@pytest.mark.parametrize("s", [1, 2])
+def test_gauss_legendre_tableau_collocation_and_quadrature(s):
+    # Gauss collocation: A c^(k-1) = c^k / k (k = 1..s) and b . c^(k-1) = 1/k
+    # (k = 1..2s), which fix A, b and c
+    a, b, c = o.gauss_legendre(s)
+    for k in range(1, s + 1):
+        np.testing.assert_allclose(a @ c ** (k - 1), c ** k / k, rtol=0, atol=1e-15)
+    for k in range(1, 2 * s + 1):
+        assert abs(b @ c ** (k - 1) - 1.0 / k) <= 1e-15
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_gauss_legendre_conserves_energy(d):
+    # dirac.rs:737-754: unit 2-grid, dt = 0.05, 100 steps, 1/2 y^T M y to 1e-9
+    c, _ = unit_complex(d)
+    offsets, _, _ = o.hodge_dirac(c, -1.0)
+    y, e = o.dirac_irk(c, 0.05, 100, seed_field(int(offsets[-1])))
+    assert e[0] > 0.0
+    assert np.abs(e - e[0]).max() <= 1e-9 * e[0]
+
+
+@pytest.mark.parametrize("s,order", [(1, 2), (2, 4)])
+def test_gauss_legendre_order(s, order):
+    c, L = unit_complex(2, 3)
+    offsets, mass, op = o.hodge_dirac(c, -1.0)
+    y0 = seed_field(int(offsets[-1]))
+    M = mass.to_scipy().toarray()
+    A = op.to_scipy().toarray()
+    dt, n = 0.01, 8  # dt times the spectral radius of M^-1 A (18) well below 1
+    exact = sla.expm(n * dt * np.linalg.solve(M, A)) @ y0
+    errs = [np.linalg.norm(o.dirac_irk(c, dt / k, n * k, y0, stages=s)[0] - exact) for k in (1, 2, 4)]
+    ratio = 2.0 ** order
+    assert 0.75 * ratio <= errs[0] / errs[1] <= 1.25 * ratio, errs
+    assert 0.75 * ratio <= errs[1] / errs[2] <= 1.25 * ratio, errs
