@@ -149,7 +149,8 @@ def run_leapfrog(args):
     complex of a jittered 3D Kuhn mesh (default 32^3, seed 6), dt = 0.1 h_min,
     all grades: GPU steps/s with the CG mass solves, the staggered invariant's
     drift, and the oracle (SuperLU-factored color masses) timed on a 12^3
-    sample of the same workload."""
+    sample of the same workload. Beside it, a few implicit Gauss-Legendre
+    steps (solve_dirac's integrator) on the same mesh and dt."""
     import formoniq_amd as fa
     from oracle import oracle as o
     n = args.n if args.n != 64 else 32
@@ -173,6 +174,11 @@ def run_leapfrog(args):
     y, e, rep = fa.dirac_leapfrog(wc, dt, nsteps, y0)
     wall = time.perf_counter() - t0
     drift = float(np.abs(e - e[0]).max() / e[0])
+    # implicit Gauss-Legendre (2 stages) on the same mesh and dt, a few steps
+    irk_steps = 3
+    fa.dirac_irk(wc, dt, 1, y0)  # warm-up
+    yi, ei, repi = fa.dirac_irk(wc, dt, irk_steps, y0)
+    irk_drift = float(np.abs(ei - ei[0]).max() / ei[0])
     # the oracle on a 12^3 sample, same dt rule
     cs, xs = mesh(12)
     ocx = o.Complex(3, cs)
@@ -191,6 +197,9 @@ def run_leapfrog(args):
         "steps": nsteps, "steps_per_s": nsteps / rep.seconds, "device_seconds": rep.seconds,
         "wall_seconds": wall, "cg_iters_per_solve": rep.iters / (2 * nsteps + 1),
         "energy_rel_drift": drift, "dtype": "f64",
+        "gauss_legendre_2": {"steps": irk_steps, "steps_per_s": irk_steps / repi.seconds,
+                             "gmres_iters_per_step": repi.iters / irk_steps,
+                             "energy_rel_drift": irk_drift},
         "cpu_baseline": {"steps_per_s": 20 / cpu, "kind": "port", "cores": 1,
                          "sample": "oracle leapfrog (SuperLU-factored color masses), 20 steps on a 12^3 "
                                    "Kuhn mesh, stepping only", "sample_dofs": int(ys.size)},
