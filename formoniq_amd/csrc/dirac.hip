@@ -53,6 +53,52 @@ __global__ void k_scaled(const double* __restrict__ a, int64_t n, double s, doub
   if (i < n) x[i] = s * a[i];
 }
 
+// Sum of a block's values on thread 0 (fixed order: wavefront tree, then the
+// block's wavefronts in order).
+__device__ double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (unsigned w = 0; w < blockDim.x / 64; ++w) t += sh[w];
+  return t;
+}
+
+// part[i * gridDim.x + b] = block b's share of V_i . w (blockIdx.y = i)
+__global__ void __launch_bounds__(TPB) k_multi_dot(const double* __restrict__ V, int64_t stride,
+                                                   const double* __restrict__ w, int64_t n,
+                                                   double* __restrict__ part) {
+  __shared__ double sh[TPB / 64];
+  const double* v = V + blockIdx.y * stride;
+  double acc = 0.0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x)
+    acc += v[r] * w[r];
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+// h[i] (+)= sum of the np partials of i (block i); accumulate when `add`
+__global__ void __launch_bounds__(TPB) k_multi_finish(const double* __restrict__ part, int np,
+                                                      double* __restrict__ h, int add) {
+  __shared__ double sh[TPB / 64];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < np; b += blockDim.x) acc += part[blockIdx.x * np + b];
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) h[blockIdx.x] = add ? h[blockIdx.x] + t : t;
+}
+
+// w -= sum_i c[i] V_i, with c on the device (nv terms)
+__global__ void k_multi_axpy(const double* __restrict__ V, int64_t stride, int nv,
+                             const double* __restrict__ c, int64_t n, double* __restrict__ w) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double acc = w[r];
+  for (int i = 0; i < nv; ++i) acc -= c[i] * V[i * stride + r];
+  w[r] = acc;
+}
+
 // x = y on the rows whose grade has parity `odd`, 0 on the others (the
 // grade-parity coloring of the leapfrog split).
 __global__ void k_parity_part(const double* __restrict__ y, GradeOffsets go, int odd,
@@ -249,8 +295,10 @@ static GaussTableau gauss_legendre(int s) {
 // forcing): the stage derivatives K solve (I_s (x) M - dt (A_tab (x) A)) K =
 // 1_s (x) A y0, then y1 = y0 + dt sum_i b_i K_i. The reference factors the
 // stage matrix once; here each step solves it by restarted GMRES (right
-// preconditioned by the inverse diagonal of M on every stage block), with the
-// Hessenberg least-squares problem on the host and every vector on the device.
+// preconditioned by the inverse diagonal of M on every stage block; classical
+// Gram-Schmidt applied twice, each pass one launch of batched dot products and
+// one update), with the Hessenberg least-squares problem on the host and every
+// vector on the device.
 // energy[s] = 1/2 y^T M y of the state after s steps (conserved by the method).
 static void dirac_irk(Geom& g, int bc, int stages, double dt, int64_t nsteps, double* y_host,
                       double* states, double* energy, feec_report* report, double rtol) {
@@ -277,6 +325,8 @@ static void dirac_irk(Geom& g, int bc, int stages, double dt, int64_t nsteps, do
   DBuf<double> y(ctx, N), my(ctx, N), ay(ctx, N), dinv(ctx, N), rhs(ctx, SN), k(ctx, SN),
       w(ctx, SN), t(ctx, SN), u(ctx, SN), part(ctx, 2 * 4096);
   DBuf<double> V(ctx, (RESTART + 1) * SN);
+  const unsigned GD = std::min<unsigned>(grid_for(SN), 512);
+  DBuf<double> mpart(ctx, (size_t)(RESTART + 1) * GD), hcol(ctx, 2 * (RESTART + 1));
   copy_to_device(ctx, y.p, y_host, N * sizeof(double));
   csr_diagonal(D.mass_block, dinv.p);
   k_abs_inv<<<grid_for(N), TPB, 0, st>>>(dinv.p, N, dinv.p);
@@ -321,11 +371,19 @@ static void dirac_irk(Geom& g, int bc, int stages, double dt, int64_t nsteps, do
         double* vn = V.p + (j + 1) * SN;
         precond(vj, u.p);
         stage_apply(u.p, vn);
-        for (int i = 0; i <= j; ++i) {  // modified Gram-Schmidt
-          const double h = dot(vn, V.p + i * SN);
-          H[i * RESTART + j] = h;
-          axpy64(ctx, V.p + i * SN, SN, -h, vn);
+        // classical Gram-Schmidt, twice (one dot launch and one update launch
+        // per pass over V_0..V_j; the coefficients stay on the device)
+        const int nv = j + 1;
+        for (int pass = 0; pass < 2; ++pass) {
+          k_multi_dot<<<dim3(GD, nv), TPB, 0, st>>>(V.p, SN, vn, SN, mpart.p);
+          k_multi_finish<<<nv, TPB, 0, st>>>(mpart.p, (int)GD, hcol.p + pass * (RESTART + 1), 0);
+          k_multi_axpy<<<grid_for(SN), TPB, 0, st>>>(V.p, SN, nv, hcol.p + pass * (RESTART + 1), SN,
+                                                     vn);
         }
+        FEEC_HIP(hipGetLastError());
+        std::vector<double> hh(2 * (RESTART + 1));
+        copy_to_host(ctx, hh.data(), hcol.p, hh.size() * sizeof(double));
+        for (int i = 0; i <= j; ++i) H[i * RESTART + j] = hh[i] + hh[RESTART + 1 + i];
         const double hn = std::sqrt(dot(vn, vn));
         H[(j + 1) * RESTART + j] = hn;
         if (hn > 0.0) k_scaled<<<grid_for(SN), TPB, 0, st>>>(vn, SN, 1.0 / hn, vn);
