@@ -34,10 +34,18 @@ def standalone_u(r):
     return u
 
 
+def continued_req():
+    # 2D 64^2 k=1, jitter 0.1 (mesh seed 94, source seed 31): an FP64_PC32 solve
+    # whose first run falls short of the true residual and continues
+    r = req(2, 1, 64, 94, fa.FP64_PC32, jitter=0.1)
+    r.source_seed = 31
+    return r
+
+
 MIX = [req(2, 0, 12, 1), req(2, 1, 10, 2), req(3, 0, 5, 3), req(3, 1, 4, 4), req(3, 2, 4, 5),
        req(4, 0, 2, 6), req(4, 1, 2, 7), req(4, 3, 2, 8), req(2, 1, 16, 9, fa.FP64),
        req(3, 1, 5, 10, fa.FP64_PC32), req(2, 2, 12, 11, fa.FP64_PC32),
-       req(2, 1, 96, 12, fa.FP64_PC32, jitter=0.1)]
+       req(2, 1, 96, 12, fa.FP64_PC32, jitter=0.1), continued_req()]
 
 
 def checksum(u):
@@ -68,6 +76,8 @@ def test_batch_lockstep_is_bit_identical_to_single_solves():
         assert (r.iters, r.refinements) == (s_.iters, s_.refinements), (q.dim, q.grade)
         assert r.residual == s_.residual, (q.dim, q.grade)
         assert r.u_norm == s_.u_norm and r.u_checksum == s_.u_checksum, (q.dim, q.grade)
+    # the last request's run continued (lockstep in the batch, by itself in the server)
+    assert res[-1].refinements == 1, res[-1].as_dict()
 
 
 def run_server(reqs, inflight, budget=32 << 30):
