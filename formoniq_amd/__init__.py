@@ -187,6 +187,17 @@ class Csr:
         check(load().feec_csr_spmv(self._h, _ptr(x, _lib.f64p), _ptr(y, _lib.f64p)))
         return y
 
+    def cg(self, b, rtol=1e-10, max_iters=10_000, raise_on_failure=True):
+        """krylov::cg with Jacobi (krylov.rs:48-95) on the device: (x, Report)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.shape[0])
+        rep = Report()
+        st = load().feec_csr_cg(self._h, _ptr(b, _lib.f64p), _ptr(x, _lib.f64p), float(rtol),
+                                int(max_iters), C.byref(rep))
+        if raise_on_failure:
+            check(st)
+        return x, rep
+
 
 class Complex:
     """A simplicial complex numbered on the device (Complex::from_cells)."""

@@ -94,6 +94,7 @@ SIGNATURES = {
     "feec_csr_shape": (S, [vp, i64p, i64p, i64p]),
     "feec_csr_copy": (S, [vp, i64p, i32p, f64p]),
     "feec_csr_spmv": (S, [vp, f64p, f64p]),
+    "feec_csr_cg": (S, [vp, f64p, f64p, C.c_double, C.c_int64, C.POINTER(Report)]),
     "feec_csr_destroy": (None, [vp]),
     "feec_harmonics": (S, [vp, C.c_int, C.c_int, i64p, f64p]),
     "feec_mixed_kkt": (S, [vp, C.c_int, C.c_int, C.POINTER(vp), i64p]),

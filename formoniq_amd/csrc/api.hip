@@ -1011,6 +1011,43 @@ feec_status feec_csr_spmv(feec_csr* a, const double* x, double* y) {
 
 void feec_csr_destroy(feec_csr* a) { delete a; }
 
+namespace {
+__global__ void k_inverse(const double* d, int64_t n, double* out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = d[i] != 0.0 ? 1.0 / d[i] : 1.0;
+}
+}  // namespace
+
+feec_status feec_csr_cg(feec_csr* a, const double* b, double* x, double rtol, int64_t max_iters,
+                        feec_report* report) {
+  return guard([&] {
+    FEEC_REQUIRE(a && b && x, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(a->nrows == a->ncols, FEEC_E_ARG, "CG needs a square matrix");
+    FEEC_REQUIRE(rtol >= 0.0 && max_iters >= 0, FEEC_E_ARG, "rtol >= 0 and max_iters >= 0");
+    Ctx* ctx = a->ctx;
+    FEEC_HIP(hipSetDevice(ctx->device));
+    const int64_t n = a->nrows;
+    const int64_t nb = std::max<int64_t>(n, 1);
+    DBuf<double> db(ctx, nb), dx(ctx, nb), dinv(ctx, nb);
+    copy_to_device(ctx, db.p, b, n * 8);
+    // Jacobi (precond.rs:53-123): the inverse diagonal, 1 where it is zero
+    csr_diagonal(*a, dinv.p);
+    k_inverse<<<(unsigned)std::max<int64_t>(1, (n + 255) / 256), 256, 0, ctx->stream>>>(dinv.p, n,
+                                                                                      dinv.p);
+    FEEC_HIP(hipGetLastError());
+    KrylovStats ks = cg_solve_device(ctx, *a, dinv.p, db.p, dx.p, rtol, max_iters);
+    copy_to_host(ctx, x, dx.p, n * 8);
+    if (report) {
+      *report = feec_report{};
+      report->iters = ks.iters;
+      report->converged = ks.converged ? 1 : 0;
+      report->residual = ks.precond_residual;
+      report->precond_residual = ks.precond_residual;
+    }
+    FEEC_REQUIRE(ks.converged, FEEC_E_NOT_CONVERGED, "CG did not reach rtol within max_iters");
+  });
+}
+
 // ---- problems ---------------------------------------------------------------------
 feec_status feec_harmonics(feec_geom* g, feec_bc bc, int grade, int64_t* count, double* basis) {
   return guard([&] {

@@ -351,9 +351,8 @@ void kkt_apply_preconditioner(KktSystem& s, int precision, int storage, const do
 // The feec_pc_format a solve of `precision` with `storage` applies.
 int kkt_preconditioner_format(const KktSystem& s, int precision, int storage);
 double time_cheb_step(KktSystem& s, bool fp32, int reps, int storage);
-KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
-                     double rtol, int64_t max_iters);
-// The same CG with its scalars on the device (host read every 8 iterations).
+// Jacobi-preconditioned CG (krylov.rs:48-95: x0 = 0, stop when |r| <= rtol |b|)
+// with its scalars on the device (host read every 8 iterations).
 KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
                             double rtol, int64_t max_iters);
 // a . b (part: 2 * 4096 doubles of scratch) and x += s a, on device doubles

@@ -1006,10 +1006,6 @@ __global__ void k_power_normalize(const double* y, int64_t split, int64_t n, con
   if (i == (b ? split : 0) && nx > 0) lam[b] = ny / nx;
   if (ny > 0) x[i] = (1.0 / ny) * y[i];
 }
-__global__ void k_cg_p(double* pp, const double* zz, double bb, int64_t nn) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < nn) pp[i] = bb * pp[i] + zz[i];
-}
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
 inline unsigned red_grid(int64_t n) {
@@ -2932,58 +2928,5 @@ KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const do
   ks.converged = h.bnorm2 == 0.0 || h.res2 <= h.rtol2 * h.bnorm2;
   return ks;
 }
-
-// Jacobi-preconditioned CG (krylov.rs:48-95) on an FP64 CSR; host-checked
-// every iteration (used for mass solves and the harmonic normal equations).
-KrylovStats cg_solve(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
-                     double rtol, int64_t max_iters) {
-  const int64_t n = a.nrows;
-  hipStream_t st = ctx->stream;
-  DBuf<double> r(ctx, n), z(ctx, n), p(ctx, n), ap(ctx, n), part(ctx, NBLK * 2);
-  auto dot = [&](const double* u, const double* v) {
-    k_dot<double><<<red_grid(n), TPB, 0, st>>>(u, v, n, part.p);
-    k_finish_sum<<<1, TPB, 0, st>>>(part.p, red_grid(n), part.p + NBLK);
-    double h = 0.0;
-    FEEC_HIP(hipMemcpyAsync(&h, part.p + NBLK, sizeof(double), hipMemcpyDeviceToHost, st));
-    FEEC_HIP(hipStreamSynchronize(st));
-    return h;
-  };
-  KrylovStats ks;
-  FEEC_HIP(hipMemsetAsync(x, 0, n * sizeof(double), st));
-  double bn = std::sqrt(dot(b, b));
-  if (bn == 0.0) {
-    ks.converged = true;
-    ks.precond_residual = 0.0;
-    return ks;
-  }
-  FEEC_HIP(hipMemcpyAsync(r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-  auto apply_p = [&](double* out, const double* in) {
-    k_jacobi<<<grid_for(n), TPB, 0, st>>>(dinv, in, out, n);
-  };
-  apply_p(z.p, r.p);
-  FEEC_HIP(hipMemcpyAsync(p.p, z.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-  double rz = dot(r.p, z.p);
-  int64_t iters = 0;
-  double res = 1.0;
-  while (true) {
-    res = std::sqrt(dot(r.p, r.p)) / bn;
-    if (res <= rtol || iters >= max_iters) break;
-    spmv64(ctx, a, p.p, ap.p);
-    double alpha = rz / dot(p.p, ap.p);
-    k_axpy<<<grid_for(n), TPB, 0, st>>>(p.p, n, alpha, x);
-    k_axpy<<<grid_for(n), TPB, 0, st>>>(ap.p, n, -alpha, r.p);
-    apply_p(z.p, r.p);
-    double rzn = dot(r.p, z.p);
-    double beta = rzn / rz;
-    k_cg_p<<<grid_for(n), TPB, 0, st>>>(p.p, z.p, beta, n);
-    rz = rzn;
-    ++iters;
-  }
-  ks.iters = iters;
-  ks.precond_residual = res;
-  ks.converged = res <= rtol;
-  return ks;
-}
-
 
 }  // namespace feec

@@ -188,6 +188,11 @@ feec_status feec_assemble(feec_geom *geom, feec_bc bc, feec_form_kind kind, int 
 feec_status feec_csr_shape(const feec_csr *a, int64_t *nrows, int64_t *ncols, int64_t *nnz);
 feec_status feec_csr_copy(const feec_csr *a, int64_t *row_ptr, int32_t *col, double *val);
 feec_status feec_csr_spmv(feec_csr *a, const double *x, double *y); /* host vectors */
+/* krylov::cg with the Jacobi preconditioner (krylov.rs:48-95, precond.rs:53-123) on a library
+ * CSR, scalars on the device: x (host) from x0 = 0 until |r| <= rtol |b|. The report holds the
+ * iterations and |r|/|b|; FEEC_E_NOT_CONVERGED when max_iters ends the run first. */
+feec_status feec_csr_cg(feec_csr *a, const double *b, double *x, double rtol, int64_t max_iters,
+                        feec_report *report);
 void feec_csr_destroy(feec_csr *a);
 
 /* ---- problems ------------------------------------------------------------ */

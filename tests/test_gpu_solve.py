@@ -354,6 +354,28 @@ def test_dirac_gauss_legendre_matches_oracle(d, n, rel, stages):
         fa.dirac_irk(gc, dt, 1, y0, stages=3)
 
 
+@pytest.mark.parametrize("d,n,k", [(2, 8, 1), (3, 4, 1), (3, 4, 2)])
+def test_csr_cg_matches_oracle(d, n, k):
+    """krylov::cg with Jacobi (krylov.rs:48-95) on a GPU-assembled mass matrix:
+    the oracle's CG (the same algorithm in FP64 on the host) takes the same
+    number of iterations to within one, and both match a direct solve."""
+    ocx, oL, gcx, geo, _, _ = both(d, n, 0.15, seed=60 + d + k)
+    M = fa.WhitneyComplex(geo).mass(k)
+    oM = o.assemble(ocx, oL, o.MASS, k)
+    b = np.sin(np.arange(M.shape[0]) * 0.5 + 0.5)
+    x, rep = M.cg(b, rtol=1e-12)
+    xo, repo = o.cg(oM, b, 1.0 / oM.diagonal(), 1e-12)
+    assert rep.converged and abs(rep.iters - repo.iters) <= 1, (rep.as_dict(), repo)
+    direct = spla.spsolve(oM.to_scipy().tocsc(), b)
+    np.testing.assert_allclose(x, direct, rtol=1e-9, atol=1e-12 * np.abs(direct).max())
+    np.testing.assert_allclose(x, xo, rtol=1e-9, atol=1e-12 * np.abs(direct).max())
+    # a budget too small: FEEC_E_NOT_CONVERGED, the report still filled
+    with pytest.raises(fa.FeecError):
+        M.cg(b, rtol=1e-12, max_iters=2)
+    _, short = M.cg(b, rtol=1e-12, max_iters=2, raise_on_failure=False)
+    assert short.iters == 2 and not short.converged
+
+
 def test_dirac_lorentzian_4d_assembly():
     ocx, oL, gcx, geo, _, _ = both(4, 2, 0.02, seed=3, minkowski=True, axes=[1, 2, 3])
     oc = o.WhitneyComplex(ocx, oL)
