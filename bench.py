@@ -378,8 +378,8 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="c5: requests resident at once")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported in this build "
-              "(the path shards by design only; see the multi-GPU section of DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the multi-GPU "
+              "sharding of assembly, MINRES and the batches is designed but not built)", file=sys.stderr)
         sys.exit(2)
     if args.config != "c2":
         {"c3": run_c3, "c4": run_c4, "c5": run_c5, "kernels": run_kernels,
