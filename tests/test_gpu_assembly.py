@@ -2,7 +2,7 @@
 
 Numbering, patterns, orientation signs and boundary flags are compared bit for
 bit; element and Galerkin matrix values within 1e-12 of the matrix maximum
-(the north-star tolerance for FP64 element and global matrices)."""
+(the tolerance required of FP64 element and global matrices)."""
 import numpy as np
 import pytest
 

@@ -420,6 +420,24 @@ def test_dirac_source_matches_direct(m):
     assert (out[~keep] == 0).all()
 
 
+def test_full_size_3d_64_agrees_across_precisions():
+    """The 3D Kuhn 64^3 k=1 workload at full size (2.1 M KKT rows): the
+    FP64_PC32 and FP64 solves each meet the 1e-10 certificate, and their
+    solutions agree to 1e-8 (a size-independent check of the mixed path)."""
+    g = fa.CartesianGrid(3, 64)
+    cells, coords = g.triangulate()
+    coords = g.jitter(coords, 0.05, 2)
+    topo = fa.Complex(3, cells, nverts=len(coords))
+    wc = fa.WhitneyComplex(fa.MeshLengthsSq.from_coords(topo, coords))
+    f = fa.normal_vector(topo.nsimplices(1), 2)
+    _, u32, _, r32 = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64_PC32))
+    _, u64, _, r64 = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64))
+    assert r32.converged and r32.residual <= 1e-10, r32.as_dict()
+    assert r64.converged and r64.residual <= 1e-10, r64.as_dict()
+    assert r32.iters < r64.iters
+    assert np.linalg.norm(u32 - u64) <= 1e-8 * np.linalg.norm(u64)
+
+
 def test_fp64_pc32_continued_run():
     """When the first FP64_PC32 run meets its target in the preconditioner
     norm but the true residual falls short, the same MINRES run continues to a
