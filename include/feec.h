@@ -43,7 +43,7 @@ typedef enum feec_status {
   FEEC_E_DEGENERATE_CELL = 5, /* a cell metric with det g == 0 */
   FEEC_E_NOT_SPD = 6,         /* an SPD-only path met an indefinite block */
   FEEC_E_NOT_CONVERGED = 7,   /* Krylov budget spent; the report holds the state */
-  FEEC_E_UNSUPPORTED = 8,     /* a case outside what this build implements */
+  FEEC_E_UNSUPPORTED = 8,     /* a case outside what the library implements */
   FEEC_E_ADMISSION = 9,       /* serving: request larger than the HBM budget */
   FEEC_E_AGAIN = 10           /* serving: result not ready yet */
 } feec_status;
