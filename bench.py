@@ -325,7 +325,7 @@ def run_c5(args):
 
 
 def run_kernels(args):
-    """Short run for hardware-counter passes: the C2 mixed problem's element
+    """Short run for hardware-counter passes: the 3D 64^3 k=1 mixed problem's element
     kernel, KKT scatter and fused MINRES SpMV (FP32, FP64) launched a few
     times each, then a streaming kernel of known size (torch multiply of
     256 Mi floats: 1 GiB read, 1 GiB written) to calibrate byte counters."""
@@ -368,7 +368,7 @@ def main():
     ap.add_argument("--graph-iters", type=int, default=32,
                     help="MINRES iterations per replayed graph segment")
     ap.add_argument("--no-warmup", action="store_true",
-                    help="C5: start the timed stream on a cold server (no start-up requests)")
+                    help="c5: start the timed stream on a cold server (no start-up requests)")
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "kernels", "leapfrog"],
                     help="c2 (default): the headline line; c3/c4/c5: the 4D Dirac element, batched "
                          "and serving workloads, each printed as its own JSON line")

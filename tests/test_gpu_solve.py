@@ -73,7 +73,7 @@ def test_solve_source(d, n, amp, rel, precision):
 
 
 def test_c1_configuration_fp64_and_mixed():
-    """C1: 2D 32x32, jitter 0.1h seed 0, k=1, source N(0,1) seed 1, full complex."""
+    """2D 32x32, jitter 0.1h seed 0, k=1, source N(0,1) seed 1, full complex."""
     ocx, oL, gcx, geo, _, _ = both(2, 32, 0.1, seed=0)
     oc = o.WhitneyComplex(ocx, oL)
     gc = fa.WhitneyComplex(geo)
