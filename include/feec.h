@@ -223,7 +223,8 @@ feec_status feec_solve_dirac_source(feec_geom *geom, double mass_term, const dou
  * advances nsteps steps of dt in place; states ((nsteps + 1) * ndofs values, or null) receives
  * every state from the initial one on, as the reference's returned vector of fields; energy
  * (nsteps + 1 values, or null) the staggered invariant of each. Mass solves by Jacobi CG to
- * 1e-14 (the reference factors them); the report counts CG iterations and device seconds. */
+ * 1e-14 (the reference factors them), so the mass matrix must be SPD (a Riemannian geometry;
+ * otherwise FEEC_E_NOT_CONVERGED); the report counts CG iterations and device seconds. */
 feec_status feec_dirac_leapfrog(feec_geom *geom, feec_bc bc, double dt, int64_t nsteps, double *y,
                                 double *states, double *energy, feec_report *report);
 /* LinearIrk::step with zero forcing (time.rs:191-274) and Tableau::gauss_legendre(stages),
