@@ -549,12 +549,14 @@ def dirac_irk(complex_: WhitneyComplex, dt, nsteps, y0, stages=2, keep_states=Fa
 
 
 def solve_dirac_source(geometry: MeshLengthsSq, mass_term, load_field, lift, opts=None):
-    """solve_dirac_source on the relative complex; flat full-layout fields."""
+    """solve_dirac_source on the relative complex; flat full-layout fields
+    (lift None: the zero lift)."""
     f = np.ascontiguousarray(load_field, dtype=np.float64)
-    l = np.ascontiguousarray(lift, dtype=np.float64)
+    l = None if lift is None else np.ascontiguousarray(lift, dtype=np.float64)
     out = np.zeros_like(f)
     rep = Report()
-    check(load().feec_solve_dirac_source(geometry._h, mass_term, _ptr(f, _lib.f64p), _ptr(l, _lib.f64p),
+    check(load().feec_solve_dirac_source(geometry._h, mass_term, _ptr(f, _lib.f64p),
+                                         None if l is None else _ptr(l, _lib.f64p),
                                          C.byref(opts) if opts is not None else None,
                                          _ptr(out, _lib.f64p), C.byref(rep)))
     return out, rep
@@ -600,6 +602,16 @@ class Server:
 
     def drain(self):
         check(load().feec_server_drain(self._h))
+
+    def release(self, ticket):
+        """Forget a finished request (feec_server_release)."""
+        check(load().feec_server_release(self._h, ticket))
+
+    def held(self):
+        """(requests held, solution doubles held on the host)."""
+        e, d = C.c_int64(), C.c_int64()
+        check(load().feec_server_held(self._h, C.byref(e), C.byref(d)))
+        return e.value, d.value
 
     def close(self, _load=load):
         if getattr(self, "_h", None):

@@ -127,6 +127,8 @@ SIGNATURES = {
     "feec_server_poll": (S, [vp, C.c_int64, C.POINTER(Result)]),
     "feec_server_solution": (S, [vp, C.c_int64, f64p, i64p]),
     "feec_server_drain": (S, [vp]),
+    "feec_server_release": (S, [vp, C.c_int64]),
+    "feec_server_held": (S, [vp, i64p, i64p]),
     "feec_server_destroy": (None, [vp]),
 }
 

@@ -223,16 +223,22 @@ int64_t euler_characteristic(const Mesh& m) {
 
 // Harmonic basis on a ball (elliptic.rs:197-220): b_0 = 1 on the full complex
 // (the normalized constant), b_n = 1 on the relative complex (M_n^-1 times the
-// orientation, dirac.rs:553-568); every other space is trivial. Anything that
-// is not a ball (chi != 1) is outside what the closed forms cover.
+// orientation, dirac.rs:553-568); every other space is trivial. The closed
+// forms are used only when the mesh passes the ball test -- one connected
+// component and Euler characteristic 1 -- and every grade of any other mesh
+// (an annulus, a disconnected union) is FEEC_E_UNSUPPORTED, so no problem is
+// built without the harmonic border it needs. (The test is necessary, not
+// sufficient: in 3D a connected mesh with chi = 1 can still have b_1 = b_2 > 0.)
 void harmonic_basis(Geom& g, int bc, int k, std::vector<double>& basis, int64_t& count) {
   Mesh& m = *g.mesh;
   count = 0;
   basis.clear();
+  m.ensure_components();
+  FEEC_REQUIRE(m.components == 1 && euler_characteristic(m) == 1, FEEC_E_UNSUPPORTED,
+               "harmonic forms are implemented in closed form for balls (one component, Euler "
+               "characteristic 1)");
   bool candidate = (bc == FEEC_FULL && k == 0) || (bc == FEEC_RELATIVE && k == m.d);
   if (!candidate) return;
-  FEEC_REQUIRE(euler_characteristic(m) == 1, FEEC_E_UNSUPPORTED,
-               "harmonic forms are implemented in closed form for balls (Euler characteristic 1)");
   Csr M;
   assemble_form(g, bc, FEEC_MASS, k, M);
   int64_t n = M.nrows;
@@ -644,7 +650,6 @@ void feec_ctx_destroy(feec_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  elmat_release_tables(ctx);
   if (ctx->temp) device_free(ctx, ctx->temp, ctx->temp_bytes);
   if (ctx->dflag) device_free(ctx, ctx->dflag, sizeof(int));
   if (ctx->dcount) device_free(ctx, ctx->dcount, sizeof(unsigned long long));

@@ -476,6 +476,8 @@ feec_status feec_solve_dirac_source(feec_geom* g, double mass_term, const double
                                     const double* lift, const feec_solve_opts* opts, double* out,
                                     feec_report* report) {
   return guard([&] {
+    FEEC_REQUIRE(g && load && out, FEEC_E_ARG, "null geometry, load or output");
+    FEEC_REQUIRE(std::isfinite(mass_term), FEEC_E_ARG, "mass_term must be finite");
     Mesh& m = *g->mesh;
     Ctx* ctx = m.ctx;
     FEEC_HIP(hipSetDevice(ctx->device));
@@ -486,6 +488,11 @@ feec_status feec_solve_dirac_source(feec_geom* g, double mass_term, const double
       feec_solve_opts_default(&o);
     o.precision = FEEC_PREC_FP64;  // the reference solve is a direct FP64 factorization
     const int d = m.d;
+    int64_t nflat = 0;
+    for (int k = 0; k <= d; ++k) nflat += m.n[k];
+    for (int64_t i = 0; i < nflat; ++i)
+      FEEC_REQUIRE(std::isfinite(load[i]) && (!lift || std::isfinite(lift[i])), FEEC_E_ARG,
+                   "load and lift must be finite");
     Dirac D;
     dirac_assemble(*g, FEEC_FULL, 1.0, D);
     const int64_t nfull = D.offsets[d + 1];
@@ -516,7 +523,8 @@ feec_status feec_solve_dirac_source(feec_geom* g, double mass_term, const double
     DBuf<int32_t> dmap(ctx, nfull);
     copy_to_device(ctx, dmap.p, map.data(), nfull * 4);
     // rhs = E^T (f - S lift)
-    std::vector<double> f(load, load + nfull), l(lift, lift + nfull), sl(nfull);
+    std::vector<double> f(load, load + nfull), l(nfull, 0.0), sl(nfull);
+    if (lift) std::copy(lift, lift + nfull, l.begin());
     DBuf<double> dl(ctx, nfull), dsl(ctx, nfull);
     copy_to_device(ctx, dl.p, l.data(), nfull * 8);
     spmv64(ctx, S, dl.p, dsl.p);

@@ -1,33 +1,41 @@
-// Per-cell Regge metric and Whitney element matrices, FP64, one thread per cell.
+// Per-cell Regge metric and Whitney element matrices, FP64, one lane per cell.
 //
 // Reference behaviour (luiswirth/formoniq, crates/):
 //   SimplexLengthsSq::metric       regge/src/lengths/simplex.rs:308-326
 //   Metric::{det, dual}            metric/src/lib.rs:179-190, 250-257
-//   cell_volume                    regge/src/lib.rs:26-28
+//   cell_volume                    regge/src/lib.rs:26-28, simplicial/src/atlas.rs:102-105
 //   multiform_metric               metric/src/tensor.rs:127-131
+//   determinant (Leibniz)          multialgebra/src/lib.rs:447-456,
+//                                  multiindex/src/permutation.rs:160-169
 //   HodgeMass::element             formoniq/src/operators.rs:84-94
 //   WhitneyExpansion::pullback     derham/src/interpolate/form.rs:195-232
+//   unit_bary_gramian              simplicial/src/atlas.rs:198-204
 //   WhitneyPairing::element        formoniq/src/operators.rs:201-211
 //
-// Formulation. As the reference, H = P (Lambda^K g^-1) P^T with P the K-th
-// compound of the barycentric differentials Dlambda (row 0 = -1, rows 1..d =
-// I); P has entries 0 and +-1 in closed form. The pullback sums (K+1)^2 signed
-// H entries per mass entry with weights 1 or 2 (the barycentric Gramian
-// Q = (1 + delta)/((d+1)(d+2))), and the exterior derivative sides are +-1
-// incidences, so every element block -- mass, dif_trial, dif_test, dif_both --
-// is a fixed integer-coefficient linear map of the NU = n(n+1)/2 unique
-// entries of the symmetric Lambda^K g^-1 (n = C(d,K), NU <= 21 for d <= 4),
-// times vol (K!)^2/((d+1)(d+2)). The integer maps are built once on the host.
+// Arithmetic. Every element entry is evaluated with the reference's own
+// operations in the reference's order, with FP contraction off, so the
+// matrices are bit-identical to the CPU restatement of that order and an entry
+// that the reference's arithmetic leaves at a roundoff residue (an
+// exactly-representable cell) is nonzero here too: the assembled sparsity
+// pattern (`push if v != 0.0`, galerkin.rs:173) is the reference's. Per cell:
+//   g       polarization, 0.5 * ((s0i + s0j) - sij);
+//   det g   the Leibniz sum over the permutations in colex order, from 0.0;
+//   g^-1    adjugate: (+-Leibniz cofactor) / det;
+//   L       Lambda^K g^-1, entry (I, J) the Leibniz determinant of the K x K
+//           minor of g^-1 on axis subsets I, J (colex);
+//   H       (P L) P^T with P = Lambda^K(Dlambda) in {0, +-1}: natural-order
+//           sums over the nonzero terms (a product by +-1 is exact);
+//   M       vol * sum_{p,q} ((a_p b_q) H[..]) Q[..] over the deletion pairs of
+//           the two DOFs, p-major, a_p = (-1)^p K!, Q = (1 + delta)/((d+1)(d+2));
+//   dif     M d^T, d M, d (M d^T): natural-order sums of +-1 multiples.
+// Products by 0 and +-1 change no bits and the sign of a zero never changes a
+// nonzero result, so the sums skip the zero terms and start at their first.
 //
-// Kernel shape: one wavefront per group of 64 cells, one lane per cell. The
-// lane computes g, det g and g^-1 in registers, then for each grade K the
-// unique K x K minors L of g^-1 in registers, and emits every block of that
-// grade as a dense contraction out_e = scale * sum_u C[e][u] L[u]. C is
-// wave-uniform: it is read with scalar loads and enters the FP64 FMAs as an
-// SGPR operand, so the inner loop touches neither LDS nor vector memory.
-// Element matrices are written structure-of-arrays,
-// out[entry * ncells + cell], so each store instruction of a wavefront covers
-// 64 consecutive doubles.
+// Kernel shape: one wavefront per group of 64 cells, one lane per cell; all
+// index tables are compile-time constants per (d, K), so the whole evaluation
+// is straight-line register code. Element matrices are written
+// structure-of-arrays, out[entry * ncells + cell], so each store instruction of
+// a wavefront covers 64 consecutive doubles.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -40,7 +48,111 @@ namespace feec {
 namespace {
 
 constexpr int TPB = 256;
+constexpr int WAVE = 64;
 
+constexpr double factorial(int n) { return n <= 1 ? 1.0 : n * factorial(n - 1); }
+
+// The permutations of {0..N-1} in Permutation::all's order: a word runs
+// through S_N lexicographically and the permutation is the word reversed.
+template <int N>
+struct Perms {
+  static constexpr int count = N <= 1 ? 1 : (N == 2 ? 2 : (N == 3 ? 6 : 24));
+  int p[count][N > 0 ? N : 1];
+  bool odd[count];
+  constexpr Perms() : p(), odd() {
+    int word[N > 0 ? N : 1] = {};
+    for (int i = 0; i < N; ++i) word[i] = i;
+    for (int c = 0; c < count; ++c) {
+      int inv = 0;
+      for (int i = 0; i < N; ++i) p[c][i] = word[N - 1 - i];
+      for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j)
+          if (p[c][i] > p[c][j]) ++inv;
+      odd[c] = (inv & 1) != 0;
+      // lexicographic successor of the word
+      int piv = -1;
+      for (int i = N - 2; i >= 0; --i)
+        if (word[i] < word[i + 1]) {
+          piv = i;
+          break;
+        }
+      if (piv < 0) break;
+      int succ = N - 1;
+      while (word[succ] <= word[piv]) --succ;
+      int t = word[piv];
+      word[piv] = word[succ];
+      word[succ] = t;
+      for (int i = piv + 1, j = N - 1; i < j; ++i, --j) {
+        t = word[i];
+        word[i] = word[j];
+        word[j] = t;
+      }
+    }
+  }
+};
+
+// Leibniz determinant, products folded left to right, terms summed in order.
+template <int N>
+__device__ __forceinline__ double leibniz(const double (&a)[N > 0 ? N : 1][N > 0 ? N : 1]) {
+#pragma clang fp contract(off)
+  if constexpr (N == 0) {
+    return 1.0;
+  } else {
+    constexpr Perms<N> P{};
+    double total = 0.0;
+#pragma unroll
+    for (int c = 0; c < Perms<N>::count; ++c) {
+      double prod = a[0][P.p[c][0]];
+#pragma unroll
+      for (int i = 1; i < N; ++i) prod = prod * a[i][P.p[c][i]];
+      if (c == 0)
+        total = P.odd[c] ? -prod : prod;
+      else
+        total = P.odd[c] ? total - prod : total + prod;
+    }
+    return total;
+  }
+}
+
+template <int D>
+struct CellGeom {
+  double g[D][D];
+  double ginv[D][D];
+  double det;
+};
+
+__host__ __device__ constexpr int edge_index(int i, int j) { return i + j * (j - 1) / 2; }
+
+template <int D>
+__device__ __forceinline__ void cell_geometry(const double* s, CellGeom<D>& cg) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int i = 0; i < D; ++i) cg.g[i][i] = s[edge_index(0, i + 1)];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = i + 1; j < D; ++j) {
+      const double v = 0.5 * ((s[edge_index(0, i + 1)] + s[edge_index(0, j + 1)]) -
+                              s[edge_index(i + 1, j + 1)]);
+      cg.g[i][j] = v;
+      cg.g[j][i] = v;
+    }
+  cg.det = leibniz<D>(cg.g);
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      double m[D > 1 ? D - 1 : 1][D > 1 ? D - 1 : 1];
+#pragma unroll
+      for (int a = 0; a < D - 1; ++a)
+#pragma unroll
+        for (int b = 0; b < D - 1; ++b) m[a][b] = cg.g[a < i ? a : a + 1][b < j ? b : b + 1];
+      const double cof = leibniz<D - 1>(m);
+      cg.ginv[j][i] = (((i + j) & 1) ? -cof : cof) / cg.det;
+    }
+}
+
+// Compile-time index tables of grade K on the d-simplex.
 constexpr int popc(unsigned m) {
   int c = 0;
   while (m) {
@@ -49,8 +161,16 @@ constexpr int popc(unsigned m) {
   }
   return c;
 }
-
-constexpr int colex_rank(unsigned m) {
+constexpr int nth_elem(unsigned m, int p) {
+  for (int e = 0; e < 32; ++e)
+    if (m & (1u << e)) {
+      if (p == 0) return e;
+      --p;
+    }
+  return -1;
+}
+// position of mask m among the size-popc(m) subsets of {0..n-1} in colex order
+constexpr int colex_pos(unsigned m) {
   int r = 0, i = 0;
   for (int e = 0; e < 32; ++e)
     if (m & (1u << e)) {
@@ -59,433 +179,489 @@ constexpr int colex_rank(unsigned m) {
     }
   return r;
 }
-
-constexpr double factorial(int n) { return n <= 1 ? 1.0 : n * factorial(n - 1); }
-
+// k-subsets of {0..n-1}, colex order = increasing mask
+constexpr unsigned nth_subset(int n, int k, int idx) {
+  for (unsigned m = 0; m < (1u << n); ++m)
+    if (popc(m) == k && idx-- == 0) return m;
+  return 0;
+}
+// integer determinant of a {0, +-1} K x K matrix (exact, any order)
 template <int K>
-__device__ __forceinline__ double det(const double (&a)[K > 0 ? K : 1][K > 0 ? K : 1]) {
+constexpr int int_det(const int (&a)[4][4]) {
   if constexpr (K == 0) {
-    return 1.0;
-  } else if constexpr (K == 1) {
-    return a[0][0];
-  } else if constexpr (K == 2) {
-    return a[0][0] * a[1][1] - a[0][1] * a[1][0];
-  } else if constexpr (K == 3) {
-    return a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
-           a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
-           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    return 1;
   } else {
-    static_assert(K == 4, "K <= 4");
-    // expansion along row 0 with 2x2 minors of rows 2,3 shared
-    double s01 = a[2][0] * a[3][1] - a[2][1] * a[3][0];
-    double s02 = a[2][0] * a[3][2] - a[2][2] * a[3][0];
-    double s03 = a[2][0] * a[3][3] - a[2][3] * a[3][0];
-    double s12 = a[2][1] * a[3][2] - a[2][2] * a[3][1];
-    double s13 = a[2][1] * a[3][3] - a[2][3] * a[3][1];
-    double s23 = a[2][2] * a[3][3] - a[2][3] * a[3][2];
-    double c0 = a[1][1] * s23 - a[1][2] * s13 + a[1][3] * s12;
-    double c1 = a[1][0] * s23 - a[1][2] * s03 + a[1][3] * s02;
-    double c2 = a[1][0] * s13 - a[1][1] * s03 + a[1][3] * s01;
-    double c3 = a[1][0] * s12 - a[1][1] * s02 + a[1][2] * s01;
-    return a[0][0] * c0 - a[0][1] * c1 + a[0][2] * c2 - a[0][3] * c3;
+    constexpr Perms<K> P{};
+    int t = 0;
+    for (int c = 0; c < Perms<K>::count; ++c) {
+      int prod = 1;
+      for (int i = 0; i < K; ++i) prod *= a[i][P.p[c][i]];
+      t += P.odd[c] ? -prod : prod;
+    }
+    return t;
   }
 }
 
-// Metric by polarization, det and inverse by the adjugate.
-template <int D>
-struct CellGeom {
-  double g[D][D];
-  double ginv[D][D];
-  double det;
+template <int D, int K>
+struct Grade {
+  static constexpr int NA = binom(D, K);          // Lambda^K g^-1 is NA x NA
+  static constexpr int NP = binom(D + 1, K);      // H is NP x NP (vertex K-subsets)
+  static constexpr int ND = binom(D + 1, K + 1);  // Whitney DOFs
+  int ax[NA][K > 0 ? K : 1];    // axis subsets
+  int P[NP][NA];                // Lambda^K(Dlambda)
+  int dv[ND][K + 1];            // vertex deleted at position p of DOF i
+  int dr[ND][K + 1];            // rank of DOF i without it (an H index)
+  int bd[NP][ND];               // local boundary d_K: faces x DOFs
+  constexpr Grade() : ax(), P(), dv(), dr(), bd() {
+    for (int I = 0; I < NA; ++I) {
+      unsigned m = nth_subset(D, K, I);
+      for (int a = 0; a < K; ++a) ax[I][a] = nth_elem(m, a);
+    }
+    for (int A = 0; A < NP; ++A) {
+      unsigned ma = nth_subset(D + 1, K, A);
+      for (int I = 0; I < NA; ++I) {
+        unsigned mi = nth_subset(D, K, I);
+        int minor[4][4] = {};
+        for (int a = 0; a < K; ++a)
+          for (int b = 0; b < K; ++b) {
+            const int row = nth_elem(ma, a), col = nth_elem(mi, b);
+            minor[a][b] = row == 0 ? -1 : (row - 1 == col ? 1 : 0);
+          }
+        P[A][I] = int_det<K>(minor);
+      }
+    }
+    for (int i = 0; i < ND; ++i) {
+      unsigned m = nth_subset(D + 1, K + 1, i);
+      for (int p = 0; p <= K; ++p) {
+        const int v = nth_elem(m, p);
+        dv[i][p] = v;
+        dr[i][p] = colex_pos(m & ~(1u << v));
+        if (K >= 1) bd[dr[i][p]][i] = (p & 1) ? -1 : 1;
+      }
+    }
+  }
 };
 
-__device__ __forceinline__ constexpr int edge_index(int i, int j) { return i + j * (j - 1) / 2; }
-
-template <int D>
-__device__ __forceinline__ void cell_geometry(const double* s, CellGeom<D>& cg) {
-#pragma unroll
-  for (int i = 0; i < D; ++i) cg.g[i][i] = s[edge_index(0, i + 1)];
-#pragma unroll
-  for (int i = 0; i < D; ++i)
-#pragma unroll
-    for (int j = i + 1; j < D; ++j) {
-      double v = 0.5 * (s[edge_index(0, i + 1)] + s[edge_index(0, j + 1)] -
-                        s[edge_index(i + 1, j + 1)]);
-      cg.g[i][j] = v;
-      cg.g[j][i] = v;
-    }
-  cg.det = det<D>(cg.g);
-  const double inv = 1.0 / cg.det;
-  if constexpr (D == 1) {
-    cg.ginv[0][0] = inv;
-  } else {
-#pragma unroll
-    for (int i = 0; i < D; ++i)
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        double m[D - 1][D - 1];
-#pragma unroll
-        for (int a = 0; a < D - 1; ++a)
-#pragma unroll
-          for (int b = 0; b < D - 1; ++b) m[a][b] = cg.g[a < i ? a : a + 1][b < j ? b : b + 1];
-        double cof = det<D - 1>(m);
-        cg.ginv[j][i] = (((i + j) & 1) ? -cof : cof) * inv;
-      }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// coefficient tables (host)
-// ---------------------------------------------------------------------------
 // Block kinds: 0 mass, 1 dif_trial, 2 dif_test, 3 dif_both, 4 diag(mass), 5 diag(dif_both)
 constexpr int KIND_DIAG_MASS = 4, KIND_DIAG_BOTH = 5;
+constexpr unsigned B_MASS = 1u << 0, B_TRIAL = 1u << 1, B_TEST = 1u << 2, B_BOTH = 1u << 3,
+                   B_DMASS = 1u << 4, B_DBOTH = 1u << 5;
 
-struct HostBlock {
-  int K = 0, kind = 0, rows = 0, cols = 0;
-  std::vector<int32_t> start;  // nout + 1
-  std::vector<uint16_t> h;
-  std::vector<int16_t> c;
+__host__ __device__ constexpr int block_rows(int d, int kind, int K) {
+  return (kind == 2 || kind == 3 || kind == KIND_DIAG_BOTH) ? binom(d + 1, K) : binom(d + 1, K + 1);
+}
+__host__ __device__ constexpr int block_cols(int d, int kind, int K) {
+  return (kind == KIND_DIAG_MASS || kind == KIND_DIAG_BOTH) ? 1
+         : (kind == 1 || kind == 3)                           ? binom(d + 1, K)
+                                                              : binom(d + 1, K + 1);
+}
+
+// Output arrays of one grade, by block kind (SoA [entry][ncells]).
+struct Outs {
+  double* p[6];
 };
 
-static std::vector<unsigned> masks_of(int n, int size) {
-  std::vector<unsigned> out;
-  if (size < 0 || size > n) return out;
-  for (unsigned m = 0; m < (1u << n); ++m)
-    if (popc(m) == size) out.push_back(m);
-  return out;
-}
-static int elem_of(unsigned m, int p) {
-  for (int e = 0; e < 32; ++e)
-    if (m & (1u << e)) {
-      if (p == 0) return e;
-      --p;
-    }
-  return -1;
-}
-// index of the symmetric pair (a, b), a <= b, row-major over the upper triangle
-static int sym_index(int a, int b, int nb) {
-  if (a > b) std::swap(a, b);
-  return a * nb - a * (a - 1) / 2 + (b - a);
-}
-
-using LinH = std::map<int, long>;  // unique Lambda^K g^-1 index -> integer coefficient
-
-// Entry (a, S) of P = Lambda^K(Dlambda), Dlambda row 0 = -1 and rows v >= 1 =
-// e_{v-1}: a is a vertex subset (bit mask), S a subset of the d axes.
-static int compound_dlambda(unsigned a, unsigned S) {
-  if (!(a & 1u)) return S == (a >> 1) ? 1 : 0;
-  const unsigned rest = (a & ~1u) >> 1;
-  if ((S & rest) != rest || popc(S) != popc(a)) return 0;
-  const unsigned s = S & ~rest;
-  if (popc(s) != 1) return 0;
-  return (popc(S & (s - 1u)) & 1) ? 1 : -1;
-}
-
-static HostBlock make_block(int D, int K, int kind) {
-  const int nv = D + 1;
-  auto blades = masks_of(nv, K);
-  auto dofs = masks_of(nv, K + 1);
-  auto axes = masks_of(D, K);
-  const int nb = (int)blades.size(), nd = (int)dofs.size(), na = (int)axes.size();
-  auto blade_idx = [&](unsigned m) {
-    for (int i = 0; i < nb; ++i)
-      if (blades[i] == m) return i;
-    return -1;
-  };
-  // H[a][b] = sum_{S,T} P[a][S] P[b][T] Lambda[S][T] (operators.rs:76, 84-94)
-  std::vector<LinH> Hl((size_t)nb * nb);
-  for (int a = 0; a < nb; ++a)
-    for (int b = 0; b < nb; ++b)
-      for (int S = 0; S < na; ++S) {
-        int pa = compound_dlambda(blades[a], axes[S]);
-        if (!pa) continue;
-        for (int T = 0; T < na; ++T) {
-          int pb = compound_dlambda(blades[b], axes[T]);
-          if (pb) Hl[(size_t)a * nb + b][sym_index(S, T, na)] += (long)pa * pb;
-        }
-      }
-  // Whitney mass M[s][t] as a combination of Lambda entries (form.rs:195-232)
-  std::vector<LinH> M((size_t)nd * nd);
-  for (int i = 0; i < nd; ++i)
-    for (int j = 0; j < nd; ++j) {
-      LinH acc;
-      for (int p = 0; p <= K; ++p)
-        for (int q = 0; q <= K; ++q) {
-          int va = elem_of(dofs[i], p), vb = elem_of(dofs[j], q);
-          int a = blade_idx(dofs[i] & ~(1u << va)), b = blade_idx(dofs[j] & ~(1u << vb));
-          long w = (((p + q) & 1) ? -1 : 1) * (va == vb ? 2 : 1);
-          for (auto& kv : Hl[(size_t)a * nb + b]) acc[kv.first] += w * kv.second;
-        }
-      M[(size_t)i * nd + j] = acc;
-    }
-  // local boundary d_K: rows = K-subsets (faces), cols = dofs (simplex.rs:223-238)
-  std::vector<int> bd((size_t)nb * nd, 0);
-  if (K >= 1)
-    for (int s = 0; s < nd; ++s)
-      for (int p = 0; p <= K; ++p) {
-        int f = blade_idx(dofs[s] & ~(1u << elem_of(dofs[s], p)));
-        bd[(size_t)f * nd + s] = (p & 1) ? -1 : 1;
-      }
-  auto add = [](LinH& dst, const LinH& src, long w) {
-    for (auto& kv : src) dst[kv.first] += w * kv.second;
-  };
-  HostBlock hb;
-  hb.K = K;
-  hb.kind = kind;
-  std::vector<LinH> out;
-  if (kind == 0 || kind == KIND_DIAG_MASS) {
-    hb.rows = nd;
-    hb.cols = nd;
-    out = M;
-  } else if (kind == 1) {  // M d^T
-    hb.rows = nd;
-    hb.cols = nb;
-    out.assign((size_t)nd * nb, {});
-    for (int s = 0; s < nd; ++s)
-      for (int f = 0; f < nb; ++f)
-        for (int t = 0; t < nd; ++t)
-          if (bd[(size_t)f * nd + t]) add(out[(size_t)s * nb + f], M[(size_t)s * nd + t], bd[(size_t)f * nd + t]);
-  } else if (kind == 2) {  // d M
-    hb.rows = nb;
-    hb.cols = nd;
-    out.assign((size_t)nb * nd, {});
-    for (int f = 0; f < nb; ++f)
-      for (int t = 0; t < nd; ++t)
-        for (int s = 0; s < nd; ++s)
-          if (bd[(size_t)f * nd + s]) add(out[(size_t)f * nd + t], M[(size_t)s * nd + t], bd[(size_t)f * nd + s]);
-  } else {  // d M d^T
-    hb.rows = nb;
-    hb.cols = nb;
-    out.assign((size_t)nb * nb, {});
-    for (int f = 0; f < nb; ++f)
-      for (int g = 0; g < nb; ++g)
-        for (int s = 0; s < nd; ++s)
-          for (int t = 0; t < nd; ++t) {
-            long w = (long)bd[(size_t)f * nd + s] * bd[(size_t)g * nd + t];
-            if (w) add(out[(size_t)f * nb + g], M[(size_t)s * nd + t], w);
-          }
-  }
-  std::vector<int> pick;
-  if (kind == KIND_DIAG_MASS || kind == KIND_DIAG_BOTH) {
-    for (int i = 0; i < hb.rows; ++i) pick.push_back(i * hb.cols + i);
-    hb.cols = 1;
-  } else {
-    for (int e = 0; e < hb.rows * hb.cols; ++e) pick.push_back(e);
-  }
-  hb.start.push_back(0);
-  for (int e : pick) {
-    for (auto& kv : out[e])
-      if (kv.second != 0) {
-        hb.h.push_back((uint16_t)kv.first);
-        hb.c.push_back((int16_t)kv.second);
-      }
-    hb.start.push_back((int32_t)hb.h.size());
-  }
-  return hb;
-}
-
-// Device-resident coefficient tables of one dimension: per (grade, kind)
-// block, a dense [nout][NU] table of integer coefficients (stored as double)
-// over the NU unique minors of that grade.
-struct DevTables {
-  DBuf<double> coef;
-  int nU[5] = {0, 0, 0, 0, 0};
-  int block_coef_off[5][6];
-  int block_nout[5][6];
-};
-
-static std::mutex g_tab_mu;
-static std::map<std::pair<Ctx*, int>, std::unique_ptr<DevTables>> g_tabs;
-
-}  // namespace
-
-void elmat_release_tables(Ctx* ctx) {
-  std::lock_guard<std::mutex> lock(g_tab_mu);
-  for (auto it = g_tabs.begin(); it != g_tabs.end();)
-    it = (it->first.first == ctx) ? g_tabs.erase(it) : std::next(it);
-}
-
-namespace {
-
-static DevTables& tables_for(Ctx* ctx, int D) {
-  std::lock_guard<std::mutex> lock(g_tab_mu);
-  auto key = std::make_pair(ctx, D);
-  auto it = g_tabs.find(key);
-  if (it != g_tabs.end()) return *it->second;
-  auto T = std::make_unique<DevTables>();
-  std::vector<double> coef;
-  const int nv = D + 1;
-  for (int K = 0; K <= D; ++K) {
-    const int na = binom(D, K);
-    const int nu = na * (na + 1) / 2;
-    T->nU[K] = nu;
-    for (int kind = 0; kind < 6; ++kind) {
-      T->block_nout[K][kind] = 0;
-      T->block_coef_off[K][kind] = (int)coef.size();
-      if (K == 0 && kind != 0 && kind != KIND_DIAG_MASS) continue;
-      HostBlock hb = make_block(D, K, kind);
-      const int nout = (int)hb.start.size() - 1;
-      T->block_nout[K][kind] = nout;
-      size_t base = coef.size();
-      coef.resize(base + (size_t)nout * nu, 0.0);
-      for (int e = 0; e < nout; ++e)
-        for (int t = hb.start[e]; t < hb.start[e + 1]; ++t)
-          coef[base + (size_t)e * nu + hb.h[t]] += (double)hb.c[t];
-    }
-  }
-  T->coef.alloc(ctx, std::max<size_t>(coef.size(), 1));
-  copy_to_device(ctx, T->coef.p, coef.data(), coef.size() * sizeof(double));
-  auto& slot = g_tabs[key];
-  slot = std::move(T);
-  return *slot;
-}
-
-// A launch request: the grades whose minors are needed and the blocks emitted.
-struct JobBlock {
-  int K;
-  int nout;
-  int coef_off;  // into the dense coefficient table of the dimension
-  double* out;   // SoA [nout][ncells]
-};
-struct Job {
-  int ngrades;
-  int grades[5];
-  int nblocks;
-  JobBlock b[12];
-};
-
-constexpr int WAVE = 64;
-
-// The K-subsets of {0..N-1} in colex order (= increasing bit masks).
-template <int N, int K>
-struct Blades {
-  static constexpr int count = binom(N, K);
-  int e[count > 0 ? count : 1][K > 0 ? K : 1];
-  constexpr Blades() : e() {
-    int i = 0;
-    for (unsigned m = 0; m < (1u << N); ++m) {
-      int c = 0;
-      for (int b = 0; b < N; ++b) c += (m >> b) & 1u;
-      if (c != K) continue;
-      int p = 0;
-      for (int b = 0; b < N; ++b)
-        if ((m >> b) & 1u) e[i][p++] = b;
-      ++i;
-    }
-  }
-};
-
-// Coefficients are read through the constant address space, so the compiler
-// issues wave-uniform scalar loads (s_load_dwordx16) and the FMAs take them as
-// SGPR operands.
-typedef const __attribute__((address_space(4))) double* ConstDouble;
-
-// Every block of grade K: the unique K x K minors L of g^-1 in registers,
-// then out_e = scale * sum_u C[e][u] L[u] with the dense integer table C.
+// The index tables of grade K for the runtime row loop, in constant memory:
+// the deletion pairs of every DOF and the values of P and of d_K as doubles.
 template <int D, int K>
-__device__ __forceinline__ void grade_blocks(const double (&gi)[D][D], double vol, const Job& job,
-                                             const double* __restrict__ coef, int64_t nc,
-                                             int64_t cell, bool active) {
-  constexpr Blades<D, K> B{};
-  constexpr int NB = Blades<D, K>::count;
-  constexpr int NU = NB * (NB + 1) / 2;
-  double H[NU];
+struct RowTables {
+  int dr[Grade<D, K>::ND][K + 1];
+  int dv[Grade<D, K>::ND][K + 1];
+  double P[Grade<D, K>::NP][Grade<D, K>::NA];
+  double bd[Grade<D, K>::NP][Grade<D, K>::ND];
+  constexpr RowTables() : dr(), dv(), P(), bd() {
+    constexpr Grade<D, K> T{};
+    for (int i = 0; i < Grade<D, K>::ND; ++i)
+      for (int p = 0; p <= K; ++p) {
+        dr[i][p] = T.dr[i][p];
+        dv[i][p] = T.dv[i][p];
+      }
+    for (int A = 0; A < Grade<D, K>::NP; ++A)
+      for (int I = 0; I < Grade<D, K>::NA; ++I) P[A][I] = T.P[A][I];
+    for (int f = 0; f < Grade<D, K>::NP; ++f)
+      for (int i = 0; i < Grade<D, K>::ND; ++i) bd[f][i] = T.bd[f][i];
+  }
+};
+template <int D, int K>
+__constant__ RowTables<D, K> kRows = RowTables<D, K>{};
+
+// first l with s[l] != 0 (N if none)
+template <int N>
+constexpr int first_nz(const int (&s)[N]) {
+  for (int l = 0; l < N; ++l)
+    if (s[l]) return l;
+  return N;
+}
+
+// sum_l x[l] * s[l] over s in {0, +-1}, in order of l, from the first nonzero term
+template <int N, class F>
+__device__ __forceinline__ double signed_sum(const int (&s)[N], F x) {
+#pragma clang fp contract(off)
+  double acc = 0.0;
+  bool first = true;
 #pragma unroll
-  for (int a = 0; a < NB; ++a)
+  for (int l = 0; l < N; ++l) {
+    if (s[l] == 0) continue;
+    const double v = x(l);
+    if (first)
+      acc = s[l] > 0 ? v : -v;
+    else
+      acc = s[l] > 0 ? acc + v : acc - v;
+    first = false;
+  }
+  return acc;
+}
+
+template <int K>
+__host__ __device__ constexpr double pair_weight(int p, int q) {
+  // a_p b_q = (+-K!)(+-K!), exact
+  return ((p & 1) ? -factorial(K) : factorial(K)) * ((q & 1) ? -factorial(K) : factorial(K));
+}
+
+__device__ __forceinline__ void put(double* __restrict__ out, int64_t entry, int64_t nc, int64_t cell,
+                                    bool active, double v) {
+  if (active) out[entry * nc + cell] = v;
+}
+
+// The blocks KINDS of grade K for one cell. M is produced one row at a time
+// (row i of the Whitney mass, all columns); rows are emitted as they come and
+// the d-products accumulate over them in the reference's summation order:
+//   dif_test  (d M)[f][j]      = sum_l d[f][l] M[l][j]   (l = row order)
+//   dif_trial (M d^T)[i][g]    = sum_t M[i][t] d[g][t]   (within the row)
+//   dif_both  (d (M d^T))[f][g] = sum_l d[f][l] (M d^T)[l][g]
+// Small grades (H at most 6 x 6) are fully unrolled with H in registers; the
+// 10 x 10 H of the 4-simplex's middle grades is streamed: a runtime loop over
+// rows computes, for each deletion p of the row's DOF in order, the one H row
+// it needs and adds its terms, so that only that row is live.
+template <int D, int K, unsigned KINDS>
+__device__ __forceinline__ void grade_blocks(const double (&gi)[D][D], double vol, const Outs& o,
+                                             int64_t nc, int64_t cell, bool active) {
+#pragma clang fp contract(off)
+  using G = Grade<D, K>;
+  constexpr G T{};
+  constexpr int NA = G::NA, NP = G::NP, ND = G::ND;
+  constexpr double qscale = 1.0 / (double)((D + 1) * (D + 2));
+  constexpr double q2 = 2.0 * qscale;
+  constexpr bool TEST = (KINDS & B_TEST) != 0;
+  constexpr bool BOTH = (KINDS & (B_BOTH | B_DBOTH)) != 0;
+  constexpr bool MD = (KINDS & (B_TRIAL | B_BOTH | B_DBOTH)) != 0;
+  static_assert(K >= 1 || (KINDS & ~(B_MASS | B_DMASS)) == 0, "grade 0 has no d blocks");
+  // Lambda^K g^-1
+  double L[NA][NA];
 #pragma unroll
-    for (int b = a; b < NB; ++b) {
+  for (int I = 0; I < NA; ++I)
+#pragma unroll
+    for (int J = 0; J < NA; ++J) {
       double m[K > 0 ? K : 1][K > 0 ? K : 1];
 #pragma unroll
-      for (int i = 0; i < K; ++i)
+      for (int a = 0; a < K; ++a)
 #pragma unroll
-        for (int j = 0; j < K; ++j) m[i][j] = gi[B.e[a][i]][B.e[b][j]];
-      H[a * NB - a * (a - 1) / 2 + (b - a)] = det<K>(m);
+        for (int b = 0; b < K; ++b) m[a][b] = gi[T.ax[I][a]][T.ax[J][b]];
+      L[I][J] = leibniz<K>(m);
     }
-  constexpr double kf = factorial(K);
-  const double scale = vol * (kf * kf / double((D + 1) * (D + 2)));
-  for (int bi = 0; bi < job.nblocks; ++bi) {
-    if (job.b[bi].K != K) continue;
-    ConstDouble c = (ConstDouble)(coef + job.b[bi].coef_off);
-    double* __restrict__ out = job.b[bi].out;
-    const int nout = job.b[bi].nout;
-    // small tables: two outputs per pass (independent FMA chains on the same
-    // H operands); large ones keep one row of coefficients in SGPRs
-    constexpr int PAIR = NU <= 28 ? 2 : 1;
-    int e = 0;
-    for (; PAIR == 2 && e + 1 < nout; e += 2, c += 2 * NU) {
-      double a0 = 0.0, a1 = 0.0;
+  double tacc[TEST ? NP : 1][TEST ? ND : 1];
+  double bacc[BOTH ? NP : 1][BOTH ? NP : 1];
+  constexpr bool STREAM = NP > 6;
+  if constexpr (!STREAM) {
+    // H = (P L) P^T
+    double H[NP][NP];
 #pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        a0 += c[u] * H[u];
-        a1 += c[NU + u] * H[u];
+    for (int A = 0; A < NP; ++A) {
+      double pl[NA];
+#pragma unroll
+      for (int J = 0; J < NA; ++J) pl[J] = signed_sum<NA>(T.P[A], [&](int I) { return L[I][J]; });
+#pragma unroll
+      for (int B = 0; B < NP; ++B) H[A][B] = signed_sum<NA>(T.P[B], [&](int J) { return pl[J]; });
+    }
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      double mrow[ND];
+#pragma unroll
+      for (int j = 0; j < ND; ++j) {
+        double sum = 0.0;
+#pragma unroll
+        for (int p = 0; p <= K; ++p)
+#pragma unroll
+          for (int q = 0; q <= K; ++q) {
+            const double t = (pair_weight<K>(p, q) * H[T.dr[i][p]][T.dr[j][q]]) *
+                             (T.dv[i][p] == T.dv[j][q] ? q2 : qscale);
+            sum = (p == 0 && q == 0) ? t : sum + t;
+          }
+        mrow[j] = vol * sum;
       }
-      if (active) {
-        out[(int64_t)e * nc + cell] = scale * a0;
-        out[(int64_t)(e + 1) * nc + cell] = scale * a1;
+      if constexpr ((KINDS & B_MASS) != 0) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) put(o.p[0], i * ND + j, nc, cell, active, mrow[j]);
+      }
+      if constexpr ((KINDS & B_DMASS) != 0) put(o.p[KIND_DIAG_MASS], i, nc, cell, active, mrow[i]);
+      if constexpr (TEST) {
+#pragma unroll
+        for (int f = 0; f < NP; ++f) {
+          if (T.bd[f][i] == 0) continue;
+#pragma unroll
+          for (int j = 0; j < ND; ++j) {
+            const double v = T.bd[f][i] > 0 ? mrow[j] : -mrow[j];
+            tacc[f][j] = (first_nz(T.bd[f]) == i) ? v : tacc[f][j] + v;
+          }
+        }
+      }
+      if constexpr (MD) {
+        double mdrow[NP];
+#pragma unroll
+        for (int g = 0; g < NP; ++g) mdrow[g] = signed_sum<ND>(T.bd[g], [&](int t) { return mrow[t]; });
+        if constexpr ((KINDS & B_TRIAL) != 0) {
+#pragma unroll
+          for (int g = 0; g < NP; ++g) put(o.p[1], i * NP + g, nc, cell, active, mdrow[g]);
+        }
+        if constexpr (BOTH) {
+#pragma unroll
+          for (int f = 0; f < NP; ++f) {
+            if (T.bd[f][i] == 0) continue;
+#pragma unroll
+            for (int g = 0; g < NP; ++g) {
+              if ((KINDS & B_BOTH) == 0 && g != f) continue;
+              const double v = T.bd[f][i] > 0 ? mdrow[g] : -mdrow[g];
+              bacc[f][g] = (first_nz(T.bd[f]) == i) ? v : bacc[f][g] + v;
+            }
+          }
+        }
       }
     }
-    for (; e < nout; ++e, c += NU) {
-      double a0 = 0.0;
+  } else {
+    const RowTables<D, K>& R = kRows<D, K>;
+    if constexpr (TEST) {
 #pragma unroll
-      for (int u = 0; u < NU; ++u) a0 += c[u] * H[u];
-      if (active) out[(int64_t)e * nc + cell] = scale * a0;
+      for (int f = 0; f < NP; ++f)
+#pragma unroll
+        for (int j = 0; j < ND; ++j) tacc[f][j] = 0.0;
     }
+    if constexpr (BOTH) {
+#pragma unroll
+      for (int f = 0; f < NP; ++f)
+#pragma unroll
+        for (int g = 0; g < NP; ++g) bacc[f][g] = 0.0;
+    }
+#pragma unroll 1
+    for (int i = 0; i < ND; ++i) {
+      double mrow[ND];
+#pragma unroll
+      for (int p = 0; p <= K; ++p) {
+        const int A = R.dr[i][p], va = R.dv[i][p];
+        // row A of P L: all NA terms from 0.0 (a product by 0 or +-1 is exact)
+        double pl[NA];
+#pragma unroll
+        for (int J = 0; J < NA; ++J) {
+          double acc = 0.0;
+#pragma unroll
+          for (int I = 0; I < NA; ++I) acc = acc + R.P[A][I] * L[I][J];
+          pl[J] = acc;
+        }
+        double h[NP];
+#pragma unroll
+        for (int B = 0; B < NP; ++B) h[B] = signed_sum<NA>(T.P[B], [&](int J) { return pl[J]; });
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+#pragma unroll
+          for (int q = 0; q <= K; ++q) {
+            const double t = (pair_weight<K>(p, q) * h[T.dr[j][q]]) * (va == T.dv[j][q] ? q2 : qscale);
+            mrow[j] = (p == 0 && q == 0) ? t : mrow[j] + t;
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < ND; ++j) mrow[j] = vol * mrow[j];
+      if constexpr ((KINDS & B_MASS) != 0) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) put(o.p[0], i * ND + j, nc, cell, active, mrow[j]);
+      }
+      if constexpr ((KINDS & B_DMASS) != 0) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+          if (j == i) put(o.p[KIND_DIAG_MASS], i, nc, cell, active, mrow[j]);
+      }
+      if constexpr (TEST) {
+#pragma unroll
+        for (int f = 0; f < NP; ++f) {
+          const double s = R.bd[f][i];
+#pragma unroll
+          for (int j = 0; j < ND; ++j) tacc[f][j] = tacc[f][j] + s * mrow[j];
+        }
+      }
+      if constexpr (MD) {
+        double mdrow[NP];
+#pragma unroll
+        for (int g = 0; g < NP; ++g) mdrow[g] = signed_sum<ND>(T.bd[g], [&](int t) { return mrow[t]; });
+        if constexpr ((KINDS & B_TRIAL) != 0) {
+#pragma unroll
+          for (int g = 0; g < NP; ++g) put(o.p[1], i * NP + g, nc, cell, active, mdrow[g]);
+        }
+        if constexpr (BOTH) {
+#pragma unroll
+          for (int f = 0; f < NP; ++f) {
+            const double s = R.bd[f][i];
+#pragma unroll
+            for (int g = 0; g < NP; ++g)
+              if ((KINDS & B_BOTH) != 0 || g == f) bacc[f][g] = bacc[f][g] + s * mdrow[g];
+          }
+        }
+      }
+    }
+  }
+  if constexpr (TEST) {
+#pragma unroll
+    for (int f = 0; f < NP; ++f)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) put(o.p[2], f * ND + j, nc, cell, active, tacc[f][j]);
+  }
+  if constexpr ((KINDS & B_BOTH) != 0) {
+#pragma unroll
+    for (int f = 0; f < NP; ++f)
+#pragma unroll
+      for (int g = 0; g < NP; ++g) put(o.p[3], f * NP + g, nc, cell, active, bacc[f][g]);
+  }
+  if constexpr ((KINDS & B_DBOTH) != 0) {
+#pragma unroll
+    for (int f = 0; f < NP; ++f) put(o.p[KIND_DIAG_BOTH], f, nc, cell, active, bacc[f][f]);
   }
 }
 
-// One wavefront, 64 consecutive cells starting at cell0, one lane per cell.
+// Per-cell set-up shared by every job: lengths, g, det g, g^-1, vol.
 template <int D>
-__device__ __forceinline__ void elmat_group(const int32_t* __restrict__ edges,
-                                            const double* __restrict__ len, int64_t nc,
-                                            int64_t cell0, const Job& job,
-                                            const double* __restrict__ coef, int* degenerate) {
-  constexpr int NE = binom(D + 1, 2);
-  const int64_t cell = cell0 + threadIdx.x;
-  const bool active = cell < nc;
-  double s[NE];
-#pragma unroll
-  for (int e = 0; e < NE; ++e) s[e] = active ? len[edges[cell * NE + e]] : ((e < D) ? 1.0 : 2.0);
+struct CellIn {
   CellGeom<D> cg;
-  cell_geometry<D>(s, cg);
-  if (active && cg.det == 0.0) atomicOr(degenerate, 1);
-  const double vol = (1.0 / factorial(D)) * sqrt(fabs(cg.det));
-  for (int gk = 0; gk < job.ngrades; ++gk) {
-    switch (job.grades[gk]) {
-      case 0: grade_blocks<D, 0>(cg.ginv, vol, job, coef, nc, cell, active); break;
-      case 1: grade_blocks<D, 1>(cg.ginv, vol, job, coef, nc, cell, active); break;
-      case 2: if constexpr (D >= 2) grade_blocks<D, 2>(cg.ginv, vol, job, coef, nc, cell, active); break;
-      case 3: if constexpr (D >= 3) grade_blocks<D, 3>(cg.ginv, vol, job, coef, nc, cell, active); break;
-      case 4: if constexpr (D >= 4) grade_blocks<D, 4>(cg.ginv, vol, job, coef, nc, cell, active); break;
-      default: break;
+  double vol;
+  bool active;
+  int64_t cell;
+};
+
+template <int D>
+__device__ __forceinline__ void cell_in(const int32_t* __restrict__ edges, const double* __restrict__ len,
+                                        int64_t nc, int64_t cell0, int* degenerate, CellIn<D>& c) {
+#pragma clang fp contract(off)
+  constexpr int NE = binom(D + 1, 2);
+  c.cell = cell0 + threadIdx.x;
+  c.active = c.cell < nc;
+  double s[NE];
+  // inactive lanes evaluate the unit right-angled simplex (nonsingular)
+#pragma unroll
+  for (int e = 0; e < NE; ++e) s[e] = c.active ? len[edges[c.cell * NE + e]] : ((e < D) ? 1.0 : 2.0);
+  cell_geometry<D>(s, c.cg);
+  if (c.active && c.cg.det == 0.0) atomicOr(degenerate, 1);
+  constexpr double unit_vol = 1.0 / factorial(D);
+  c.vol = unit_vol * sqrt(fabs(c.cg.det));
+}
+
+// Jobs. KKT of grade k: M_{k-1}; dif_test(k), M_k, diag(M_k), dif_both(k) and
+// its diagonal; dif_both(k+1). Dirac: M_k (all k) and dif_trial(k) (k >= 1).
+// Single: one block (K, kind) chosen at run time.
+struct KktOuts {
+  Outs g[3];  // grades k-1, k, k+1
+};
+struct DiracOuts {
+  Outs g[5];
+};
+
+template <int D, int k>
+__device__ __forceinline__ void kkt_cell(const CellIn<D>& c, const KktOuts& o, int64_t nc) {
+  if constexpr (k >= 1) grade_blocks<D, k - 1, B_MASS>(c.cg.ginv, c.vol, o.g[0], nc, c.cell, c.active);
+  if constexpr (k >= 1)
+    grade_blocks<D, k, B_MASS | B_DMASS | B_TEST | B_BOTH | B_DBOTH>(c.cg.ginv, c.vol, o.g[1], nc,
+                                                                     c.cell, c.active);
+  else
+    grade_blocks<D, k, B_MASS | B_DMASS>(c.cg.ginv, c.vol, o.g[1], nc, c.cell, c.active);
+  if constexpr (k + 1 <= D) grade_blocks<D, k + 1, B_BOTH>(c.cg.ginv, c.vol, o.g[2], nc, c.cell, c.active);
+}
+
+template <int D, int K = 0>
+__device__ __forceinline__ void dirac_cell(const CellIn<D>& c, const DiracOuts& o, int64_t nc) {
+  if constexpr (K <= D) {
+    if constexpr (K == 0)
+      grade_blocks<D, 0, B_MASS>(c.cg.ginv, c.vol, o.g[0], nc, c.cell, c.active);
+    else
+      grade_blocks<D, K, B_MASS | B_TRIAL>(c.cg.ginv, c.vol, o.g[K], nc, c.cell, c.active);
+    dirac_cell<D, K + 1>(c, o, nc);
+  }
+}
+
+template <int D, int K = 0>
+__device__ __forceinline__ void single_cell(const CellIn<D>& c, int grade, int kind, const Outs& o,
+                                            int64_t nc) {
+  if constexpr (K <= D) {
+    if (grade == K) {
+      switch (kind) {
+        case 0: grade_blocks<D, K, B_MASS>(c.cg.ginv, c.vol, o, nc, c.cell, c.active); return;
+        case KIND_DIAG_MASS: grade_blocks<D, K, B_DMASS>(c.cg.ginv, c.vol, o, nc, c.cell, c.active); return;
+        default: break;
+      }
+      if constexpr (K >= 1) {
+        switch (kind) {
+          case 1: grade_blocks<D, K, B_TRIAL>(c.cg.ginv, c.vol, o, nc, c.cell, c.active); return;
+          case 2: grade_blocks<D, K, B_TEST>(c.cg.ginv, c.vol, o, nc, c.cell, c.active); return;
+          case 3: grade_blocks<D, K, B_BOTH>(c.cg.ginv, c.vol, o, nc, c.cell, c.active); return;
+          case KIND_DIAG_BOTH: grade_blocks<D, K, B_DBOTH>(c.cg.ginv, c.vol, o, nc, c.cell, c.active); return;
+          default: break;
+        }
+      }
+      return;
     }
+    single_cell<D, K + 1>(c, grade, kind, o, nc);
   }
 }
 
 template <int D>
-__global__ void __launch_bounds__(WAVE) k_elmat(const int32_t* __restrict__ edges,
-                                                const double* __restrict__ len, int64_t nc, Job job,
-                                                const double* __restrict__ coef, int* degenerate) {
-  elmat_group<D>(edges, len, nc, blockIdx.x * (int64_t)WAVE, job, coef, degenerate);
+__global__ void __launch_bounds__(WAVE) k_elmat_single(const int32_t* __restrict__ edges,
+                                                       const double* __restrict__ len, int64_t nc,
+                                                       int grade, int kind, Outs o, int* degenerate) {
+  CellIn<D> c;
+  cell_in<D>(edges, len, nc, blockIdx.x * (int64_t)WAVE, degenerate, c);
+  single_cell<D>(c, grade, kind, o, nc);
 }
 
-// Many meshes of mixed dimension in one persistent launch: every wavefront
-// takes (mesh, 64-cell range) items from a global counter until the queue is
-// empty, so the grid drains whatever the mix of sizes.
+template <int D, int k>
+__global__ void __launch_bounds__(WAVE) k_elmat_kkt(const int32_t* __restrict__ edges,
+                                                    const double* __restrict__ len, int64_t nc,
+                                                    KktOuts o, int* degenerate) {
+  CellIn<D> c;
+  cell_in<D>(edges, len, nc, blockIdx.x * (int64_t)WAVE, degenerate, c);
+  kkt_cell<D, k>(c, o, nc);
+}
+
+template <int D>
+__global__ void __launch_bounds__(WAVE) k_elmat_dirac(const int32_t* __restrict__ edges,
+                                                      const double* __restrict__ len, int64_t nc,
+                                                      DiracOuts o, int* degenerate) {
+  CellIn<D> c;
+  cell_in<D>(edges, len, nc, blockIdx.x * (int64_t)WAVE, degenerate, c);
+  dirac_cell<D>(c, o, nc);
+}
+
+// Many meshes of mixed dimension and grade in one persistent launch: every
+// wavefront takes (mesh, 64-cell range) items from a global counter until the
+// queue is empty, so the grid drains whatever the mix of sizes.
 struct BatchMesh {
-  int D;
+  int D, k;
   int64_t nc;
   const int32_t* edges;
   const double* len;
   int64_t item0;  // first work item of this mesh
-  Job job;
+  KktOuts outs;
 };
+
+template <int D, int k = 0>
+__device__ __forceinline__ void batched_group(const BatchMesh& m, int64_t cell0, int* degenerate) {
+  if constexpr (k <= D) {
+    if (m.k == k) {
+      CellIn<D> c;
+      cell_in<D>(m.edges, m.len, m.nc, cell0, degenerate, c);
+      kkt_cell<D, k>(c, m.outs, m.nc);
+      return;
+    }
+    batched_group<D, k + 1>(m, cell0, degenerate);
+  }
+}
 
 __global__ void __launch_bounds__(WAVE) k_elmat_batched(const BatchMesh* __restrict__ meshes,
                                                         int nmeshes, int64_t nitems,
                                                         unsigned long long* counter,
-                                                        const double* __restrict__ coef2,
-                                                        const double* __restrict__ coef3,
-                                                        const double* __restrict__ coef4,
                                                         int* degenerate) {
   __shared__ long long item;
   for (;;) {
@@ -502,9 +678,9 @@ __global__ void __launch_bounds__(WAVE) k_elmat_batched(const BatchMesh* __restr
     const BatchMesh& m = meshes[lo];
     const int64_t cell0 = (it - m.item0) * WAVE;
     switch (m.D) {
-      case 2: elmat_group<2>(m.edges, m.len, m.nc, cell0, m.job, coef2, degenerate); break;
-      case 3: elmat_group<3>(m.edges, m.len, m.nc, cell0, m.job, coef3, degenerate); break;
-      case 4: elmat_group<4>(m.edges, m.len, m.nc, cell0, m.job, coef4, degenerate); break;
+      case 2: batched_group<2>(m, cell0, degenerate); break;
+      case 3: batched_group<3>(m, cell0, degenerate); break;
+      case 4: batched_group<4>(m, cell0, degenerate); break;
       default: break;
     }
   }
@@ -599,83 +775,76 @@ int element_cols(int d, int kind, int grade) {
 
 namespace {
 
-// Launch k_elmat for a job on the mesh of g.
-void run_job(Geom& g, Job& job, bool check) {
-  Mesh& m = *g.mesh;
-  Ctx* ctx = m.ctx;
-  const int D = m.d;
-  int64_t nc = m.n[D];
-  DevTables& T = tables_for(ctx, D);
-  int* deg = ctx->dflag;
-  FEEC_HIP(hipMemsetAsync(deg, 0, sizeof(int), ctx->stream));
-  unsigned grid = (unsigned)std::max<int64_t>(1, (nc + WAVE - 1) / WAVE);
-  const int32_t* edges = m.faces[1].p;
-  const double* cf = T.coef.p;
-  switch (D) {
-    case 1: k_elmat<1><<<grid, WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, job, cf, deg); break;
-    case 2: k_elmat<2><<<grid, WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, job, cf, deg); break;
-    case 3: k_elmat<3><<<grid, WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, job, cf, deg); break;
-    case 4: k_elmat<4><<<grid, WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, job, cf, deg); break;
-  }
-  FEEC_HIP(hipGetLastError());
-  if (check) check_degenerate(ctx, deg);
-}
+unsigned wave_grid(int64_t nc) { return (unsigned)std::max<int64_t>(1, (nc + WAVE - 1) / WAVE); }
 
-Job kkt_job(DevTables& T, int d, int k, double* const* outs);
+void reset_degenerate(Ctx* ctx) { FEEC_HIP(hipMemsetAsync(ctx->dflag, 0, sizeof(int), ctx->stream)); }
 
 // Autotuned persistent grid (in wavefronts) per device, chosen by timing.
 std::mutex g_tune_mu;
 std::map<int, unsigned> g_tuned_waves;
 
-void job_add_grade(Job& job, int K) {
-  for (int i = 0; i < job.ngrades; ++i)
-    if (job.grades[i] == K) return;
-  job.grades[job.ngrades++] = K;
+KktOuts kkt_outs(int k, double* const* outs) {
+  KktOuts o{};
+  if (k >= 1) o.g[0].p[0] = outs[0];
+  o.g[1].p[2] = outs[1];
+  o.g[1].p[KIND_DIAG_BOTH] = outs[3];
+  o.g[1].p[KIND_DIAG_MASS] = outs[4];
+  o.g[1].p[0] = outs[5];
+  o.g[1].p[3] = outs[6];
+  o.g[2].p[3] = outs[2];
+  return o;
 }
 
-void job_add_block(Job& job, DevTables& T, int K, int kind, double* out) {
-  FEEC_REQUIRE(job.nblocks < 12, FEEC_E_ARG, "too many element blocks in one launch");
-  job_add_grade(job, K);
-  JobBlock& b = job.b[job.nblocks++];
-  b.K = K;
-  b.nout = T.block_nout[K][kind];
-  b.coef_off = T.block_coef_off[K][kind];
-  b.out = out;
+template <int D, int k = 0>
+void launch_kkt(Ctx* ctx, const int32_t* edges, const double* len, int64_t nc, int grade,
+                const KktOuts& o) {
+  if constexpr (k <= D) {
+    if (grade == k) {
+      k_elmat_kkt<D, k><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, len, nc, o, ctx->dflag);
+      return;
+    }
+    launch_kkt<D, k + 1>(ctx, edges, len, nc, grade, o);
+  } else {
+    fail(FEEC_E_ARG, "grade out of range for a KKT system");
+  }
 }
 
 }  // namespace
 
 void element_matrices_soa(Geom& g, int kind, int grade, double* out_dev) {
   Mesh& m = *g.mesh;
-  DevTables& T = tables_for(m.ctx, m.d);
-  Job job{};
-  job_add_block(job, T, grade, kind, out_dev);
-  run_job(g, job, true);
-}
-
-namespace {
-Job kkt_job(DevTables& T, int d, int k, double* const* outs) {
-  Job job{};
-  if (k >= 1) {
-    job_add_block(job, T, k - 1, 0, outs[0]);
-    job_add_block(job, T, k, 2, outs[1]);
-    job_add_block(job, T, k, KIND_DIAG_BOTH, outs[3]);
+  Ctx* ctx = m.ctx;
+  const int64_t nc = m.n[m.d];
+  Outs o{};
+  o.p[kind] = out_dev;
+  reset_degenerate(ctx);
+  const int32_t* edges = m.faces[1].p;
+  switch (m.d) {
+    case 1: k_elmat_single<1><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, grade, kind, o, ctx->dflag); break;
+    case 2: k_elmat_single<2><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, grade, kind, o, ctx->dflag); break;
+    case 3: k_elmat_single<3><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, grade, kind, o, ctx->dflag); break;
+    case 4: k_elmat_single<4><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, grade, kind, o, ctx->dflag); break;
   }
-  job_add_block(job, T, k, KIND_DIAG_MASS, outs[4]);
-  if (k + 1 <= d) job_add_block(job, T, k + 1, 3, outs[2]);
-  // the two hdif_gram blocks of the preconditioner need M_k and dif_both(k) in full
-  job_add_block(job, T, k, 0, outs[5]);
-  if (k >= 1) job_add_block(job, T, k, 3, outs[6]);
-  return job;
+  FEEC_HIP(hipGetLastError());
+  check_degenerate(ctx, ctx->dflag);
 }
-}  // namespace
 
 // outs: M_{k-1}, dif_test(k), dif_both(k+1), diag(dif_both(k)), diag(M_k), M_k, dif_both(k)
 void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check) {
   Mesh& m = *g.mesh;
-  DevTables& T = tables_for(m.ctx, m.d);
-  Job job = kkt_job(T, m.d, grade, outs);
-  run_job(g, job, check);
+  Ctx* ctx = m.ctx;
+  const int64_t nc = m.n[m.d];
+  const KktOuts o = kkt_outs(grade, outs);
+  reset_degenerate(ctx);
+  const int32_t* edges = m.faces[1].p;
+  switch (m.d) {
+    case 1: launch_kkt<1>(ctx, edges, g.lengths.p, nc, grade, o); break;
+    case 2: launch_kkt<2>(ctx, edges, g.lengths.p, nc, grade, o); break;
+    case 3: launch_kkt<3>(ctx, edges, g.lengths.p, nc, grade, o); break;
+    case 4: launch_kkt<4>(ctx, edges, g.lengths.p, nc, grade, o); break;
+  }
+  FEEC_HIP(hipGetLastError());
+  if (check) check_degenerate(ctx, ctx->dflag);
 }
 
 // Element blocks of many KKT problems (dimensions 2..4) in one persistent
@@ -688,27 +857,25 @@ double kkt_element_blocks_batched(Ctx* ctx, const std::vector<BatchElementReques
   for (size_t i = 0; i < reqs.size(); ++i) {
     Mesh& m = *reqs[i].g->mesh;
     FEEC_REQUIRE(m.d >= 2 && m.d <= 4, FEEC_E_UNSUPPORTED, "batched assembly covers dimensions 2..4");
-    DevTables& T = tables_for(ctx, m.d);
+    FEEC_REQUIRE(reqs[i].grade >= 0 && reqs[i].grade <= m.d, FEEC_E_ARG, "grade out of range");
     BatchMesh& b = host[i];
     b.D = m.d;
+    b.k = reqs[i].grade;
     b.nc = m.n[m.d];
     b.edges = m.faces[1].p;
     b.len = reqs[i].g->lengths.p;
     b.item0 = items;
-    b.job = kkt_job(T, m.d, reqs[i].grade, reqs[i].outs);
+    b.outs = kkt_outs(reqs[i].grade, reqs[i].outs);
     items += (b.nc + WAVE - 1) / WAVE;
   }
-  DevTables& T2 = tables_for(ctx, 2);
-  DevTables& T3 = tables_for(ctx, 3);
-  DevTables& T4 = tables_for(ctx, 4);
   DBuf<BatchMesh> dm(ctx, host.size());
   FEEC_HIP(hipMemcpyAsync(dm.p, host.data(), host.size() * sizeof(BatchMesh), hipMemcpyHostToDevice,
                           ctx->stream));
-  FEEC_HIP(hipMemsetAsync(ctx->dflag, 0, sizeof(int), ctx->stream));
+  reset_degenerate(ctx);
   auto launch = [&](unsigned waves) {
     FEEC_HIP(hipMemsetAsync(ctx->dcount, 0, sizeof(unsigned long long), ctx->stream));
     k_elmat_batched<<<waves, WAVE, 0, ctx->stream>>>(dm.p, (int)host.size(), items, ctx->dcount,
-                                                     T2.coef.p, T3.coef.p, T4.coef.p, ctx->dflag);
+                                                     ctx->dflag);
     FEEC_HIP(hipGetLastError());
   };
   unsigned waves = 0;
@@ -763,20 +930,29 @@ int dirac_element_count(int d) {
 // (k = 1..d), back to back per cell in entry-major SoA order.
 void dirac_element_matrices(Geom& g, double* out_dev, bool check) {
   Mesh& m = *g.mesh;
+  Ctx* ctx = m.ctx;
   const int d = m.d;
   const int64_t nc = m.n[d];
-  DevTables& T = tables_for(m.ctx, d);
-  Job job{};
+  DiracOuts o{};
   int64_t off = 0;
   for (int k = 0; k <= d; ++k) {
-    job_add_block(job, T, k, 0, out_dev + off * nc);
-    off += T.block_nout[k][0];
+    o.g[k].p[0] = out_dev + off * nc;
+    off += block_rows(d, 0, k) * block_cols(d, 0, k);
     if (k >= 1) {
-      job_add_block(job, T, k, 1, out_dev + off * nc);
-      off += T.block_nout[k][1];
+      o.g[k].p[1] = out_dev + off * nc;
+      off += block_rows(d, 1, k) * block_cols(d, 1, k);
     }
   }
-  run_job(g, job, check);
+  reset_degenerate(ctx);
+  const int32_t* edges = m.faces[1].p;
+  switch (d) {
+    case 1: k_elmat_dirac<1><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, o, ctx->dflag); break;
+    case 2: k_elmat_dirac<2><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, o, ctx->dflag); break;
+    case 3: k_elmat_dirac<3><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, o, ctx->dflag); break;
+    case 4: k_elmat_dirac<4><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, o, ctx->dflag); break;
+  }
+  FEEC_HIP(hipGetLastError());
+  if (check) check_degenerate(ctx, ctx->dflag);
 }
 
 void cell_metrics(Geom& g, double* gmat, double* ginv, double* det) {

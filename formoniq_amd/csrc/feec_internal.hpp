@@ -193,6 +193,7 @@ struct Mesh {
   std::vector<int8_t> orientation;  // host, lazily
   bool have_orientation = false;
   bool orientable = false;
+  int64_t components = -1;  // connected components (vertices joined by cells), lazily
   // cached assembly plans keyed by (bc, row grade, col grade)
   std::map<std::tuple<int, int, int>, std::unique_ptr<CooPlan>> plans;
   std::mutex mu;
@@ -200,6 +201,7 @@ struct Mesh {
   int nfaces(int k) const { return binom(d + 1, k + 1); }
   void ensure_interior();
   void ensure_orientation();
+  void ensure_components();
 };
 
 struct Geom {
@@ -258,7 +260,6 @@ void element_matrices_soa(Geom& g, int kind, int grade, double* out_dev);
 int element_rows(int d, int kind, int grade);
 int element_cols(int d, int kind, int grade);
 void cell_metrics(Geom& g, double* gmat, double* ginv, double* det);
-void elmat_release_tables(Ctx* ctx);
 void kkt_element_blocks(Geom& g, int grade, double* const* outs, bool check);
 struct BatchElementRequest {
   Geom* g;

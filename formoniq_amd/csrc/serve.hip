@@ -414,6 +414,7 @@ feec_status feec_server_submit(feec_server* s, const feec_request* req, int64_t*
 feec_status feec_server_poll(feec_server* s, int64_t ticket, feec_result* result) {
   feec_status st = FEEC_OK;
   feec_status g = guard([&] {
+    FEEC_REQUIRE(s && result, FEEC_E_ARG, "null argument");
     std::lock_guard<std::mutex> lock(s->mu);
     auto it = s->entries.find(ticket);
     FEEC_REQUIRE(it != s->entries.end(), FEEC_E_ARG, "unknown ticket");
@@ -428,6 +429,7 @@ feec_status feec_server_poll(feec_server* s, int64_t ticket, feec_result* result
 
 feec_status feec_server_solution(feec_server* s, int64_t ticket, double* u, int64_t* n) {
   return guard([&] {
+    FEEC_REQUIRE(s, FEEC_E_ARG, "null server");
     std::lock_guard<std::mutex> lock(s->mu);
     auto it = s->entries.find(ticket);
     FEEC_REQUIRE(it != s->entries.end(), FEEC_E_ARG, "unknown ticket");
@@ -439,8 +441,31 @@ feec_status feec_server_solution(feec_server* s, int64_t ticket, double* u, int6
 
 feec_status feec_server_drain(feec_server* s) {
   return guard([&] {
+    FEEC_REQUIRE(s, FEEC_E_ARG, "null server");
     std::unique_lock<std::mutex> lock(s->mu);
     s->cv.wait(lock, [&] { return s->queue.empty() && s->running == 0; });
+  });
+}
+
+feec_status feec_server_release(feec_server* s, int64_t ticket) {
+  return guard([&] {
+    FEEC_REQUIRE(s, FEEC_E_ARG, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->entries.find(ticket);
+    FEEC_REQUIRE(it != s->entries.end(), FEEC_E_ARG, "unknown ticket");
+    FEEC_REQUIRE(it->second.state == 2, FEEC_E_AGAIN, "request not finished");
+    s->entries.erase(it);
+  });
+}
+
+feec_status feec_server_held(feec_server* s, int64_t* entries, int64_t* solution_doubles) {
+  return guard([&] {
+    FEEC_REQUIRE(s, FEEC_E_ARG, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    int64_t held = 0;
+    for (auto& kv : s->entries) held += (int64_t)kv.second.u.size();
+    if (entries) *entries = (int64_t)s->entries.size();
+    if (solution_doubles) *solution_doubles = held;
   });
 }
 

@@ -404,6 +404,40 @@ void Mesh::ensure_interior() {
 
 // Coherent orientation by propagation across shared facets: the two cells of an
 // interior facet induce opposite orientations on it.
+// Connected components of the vertex graph in which every cell joins its
+// vertices (union-find over the cells' vertex lists).
+void Mesh::ensure_components() {
+  std::lock_guard<std::mutex> lock(mu);
+  if (components >= 0) return;
+  const int64_t nc = n[d], nv = n[0];
+  const int nvert = d + 1;
+  std::vector<int32_t> cv((size_t)nc * nvert);
+  FEEC_HIP(hipMemcpyAsync(cv.data(), faces[0].p, cv.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  ctx->sync();
+  std::vector<int64_t> parent(nv);
+  for (int64_t v = 0; v < nv; ++v) parent[v] = v;
+  auto find = [&](int64_t v) {
+    while (parent[v] != v) {
+      parent[v] = parent[parent[v]];
+      v = parent[v];
+    }
+    return v;
+  };
+  int64_t comps = nv;
+  for (int64_t c = 0; c < nc; ++c) {
+    const int64_t r0 = find(cv[c * nvert]);
+    for (int i = 1; i < nvert; ++i) {
+      const int64_t r = find(cv[c * nvert + i]);
+      if (r != r0) {
+        parent[r] = r0;
+        --comps;
+      }
+    }
+  }
+  components = comps;
+}
+
 void Mesh::ensure_orientation() {
   std::lock_guard<std::mutex> lock(mu);
   if (have_orientation) return;

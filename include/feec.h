@@ -180,9 +180,11 @@ feec_status feec_causal_census(feec_geom *geom, int64_t counts[3], uint8_t *cell
 /* Row-major [N_d][rows][cols] in local colex face order. */
 feec_status feec_element_shape(int dim, feec_form_kind kind, int grade, int *rows, int *cols);
 feec_status feec_element_matrices(feec_geom *geom, feec_form_kind kind, int grade, double *out);
-/* The Galerkin matrix: pattern, numbering and signs bit-exact with the
- * reference (exact-zero element entries dropped, duplicates summed in cell
- * order, summed zeros kept); relative = E^T A E. */
+/* The Galerkin matrix: element entries evaluated with the reference's
+ * operations in its order, so pattern, numbering, signs and values are
+ * bit-exact with the CPU restatement of the reference (exact-zero element
+ * entries dropped, duplicates summed in cell order, summed zeros kept);
+ * relative = E^T A E. */
 feec_status feec_assemble(feec_geom *geom, feec_bc bc, feec_form_kind kind, int grade,
                           feec_csr **out);
 feec_status feec_csr_shape(const feec_csr *a, int64_t *nrows, int64_t *ncols, int64_t *nnz);
@@ -198,7 +200,10 @@ void feec_csr_destroy(feec_csr *a);
 /* ---- problems ------------------------------------------------------------ */
 /* b_k and an M_k-orthonormal harmonic basis [ndofs][count] (basis may be NULL).
  * Closed forms on a ball (full k=0: the constant; relative k=n: M_n^-1 times the
- * orientation); other harmonic spaces give FEEC_E_UNSUPPORTED. */
+ * orientation, b_k = 0 otherwise). A mesh that fails the ball test (one connected
+ * component, Euler characteristic 1) gives FEEC_E_UNSUPPORTED at every grade, here
+ * and in every entry point that needs the harmonic border (feec_mixed_kkt,
+ * feec_solve_source, feec_problem_create). */
 feec_status feec_harmonics(feec_geom *geom, feec_bc bc, int grade, int64_t *count, double *basis);
 /* The symmetric mixed KKT (elliptic.rs:54-114), with its harmonic border and
  * the sigma-row sign flip; sizes = {n_sigma, n_u, n_harmonic}. */
@@ -213,7 +218,8 @@ feec_status feec_solve_source(feec_geom *geom, feec_bc bc, int grade, const doub
 feec_status feec_hodge_dirac_assemble(feec_geom *geom, feec_bc bc, double sign,
                                       feec_csr **mass_block, feec_csr **op, int64_t *offsets);
 /* solve_dirac_source on the relative complex: (A + m M) u = f with lift;
- * load and lift are flat full-complex fields, out is the flat full field. */
+ * load and lift are flat full-complex fields (lift NULL: zero), out is the flat
+ * full field lift + E u_rel. Non-finite load, lift or mass_term: FEEC_E_ARG. */
 feec_status feec_solve_dirac_source(feec_geom *geom, double mass_term, const double *load,
                                     const double *lift, const feec_solve_opts *opts, double *out,
                                     feec_report *report);
@@ -354,6 +360,12 @@ feec_status feec_server_poll(feec_server *srv, int64_t ticket, feec_result *resu
 /* Copy the solution u (N_k) of a finished request; NULL-safe size query. */
 feec_status feec_server_solution(feec_server *srv, int64_t ticket, double *u, int64_t *n);
 feec_status feec_server_drain(feec_server *srv);
+/* Forget a finished request (its result and host copy of u): a long stream
+ * holds only what its caller has not released. FEEC_E_AGAIN while it runs. */
+feec_status feec_server_release(feec_server *srv, int64_t ticket);
+/* Requests the server holds (queued, running or finished and not released)
+ * and the doubles of solution held on the host for them. */
+feec_status feec_server_held(feec_server *srv, int64_t *entries, int64_t *solution_doubles);
 void feec_server_destroy(feec_server *srv);
 
 #ifdef __cplusplus

@@ -95,15 +95,19 @@ static double factorial(int n) {
   return f;
 }
 
-/* Leibniz determinant (multialgebra/src/lib.rs:447-456): sum over all
- * permutations of sign * prod_i a[i][sigma(i)]. Row-major n x n, n <= 4. */
+/* Leibniz determinant (multialgebra/src/lib.rs:447-456): the sum, from 0.0,
+ * of sign(sigma) * prod_i a[i][sigma(i)] (the product folded from 1.0), over
+ * the permutations in the order Permutation::all enumerates them
+ * (multiindex/src/permutation.rs:160-169): the lexicographic successor drives a
+ * word and the emitted permutation is the word reversed (colex order). For
+ * n >= 3 this order decides the rounding of the sum. Row-major n x n, n <= 4. */
 static double leibniz_det(const double *a, int n) {
   if (n == 0) return 1.0;
-  int perm[MAXD];
-  for (int i = 0; i < n; ++i) perm[i] = i;
+  int word[MAXD], perm[MAXD];
+  for (int i = 0; i < n; ++i) word[i] = i;
   double total = 0.0;
-  /* lexicographic enumeration of permutations */
   for (;;) {
+    for (int i = 0; i < n; ++i) perm[i] = word[n - 1 - i];
     int inv = 0;
     for (int i = 0; i < n; ++i)
       for (int j = i + 1; j < n; ++j)
@@ -113,20 +117,20 @@ static double leibniz_det(const double *a, int n) {
     total += (inv & 1) ? -prod : prod;
     int piv = -1;
     for (int i = n - 2; i >= 0; --i)
-      if (perm[i] < perm[i + 1]) {
+      if (word[i] < word[i + 1]) {
         piv = i;
         break;
       }
     if (piv < 0) break;
     int succ = n - 1;
-    while (perm[succ] <= perm[piv]) --succ;
-    int t = perm[piv];
-    perm[piv] = perm[succ];
-    perm[succ] = t;
+    while (word[succ] <= word[piv]) --succ;
+    int t = word[piv];
+    word[piv] = word[succ];
+    word[succ] = t;
     for (int i = piv + 1, j = n - 1; i < j; ++i, --j) {
-      t = perm[i];
-      perm[i] = perm[j];
-      perm[j] = t;
+      t = word[i];
+      word[i] = word[j];
+      word[j] = t;
     }
   }
   return total;
@@ -141,9 +145,13 @@ static double leibniz_det(const double *a, int n) {
 static int edge_index(int i, int j) { return i + j * (j - 1) / 2; }
 
 /* Polarization g_aa = s(0,a+1), g_ab = (s(0,a+1)+s(0,b+1)-s(a+1,b+1))/2
- * (regge/src/lengths/simplex.rs:308-326); det and inverse by the adjugate
- * (the reference calls nalgebra's determinant and try_inverse,
- * metric/src/lib.rs:179-190, 250-257). */
+ * (regge/src/lengths/simplex.rs:308-326). det g and g^-1: the reference calls
+ * nalgebra 0.35's determinant and try_inverse (metric/src/lib.rs:179-190,
+ * 250-257), closed forms up to 3x3 (4x4: LU / cofactor expansion) whose
+ * rounding is not vendored in the reference. Here: the Leibniz sum above and
+ * the adjugate, cofactor / det. Parity with nalgebra is unpinned in the last
+ * bit; on every cell whose g has small-integer entries times a power of two
+ * (the unjittered Kuhn cells) every step is exact under either algorithm. */
 void orc_cell_metric(int d, const double *s, double *g, double *ginv, double *det) {
   for (int i = 0; i < d; ++i) g[i * d + i] = s[edge_index(0, i + 1)];
   for (int i = 0; i < d; ++i)

@@ -1,8 +1,11 @@
 """GPU parity of topology, element matrices and assembly against the oracle.
 
-Numbering, patterns, orientation signs and boundary flags are compared bit for
-bit; element and Galerkin matrix values within 1e-12 of the matrix maximum
-(the tolerance required of FP64 element and global matrices)."""
+Everything is compared bit for bit: numbering, orientation signs, boundary
+flags, cell metrics, element matrices, and the sparsity patterns and values of
+the Galerkin matrices. The element kernel evaluates each entry with the
+reference's operations in the reference's order (FP contraction off), so an
+entry the reference's arithmetic leaves at a roundoff residue on an
+exactly-representable cell is kept exactly where the oracle keeps it."""
 import numpy as np
 import pytest
 
@@ -31,7 +34,7 @@ def test_numbering_bit_exact(d, n, amp):
             oc = ocx.coboundary(k)
             if bc == fa.RELATIVE:
                 oc = oc.select(ocx.boundary_flags(k + 1) == 0, ocx.boundary_flags(k) == 0)
-            assert_csr_parity(g, oc, 0.0, f"D_{k} bc={bc}", exact_pattern=True)
+            assert_csr_parity(g, oc, 0.0, f"D_{k} bc={bc}")
     np.testing.assert_array_equal(geo.lengths(), oL)
 
 
@@ -43,11 +46,11 @@ def test_numbering_c1_config_size():
     np.testing.assert_array_equal(geo.lengths(), oL)
 
 
-@pytest.mark.parametrize("d,n,amp", MESHES)
+@pytest.mark.parametrize("d,n,amp", MESHES + [(2, 3, 0.0), (3, 2, 0.0), (4, 1, 0.0)])
 def test_element_matrices(d, n, amp):
+    """Every block of every grade, bit-identical to the oracle's."""
     ocx, oL, gcx, geo, _, _ = both(d, n, amp, seed=d)
     wc = fa.WhitneyComplex(geo)
-    worst = 0.0
     for k in range(d + 1):
         for kind in range(4):
             if k == 0 and kind != fa.MASS:
@@ -55,14 +58,13 @@ def test_element_matrices(d, n, amp):
             g = wc.element(kind, k)
             e = o.element_matrices(ocx, oL, kind, k)
             assert g.shape == e.shape
-            per_cell = np.abs(g - e).max(axis=(1, 2)) / np.maximum(np.abs(e).max(axis=(1, 2)), 1e-300)
-            worst = max(worst, per_cell.max())
-            assert per_cell.max() <= 1e-12, (d, k, kind, per_cell.max())
-    print(f"d={d} worst element rel err {worst:.2e}")
+            bad = np.argwhere(g != e)
+            assert bad.size == 0, (d, k, kind, len(bad), bad[:3], g[tuple(bad[0])], e[tuple(bad[0])])
 
 
 def test_element_matrices_lorentzian():
-    """4D Minkowski box, time axis unjittered: null and timelike edges present."""
+    """4D Minkowski box, time axis unjittered: null and timelike edges present.
+    Metric, inverse, determinant and all blocks bit-identical to the oracle."""
     ocx, oL, gcx, geo, _, _ = both(4, 2, 0.02, seed=3, minkowski=True, axes=[1, 2, 3])
     counts, riem = geo.causal_census()
     assert counts[0] > 0 and counts[1] > 0 and counts[2] > 0
@@ -75,45 +77,45 @@ def test_element_matrices_lorentzian():
                 continue
             g = wc.element(kind, k)
             e = o.element_matrices(ocx, oL, kind, k)
-            per_cell = np.abs(g - e).max(axis=(1, 2)) / np.maximum(np.abs(e).max(axis=(1, 2)), 1e-300)
-            assert per_cell.max() <= 1e-12, (k, kind, per_cell.max())
+            assert np.array_equal(g, e), (k, kind, int((g != e).sum()))
     gm, gi, det = geo.cell_metrics()
-    for c in range(0, len(det), 7):
-        s = oL[ocx.cell_faces(1)[c]]
-        og, ogi, odet = o.cell_metric(4, s)
-        np.testing.assert_allclose(gm[c], og, rtol=0, atol=1e-15)
-        assert abs(det[c] - odet) <= 1e-12 * abs(odet)
-        assert rel_max(gi[c], ogi) <= 1e-12
+    faces = ocx.cell_faces(1)
+    for c in range(len(det)):
+        og, ogi, odet = o.cell_metric(4, oL[faces[c]])
+        assert np.array_equal(gm[c], og) and np.array_equal(gi[c], ogi) and det[c] == odet, c
 
 
-@pytest.mark.parametrize("d,n,amp", MESHES + [(2, 4, 0.0), (3, 2, 0.0)])
+@pytest.mark.parametrize("d,n,amp", MESHES + [(2, 4, 0.0), (3, 2, 0.0), (4, 2, 0.0), (3, 4, 0.05)])
 def test_galerkin_matrices(d, n, amp):
+    """Every Galerkin matrix of the full and the relative complex: pattern and
+    values bit-identical. The unjittered grids and the boundary of a jittered
+    one have exactly-representable cells."""
     ocx, oL, gcx, geo, _, _ = both(d, n, amp, seed=d + 10)
     for rel in (False, True):
         oc = o.WhitneyComplex(ocx, oL, relative=rel)
         gc = fa.WhitneyComplex(geo, relative=rel)
         for k in range(d + 1):
-            assert_csr_parity(gc.mass(k), oc.mass(k), 1e-12, f"mass {k} rel={rel}")
+            assert_csr_parity(gc.mass(k), oc.mass(k), 0.0, f"mass {k} rel={rel}")
             if k >= 1:
-                assert_csr_parity(gc.dif_trial(k), oc.dif_trial(k), 1e-12, f"dif_trial {k} rel={rel}")
-                assert_csr_parity(gc.dif_test(k), oc.dif_test(k), 1e-12, f"dif_test {k} rel={rel}")
-                assert_csr_parity(gc.dif_both(k), oc.dif_both(k), 1e-12, f"dif_both {k} rel={rel}")
-            assert_csr_parity(gc.hdif_gram(k), oc.hdif_gram(k), 1e-12, f"hdif_gram {k} rel={rel}")
+                assert_csr_parity(gc.dif_trial(k), oc.dif_trial(k), 0.0, f"dif_trial {k} rel={rel}")
+                assert_csr_parity(gc.dif_test(k), oc.dif_test(k), 0.0, f"dif_test {k} rel={rel}")
+                assert_csr_parity(gc.dif_both(k), oc.dif_both(k), 0.0, f"dif_both {k} rel={rel}")
+            assert_csr_parity(gc.hdif_gram(k), oc.hdif_gram(k), 0.0, f"hdif_gram {k} rel={rel}")
 
 
 @pytest.mark.parametrize("d,n", [(2, 6), (3, 3), (4, 2)])
-def test_galerkin_patterns_bit_exact_when_no_cell_is_exact(d, n):
-    """Every vertex perturbed, boundary included: no element entry is zero
-    by cancellation, so every pattern must match the oracle exactly."""
+def test_galerkin_patterns_when_no_cell_is_exact(d, n):
+    """Every vertex perturbed, boundary included: no element entry is zero by
+    cancellation."""
     ocx, oL, gcx, geo, _, _ = both(d, n, 0.0, seed=d, perturb_all=0.1)
     for rel in (False, True):
         oc = o.WhitneyComplex(ocx, oL, relative=rel)
         gc = fa.WhitneyComplex(geo, relative=rel)
         for k in range(d + 1):
-            assert_csr_parity(gc.mass(k), oc.mass(k), 1e-12, f"mass {k}", exact_pattern=True)
+            assert_csr_parity(gc.mass(k), oc.mass(k), 0.0, f"mass {k}")
             if k >= 1:
-                assert_csr_parity(gc.dif_test(k), oc.dif_test(k), 1e-12, f"dif_test {k}", exact_pattern=True)
-                assert_csr_parity(gc.dif_both(k), oc.dif_both(k), 1e-12, f"dif_both {k}", exact_pattern=True)
+                assert_csr_parity(gc.dif_test(k), oc.dif_test(k), 0.0, f"dif_test {k}")
+                assert_csr_parity(gc.dif_both(k), oc.dif_both(k), 0.0, f"dif_both {k}")
 
 
 def test_exact_zero_drop_unjittered():
@@ -121,7 +123,25 @@ def test_exact_zero_drop_unjittered():
     entries; the GPU pattern drops exactly the entries the oracle drops."""
     ocx, oL, gcx, geo, _, _ = both(2, 2)
     assert (o.element_matrices(ocx, oL, o.MASS, 1) == 0).any()
-    assert_csr_parity(fa.WhitneyComplex(geo).mass(1), o.assemble(ocx, oL, o.MASS, 1), 1e-12, "mass1")
+    assert_csr_parity(fa.WhitneyComplex(geo).mass(1), o.assemble(ocx, oL, o.MASS, 1), 0.0, "mass1")
+
+
+def test_roundoff_residue_entries_kept():
+    """A 3D 8^3 Kuhn mesh jittered like the 64^3 configuration (interior
+    vertices only): the cells along the cube's edges are exactly
+    representable, and the reference's arithmetic leaves entries there that
+    are zero in exact arithmetic but a roundoff residue in floating point.
+    The oracle keeps them (v != 0.0); so must the GPU, with the same bits."""
+    ocx, oL, gcx, geo, _, _ = both(3, 8, 0.05, seed=2)
+    oc = o.WhitneyComplex(ocx, oL)
+    gc = fa.WhitneyComplex(geo)
+    residues = 0
+    for k, m in ((1, oc.mass(1)), (2, oc.dif_both(2))):
+        residues += int((np.abs(m.val) <= 1e-14 * np.abs(m.val).max()).sum())
+    assert residues > 0
+    assert_csr_parity(gc.mass(1), oc.mass(1), 0.0, "mass1")
+    assert_csr_parity(gc.dif_both(2), oc.dif_both(2), 0.0, "dif_both2")
+    assert_csr_parity(gc.dif_test(1), oc.dif_test(1), 0.0, "dif_test1")
 
 
 def test_fem3d_and_fdm_on_gpu(golden):

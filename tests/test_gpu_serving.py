@@ -127,3 +127,28 @@ def test_server_admission_budget():
     srv.drain()
     assert srv.poll(t).status == 0
     srv.close()
+
+
+def test_server_release_bounds_held_state():
+    """A stream of small requests whose results are read and released holds
+    nothing afterwards: finished entries (result and host copy of u) are
+    erased, so a long continuous-batching stream does not grow host memory."""
+    opts = fa.ServerOpts()
+    opts.max_inflight = 2
+    opts.solve = fa.solve_opts()
+    srv = fa.Server(opts)
+    peak = 0
+    for wave in range(3):
+        tickets = [srv.submit(req(2, 1, 6, 100 + 4 * wave + i)) for i in range(4)]
+        srv.drain()
+        peak = max(peak, srv.held()[0])
+        for t in tickets:
+            assert srv.poll(t).status == 0
+            assert srv.solution(t).size > 0
+            srv.release(t)
+            with pytest.raises(fa.FeecError) as e:
+                srv.poll(t)
+            assert e.value.name == "FEEC_E_ARG"
+        assert srv.held() == (0, 0)
+    assert peak == 4
+    srv.close()
