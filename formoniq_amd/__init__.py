@@ -85,6 +85,12 @@ class Context:
         check(load().feec_ctx_memory(self._h, C.byref(b)))
         return b.value
 
+    def allocations(self) -> int:
+        """Device allocations made by this context so far (feec_ctx_allocations)."""
+        c = C.c_int64()
+        check(load().feec_ctx_allocations(self._h, C.byref(c)))
+        return c.value
+
 
 _DEFAULT_CTX = None
 
@@ -194,6 +200,18 @@ class Csr:
         rep = Report()
         st = load().feec_csr_cg(self._h, _ptr(b, _lib.f64p), _ptr(x, _lib.f64p), float(rtol),
                                 int(max_iters), C.byref(rep))
+        if raise_on_failure:
+            check(st)
+        return x, rep
+
+    def cg_noalloc(self, b, rtol=1e-10, max_iters=10_000, raise_on_failure=True):
+        """cg() on the library's fixed device scratch (feec_csr_cg_noalloc):
+        no device allocation, at most 262144 rows."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.shape[0])
+        rep = Report()
+        st = load().feec_csr_cg_noalloc(self._h, _ptr(b, _lib.f64p), _ptr(x, _lib.f64p), float(rtol),
+                                        int(max_iters), C.byref(rep))
         if raise_on_failure:
             check(st)
         return x, rep

@@ -69,6 +69,7 @@ SIGNATURES = {
     "feec_ctx_destroy": (None, [vp]),
     "feec_ctx_synchronize": (S, [vp]),
     "feec_ctx_memory": (S, [vp, i64p]),
+    "feec_ctx_allocations": (S, [vp, i64p]),
     "feec_kuhn_counts": (S, [C.c_int, C.c_int, i64p, i64p]),
     "feec_kuhn_grid": (S, [C.c_int, C.c_int, i64p, f64p]),
     "feec_jitter_interior": (S, [C.c_int, C.c_int, f64p, C.c_double, C.c_uint64, C.c_uint32]),
@@ -95,6 +96,7 @@ SIGNATURES = {
     "feec_csr_copy": (S, [vp, i64p, i32p, f64p]),
     "feec_csr_spmv": (S, [vp, f64p, f64p]),
     "feec_csr_cg": (S, [vp, f64p, f64p, C.c_double, C.c_int64, C.POINTER(Report)]),
+    "feec_csr_cg_noalloc": (S, [vp, f64p, f64p, C.c_double, C.c_int64, C.POINTER(Report)]),
     "feec_csr_destroy": (None, [vp]),
     "feec_harmonics": (S, [vp, C.c_int, C.c_int, i64p, f64p]),
     "feec_mixed_kkt": (S, [vp, C.c_int, C.c_int, C.POINTER(vp), i64p]),

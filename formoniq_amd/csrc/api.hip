@@ -78,6 +78,7 @@ void* device_alloc(Ctx* ctx, size_t bytes) {
     }
   }
   ctx->bytes += (int64_t)cls;
+  ++ctx->allocs;
   return p;
 }
 
@@ -638,6 +639,7 @@ feec_status feec_ctx_create(int device, feec_ctx** out) {
     try {
       c->dflag = static_cast<int*>(device_alloc(c, sizeof(int)));
       c->dcount = static_cast<unsigned long long*>(device_alloc(c, sizeof(unsigned long long)));
+      noalloc_scratch_init(device);
     } catch (...) {
       feec_ctx_destroy(c);
       throw;
@@ -666,6 +668,13 @@ feec_status feec_ctx_synchronize(feec_ctx* ctx) {
 
 feec_status feec_ctx_memory(const feec_ctx* ctx, int64_t* bytes) {
   return guard([&] { *bytes = ctx->bytes.load(); });
+}
+
+feec_status feec_ctx_allocations(const feec_ctx* ctx, int64_t* count) {
+  return guard([&] {
+    FEEC_REQUIRE(ctx && count, FEEC_E_ARG, "null argument");
+    *count = ctx->allocs.load();
+  });
 }
 
 // ---- meshers --------------------------------------------------------------

@@ -131,6 +131,7 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::atomic<int64_t> bytes{0};
+  std::atomic<int64_t> allocs{0};  // device_alloc calls (cache hits included)
   // rocPRIM temporary storage, grown on demand
   void* temp = nullptr;
   size_t temp_bytes = 0;
@@ -356,6 +357,9 @@ double time_cheb_step(KktSystem& s, bool fp32, int reps, int storage);
 // with its scalars on the device (host read every 8 iterations).
 KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
                             double rtol, int64_t max_iters);
+// Sets up (once per device, that device current) the address of the scratch
+// array of feec_csr_cg_noalloc on the current device.
+void noalloc_scratch_init(int device);
 // a . b (part: 2 * 4096 doubles of scratch) and x += s a, on device doubles
 double dot64(Ctx* ctx, const double* a, const double* b, int64_t n, double* part);
 void axpy64(Ctx* ctx, const double* a, int64_t n, double s, double* x);

@@ -2895,8 +2895,16 @@ __global__ void k_cg_dir(double* __restrict__ p, const double* __restrict__ z, i
 
 // Jacobi CG (krylov.rs:48-95: stop when |r| <= rtol |b|) with its scalars on
 // the device: five launches per iteration, one host read per CG_SEG iterations.
-KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
-                            double rtol, int64_t max_iters) {
+namespace {
+
+// Device buffers of one CG run.
+struct CgWork {
+  double *r, *z, *p, *ap, *part;
+  CgS* S;
+};
+
+KrylovStats cg_solve_ws(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
+                        double rtol, int64_t max_iters, const CgWork& w) {
   const int64_t n = a.nrows;
   hipStream_t st = ctx->stream;
   KrylovStats ks;
@@ -2905,22 +2913,20 @@ KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const do
     ks.precond_residual = 0.0;
     return ks;
   }
-  DBuf<double> r(ctx, n), z(ctx, n), p(ctx, n), ap(ctx, n), part(ctx, 2 * NBLK);
-  DBuf<CgS> S(ctx, 1);
   const unsigned gr = red_grid(n), gs = spmv_grid(n);
-  k_cg_init<<<gr, TPB, 0, st>>>(b, dinv, n, x, r.p, z.p, p.p, part.p);
-  k_cg_start<<<1, TPB, 0, st>>>(S.p, part.p, (int)gr, rtol, (long long)max_iters);
+  k_cg_init<<<gr, TPB, 0, st>>>(b, dinv, n, x, w.r, w.z, w.p, w.part);
+  k_cg_start<<<1, TPB, 0, st>>>(w.S, w.part, (int)gr, rtol, (long long)max_iters);
   CgS h{};
   while (true) {
     for (int it = 0; it < CG_SEG; ++it) {
-      k_cg_spmv<<<gs, TPB, 0, st>>>(a.row_ptr.p, a.col.p, a.val.p, n, p.p, ap.p, S.p, part.p);
-      k_cg_alpha<<<1, TPB, 0, st>>>(S.p, part.p, (int)gs);
-      k_cg_update<<<gr, TPB, 0, st>>>(x, r.p, z.p, p.p, ap.p, dinv, n, S.p, part.p);
-      k_cg_beta<<<1, TPB, 0, st>>>(S.p, part.p, (int)gr);
-      k_cg_dir<<<gr, TPB, 0, st>>>(p.p, z.p, n, S.p);
+      k_cg_spmv<<<gs, TPB, 0, st>>>(a.row_ptr.p, a.col.p, a.val.p, n, w.p, w.ap, w.S, w.part);
+      k_cg_alpha<<<1, TPB, 0, st>>>(w.S, w.part, (int)gs);
+      k_cg_update<<<gr, TPB, 0, st>>>(x, w.r, w.z, w.p, w.ap, dinv, n, w.S, w.part);
+      k_cg_beta<<<1, TPB, 0, st>>>(w.S, w.part, (int)gr);
+      k_cg_dir<<<gr, TPB, 0, st>>>(w.p, w.z, n, w.S);
     }
     FEEC_HIP(hipGetLastError());
-    copy_to_host(ctx, &h, S.p, sizeof(h));
+    copy_to_host(ctx, &h, w.S, sizeof(h));
     if (h.done) break;
   }
   ks.iters = h.iter;
@@ -2929,4 +2935,97 @@ KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const do
   return ks;
 }
 
+// ---------------------------------------------------------------------------
+// feec_csr_cg_noalloc's scratch: a file-scope device array, one copy per GPU
+// (each at its own address), held until the process ends. Layout, in doubles:
+// the CG scalars (padded to 256 B), 2 * NBLK partial sums, then seven vectors
+// of kNoAllocMaxRows: b, x (staging), the inverse diagonal, r, z, p, A p.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMaxRows = int64_t(1) << 18;
+constexpr int64_t kScalarDoubles = 32;
+static_assert(sizeof(CgS) <= kScalarDoubles * sizeof(double), "CG scalars fit their slot");
+constexpr int64_t kNoAllocDoubles = kScalarDoubles + 2 * NBLK + 7 * kNoAllocMaxRows;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;  // device -> that device's copy of `scratch`
+
+double* scratch_for(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  return it == g_scratch_ptr.end() ? nullptr : it->second;
+}
+
+__global__ void k_reciprocal_or_one(const double* d, int64_t n, double* out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = d[i] != 0.0 ? 1.0 / d[i] : 1.0;
+}
+
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr.count(device)) return;
+  void* p = nullptr;
+  FEEC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  FEEC_REQUIRE(p, FEEC_E_DEVICE, "no device address for the scratch array");
+  g_scratch_ptr[device] = static_cast<double*>(p);
+}
+
+KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
+                            double rtol, int64_t max_iters) {
+  const int64_t n = std::max<int64_t>(a.nrows, 1);
+  DBuf<double> r(ctx, n), z(ctx, n), p(ctx, n), ap(ctx, n), part(ctx, 2 * NBLK);
+  DBuf<CgS> S(ctx, 1);
+  return cg_solve_ws(ctx, a, dinv, b, x, rtol, max_iters, CgWork{r.p, z.p, p.p, ap.p, part.p, S.p});
+}
+
 }  // namespace feec
+
+using namespace feec;
+
+extern "C" {
+
+// The Jacobi CG of feec_csr_cg on the scratch array above instead of
+// allocated buffers: the same kernels in the same order, so the same bits.
+feec_status feec_csr_cg_noalloc(feec_csr* a, const double* b, double* x, double rtol,
+                                int64_t max_iters, feec_report* report) {
+  return guard([&] {
+    FEEC_REQUIRE(a && b && x, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(a->nrows == a->ncols, FEEC_E_ARG, "CG needs a square matrix");
+    FEEC_REQUIRE(rtol >= 0.0 && max_iters >= 0, FEEC_E_ARG, "rtol >= 0 and max_iters >= 0");
+    FEEC_REQUIRE(a->nrows <= kNoAllocMaxRows, FEEC_E_ARG,
+                 "feec_csr_cg_noalloc takes at most 262144 rows (its fixed scratch); "
+                 "feec_csr_cg has no such limit");
+    Ctx* ctx = a->ctx;
+    FEEC_HIP(hipSetDevice(ctx->device));
+    double* base = scratch_for(ctx->device);
+    FEEC_REQUIRE(base, FEEC_E_DEVICE, "scratch not set up on this device");
+    const int64_t n = a->nrows;
+    double* part = base + kScalarDoubles;
+    double* v = part + 2 * NBLK;
+    double* db = v;
+    double* dx = v + kNoAllocMaxRows;
+    double* dinv = v + 2 * kNoAllocMaxRows;
+    const CgWork w{v + 3 * kNoAllocMaxRows, v + 4 * kNoAllocMaxRows, v + 5 * kNoAllocMaxRows,
+                   v + 6 * kNoAllocMaxRows, part, reinterpret_cast<CgS*>(base)};
+    copy_to_device(ctx, db, b, n * 8);
+    // Jacobi (precond.rs:53-123): the inverse diagonal, 1 where it is zero
+    csr_diagonal(*a, dinv);
+    k_reciprocal_or_one<<<(unsigned)std::max<int64_t>(1, (n + 255) / 256), 256, 0, ctx->stream>>>(
+        dinv, n, dinv);
+    FEEC_HIP(hipGetLastError());
+    KrylovStats ks = cg_solve_ws(ctx, *a, dinv, db, dx, rtol, max_iters, w);
+    copy_to_host(ctx, x, dx, n * 8);
+    if (report) {
+      *report = feec_report{};
+      report->iters = ks.iters;
+      report->converged = ks.converged ? 1 : 0;
+      report->residual = ks.precond_residual;
+      report->precond_residual = ks.precond_residual;
+    }
+    FEEC_REQUIRE(ks.converged, FEEC_E_NOT_CONVERGED, "CG did not reach rtol within max_iters");
+  });
+}
+
+}  // extern "C"

@@ -133,6 +133,9 @@ void feec_ctx_destroy(feec_ctx *ctx);
 feec_status feec_ctx_synchronize(feec_ctx *ctx);
 /* Device bytes currently held by handles of this context. */
 feec_status feec_ctx_memory(const feec_ctx *ctx, int64_t *bytes_in_use);
+/* Device allocations this context has made so far (blocks reused from its
+ * cache included): a caller can check that a call allocated nothing. */
+feec_status feec_ctx_allocations(const feec_ctx *ctx, int64_t *count);
 
 /* ---- meshers (host) ---------------------------------------------------- */
 /* CartesianGrid::new_unit(dim, n).triangulate() (regge/src/mesher/cartesian.rs:109-233):
@@ -195,6 +198,16 @@ feec_status feec_csr_spmv(feec_csr *a, const double *x, double *y); /* host vect
  * iterations and |r|/|b|; FEEC_E_NOT_CONVERGED when max_iters ends the run first. */
 feec_status feec_csr_cg(feec_csr *a, const double *b, double *x, double rtol, int64_t max_iters,
                         feec_report *report);
+/* feec_csr_cg without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same kernels,
+ * same order, same results as feec_csr_cg; its vectors, partial sums and scalars
+ * live in a file-scope device array set up by feec_ctx_create and held by each GPU
+ * until the process ends (14.1 MiB: 262144 rows at most; larger a: FEEC_E_ARG).
+ * Every call on one GPU shares that array, so calls that overlap on one GPU
+ * (from several threads or contexts) race; serialize them. */
+feec_status feec_csr_cg_noalloc(feec_csr *a, const double *b, double *x, double rtol,
+                                int64_t max_iters, feec_report *report);
 void feec_csr_destroy(feec_csr *a);
 
 /* ---- problems ------------------------------------------------------------ */
