@@ -464,6 +464,30 @@ def test_dirac_source_rejects_bad_arguments():
     np.testing.assert_array_equal(a, b)
 
 
+def test_c2_kkt_pattern_at_full_size():
+    """The 3D 64^3 k=1 KKT system (jitter 0.05h seed 2): sizes, nonzero count
+    and SHA-256 digests of row_ptr, col and values equal the oracle's
+    (tests/golden/c2_kkt_counts.json, written by make_c2_kkt_counts.py):
+    bit-identical at full size, with 60,856,610 of the 60,859,905 structural
+    entries kept (the others are exactly 0.0 in the reference's arithmetic)."""
+    import hashlib
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "c2_kkt_counts.json")) as fh:
+        ref = json.load(fh)
+    cells, coords = kuhn_mesh(3, 64, 0.05, seed=2)
+    gcx = fa.Complex(3, cells, nverts=len(coords))
+    geo = fa.MeshLengthsSq.from_coords(gcx, coords)
+    g, sizes = fa.WhitneyComplex(geo).mixed_kkt(1)
+    assert tuple(sizes) == (ref["n_sigma"], ref["n_u"], ref["n_harmonic"])
+    assert g.nnz == ref["kkt_nnz"] == 60856610 and ref["kkt_nnz_structural"] == 60859905
+    rp, col, val = g.host()
+    got = {"row_ptr": hashlib.sha256(np.ascontiguousarray(rp, "<i8").tobytes()).hexdigest(),
+           "col": hashlib.sha256(np.ascontiguousarray(col, "<i4").tobytes()).hexdigest(),
+           "val": hashlib.sha256(np.ascontiguousarray(val, "<f8").tobytes()).hexdigest()}
+    assert got == ref["sha256"]
+
+
 def test_full_size_3d_64_agrees_across_precisions():
     """The 3D Kuhn 64^3 k=1 workload at full size (2.1 M KKT rows): the
     FP64_PC32 and FP64 solves each meet the 1e-10 certificate, and their
