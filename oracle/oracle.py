@@ -30,6 +30,17 @@ _LIB = None
 
 MASS, DIF_TRIAL, DIF_TEST, DIF_BOTH = 0, 1, 2, 3
 
+# Deliberate mistakes in the KKT and Hodge-Dirac assembly below, switched on
+# only by the tests that check the reference's laws catch them (0 = the
+# restatement as written):
+#   11 U_k = D_{k-1} M_{k-1} (the mass of the wrong grade, strong side)
+#   12 U_k scaled by 2
+#   13 the sub-diagonal block negated (the super-diagonal kept)
+#   21 the sigma-row sign flip left off the coupling block
+#   22 the sigma mass scaled by 2
+#   23 the u-u block (dif_both(k+1)) negated
+MUTATION = 0
+
 i64p = C.POINTER(C.c_int64)
 f64p = C.POINTER(C.c_double)
 u8p = C.POINTER(C.c_uint8)
@@ -452,10 +463,15 @@ def assemble_mixed_kkt(c: WhitneyComplex, source, k, harmonics):
     blocks = [[b["mass_sigma"], b["dif_test"], None],
               [b["dif_test"].T, b["dif_both"], Csr.from_dense_rows(mh) if nh else None],
               [None, Csr.from_dense_rows(mh.T) if nh else None, None]]
-    scales = [[-1.0, -1.0, 1.0], [1.0, 1.0, 1.0], [1.0, 1.0, 1.0]]
     # mixed_hodge_laplacian has -dif_test in the sigma row; the sign flip of
     # the sigma rows turns [M, -B] into [-M, B].
-    scales[0][1] = 1.0
+    scales = [[-1.0, 1.0, 1.0], [1.0, 1.0, 1.0], [1.0, 1.0, 1.0]]
+    if MUTATION == 21:
+        scales[0][1] = -1.0
+    if MUTATION == 22:
+        scales[0][0] = -2.0
+    if MUTATION == 23:
+        scales[1][1] = -1.0
     kkt = block(blocks, scales, [ns, nu, nh], [ns, nu, nh])
     src = np.asarray(source, dtype=np.float64)[c.inclusion_indices(k)]
     rhs = np.concatenate([np.zeros(ns), src, np.zeros(nh)])
@@ -703,11 +719,15 @@ def hodge_dirac(c: WhitneyComplex, sign):
     rows, cols, vals = [], [], []
     for k in range(1, n + 1):
         u = masses[k] @ difs[k - 1]
+        if MUTATION == 11:
+            u = difs[k - 1] @ masses[k - 1]
+        if MUTATION == 12:
+            u = Csr(u.shape, u.row_ptr, u.col, 2.0 * u.val)
         coo_r = np.repeat(np.arange(u.shape[0]), np.diff(u.row_ptr))
         r0, c0 = offsets[k], offsets[k - 1]
         rows += [r0 + coo_r, c0 + u.col]
         cols += [c0 + u.col, r0 + coo_r]
-        vals += [u.val, sign * u.val]
+        vals += [-u.val if MUTATION == 13 else u.val, sign * u.val]
     total = int(offsets[-1])
     if rows:
         r = np.concatenate(rows)
