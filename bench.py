@@ -207,7 +207,10 @@ def run_leapfrog(args):
     print(json.dumps(line), flush=True)
 
 
-def _requests(fa, count, seed, dims, cells_lo, cells_hi):
+def config_requests(fa, count, seed, dims, cells_lo, cells_hi):
+    """The request mix of the batched and serving configurations: d uniform in
+    dims, k uniform in 0..d-1, cells log-uniform in [cells_lo, cells_hi], jitter
+    0.1h / 0.05h / 0.02h by dimension, mesh and source seeds from `seed`."""
     rng = np.random.default_rng(seed)
     amp = {2: 0.1, 3: 0.05, 4: 0.02}
     out = []
@@ -236,7 +239,7 @@ def run_c4(args):
     The same batch with FP32 MINRES inside FP64 refinement and in FP64
     beside it."""
     import formoniq_amd as fa
-    reqs = _requests(fa, args.batch, 4, [2, 3, 4], 1e3, 1e5)
+    reqs = config_requests(fa, args.batch, 4, [2, 3, 4], 1e3, 1e5)
     opts = fa.solve_opts(use_graph=0 if args.no_graph else 1)
     names = _mode_names(fa)
     out = {}
@@ -270,7 +273,7 @@ def run_c5(args):
     FP32 polynomial on FP16 storage) to 1e-10. Throughput and p50/p99
     latency, and the same stream on the FP64 path."""
     import formoniq_amd as fa
-    reqs = _requests(fa, args.requests, 5, [2, 3], 1e3, 2e6)
+    reqs = config_requests(fa, args.requests, 5, [2, 3], 1e3, 2e6)
     gaps = np.random.default_rng(5).exponential(1.0 / args.rate, size=len(reqs))
     names = _mode_names(fa)
     out = {}
@@ -284,7 +287,7 @@ def run_c5(args):
         srv = fa.Server(opts)
         if not args.no_warmup:
             # a server's start-up: one small request of every (d, k) the stream holds
-            warm = [r for r in _requests(fa, 64, 55, [2, 3], 1e3, 2e3)]
+            warm = [r for r in config_requests(fa, 64, 55, [2, 3], 1e3, 2e3)]
             seen = {}
             for r in warm:
                 seen.setdefault((r.dim, r.grade), r)

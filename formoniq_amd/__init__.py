@@ -136,6 +136,15 @@ class CartesianGrid:
         return coords
 
 
+def load_vector(geometry: "MeshLengthsSq", grade, cochain):
+    """The Galerkin load b = M_k c of a cochain (feec_load_vector), the
+    type-correct source for solve_source (galerkin.rs:324-371)."""
+    c = np.ascontiguousarray(cochain, dtype=np.float64)
+    out = np.zeros_like(c)
+    check(load().feec_load_vector(geometry._h, grade, _ptr(c, _lib.f64p), _ptr(out, _lib.f64p)))
+    return out
+
+
 def normal_vector(n, seed):
     out = np.zeros(n)
     check(load().feec_normal_vector(n, seed, _ptr(out, _lib.f64p)))
@@ -203,6 +212,19 @@ class Csr:
         if raise_on_failure:
             check(st)
         return x, rep
+
+    def view(self, on_device):
+        """feec_csr_view: (nrows, ncols, nnz, row_ptr, col, val) with the three
+        arrays as raw addresses (device memory when on_device)."""
+        nr, nc, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        rp, col, val = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(load().feec_csr_view(self._h, C.byref(nr), C.byref(nc), C.byref(nnz), C.byref(rp),
+                                   C.byref(col), C.byref(val), int(bool(on_device))))
+        return nr.value, nc.value, nnz.value, rp.value, col.value, val.value
+
+    def spmv_device(self, x_addr, y_addr):
+        """y = A x on device arrays (feec_csr_spmv_device)."""
+        check(load().feec_csr_spmv_device(self._h, C.c_void_p(x_addr), C.c_void_p(y_addr)))
 
     def cg_noalloc(self, b, rtol=1e-10, max_iters=10_000, raise_on_failure=True):
         """cg() on the library's fixed device scratch (feec_csr_cg_noalloc):
@@ -495,6 +517,17 @@ class Problem:
         else:
             src = np.ascontiguousarray(source, dtype=np.float64)
             check(load().feec_problem_solve(self._h, _ptr(src, _lib.f64p), C.byref(rep)))
+        return rep
+
+    def solve_cochain(self, cochain=None) -> Report:
+        """Solve with the Galerkin load b = M_k c of a cochain c, formed on the
+        device (feec_problem_solve_cochain*); None: c is in source_buffer()."""
+        rep = Report()
+        if cochain is None:
+            check(load().feec_problem_solve_cochain_device(self._h, self.source_buffer(), C.byref(rep)))
+        else:
+            c = np.ascontiguousarray(cochain, dtype=np.float64)
+            check(load().feec_problem_solve_cochain(self._h, _ptr(c, _lib.f64p), C.byref(rep)))
         return rep
 
     def solution(self):

@@ -114,6 +114,13 @@ SIGNATURES = {
     "feec_problem_assemble": (S, [vp]),
     "feec_problem_solve_device": (S, [vp, vp, C.POINTER(Report)]),
     "feec_problem_solve": (S, [vp, f64p, C.POINTER(Report)]),
+    "feec_problem_solve_cochain_device": (S, [vp, vp, C.POINTER(Report)]),
+    "feec_problem_solve_cochain": (S, [vp, f64p, C.POINTER(Report)]),
+    "feec_csr_view": (S, [vp, i64p, i64p, i64p, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int]),
+    "feec_csr_spmv_device": (S, [vp, vp, vp]),
+    "feec_vec_dot_device": (S, [vp, C.c_int64, vp, vp, f64p]),
+    "feec_vec_axpy_device": (S, [vp, C.c_int64, C.c_double, vp, vp]),
+    "feec_load_vector": (S, [vp, C.c_int, f64p, f64p]),
     "feec_problem_solution": (S, [vp, f64p, f64p, f64p]),
     "feec_problem_sizes": (S, [vp, i64p, i64p]),
     "feec_problem_source_buffer": (S, [vp, C.POINTER(vp)]),

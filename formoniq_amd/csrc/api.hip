@@ -153,6 +153,15 @@ struct Problem {
   DBuf<double> rhs, x, source;
   DBuf<double> harmonics;  // [nh][nu] M-orthonormal basis
   std::vector<double> harmonics_host;
+  // b = M_k c for cochain sources: the full-complex M_k assembled from the M_k
+  // element block (plan made on first use, values refreshed when the element
+  // blocks change)
+  CooPlan mass_plan;
+  CompactMap mass_map;
+  Csr Mk;
+  bool have_mass_plan = false;
+  int64_t el_version = 0, mass_version = -1;
+  DBuf<double> load;
   feec_report last{};
   ~Problem() { solver_state_free(ws); }
 };
@@ -452,6 +461,7 @@ void problem_assemble_from_elements(Problem& P) {
   Geom& g = *P.g;
   Mesh& m = *g.mesh;
   Ctx* ctx = m.ctx;
+  ++P.el_version;
   bool rebuilt = assemble_compacted(ctx, P.kkt_plan, P.el.p, P.kkt_map, P.sys.A);
   coo_plan_dense_column(ctx, P.diag_plan, P.el.p, P.diag.p);
   k_inv_diag<<<grid_for(P.sys.n + P.nh), TPB, 0, ctx->stream>>>(P.diag.p, P.sys.n, P.nh,
@@ -1023,6 +1033,89 @@ feec_status feec_csr_spmv(feec_csr* a, const double* x, double* y) {
   });
 }
 
+feec_status feec_csr_view(const feec_csr* a, int64_t* nrows, int64_t* ncols, int64_t* nnz,
+                          const int64_t** row_ptr, const int32_t** col, const double** val,
+                          int on_device) {
+  return guard([&] {
+    FEEC_REQUIRE(a, FEEC_E_ARG, "null matrix");
+    if (nrows) *nrows = a->nrows;
+    if (ncols) *ncols = a->ncols;
+    if (nnz) *nnz = a->nnz;
+    if (on_device) {
+      // the arrays are written on the context's stream: drain it first
+      a->ctx->sync();
+      if (row_ptr) *row_ptr = a->row_ptr.p;
+      if (col) *col = a->col.p;
+      if (val) *val = a->val.p;
+      return;
+    }
+    auto* h = const_cast<feec_csr*>(a);
+    if (!h->have_host) {
+      h->h_row_ptr.resize(a->nrows + 1);
+      h->h_col.resize(a->nnz);
+      h->h_val.resize(a->nnz);
+      copy_to_host(a->ctx, h->h_row_ptr.data(), a->row_ptr.p, (a->nrows + 1) * 8);
+      if (a->nnz) {
+        copy_to_host(a->ctx, h->h_col.data(), a->col.p, a->nnz * 4);
+        copy_to_host(a->ctx, h->h_val.data(), a->val.p, a->nnz * 8);
+      }
+      h->have_host = true;
+    }
+    if (row_ptr) *row_ptr = h->h_row_ptr.data();
+    if (col) *col = h->h_col.data();
+    if (val) *val = h->h_val.data();
+  });
+}
+
+feec_status feec_csr_spmv_device(const feec_csr* a, const double* x, double* y) {
+  return guard([&] {
+    FEEC_REQUIRE(a && (x || !a->ncols) && (y || !a->nrows), FEEC_E_ARG, "null argument");
+    FEEC_HIP(hipSetDevice(a->ctx->device));
+    spmv64(a->ctx, *a, x, y);
+    a->ctx->sync();
+  });
+}
+
+feec_status feec_vec_dot_device(feec_ctx* ctx, int64_t n, const double* x, const double* y,
+                                double* out) {
+  return guard([&] {
+    FEEC_REQUIRE(ctx && out && n >= 0 && (n == 0 || (x && y)), FEEC_E_ARG, "bad argument");
+    FEEC_HIP(hipSetDevice(ctx->device));
+    if (n == 0) {
+      *out = 0.0;
+      return;
+    }
+    DBuf<double> part(ctx, 2 * 4096);
+    *out = dot64(ctx, x, y, n, part.p);
+  });
+}
+
+feec_status feec_vec_axpy_device(feec_ctx* ctx, int64_t n, double a, const double* x, double* y) {
+  return guard([&] {
+    FEEC_REQUIRE(ctx && n >= 0 && (n == 0 || (x && y)), FEEC_E_ARG, "bad argument");
+    FEEC_HIP(hipSetDevice(ctx->device));
+    axpy64(ctx, x, n, a, y);
+    ctx->sync();
+  });
+}
+
+feec_status feec_load_vector(feec_geom* g, int grade, const double* cochain, double* out) {
+  return guard([&] {
+    FEEC_REQUIRE(g && cochain && out, FEEC_E_ARG, "null argument");
+    Mesh& m = *g->mesh;
+    FEEC_REQUIRE(grade >= 0 && grade <= m.d, FEEC_E_ARG, "grade out of range");
+    Ctx* ctx = m.ctx;
+    FEEC_HIP(hipSetDevice(ctx->device));
+    Csr M;
+    assemble_form(*g, FEEC_FULL, FEEC_MASS, grade, M);
+    const int64_t n = m.n[grade];
+    DBuf<double> dc(ctx, std::max<int64_t>(n, 1)), db(ctx, std::max<int64_t>(n, 1));
+    copy_to_device(ctx, dc.p, cochain, n * 8);
+    spmv_rowseq(ctx, M, dc.p, db.p);
+    copy_to_host(ctx, out, db.p, n * 8);
+  });
+}
+
 void feec_csr_destroy(feec_csr* a) { delete a; }
 
 namespace {
@@ -1291,6 +1384,54 @@ feec_status feec_problem_solve(feec_problem* p, const double* src, feec_report* 
     Mesh& m = *p->g->mesh;
     copy_to_device(m.ctx, p->source.p, src, m.n[p->k] * 8);
     feec_status s = feec_problem_solve_device(p, p->source.p, report);
+    if (s != FEEC_OK) fail(s, g_last_error);
+  });
+}
+
+namespace feec {
+// b = M_k c on the device (full complex, galerkin.rs:324-371 reading of a
+// cochain source), from the problem's element blocks.
+static void problem_load(Problem& P, const double* cochain_dev) {
+  Mesh& m = *P.g->mesh;
+  Ctx* ctx = m.ctx;
+  const int k = P.k, nf = P.nf;
+  const int64_t nc = m.n[m.d], nk = m.n[k];
+  if (!P.have_mass_plan) {
+    const int64_t total = nc * nf * nf;
+    DBuf<int32_t> rows(ctx, std::max<int64_t>(total, 1)), cols(ctx, std::max<int64_t>(total, 1));
+    DBuf<uint32_t> srcs(ctx, std::max<int64_t>(total, 1));
+    form_contributions(m, FEEC_FULL, k, k, nf, nf, 0, 0, false, (uint32_t)P.off[5], false, rows.p,
+                       cols.p, srcs.p);
+    coo_plan_build(ctx, P.mass_plan, nk, nk, total, rows.p, cols.p, srcs.p);
+    P.load.alloc(ctx, std::max<int64_t>(nk, 1));
+    P.have_mass_plan = true;
+  }
+  if (P.mass_version != P.el_version) {
+    assemble_compacted(ctx, P.mass_plan, P.el.p, P.mass_map, P.Mk);
+    P.mass_version = P.el_version;
+  }
+  spmv_rowseq(ctx, P.Mk, cochain_dev, P.load.p);
+}
+}  // namespace feec
+
+feec_status feec_problem_solve_cochain_device(feec_problem* p, const double* cochain,
+                                              feec_report* report) {
+  return guard([&] {
+    FEEC_REQUIRE(p && cochain, FEEC_E_ARG, "null argument");
+    FEEC_HIP(hipSetDevice(p->g->mesh->ctx->device));
+    problem_load(*p, cochain);
+    feec_status s = feec_problem_solve_device(p, p->load.p, report);
+    if (s != FEEC_OK) fail(s, g_last_error);
+  });
+}
+
+feec_status feec_problem_solve_cochain(feec_problem* p, const double* cochain, feec_report* report) {
+  return guard([&] {
+    FEEC_REQUIRE(p && cochain, FEEC_E_ARG, "null argument");
+    Mesh& m = *p->g->mesh;
+    FEEC_HIP(hipSetDevice(m.ctx->device));
+    copy_to_device(m.ctx, p->source.p, cochain, m.n[p->k] * 8);
+    feec_status s = feec_problem_solve_cochain_device(p, p->source.p, report);
     if (s != FEEC_OK) fail(s, g_last_error);
   });
 }

@@ -175,6 +175,11 @@ struct Csr {
   DBuf<int64_t> row_ptr;
   DBuf<int32_t> col;
   DBuf<double> val;
+  // host copy behind feec_csr_view(..., on_device = 0), made on first request
+  std::vector<int64_t> h_row_ptr;
+  std::vector<int32_t> h_col;
+  std::vector<double> h_val;
+  bool have_host = false;
 };
 
 struct Mesh {
@@ -346,6 +351,9 @@ void kkt_solve_batch(std::vector<KktSystem*>& systems, std::vector<const double*
                      std::vector<double*>& x, const std::vector<feec_solve_opts>& opts,
                      std::vector<SolverState**>& ws, std::vector<feec_report>& reps, Ctx* lead);
 void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y);
+// y = A x with one lane per row summing its entries in column order from the
+// first product, FP contraction off: nalgebra-sparse's CSR x dense order.
+void spmv_rowseq(Ctx* ctx, const Csr& a, const double* x, double* y);
 double time_minres_spmv(KktSystem& s, bool fp32, int reps);
 // z = B r with the block preconditioner as a solve of `precision` with
 // `storage` (feec_pc_storage) applies it; device vectors of n + nh doubles.

@@ -2723,6 +2723,26 @@ void spmv64(Ctx* ctx, const Csr& a, const double* x, double* y) {
 }
 
 // a . b over n device doubles (fixed-order partial sums), read on the host.
+namespace {
+__global__ void k_spmv_rowseq(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                              const double* __restrict__ val, int64_t n, const double* __restrict__ x,
+                              double* __restrict__ y) {
+#pragma clang fp contract(off)
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double acc = 0.0;
+  for (int64_t p = rp[r]; p < rp[r + 1]; ++p) acc = acc + val[p] * x[col[p]];
+  y[r] = acc;
+}
+}  // namespace
+
+void spmv_rowseq(Ctx* ctx, const Csr& a, const double* x, double* y) {
+  if (a.nrows == 0) return;
+  k_spmv_rowseq<<<grid_for(a.nrows), TPB, 0, ctx->stream>>>(a.row_ptr.p, a.col.p, a.val.p, a.nrows,
+                                                            x, y);
+  FEEC_HIP(hipGetLastError());
+}
+
 double dot64(Ctx* ctx, const double* a, const double* b, int64_t n, double* part) {
   hipStream_t st = ctx->stream;
   k_dot<double><<<red_grid(n), TPB, 0, st>>>(a, b, n, part);

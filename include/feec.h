@@ -193,6 +193,26 @@ feec_status feec_assemble(feec_geom *geom, feec_bc bc, feec_form_kind kind, int 
 feec_status feec_csr_shape(const feec_csr *a, int64_t *nrows, int64_t *ncols, int64_t *nnz);
 feec_status feec_csr_copy(const feec_csr *a, int64_t *row_ptr, int32_t *col, double *val);
 feec_status feec_csr_spmv(feec_csr *a, const double *x, double *y); /* host vectors */
+/* Views of a library CSR's arrays (int64 row_ptr, int32 col, FP64 val), valid until
+ * feec_csr_destroy. on_device = 1: the device arrays themselves, in the context's memory
+ * (the context's stream is drained first, so device code of the caller may read them);
+ * on_device = 0: a library-owned host copy made on the first request. Any pointer may be NULL. */
+feec_status feec_csr_view(const feec_csr *a, int64_t *nrows, int64_t *ncols, int64_t *nnz,
+                          const int64_t **row_ptr, const int32_t **col, const double **val,
+                          int on_device);
+/* The device-vector seam of iterative/src/lib.rs:58-186 (LinearOperator::apply,
+ * InnerProductSpace::{dot, add_scaled}) on caller-owned device arrays of the matrix's
+ * context's device; each call is complete on return. */
+feec_status feec_csr_spmv_device(const feec_csr *a, const double *x, double *y); /* y = A x */
+feec_status feec_vec_dot_device(feec_ctx *ctx, int64_t n, const double *x, const double *y,
+                                double *out); /* *out (host) = x . y, fixed summation order */
+feec_status feec_vec_axpy_device(feec_ctx *ctx, int64_t n, double a, const double *x,
+                                 double *y); /* y += a x */
+/* The Galerkin load of a cochain, b = M_k c (N_k entries, full complex): the L^2
+ * functional of its Whitney interpolant, galerkin.rs:324-371, the type-correct way to pass
+ * a cochain to feec_solve_source. M_k is assembled as feec_assemble does; the product sums
+ * each row in column order (nalgebra-sparse's CSR x dense). */
+feec_status feec_load_vector(feec_geom *geom, int grade, const double *cochain, double *out);
 /* krylov::cg with the Jacobi preconditioner (krylov.rs:48-95, precond.rs:53-123) on a library
  * CSR, scalars on the device: x (host) from x0 = 0 until |r| <= rtol |b|. The report holds the
  * iterations and |r|/|b|; FEEC_E_NOT_CONVERGED when max_iters ends the run first. */
@@ -265,6 +285,14 @@ feec_status feec_problem_assemble(feec_problem *prob);
 feec_status feec_problem_solve_device(feec_problem *prob, const double *source_device,
                                       feec_report *report);
 feec_status feec_problem_solve(feec_problem *prob, const double *source_host, feec_report *report);
+/* Solve with a cochain source c (N_k doubles, device or host): the Galerkin load b = M_k c
+ * (as feec_load_vector, bit for bit) is formed on the device from the problem's own
+ * element blocks, then solved as feec_problem_solve_device(b). The first call builds the
+ * full-complex M_k plan (kept with the problem). */
+feec_status feec_problem_solve_cochain_device(feec_problem *prob, const double *cochain_device,
+                                              feec_report *report);
+feec_status feec_problem_solve_cochain(feec_problem *prob, const double *cochain_host,
+                                       feec_report *report);
 feec_status feec_problem_solution(feec_problem *prob, double *sigma, double *u, double *p);
 feec_status feec_problem_sizes(const feec_problem *prob, int64_t sizes[3], int64_t *nnz);
 /* Preconditioner matrix as a solve uses it (feec_problem_preconditioner). */
