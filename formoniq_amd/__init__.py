@@ -34,6 +34,8 @@ EUCLIDEAN, MINKOWSKI = 0, 1
 FP64, MIXED, FP64_PC32 = 0, 1, 2
 # feec_pc_storage: how the preconditioner matrix the polynomial runs on is stored
 PC_SELL, PC_CSR_HALF, PC_CSR = 0, 1, 2
+# feec_preconditioner: the block preconditioner of the mixed KKT system
+PC_AUTO, PC_CHEBYSHEV, PC_MULTIGRID = 0, 1, 2
 
 __all__ = [
     "Context", "CartesianGrid", "Complex", "MeshLengthsSq", "WhitneyComplex", "Csr",
@@ -42,7 +44,8 @@ __all__ = [
     "FeecError",
     "normal_vector", "MASS", "DIF_TRIAL", "DIF_TEST", "DIF_BOTH", "HDIF_GRAM", "FULL",
     "RELATIVE", "EUCLIDEAN", "MINKOWSKI", "FP64", "MIXED", "FP64_PC32", "Report", "Request", "Result",
-    "ServerOpts", "SolveOpts", "Server", "batch_run",
+    "ServerOpts", "SolveOpts", "Server", "batch_run", "load_vector", "PC_AUTO", "PC_CHEBYSHEV",
+    "PC_MULTIGRID",
 ]
 
 

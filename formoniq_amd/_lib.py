@@ -24,7 +24,8 @@ class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("max_iters", C.c_int64), ("precision", C.c_int32),
                 ("max_refinements", C.c_int32), ("inner_rtol", C.c_double),
                 ("use_graph", C.c_int32), ("graph_iters", C.c_int32),
-                ("cheb_degree", C.c_int32), ("cheb_lower", C.c_double), ("precond_storage", C.c_int32)]
+                ("cheb_degree", C.c_int32), ("cheb_lower", C.c_double), ("precond_storage", C.c_int32),
+                ("preconditioner", C.c_int32), ("mg_sweeps", C.c_int32)]
 
 
 class Report(C.Structure):

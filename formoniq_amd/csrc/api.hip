@@ -162,8 +162,14 @@ struct Problem {
   bool have_mass_plan = false;
   int64_t el_version = 0, mass_version = -1;
   DBuf<double> load;
+  // the multigrid / auxiliary-space preconditioner, when chosen
+  Hx* hx = nullptr;
+  bool hx_tried = false;
   feec_report last{};
-  ~Problem() { solver_state_free(ws); }
+  ~Problem() {
+    solver_state_free(ws);
+    hx_free(hx);
+  }
 };
 
 namespace {
@@ -455,6 +461,34 @@ static int resolve_cheb_degree(const Problem& P) {
   return std::clamp(deg, 16, 128);
 }
 
+// The multigrid / auxiliary-space preconditioner: built on the first
+// assembly when the options ask for it (AUTO: where it applies and the KKT
+// system has at least 20,000 rows), refreshed with the values afterwards.
+constexpr int64_t kMultigridAutoRows = 20000;
+void problem_multigrid(Problem& P) {
+  const int pc = P.opts.preconditioner;
+  if (pc == FEEC_PC_CHEBYSHEV || (pc == FEEC_PC_AUTO && P.sys.n < kMultigridAutoRows)) {
+    P.sys.hx = nullptr;
+    return;
+  }
+  if (P.hx) {
+    hx_refresh(*P.hx, *P.g, P.sys);
+    P.sys.hx = P.hx;
+    return;
+  }
+  if (P.hx_tried && pc == FEEC_PC_AUTO) return;
+  P.hx_tried = true;
+  std::string why;
+  P.hx = hx_create(*P.g, P.bc, P.k, P.sys, P.opts.mg_sweeps, &why);
+  FEEC_REQUIRE(P.hx || pc != FEEC_PC_MULTIGRID, FEEC_E_UNSUPPORTED,
+               "multigrid preconditioner unavailable: " + why);
+  P.sys.hx = P.hx;
+  if (P.hx && P.ws) {  // graphs captured with the polynomial
+    solver_state_free(P.ws);
+    P.ws = nullptr;
+  }
+}
+
 // CSR values, preconditioner diagonal and FP32 copy from the element blocks
 // already in P.el.
 void problem_assemble_from_elements(Problem& P) {
@@ -472,13 +506,16 @@ void problem_assemble_from_elements(Problem& P) {
     P.ws = nullptr;
   }
   if (P.sys.P.nnz) coo_plan_values(ctx, P.pc_plan, P.el.p, P.sys.P.val.p, nullptr);
+  problem_multigrid(P);
   // the FP32/FP16 copies are made for mixed and FP64_PC32 problems, the FP64
   // SELL copy for FP64 ones; either is refreshed once made
   if (P.opts.precision != FEEC_PREC_FP64 || P.sys.val32.p) kkt_refresh_fp32(P.sys);
-  if ((P.opts.precision == FEEC_PREC_FP64 && P.opts.precond_storage == FEEC_PC_STORE_SELL) ||
-      P.sys.Psell64.p)
-    kkt_refresh_sell64(P.sys);
-  kkt_set_chebyshev(P.sys, resolve_cheb_degree(P), P.opts.cheb_lower);
+  if (!P.hx) {
+    if ((P.opts.precision == FEEC_PREC_FP64 && P.opts.precond_storage == FEEC_PC_STORE_SELL) ||
+        P.sys.Psell64.p)
+      kkt_refresh_sell64(P.sys);
+    kkt_set_chebyshev(P.sys, resolve_cheb_degree(P), P.opts.cheb_lower);
+  }
   ctx->sync();
 }
 
@@ -507,6 +544,9 @@ static void validate_opts(const feec_solve_opts& o) {
                FEEC_E_ARG, "FEEC_PREC_FP64_PC32 runs on the SELL storage only");
   FEEC_REQUIRE(o.cheb_degree >= -1, FEEC_E_ARG, "cheb_degree must be >= -1");
   FEEC_REQUIRE(o.cheb_lower >= 0.0 && o.cheb_lower < 1.0, FEEC_E_ARG, "cheb_lower must be in [0, 1)");
+  FEEC_REQUIRE(o.preconditioner >= FEEC_PC_AUTO && o.preconditioner <= FEEC_PC_MULTIGRID, FEEC_E_ARG,
+               "preconditioner must be a feec_preconditioner value");
+  FEEC_REQUIRE(o.mg_sweeps >= 1 && o.mg_sweeps <= 8, FEEC_E_ARG, "mg_sweeps must be 1..8");
 }
 
 Problem* problem_new(Geom* g, int bc, int k, const feec_solve_opts& o) {
@@ -621,6 +661,8 @@ void feec_solve_opts_default(feec_solve_opts* o) {
   o->cheb_degree = -1;
   o->cheb_lower = 0.0;
   o->precond_storage = FEEC_PC_STORE_SELL;
+  o->preconditioner = FEEC_PC_AUTO;
+  o->mg_sweeps = 2;
 }
 
 feec_status feec_ctx_create(int device, feec_ctx** out) {
@@ -929,6 +971,8 @@ feec_status feec_geom_from_coords(feec_mesh* m, const double* coords, feec_ambie
     DBuf<double> c(m->ctx, m->n[0] * m->d);
     copy_to_device(m->ctx, c.p, coords, m->n[0] * m->d * 8);
     lengths_from_coords(*m, c.p, amb == FEEC_MINKOWSKI, g->lengths.p);
+    g->coords.assign(coords, coords + m->n[0] * m->d);
+    g->euclidean = amb != FEEC_MINKOWSKI;
     m->ctx->sync();
     *out = g;
   });

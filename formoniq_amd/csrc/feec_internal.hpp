@@ -200,6 +200,7 @@ struct Mesh {
   bool have_orientation = false;
   bool orientable = false;
   int64_t components = -1;  // connected components (vertices joined by cells), lazily
+  int kuhn_n = -1;          // cells per axis when a Kuhn grid (0: not one), lazily
   // cached assembly plans keyed by (bc, row grade, col grade)
   std::map<std::tuple<int, int, int>, std::unique_ptr<CooPlan>> plans;
   std::mutex mu;
@@ -213,6 +214,10 @@ struct Mesh {
 struct Geom {
   Mesh* mesh = nullptr;
   DBuf<double> lengths;  // N_1 signed squared edge lengths
+  // vertex coordinates [N_0][d] when the geometry came from coordinates (host
+  // copy; the auxiliary-space preconditioner reads its frame off them)
+  std::vector<double> coords;
+  bool euclidean = false;
 };
 
 // ---------------------------------------------------------------------------
@@ -332,6 +337,9 @@ struct KktSystem {
   int cheb_m = 0;
   DBuf<double> cheb;
   double lambda_max[2] = {0.0, 0.0};
+  // multigrid / auxiliary-space block preconditioner (owned by the problem);
+  // when set, solves use it instead of the Chebyshev polynomial
+  struct Hx* hx = nullptr;
 };
 // Estimate the spectra of D^-1 P per block and set degree-m Chebyshev
 // coefficients (m = 0 disables the polynomial).
@@ -368,6 +376,27 @@ KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const do
 // Sets up (once per device, that device current) the address of the scratch
 // array of feec_csr_cg_noalloc on the current device.
 void noalloc_scratch_init(int device);
+// multigrid.hip
+struct Hx;
+struct Enq;
+struct MinresS;
+// n when the mesh is the Kuhn triangulation of the n^d box with the vertex
+// numbering of feec_kuhn_grid (sum_a c_a (n+1)^a); 0 otherwise.
+int kuhn_grid_n(Mesh& m);
+// The HX block preconditioner diag(B_{k-1}, B_k, I) of a KKT system of grade k
+// (values from s's preconditioner blocks); nullptr, with the reason, when it
+// does not apply (no Kuhn grid, no coordinates, non-Euclidean, k > 2).
+Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::string* why);
+// New values after re-assembly (same pattern).
+void hx_refresh(Hx& h, Geom& g, const KktSystem& s);
+void hx_free(Hx* h);
+int64_t hx_bytes_per_apply(const Hx& h);
+int hx_launches_per_apply(const Hx& h);
+// z = B r with partial sums of r.z into part (red_grid(N) blocks): FP64
+// vectors with the cycle in FP32 (fp32) or FP64, or FP32 throughout.
+void hx_enqueue(Enq& q, Hx& h, const double* r, double* z, const MinresS* S, double* part, bool fp32);
+void hx_enqueue(Enq& q, Hx& h, const float* r, float* z, const MinresS* S, double* part);
+
 // a . b (part: 2 * 4096 doubles of scratch) and x += s a, on device doubles
 double dot64(Ctx* ctx, const double* a, const double* b, int64_t n, double* part);
 void axpy64(Ctx* ctx, const double* a, int64_t n, double s, double* x);

@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "feec_internal.hpp"
+#include "enqueue.hpp"
 
 namespace feec {
 
@@ -50,65 +51,6 @@ constexpr int TPB = 256;
 constexpr int NBLK = 4096;  // fixed reduction grid: 16 blocks per CU on 256 CUs
 constexpr int LANES = 8;    // lanes per CSR row in the KKT SpMV (about 28 nonzeros a row)
 constexpr int PLANES = 4;   // lanes per row in the preconditioner SpMV (about 16 a row)
-
-// Enqueues kernels on a stream, or appends them as a linear chain of kernel
-// nodes to a graph under construction. Node arguments are kept alive in
-// `keep` for as long as the graph exists, so nothing that inspects a node's
-// parameters after its creation can see freed memory.
-template <class... P>
-struct KernelArgs {
-  std::tuple<P...> values;
-  void* ptrs[sizeof...(P) > 0 ? sizeof...(P) : 1];
-  template <class... A>
-  explicit KernelArgs(A... a) : values(static_cast<P>(a)...) {
-    std::apply(
-        [&](auto&... x) {
-          int i = 0;
-          ((ptrs[i++] = static_cast<void*>(&x)), ...);
-        },
-        values);
-  }
-};
-
-struct Enq {
-  hipStream_t st = nullptr;
-  hipGraph_t graph = nullptr;
-  std::vector<std::shared_ptr<void>>* keep = nullptr;
-  hipGraphNode_t last = nullptr;
-
-  template <class... P, class... A>
-  void operator()(void (*kernel)(P...), dim3 grid, dim3 block, A... a) {
-    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
-    if (!graph) {
-      KernelArgs<std::decay_t<P>...> args(a...);
-      FEEC_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, args.ptrs, 0, st));
-      return;
-    }
-    auto args = std::make_shared<KernelArgs<std::decay_t<P>...>>(a...);
-    keep->push_back(args);
-    hipKernelNodeParams p{};
-    p.func = reinterpret_cast<void*>(kernel);
-    p.gridDim = grid;
-    p.blockDim = block;
-    p.sharedMemBytes = 0;
-    p.kernelParams = args->ptrs;
-    p.extra = nullptr;
-    hipGraphNode_t node = nullptr;
-    FEEC_HIP(hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &p));
-    last = node;
-  }
-};
-
-struct MinresS {
-  double beta, oldb, beta1, dbar, epsln, phibar, cs, sn;
-  double alfa, residual;
-  double invb_prev, oldeps, delta, invg, phi;
-  double rtol;
-  double extra_alpha;
-  long long iter, max_iters;
-  int done;
-  int pad;
-};
 
 template <class T>
 __device__ __forceinline__ double to_d(T v) {
@@ -1064,6 +1006,9 @@ struct MinresWork {
   SellF16W sell16w{};
   const float* sd32 = nullptr;
   DBuf<float> res32, d0_32, d1_32;
+  // multigrid / auxiliary-space preconditioner (replaces the polynomial)
+  Hx* hx = nullptr;
+  bool hx_fp32 = false;
   ~MinresWork() {
     if (graph) (void)hipGraphExecDestroy(graph);
   }
@@ -1278,7 +1223,13 @@ static void minres_iteration(Enq& q, MinresWork<T>& w, const int64_t* rp, const 
   q(k_mr_alpha, 1, TPB, w.S.p, w.part.p, spmv_grid(n), w.nh ? 1 : 0);
   bool pc32 = false;
   if constexpr (std::is_same_v<T, double>) pc32 = w.pc32 && w.cheb_m > 0;
-  if (pc32) {
+  if (w.hx) {
+    q(k_mr_update_only<T>, red_grid(N), TPB, r1, r2, N, w.S.p);
+    if constexpr (std::is_same_v<T, double>)
+      hx_enqueue(q, *w.hx, (const double*)r1, z, w.S.p, w.part2.p, w.hx_fp32);
+    else
+      hx_enqueue(q, *w.hx, (const float*)r1, z, w.S.p, w.part2.p);
+  } else if (pc32) {
     if constexpr (std::is_same_v<T, double>) {
       q(k_mr_update_cheb3_init<double, float>, red_grid(N), TPB, r1, r2, w.sd32, w.sell_perm,
         w.res32.p, w.d0_32.p, w.d1_32.p, n, N, w.split, w.coef, w.S.p);
@@ -1317,6 +1268,10 @@ static int segment_iters(const MinresWork<T>& w, const feec_solve_opts& o) {
     bytes += (w.cheb_m - 1) * (eb * (double)w.p_entries + 4 * pb * (double)w.n);
     launches += w.cheb_m + 1;
   }
+  if (w.hx) {
+    bytes += (double)hx_bytes_per_apply(*w.hx) * (std::is_same_v<T, float> || w.hx_fp32 ? 1.0 : 1.6);
+    launches += hx_launches_per_apply(*w.hx);
+  }
   const double t = bytes / 4e12 + launches * 4e-6;
   const int seg = (int)std::min(1e6, 0.016 / t);
   const int cap = std::max(2, o.graph_iters - (o.graph_iters & 1));
@@ -1342,6 +1297,12 @@ static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const
     // the buffer roles rotate with the executed iterations and a segment
     // starts at parity 0: an odd count takes one iteration outside the graph
     if (start_iter & 1) minres_iteration<T>(q, w, rp, col, val, 1);
+  } else if (w.hx) {
+    q(k_mr_zero<T>, red_grid(N), TPB, w.b.p, N, w.r1.p, w.r2.p, w.x.p, w.w.p, w.w2.p);
+    if constexpr (std::is_same_v<T, double>)
+      hx_enqueue(q, *w.hx, (const double*)w.b.p, w.y.p, nullptr, w.part.p, w.hx_fp32);
+    else
+      hx_enqueue(q, *w.hx, (const float*)w.b.p, w.y.p, nullptr, w.part.p);
   } else if (w.cheb_m > 0) {
     q(k_mr_zero<T>, red_grid(N), TPB, w.b.p, N, w.r1.p, w.r2.p, w.x.p, w.w.p, w.w2.p);
     cheb_apply<T>(q, w, w.b.p, w.y.p, nullptr, w.part.p);
@@ -1351,7 +1312,7 @@ static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const
   }
   if (!resume) q(k_mr_start, 1, TPB, w.S.p, w.part.p, red_grid(N), rtol, (long long)max_iters);
   const int seg = segment_iters(w, o);
-  if (o.use_graph && (!w.graph || w.graph_iters != seg || w.graph_cheb != w.cheb_m)) {
+  if (o.use_graph && (!w.graph || w.graph_iters != seg || w.graph_cheb != (w.hx ? -2 : w.cheb_m))) {
     if (w.graph) {
       (void)hipGraphExecDestroy(w.graph);
       w.graph = nullptr;
@@ -1371,7 +1332,7 @@ static KrylovStats minres_run(Ctx* ctx, SolverState& ss, MinresWork<T>& w, const
     (void)hipGraphDestroy(g);
     FEEC_HIP(ie);
     w.graph_iters = seg;
-    w.graph_cheb = w.cheb_m;
+    w.graph_cheb = w.hx ? -2 : w.cheb_m;
   }
   KrylovStats ks;
   // One segment at a time; the host reads the scalar block between segments.
@@ -1586,7 +1547,9 @@ void kkt_refresh_fp32(KktSystem& s) {
   int64_t N = s.n + s.nh;
   if (!s.inv_diag32.p || (int64_t)s.inv_diag32.n != N) s.inv_diag32.alloc(s.ctx, N);
   k_f64_to_f32<<<grid_for(N), TPB, 0, s.ctx->stream>>>(s.inv_diag.p, N, s.inv_diag32.p);
-  if (s.P.nnz) {
+  // the polynomial's copies of P are not needed when the multigrid
+  // preconditioner replaces it
+  if (s.P.nnz && !s.hx) {
     hipStream_t st = s.ctx->stream;
     ensure_band(s);
     if (!s.P32.p || (int64_t)s.P32.n != s.P.nnz) s.P32.alloc(s.ctx, s.P.nnz);
@@ -1816,7 +1779,29 @@ static double device_norm(Ctx* ctx, SolverState& ss, const double* a, int64_t n)
 
 // The polynomial's matrix for each path and feec_pc_storage (the SELL copy
 // falls back to CSR where the pattern allows none), made on first use.
+// The multigrid / auxiliary-space preconditioner in place of the polynomial.
+template <class T>
+static void bind_hx(KktSystem& s, MinresWork<T>& w, bool fp32) {
+  if (w.graph && (w.hx != s.hx || w.hx_fp32 != fp32 || w.cheb_m != 0)) {
+    (void)hipGraphExecDestroy(w.graph);
+    w.graph = nullptr;
+  }
+  w.hx = s.hx;
+  w.hx_fp32 = fp32;
+  w.pc32 = false;
+  w.cheb_m = 0;
+  w.has_sell = false;
+  w.split = s.split;
+  w.a_nnz = s.A.nnz;
+  w.p_entries = 0;
+}
+
 static void bind_fp64(KktSystem& s, MinresWork<double>& w, int storage) {
+  if (s.hx) {
+    bind_hx(s, w, false);
+    return;
+  }
+  w.hx = nullptr;
   if (storage == FEEC_PC_STORE_SELL && s.P.nnz && !s.Psell64.p) kkt_refresh_sell64(s);
   if (storage == FEEC_PC_STORE_SELL && s.Psell64.p)
     bind_preconditioner(s, w, s.P.val.p, nullptr, s.sdinv64.p, true);
@@ -1826,6 +1811,11 @@ static void bind_fp64(KktSystem& s, MinresWork<double>& w, int storage) {
 
 static void bind_fp32(KktSystem& s, MinresWork<float>& w, int storage) {
   if (!s.val32.p) kkt_refresh_fp32(s);
+  if (s.hx) {
+    bind_hx(s, w, true);
+    return;
+  }
+  w.hx = nullptr;
   if (storage == FEEC_PC_STORE_CSR_HALF && s.P.nnz && !s.P16.p) kkt_refresh_half_csr(s);
   if (storage == FEEC_PC_STORE_SELL && (s.Psell.p || s.Psell_v16.p))
     bind_preconditioner(s, w, s.P32.p, nullptr, s.sdinv32.p, true);
@@ -1840,6 +1830,11 @@ static void bind_fp32(KktSystem& s, MinresWork<float>& w, int storage) {
 // without the SELL copy, the FP64 binding on P's CSR values.
 static void bind_pc32(KktSystem& s, MinresWork<double>& w) {
   if (!s.val32.p) kkt_refresh_fp32(s);
+  if (s.hx) {
+    bind_hx(s, w, true);
+    return;
+  }
+  w.hx = nullptr;
   const int m = (s.P.nnz > 0 && s.cheb_m > 0 && s.cheb.p) ? s.cheb_m : 0;
   if (m == 0 || !(s.Psell.p || s.Psell_v16.p) || !s.sdinv32.p) {
     bind_preconditioner(s, w, s.P.val.p);
@@ -2664,7 +2659,9 @@ void kkt_apply_preconditioner(KktSystem& s, int precision, int storage, const do
     FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag.p, N * sizeof(double), hipMemcpyDeviceToDevice, st));
     FEEC_HIP(hipMemcpyAsync(w.b.p, r, N * sizeof(double), hipMemcpyDeviceToDevice, st));
     Enq q{st};
-    if (w.cheb_m > 0)
+    if (w.hx)
+      hx_enqueue(q, *w.hx, (const double*)w.b.p, w.y.p, nullptr, w.part.p, w.hx_fp32);
+    else if (w.cheb_m > 0)
       cheb_apply<double>(q, w, w.b.p, w.y.p, nullptr, w.part.p);
     else
       k_jacobi<<<grid_for(N), TPB, 0, st>>>(w.b.p, w.dinv.p, w.y.p, N);
@@ -2678,7 +2675,9 @@ void kkt_apply_preconditioner(KktSystem& s, int precision, int storage, const do
   FEEC_HIP(hipMemcpyAsync(w.dinv.p, s.inv_diag32.p, N * sizeof(float), hipMemcpyDeviceToDevice, st));
   k_f64_to_f32<<<grid_for(N), TPB, 0, st>>>(r, N, w.b.p);
   Enq q{st};
-  if (w.cheb_m > 0)
+  if (w.hx)
+    hx_enqueue(q, *w.hx, (const float*)w.b.p, w.y.p, nullptr, w.part.p);
+  else if (w.cheb_m > 0)
     cheb_apply<float>(q, w, w.b.p, w.y.p, nullptr, w.part.p);
   else
     k_jacobi32<<<grid_for(N), TPB, 0, st>>>(w.b.p, w.dinv.p, w.y.p, N);

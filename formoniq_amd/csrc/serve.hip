@@ -261,6 +261,9 @@ feec_status feec_batch_run(feec_ctx* ctx, int nreq, const feec_request* reqs,
       base = *opts;
     else
       feec_solve_opts_default(&base);
+    // the lockstep runs advance the polynomial preconditioner of every system
+    // in one launch per phase; the multigrid cycle is per system
+    if (base.preconditioner == FEEC_PC_AUTO) base.preconditioner = FEEC_PC_CHEBYSHEV;
     for (int i = 0; i < nreq; ++i) check_request(reqs[i]);
     const int W = std::min(nreq, 8);
     std::vector<feec_ctx*> wctx(W, nullptr);

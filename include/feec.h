@@ -114,7 +114,25 @@ typedef struct feec_solve_opts {
                             1.2e-4 up to 96, else 6e-5 (default 0) */
   int32_t precond_storage; /* feec_pc_storage: how the matrix the polynomial runs on is
                               stored (default FEEC_PC_STORE_SELL) */
+  int32_t preconditioner; /* feec_preconditioner: the block preconditioner (default FEEC_PC_AUTO) */
+  int32_t mg_sweeps;      /* Jacobi smoothing sweeps before and after the coarse correction on
+                             each multigrid level (default 2) */
 } feec_solve_opts;
+
+/* The block preconditioner diag(B_{k-1}, B_k, I) of the mixed KKT system, B_j approximating
+ * hdif_gram(j)^-1 (elliptic.rs:29-47).
+ * FEEC_PC_MULTIGRID: the Hiptmair-Xu auxiliary-space preconditioner (hx.rs:150-275) with
+ *   geometric V-cycles of the nodal problem (multigrid.rs:55-213): iteration counts bounded
+ *   under refinement. It needs a Kuhn-grid mesh numbered as feec_kuhn_grid numbers it, a
+ *   geometry from Euclidean coordinates and grade <= 2; otherwise FEEC_E_UNSUPPORTED.
+ * FEEC_PC_CHEBYSHEV: a Chebyshev-Jacobi polynomial of degree cheb_degree on each block.
+ * FEEC_PC_AUTO: multigrid where it applies and the system is not small (at least 20,000
+ *   KKT rows), else Chebyshev. Batched solves (feec_batch_run) always run Chebyshev. */
+typedef enum {
+  FEEC_PC_AUTO = 0,
+  FEEC_PC_CHEBYSHEV = 1,
+  FEEC_PC_MULTIGRID = 2
+} feec_preconditioner;
 
 typedef struct feec_report {
   int64_t iters;           /* Krylov iterations over all refinements */

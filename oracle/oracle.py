@@ -39,6 +39,10 @@ MASS, DIF_TRIAL, DIF_TEST, DIF_BOTH = 0, 1, 2, 3
 #   21 the sigma-row sign flip left off the coupling block
 #   22 the sigma mass scaled by 2
 #   23 the u-u block (dif_both(k+1)) negated
+# and in the multigrid / auxiliary-space preconditioner:
+#   31 the midpoint prolongation weights (1/2, 1/2) replaced by (3/4, 1/4)
+#   32 the post-smoothing sweeps of the V-cycle left out
+#   33 the vector-nodal factor 1/(k+1)! replaced by 1/k!
 MUTATION = 0
 
 i64p = C.POINTER(C.c_int64)
@@ -848,3 +852,161 @@ def dirac_irk(c: WhitneyComplex, dt, nsteps, y0, stages=2):
         y = irk.step(y)
         energy.append(irk.energy(y))
     return y, np.array(energy)
+
+
+# ---------------------------------------------------------------------------
+# Mesh-independent block preconditioner: geometric multigrid on Kuhn grids
+# inside the Hiptmair-Xu auxiliary-space preconditioner
+#   VCycle::cycle           iterative/src/multigrid.rs (smooth, residual, restrict,
+#                           coarse correction, prolong, smooth; direct coarsest)
+#   Jacobi::smoother        iterative/src/precond.rs:89-94 (omega = 2/3)
+#   stationary::sweeps      iterative/src/stationary.rs:61-74
+#   AuxiliarySpace          iterative/src/aux_space.rs (B = S + sum Pi B_i Pi^T)
+#   hx_preconditioner       formoniq/src/hx.rs:244-275 (recursing in grade)
+#   vector_nodal_prolongation  formoniq/src/hx.rs:86-112
+#   RefinementTower         formoniq/src/multigrid.rs:55-138 (the reference refines
+#                           a base mesh; on a Kuhn grid of n cells per axis the
+#                           tower is recovered by coarsening to ceil(n/2), with
+#                           Galerkin coarse operators P^T A P)
+# ---------------------------------------------------------------------------
+MG_COARSE_MAX = 200  # a level with at most this many unknowns is solved directly
+MG_OMEGA = 2.0 / 3.0
+
+
+def _grid_coords(d, g):
+    ids = np.arange((g + 1) ** d)
+    return ids, np.stack([(ids // (g + 1) ** a) % (g + 1) for a in range(d)], axis=1)
+
+
+def kuhn_nodal_prolongation(d, g, interior=False):
+    """Nodal prolongation from the Kuhn grid of ceil(g/2) cells per axis to
+    that of g (vertex c at sum_a c_a (g+1)^a): fine vertex 2w + delta, delta in
+    {0,1}^d, is the midpoint of the coarse Kuhn edge (w, w + delta) and takes
+    1 (delta = 0) or 1/2, 1/2 of its ends. `interior`: rows the fine interior
+    vertices, columns the coarse vertices with 0 < w and 2 w <= g - 1 (the
+    interior of the coarse grid), entries elsewhere dropped."""
+    import scipy.sparse as sp
+    gc = (g + 1) // 2
+    ids, c = _grid_coords(d, g)
+    delta = c % 2
+    lo = (c - delta) // 2
+    hi = lo + delta
+    cid = lambda q: sum(q[:, a] * (gc + 1) ** a for a in range(d))
+    z = delta.sum(1) == 0
+    rows = np.concatenate([ids[z], ids[~z], ids[~z]])
+    cols = np.concatenate([cid(lo)[z], cid(lo)[~z], cid(hi)[~z]])
+    w_lo, w_hi = (0.75, 0.25) if MUTATION == 31 else (0.5, 0.5)
+    vals = np.concatenate([np.ones(z.sum()), np.full((~z).sum(), w_lo), np.full((~z).sum(), w_hi)])
+    P = sp.csr_matrix((vals, (rows, cols)), shape=((g + 1) ** d, (gc + 1) ** d))
+    if interior:
+        fin = np.all((c > 0) & (c < g), axis=1)
+        _, w = _grid_coords(d, gc)
+        cin = np.all((w > 0) & (2 * w <= g - 1), axis=1)
+        P = P[fin][:, cin]
+    return P.tocsr()
+
+
+class NodalVCycle:
+    """One V-cycle of the nodal problem A0 (hdif_gram(0) of the full or the
+    relative complex on the Kuhn grid of n cells per axis): levels while a
+    level has more than MG_COARSE_MAX unknowns, Galerkin coarse operators,
+    weighted Jacobi smoothing with `sweeps` sweeps before and after the coarse
+    correction, the coarsest level inverted directly."""
+
+    def __init__(self, A0, d, n, relative, sweeps=2):
+        self.sweeps = sweeps
+        self.levels = []
+        A = A0.tocsr()
+        g = n
+        while A.shape[0] > MG_COARSE_MAX and g >= 2:
+            P = kuhn_nodal_prolongation(d, g, relative)
+            self.levels.append((A, MG_OMEGA / A.diagonal(), P))
+            A = (P.T @ (A @ P)).tocsr()
+            g = (g + 1) // 2
+        self.coarse = np.linalg.inv(A.toarray())
+
+    def _cycle(self, i, r):
+        if i == len(self.levels):
+            return self.coarse @ r
+        A, dinv, P = self.levels[i]
+        x = np.zeros_like(r)
+        for _ in range(self.sweeps):
+            x = x + dinv * (r - A @ x)
+        x = x + P @ self._cycle(i + 1, P.T @ (r - A @ x))
+        for _ in range(self.sweeps if MUTATION != 32 else 0):
+            x = x + dinv * (r - A @ x)
+        return x
+
+    def apply(self, r):
+        return self._cycle(0, r)
+
+
+def vector_nodal_prolongation(cx: Complex, coords, k):
+    """hx.rs:86-112: the de Rham map of phi_a e_I, column I * N_0 + a (I the
+    colex k-subsets of the d axes): on a k-simplex sigma containing a,
+    (1/(k+1)!) <e_I, tau_sigma>, tau_sigma the wedge of sigma's edge vectors
+    from its first vertex. Identity at k = 0."""
+    import itertools
+    import scipy.sparse as sp
+    d = cx.dim
+    n0 = cx.nsimplices(0)
+    if k == 0:
+        return sp.identity(n0, format="csr")
+    simp = cx.skeleton(k)
+    E = coords[simp[:, 1:]] - coords[simp[:, :1]]
+    factor = 1.0 / math.factorial(k + (0 if MUTATION == 33 else 1))
+    rows, cols, vals = [], [], []
+    for bi, I in enumerate(itertools.combinations(range(d), k)):
+        comp = np.linalg.det(E[:, :, list(I)])
+        for p in range(k + 1):
+            rows.append(np.arange(len(simp)))
+            cols.append(bi * n0 + simp[:, p])
+            vals.append(factor * comp)
+    nb = math.comb(d, k)
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(len(simp), nb * n0))
+
+
+class HxPreconditioner:
+    """hx_preconditioner (hx.rs:244-275) with multigrid blocks (hx.rs:302-313):
+    B_k = S_k + D_{k-1} B_{k-1} D_{k-1}^T + Pi_k (V (x) I) Pi_k^T, B_0 = S_0 + V,
+    S_j = 2/3 diag(hdif_gram(j))^-1, V the nodal V-cycle."""
+
+    def __init__(self, c: WhitneyComplex, coords, k, n, sweeps=2, nodal=None):
+        import scipy.sparse as sp
+        self.S = MG_OMEGA / c.hdif_gram(k).diagonal()
+        self.nodal = nodal or NodalVCycle(c.hdif_gram(0).to_scipy(), c.dim, n, c.relative, sweeps)
+        self.grad = None
+        if k > 0 and c.ndofs(k - 1) > 0:
+            self.D = c.dif(k - 1).to_scipy().tocsr()
+            self.grad = HxPreconditioner(c, coords, k - 1, n, sweeps, self.nodal)
+        Pi = vector_nodal_prolongation(c.cx, coords, k)
+        n0 = c.cx.nsimplices(0)
+        self.Pi = []
+        for b in range(Pi.shape[1] // n0):
+            blk = Pi[:, b * n0:(b + 1) * n0]
+            if c.relative:
+                blk = blk[c.interior(k) == 1][:, c.interior(0) == 1]
+            self.Pi.append(sp.csr_matrix(blk))
+
+    def apply(self, r):
+        z = self.S * r
+        if self.grad is not None:
+            z = z + self.D @ self.grad.apply(self.D.T @ r)
+        for P in self.Pi:
+            z = z + P @ self.nodal.apply(P.T @ r)
+        return z
+
+
+def hx_block_preconditioner(c: WhitneyComplex, coords, k, n, nh, sweeps=2):
+    """The KKT block preconditioner diag(B_{k-1}, B_k, I) (elliptic.rs:29-47)
+    with HX blocks; a callable r -> z."""
+    ns, nu = c.ndofs(k - 1), c.ndofs(k)
+    hu = HxPreconditioner(c, coords, k, n, sweeps)
+    hs = HxPreconditioner(c, coords, k - 1, n, sweeps, hu.nodal) if ns else None
+
+    def prec(r):
+        parts = [hs.apply(r[:ns])] if ns else []
+        parts += [hu.apply(r[ns:ns + nu]), r[ns + nu:]]
+        return np.concatenate(parts)
+    return prec

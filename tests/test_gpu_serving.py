@@ -81,10 +81,11 @@ def test_batch_lockstep_is_bit_identical_to_single_solves():
 
 
 def run_server(reqs, inflight, budget=32 << 30):
+    # the polynomial preconditioner, as the batch runs it (single solves equal lockstep ones)
     opts = fa.ServerOpts()
     opts.hbm_budget_bytes = budget
     opts.max_inflight = inflight
-    opts.solve = fa.solve_opts()
+    opts.solve = fa.solve_opts(preconditioner=fa.PC_CHEBYSHEV)
     srv = fa.Server(opts)
     tickets = [srv.submit(r) for r in reqs]
     srv.drain()

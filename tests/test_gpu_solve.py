@@ -17,6 +17,8 @@ from _common import assert_csr_parity, both, kuhn_mesh
 
 pytestmark = pytest.mark.gpu
 
+CHEB = fa.PC_CHEBYSHEV  # the tests of the polynomial preconditioner ask for it by name
+
 
 def assert_kkt_parity(g, kkt, n, what):
     """KKT pattern bit-exact; values bit-identical except in the harmonic
@@ -208,7 +210,7 @@ def test_wide_band_falls_back_to_csr(precision):
     for c, x in meshes:
         cx = fa.Complex(3, c, nverts=nv)
         wc = fa.WhitneyComplex(fa.MeshLengthsSq.from_coords(cx, x))
-        out.append((cx, wc, fa.Problem(wc, 1, fa.solve_opts(precision=precision))))
+        out.append((cx, wc, fa.Problem(wc, 1, fa.solve_opts(precision=precision, preconditioner=CHEB))))
     ea = out[0][0].skeleton(1)  # sorted vertex pairs, colex order
     eb = out[1][0].skeleton(1)
     index_b = {(int(a), int(b)): j for j, (a, b) in enumerate(eb)}
@@ -231,7 +233,8 @@ def test_wide_band_falls_back_to_csr(precision):
     # the wide slices apply the same polynomial as the 32-bit-column CSR storages
     _, wc_b, prob_b = out[1]
     for store in (fa.PC_CSR_HALF, fa.PC_CSR):
-        p2 = fa.Problem(wc_b, 1, fa.solve_opts(precision=precision, precond_storage=store))
+        p2 = fa.Problem(wc_b, 1, fa.solve_opts(precision=precision, precond_storage=store,
+                                               preconditioner=CHEB))
         rep = p2.solve(fb_src)
         assert rep.converged and rep.residual <= 1e-10, (store, rep.as_dict())
         assert np.linalg.norm(sign * p2.solution()[1][match] - u0) <= 1e-8 * np.linalg.norm(u0)
@@ -489,17 +492,18 @@ def test_c2_kkt_pattern_at_full_size():
 
 
 def test_full_size_3d_64_agrees_across_precisions():
-    """The 3D Kuhn 64^3 k=1 workload at full size (2.1 M KKT rows): the
-    FP64_PC32 and FP64 solves each meet the 1e-10 certificate, and their
-    solutions agree to 1e-8 (a size-independent check of the mixed path)."""
+    """The 3D Kuhn 64^3 k=1 workload at full size (2.1 M KKT rows) with the
+    polynomial preconditioner: the FP64_PC32 and FP64 solves each meet the
+    1e-10 certificate, and their solutions agree to 1e-8 (a size-independent
+    check of the mixed path)."""
     g = fa.CartesianGrid(3, 64)
     cells, coords = g.triangulate()
     coords = g.jitter(coords, 0.05, 2)
     topo = fa.Complex(3, cells, nverts=len(coords))
     wc = fa.WhitneyComplex(fa.MeshLengthsSq.from_coords(topo, coords))
     f = fa.normal_vector(topo.nsimplices(1), 2)
-    _, u32, _, r32 = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64_PC32))
-    _, u64, _, r64 = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64))
+    _, u32, _, r32 = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64_PC32, preconditioner=CHEB))
+    _, u64, _, r64 = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64, preconditioner=CHEB))
     assert r32.converged and r32.residual <= 1e-10, r32.as_dict()
     assert r64.converged and r64.residual <= 1e-10, r64.as_dict()
     assert r32.iters < r64.iters
@@ -519,8 +523,8 @@ def test_fp64_pc32_continued_run():
         cx = fa.Complex(2, cells, nverts=coords.shape[0])
         wc = fa.WhitneyComplex(fa.MeshLengthsSq.from_coords(cx, coords))
         f = fa.normal_vector(cx.nsimplices(1), 31)
-        _, u64, _, _ = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64))
-        _, u, _, rep = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64_PC32))
+        _, u64, _, _ = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64, preconditioner=CHEB))
+        _, u, _, rep = fa.solve_source(wc, f, 1, fa.solve_opts(precision=fa.FP64_PC32, preconditioner=CHEB))
         print(n, rep.as_dict(), flush=True)
         assert rep.converged and rep.residual <= 1e-10, (n, rep.as_dict())
         assert np.linalg.norm(u - u64) <= 1e-8 * np.linalg.norm(u64), n
