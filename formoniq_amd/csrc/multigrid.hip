@@ -23,7 +23,7 @@
 // cycle for every copy. Since D_{j-1} D_{j-2} = 0, the recursion ends one grade
 // down: D_{j-1} B_{j-1} D_{j-1}^T = D_{j-1} (S_{j-1} + Pi_{j-1} V Pi_{j-1}^T) D_{j-1}^T.
 // Every V of one application is the same operator, so all of them run as one
-// multi-right-hand-side cycle ("slots", slot-major vectors of the nodal size).
+// multi-right-hand-side cycle over "slots".
 //
 // Hierarchy. The reference refines a coarse mesh into a tower. Here the fine
 // mesh is given, and when it is the Kuhn triangulation of the n^d box (vertex
@@ -37,6 +37,21 @@
 // coarse geometry is needed and jittered fine meshes are covered. Levels are
 // added while a level has more than kCoarseMax unknowns; the coarsest is
 // solved by its dense inverse, formed on the host by Cholesky.
+//
+// Layout. Each level lives on its full vertex box ((g+1)^d vertices for g
+// cells per axis). Its operator is stored by Kuhn-stencil diagonals: the
+// neighbours of vertex i are i +- sum_{a in delta} stride_a for the 2^d - 1
+// nonempty delta in {0,1}^d (every such diagonal of a box is an edge of the
+// Kuhn triangulation, and the Galerkin product of the Kuhn-grid prolongation
+// keeps that stencil), A[s * nb + i] for stencil point s (0 the diagonal,
+// 1..2^d-1 the +delta, 2^d..2^{d+1}-2 the -delta), zero where the neighbour
+// is not an unknown. Vertices that are not unknowns (the boundary of the
+// relative complex, their coarse counterparts) are "dead": zero rows and
+// columns and a zero smoother entry, so they stay zero through the cycle.
+// Slot vectors are interleaved, x[i * nv + slot], and the cycle's kernels run
+// one thread per (vertex, slot), so every vector access of a wavefront is
+// contiguous and no column index is read. The prolongation and restriction
+// are computed from grid coordinates, without matrices.
 //
 // Precision. The FP64 path runs every piece in FP64; the mixed paths in FP32
 // (the FP64_PC32 solve rounds r to FP32 on entry and widens z on exit).
@@ -56,8 +71,10 @@ namespace {
 
 constexpr int TPB = 256;
 constexpr int NBLK = 4096;
-constexpr int MAXNV = 16;           // slots of one multi-right-hand-side cycle
-constexpr int64_t kCoarseMax = 200;  // largest level solved directly
+constexpr int MAXNV = 16;               // slots of one multi-right-hand-side cycle
+constexpr int MAXC = 8;                 // values per entry of a transfer matrix
+constexpr int64_t kCoarseMax = 200;     // largest level solved directly (unknowns)
+constexpr int64_t kDenseMaxBox = 1024;  // largest coarsest box
 constexpr double kOmega = 2.0 / 3.0;
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
@@ -67,95 +84,166 @@ inline unsigned red_grid(int64_t n) {
 
 __device__ __forceinline__ bool skip(const MinresS* S) { return S && S->done; }
 
-// ---------------------------------------------------------------------------
-// multi-vector kernels: nv slots of length n, slot-major (v * n + i)
-// ---------------------------------------------------------------------------
-template <class TP>
-__global__ void __launch_bounds__(TPB) k_mg_scale(const TP* __restrict__ dinv, const TP* __restrict__ r,
-                                                  TP* __restrict__ x, int64_t n, int nv,
-                                                  const MinresS* S) {
-  if (skip(S)) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const TP d = dinv[i];
-  for (int v = 0; v < nv; ++v) x[v * n + i] = d * r[v * n + i];
+// A level's vertex box, passed by value.
+struct Box {
+  int g = 0;   // cells per axis
+  int nb = 0;  // (g+1)^d vertices
+  int st[4] = {0, 0, 0, 0};
+};
+
+// Linear offset of stencil point s on a box of strides st.
+template <int D>
+__device__ __host__ __forceinline__ int stencil_offset(int s, const int* st) {
+  constexpr int H = (1 << D) - 1;
+  const int mask = s <= H ? s : s - H;
+  int o = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+    if (mask & (1 << a)) o += st[a];
+  return s == 0 ? 0 : (s <= H ? o : -o);
 }
 
-// xo = x + dinv (r - A x)   (mode 0: one Jacobi sweep)
-// e  = r - A x               (mode 1: residual)
+// Stencil point of the coordinate difference `diff` (entries in {-1, 0, 1}),
+// or -1 when it is not a Kuhn edge (mixed signs).
+template <int D>
+__device__ __host__ __forceinline__ int stencil_point(const int* diff) {
+  int pos = 0, neg = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    if (diff[a] == 1) pos |= 1 << a;
+    if (diff[a] == -1) neg |= 1 << a;
+  }
+  if (pos && neg) return -1;
+  return neg ? neg + (1 << D) - 1 : pos;
+}
+
+// ---------------------------------------------------------------------------
+// the V-cycle on stencil levels: one thread per (vertex, slot)
+// ---------------------------------------------------------------------------
+// x = dinv r (the first sweep from x = 0)
 template <class TP>
-__global__ void __launch_bounds__(TPB) k_mg_rowop(const int64_t* __restrict__ rp,
-                                                  const int32_t* __restrict__ col,
-                                                  const TP* __restrict__ val,
-                                                  const TP* __restrict__ dinv,
+__global__ void __launch_bounds__(TPB) k_st_scale(const TP* __restrict__ dinv, const TP* __restrict__ r,
+                                                  TP* __restrict__ x, unsigned nb, unsigned nv,
+                                                  const MinresS* S) {
+  if (skip(S)) return;
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * nv) return;
+  x[t] = dinv[t / nv] * r[t];
+}
+
+// mode 0: out = x + dinv (r - A x)     (one weighted Jacobi sweep)
+// mode 1: out = r - A x, 0 on dead rows (the residual)
+template <class TP, int D>
+__global__ void __launch_bounds__(TPB) k_st_sweep(const TP* __restrict__ A, const TP* __restrict__ dinv,
                                                   const TP* __restrict__ r, const TP* __restrict__ x,
-                                                  TP* __restrict__ out, int64_t n, int nv, int mode,
+                                                  TP* __restrict__ out, Box b, unsigned nv, int mode,
                                                   const MinresS* S) {
+  constexpr int NS = (1 << (D + 1)) - 1;
   if (skip(S)) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  TP acc[MAXNV];
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned nb = (unsigned)b.nb;
+  if (t >= nb * nv) return;
+  const unsigned i = t / nv;
+  const unsigned v = t - i * nv;
+  TP acc = TP(0);
 #pragma unroll
-  for (int v = 0; v < MAXNV; ++v) acc[v] = TP(0);
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-    const int64_t c = col[p];
-    const TP a = val[p];
-#pragma unroll
-    for (int v = 0; v < MAXNV; ++v)
-      if (v < nv) acc[v] += a * x[v * n + c];
+  for (int s = 0; s < NS; ++s) {
+    const TP a = A[(size_t)s * nb + i];
+    int j = (int)i + stencil_offset<D>(s, b.st);
+    j = j < 0 ? 0 : (j >= (int)nb ? (int)nb - 1 : j);  // a is 0 for a neighbour outside the box
+    acc += a * x[(unsigned)j * nv + v];
   }
-  const TP d = mode == 0 ? dinv[i] : TP(0);
-#pragma unroll
-  for (int v = 0; v < MAXNV; ++v) {
-    if (v >= nv) break;
-    const TP res = r[v * n + i] - acc[v];
-    out[v * n + i] = mode == 0 ? x[v * n + i] + d * res : res;
-  }
+  const TP d = dinv[i];
+  const TP res = r[t] - acc;
+  out[t] = mode == 0 ? x[t] + d * res : (d != TP(0) ? res : TP(0));
 }
 
-// y = M x (add = 0) or y += M x (add = 1), M an (nr x nc) CSR, x of stride nc.
-template <class TP>
-__global__ void __launch_bounds__(TPB) k_mg_spmv(const int64_t* __restrict__ rp,
-                                                 const int32_t* __restrict__ col,
-                                                 const TP* __restrict__ val, const TP* __restrict__ x,
-                                                 TP* __restrict__ y, int64_t nr, int64_t nc, int nv,
-                                                 int add, const MinresS* S) {
+// rc = P^T ef on the live coarse vertices (0 elsewhere): coarse vertex w
+// collects its fine children 2w (weight 1) and 2w +- delta (weight 1/2).
+template <class TP, int D>
+__global__ void __launch_bounds__(TPB) k_st_restrict(const TP* __restrict__ ef, TP* __restrict__ rc,
+                                                     const uint8_t* __restrict__ live_c, Box fb, Box cb,
+                                                     unsigned nv, const MinresS* S) {
+  constexpr int H = (1 << D) - 1;
   if (skip(S)) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nr) return;
-  TP acc[MAXNV];
-#pragma unroll
-  for (int v = 0; v < MAXNV; ++v) acc[v] = TP(0);
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-    const int64_t c = col[p];
-    const TP a = val[p];
-#pragma unroll
-    for (int v = 0; v < MAXNV; ++v)
-      if (v < nv) acc[v] += a * x[v * nc + c];
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)cb.nb * nv) return;
+  const unsigned w = t / nv;
+  const unsigned v = t - w * nv;
+  if (!live_c[w]) {
+    rc[t] = TP(0);
+    return;
   }
+  int wc[D];
+  {
+    unsigned r = w;
 #pragma unroll
-  for (int v = 0; v < MAXNV; ++v) {
-    if (v >= nv) break;
-    y[v * nr + i] = add ? y[v * nr + i] + acc[v] : acc[v];
+    for (int a = 0; a < D; ++a) {
+      wc[a] = (int)(r % (unsigned)(cb.g + 1));
+      r /= (unsigned)(cb.g + 1);
+    }
   }
+  TP acc = TP(0);
+#pragma unroll
+  for (int s = 0; s <= 2 * H; ++s) {
+    const int mask = s <= H ? s : s - H;
+    const int sg = s <= H ? 1 : -1;
+    bool in = true;
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const int c = 2 * wc[a] + ((mask >> a) & 1) * sg;
+      in = in && c >= 0 && c <= fb.g;
+      k += c * fb.st[a];
+    }
+    if (in) acc += (s == 0 ? TP(1) : TP(0.5)) * ef[(unsigned)k * nv + v];
+  }
+  rc[t] = acc;
 }
 
-// x = C r, C dense (n x n, row-major): one block per row, all slots.
+// xf += P xc on the live fine vertices (dinv_f != 0).
+template <class TP, int D>
+__global__ void __launch_bounds__(TPB) k_st_prolong(const TP* __restrict__ xc, TP* __restrict__ xf,
+                                                    const TP* __restrict__ dinv_f, Box fb, Box cb,
+                                                    unsigned nv, const MinresS* S) {
+  if (skip(S)) return;
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)fb.nb * nv) return;
+  const unsigned k = t / nv;
+  const unsigned v = t - k * nv;
+  if (dinv_f[k] == TP(0)) return;
+  unsigned r = k;
+  int lo = 0, hi = 0;
+  bool odd = false;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const int c = (int)(r % (unsigned)(fb.g + 1));
+    r /= (unsigned)(fb.g + 1);
+    lo += (c >> 1) * cb.st[a];
+    hi += ((c >> 1) + (c & 1)) * cb.st[a];
+    odd = odd || (c & 1);
+  }
+  const TP add = odd ? TP(0.5) * (xc[(unsigned)lo * nv + v] + xc[(unsigned)hi * nv + v])
+                     : xc[(unsigned)lo * nv + v];
+  xf[t] += add;
+}
+
+// x = C r on the coarsest box, C dense (nb x nb, row-major, zero on dead
+// rows and columns): one block per row, all slots.
 template <class TP>
-__global__ void __launch_bounds__(TPB) k_mg_dense(const TP* __restrict__ C, const TP* __restrict__ r,
-                                                  TP* __restrict__ x, int64_t n, int nv,
-                                                  const MinresS* S) {
+__global__ void __launch_bounds__(TPB) k_st_dense(const TP* __restrict__ C, const TP* __restrict__ r,
+                                                  TP* __restrict__ x, int nb, int nv, const MinresS* S) {
   if (skip(S)) return;
   __shared__ TP sh[TPB / 64][MAXNV];
-  const int64_t i = blockIdx.x;
+  const int i = blockIdx.x;
   TP acc[MAXNV];
 #pragma unroll
   for (int v = 0; v < MAXNV; ++v) acc[v] = TP(0);
-  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
-    const TP c = C[i * n + j];
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+    const TP c = C[(size_t)i * nb + j];
 #pragma unroll
     for (int v = 0; v < MAXNV; ++v)
-      if (v < nv) acc[v] += c * r[v * n + j];
+      if (v < nv) acc[v] += c * r[j * nv + v];
   }
 #pragma unroll
   for (int v = 0; v < MAXNV; ++v) {
@@ -169,74 +257,166 @@ __global__ void __launch_bounds__(TPB) k_mg_dense(const TP* __restrict__ C, cons
   if (threadIdx.x < (unsigned)nv) {
     TP t = TP(0);
     for (int w = 0; w < TPB / 64; ++w) t += sh[w][threadIdx.x];
-    x[threadIdx.x * n + i] = t;
+    x[i * nv + threadIdx.x] = t;
   }
 }
 
 // ---------------------------------------------------------------------------
-// HX pieces on the KKT vector
+// operators of the hierarchy (FP64)
 // ---------------------------------------------------------------------------
-// z = s r on the preconditioned rows (s = 2/3 inv diag), z = r on the border rows.
-template <class TIN, class TP>
-__global__ void k_hx_init(const TIN* __restrict__ r, const TP* __restrict__ s, TP* __restrict__ z,
-                          int64_t n, int64_t N, const MinresS* S) {
-  if (skip(S)) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  z[i] = i < n ? s[i] * (TP)r[i] : (TP)r[i];
+__global__ void k_scatter_map(const double* __restrict__ val, const int32_t* __restrict__ map, int64_t n,
+                              double* __restrict__ out) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p < n) out[map[p]] = val[p];
 }
 
-// y[v * nr + row] = sum_p val[p * C + v] x[col[p]] for v < C: the C components
-// of a transposed vector-nodal interpolation (or, with C = 1, any CSR).
-template <class TX, class TP>
-__global__ void k_hx_gather(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
-                            const TP* __restrict__ val, int C, const TX* __restrict__ x,
-                            TP* __restrict__ y, int64_t nr, const MinresS* S) {
-  if (skip(S)) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nr) return;
-  TP acc[MAXNV];
-#pragma unroll
-  for (int v = 0; v < MAXNV; ++v) acc[v] = TP(0);
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-    const TP xv = (TP)x[col[p]];
-#pragma unroll
-    for (int v = 0; v < MAXNV; ++v)
-      if (v < C) acc[v] += val[p * C + v] * xv;
-  }
-  for (int v = 0; v < C; ++v) y[v * nr + i] = acc[v];
+// dinv = omega / A_ii on live rows, 0 on dead ones.
+__global__ void k_st_dinv(const double* __restrict__ A, const uint8_t* __restrict__ live, int nb,
+                          double omega, double* __restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) dinv[i] = live[i] && A[i] != 0.0 ? omega / A[i] : 0.0;
 }
 
-// y[row] = (a ? a[row] b[row] : y[row] if add else 0) + sum_p sum_v val[p * C + v] x[v * nx + col[p]]
-template <class TP>
-__global__ void k_hx_scatter(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
-                             const TP* __restrict__ val, int C, const TP* __restrict__ x, int64_t nx,
-                             const TP* __restrict__ a, const TP* __restrict__ b,
-                             TP* __restrict__ y, int64_t nr, int add, const MinresS* S) {
-  if (skip(S)) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nr) return;
-  TP acc = TP(0);
-  if (rp)
-    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-      const int64_t c = col[p];
-      for (int v = 0; v < C; ++v) acc += val[p * C + v] * x[v * nx + c];
+// A_c = P^T A_f P on the live coarse vertices, one thread per coarse vertex:
+// for each fine child k of w (weight 1 or 1/2), each fine stencil neighbour
+// l of k and each coarse parent p of l, the term w_k A_f(k, l) w_l lands on
+// the coarse stencil point of p - w. bad[w] = 1 if a term falls outside the
+// stencil (never, for the Kuhn prolongation; checked once at set-up).
+template <int D>
+__global__ void k_st_galerkin(const double* __restrict__ Af, const uint8_t* __restrict__ live_f,
+                              const uint8_t* __restrict__ live_c, Box fb, Box cb, double* __restrict__ Ac,
+                              uint8_t* __restrict__ bad) {
+  constexpr int H = (1 << D) - 1;
+  constexpr int NS = 2 * H + 1;
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= cb.nb) return;
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  uint8_t flag = 0;
+  if (live_c[w]) {
+    int wc[D];
+    {
+      int r = w;
+      for (int a = 0; a < D; ++a) {
+        wc[a] = r % (cb.g + 1);
+        r /= (cb.g + 1);
+      }
     }
-  if (a)
-    acc += a[i] * b[i];
-  else if (add)
-    acc += y[i];
-  y[i] = acc;
+    for (int cs = 0; cs < NS; ++cs) {  // fine child 2w + (+-delta)
+      const int cm = cs <= H ? cs : cs - H;
+      const int csg = cs <= H ? 1 : -1;
+      int kc[D];
+      bool in = true;
+      int k = 0;
+      for (int a = 0; a < D; ++a) {
+        kc[a] = 2 * wc[a] + ((cm >> a) & 1) * csg;
+        in = in && kc[a] >= 0 && kc[a] <= fb.g;
+        k += kc[a] * fb.st[a];
+      }
+      if (!in || !live_f[k]) continue;
+      const double wk = cs == 0 ? 1.0 : 0.5;
+      for (int s = 0; s < NS; ++s) {  // fine neighbour l = k + (+-delta)
+        const double a_kl = Af[(size_t)s * fb.nb + k];
+        if (a_kl == 0.0) continue;
+        const int sm = s <= H ? s : s - H;
+        const int ssg = s <= H ? 1 : -1;
+        int lo[D], hi[D];
+        bool odd = false;
+        for (int a = 0; a < D; ++a) {
+          const int lc = kc[a] + ((sm >> a) & 1) * ssg;
+          lo[a] = lc >> 1;
+          hi[a] = (lc >> 1) + (lc & 1);
+          odd = odd || (lc & 1);
+        }
+        for (int which = 0; which < (odd ? 2 : 1); ++which) {
+          int pidx = 0, diff[D];
+          for (int a = 0; a < D; ++a) {
+            const int pc = which == 0 ? lo[a] : hi[a];
+            pidx += pc * cb.st[a];
+            diff[a] = pc - wc[a];
+          }
+          if (!live_c[pidx]) continue;
+          const int slot = stencil_point<D>(diff);
+          if (slot < 0) {
+            flag = 1;
+            continue;
+          }
+          const double term = wk * a_kl * (odd ? 0.5 : 1.0);
+#pragma unroll
+          for (int q = 0; q < NS; ++q)  // keeps the accumulators in registers
+            if (q == slot) acc[q] += term;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) Ac[(size_t)s * cb.nb + w] = acc[s];
+  bad[w] = flag;
 }
 
-// slot = r (TIN -> TP), or y += x (elementwise) for the grade-0 V terms.
-template <class TIN, class TP>
-__global__ void k_hx_copy(const TIN* __restrict__ x, TP* __restrict__ y, int64_t n, int add,
-                          const MinresS* S) {
+// ---------------------------------------------------------------------------
+// HX transfers between the KKT vector and the level-0 slots
+// ---------------------------------------------------------------------------
+// y[row * nv + slot0 + v] = sum_p val[p * C + v] x[col[p]] for v < C; LPR
+// lanes share a row (entries strided over them, summed by shuffles).
+template <class TX, class TP, int LPR>
+__global__ void __launch_bounds__(TPB) k_hx_gather(const int64_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col,
+                                                   const TP* __restrict__ val, int C,
+                                                   const TX* __restrict__ x, TP* __restrict__ y, int nv,
+                                                   int slot0, int64_t nrows, const MinresS* S) {
+  if (skip(S)) return;
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gid / LPR;
+  const int lane = (int)(gid % LPR);
+  TP acc[MAXC];
+#pragma unroll
+  for (int v = 0; v < MAXC; ++v) acc[v] = TP(0);
+  if (row < nrows) {
+    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LPR) {
+      const TP xv = (TP)x[col[p]];
+#pragma unroll
+      for (int v = 0; v < MAXC; ++v)
+        if (v < C) acc[v] += val[p * C + v] * xv;
+    }
+  }
+  if (LPR > 1) {
+#pragma unroll
+    for (int v = 0; v < MAXC; ++v) {
+      TP t = acc[v];
+#pragma unroll
+      for (int off = LPR / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+      acc[v] = t;
+    }
+  }
+  if (row < nrows && lane == 0)
+    for (int v = 0; v < C; ++v) y[row * nv + slot0 + v] = acc[v];
+}
+
+// y[row] = (a ? a[row] b[row] : add ? y[row] : 0)
+//          + sum_p sum_v val[p * C + v] X(col[p], v),
+// X(c, v) = x0[c * nv + slot0 + v], except X(c, 0) = x0[..] + dl[c] r0[..]
+// when `grad` (the grade-0 term S_0 t + V t of the gradient correction).
+template <class TP, class TA, class TB>
+__global__ void __launch_bounds__(TPB) k_hx_out(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                const TP* __restrict__ val, int C, int grad,
+                                                const TP* __restrict__ x0, const TP* __restrict__ r0,
+                                                const TP* __restrict__ dl, int nv, int slot0,
+                                                const TA* __restrict__ a, const TB* __restrict__ b,
+                                                TP* __restrict__ y, int64_t nrows, int add, const MinresS* S) {
   if (skip(S)) return;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  y[i] = add ? y[i] + (TP)x[i] : (TP)x[i];
+  if (i >= nrows) return;
+  TP acc = a ? (TP)a[i] * (TP)b[i] : (add ? y[i] : TP(0));
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+    const int64_t base = (int64_t)col[p] * nv + slot0;
+    TP x = x0[base];
+    if (grad) x += dl[col[p]] * r0[base];
+    acc += val[p * C] * x;
+    for (int v = 1; v < C; ++v) acc += val[p * C + v] * x0[base + v];
+  }
+  y[i] = acc;
 }
 
 __device__ __forceinline__ double dev_block_sum(double v, double* sh) {
@@ -250,17 +430,18 @@ __device__ __forceinline__ double dev_block_sum(double v, double* sh) {
   return r;
 }
 
-// z (TOUT) = zt (TP) and the partial sums of r . z (red_grid(N) blocks).
+// z = zt on the preconditioned rows, z = r on the border rows, and the
+// partial sums of r . z (red_grid(N) blocks).
 template <class TIN, class TP, class TOUT>
 __global__ void __launch_bounds__(TPB) k_hx_final(const TIN* __restrict__ r, const TP* __restrict__ zt,
-                                                  TOUT* __restrict__ z, int64_t N, const MinresS* S,
+                                                  TOUT* __restrict__ z, int64_t n, int64_t N, const MinresS* S,
                                                   double* __restrict__ part) {
   if (skip(S)) return;
   __shared__ double sh[TPB / 64];
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const TOUT v = (TOUT)zt[i];
+    const TOUT v = i < n ? (TOUT)zt[i] : (TOUT)r[i];
     z[i] = v;
     acc += (double)r[i] * (double)v;
   }
@@ -272,17 +453,6 @@ template <class TS, class TD>
 __global__ void k_convert(const TS* __restrict__ a, TD* __restrict__ b, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) b[i] = (TD)a[i];
-}
-
-__global__ void k_scaled_inverse_diag(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                      const double* __restrict__ val, int64_t n, double omega,
-                                      double* __restrict__ out) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double d = 0.0;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
-    if (col[p] == i) d = val[p];
-  out[i] = d != 0.0 ? omega / d : 0.0;
 }
 
 __global__ void k_scale(const double* __restrict__ a, int64_t n, double s, double* __restrict__ b) {
@@ -300,19 +470,6 @@ struct HostCsr {
   std::vector<double> val;  // nnz * C
   int C = 1;
 };
-
-void upload(Ctx* ctx, const HostCsr& h, Csr& out) {
-  out.ctx = ctx;
-  out.nrows = h.nrows;
-  out.ncols = h.ncols;
-  out.nnz = (int64_t)h.col.size();
-  out.row_ptr.alloc(ctx, h.nrows + 1);
-  out.col.alloc(ctx, std::max<int64_t>(out.nnz, 1));
-  out.val.alloc(ctx, std::max<size_t>(h.val.size(), 1));
-  copy_to_device(ctx, out.row_ptr.p, h.rp.data(), (h.nrows + 1) * 8);
-  if (out.nnz) copy_to_device(ctx, out.col.p, h.col.data(), out.nnz * 4);
-  if (!h.val.empty()) copy_to_device(ctx, out.val.p, h.val.data(), h.val.size() * 8);
-}
 
 // Transpose of a host CSR with C values per entry.
 HostCsr transpose(const HostCsr& a) {
@@ -339,68 +496,6 @@ int64_t ipow(int64_t b, int e) {
   int64_t r = 1;
   for (int i = 0; i < e; ++i) r *= b;
   return r;
-}
-
-// Nodal prolongation of the Kuhn grid of g cells per axis from the grid of
-// ceil(g/2): fine vertex 2w + delta takes 1 (delta = 0) or 1/2, 1/2 of the
-// coarse vertices w and w + delta. With `interior`, rows and columns are the
-// interior vertices (all coordinates in 1..g-1 / coarse 2w <= g - 1), numbered
-// in vertex order, and entries on other coarse vertices are dropped.
-HostCsr kuhn_prolongation(int d, int g, bool interior) {
-  const int gc = (g + 1) / 2;
-  const int64_t nf = ipow(g + 1, d), nc = ipow(gc + 1, d);
-  std::vector<int32_t> fmap(nf, -1), cmap(nc, -1);
-  int64_t nfi = 0, nci = 0;
-  for (int64_t v = 0; v < nf; ++v) {
-    bool in = true;
-    int64_t r = v;
-    for (int a = 0; a < d; ++a, r /= (g + 1)) {
-      const int64_t c = r % (g + 1);
-      in = in && c > 0 && c < g;
-    }
-    if (!interior || in) fmap[v] = (int32_t)nfi++;
-  }
-  for (int64_t w = 0; w < nc; ++w) {
-    bool in = true;
-    int64_t r = w;
-    for (int a = 0; a < d; ++a, r /= (gc + 1)) {
-      const int64_t c = r % (gc + 1);
-      in = in && c > 0 && 2 * c <= g - 1;
-    }
-    if (!interior || in) cmap[w] = (int32_t)nci++;
-  }
-  HostCsr P;
-  P.nrows = nfi;
-  P.ncols = nci;
-  P.rp.assign(nfi + 1, 0);
-  P.col.reserve(nfi * 2);
-  P.val.reserve(nfi * 2);
-  int64_t row = 0;
-  for (int64_t v = 0; v < nf; ++v) {
-    if (fmap[v] < 0) continue;
-    int64_t lo = 0, hi = 0, stride = 1, r = v;
-    bool odd = false;
-    for (int a = 0; a < d; ++a, r /= (g + 1), stride *= (gc + 1)) {
-      const int64_t c = r % (g + 1), dl = c & 1;
-      lo += (c >> 1) * stride;
-      hi += ((c >> 1) + dl) * stride;
-      odd = odd || dl;
-    }
-    auto push = [&](int64_t w, double x) {
-      if (cmap[w] >= 0) {
-        P.col.push_back(cmap[w]);
-        P.val.push_back(x);
-      }
-    };
-    if (!odd) {
-      push(lo, 1.0);
-    } else if (lo < hi) {
-      push(lo, 0.5);
-      push(hi, 0.5);
-    }
-    P.rp[++row] = (int64_t)P.col.size();
-  }
-  return P;
 }
 
 // Dense inverse of an SPD matrix by Cholesky; false if it is not SPD.
@@ -481,43 +576,51 @@ int kuhn_grid_n(Mesh& m) {
 // the preconditioner object
 // ---------------------------------------------------------------------------
 struct MgLevel {
-  int grid = 0;  // cells per axis
-  int64_t n = 0;
-  Csr A;                  // FP64 operator
+  Box box;
+  int64_t live = 0;     // unknowns
+  DBuf<uint8_t> alive;  // per box vertex
+  DBuf<double> A;       // stencil diagonals, NS * nb
   DBuf<float> A32;
-  DBuf<double> dinv;      // omega / diag(A)
+  DBuf<double> dinv;  // omega / diag on live rows, 0 on dead ones
   DBuf<float> dinv32;
-  Csr P, R;               // from the next coarser level (n x n_c) and its transpose
-  DBuf<float> P32, R32;
-  // slot vectors: rhs, iterate (two), residual
+  DBuf<uint8_t> bad;  // Galerkin check (coarse levels)
+  // slot vectors (nb * nv): rhs, two iterates, residual
   DBuf<double> r64, x64, y64, e64;
   DBuf<float> r32, x32, y32, e32;
 };
 
+// A CSR transfer matrix with C values per entry, FP64 and FP32.
+struct XCsr {
+  Csr m;
+  DBuf<float> v32;
+  int C = 1;
+  double avg_row = 0.0;
+};
+
 struct HxBlock {
-  int j = -1;           // grade of the block
+  int j = -1;  // grade of the block
   int64_t off = 0, n = 0;
-  // gradient part (j >= 1): D_{j-1} and D_{j-1}^T, and the smoother of grade j-1
-  Csr D, DT;
-  DBuf<float> D32, DT32;
-  DBuf<double> sl;      // 2/3 inv diag hdif_gram(j-1), on the (j-1)-DOFs
-  DBuf<float> sl32;
-  DBuf<double> t64, u64;  // D^T r and the grade-(j-1) correction
-  DBuf<float> t32, u32;
-  // vector-nodal interpolations: of grade j (j >= 1) and of grade j-1 (j-1 >= 1)
-  int C = 0, Cl = 0;
-  Csr Pi, PiT, Pil, PilT;  // values C (Cl) per entry
-  DBuf<float> Pi32, PiT32, Pil32, PilT32;
-  int slot_v = -1;      // first slot of grade j's V terms (or the single slot at j = 0)
-  int slot_t = -1;      // first slot of the gradient's V terms
+  // into the level-0 slots from the block's residual: j = 0 the nodal DOF
+  // map, j = 1 [D_0^T | Pi_1^T] (the gradient slot, then the Pi slots), j = 2 Pi_2^T
+  XCsr g_main;
+  int slot_main = 0;
+  // back onto the block: j = 0 the map, j = 1 [D_0 | Pi_1] (gradient), j = 2 Pi_2
+  XCsr o_main;
+  // j = 2 only: t = D_1^T r, Pi_1^T t into the slots, u = S_1 t + Pi_1 (V ..), z += D_1 u
+  XCsr g_t, g_tl, o_ul, o_d;
+  int slot_tl = 0;
+  DBuf<double> sl, t64, u64;  // sl = S_1 on the grade-1 DOFs
+  DBuf<float> sl32, t32, u32;
 };
 
 struct Hx {
   Ctx* ctx = nullptr;
   int d = 0, k = 0, bc = 0, grid = 0, sweeps = 2;
-  int64_t n = 0, N = 0, nv_nodes = 0;
-  int nv = 0;  // slots
+  int64_t n = 0, N = 0;
+  int nv = 0;              // slots
   std::vector<MgLevel> L;  // finest first; the last is solved densely
+  DBuf<int32_t> l0_map;    // level-0 operator CSR entry -> stencil position
+  int64_t l0_nnz = 0;
   DBuf<double> cinv;
   DBuf<float> cinv32;
   std::vector<HxBlock> blocks;
@@ -532,11 +635,54 @@ int hx_launches_per_apply(const Hx& h) { return (int)h.launches; }
 
 namespace {
 
+void upload(Ctx* ctx, const HostCsr& h, XCsr& out) {
+  Csr& c = out.m;
+  c.ctx = ctx;
+  c.nrows = h.nrows;
+  c.ncols = h.ncols;
+  c.nnz = (int64_t)h.col.size();
+  c.row_ptr.alloc(ctx, h.nrows + 1);
+  c.col.alloc(ctx, std::max<int64_t>(c.nnz, 1));
+  c.val.alloc(ctx, std::max<size_t>(h.val.size(), 1));
+  copy_to_device(ctx, c.row_ptr.p, h.rp.data(), (h.nrows + 1) * 8);
+  if (c.nnz) copy_to_device(ctx, c.col.p, h.col.data(), c.nnz * 4);
+  if (!h.val.empty()) copy_to_device(ctx, c.val.p, h.val.data(), h.val.size() * 8);
+  out.C = h.C;
+  out.avg_row = h.nrows ? (double)c.nnz / (double)h.nrows : 0.0;
+  out.v32.alloc(ctx, std::max<size_t>(h.val.size(), 1));
+  if (!h.val.empty())
+    k_convert<double, float><<<grid_for((int64_t)h.val.size()), TPB, 0, ctx->stream>>>(
+        c.val.p, out.v32.p, (int64_t)h.val.size());
+  FEEC_HIP(hipGetLastError());
+}
+
+HostCsr host_of(Ctx* ctx, const Csr& a) {
+  HostCsr h;
+  h.nrows = a.nrows;
+  h.ncols = a.ncols;
+  h.rp.resize(a.nrows + 1);
+  h.col.resize(a.nnz);
+  h.val.resize(a.nnz);
+  copy_to_host(ctx, h.rp.data(), a.row_ptr.p, (a.nrows + 1) * 8);
+  if (a.nnz) {
+    copy_to_host(ctx, h.col.data(), a.col.p, a.nnz * 4);
+    copy_to_host(ctx, h.val.data(), a.val.p, a.nnz * 8);
+  }
+  return h;
+}
+
+void to32(Ctx* ctx, const double* a, int64_t n, DBuf<float>& out) {
+  if ((int64_t)out.n != std::max<int64_t>(n, 1)) out.alloc(ctx, std::max<int64_t>(n, 1));
+  if (n) k_convert<double, float><<<grid_for(n), TPB, 0, ctx->stream>>>(a, out.p, n);
+  FEEC_HIP(hipGetLastError());
+}
+
 // Vector-nodal interpolation of grade j (formoniq/src/hx.rs:86-112): row of
 // the j-simplex sigma, entry per vertex a of sigma with C = C(d, j) values,
 // value I = (1/(j+1)!) det(E_I), E the edge vectors of sigma from its first
 // vertex and E_I their components I (colex j-subsets of the d axes).
-// Rows and columns restricted to the complex's DOFs (relative: interior).
+// rowmap: the block's DOF of each j-simplex (-1: none; null: identity);
+// colmap: the column of each vertex (-1: dropped; null: the vertex itself).
 HostCsr vector_nodal(Mesh& m, const std::vector<double>& coords, int j, const int32_t* rowmap,
                      const int32_t* colmap, int64_t nrows, int64_t ncols) {
   const int d = m.d;
@@ -562,7 +708,7 @@ HostCsr vector_nodal(Mesh& m, const std::vector<double>& coords, int j, const in
   Pi.ncols = ncols;
   Pi.C = C;
   Pi.rp.assign(nrows + 1, 0);
-  std::vector<int64_t> order(ns, -1);
+  std::vector<int64_t> order(nrows, -1);
   for (int64_t s = 0; s < ns; ++s) {
     const int32_t r = rowmap ? rowmap[s] : (int32_t)s;
     if (r >= 0) order[r] = s;
@@ -604,29 +750,59 @@ HostCsr vector_nodal(Mesh& m, const std::vector<double>& coords, int j, const in
   return Pi;
 }
 
-HostCsr host_of(Ctx* ctx, const Csr& a) {
-  HostCsr h;
-  h.nrows = a.nrows;
-  h.ncols = a.ncols;
-  h.rp.resize(a.nrows + 1);
-  h.col.resize(a.nnz);
-  h.val.resize(a.nnz);
-  copy_to_host(ctx, h.rp.data(), a.row_ptr.p, (a.nrows + 1) * 8);
-  if (a.nnz) {
-    copy_to_host(ctx, h.col.data(), a.col.p, a.nnz * 4);
-    copy_to_host(ctx, h.val.data(), a.val.p, a.nnz * 8);
+// [a | b]: the entries of two CSRs of the same pattern, a's values first.
+HostCsr interleave(const HostCsr& a, const HostCsr& b) {
+  FEEC_REQUIRE(a.nrows == b.nrows && a.col == b.col && a.rp == b.rp, FEEC_E_DEVICE,
+               "transfer patterns differ");
+  HostCsr o;
+  o.nrows = a.nrows;
+  o.ncols = a.ncols;
+  o.rp = a.rp;
+  o.col = a.col;
+  o.C = a.C + b.C;
+  o.val.resize(o.col.size() * o.C);
+  for (size_t p = 0; p < o.col.size(); ++p) {
+    for (int v = 0; v < a.C; ++v) o.val[p * o.C + v] = a.val[p * a.C + v];
+    for (int v = 0; v < b.C; ++v) o.val[p * o.C + a.C + v] = b.val[p * b.C + v];
   }
-  return h;
+  return o;
 }
 
-void to32(Ctx* ctx, const double* a, int64_t n, DBuf<float>& out) {
-  if ((int64_t)out.n != std::max<int64_t>(n, 1)) out.alloc(ctx, std::max<int64_t>(n, 1));
-  if (n) k_convert<double, float><<<grid_for(n), TPB, 0, ctx->stream>>>(a, out.p, n);
-  FEEC_HIP(hipGetLastError());
+// The coboundary D_{j-1} on the block's DOFs: rows the j-DOFs (rowmap),
+// columns mapped by colmap (-1: dropped), entries sorted by column.
+HostCsr host_coboundary(Mesh& m, int j, const int32_t* rowmap, int64_t nrows, const int32_t* colmap,
+                        int64_t ncols) {
+  Csr full;
+  mesh_coboundary(m, j - 1, full);
+  HostCsr f = host_of(m.ctx, full);
+  HostCsr o;
+  o.nrows = nrows;
+  o.ncols = ncols;
+  o.rp.assign(nrows + 1, 0);
+  std::vector<int64_t> order(nrows, -1);
+  for (int64_t s = 0; s < f.nrows; ++s) {
+    const int32_t r = rowmap ? rowmap[s] : (int32_t)s;
+    if (r >= 0) order[r] = s;
+  }
+  for (int64_t r = 0; r < nrows; ++r) {
+    const int64_t s = order[r];
+    std::vector<std::pair<int32_t, double>> ents;
+    for (int64_t p = f.rp[s]; p < f.rp[s + 1]; ++p) {
+      const int32_t c = colmap ? colmap[f.col[p]] : f.col[p];
+      if (c >= 0) ents.push_back({c, f.val[p]});
+    }
+    std::sort(ents.begin(), ents.end());
+    for (auto& e : ents) {
+      o.col.push_back(e.first);
+      o.val.push_back(e.second);
+    }
+    o.rp[r + 1] = (int64_t)o.col.size();
+  }
+  return o;
 }
 
-// The nodal operator's block of the KKT preconditioner matrix P (rows and
-// columns lo..hi-1), or hdif_gram(0) assembled on its own.
+// The nodal operator hdif_gram(0) on the complex's grade-0 DOFs: P's block
+// when the KKT preconditioner holds it (k <= 1), else assembled on its own.
 void nodal_operator(Geom& g, int bc, const KktSystem& s, int k, Csr& out) {
   Mesh& m = *g.mesh;
   if (k <= 1 && s.P.nnz) {
@@ -641,51 +817,106 @@ void nodal_operator(Geom& g, int bc, const KktSystem& s, int k, Csr& out) {
   assemble_form(g, bc, FEEC_HDIF_GRAM, 0, out);
 }
 
-void level_refresh_derived(Ctx* ctx, MgLevel& l) {
-  l.dinv.alloc(ctx, std::max<int64_t>(l.n, 1));
-  if (l.n)
-    k_scaled_inverse_diag<<<grid_for(l.n), TPB, 0, ctx->stream>>>(l.A.row_ptr.p, l.A.col.p, l.A.val.p,
-                                                                   l.n, kOmega, l.dinv.p);
-  FEEC_HIP(hipGetLastError());
-  to32(ctx, l.dinv.p, l.n, l.dinv32);
-  to32(ctx, l.A.val.p, l.A.nnz, l.A32);
+int nstencil(int d) { return (1 << (d + 1)) - 1; }
+
+int host_stencil_offset(int d, int s, const int* st) {
+  switch (d) {
+    case 1: return stencil_offset<1>(s, st);
+    case 2: return stencil_offset<2>(s, st);
+    case 3: return stencil_offset<3>(s, st);
+    default: return stencil_offset<4>(s, st);
+  }
 }
 
-// Galerkin operators and the coarse inverse from the finest operator's values.
-// Pattern buffers stay where they are, so captured graphs stay valid.
-void hierarchy_values(Hx& h) {
+int host_stencil_point(int d, const int* diff) {
+  switch (d) {
+    case 1: return stencil_point<1>(diff);
+    case 2: return stencil_point<2>(diff);
+    case 3: return stencil_point<3>(diff);
+    default: return stencil_point<4>(diff);
+  }
+}
+
+void launch_galerkin(Ctx* ctx, int d, MgLevel& f, MgLevel& c) {
+  const unsigned gr = grid_for(c.box.nb);
+  switch (d) {
+    case 1: k_st_galerkin<1><<<gr, TPB, 0, ctx->stream>>>(f.A.p, f.alive.p, c.alive.p, f.box, c.box, c.A.p, c.bad.p); break;
+    case 2: k_st_galerkin<2><<<gr, TPB, 0, ctx->stream>>>(f.A.p, f.alive.p, c.alive.p, f.box, c.box, c.A.p, c.bad.p); break;
+    case 3: k_st_galerkin<3><<<gr, TPB, 0, ctx->stream>>>(f.A.p, f.alive.p, c.alive.p, f.box, c.box, c.A.p, c.bad.p); break;
+    default: k_st_galerkin<4><<<gr, TPB, 0, ctx->stream>>>(f.A.p, f.alive.p, c.alive.p, f.box, c.box, c.A.p, c.bad.p); break;
+  }
+  FEEC_HIP(hipGetLastError());
+}
+
+void level_derived(Ctx* ctx, int d, MgLevel& l) {
+  const int64_t ns = (int64_t)nstencil(d) * l.box.nb;
+  k_st_dinv<<<grid_for(l.box.nb), TPB, 0, ctx->stream>>>(l.A.p, l.alive.p, l.box.nb, kOmega, l.dinv.p);
+  FEEC_HIP(hipGetLastError());
+  to32(ctx, l.dinv.p, l.box.nb, l.dinv32);
+  to32(ctx, l.A.p, ns, l.A32);
+}
+
+// Level operators from the finest one's values: the level-0 stencil from the
+// nodal CSR, Galerkin products down the hierarchy, the coarsest inverse.
+// Buffers stay where they are, so captured graphs stay valid.
+void hierarchy_values(Hx& h, const Csr& A0, bool check) {
   Ctx* ctx = h.ctx;
+  const int d = h.d;
+  MgLevel& l0 = h.L[0];
+  FEEC_REQUIRE(A0.nnz == h.l0_nnz, FEEC_E_DEVICE, "nodal operator pattern changed");
+  FEEC_HIP(hipMemsetAsync(l0.A.p, 0, l0.A.n * sizeof(double), ctx->stream));
+  if (A0.nnz)
+    k_scatter_map<<<grid_for(A0.nnz), TPB, 0, ctx->stream>>>(A0.val.p, h.l0_map.p, A0.nnz, l0.A.p);
+  FEEC_HIP(hipGetLastError());
+  level_derived(ctx, d, l0);
   for (size_t i = 0; i + 1 < h.L.size(); ++i) {
     MgLevel& f = h.L[i];
     MgLevel& c = h.L[i + 1];
-    Csr AP, RAP;
-    csr_mul(f.A, f.P, AP);
-    csr_mul(f.R, AP, RAP);
-    if (!c.A.row_ptr.p) {
-      c.A = std::move(RAP);
-    } else {
-      FEEC_REQUIRE(RAP.nnz == c.A.nnz, FEEC_E_DEVICE, "coarse operator pattern changed");
-      FEEC_HIP(hipMemcpyAsync(c.A.val.p, RAP.val.p, RAP.nnz * 8, hipMemcpyDeviceToDevice, ctx->stream));
-      ctx->sync();
+    launch_galerkin(ctx, d, f, c);
+    level_derived(ctx, d, c);
+    if (check) {
+      std::vector<uint8_t> bad(c.box.nb);
+      copy_to_host(ctx, bad.data(), c.bad.p, c.box.nb);
+      for (uint8_t b : bad) FEEC_REQUIRE(!b, FEEC_E_DEVICE, "Galerkin product left the Kuhn stencil");
     }
-    level_refresh_derived(ctx, c);
   }
+  // the coarsest level: dense inverse on its live vertices, embedded in its box
   MgLevel& last = h.L.back();
-  HostCsr a = host_of(ctx, last.A);
-  const int64_t n = last.n;
-  std::vector<double> dense(n * n, 0.0);
-  for (int64_t r = 0; r < n; ++r)
-    for (int64_t p = a.rp[r]; p < a.rp[r + 1]; ++p) dense[r * n + a.col[p]] += a.val[p];
-  FEEC_REQUIRE(spd_inverse(n, dense), FEEC_E_NOT_SPD, "coarsest multigrid operator is not SPD");
-  h.cinv.alloc(ctx, std::max<int64_t>(n * n, 1));
-  copy_to_device(ctx, h.cinv.p, dense.data(), n * n * 8);
-  to32(ctx, h.cinv.p, n * n, h.cinv32);
+  const int nb = last.box.nb, NS = nstencil(d);
+  std::vector<double> A((size_t)NS * nb);
+  std::vector<uint8_t> alive(nb);
+  copy_to_host(ctx, A.data(), last.A.p, A.size() * 8);
+  copy_to_host(ctx, alive.data(), last.alive.p, nb);
+  std::vector<int> pos(nb, -1), live;
+  for (int i = 0; i < nb; ++i)
+    if (alive[i]) {
+      pos[i] = (int)live.size();
+      live.push_back(i);
+    }
+  const int64_t nl = (int64_t)live.size();
+  std::vector<double> dense(nl * nl, 0.0);
+  for (int64_t r = 0; r < nl; ++r) {
+    const int i = live[r];
+    for (int s = 0; s < NS; ++s) {
+      const double a = A[(size_t)s * nb + i];
+      if (a == 0.0) continue;
+      const int jv = i + host_stencil_offset(d, s, last.box.st);
+      FEEC_REQUIRE(jv >= 0 && jv < nb && pos[jv] >= 0, FEEC_E_DEVICE, "coarsest stencil leaves its box");
+      dense[r * nl + pos[jv]] += a;
+    }
+  }
+  FEEC_REQUIRE(spd_inverse(nl, dense), FEEC_E_NOT_SPD, "coarsest multigrid operator is not SPD");
+  std::vector<double> boxinv((size_t)nb * nb, 0.0);
+  for (int64_t r = 0; r < nl; ++r)
+    for (int64_t c = 0; c < nl; ++c) boxinv[(size_t)live[r] * nb + live[c]] = dense[r * nl + c];
+  copy_to_device(ctx, h.cinv.p, boxinv.data(), boxinv.size() * 8);
+  to32(ctx, h.cinv.p, (int64_t)boxinv.size(), h.cinv32);
   ctx->sync();
 }
 
 void alloc_slots(Ctx* ctx, Hx& h) {
   for (auto& l : h.L) {
-    const int64_t m = std::max<int64_t>(l.n * h.nv, 1);
+    const int64_t m = std::max<int64_t>((int64_t)l.box.nb * h.nv, 1);
     l.r64.alloc(ctx, m);
     l.x64.alloc(ctx, m);
     l.y64.alloc(ctx, m);
@@ -699,22 +930,20 @@ void alloc_slots(Ctx* ctx, Hx& h) {
 
 void smoother_diag(Hx& h, const KktSystem& s) {
   Ctx* ctx = h.ctx;
-  h.s64.alloc(ctx, std::max<int64_t>(h.n, 1));
+  if ((int64_t)h.s64.n != std::max<int64_t>(h.n, 1)) h.s64.alloc(ctx, std::max<int64_t>(h.n, 1));
   if (h.n) k_scale<<<grid_for(h.n), TPB, 0, ctx->stream>>>(s.inv_diag.p, h.n, kOmega, h.s64.p);
   FEEC_HIP(hipGetLastError());
   to32(ctx, h.s64.p, h.n, h.s32);
   for (auto& b : h.blocks) {
-    if (b.j < 1) continue;
-    // grade j-1 smoother: the sigma block's when it is the other block, else the nodal one
+    if (b.j != 2) continue;
+    // S_1 on the grade-1 DOFs: the sigma block's smoother
     const HxBlock* lower = nullptr;
     for (auto& o : h.blocks)
-      if (o.j == b.j - 1) lower = &o;
-    const int64_t nl = b.D.ncols;
-    b.sl.alloc(ctx, std::max<int64_t>(nl, 1));
-    if (lower)
-      FEEC_HIP(hipMemcpyAsync(b.sl.p, h.s64.p + lower->off, nl * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    else if (b.j - 1 == 0)
-      FEEC_HIP(hipMemcpyAsync(b.sl.p, h.L[0].dinv.p, nl * 8, hipMemcpyDeviceToDevice, ctx->stream));
+      if (o.j == 1) lower = &o;
+    FEEC_REQUIRE(lower, FEEC_E_DEVICE, "grade-2 block without its grade-1 block");
+    const int64_t nl = b.o_d.m.ncols;
+    if ((int64_t)b.sl.n != std::max<int64_t>(nl, 1)) b.sl.alloc(ctx, std::max<int64_t>(nl, 1));
+    FEEC_HIP(hipMemcpyAsync(b.sl.p, h.s64.p + lower->off, nl * 8, hipMemcpyDeviceToDevice, ctx->stream));
     to32(ctx, b.sl.p, nl, b.sl32);
   }
 }
@@ -735,9 +964,11 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::strin
     return no("unexpected harmonic border");
   const int n = kuhn_grid_n(m);
   if (n < 2) return no("the mesh is not a Kuhn grid of the library's numbering");
+  const int d = m.d;
+  if (ipow(n + 1, d) * MAXNV >= ((int64_t)1 << 31)) return no("the grid is too large for 32-bit slot indices");
   auto h = std::make_unique<Hx>();
   h->ctx = ctx;
-  h->d = m.d;
+  h->d = d;
   h->k = k;
   h->bc = bc;
   h->grid = n;
@@ -746,40 +977,100 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::strin
   h->N = s.n + s.nh;
   const bool rel = bc == FEEC_RELATIVE;
   if (rel) m.ensure_interior();
-  // the nodal hierarchy
-  {
-    MgLevel l0;
-    l0.grid = n;
-    nodal_operator(g, bc, s, k, l0.A);
-    l0.n = l0.A.nrows;
-    h->nv_nodes = l0.n;
-    level_refresh_derived(ctx, l0);
-    h->L.push_back(std::move(l0));
-    int gr = n;
-    while (h->L.back().n > kCoarseMax && gr >= 2) {
-      MgLevel& f = h->L.back();
-      HostCsr P = kuhn_prolongation(m.d, gr, rel);
-      FEEC_REQUIRE(P.nrows == f.n, FEEC_E_DEVICE, "prolongation does not match the level");
-      HostCsr R = transpose(P);
-      upload(ctx, P, f.P);
-      upload(ctx, R, f.R);
-      to32(ctx, f.P.val.p, f.P.nnz, f.P32);
-      to32(ctx, f.R.val.p, f.R.nnz, f.R32);
-      MgLevel c;
-      gr = (gr + 1) / 2;
-      c.grid = gr;
-      c.n = P.ncols;
-      h->L.push_back(std::move(c));
-    }
-    if (h->L.back().n > 4 * kCoarseMax) return no("the coarsest multigrid level is too large");
+  const int NS = nstencil(d);
+  // vertex -> its grade-0 DOF (-1: none) and -> its box column (-1 off the unknowns)
+  std::vector<int32_t> dof0(m.n[0]), live_col(m.n[0]);
+  if (rel) copy_to_host(ctx, dof0.data(), m.interior[0].p, m.n[0] * 4);
+  for (int64_t v = 0; v < m.n[0]; ++v) {
+    if (!rel) dof0[v] = (int32_t)v;
+    live_col[v] = dof0[v] >= 0 ? (int32_t)v : -1;
   }
-  try {
-    hierarchy_values(*h);
-  } catch (const Error& e) {
-    return no(e.what());
+  // the levels
+  {
+    auto add_level = [&](int gcells, const std::vector<uint8_t>& al, int64_t nlive) {
+      MgLevel l;
+      l.box.g = gcells;
+      l.box.nb = (int)ipow(gcells + 1, d);
+      int st = 1;
+      for (int a = 0; a < d; ++a) {
+        l.box.st[a] = st;
+        st *= gcells + 1;
+      }
+      l.live = nlive;
+      l.alive.alloc(ctx, l.box.nb);
+      copy_to_device(ctx, l.alive.p, al.data(), l.box.nb);
+      l.A.alloc(ctx, (size_t)NS * l.box.nb);
+      l.dinv.alloc(ctx, l.box.nb);
+      l.bad.alloc(ctx, l.box.nb);
+      h->L.push_back(std::move(l));
+    };
+    std::vector<uint8_t> alive(ipow(n + 1, d));
+    int64_t live = 0;
+    for (size_t v = 0; v < alive.size(); ++v) {
+      alive[v] = dof0[v] >= 0;
+      live += alive[v];
+    }
+    add_level(n, alive, live);
+    int gr = n;
+    while (live > kCoarseMax && gr >= 2) {
+      const int gc = (gr + 1) / 2;
+      std::vector<uint8_t> ac(ipow(gc + 1, d));
+      live = 0;
+      for (size_t w = 0; w < ac.size(); ++w) {
+        bool in = true;
+        int64_t r = (int64_t)w;
+        for (int a = 0; a < d; ++a, r /= (gc + 1)) {
+          const int64_t c = r % (gc + 1);
+          if (rel) in = in && c > 0 && 2 * c <= gr - 1;
+        }
+        ac[w] = in;
+        live += in;
+      }
+      add_level(gc, ac, live);
+      gr = gc;
+    }
+    if (h->L.back().box.nb > kDenseMaxBox) return no("the coarsest multigrid level is too large");
+  }
+  // the level-0 operator's stencil positions, then the hierarchy's values
+  {
+    Csr A0;
+    nodal_operator(g, bc, s, k, A0);
+    HostCsr a = host_of(ctx, A0);
+    const Box& b0 = h->L[0].box;
+    std::vector<int32_t> box_of(a.nrows, 0);
+    for (int64_t v = 0; v < m.n[0]; ++v)
+      if (dof0[v] >= 0 && dof0[v] < a.nrows) box_of[dof0[v]] = (int32_t)v;
+    std::vector<int32_t> map(a.col.size());
+    for (int64_t r = 0; r < a.nrows; ++r)
+      for (int64_t p = a.rp[r]; p < a.rp[r + 1]; ++p) {
+        int ri = box_of[r], rj = box_of[a.col[p]];
+        const int bi = ri;
+        int diff[4] = {0, 0, 0, 0};
+        bool ok = true;
+        for (int q = 0; q < d; ++q) {
+          diff[q] = rj % (b0.g + 1) - ri % (b0.g + 1);
+          ok = ok && diff[q] >= -1 && diff[q] <= 1;
+          ri /= (b0.g + 1);
+          rj /= (b0.g + 1);
+        }
+        const int sp = ok ? host_stencil_point(d, diff) : -1;
+        if (sp < 0) return no("the nodal operator is not a Kuhn stencil");
+        map[p] = sp * b0.nb + bi;
+      }
+    h->l0_nnz = (int64_t)map.size();
+    h->l0_map.alloc(ctx, std::max<int64_t>(h->l0_nnz, 1));
+    if (h->l0_nnz) copy_to_device(ctx, h->l0_map.p, map.data(), h->l0_nnz * 4);
+    const int nbc = h->L.back().box.nb;
+    h->cinv.alloc(ctx, (size_t)nbc * nbc);
+    try {
+      hierarchy_values(*h, A0, true);
+    } catch (const Error& e) {
+      return no(e.what());
+    }
   }
   // blocks: sigma (grade k-1, when k >= 1) and u (grade k)
   const int64_t ns = s.split;
+  const int64_t nb0 = h->L[0].box.nb;
   std::vector<int> grades;
   if (k >= 1) grades.push_back(k - 1);
   grades.push_back(k);
@@ -789,80 +1080,87 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::strin
     b.j = j;
     b.off = (j == k) ? ns : 0;
     b.n = (j == k) ? s.n - ns : ns;
-    const int32_t* map_j = nullptr;
-    std::vector<int32_t> hmap_j, hmap_l, hmap_0;
+    std::vector<int32_t> hmap_j, hmap_l;
     auto host_map = [&](int grade, std::vector<int32_t>& out) -> const int32_t* {
       if (!rel) return nullptr;
       out.resize(m.n[grade]);
       copy_to_host(ctx, out.data(), m.interior[grade].p, m.n[grade] * 4);
       return out.data();
     };
-    map_j = host_map(j, hmap_j);
-    const int32_t* map_0 = host_map(0, hmap_0);
+    const int32_t* map_j = host_map(j, hmap_j);
     if (j == 0) {
-      b.slot_v = slot++;
+      HostCsr o;  // block DOF -> its vertex
+      o.nrows = b.n;
+      o.ncols = nb0;
+      o.rp.resize(b.n + 1);
+      o.col.assign(b.n, 0);
+      o.val.assign(b.n, 1.0);
+      for (int64_t v = 0; v < m.n[0]; ++v)
+        if (dof0[v] >= 0 && dof0[v] < b.n) o.col[dof0[v]] = (int32_t)v;
+      for (int64_t i = 0; i <= b.n; ++i) o.rp[i] = i;
+      upload(ctx, transpose(o), b.g_main);
+      upload(ctx, o, b.o_main);
+      b.slot_main = slot;
+      slot += 1;
+    } else if (j == 1) {
+      HostCsr D = host_coboundary(m, 1, map_j, b.n, live_col.data(), nb0);
+      HostCsr Pi = vector_nodal(m, g.coords, 1, map_j, live_col.data(), b.n, nb0);
+      HostCsr DP = interleave(D, Pi);
+      upload(ctx, transpose(DP), b.g_main);
+      upload(ctx, DP, b.o_main);
+      b.slot_main = slot;
+      slot += DP.C;
     } else {
-      // D_{j-1} on the complex's DOFs and its transpose
-      Csr Dfull;
-      mesh_coboundary(m, j - 1, Dfull);
-      if (rel)
-        csr_select(Dfull, m.interior[j].p, m.n_interior[j], m.interior[j - 1].p, m.n_interior[j - 1],
-                   b.D);
-      else
-        b.D = std::move(Dfull);
-      csr_transpose(b.D, b.DT);
-      to32(ctx, b.D.val.p, b.D.nnz, b.D32);
-      to32(ctx, b.DT.val.p, b.DT.nnz, b.DT32);
-      const int64_t nl = b.D.ncols;
+      const int32_t* map_l = host_map(1, hmap_l);
+      const int64_t nl = rel ? m.n_interior[1] : m.n[1];
+      HostCsr D = host_coboundary(m, 2, map_j, b.n, map_l, nl);
+      HostCsr Pi = vector_nodal(m, g.coords, 2, map_j, live_col.data(), b.n, nb0);
+      HostCsr Pl = vector_nodal(m, g.coords, 1, map_l, live_col.data(), nl, nb0);
+      upload(ctx, transpose(D), b.g_t);
+      upload(ctx, D, b.o_d);
+      upload(ctx, transpose(Pl), b.g_tl);
+      upload(ctx, Pl, b.o_ul);
+      upload(ctx, transpose(Pi), b.g_main);
+      upload(ctx, Pi, b.o_main);
+      b.slot_main = slot;
+      slot += Pi.C;
+      b.slot_tl = slot;
+      slot += Pl.C;
       b.t64.alloc(ctx, std::max<int64_t>(nl, 1));
       b.u64.alloc(ctx, std::max<int64_t>(nl, 1));
       b.t32.alloc(ctx, std::max<int64_t>(nl, 1));
       b.u32.alloc(ctx, std::max<int64_t>(nl, 1));
-      // vector-nodal of grade j
-      HostCsr Pi = vector_nodal(m, g.coords, j, map_j, map_0, b.n, h->nv_nodes);
-      HostCsr PiT = transpose(Pi);
-      b.C = Pi.C;
-      upload(ctx, Pi, b.Pi);
-      upload(ctx, PiT, b.PiT);
-      to32(ctx, b.Pi.val.p, (int64_t)Pi.val.size(), b.Pi32);
-      to32(ctx, b.PiT.val.p, (int64_t)PiT.val.size(), b.PiT32);
-      b.slot_v = slot;
-      slot += b.C;
-      if (j - 1 == 0) {
-        b.slot_t = slot++;
-      } else {
-        const int32_t* map_l = host_map(j - 1, hmap_l);
-        HostCsr Pl = vector_nodal(m, g.coords, j - 1, map_l, map_0, nl, h->nv_nodes);
-        HostCsr PlT = transpose(Pl);
-        b.Cl = Pl.C;
-        upload(ctx, Pl, b.Pil);
-        upload(ctx, PlT, b.PilT);
-        to32(ctx, b.Pil.val.p, (int64_t)Pl.val.size(), b.Pil32);
-        to32(ctx, b.PilT.val.p, (int64_t)PlT.val.size(), b.PilT32);
-        b.slot_t = slot;
-        slot += b.Cl;
-      }
     }
     h->blocks.push_back(std::move(b));
   }
   if (slot > MAXNV) return no("too many auxiliary copies for one cycle");
+  for (const auto& b : h->blocks)
+    for (const XCsr* x : {&b.g_main, &b.o_main, &b.g_t, &b.g_tl, &b.o_ul, &b.o_d})
+      if (x->C > MAXC) return no("too many components per transfer entry");
   h->nv = slot;
   alloc_slots(ctx, *h);
   h->zt64.alloc(ctx, std::max<int64_t>(h->N, 1));
   h->zt32.alloc(ctx, std::max<int64_t>(h->N, 1));
   smoother_diag(*h, s);
-  // work estimate of one application in FP32: per level 3 + 2 sweeps passes
-  // over the operator and the slot vectors, plus the transfers
-  int64_t bytes = 0, launches = 4;
+  // work estimate of one application in FP32
+  int64_t bytes = 0, launches = 1;
   for (size_t i = 0; i < h->L.size(); ++i) {
     const MgLevel& l = h->L[i];
-    bytes += (int64_t)(2 * h->sweeps + 1) * (l.A.nnz * 8 + 12 * l.n * h->nv);
-    launches += i + 1 < h->L.size() ? 3 + 2 * h->sweeps : 1;
+    const int64_t nbv = (int64_t)l.box.nb * h->nv;
+    if (i + 1 < h->L.size()) {
+      bytes += (int64_t)(2 * h->sweeps) * ((int64_t)NS * l.box.nb * 4 + 12 * nbv) + 16 * nbv;
+      launches += 2 + 2 * h->sweeps;
+    } else {
+      bytes += (int64_t)l.box.nb * l.box.nb * 4;
+      launches += 1;
+    }
   }
-  for (const auto& b : h->blocks) {
-    bytes += (b.D.nnz + b.DT.nnz) * 8 + (b.Pi.nnz + b.PiT.nnz) * 4 * (b.C + 1);
-    launches += 6;
-  }
+  for (const auto& b : h->blocks)
+    for (const XCsr* x : {&b.g_main, &b.o_main, &b.g_t, &b.g_tl, &b.o_ul, &b.o_d})
+      if (x->m.nnz) {
+        bytes += x->m.nnz * 4 * (1 + x->C) + x->m.nrows * 8;
+        launches += 1;
+      }
   h->bytes32 = bytes;
   h->launches = launches;
   ctx->sync();
@@ -870,16 +1168,11 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::strin
 }
 
 void hx_refresh(Hx& h, Geom& g, const KktSystem& s) {
-  Ctx* ctx = h.ctx;
   Csr A0;
   nodal_operator(g, h.bc, s, h.k, A0);
-  MgLevel& l0 = h.L[0];
-  FEEC_REQUIRE(A0.nnz == l0.A.nnz && A0.nrows == l0.n, FEEC_E_DEVICE, "nodal operator pattern changed");
-  FEEC_HIP(hipMemcpyAsync(l0.A.val.p, A0.val.p, A0.nnz * 8, hipMemcpyDeviceToDevice, ctx->stream));
-  level_refresh_derived(ctx, l0);
-  hierarchy_values(h);
+  hierarchy_values(h, A0, false);
   smoother_diag(h, s);
-  ctx->sync();
+  h.ctx->sync();
 }
 
 namespace {
@@ -888,10 +1181,8 @@ template <class TP>
 struct Sel;
 template <>
 struct Sel<double> {
-  static const double* A(const MgLevel& l) { return l.A.val.p; }
+  static const double* A(const MgLevel& l) { return l.A.p; }
   static const double* dinv(const MgLevel& l) { return l.dinv.p; }
-  static const double* P(const MgLevel& l) { return l.P.val.p; }
-  static const double* R(const MgLevel& l) { return l.R.val.p; }
   static double* r(MgLevel& l) { return l.r64.p; }
   static double* x(MgLevel& l) { return l.x64.p; }
   static double* y(MgLevel& l) { return l.y64.p; }
@@ -899,12 +1190,7 @@ struct Sel<double> {
   static const double* cinv(const Hx& h) { return h.cinv.p; }
   static const double* s(const Hx& h) { return h.s64.p; }
   static double* zt(Hx& h) { return h.zt64.p; }
-  static const double* D(const HxBlock& b) { return b.D.val.p; }
-  static const double* DT(const HxBlock& b) { return b.DT.val.p; }
-  static const double* Pi(const HxBlock& b) { return b.Pi.val.p; }
-  static const double* PiT(const HxBlock& b) { return b.PiT.val.p; }
-  static const double* Pil(const HxBlock& b) { return b.Pil.val.p; }
-  static const double* PilT(const HxBlock& b) { return b.PilT.val.p; }
+  static const double* v(const XCsr& x) { return x.m.val.p; }
   static const double* sl(const HxBlock& b) { return b.sl.p; }
   static double* t(HxBlock& b) { return b.t64.p; }
   static double* u(HxBlock& b) { return b.u64.p; }
@@ -913,8 +1199,6 @@ template <>
 struct Sel<float> {
   static const float* A(const MgLevel& l) { return l.A32.p; }
   static const float* dinv(const MgLevel& l) { return l.dinv32.p; }
-  static const float* P(const MgLevel& l) { return l.P32.p; }
-  static const float* R(const MgLevel& l) { return l.R32.p; }
   static float* r(MgLevel& l) { return l.r32.p; }
   static float* x(MgLevel& l) { return l.x32.p; }
   static float* y(MgLevel& l) { return l.y32.p; }
@@ -922,103 +1206,102 @@ struct Sel<float> {
   static const float* cinv(const Hx& h) { return h.cinv32.p; }
   static const float* s(const Hx& h) { return h.s32.p; }
   static float* zt(Hx& h) { return h.zt32.p; }
-  static const float* D(const HxBlock& b) { return b.D32.p; }
-  static const float* DT(const HxBlock& b) { return b.DT32.p; }
-  static const float* Pi(const HxBlock& b) { return b.Pi32.p; }
-  static const float* PiT(const HxBlock& b) { return b.PiT32.p; }
-  static const float* Pil(const HxBlock& b) { return b.Pil32.p; }
-  static const float* PilT(const HxBlock& b) { return b.PilT32.p; }
+  static const float* v(const XCsr& x) { return x.v32.p; }
   static const float* sl(const HxBlock& b) { return b.sl32.p; }
   static float* t(HxBlock& b) { return b.t32.p; }
   static float* u(HxBlock& b) { return b.u32.p; }
 };
 
-// One V-cycle on level i: rhs in r(i), result in x(i) (iterate buffers swap).
-template <class TP>
-void vcycle(Enq& q, Hx& h, size_t i, const MinresS* S) {
+// One V-cycle on level i with its rhs in r(i); returns the buffer holding
+// the result (x(i) or y(i)).
+template <class TP, int D>
+TP* vcycle(Enq& q, Hx& h, size_t i, const MinresS* S) {
   using X = Sel<TP>;
   MgLevel& l = h.L[i];
-  const int nv = h.nv;
+  const unsigned nv = (unsigned)h.nv;
+  const int64_t nbv = (int64_t)l.box.nb * nv;
   if (i + 1 == h.L.size()) {
-    q(k_mg_dense<TP>, (unsigned)std::max<int64_t>(l.n, 1), TPB, X::cinv(h), (const TP*)X::r(l), X::x(l),
-      l.n, nv, S);
-    return;
+    q(k_st_dense<TP>, (unsigned)l.box.nb, TPB, X::cinv(h), (const TP*)X::r(l), X::x(l), l.box.nb, h.nv, S);
+    return X::x(l);
   }
   TP* x = X::x(l);
   TP* y = X::y(l);
   const TP* r = X::r(l);
   auto sweep = [&]() {
-    q(k_mg_rowop<TP>, grid_for(l.n), TPB, l.A.row_ptr.p, l.A.col.p, X::A(l), X::dinv(l), r,
-      (const TP*)x, y, l.n, nv, 0, S);
+    q(k_st_sweep<TP, D>, grid_for(nbv), TPB, X::A(l), X::dinv(l), r, (const TP*)x, y, l.box, nv, 0, S);
     std::swap(x, y);
   };
-  q(k_mg_scale<TP>, grid_for(l.n), TPB, X::dinv(l), r, x, l.n, nv, S);
+  q(k_st_scale<TP>, grid_for(nbv), TPB, X::dinv(l), r, x, (unsigned)l.box.nb, nv, S);
   for (int s = 1; s < h.sweeps; ++s) sweep();
-  q(k_mg_rowop<TP>, grid_for(l.n), TPB, l.A.row_ptr.p, l.A.col.p, X::A(l), X::dinv(l), r,
-    (const TP*)x, X::e(l), l.n, nv, 1, S);
+  q(k_st_sweep<TP, D>, grid_for(nbv), TPB, X::A(l), X::dinv(l), r, (const TP*)x, X::e(l), l.box, nv, 1, S);
   MgLevel& c = h.L[i + 1];
-  q(k_mg_spmv<TP>, grid_for(c.n), TPB, l.R.row_ptr.p, l.R.col.p, X::R(l), (const TP*)X::e(l), X::r(c),
-    c.n, l.n, nv, 0, S);
-  vcycle<TP>(q, h, i + 1, S);
-  q(k_mg_spmv<TP>, grid_for(l.n), TPB, l.P.row_ptr.p, l.P.col.p, X::P(l), (const TP*)X::x(c), x, l.n,
-    c.n, nv, 1, S);
+  q(k_st_restrict<TP, D>, grid_for((int64_t)c.box.nb * nv), TPB, (const TP*)X::e(l), X::r(c),
+    (const uint8_t*)c.alive.p, l.box, c.box, nv, S);
+  TP* xc = vcycle<TP, D>(q, h, i + 1, S);
+  q(k_st_prolong<TP, D>, grid_for(nbv), TPB, (const TP*)xc, x, X::dinv(l), l.box, c.box, nv, S);
   for (int s = 0; s < h.sweeps; ++s) sweep();
-  if (x != X::x(l))  // leave the result in x(l)
-    q(k_hx_copy<TP, TP>, grid_for(l.n * nv), TPB, (const TP*)x, X::x(l), l.n * nv, 0, S);
+  return x;
+}
+
+template <class TX, class TP>
+void gather(Enq& q, const XCsr& g, const TX* x, TP* y, int nv, int slot0, const MinresS* S) {
+  using X = Sel<TP>;
+  const int64_t nr = g.m.nrows;
+  if (g.avg_row >= 6.0)
+    q(k_hx_gather<TX, TP, 4>, grid_for(nr * 4), TPB, g.m.row_ptr.p, g.m.col.p, X::v(g), g.C, x, y, nv, slot0,
+      nr, S);
+  else
+    q(k_hx_gather<TX, TP, 1>, grid_for(nr), TPB, g.m.row_ptr.p, g.m.col.p, X::v(g), g.C, x, y, nv, slot0, nr,
+      S);
+}
+
+template <class TIN, class TP, class TOUT, int D>
+void hx_apply_d(Enq& q, Hx& h, const TIN* r, TOUT* z, const MinresS* S, double* part) {
+  using X = Sel<TP>;
+  MgLevel& l0 = h.L[0];
+  TP* slots = X::r(l0);
+  TP* zt = X::zt(h);
+  const int nv = h.nv;
+  for (HxBlock& b : h.blocks) {
+    const TIN* rb = r + b.off;
+    if (b.j == 2) {
+      gather<TIN, TP>(q, b.g_t, rb, X::t(b), 1, 0, S);
+      gather<TP, TP>(q, b.g_tl, (const TP*)X::t(b), slots, nv, b.slot_tl, S);
+    }
+    gather<TIN, TP>(q, b.g_main, rb, slots, nv, b.slot_main, S);
+  }
+  const TP* xs = vcycle<TP, D>(q, h, 0, S);
+  for (HxBlock& b : h.blocks) {
+    TP* zb = zt + b.off;
+    const TIN* rb = r + b.off;
+    const TP* sb = X::s(h) + b.off;
+    const XCsr& om = b.o_main;
+    if (b.j == 2) {
+      const XCsr& ul = b.o_ul;
+      q(k_hx_out<TP, TP, TP>, grid_for(ul.m.nrows), TPB, ul.m.row_ptr.p, ul.m.col.p, X::v(ul), ul.C, 0, xs,
+        (const TP*)slots, X::dinv(l0), nv, b.slot_tl, X::sl(b), (const TP*)X::t(b), X::u(b), ul.m.nrows, 0, S);
+      const XCsr& od = b.o_d;
+      q(k_hx_out<TP, TP, TIN>, grid_for(od.m.nrows), TPB, od.m.row_ptr.p, od.m.col.p, X::v(od), od.C, 0,
+        (const TP*)X::u(b), (const TP*)nullptr, (const TP*)nullptr, 1, 0, sb, rb, zb, od.m.nrows, 0, S);
+      q(k_hx_out<TP, TP, TIN>, grid_for(om.m.nrows), TPB, om.m.row_ptr.p, om.m.col.p, X::v(om), om.C, 0, xs,
+        (const TP*)slots, X::dinv(l0), nv, b.slot_main, (const TP*)nullptr, (const TIN*)nullptr, zb,
+        om.m.nrows, 1, S);
+    } else {
+      q(k_hx_out<TP, TP, TIN>, grid_for(om.m.nrows), TPB, om.m.row_ptr.p, om.m.col.p, X::v(om), om.C,
+        b.j == 1 ? 1 : 0, xs, (const TP*)slots, X::dinv(l0), nv, b.slot_main, sb, rb, zb, om.m.nrows, 0, S);
+    }
+  }
+  q(k_hx_final<TIN, TP, TOUT>, red_grid(h.N), TPB, r, (const TP*)zt, z, h.n, h.N, S, part);
 }
 
 template <class TIN, class TP, class TOUT>
 void hx_apply(Enq& q, Hx& h, const TIN* r, TOUT* z, const MinresS* S, double* part) {
-  using X = Sel<TP>;
-  const int64_t n0 = h.nv_nodes;
-  MgLevel& l0 = h.L[0];
-  TP* slots = X::r(l0);
-  TP* zt = X::zt(h);
-  q(k_hx_init<TIN, TP>, grid_for(h.N), TPB, r, X::s(h), zt, h.n, h.N, S);
-  for (HxBlock& b : h.blocks) {
-    const TIN* rb = r + b.off;
-    if (b.j == 0) {
-      q(k_hx_copy<TIN, TP>, grid_for(b.n), TPB, rb, slots + b.slot_v * n0, b.n, 0, S);
-      continue;
-    }
-    const int64_t nl = b.D.ncols;
-    // t = D^T r
-    q(k_hx_gather<TIN, TP>, grid_for(nl), TPB, b.DT.row_ptr.p, b.DT.col.p, X::DT(b), 1, rb, X::t(b), nl,
-      S);
-    if (b.j - 1 == 0)
-      q(k_hx_copy<TP, TP>, grid_for(nl), TPB, (const TP*)X::t(b), slots + b.slot_t * n0, nl, 0, S);
-    else
-      q(k_hx_gather<TP, TP>, grid_for(n0), TPB, b.PilT.row_ptr.p, b.PilT.col.p, X::PilT(b), b.Cl,
-        (const TP*)X::t(b), slots + b.slot_t * n0, n0, S);
-    q(k_hx_gather<TIN, TP>, grid_for(n0), TPB, b.PiT.row_ptr.p, b.PiT.col.p, X::PiT(b), b.C, rb,
-      slots + b.slot_v * n0, n0, S);
+  switch (h.d) {
+    case 1: hx_apply_d<TIN, TP, TOUT, 1>(q, h, r, z, S, part); break;
+    case 2: hx_apply_d<TIN, TP, TOUT, 2>(q, h, r, z, S, part); break;
+    case 3: hx_apply_d<TIN, TP, TOUT, 3>(q, h, r, z, S, part); break;
+    default: hx_apply_d<TIN, TP, TOUT, 4>(q, h, r, z, S, part); break;
   }
-  vcycle<TP>(q, h, 0, S);
-  const TP* xs = X::x(l0);
-  for (HxBlock& b : h.blocks) {
-    TP* zb = zt + b.off;
-    if (b.j == 0) {
-      q(k_hx_copy<TP, TP>, grid_for(b.n), TPB, xs + b.slot_v * n0, zb, b.n, 1, S);
-      continue;
-    }
-    const int64_t nl = b.D.ncols;
-    // u = S_{j-1} t + (V t or Pi_{j-1} V Pi_{j-1}^T t)
-    if (b.j - 1 == 0) {
-      q(k_hx_scatter<TP>, grid_for(nl), TPB, (const int64_t*)nullptr, (const int32_t*)nullptr,
-        (const TP*)nullptr, 0, (const TP*)nullptr, (int64_t)0, X::sl(b), (const TP*)X::t(b), X::u(b), nl,
-        0, S);
-      q(k_hx_copy<TP, TP>, grid_for(nl), TPB, xs + b.slot_t * n0, X::u(b), nl, 1, S);
-    } else {
-      q(k_hx_scatter<TP>, grid_for(nl), TPB, b.Pil.row_ptr.p, b.Pil.col.p, X::Pil(b), b.Cl,
-        xs + b.slot_t * n0, n0, X::sl(b), (const TP*)X::t(b), X::u(b), nl, 0, S);
-    }
-    // z_j += D u + Pi V Pi^T r
-    q(k_hx_scatter<TP>, grid_for(b.n), TPB, b.D.row_ptr.p, b.D.col.p, X::D(b), 1, (const TP*)X::u(b), nl,
-      (const TP*)nullptr, (const TP*)nullptr, zb, b.n, 1, S);
-    q(k_hx_scatter<TP>, grid_for(b.n), TPB, b.Pi.row_ptr.p, b.Pi.col.p, X::Pi(b), b.C,
-      xs + b.slot_v * n0, n0, (const TP*)nullptr, (const TP*)nullptr, zb, b.n, 1, S);
-  }
-  q(k_hx_final<TIN, TP, TOUT>, red_grid(h.N), TPB, r, (const TP*)zt, z, h.N, S, part);
 }
 
 }  // namespace
