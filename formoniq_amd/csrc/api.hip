@@ -465,6 +465,11 @@ static int resolve_cheb_degree(const Problem& P) {
 // assembly when the options ask for it (AUTO: where it applies and the KKT
 // system has at least 20,000 rows), refreshed with the values afterwards.
 constexpr int64_t kMultigridAutoRows = 20000;
+// Chebyshev acceleration of the HX blocks: applications of B per block
+// application (cheb_degree = -1) and the interval's lower end as a fraction of
+// its upper one (cheb_lower <= 0).
+constexpr int kMultigridChebDefault = 4;
+constexpr double kMultigridChebFraction = 0.25;
 void problem_multigrid(Problem& P) {
   const int pc = P.opts.preconditioner;
   if (pc == FEEC_PC_CHEBYSHEV || (pc == FEEC_PC_AUTO && P.sys.n < kMultigridAutoRows)) {
@@ -479,7 +484,9 @@ void problem_multigrid(Problem& P) {
   if (P.hx_tried && pc == FEEC_PC_AUTO) return;
   P.hx_tried = true;
   std::string why;
-  P.hx = hx_create(*P.g, P.bc, P.k, P.sys, P.opts.mg_sweeps, &why);
+  const int m = P.opts.cheb_degree < 0 ? kMultigridChebDefault : std::max(1, P.opts.cheb_degree);
+  const double frac = P.opts.cheb_lower > 0.0 ? P.opts.cheb_lower : kMultigridChebFraction;
+  P.hx = hx_create(*P.g, P.bc, P.k, P.sys, P.opts.mg_sweeps, m, frac, &why);
   FEEC_REQUIRE(P.hx || pc != FEEC_PC_MULTIGRID, FEEC_E_UNSUPPORTED,
                "multigrid preconditioner unavailable: " + why);
   P.sys.hx = P.hx;
@@ -1383,6 +1390,7 @@ feec_status feec_problem_preconditioner(const feec_problem* p, int64_t* nnz, int
       lambda_max[0] = p->sys.lambda_max[0];
       lambda_max[1] = p->sys.lambda_max[1];
     }
+    if (s.hx) hx_info(*s.hx, cheb_degree, lambda_max);
   });
 }
 

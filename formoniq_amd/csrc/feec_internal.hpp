@@ -384,15 +384,20 @@ struct MinresS;
 // numbering of feec_kuhn_grid (sum_a c_a (n+1)^a); 0 otherwise.
 int kuhn_grid_n(Mesh& m);
 // The HX block preconditioner diag(B_{k-1}, B_k, I) of a KKT system of grade k
-// (values from s's preconditioner blocks); nullptr, with the reason, when it
-// does not apply (no Kuhn grid, no coordinates, non-Euclidean, k > 2).
-Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::string* why);
+// (values from s's preconditioner blocks), each block accelerated by a
+// Chebyshev polynomial in B P with cheb_m applications of B (1: plain B) on
+// [frac b, b]; nullptr, with the reason, when it does not apply (no Kuhn
+// grid, no coordinates, non-Euclidean, k > 2).
+Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m, double frac,
+              std::string* why);
+void hx_info(const Hx& h, int* cheb_m, double lam[2]);
 // New values after re-assembly (same pattern).
 void hx_refresh(Hx& h, Geom& g, const KktSystem& s);
 void hx_free(Hx* h);
 int64_t hx_bytes_per_apply(const Hx& h);
 int hx_launches_per_apply(const Hx& h);
-// z = B r with partial sums of r.z into part (red_grid(N) blocks): FP64
+// z = B r, or p(B P) B r per block with the Chebyshev acceleration, with
+// partial sums of r.z into part (red_grid(N) blocks): FP64
 // vectors with the cycle in FP32 (fp32) or FP64, or FP32 throughout.
 void hx_enqueue(Enq& q, Hx& h, const double* r, double* z, const MinresS* S, double* part, bool fp32);
 void hx_enqueue(Enq& q, Hx& h, const float* r, float* z, const MinresS* S, double* part);

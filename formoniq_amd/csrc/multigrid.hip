@@ -461,6 +461,99 @@ __global__ void k_scale(const double* __restrict__ a, int64_t n, double s, doubl
 }
 
 // ---------------------------------------------------------------------------
+// Chebyshev acceleration of the HX blocks: z = p(B P) B r per block
+// ---------------------------------------------------------------------------
+// coef: [1/theta_sigma, 1/theta_u, then per step k: rho_{k+1} rho_k,
+// 2 rho_{k+1}/delta_sigma, 2 rho_{k+1}/delta_u]; row i < split is sigma's.
+
+// y = P x, P the KKT preconditioner matrix (CSR); LPR lanes per row.
+template <class TV, class TP, int LPR>
+__global__ void __launch_bounds__(TPB) k_pc_spmv(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                 const TV* __restrict__ val, const TP* __restrict__ x,
+                                                 TP* __restrict__ y, int64_t nrows, const MinresS* S) {
+  if (skip(S)) return;
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gid / LPR;
+  const int lane = (int)(gid % LPR);
+  TP acc = TP(0);
+  if (row < nrows)
+    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LPR) acc += (TP)val[p] * x[col[p]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (row < nrows && lane == 0) y[row] = acc;
+}
+
+// res = zt, d = res / theta, z = 0
+template <class TP>
+__global__ void __launch_bounds__(TPB) k_hxc_first(const TP* __restrict__ zt, TP* __restrict__ res,
+                                                   TP* __restrict__ d, TP* __restrict__ z, int64_t n,
+                                                   int64_t split, const double* __restrict__ coef,
+                                                   const MinresS* S) {
+  if (skip(S)) return;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const TP r = zt[i];
+  res[i] = r;
+  d[i] = r * (TP)coef[i < split ? 0 : 1];
+  z[i] = TP(0);
+}
+
+// step k: res -= zt, z += d, d = rho_{k+1} rho_k d + 2 rho_{k+1}/delta res
+template <class TP>
+__global__ void __launch_bounds__(TPB) k_hxc_step(const TP* __restrict__ zt, TP* __restrict__ res,
+                                                  TP* __restrict__ d, TP* __restrict__ z, int64_t n,
+                                                  int64_t split, const double* __restrict__ coef, int k,
+                                                  const MinresS* S) {
+  if (skip(S)) return;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* c = coef + 2 + 3 * k;
+  const TP r = res[i] - zt[i];
+  const TP di = d[i];
+  res[i] = r;
+  z[i] += di;
+  d[i] = (TP)c[0] * di + (TP)c[i < split ? 1 : 2] * r;
+}
+
+// out = z + d on the preconditioned rows, r on the border rows, and the
+// partial sums of r . out (red_grid(N) blocks).
+template <class TIN, class TP, class TOUT>
+__global__ void __launch_bounds__(TPB) k_hxc_final(const TIN* __restrict__ r, const TP* __restrict__ z,
+                                                   const TP* __restrict__ d, TOUT* __restrict__ out, int64_t n,
+                                                   int64_t N, const MinresS* S, double* __restrict__ part) {
+  if (skip(S)) return;
+  __shared__ double sh[TPB / 64];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const TOUT v = i < n ? (TOUT)(z[i] + d[i]) : (TOUT)r[i];
+    out[i] = v;
+    acc += (double)r[i] * (double)v;
+  }
+  const double t = dev_block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// partial sums of x[i]^2 over [lo, hi) (red_grid(hi - lo) blocks)
+__global__ void __launch_bounds__(TPB) k_sumsq(const double* __restrict__ x, int64_t lo, int64_t hi,
+                                               double* __restrict__ part) {
+  __shared__ double sh[TPB / 64];
+  double acc = 0.0;
+  for (int64_t i = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hi;
+       i += (int64_t)gridDim.x * blockDim.x)
+    acc += x[i] * x[i];
+  const double t = dev_block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// x = y scaled by s0 on [0, split) and s1 on [split, n)
+__global__ void k_block_scale(const double* __restrict__ y, double* __restrict__ x, int64_t n, int64_t split,
+                              double s0, double s1) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = y[i] * (i < split ? s0 : s1);
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 struct HostCsr {
@@ -626,6 +719,19 @@ struct Hx {
   std::vector<HxBlock> blocks;
   DBuf<double> s64, zt64;  // KKT-vector smoother diagonal and work vector
   DBuf<float> s32, zt32;
+  // Chebyshev acceleration (cheb_m applications of B per block application)
+  int cheb_m = 1;
+  double frac = 0.25;
+  double lam[2] = {0.0, 0.0};  // lambda_max of B P per block (sigma, u)
+  int64_t split = 0;
+  const int64_t* P_rp = nullptr;
+  const int32_t* P_col = nullptr;
+  const double* P_val = nullptr;
+  int64_t P_nnz = 0;
+  DBuf<float> P32;
+  DBuf<double> coef, pvec, part;
+  DBuf<double> res64, d64, zc64, w64;
+  DBuf<float> res32, d32, zc32, w32;
   int64_t bytes32 = 0, launches = 0;
 };
 
@@ -950,7 +1056,22 @@ void smoother_diag(Hx& h, const KktSystem& s) {
 
 }  // namespace
 
-Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::string* why) {
+namespace {
+void hx_spectrum(Hx& h);
+
+// The KKT preconditioner matrix P (blockdiag(hdif_gram(k-1), hdif_gram(k)))
+// the Chebyshev steps multiply by, and its FP32 copy.
+void bind_pc_matrix(Hx& h, const KktSystem& s) {
+  h.P_rp = s.P.row_ptr.p;
+  h.P_col = s.P.col.p;
+  h.P_val = s.P.val.p;
+  h.P_nnz = s.P.nnz;
+  if (h.cheb_m > 1) to32(h.ctx, s.P.val.p, s.P.nnz, h.P32);
+}
+}  // namespace
+
+Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m, double frac,
+              std::string* why) {
   Mesh& m = *g.mesh;
   Ctx* ctx = m.ctx;
   auto no = [&](const std::string& msg) -> Hx* {
@@ -975,6 +1096,11 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::strin
   h->sweeps = std::max(1, sweeps);
   h->n = s.n;
   h->N = s.n + s.nh;
+  h->split = s.split;
+  h->cheb_m = std::max(1, cheb_m);
+  h->frac = frac;
+  if (h->cheb_m > 1 && s.P.nnz == 0) return no("no preconditioner matrix for the Chebyshev steps");
+  if (!(frac > 0.0 && frac < 1.0)) return no("the Chebyshev interval fraction must lie in (0, 1)");
   const bool rel = bc == FEEC_RELATIVE;
   if (rel) m.ensure_interior();
   const int NS = nstencil(d);
@@ -1142,6 +1268,20 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::strin
   h->zt64.alloc(ctx, std::max<int64_t>(h->N, 1));
   h->zt32.alloc(ctx, std::max<int64_t>(h->N, 1));
   smoother_diag(*h, s);
+  // the Chebyshev steps
+  bind_pc_matrix(*h, s);
+  if (h->cheb_m > 1) {
+    const int64_t nn = std::max<int64_t>(h->n, 1);
+    for (DBuf<double>* b : {&h->res64, &h->d64, &h->zc64, &h->w64, &h->pvec}) b->alloc(ctx, nn);
+    for (DBuf<float>* b : {&h->res32, &h->d32, &h->zc32, &h->w32}) b->alloc(ctx, nn);
+    h->coef.alloc(ctx, 2 + 3 * (h->cheb_m - 1));
+    h->part.alloc(ctx, 4 * (size_t)NBLK);
+    try {
+      hx_spectrum(*h);
+    } catch (const Error& e) {
+      return no(e.what());
+    }
+  }
   // work estimate of one application in FP32
   int64_t bytes = 0, launches = 1;
   for (size_t i = 0; i < h->L.size(); ++i) {
@@ -1161,17 +1301,25 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, std::strin
         bytes += x->m.nnz * 4 * (1 + x->C) + x->m.nrows * 8;
         launches += 1;
       }
+  if (h->cheb_m > 1) {
+    bytes = bytes * h->cheb_m + (h->cheb_m - 1) * (s.P.nnz * 8 + h->n * 28) + h->n * 12;
+    launches = launches * h->cheb_m + 2 * (h->cheb_m - 1) + 2;
+  }
   h->bytes32 = bytes;
   h->launches = launches;
   ctx->sync();
   return h.release();
 }
 
+
 void hx_refresh(Hx& h, Geom& g, const KktSystem& s) {
   Csr A0;
   nodal_operator(g, h.bc, s, h.k, A0);
   hierarchy_values(h, A0, false);
   smoother_diag(h, s);
+  FEEC_REQUIRE(s.P.nnz == h.P_nnz, FEEC_E_DEVICE, "preconditioner matrix pattern changed");
+  bind_pc_matrix(h, s);
+  hx_spectrum(h);
   h.ctx->sync();
 }
 
@@ -1194,6 +1342,11 @@ struct Sel<double> {
   static const double* sl(const HxBlock& b) { return b.sl.p; }
   static double* t(HxBlock& b) { return b.t64.p; }
   static double* u(HxBlock& b) { return b.u64.p; }
+  static double* res(Hx& h) { return h.res64.p; }
+  static double* d(Hx& h) { return h.d64.p; }
+  static double* zc(Hx& h) { return h.zc64.p; }
+  static double* w(Hx& h) { return h.w64.p; }
+  static const double* pval(const Hx& h) { return h.P_val; }
 };
 template <>
 struct Sel<float> {
@@ -1210,6 +1363,11 @@ struct Sel<float> {
   static const float* sl(const HxBlock& b) { return b.sl32.p; }
   static float* t(HxBlock& b) { return b.t32.p; }
   static float* u(HxBlock& b) { return b.u32.p; }
+  static float* res(Hx& h) { return h.res32.p; }
+  static float* d(Hx& h) { return h.d32.p; }
+  static float* zc(Hx& h) { return h.zc32.p; }
+  static float* w(Hx& h) { return h.w32.p; }
+  static const float* pval(const Hx& h) { return h.P32.p; }
 };
 
 // One V-cycle on level i with its rhs in r(i); returns the buffer holding
@@ -1255,8 +1413,9 @@ void gather(Enq& q, const XCsr& g, const TX* x, TP* y, int nv, int slot0, const 
       S);
 }
 
-template <class TIN, class TP, class TOUT, int D>
-void hx_apply_d(Enq& q, Hx& h, const TIN* r, TOUT* z, const MinresS* S, double* part) {
+// zt = B r on the preconditioned rows (r of length n at least).
+template <class TIN, class TP, int D>
+void hx_core_d(Enq& q, Hx& h, const TIN* r, const MinresS* S) {
   using X = Sel<TP>;
   MgLevel& l0 = h.L[0];
   TP* slots = X::r(l0);
@@ -1291,7 +1450,30 @@ void hx_apply_d(Enq& q, Hx& h, const TIN* r, TOUT* z, const MinresS* S, double* 
         b.j == 1 ? 1 : 0, xs, (const TP*)slots, X::dinv(l0), nv, b.slot_main, sb, rb, zb, om.m.nrows, 0, S);
     }
   }
-  q(k_hx_final<TIN, TP, TOUT>, red_grid(h.N), TPB, r, (const TP*)zt, z, h.n, h.N, S, part);
+}
+
+// z = B r (cheb_m = 1) or p(B P) B r per block, z = r on the border rows,
+// and the partial sums of r . z.
+template <class TIN, class TP, class TOUT, int D>
+void hx_apply_d(Enq& q, Hx& h, const TIN* r, TOUT* z, const MinresS* S, double* part) {
+  using X = Sel<TP>;
+  hx_core_d<TIN, TP, D>(q, h, r, S);
+  if (h.cheb_m <= 1) {
+    q(k_hx_final<TIN, TP, TOUT>, red_grid(h.N), TPB, r, (const TP*)X::zt(h), z, h.n, h.N, S, part);
+    return;
+  }
+  TP* res = X::res(h);
+  TP* d = X::d(h);
+  TP* zc = X::zc(h);
+  TP* w = X::w(h);
+  const double* coef = h.coef.p;
+  q(k_hxc_first<TP>, grid_for(h.n), TPB, (const TP*)X::zt(h), res, d, zc, h.n, h.split, coef, S);
+  for (int k = 0; k + 1 < h.cheb_m; ++k) {
+    q(k_pc_spmv<TP, TP, 4>, grid_for(h.n * 4), TPB, h.P_rp, h.P_col, X::pval(h), (const TP*)d, w, h.n, S);
+    hx_core_d<TP, TP, D>(q, h, (const TP*)w, S);
+    q(k_hxc_step<TP>, grid_for(h.n), TPB, (const TP*)X::zt(h), res, d, zc, h.n, h.split, coef, k, S);
+  }
+  q(k_hxc_final<TIN, TP, TOUT>, red_grid(h.N), TPB, r, (const TP*)zc, (const TP*)d, z, h.n, h.N, S, part);
 }
 
 template <class TIN, class TP, class TOUT>
@@ -1304,7 +1486,99 @@ void hx_apply(Enq& q, Hx& h, const TIN* r, TOUT* z, const MinresS* S, double* pa
   }
 }
 
+void hx_core(Enq& q, Hx& h, const double* r) {
+  switch (h.d) {
+    case 1: hx_core_d<double, double, 1>(q, h, r, nullptr); break;
+    case 2: hx_core_d<double, double, 2>(q, h, r, nullptr); break;
+    case 3: hx_core_d<double, double, 3>(q, h, r, nullptr); break;
+    default: hx_core_d<double, double, 4>(q, h, r, nullptr); break;
+  }
+}
+
+// lambda_max of B P per block by kPowerIters power iterations (FP64) from the
+// same deterministic start every time, then the Chebyshev coefficients.
+constexpr int kPowerIters = 12;
+constexpr double kLmaxMargin = 1.05;
+
+void hx_spectrum(Hx& h) {
+  Ctx* ctx = h.ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = h.n, split = h.split;
+  if (h.cheb_m <= 1 || n == 0) return;
+  {
+    std::vector<double> x(n);
+    uint64_t z = 0x2545F4914F6CDD1DULL;
+    for (int64_t i = 0; i < n; ++i) {  // 0.5 + U[0,1) from xorshift64 (13, 7, 17)
+      z ^= z << 13;
+      z ^= z >> 7;
+      z ^= z << 17;
+      x[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    }
+    copy_to_device(ctx, h.pvec.p, x.data(), n * 8);
+  }
+  Enq q{st};
+  const int64_t lo[2] = {0, split}, hi[2] = {split, n};
+  std::vector<double> part(4 * (size_t)NBLK);
+  for (int it = 0; it < kPowerIters; ++it) {
+    k_pc_spmv<double, double, 4><<<grid_for(n * 4), TPB, 0, st>>>(h.P_rp, h.P_col, h.P_val, h.pvec.p, h.w64.p,
+                                                                  n, nullptr);
+    FEEC_HIP(hipGetLastError());
+    hx_core(q, h, h.w64.p);
+    unsigned g[2];
+    for (int b = 0; b < 2; ++b) {
+      g[b] = red_grid(hi[b] - lo[b]);
+      k_sumsq<<<g[b], TPB, 0, st>>>(h.zt64.p, lo[b], hi[b], h.part.p + (2 * b) * NBLK);
+      k_sumsq<<<g[b], TPB, 0, st>>>(h.pvec.p, lo[b], hi[b], h.part.p + (2 * b + 1) * NBLK);
+    }
+    FEEC_HIP(hipGetLastError());
+    copy_to_host(ctx, part.data(), h.part.p, part.size() * 8);
+    double sc[2] = {1.0, 1.0};
+    for (int b = 0; b < 2; ++b) {
+      if (hi[b] <= lo[b]) continue;
+      double yy = 0.0, xx = 0.0;
+      for (unsigned i = 0; i < g[b]; ++i) {
+        yy += part[(2 * b) * NBLK + i];
+        xx += part[(2 * b + 1) * NBLK + i];
+      }
+      FEEC_REQUIRE(yy > 0.0 && xx > 0.0 && std::isfinite(yy), FEEC_E_NOT_SPD,
+                   "multigrid preconditioner spectrum estimate failed");
+      h.lam[b] = std::sqrt(yy / xx);
+      sc[b] = 1.0 / std::sqrt(yy);
+    }
+    k_block_scale<<<grid_for(n), TPB, 0, st>>>(h.zt64.p, h.pvec.p, n, split, sc[0], sc[1]);
+    FEEC_HIP(hipGetLastError());
+  }
+  // coefficients on [frac b, b], b = 1.05 lambda_max, per block
+  std::vector<double> c(2 + 3 * (size_t)(h.cheb_m - 1));
+  double theta[2], delta[2];
+  for (int b = 0; b < 2; ++b) {
+    const double bb = kLmaxMargin * (h.lam[b] > 0.0 ? h.lam[b] : 1.0), aa = h.frac * bb;
+    theta[b] = 0.5 * (bb + aa);
+    delta[b] = 0.5 * (bb - aa);
+    c[b] = 1.0 / theta[b];
+  }
+  const double sigma1 = (1.0 + h.frac) / (1.0 - h.frac);  // theta / delta, both blocks
+  double rho = 1.0 / sigma1;
+  for (int k = 0; k + 1 < h.cheb_m; ++k) {
+    const double rn = 1.0 / (2.0 * sigma1 - rho);
+    c[2 + 3 * k] = rn * rho;
+    c[3 + 3 * k] = 2.0 * rn / delta[0];
+    c[4 + 3 * k] = 2.0 * rn / delta[1];
+    rho = rn;
+  }
+  copy_to_device(ctx, h.coef.p, c.data(), c.size() * 8);
+  ctx->sync();
+}
+
 }  // namespace
+
+void hx_info(const Hx& h, int* cheb_m, double lam[2]) {
+  if (cheb_m) *cheb_m = h.cheb_m;
+  if (lam) {
+    lam[0] = h.lam[0];
+    lam[1] = h.lam[1];
+  }
+}
 
 void hx_enqueue(Enq& q, Hx& h, const double* r, double* z, const MinresS* S, double* part, bool fp32) {
   if (fp32)

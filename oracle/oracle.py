@@ -998,15 +998,72 @@ class HxPreconditioner:
         return z
 
 
-def hx_block_preconditioner(c: WhitneyComplex, coords, k, n, nh, sweeps=2):
-    """The KKT block preconditioner diag(B_{k-1}, B_k, I) (elliptic.rs:29-47)
-    with HX blocks; a callable r -> z."""
+HX_CHEB_FRACTION = 0.25  # lower end of the interval, a fraction of its upper end
+
+
+def hx_block_lambdas(c: WhitneyComplex, coords, k, n, iters=CHEB_POWER_ITERS, sweeps=2):
+    """lambda_max of B_j hdif_gram(j) per block (sigma, u) by power iteration
+    from the xorshift start, as for the Jacobi polynomial."""
     ns, nu = c.ndofs(k - 1), c.ndofs(k)
     hu = HxPreconditioner(c, coords, k, n, sweeps)
     hs = HxPreconditioner(c, coords, k - 1, n, sweeps, hu.nodal) if ns else None
+    blocks = [(hs, c.hdif_gram(k - 1).to_scipy().tocsr() if ns else None, 0, ns),
+              (hu, c.hdif_gram(k).to_scipy().tocsr(), ns, ns + nu)]
+    x = _xorshift_start(ns + nu)
+    lam = [1.0, 1.0]
+    for _ in range(iters):
+        for b, (B, A, lo, hi) in enumerate(blocks):
+            if hi <= lo:
+                continue
+            y = B.apply(A @ x[lo:hi])
+            ny, nx = np.linalg.norm(y), np.linalg.norm(x[lo:hi])
+            lam[b] = ny / nx
+            x[lo:hi] = y / ny
+    return lam
+
+
+def hx_block_preconditioner(c: WhitneyComplex, coords, k, n, nh, sweeps=2, cheb_m=1, lam=None,
+                            frac=HX_CHEB_FRACTION):
+    """The KKT block preconditioner diag(B_{k-1}, B_k, I) (elliptic.rs:29-47)
+    with HX blocks; a callable r -> z. With cheb_m > 1 each block is
+    p(B A) B with p the degree-(cheb_m - 1) Chebyshev polynomial of the
+    interval [frac b, b], b = 1.05 lambda_max(B A) (Saad, Algorithm 12.1,
+    with B in place of the Jacobi diagonal): cheb_m applications of B and
+    cheb_m - 1 of A = hdif_gram per block. `lam`: lambda_max per block
+    (sigma, u), by default from hx_block_lambdas."""
+    ns, nu = c.ndofs(k - 1), c.ndofs(k)
+    hu = HxPreconditioner(c, coords, k, n, sweeps)
+    hs = HxPreconditioner(c, coords, k - 1, n, sweeps, hu.nodal) if ns else None
+    if cheb_m <= 1:
+        def prec(r):
+            parts = [hs.apply(r[:ns])] if ns else []
+            parts += [hu.apply(r[ns:ns + nu]), r[ns + nu:]]
+            return np.concatenate(parts)
+        return prec
+    if lam is None:
+        lam = hx_block_lambdas(c, coords, k, n, sweeps=sweeps)
+    As = c.hdif_gram(k - 1).to_scipy().tocsr() if ns else None
+    Au = c.hdif_gram(k).to_scipy().tocsr()
+
+    def cheb(B, A, lmax, r):
+        b = CHEB_LMAX_MARGIN * lmax
+        a = frac * b
+        theta, delta = 0.5 * (b + a), 0.5 * (b - a)
+        sigma1 = theta / delta
+        rho = 1.0 / sigma1
+        res = B.apply(r)
+        d = res / theta
+        z = np.zeros_like(r)
+        for _ in range(cheb_m - 1):
+            res = res - B.apply(A @ d)
+            z = z + d
+            rho_next = 1.0 / (2.0 * sigma1 - rho)
+            d = rho_next * rho * d + (2.0 * rho_next / delta) * res
+            rho = rho_next
+        return z + d
 
     def prec(r):
-        parts = [hs.apply(r[:ns])] if ns else []
-        parts += [hu.apply(r[ns:ns + nu]), r[ns + nu:]]
+        parts = [cheb(hs, As, lam[0], r[:ns])] if ns else []
+        parts += [cheb(hu, Au, lam[1], r[ns:ns + nu]), r[ns + nu:]]
         return np.concatenate(parts)
     return prec

@@ -26,33 +26,49 @@ CASES = [(2, 12, 0.1, 1, False), (2, 13, 0.1, 1, True), (2, 10, 0.1, 0, False), 
          (3, 6, 0.05, 1, False), (3, 5, 0.05, 1, True), (3, 5, 0.05, 2, False), (3, 6, 0.05, 0, True)]
 
 
+@pytest.mark.parametrize("m", [1, 4])
 @pytest.mark.parametrize("d,n,amp,k,rel", CASES)
-def test_hx_application_matches_oracle(d, n, amp, k, rel):
-    """z = B r with B = diag(HX(k-1), HX(k), I): FP64 equal to the oracle's to
-    1e-10 (summation order and the coarse inverse's method differ), FP32 to 1e-5."""
+def test_hx_application_matches_oracle(d, n, amp, k, rel, m):
+    """z = B r with B = diag(HX(k-1), HX(k), I) (m = 1), or each block
+    accelerated by the Chebyshev polynomial in B hdif_gram with m
+    applications of B: FP64 equal to the oracle's to 1e-10 (summation order
+    and the coarse inverse's method differ), FP32 to 1e-5 (m = 1) or 1e-4
+    (m = 4: four cycles and three products by hdif_gram in FP32). The
+    lambda_max estimates (12 power iterations from the same start) equal the
+    oracle's to 1e-9."""
     ocx, oL, gcx, geo, cells, coords = both(d, n, amp, seed=n)
     gc = fa.WhitneyComplex(geo, relative=rel)
-    prob = fa.Problem(gc, k, mg_opts(precision=fa.FP64))
+    prob = fa.Problem(gc, k, mg_opts(precision=fa.FP64, cheb_degree=m))
     prob.assemble()
     oc = o.WhitneyComplex(ocx, oL, relative=rel)
     ns, nu, nh = prob.sizes
-    prec = o.hx_block_preconditioner(oc, coords, k, n, nh)
+    info = prob.preconditioner()
+    assert info["degree"] == m
+    lam = None
+    if m > 1:
+        lam = info["lambda_max"]
+        want_lam = o.hx_block_lambdas(oc, coords, k, n, iters=12)
+        for b in range(2 if k > 0 else 1):
+            i = b if k > 0 else 1
+            assert abs(lam[i] - want_lam[i]) <= 1e-9 * want_lam[i], (lam, want_lam)
+    prec = o.hx_block_preconditioner(oc, coords, k, n, nh, cheb_m=m, lam=lam)
     r = np.sin(np.arange(ns + nu + nh) * 0.7 + 0.3)
     want = prec(r)
     got = prob.apply_preconditioner(r, precision=fa.FP64)
     scale = np.abs(want).max()
     assert np.abs(got - want).max() <= 1e-10 * scale, np.abs(got - want).max() / scale
+    tol32 = 1e-5 if m == 1 else 1e-4
     got32 = prob.apply_preconditioner(r, precision=fa.MIXED)
-    assert np.abs(got32 - want).max() <= 1e-5 * scale, np.abs(got32 - want).max() / scale
+    assert np.abs(got32 - want).max() <= tol32 * scale, np.abs(got32 - want).max() / scale
     got_pc32 = prob.apply_preconditioner(r, precision=fa.FP64_PC32)
-    assert np.abs(got_pc32 - want).max() <= 1e-5 * scale
+    assert np.abs(got_pc32 - want).max() <= tol32 * scale
 
 
-def _iters(d, n, k, rel, amp, precision=fa.FP64):
+def _iters(d, n, k, rel, amp, precision=fa.FP64, m=1):
     ocx, oL, gcx, geo, _, _ = both(d, n, amp, seed=0)
     gc = fa.WhitneyComplex(geo, relative=rel)
     f = np.array([((i % 5) - 2.0) * 0.3 for i in range(gcx.nsimplices(k))])
-    opts = mg_opts(precision=precision, max_refinements=0)
+    opts = mg_opts(precision=precision, max_refinements=0, cheb_degree=m)
     sig, u, p, rep = fa.solve_source(gc, f, k, opts, raise_on_failure=False)
     return rep
 
@@ -72,6 +88,11 @@ def test_block_minres_iteration_count_is_mesh_independent():
     for coarse, fine in zip(counts, counts[1:]):
         assert fine <= coarse + 16, counts
     assert counts[-1] <= 1.4 * counts[0], counts
+    accel = [_iters(2, n, 1, True, 0.0, m=4).iters for n in (16, 32, 64, 128)]
+    print("  with the degree-3 Chebyshev acceleration (m = 4):", accel, flush=True)
+    for coarse, fine in zip(accel, accel[1:]):
+        assert fine <= coarse + 8, accel
+    assert max(accel) <= 0.4 * min(counts), (accel, counts)
     ocx, oL, gcx, geo, cells, coords = both(2, 16, 0.0, seed=0)
     oc = o.WhitneyComplex(ocx, oL, relative=True)
     f = np.array([((i % 5) - 2.0) * 0.3 for i in range(ocx.nsimplices(1))])
@@ -82,12 +103,16 @@ def test_block_minres_iteration_count_is_mesh_independent():
 
 
 def test_3d_iteration_count_is_bounded():
-    """3D k = 1 on the jittered full complex: 8^3, 16^3, 32^3, 64^3."""
+    """3D k = 1 on the jittered full complex: 8^3, 16^3, 32^3, 64^3, plain
+    HX blocks and with the Chebyshev acceleration (m = 4)."""
     counts = [_iters(3, n, 1, False, 0.05).iters for n in (8, 16, 32, 64)]
-    print("3D k=1 iterations by n = 8, 16, 32, 64:", counts, flush=True)
+    accel = [_iters(3, n, 1, False, 0.05, m=4).iters for n in (8, 16, 32, 64)]
+    print("3D k=1 iterations by n = 8, 16, 32, 64:", counts, "with m = 4:", accel, flush=True)
     for coarse, fine in zip(counts, counts[1:]):
         assert fine <= coarse + 16, counts
     assert counts[-1] <= 1.4 * counts[0], counts
+    for coarse, fine in zip(accel, accel[1:]):
+        assert fine <= coarse + 8, accel
 
 
 @pytest.mark.parametrize("precision", [fa.FP64, fa.MIXED, fa.FP64_PC32])
