@@ -468,7 +468,7 @@ constexpr int64_t kMultigridAutoRows = 20000;
 // Chebyshev acceleration of the HX blocks: applications of B per block
 // application (cheb_degree = -1) and the interval's lower end as a fraction of
 // its upper one (cheb_lower <= 0).
-constexpr int kMultigridChebDefault = 4;
+constexpr int kMultigridChebDefault = 5;
 constexpr double kMultigridChebFraction = 0.25;
 void problem_multigrid(Problem& P) {
   const int pc = P.opts.preconditioner;

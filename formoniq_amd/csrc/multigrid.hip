@@ -534,24 +534,32 @@ __global__ void __launch_bounds__(TPB) k_hxc_final(const TIN* __restrict__ r, co
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// partial sums of x[i]^2 over [lo, hi) (red_grid(hi - lo) blocks)
-__global__ void __launch_bounds__(TPB) k_sumsq(const double* __restrict__ x, int64_t lo, int64_t hi,
-                                               double* __restrict__ part) {
+// partial sums of x[i] y[i] over [lo, hi) (red_grid(hi - lo) blocks)
+__global__ void __launch_bounds__(TPB) k_dotr(const double* __restrict__ x, const double* __restrict__ y,
+                                              int64_t lo, int64_t hi, double* __restrict__ part) {
   __shared__ double sh[TPB / 64];
   double acc = 0.0;
   for (int64_t i = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hi;
        i += (int64_t)gridDim.x * blockDim.x)
-    acc += x[i] * x[i];
+    acc += x[i] * y[i];
   const double t = dev_block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// x = y scaled by s0 on [0, split) and s1 on [split, n)
-__global__ void k_block_scale(const double* __restrict__ y, double* __restrict__ x, int64_t n, int64_t split,
-                              double s0, double s1) {
+// y += a_b x, a_0 on [0, split) and a_1 on [split, n)
+__global__ void k_block_axpy(const double* __restrict__ x, double* __restrict__ y, int64_t n, int64_t split,
+                             double a0, double a1) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) x[i] = y[i] * (i < split ? s0 : s1);
+  if (i < n) y[i] += (i < split ? a0 : a1) * x[i];
 }
+
+// p = z + b_b p
+__global__ void k_block_xpby(const double* __restrict__ z, double* __restrict__ p, int64_t n, int64_t split,
+                             double b0, double b1) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + (i < split ? b0 : b1) * p[i];
+}
+
 
 // ---------------------------------------------------------------------------
 // host helpers
@@ -1495,58 +1503,120 @@ void hx_core(Enq& q, Hx& h, const double* r) {
   }
 }
 
-// lambda_max of B P per block by kPowerIters power iterations (FP64) from the
-// same deterministic start every time, then the Chebyshev coefficients.
-constexpr int kPowerIters = 12;
+// lambda_max of B P per block: kLanczosSteps steps of CG on P preconditioned
+// by B (both blocks at once, each with its own scalars) from the same
+// deterministic start every time, and the largest eigenvalue of the Lanczos
+// tridiagonal their coefficients make; then the Chebyshev coefficients.
+constexpr int kLanczosSteps = 6;
 constexpr double kLmaxMargin = 1.05;
+
+// Largest eigenvalue of the symmetric tridiagonal (diag, off): bisection on
+// Sturm counts inside the Gershgorin bound.
+double tridiagonal_lmax(const std::vector<double>& diag, const std::vector<double>& off) {
+  const int n = (int)diag.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i ? std::fabs(off[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(off[i]) : 0.0);
+    lo = std::min(lo, diag[i] - r);
+    hi = std::max(hi, diag[i] + r);
+  }
+  for (int it = 0; it < 60; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    int count = 0;
+    double q = 1.0;
+    for (int i = 0; i < n; ++i) {  // eigenvalues below mid = negative pivots of T - mid I
+      q = diag[i] - mid - (i ? off[i - 1] * off[i - 1] / q : 0.0);
+      if (q == 0.0) q = -1e-300;
+      count += q < 0.0;
+    }
+    if (count == n)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return hi;
+}
 
 void hx_spectrum(Hx& h) {
   Ctx* ctx = h.ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = h.n, split = h.split;
   if (h.cheb_m <= 1 || n == 0) return;
+  double* r = h.pvec.p;
+  double* pd = h.d64.p;
+  double* w = h.w64.p;
+  double* z = h.zt64.p;
   {
     std::vector<double> x(n);
-    uint64_t z = 0x2545F4914F6CDD1DULL;
+    uint64_t v = 0x2545F4914F6CDD1DULL;
     for (int64_t i = 0; i < n; ++i) {  // 0.5 + U[0,1) from xorshift64 (13, 7, 17)
-      z ^= z << 13;
-      z ^= z >> 7;
-      z ^= z << 17;
-      x[i] = 0.5 + (double)(z >> 11) * (1.0 / 9007199254740992.0);
+      v ^= v << 13;
+      v ^= v >> 7;
+      v ^= v << 17;
+      x[i] = 0.5 + (double)(v >> 11) * (1.0 / 9007199254740992.0);
     }
-    copy_to_device(ctx, h.pvec.p, x.data(), n * 8);
+    copy_to_device(ctx, r, x.data(), n * 8);
   }
   Enq q{st};
   const int64_t lo[2] = {0, split}, hi[2] = {split, n};
-  std::vector<double> part(4 * (size_t)NBLK);
-  for (int it = 0; it < kPowerIters; ++it) {
-    k_pc_spmv<double, double, 4><<<grid_for(n * 4), TPB, 0, st>>>(h.P_rp, h.P_col, h.P_val, h.pvec.p, h.w64.p,
-                                                                  n, nullptr);
-    FEEC_HIP(hipGetLastError());
-    hx_core(q, h, h.w64.p);
+  const bool has[2] = {split > 0, n > split};
+  std::vector<double> part(2 * (size_t)NBLK);
+  auto dots = [&](const double* x, const double* y, double out[2]) {
     unsigned g[2];
     for (int b = 0; b < 2; ++b) {
       g[b] = red_grid(hi[b] - lo[b]);
-      k_sumsq<<<g[b], TPB, 0, st>>>(h.zt64.p, lo[b], hi[b], h.part.p + (2 * b) * NBLK);
-      k_sumsq<<<g[b], TPB, 0, st>>>(h.pvec.p, lo[b], hi[b], h.part.p + (2 * b + 1) * NBLK);
+      if (has[b]) k_dotr<<<g[b], TPB, 0, st>>>(x, y, lo[b], hi[b], h.part.p + b * NBLK);
     }
     FEEC_HIP(hipGetLastError());
     copy_to_host(ctx, part.data(), h.part.p, part.size() * 8);
-    double sc[2] = {1.0, 1.0};
     for (int b = 0; b < 2; ++b) {
-      if (hi[b] <= lo[b]) continue;
-      double yy = 0.0, xx = 0.0;
-      for (unsigned i = 0; i < g[b]; ++i) {
-        yy += part[(2 * b) * NBLK + i];
-        xx += part[(2 * b + 1) * NBLK + i];
-      }
-      FEEC_REQUIRE(yy > 0.0 && xx > 0.0 && std::isfinite(yy), FEEC_E_NOT_SPD,
-                   "multigrid preconditioner spectrum estimate failed");
-      h.lam[b] = std::sqrt(yy / xx);
-      sc[b] = 1.0 / std::sqrt(yy);
+      out[b] = 0.0;
+      if (has[b])
+        for (unsigned i = 0; i < g[b]; ++i) out[b] += part[b * NBLK + i];
     }
-    k_block_scale<<<grid_for(n), TPB, 0, st>>>(h.zt64.p, h.pvec.p, n, split, sc[0], sc[1]);
+  };
+  hx_core(q, h, r);
+  FEEC_HIP(hipMemcpyAsync(pd, z, n * 8, hipMemcpyDeviceToDevice, st));
+  double rho[2], pw[2], rn[2];
+  dots(r, z, rho);
+  std::vector<double> al[2], be[2];
+  for (int j = 0; j < kLanczosSteps; ++j) {
+    k_pc_spmv<double, double, 4><<<grid_for(n * 4), TPB, 0, st>>>(h.P_rp, h.P_col, h.P_val, pd, w, n, nullptr);
     FEEC_HIP(hipGetLastError());
+    dots(pd, w, pw);
+    double a[2] = {0.0, 0.0};
+    for (int b = 0; b < 2; ++b)
+      if (has[b]) {
+        FEEC_REQUIRE(pw[b] > 0.0 && rho[b] > 0.0 && std::isfinite(rho[b]), FEEC_E_NOT_SPD,
+                     "multigrid preconditioner spectrum estimate failed");
+        a[b] = rho[b] / pw[b];
+        al[b].push_back(a[b]);
+      }
+    k_block_axpy<<<grid_for(n), TPB, 0, st>>>(w, r, n, split, -a[0], -a[1]);
+    FEEC_HIP(hipGetLastError());
+    hx_core(q, h, r);
+    dots(r, z, rn);
+    double bt[2] = {0.0, 0.0};
+    for (int b = 0; b < 2; ++b)
+      if (has[b]) {
+        bt[b] = rn[b] / rho[b];
+        be[b].push_back(bt[b]);
+        rho[b] = rn[b];
+      }
+    k_block_xpby<<<grid_for(n), TPB, 0, st>>>(z, pd, n, split, bt[0], bt[1]);
+    FEEC_HIP(hipGetLastError());
+  }
+  for (int b = 0; b < 2; ++b) {
+    h.lam[b] = 0.0;
+    if (!has[b]) continue;
+    const int m = (int)al[b].size();
+    std::vector<double> diag(m), off(std::max(m - 1, 0));
+    for (int j = 0; j < m; ++j) diag[j] = 1.0 / al[b][j] + (j ? be[b][j - 1] / al[b][j - 1] : 0.0);
+    for (int j = 0; j + 1 < m; ++j) {
+      FEEC_REQUIRE(be[b][j] >= 0.0, FEEC_E_NOT_SPD, "multigrid preconditioner is not positive");
+      off[j] = std::sqrt(be[b][j]) / al[b][j];
+    }
+    h.lam[b] = tridiagonal_lmax(diag, off);
   }
   // coefficients on [frac b, b], b = 1.05 lambda_max, per block
   std::vector<double> c(2 + 3 * (size_t)(h.cheb_m - 1));
@@ -1558,13 +1628,13 @@ void hx_spectrum(Hx& h) {
     c[b] = 1.0 / theta[b];
   }
   const double sigma1 = (1.0 + h.frac) / (1.0 - h.frac);  // theta / delta, both blocks
-  double rho = 1.0 / sigma1;
+  double rk = 1.0 / sigma1;
   for (int k = 0; k + 1 < h.cheb_m; ++k) {
-    const double rn = 1.0 / (2.0 * sigma1 - rho);
-    c[2 + 3 * k] = rn * rho;
-    c[3 + 3 * k] = 2.0 * rn / delta[0];
-    c[4 + 3 * k] = 2.0 * rn / delta[1];
-    rho = rn;
+    const double rnext = 1.0 / (2.0 * sigma1 - rk);
+    c[2 + 3 * k] = rnext * rk;
+    c[3 + 3 * k] = 2.0 * rnext / delta[0];
+    c[4 + 3 * k] = 2.0 * rnext / delta[1];
+    rk = rnext;
   }
   copy_to_device(ctx, h.coef.p, c.data(), c.size() * 8);
   ctx->sync();

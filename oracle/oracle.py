@@ -1001,24 +1001,63 @@ class HxPreconditioner:
 HX_CHEB_FRACTION = 0.25  # lower end of the interval, a fraction of its upper end
 
 
-def hx_block_lambdas(c: WhitneyComplex, coords, k, n, iters=CHEB_POWER_ITERS, sweeps=2):
-    """lambda_max of B_j hdif_gram(j) per block (sigma, u) by power iteration
-    from the xorshift start, as for the Jacobi polynomial."""
+HX_LANCZOS_STEPS = 6
+
+
+def tridiagonal_lmax(diag, off):
+    """Largest eigenvalue of the symmetric tridiagonal matrix (diag, off) by
+    bisection on Sturm counts inside the Gershgorin bound (60 halvings)."""
+    n = len(diag)
+    lo = min(diag[i] - (abs(off[i - 1]) if i else 0) - (abs(off[i]) if i + 1 < n else 0) for i in range(n))
+    hi = max(diag[i] + (abs(off[i - 1]) if i else 0) + (abs(off[i]) if i + 1 < n else 0) for i in range(n))
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        count, q = 0, 1.0
+        for i in range(n):  # eigenvalues below mid = negative pivots of T - mid I
+            q = diag[i] - mid - (off[i - 1] ** 2 / q if i else 0.0)
+            if q == 0.0:
+                q = -1e-300
+            count += q < 0
+        if count == n:
+            hi = mid
+        else:
+            lo = mid
+    return hi
+
+
+def hx_block_lambdas(c: WhitneyComplex, coords, k, n, steps=HX_LANCZOS_STEPS, sweeps=2):
+    """lambda_max of B_j hdif_gram(j) per block (sigma, u): `steps` steps of
+    CG on hdif_gram(j) preconditioned by B_j from the xorshift start, and the
+    largest eigenvalue of the Lanczos tridiagonal their coefficients make
+    (T_jj = 1/alpha_j + beta_{j-1}/alpha_{j-1}, T_{j,j+1} = sqrt(beta_j)/alpha_j)."""
     ns, nu = c.ndofs(k - 1), c.ndofs(k)
     hu = HxPreconditioner(c, coords, k, n, sweeps)
     hs = HxPreconditioner(c, coords, k - 1, n, sweeps, hu.nodal) if ns else None
     blocks = [(hs, c.hdif_gram(k - 1).to_scipy().tocsr() if ns else None, 0, ns),
               (hu, c.hdif_gram(k).to_scipy().tocsr(), ns, ns + nu)]
-    x = _xorshift_start(ns + nu)
-    lam = [1.0, 1.0]
-    for _ in range(iters):
-        for b, (B, A, lo, hi) in enumerate(blocks):
-            if hi <= lo:
-                continue
-            y = B.apply(A @ x[lo:hi])
-            ny, nx = np.linalg.norm(y), np.linalg.norm(x[lo:hi])
-            lam[b] = ny / nx
-            x[lo:hi] = y / ny
+    v = _xorshift_start(ns + nu)
+    lam = [0.0, 0.0]
+    for b, (B, A, lo, hi) in enumerate(blocks):
+        if hi <= lo:
+            continue
+        r = v[lo:hi].copy()
+        z = B.apply(r)
+        p = z.copy()
+        rho = r @ z
+        al, be = [], []
+        for _ in range(steps):
+            q = A @ p
+            a = rho / (p @ q)
+            r = r - a * q
+            z = B.apply(r)
+            rn = r @ z
+            al.append(a)
+            be.append(rn / rho)
+            rho = rn
+            p = z + be[-1] * p
+        diag = [1.0 / al[j] + (be[j - 1] / al[j - 1] if j else 0.0) for j in range(steps)]
+        off = [math.sqrt(be[j]) / al[j] for j in range(steps - 1)]
+        lam[b] = tridiagonal_lmax(diag, off)
     return lam
 
 

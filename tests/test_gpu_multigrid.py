@@ -34,7 +34,7 @@ def test_hx_application_matches_oracle(d, n, amp, k, rel, m):
     applications of B: FP64 equal to the oracle's to 1e-10 (summation order
     and the coarse inverse's method differ), FP32 to 1e-5 (m = 1) or 1e-4
     (m = 4: four cycles and three products by hdif_gram in FP32). The
-    lambda_max estimates (12 power iterations from the same start) equal the
+    lambda_max estimates (6 Lanczos steps from the same start) equal the
     oracle's to 1e-9."""
     ocx, oL, gcx, geo, cells, coords = both(d, n, amp, seed=n)
     gc = fa.WhitneyComplex(geo, relative=rel)
@@ -47,7 +47,7 @@ def test_hx_application_matches_oracle(d, n, amp, k, rel, m):
     lam = None
     if m > 1:
         lam = info["lambda_max"]
-        want_lam = o.hx_block_lambdas(oc, coords, k, n, iters=12)
+        want_lam = o.hx_block_lambdas(oc, coords, k, n)
         for b in range(2 if k > 0 else 1):
             i = b if k > 0 else 1
             assert abs(lam[i] - want_lam[i]) <= 1e-9 * want_lam[i], (lam, want_lam)

@@ -10,7 +10,9 @@ precond.rs, stationary.rs) pinned by what the mathematics fixes:
 - the V-cycle is symmetric and contracts with an h-independent factor;
 - the vector-nodal interpolation maps the constant coefficient fields e_I to
   the de Rham cochains of the constant forms dx^I (hx.rs:86-112);
-- the auxiliary-space block preconditioner is symmetric positive definite;
+- the auxiliary-space block preconditioner, plain or Chebyshev-accelerated,
+  is symmetric positive definite, and the Lanczos estimate of its spectrum
+  is a lower bound close enough to keep the polynomial positive;
 - the reference's iteration law (elliptic.rs:338-367) holds for its exact
   block solves.
 
@@ -165,12 +167,13 @@ def test_vector_nodal_interpolates_constant_forms(d, k):
     assert constant_form_error(d, k) <= 1e-14
 
 
+@pytest.mark.parametrize("m", [1, 4])
 @pytest.mark.parametrize("d,n,k,rel", [(2, 8, 1, True), (2, 6, 2, False), (3, 3, 1, False)])
-def test_hx_block_preconditioner_is_spd(d, n, k, rel):
+def test_hx_block_preconditioner_is_spd(d, n, k, rel, m):
     cx, coords, L = grid(d, n, 0.1, 5)
     c = o.WhitneyComplex(cx, L, relative=rel)
     nh = c.harmonics(k).shape[1]
-    prec = o.hx_block_preconditioner(c, coords, k, n, nh)
+    prec = o.hx_block_preconditioner(c, coords, k, n, nh, cheb_m=m)
     N = c.ndofs(k - 1) + c.ndofs(k) + nh
     B = np.column_stack([prec(e) for e in np.eye(N)])
     assert np.abs(B - B.T).max() <= 1e-12 * np.abs(B).max()
@@ -216,3 +219,39 @@ def test_multigrid_laws_catch_planted_mistakes(mutation):
     assert vcycle_asymmetry()[0] > 1e-3
     mutation(33)
     assert constant_form_error(3, 1) > 1e-3 and constant_form_error(3, 2) > 1e-3
+
+
+@pytest.mark.parametrize("d,n,k,rel", [(2, 8, 1, True), (3, 3, 1, False), (3, 4, 2, False)])
+def test_lanczos_estimate_is_a_close_lower_bound(d, n, k, rel):
+    """The 6-step Lanczos estimate of lambda_max(B_j hdif_gram(j)) is a Ritz
+    value, so at most the true lambda_max (dense eigensolve of the
+    A-symmetric B A), and above 0.8 of it. The Chebyshev polynomial of
+    [b/4, b], b = 1.05 x estimate, is positive on (0, 1.25 b), so any
+    estimate above true / 1.3125 keeps the block preconditioner SPD."""
+    import scipy.linalg as sla
+    cx, coords, L = grid(d, n, 0.1, 2)
+    c = o.WhitneyComplex(cx, L, relative=rel)
+    lam = o.hx_block_lambdas(c, coords, k, n)
+    hu = o.HxPreconditioner(c, coords, k, n)
+    hs = o.HxPreconditioner(c, coords, k - 1, n, nodal=hu.nodal)
+    for b, (H, j) in enumerate([(hs, k - 1), (hu, k)]):
+        A = c.hdif_gram(j).to_scipy().toarray()
+        B = np.column_stack([H.apply(e) for e in np.eye(A.shape[0])])
+        true = sla.eigh(A @ B @ A, A, eigvals_only=True)[-1]  # B A x = l x  <=>  A B A x = l A x
+        assert true * 0.8 <= lam[b] <= true * (1 + 1e-9), (b, lam[b], true)
+
+
+def test_chebyshev_hx_iteration_count_is_flat():
+    """3D k = 1 with m = 4: 4^3 -> 8^3 -> 12^3 adds at most 3 iterations each."""
+    counts = []
+    for n in (4, 8, 12):
+        cx, coords, L = grid(3, n, 0.05, 0)
+        c = o.WhitneyComplex(cx, L)
+        f = np.array([((i % 5) - 2.0) * 0.3 for i in range(cx.nsimplices(1))])
+        kkt, rhs, _, _ = o.assemble_mixed_kkt(c, f, 1, c.harmonics(1))
+        A = kkt.to_scipy()
+        _, rep = o.minres_py(lambda v: A @ v, o.hx_block_preconditioner(c, coords, 1, n, 0, cheb_m=4),
+                             rhs, 1e-10, 5000)
+        counts.append(rep.iters)
+    for coarse, fine in zip(counts, counts[1:]):
+        assert fine <= coarse + 3, counts
