@@ -8,15 +8,19 @@ U(+-0.05 h) (SplitMix64 seed 2), Euclidean squared edge lengths from the
 coordinates, source cochain N(0,1) on the edges (seed 2).
 
 One step = re-assemble every KKT element block and the CSR values from the
-resident geometry (pattern reused), then solve the KKT system to a true
-relative residual |b - A x|/|b| <= 1e-10. The headline solve is mixed
-precision: FP64 MINRES and KKT SpMV with the Chebyshev block preconditioner
-run in FP32 arithmetic on its FP16 SELL-64 copy (FEEC_PREC_FP64_PC32;
-`--mixed-mode refine` selects FP32 MINRES inside FP64 iterative refinement,
-FEEC_PREC_MIXED, which is timed beside it either way). `value` is completed
-steps per second (solves/s) with all inputs resident in HBM. The plain FP64
-GPU path (FP64 matrix, vectors, MINRES and polynomial, same 1e-10
-certificate) is timed beside it.
+resident geometry (pattern reused), refresh the block preconditioner from the
+new values, then solve the KKT system to a true relative residual
+|b - A x|/|b| <= 1e-10. The headline solve is mixed precision: FP64 MINRES and
+KKT SpMV with the multigrid block preconditioner (Hiptmair-Xu blocks over a
+geometric V-cycle, each block accelerated by a Chebyshev polynomial) applied
+in FP32 (FEEC_PREC_FP64_PC32; `--mixed-mode refine` selects FP32 MINRES
+inside FP64 iterative refinement, FEEC_PREC_MIXED, which is timed beside it
+either way). `value` is completed steps per second (solves/s) with all inputs
+resident in HBM. Beside it: the plain FP64 GPU path (FP64 matrix, vectors,
+MINRES and preconditioner, same certificate), the Chebyshev-Jacobi
+polynomial preconditioner on the same problem, the 4D Lorentzian element
+kernel (C3) and a short serving stream (C5) as sub-objects, and the oracle
+timed on the host on the same 64^3 workload.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -51,47 +55,91 @@ def build_problem(fa, n, precision, ctx):
     return topo, geo, wc
 
 
-def cpu_baseline(n_sample, gpu_iters, c2_cells, c2_nnz, cheb_degree):
-    """The oracle (C restatement of the reference algorithm) timed on this
-    host on a bounded sample: the element matrices and CSR assembly of the KKT
-    blocks of an n_sample^3 Kuhn mesh, and FP64 MINRES iterations with the
-    same Chebyshev block preconditioner as the GPU, on its KKT system; scaled
-    to the 64^3 workload per cell / per nonzero, with the iteration count of
-    the GPU FP64 path (same algorithm and preconditioner)."""
+def cpu_model():
+    """The host CPU's model name and logical CPU count from /proc/cpuinfo."""
+    name, count = "unknown", 0
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name") and name == "unknown":
+                    name = line.split(":", 1)[1].strip()
+                if line.startswith("processor"):
+                    count += 1
+    except OSError:
+        pass
+    return name, count
+
+
+def cpu_baseline(fa, n, cheb_m, cores):
+    """The oracle timed on this host on the C2 workload itself: the same 64^3
+    jittered mesh and N(0,1) source (built by the library's host functions,
+    as the GPU run builds them), the KKT system assembled by the oracle's C
+    element kernels (OpenMP, `cores` threads) and its serial COO->CSR merge,
+    the oracle's restatement of the GPU's preconditioner (HX blocks with the
+    Chebyshev acceleration of the same degree, lambda_max by the same Lanczos
+    steps) and its Python restatement of MINRES (scipy SpMV) run until the
+    true relative residual is <= 1e-10, as the GPU's certificate. Measured, not extrapolated, except the 1-core figure's element
+    kernels: those are timed on 1 thread over the first 1/8 of the cells and
+    scaled by 8 (everything after the element kernels is single-threaded
+    in both figures)."""
     from oracle import oracle as o
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    cells, coords = o.kuhn(3, n_sample)
-    rng = np.random.default_rng(2)
-    inter = np.all((coords > 1e-12) & (coords < 1 - 1e-12), axis=1)
-    coords[inter] += 0.05 / n_sample * (2 * rng.random((inter.sum(), 3)) - 1)
+    g = fa.CartesianGrid(3, n)
+    cells, coords = g.triangulate()
+    coords = g.jitter(coords, 0.05, 2)
     cx = o.Complex(3, cells)
     L = cx.lengths_from_coords(coords)
-    nc = cx.nsimplices(3)
+    f = fa.normal_vector(cx.nsimplices(1), 2)
+    blocks = ((o.MASS, 0), (o.DIF_TEST, 1), (o.DIF_BOTH, 2), (o.MASS, 1), (o.DIF_BOTH, 1))
     t0 = time.perf_counter()
-    for kind, k in ((o.MASS, 0), (o.DIF_TEST, 1), (o.DIF_BOTH, 2), (o.MASS, 1), (o.DIF_BOTH, 1)):
+    for kind, k in blocks:
         o.element_matrices(cx, L, kind, k, cores)
     t_el = time.perf_counter() - t0
-    c = o.WhitneyComplex(cx, L)
+    part = cells[: len(cells) // 8]
+    pcx = o.Complex(3, part)
+    pL = pcx.lengths_from_coords(coords)
     t0 = time.perf_counter()
-    f = np.random.default_rng(2).standard_normal(cx.nsimplices(1))
+    for kind, k in blocks:
+        o.element_matrices(pcx, pL, kind, k, 1)
+    t_el1 = (time.perf_counter() - t0) * len(cells) / len(part)
+    del pcx, pL
+    t0 = time.perf_counter()
+    c = o.WhitneyComplex(cx, L, memo=True)
     kkt, rhs, ns, nu = o.assemble_mixed_kkt(c, f, 1, c.harmonics(1))
+    A = kkt.to_scipy()
+    c.hdif_gram(0)  # the nodal block the preconditioner needs (dif_both(1))
     t_asm = time.perf_counter() - t0
-    prec = o.chebyshev_jacobi(c, 1, 0, cheb_degree)
-    iters = 60
     t0 = time.perf_counter()
-    o.minres_cheb(kkt, rhs, prec, 1e-30, iters)
-    t_it = (time.perf_counter() - t0) / iters
-    scale_cells = c2_cells / nc
-    t_step = (t_el + t_asm) * scale_cells + t_it * (c2_nnz / kkt.nnz) * gpu_iters
+    hx = o.hx_blocks(c, coords, 1, n)
+    lam = o.hx_block_lambdas(c, coords, 1, n, blocks=hx)
+    prec = o.hx_block_preconditioner(c, coords, 1, n, 0, cheb_m=cheb_m, lam=lam, blocks=hx)
+    t_setup = time.perf_counter() - t0
+    # MINRES stops on the preconditioner-norm residual; a quarter of the
+    # target there, then correction runs until the true residual meets it
+    t0 = time.perf_counter()
+    x, rep = o.minres_py(lambda v: A @ v, prec, rhs, 0.25e-10, 1000)
+    iters = rep.iters
+    bnorm = np.linalg.norm(rhs)
+    res = rhs - A @ x
+    while np.linalg.norm(res) > 1e-10 * bnorm and iters < 1000:
+        dx, rep = o.minres_py(lambda v: A @ v, prec, res, 0.25e-10 * bnorm / np.linalg.norm(res), 1000)
+        x, iters = x + dx, iters + rep.iters
+        res = rhs - A @ x
+    t_solve = time.perf_counter() - t0
+    resid = float(np.linalg.norm(res) / bnorm)
+    t_all = t_asm + t_setup + t_solve
+    t_one = t_all - t_el + t_el1
+    model, ncpu = cpu_model()
     return {
-        "value": 1.0 / t_step, "unit": "solves/s", "cores": cores, "kind": "port",
-        "sample": (f"oracle on a {n_sample}^3 Kuhn mesh ({nc} tets, KKT nnz {kkt.nnz}): element matrices "
-                   f"{t_el:.2f}s + assembly {t_asm:.2f}s scaled x{scale_cells:.1f} by cells, and {iters} "
-                   f"FP64 MINRES iterations with the degree-{cheb_degree} Chebyshev block preconditioner "
-                   f"({t_it * 1e3:.2f} ms each) scaled by nnz to the 64^3 KKT and multiplied by the GPU "
-                   f"FP64 path's {gpu_iters} iterations"),
-        "seconds_per_step_estimate": t_step,
-        "elmat_cells_per_s": nc / t_el,
+        "value": 1.0 / t_all, "unit": "solves/s", "cores": cores, "kind": "port",
+        "sample": (f"the C2 workload itself (64^3, {len(cells)} tets, KKT nnz {A.nnz}) through the oracle: "
+                   f"KKT assembly {t_asm:.1f}s (element kernels {t_el:.1f}s of it on {cores} threads), "
+                   f"HX + Chebyshev (m={cheb_m}) set-up {t_setup:.1f}s, MINRES {iters} iterations "
+                   f"{t_solve:.1f}s to true residual {resid:.1e}"),
+        "one_core": {"value": 1.0 / t_one, "seconds_per_solve": t_one,
+                     "element_kernels_seconds": t_el1,
+                     "note": "element kernels timed on 1 thread over 1/8 of the cells, x8"},
+        "seconds_per_solve": t_all, "cpu_model": model, "host_logical_cpus": ncpu,
+        "iters": int(iters), "true_residual": resid,
     }
 
 
@@ -100,15 +148,16 @@ def kuhn_n_for(d, cells):
     return max(1, int(round((cells / math.factorial(d)) ** (1.0 / d))))
 
 
-def run_c3(args):
-    """4D Lorentzian Hodge-Dirac element matrices on the Kuhn 16^4 box
-    (1,572,864 4-simplices): time axis 0 unscaled, axes 1..3 jittered by
+def c3_measure(n=16, reps=3, sample=2048):
+    """4D Lorentzian Hodge-Dirac element matrices on the Kuhn n^4 box (16^4:
+    1,572,864 4-simplices): time axis 0 unscaled, axes 1..3 jittered by
     U(+-0.02 h) (seed 3), squared lengths in the Minkowski signature
-    (-,+,+,+). Throughput of the all-grade element kernel, and the error of
-    every block on a sample of cells against the FP64 oracle."""
+    (-,+,+,+). Throughput of the all-grade element kernel (nine blocks per
+    cell: the mass of grades 0..4 and the exterior derivative of grades
+    1..4), and the error of every block on a sample of cells against the FP64
+    oracle."""
     import formoniq_amd as fa
     from oracle import oracle as o
-    n = args.n if args.n != 64 else 16
     g = fa.CartesianGrid(4, n)
     cells, coords = g.triangulate()
     coords = g.jitter(coords, 0.02, 3, axes=[1, 2, 3])
@@ -116,32 +165,35 @@ def run_c3(args):
     geo = fa.MeshLengthsSq.from_coords(topo, coords, minkowski=True)
     nc = topo.nsimplices(4)
     census, _ = geo.causal_census()
-    sec, per = geo.dirac_element_time(reps=args.steps)
+    sec, per = geo.dirac_element_time(reps=reps)
     blocks = [(fa.MASS, k) for k in range(5)] + [(fa.DIF_TRIAL, k) for k in range(1, 5)]
     rng = np.random.default_rng(3)
-    sample = np.sort(rng.choice(nc, size=min(nc, 2048), replace=False))
+    pick = np.sort(rng.choice(nc, size=min(nc, sample), replace=False))
     L = geo.lengths()
     edges = topo.cell_faces(1)
     worst = 0.0
     for kind, k in blocks:
         el = fa.WhitneyComplex(geo).element(kind, k)
-        for c in sample:
+        for c in pick:
             want = o.element_matrix(4, kind, k, L[edges[c]])
-            got = el[c]
             scale = max(np.abs(want).max(), 1e-300)
-            worst = max(worst, float(np.abs(got - want).max() / scale))
+            worst = max(worst, float(np.abs(el[c] - want).max() / scale))
         del el
     bytes_per_cell = 8 * 10 + 4 * 10 + 8 * per
-    line = {
-        "config": "C3", "workload": f"4D Lorentzian Hodge-Dirac element matrices, Kuhn {n}^4 ({nc} cells), "
-                                    "axes 1..3 jittered 0.02h seed 3, Minkowski (-,+,+,+)",
+    return {
+        "workload": f"4D Lorentzian Hodge-Dirac element matrices, Kuhn {n}^4 ({nc} cells), axes 1..3 "
+                    "jittered 0.02h seed 3, Minkowski (-,+,+,+)",
         "edges_timelike_null_spacelike": census.tolist(),
         "seconds_per_launch": sec, "values_per_cell": per,
-        "element_matrices_per_s": 9 * nc / sec, "cells_per_s": nc / sec,
+        "element_matrices_per_s": len(blocks) * nc / sec, "cells_per_s": nc / sec,
         "hbm_gbs": bytes_per_cell * nc / sec / 1e9,
-        "max_rel_err_vs_oracle": worst, "sampled_cells": int(sample.size), "dtype": "f64",
+        "max_rel_err_vs_oracle": worst, "sampled_cells": int(pick.size), "dtype": "f64",
     }
-    print(json.dumps(line), flush=True)
+
+
+def run_c3(args):
+    n = args.n if args.n != 64 else 16
+    print(json.dumps({"config": "C3", **c3_measure(n, args.steps)}), flush=True)
 
 
 def run_leapfrog(args):
@@ -266,30 +318,29 @@ def run_c4(args):
     print(json.dumps(line), flush=True)
 
 
-def run_c5(args):
+def c5_measure(count, rate, slots, precisions, graph_iters=32, warmup=True, seed=5):
     """Serving stream (seed 5): Poisson arrivals at a fixed rate of
     Hodge-Laplace solve requests, d in {2,3}, k in 0..d-1, cells log-uniform
-    in 1e3..2e6; admitted by HBM budget, solved mixed precision (FP64 MINRES,
-    FP32 polynomial on FP16 storage) to 1e-10. Throughput and p50/p99
-    latency, and the same stream on the FP64 path."""
+    in 1e3..2e6; admitted by HBM budget, each solved to 1e-10 with the
+    library's default preconditioner choice (the multigrid blocks for the
+    large Kuhn-grid requests). Throughput and p50/p99 latency per precision."""
     import formoniq_amd as fa
-    reqs = config_requests(fa, args.requests, 5, [2, 3], 1e3, 2e6)
-    gaps = np.random.default_rng(5).exponential(1.0 / args.rate, size=len(reqs))
+    reqs = config_requests(fa, count, seed, [2, 3], 1e3, 2e6)
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, size=len(reqs))
     names = _mode_names(fa)
     out = {}
-    for prec in (fa.FP64_PC32, fa.FP64):
+    for prec in precisions:
         for r in reqs:
             r.precision = prec
         opts = fa.ServerOpts()
         opts.hbm_budget_bytes = 200 << 30
-        opts.max_inflight = args.slots
-        opts.solve = fa.solve_opts(precision=prec, graph_iters=args.graph_iters)
+        opts.max_inflight = slots
+        opts.solve = fa.solve_opts(precision=prec, graph_iters=graph_iters)
         srv = fa.Server(opts)
-        if not args.no_warmup:
+        if warmup:
             # a server's start-up: one small request of every (d, k) the stream holds
-            warm = [r for r in config_requests(fa, 64, 55, [2, 3], 1e3, 2e3)]
             seen = {}
-            for r in warm:
+            for r in config_requests(fa, 64, 55, [2, 3], 1e3, 2e3):
                 seen.setdefault((r.dim, r.grade), r)
             for r in seen.values():
                 r.precision = prec
@@ -301,8 +352,7 @@ def run_c5(args):
             time.sleep(gap)
             tickets.append(srv.submit(r))
             if i % 8 == 7:
-                log(f"c5 {names[prec]}: {i + 1} submitted at "
-                    f"{time.perf_counter() - t0:.1f}s")
+                log(f"c5 {names[prec]}: {i + 1} submitted at {time.perf_counter() - t0:.1f}s")
         srv.drain()
         dt = time.perf_counter() - t0
         res = [srv.poll(t) for t in tickets]
@@ -314,7 +364,7 @@ def run_c5(args):
             "seconds": dt, "solves_per_s": len(res) / dt, "all_converged_1e-10": ok,
             "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
             "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
-            "cells": int(sum(x.ncells for x in res)),
+            "cells": int(sum(x.ncells for x in res)), "largest_cells": int(max(x.ncells for x in res)),
             "build_ms_max": float(max(x.build_seconds for x in res) * 1e3),
             "slowest": [{"dim": q.dim, "grade": q.grade, "n": q.n, "cells": int(x.ncells),
                          "dofs": int(x.ndofs), "iters": int(x.iters), "latency_ms": float(l * 1e3),
@@ -322,48 +372,76 @@ def run_c5(args):
                          "build_ms": float(x.build_seconds * 1e3), "solve_ms": float(x.solve_seconds * 1e3)}
                         for l, q, x in slow],
         }
-    line = {"config": "C5", "workload": f"{len(reqs)} requests, Poisson rate {args.rate}/s, d in 2..3, "
-                                        f"k in 0..d-1, cells logU[1e3,2e6], seed 5, {args.slots} slots", **out}
-    print(json.dumps(line), flush=True)
+    return {"workload": f"{len(reqs)} requests, Poisson rate {rate}/s, d in 2..3, k in 0..d-1, "
+                        f"cells logU[1e3,2e6], seed {seed}, {slots} slots", **out}
+
+
+def run_c5(args):
+    import formoniq_amd as fa
+    line = c5_measure(args.requests, args.rate, args.slots, (fa.FP64_PC32, fa.FP64), args.graph_iters,
+                      not args.no_warmup)
+    print(json.dumps({"config": "C5", **line}), flush=True)
 
 
 def run_kernels(args):
-    """Short run for hardware-counter passes: the 3D 64^3 k=1 mixed problem's element
-    kernel, KKT scatter and fused MINRES SpMV (FP32, FP64) launched a few
-    times each, then a streaming kernel of known size (torch multiply of
-    256 Mi floats: 1 GiB read, 1 GiB written) to calibrate byte counters."""
+    """Short run for hardware-counter passes: the headline 64^3 k=1 problem's
+    element kernel, KKT scatter, fused MINRES SpMV (FP32, FP64) and the
+    multigrid preconditioner's kernels (finest-level sweep, product by the
+    preconditioner matrix, a B application, a block preconditioner
+    application), each launched one by one a few times, then a streaming
+    kernel of known size (torch multiply of 256 Mi floats: 1 GiB read, 1 GiB
+    written) to calibrate byte counters."""
     import torch
     import formoniq_amd as fa
     ctx = fa.Context(0)
-    topo, geo, wc = build_problem(fa, args.n, fa.MIXED, ctx)
-    prob = fa.Problem(wc, 1, fa.solve_opts(precision=fa.MIXED))
+    topo, geo, wc = build_problem(fa, args.n, fa.FP64_PC32, ctx)
+    prob = fa.Problem(wc, 1, fa.solve_opts(precision=fa.FP64_PC32))
     prob.assemble()
     times = prob.kernel_times(reps=args.steps)
+    mg = prob.mg_kernel_times(reps=args.steps, use_graph=False)
     a = torch.ones(256 << 20, device="cuda:0", dtype=torch.float32)
     b = torch.empty_like(a)
     for _ in range(3):
         torch.mul(a, 2.0, out=b)
     torch.cuda.synchronize()
     print(json.dumps({"config": "kernels", "n_kkt": prob.sizes[0] + prob.sizes[1], "nnz": prob.nnz,
-                      "kernel_seconds": times, "calibration_bytes_read": a.numel() * 4,
+                      "kernel_seconds": times, "multigrid_seconds": mg,
+                      "calibration_bytes_read": a.numel() * 4,
                       "calibration_bytes_written": b.numel() * 4}), flush=True)
+
+
+def timed_steps(ctx, prob, steps, label):
+    """Time `steps` steps (assemble + solve) of a resident problem."""
+    lat, reps = [], []
+    for i in range(steps):
+        ts = time.perf_counter()
+        prob.assemble()
+        r = prob.solve()
+        ctx.synchronize()
+        lat.append(time.perf_counter() - ts)
+        reps.append(r)
+        log(f"{label} step {i}: {lat[-1] * 1e3:.1f} ms iters {r.iters} refinements {r.refinements} "
+            f"residual {r.residual:.2e} (solve {r.seconds * 1e3:.1f} ms)")
+    assert all(r.converged and r.residual <= 1e-10 for r in reps), label
+    return lat, reps
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64, help="cells per axis (the workload is 64)")
-    ap.add_argument("--fp64-steps", type=int, default=1,
+    ap.add_argument("--fp64-steps", type=int, default=2,
                     help="timed steps of the FP64 path and of the other mixed-precision mode")
     ap.add_argument("--mixed-mode", default="pc32", choices=["pc32", "refine"],
-                    help="headline solve: FP64 MINRES with the FP32 polynomial on FP16 storage (pc32), "
-                         "or FP32 MINRES inside FP64 refinement (refine)")
-    ap.add_argument("--fp64-cheb", type=int, default=24, help="Chebyshev degree of the FP64 path")
-    ap.add_argument("--cpu-n", type=int, default=32)
+                    help="headline solve: FP64 MINRES with the FP32 preconditioner (pc32), or FP32 MINRES "
+                         "inside FP64 refinement (refine)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-jacobi", action="store_true", help="skip the FP64 Jacobi-block solve")
+    ap.add_argument("--no-c3", action="store_true", help="leave out the C3 sub-object")
+    ap.add_argument("--no-c5", action="store_true", help="leave out the C5 sub-object")
+    ap.add_argument("--c5-requests", type=int, default=16, help="requests of the C5 sub-object's stream")
+    ap.add_argument("--c5-rate", type=float, default=2.0, help="arrival rate of the C5 sub-object's stream")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "c2_kernel_traffic.json"))
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the Krylov kernels one by one instead of replaying graph segments "
@@ -373,8 +451,8 @@ def main():
     ap.add_argument("--no-warmup", action="store_true",
                     help="c5: start the timed stream on a cold server (no start-up requests)")
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "kernels", "leapfrog"],
-                    help="c2 (default): the headline line; c3/c4/c5: the 4D Dirac element, batched "
-                         "and serving workloads, each printed as its own JSON line")
+                    help="c2 (default): the headline line, with C3 and C5 sub-objects; c3/c4/c5: those "
+                         "workloads in full, each printed as its own JSON line")
     ap.add_argument("--batch", type=int, default=512, help="c4: number of meshes")
     ap.add_argument("--requests", type=int, default=48, help="c5: number of requests")
     ap.add_argument("--rate", type=float, default=0.5, help="c5: Poisson arrival rate per second")
@@ -392,143 +470,105 @@ def main():
     import formoniq_amd as fa
     ctx = fa.Context(0)
     t0 = time.perf_counter()
-    topo, geo, wc = build_problem(fa, args.n, fa.MIXED, ctx)
-    t_mesh = time.perf_counter() - t0
+    topo, geo, wc = build_problem(fa, args.n, fa.FP64_PC32, ctx)
     ncells = topo.nsimplices(3)
-    log(f"mesh {args.n}^3: {ncells} tets, counts {topo.counts.tolist()}, numbering {t_mesh:.2f}s")
+    log(f"mesh {args.n}^3: {ncells} tets, counts {topo.counts.tolist()}, numbering "
+        f"{time.perf_counter() - t0:.2f}s")
     f = fa.normal_vector(topo.nsimplices(1), 2)
-
-    t0 = time.perf_counter()
     use_graph = 0 if args.no_graph else 1
     head_prec = fa.FP64_PC32 if args.mixed_mode == "pc32" else fa.MIXED
     other_prec = fa.MIXED if head_prec == fa.FP64_PC32 else fa.FP64_PC32
     mode_name = {fa.FP64_PC32: "fp64_pc32", fa.MIXED: "fp32_refine"}
-    mixed = fa.Problem(wc, 1, fa.solve_opts(precision=head_prec, use_graph=use_graph))
-    log(f"mixed problem ({mode_name[head_prec]}): sizes {mixed.sizes}, nnz {mixed.nnz}, "
-        f"setup {time.perf_counter() - t0:.2f}s")
-    n_kkt = mixed.sizes[0] + mixed.sizes[1]
 
-    # source resident on the device before timing
-    mixed.upload_source(f)
-
+    # the headline problem: the library's default preconditioner choice
+    t0 = time.perf_counter()
+    head = fa.Problem(wc, 1, fa.solve_opts(precision=head_prec, use_graph=use_graph,
+                                           graph_iters=args.graph_iters))
+    head.upload_source(f)  # source resident on the device before timing
+    head.assemble()
+    pc = head.preconditioner()
+    log(f"headline problem ({mode_name[head_prec]}, {pc['format']}): sizes {head.sizes}, nnz {head.nnz}, "
+        f"set-up {time.perf_counter() - t0:.2f}s")
+    n_kkt = head.sizes[0] + head.sizes[1]
+    kkt_nnz = int(head.nnz)
     for i in range(args.warmup):
-        mixed.assemble()
-        r = mixed.solve()
+        head.assemble()
+        r = head.solve()
         log(f"warmup {i}: iters {r.iters} refinements {r.refinements} residual {r.residual:.2e} "
             f"solve {r.seconds:.3f}s")
     ctx.synchronize()
-    lat, reps = [], []
     t_start = time.perf_counter()
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        mixed.assemble()
-        r = mixed.solve()
-        ctx.synchronize()
-        lat.append(time.perf_counter() - ts)
-        reps.append(r)
-        log(f"step {i}: {lat[-1]:.3f}s iters {r.iters} refinements {r.refinements} residual {r.residual:.2e}")
+    lat, reps = timed_steps(ctx, head, args.steps, "headline")
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
-    assert all(r.converged and r.residual <= 1e-10 for r in reps)
-    ms_per_step = elapsed / args.steps * 1e3
-
-    times = mixed.kernel_times(reps=10)
-    log(f"kernel times: {times}")
-
-    # the other mixed-precision mode on the same workload
-    alt = fa.Problem(wc, 1, fa.solve_opts(precision=other_prec, use_graph=use_graph))
-    alt.upload_source(f)
-    alt.assemble()
-    r_alt = alt.solve()
-    t_alt = []
-    for i in range(args.fp64_steps):
-        ts = time.perf_counter()
-        alt.assemble()
-        r_alt = alt.solve()
-        ctx.synchronize()
-        t_alt.append(time.perf_counter() - ts)
-        log(f"{mode_name[other_prec]} step {i}: {t_alt[-1]:.3f}s iters {r_alt.iters} "
-            f"refinements {r_alt.refinements} residual {r_alt.residual:.2e}")
-    assert r_alt.converged and r_alt.residual <= 1e-10
-    alt_rate = 1.0 / statistics.mean(t_alt)
-    del alt
-
-    # plain FP64 GPU path
-    # each path at the Chebyshev degree that is fastest for it on this
-    # workload (FP64: 24; the mixed modes run the default, 64)
-    fp64 = fa.Problem(wc, 1, fa.solve_opts(precision=fa.FP64, cheb_degree=args.fp64_cheb,
-                                           use_graph=use_graph))
-    fp64.upload_source(f)
-    fp64.assemble()
-    r64 = fp64.solve()
-    log(f"fp64 warmup: iters {r64.iters} residual {r64.residual:.2e} {r64.seconds:.3f}s")
-    t64 = []
-    for i in range(args.fp64_steps):
-        ts = time.perf_counter()
-        fp64.assemble()
-        r64 = fp64.solve()
-        ctx.synchronize()
-        t64.append(time.perf_counter() - ts)
-        log(f"fp64 step {i}: {t64[-1]:.3f}s iters {r64.iters} residual {r64.residual:.2e}")
-    assert r64.converged and r64.residual <= 1e-10
-    fp64_rate = 1.0 / statistics.mean(t64)
-    fp64_cheb = int(fp64.opts.cheb_degree)
-    del fp64
-
-    # the reference's block structure with Jacobi blocks (no polynomial): the
-    # plainest FP64 GPU path
-    jac_rate, r_jac = None, None
-    if not args.no_jacobi:
-        jac = fa.Problem(wc, 1, fa.solve_opts(precision=fa.FP64, cheb_degree=0, use_graph=use_graph))
-        jac.upload_source(f)
-        ts = time.perf_counter()
-        jac.assemble()
-        r_jac = jac.solve()
-        ctx.synchronize()
-        t_jac = time.perf_counter() - ts
-        log(f"fp64 jacobi: {t_jac:.3f}s iters {r_jac.iters} residual {r_jac.residual:.2e}")
-        assert r_jac.converged and r_jac.residual <= 1e-10
-        jac_rate = 1.0 / t_jac
-        del jac
     value = args.steps / elapsed
+    ms_per_step = elapsed / args.steps * 1e3
+    t_asm = []
+    for _ in range(3):  # the assembly part of a step, on its own
+        ts = time.perf_counter()
+        head.assemble()
+        ctx.synchronize()
+        t_asm.append(time.perf_counter() - ts)
+    times = head.kernel_times(reps=10)
+    mg = head.mg_kernel_times(reps=20) if pc["format"].startswith("multigrid") else None
+    log(f"kernel times: {times}; multigrid: {mg}")
+    cheb_m, lam = pc["degree"], pc["lambda_max"]
+    del head
 
-    # roofline of the dominant kernel of the mixed solve: the FP32 Chebyshev
-    # step (one preconditioner SpMV fused with the polynomial recurrence);
-    # the fused MINRES SpMVs are reported beside it
-    nnz = mixed.nnz
-    pc = mixed.preconditioner()
-    p_nnz, cheb_deg, p_col_bytes = pc["nnz"], pc["degree"], pc["column_bytes"]
-    spmv_bytes = 8 * nnz + 8 * (n_kkt + 1) + 12 * n_kkt  # FP32 values, int32 columns
-    spmv64_bytes = 12 * nnz + 8 * (n_kkt + 1) + 24 * n_kkt
-    if pc["format"] in ("sell64_f16", "sell64_f16_c32"):
-        # three-term step on the SELL-64 copy: 4 B per off-diagonal nonzero
-        # (FP16 value + int16 offset; 6 B with int32 columns; padding is not
-        # counted), 8 B per 64-row slice offset, x_{k+1} read (4n), x_k read
-        # and overwritten (8n), f read (4n)
-        p_val_bytes = 2
-        p_col_bytes = 2 if pc["format"] == "sell64_f16" else 4
-        cheb_bytes = ((2 + p_col_bytes) * pc["offdiag_nnz"] + 8 * ((n_kkt + 63) // 64 + 1)
-                      + 16 * n_kkt)
-    else:
-        # two-term CSR step: P values (2 B as scaled FP16, else 4 B) + columns
-        # (2 B as band offsets, else 4 B), row pointers, d (4n), res and z read
-        # and written (16n), the next direction written (4n); dinv (4n) only
-        # when unscaled
-        p_val_bytes = 2 if pc["format"] == "csr_f16" else 4
-        cheb_bytes = ((p_val_bytes + p_col_bytes) * p_nnz + 8 * (n_kkt + 1) + 24 * n_kkt
-                      + (4 * n_kkt if p_val_bytes == 4 else 0))
-    achieved = cheb_bytes / times["cheb32"] / 1e9
-    traffic = None
-    if os.path.exists(args.traffic):
-        with open(args.traffic) as fh:
-            tr = json.load(fh)
-        # only a measurement of the same kernel on the same workload applies
-        if tr.get("n_kkt") == n_kkt and tr.get("cheb_step_f32_format") == pc["format"]:
-            traffic = tr.get("cheb_step_f32_hbm_bytes_per_launch")
-    # per tet: 6 edge lengths + 4 vertex indices in; M_0, dif_test(1), dif_both(2),
-    # two diagonals, M_1 and dif_both(1) out (138 doubles)
-    elmat_bytes = ncells * (48 + 8 * (16 + 24 + 36 + 6 + 4 + 36 + 16))
+    # the other mixed-precision mode and the plain FP64 path, same options otherwise
+    rates = {}
+    for prec, name in ((other_prec, mode_name[other_prec]), (fa.FP64, "fp64")):
+        prob = fa.Problem(wc, 1, fa.solve_opts(precision=prec, use_graph=use_graph,
+                                               graph_iters=args.graph_iters))
+        prob.upload_source(f)
+        prob.assemble()
+        prob.solve()
+        lt, rp = timed_steps(ctx, prob, args.fp64_steps, name)
+        rates[name] = {"solves_per_s": 1.0 / statistics.mean(lt), "iters": int(rp[-1].iters),
+                       "refinements": int(rp[-1].refinements), "residual": float(rp[-1].residual)}
+        del prob
+    # the polynomial (Chebyshev-Jacobi) block preconditioner on the headline mode, for comparison
+    prob = fa.Problem(wc, 1, fa.solve_opts(precision=head_prec, use_graph=use_graph,
+                                           preconditioner=fa.PC_CHEBYSHEV))
+    prob.upload_source(f)
+    prob.assemble()
+    prob.solve()
+    lt, rp = timed_steps(ctx, prob, 1, "chebyshev-jacobi")
+    poly = {"solves_per_s": 1.0 / statistics.mean(lt), "iters": int(rp[-1].iters),
+            "degree": int(prob.preconditioner()["degree"])}
+    del prob
 
+    # roofline of the dominant kernel: the finest-level Jacobi sweep of the
+    # V-cycle (the largest share of a step's kernel time: 2 sweeps before and
+    # after each coarse correction, several cycles per MINRES iteration)
+    roof = None
+    if mg is not None:
+        nb0 = (args.n + 1) ** 3
+        nv = 5  # slots of one 3D k = 1 cycle: sigma's V, u's gradient V and its 3 vector copies
+        # 8 stencil diagonals (the diagonal and the 7 +delta ones; the -delta entries
+        # are the same numbers), dinv, r and x read once, the new iterate written
+        sweep_bytes = nb0 * (8 * 4 + 4 + 3 * 4 * nv)
+        achieved = sweep_bytes / mg["sweep_fine"] / 1e9
+        traffic = None
+        if os.path.exists(args.traffic):
+            with open(args.traffic) as fh:
+                tr = json.load(fh)
+            if tr.get("n_kkt") == n_kkt and "sweep_fine_hbm_bytes_per_launch" in tr:
+                traffic = tr["sweep_fine_hbm_bytes_per_launch"]
+        roof = {
+            "kernel": "k_st_sweep<float,3,0,true> (finest-level weighted-Jacobi sweep of the multigrid V-cycle "
+                      "on the Kuhn-stencil diagonals, 5 interleaved slots, FP32)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "algorithmic_bytes_per_launch": sweep_bytes, "launch_seconds": mg["sweep_fine"],
+            "finest_level_vertices": nb0, "slots": nv,
+        }
+    # per tet, the five element blocks of the KKT system: M_0 (4x4), dif_test(1)
+    # (6x4), dif_both(2) (6x6), M_1 (6x6) and dif_both(1) (4x4); in: 6 squared
+    # lengths and 4 vertex indices; out: the 116 values plus the two diagonal
+    # blocks' 10 diagonal entries the preconditioner matrix takes
+    elmat_bytes = ncells * (48 + 16 + 8 * (16 + 24 + 36 + 36 + 16 + 10))
+    spmv64_bytes = 12 * kkt_nnz + 8 * (n_kkt + 1) + 24 * n_kkt  # FP64 values, int32 columns, x, y
     line = {
         "metric": METRIC,
         "value": value,
@@ -540,54 +580,53 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f64 assembly, f64 minres, f32 preconditioner polynomial on f16 storage"
+        "dtype": ("f64 assembly, f64 minres and kkt spmv, f32 multigrid/hx preconditioner"
                   if head_prec == fa.FP64_PC32 else "f64 assembly, f32 minres + f64 refinement"),
         "data": "synthetic",
         "config": {
             "workload": f"3D mixed Hodge-Laplace k=1, Kuhn {args.n}^3 ({ncells} tets), jitter 0.05h seed 2, "
-                        "N(0,1) edge source seed 2, full complex, true residual <= 1e-10",
-            "kkt_n": n_kkt, "kkt_nnz": nnz,
+                       "N(0,1) edge source seed 2, full complex, true residual <= 1e-10",
+            "kkt_n": n_kkt, "kkt_nnz": kkt_nnz,
+            "preconditioner": pc["format"], "preconditioner_applications_per_block": int(cheb_m),
+            "lambda_max": [float(x) for x in lam],
         },
         "mixed_mode": mode_name[head_prec],
-        f"solves_per_s_{mode_name[other_prec]}": alt_rate,
-        f"iters_{mode_name[other_prec]}": int(r_alt.iters),
-        f"refinements_{mode_name[other_prec]}": int(r_alt.refinements),
-        "solves_per_s_fp64": fp64_rate,
-        "mixed_over_fp64": value / fp64_rate,
-        "solves_per_s_fp64_jacobi": jac_rate,
-        "mixed_over_fp64_jacobi": value / jac_rate if jac_rate else None,
-        "iters_fp64_jacobi": int(r_jac.iters) if r_jac is not None else None,
-        "latency_p50_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
-        "latency_p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
-        "iters_mixed": int(reps[-1].iters), "refinements_mixed": int(reps[-1].refinements),
-        "iters_fp64": int(r64.iters),
-        "residual_mixed": float(reps[-1].residual), "residual_fp64": float(r64.residual),
-        "elmat_blocks_per_s": 7 * ncells / times["elmat"],
+        "iters": int(reps[-1].iters), "refinements": int(reps[-1].refinements),
+        "residual": float(reps[-1].residual),
+        "step_ms_p50": float(np.percentile(np.array(lat) * 1e3, 50)),
+        "step_ms_max": float(np.max(lat) * 1e3),
+        "assemble_ms": float(np.median(t_asm) * 1e3),
+        f"solves_per_s_{mode_name[other_prec]}": rates[mode_name[other_prec]]["solves_per_s"],
+        f"iters_{mode_name[other_prec]}": rates[mode_name[other_prec]]["iters"],
+        "solves_per_s_fp64": rates["fp64"]["solves_per_s"],
+        "iters_fp64": rates["fp64"]["iters"],
+        "mixed_over_fp64": value / rates["fp64"]["solves_per_s"],
+        "chebyshev_jacobi": poly,
+        "elmat_blocks_per_s": 5 * ncells / times["elmat"],
         "elmat_cells_per_s": ncells / times["elmat"],
         "elmat_hbm_gbs": elmat_bytes / times["elmat"] / 1e9,
         "kernel_seconds": times,
-        "roofline": {
-            "kernel": ("k_cheb3_step (Chebyshev step on the SELL-64 FP16 preconditioner: SpMV + three-term "
-                       "recurrence, FP32 arithmetic)" if pc["format"].startswith("sell64") else
-                       "k_cheb_step<float> (Chebyshev step: preconditioner SpMV + recurrence, FP32 arithmetic)"),
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "algorithmic_bytes_per_launch": cheb_bytes,
-            "launch_seconds": times["cheb32"], "precond_nnz": p_nnz, "cheb_degree": cheb_deg,
-            "precond_column_bytes": p_col_bytes, "precond_value_bytes": p_val_bytes,
-            "precond_format": pc["format"], "precond_offdiag_nnz": pc["offdiag_nnz"],
-            "precond_stored_entries": pc["stored"],
-        },
-        "mr_spmv_f32": {"launch_seconds": times["spmv32"], "algorithmic_bytes_per_launch": spmv_bytes,
-                        "achieved_gbs": spmv_bytes / times["spmv32"] / 1e9},
+        "multigrid_kernel_seconds": mg,
         "mr_spmv_f64": {"launch_seconds": times["spmv64"], "algorithmic_bytes_per_launch": spmv64_bytes,
                         "achieved_gbs": spmv64_bytes / times["spmv64"] / 1e9},
+        "roofline": roof,
     }
+    if not args.no_c3:
+        try:
+            line["c3"] = c3_measure(16, 3, 256)
+        except Exception as e:  # reported, never fatal for the headline line
+            line["c3"] = {"error": repr(e)}
+    if not args.no_c5:
+        try:
+            line["c5"] = c5_measure(args.c5_requests, args.c5_rate, args.slots, (fa.FP64_PC32,),
+                                    args.graph_iters, not args.no_warmup)
+        except Exception as e:
+            line["c5"] = {"error": repr(e)}
     if not args.no_cpu_baseline:
         try:
-            line["cpu_baseline"] = cpu_baseline(args.cpu_n, int(r64.iters), ncells, nnz,
-                                                fp64_cheb)
-        except Exception as e:  # reported, never fatal for the GPU line
+            cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            line["cpu_baseline"] = cpu_baseline(fa, args.n, int(cheb_m), cores)
+        except Exception as e:
             line["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(line), flush=True)
 

@@ -473,8 +473,19 @@ class Problem:
         return dict(elmat=float(out[0]), scatter=float(out[1]), spmv32=float(out[2]),
                     spmv64=float(out[3]), cheb32=float(out[4]), cheb64=float(out[5]))
 
+    def mg_kernel_times(self, reps=20, use_graph=True):
+        """Seconds per launch with the multigrid preconditioner: finest-level
+        sweep, block preconditioner application (FP64_PC32), product by the
+        preconditioner matrix, one HX application B (graph replay, or launched
+        one by one)."""
+        out = np.zeros(4)
+        check(load().feec_problem_mg_kernel_times(self._h, reps, 1 if use_graph else 0,
+                                                  _ptr(out, _lib.f64p)))
+        return dict(sweep_fine=float(out[0]), precond_apply=float(out[1]), pc_spmv32=float(out[2]),
+                    hx_apply=float(out[3]))
+
     PC_FORMATS = {0: "csr_f32", 1: "csr_f16", 2: "sell64_f16", 3: "csr_f64", 4: "sell64_f64",
-                  5: "sell64_f16_c32", 6: "sell64_f64_c32"}
+                  5: "sell64_f16_c32", 6: "sell64_f64_c32", 7: "multigrid_f32", 8: "multigrid_f64"}
 
     def apply_preconditioner(self, r, precision=None, storage=None):
         """z = B r (KKT order: sigma, u, harmonic rows) with the block preconditioner

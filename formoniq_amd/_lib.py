@@ -128,6 +128,7 @@ SIGNATURES = {
     "feec_problem_preconditioner": (S, [vp, i64p, C.POINTER(C.c_int32), f64p, C.POINTER(C.c_int32),
                                         C.POINTER(C.c_int32), i64p, i64p]),
     "feec_problem_kernel_times": (S, [vp, C.c_int, f64p]),
+    "feec_problem_mg_kernel_times": (S, [vp, C.c_int, C.c_int, f64p]),
     "feec_problem_apply_preconditioner": (S, [vp, C.c_int32, C.c_int32, f64p, f64p]),
     "feec_problem_upload_source": (S, [vp, f64p]),
     "feec_dirac_element_time": (S, [vp, C.c_int, f64p, i64p]),

@@ -1547,6 +1547,14 @@ feec_status feec_problem_kernel_times(feec_problem* p, int reps, double out[6]) 
   });
 }
 
+feec_status feec_problem_mg_kernel_times(feec_problem* p, int reps, int use_graph, double out[4]) {
+  return guard([&] {
+    FEEC_REQUIRE(p && out, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(p->sys.hx, FEEC_E_UNSUPPORTED, "the problem's preconditioner is not the multigrid one");
+    hx_kernel_times(*p->sys.hx, reps, use_graph != 0, out);
+  });
+}
+
 feec_status feec_dirac_element_time(feec_geom* g, int reps, double* seconds,
                                    int64_t* values_per_cell) {
   return guard([&] {

@@ -391,6 +391,9 @@ int kuhn_grid_n(Mesh& m);
 Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m, double frac,
               std::string* why);
 void hx_info(const Hx& h, int* cheb_m, double lam[2]);
+// Seconds per launch: finest-level sweep, block preconditioner application
+// (FP64_PC32), Chebyshev product by P, one B application (FP32).
+void hx_kernel_times(Hx& h, int reps, bool use_graph, double out[4]);
 // New values after re-assembly (same pattern).
 void hx_refresh(Hx& h, Geom& g, const KktSystem& s);
 void hx_free(Hx* h);

@@ -1887,6 +1887,7 @@ static void bind_pc32(KktSystem& s, MinresWork<double>& w) {
 }
 
 int kkt_preconditioner_format(const KktSystem& s, int precision, int storage) {
+  if (s.hx) return precision == FEEC_PREC_FP64 ? FEEC_PC_MULTIGRID_F64 : FEEC_PC_MULTIGRID_F32;
   if (precision == FEEC_PREC_FP64_PC32 && s.Psell_ptr.p && s.cheb_m > 0)
     return s.sell_wide ? FEEC_PC_SELL64_F16_C32 : FEEC_PC_SELL64_F16;
   const bool sell = storage == FEEC_PC_STORE_SELL && s.Psell_ptr.p;

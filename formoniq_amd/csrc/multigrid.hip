@@ -131,31 +131,87 @@ __global__ void __launch_bounds__(TPB) k_st_scale(const TP* __restrict__ dinv, c
   x[t] = dinv[t / nv] * r[t];
 }
 
-// mode 0: out = x + dinv (r - A x)     (one weighted Jacobi sweep)
-// mode 1: out = r - A x, 0 on dead rows (the residual)
-template <class TP, int D>
+// The sweep family, one thread per (vertex, slot). The operator is
+// symmetric, so a row's -delta entries are read from the +delta diagonals at
+// the neighbour (A(i, i - o) = A(i - o, i)): only the diagonal and the 2^d - 1
+// +delta diagonals are fetched. MODE selects the iterate the sweep starts from:
+//   0: out = x + dinv (r - A x)                        (a weighted Jacobi sweep)
+//   1: out = r - A x, 0 on dead rows                   (the residual)
+//   2: the sweep from x = dinv r                       (the first two pre-sweeps)
+//   3: the sweep from x + P xc (xc on the coarser box) (prolongation, first post-sweep)
+// FINE marks the finest level's launches, which stream the operator from HBM,
+// as their own instantiation (the coarse levels are cache-resident).
+template <class TP, int D, int MODE>
+__device__ __forceinline__ TP st_iterate(int j, unsigned nv, unsigned v, const TP* __restrict__ x,
+                                         const TP* __restrict__ dinv, const TP* __restrict__ r,
+                                         const TP* __restrict__ xc, const int* cj, const Box& cb) {
+  if (MODE == 2) return dinv[j] * r[(unsigned)j * nv + v];
+  TP val = x[(unsigned)j * nv + v];
+  if (MODE == 3 && dinv[j] != TP(0)) {
+    int lo = 0, hi = 0;
+    bool odd = false;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      lo += (cj[a] >> 1) * cb.st[a];
+      hi += ((cj[a] >> 1) + (cj[a] & 1)) * cb.st[a];
+      odd = odd || (cj[a] & 1);
+    }
+    val += odd ? TP(0.5) * (xc[(unsigned)lo * nv + v] + xc[(unsigned)hi * nv + v]) : xc[(unsigned)lo * nv + v];
+  }
+  return val;
+}
+
+template <class TP, int D, int MODE, bool FINE>
 __global__ void __launch_bounds__(TPB) k_st_sweep(const TP* __restrict__ A, const TP* __restrict__ dinv,
                                                   const TP* __restrict__ r, const TP* __restrict__ x,
-                                                  TP* __restrict__ out, Box b, unsigned nv, int mode,
-                                                  const MinresS* S) {
-  constexpr int NS = (1 << (D + 1)) - 1;
+                                                  const TP* __restrict__ xc, TP* __restrict__ out, Box b, Box cb,
+                                                  unsigned nv, const MinresS* S) {
+  constexpr int H = (1 << D) - 1;
   if (skip(S)) return;
   const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned nb = (unsigned)b.nb;
   if (t >= nb * nv) return;
   const unsigned i = t / nv;
   const unsigned v = t - i * nv;
-  TP acc = TP(0);
+  int ci[D];  // grid coordinates (prolongation only)
+  if (MODE == 3) {
+    unsigned rr = i;
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const TP a = A[(size_t)s * nb + i];
-    int j = (int)i + stencil_offset<D>(s, b.st);
-    j = j < 0 ? 0 : (j >= (int)nb ? (int)nb - 1 : j);  // a is 0 for a neighbour outside the box
-    acc += a * x[(unsigned)j * nv + v];
+    for (int a = 0; a < D; ++a) {
+      ci[a] = (int)(rr % (unsigned)(b.g + 1));
+      rr /= (unsigned)(b.g + 1);
+    }
+  }
+  const TP xi = st_iterate<TP, D, MODE>((int)i, nv, v, x, dinv, r, xc, ci, cb);
+  TP acc = A[i] * xi;
+#pragma unroll
+  for (int m = 1; m <= H; ++m) {
+    const int o = stencil_offset<D>(m, b.st);
+    int cp[D], cm[D];
+    if (MODE == 3) {
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const int dl = (m >> a) & 1;
+        cp[a] = min(ci[a] + dl, b.g);  // clamped with the index; the weight is 0 there
+        cm[a] = max(ci[a] - dl, 0);
+      }
+    }
+    // +delta: A(i, i + o) at row i; -delta: A(i, i - o) = A(i - o, i) at row i - o
+    int jp = (int)i + o, jm = (int)i - o;
+    const TP ap = jp < (int)nb ? A[(size_t)m * nb + i] : TP(0);
+    const TP am = jm >= 0 ? A[(size_t)m * nb + jm] : TP(0);
+    jp = jp < (int)nb ? jp : (int)i;
+    jm = jm >= 0 ? jm : (int)i;
+    if (MODE == 3) {
+      if (ap == TP(0)) jp = (int)i;  // a box-edge wrap: the weight is 0, keep the read in range
+      if (am == TP(0)) jm = (int)i;
+    }
+    acc += ap * st_iterate<TP, D, MODE>(jp, nv, v, x, dinv, r, xc, jp == (int)i ? ci : cp, cb);
+    acc += am * st_iterate<TP, D, MODE>(jm, nv, v, x, dinv, r, xc, jm == (int)i ? ci : cm, cb);
   }
   const TP d = dinv[i];
   const TP res = r[t] - acc;
-  out[t] = mode == 0 ? x[t] + d * res : (d != TP(0) ? res : TP(0));
+  out[t] = MODE == 1 ? (d != TP(0) ? res : TP(0)) : xi + d * res;
 }
 
 // rc = P^T ef on the live coarse vertices (0 elsewhere): coarse vertex w
@@ -199,33 +255,6 @@ __global__ void __launch_bounds__(TPB) k_st_restrict(const TP* __restrict__ ef, 
     if (in) acc += (s == 0 ? TP(1) : TP(0.5)) * ef[(unsigned)k * nv + v];
   }
   rc[t] = acc;
-}
-
-// xf += P xc on the live fine vertices (dinv_f != 0).
-template <class TP, int D>
-__global__ void __launch_bounds__(TPB) k_st_prolong(const TP* __restrict__ xc, TP* __restrict__ xf,
-                                                    const TP* __restrict__ dinv_f, Box fb, Box cb,
-                                                    unsigned nv, const MinresS* S) {
-  if (skip(S)) return;
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (unsigned)fb.nb * nv) return;
-  const unsigned k = t / nv;
-  const unsigned v = t - k * nv;
-  if (dinv_f[k] == TP(0)) return;
-  unsigned r = k;
-  int lo = 0, hi = 0;
-  bool odd = false;
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    const int c = (int)(r % (unsigned)(fb.g + 1));
-    r /= (unsigned)(fb.g + 1);
-    lo += (c >> 1) * cb.st[a];
-    hi += ((c >> 1) + (c & 1)) * cb.st[a];
-    odd = odd || (c & 1);
-  }
-  const TP add = odd ? TP(0.5) * (xc[(unsigned)lo * nv + v] + xc[(unsigned)hi * nv + v])
-                     : xc[(unsigned)lo * nv + v];
-  xf[t] += add;
 }
 
 // x = C r on the coarsest box, C dense (nb x nb, row-major, zero on dead
@@ -1296,8 +1325,8 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
     const MgLevel& l = h->L[i];
     const int64_t nbv = (int64_t)l.box.nb * h->nv;
     if (i + 1 < h->L.size()) {
-      bytes += (int64_t)(2 * h->sweeps) * ((int64_t)NS * l.box.nb * 4 + 12 * nbv) + 16 * nbv;
-      launches += 2 + 2 * h->sweeps;
+      bytes += (int64_t)(2 * h->sweeps) * ((int64_t)(NS / 2 + 1) * l.box.nb * 4 + 12 * nbv) + 16 * nbv;
+      launches += 1 + 2 * h->sweeps;
     } else {
       bytes += (int64_t)l.box.nb * l.box.nb * 4;
       launches += 1;
@@ -1380,6 +1409,18 @@ struct Sel<float> {
 
 // One V-cycle on level i with its rhs in r(i); returns the buffer holding
 // the result (x(i) or y(i)).
+template <class TP, int D, bool FINE>
+void sweep_launch(Enq& q, int mode, const MgLevel& l, const TP* A, const TP* dinv, const TP* r, const TP* x,
+                  const TP* xc, TP* out, const Box& cb, unsigned nv, const MinresS* S) {
+  const unsigned g = grid_for((int64_t)l.box.nb * nv);
+  switch (mode) {
+    case 0: q(k_st_sweep<TP, D, 0, FINE>, g, TPB, A, dinv, r, x, xc, out, l.box, cb, nv, S); break;
+    case 1: q(k_st_sweep<TP, D, 1, FINE>, g, TPB, A, dinv, r, x, xc, out, l.box, cb, nv, S); break;
+    case 2: q(k_st_sweep<TP, D, 2, FINE>, g, TPB, A, dinv, r, x, xc, out, l.box, cb, nv, S); break;
+    default: q(k_st_sweep<TP, D, 3, FINE>, g, TPB, A, dinv, r, x, xc, out, l.box, cb, nv, S); break;
+  }
+}
+
 template <class TP, int D>
 TP* vcycle(Enq& q, Hx& h, size_t i, const MinresS* S) {
   using X = Sel<TP>;
@@ -1390,22 +1431,40 @@ TP* vcycle(Enq& q, Hx& h, size_t i, const MinresS* S) {
     q(k_st_dense<TP>, (unsigned)l.box.nb, TPB, X::cinv(h), (const TP*)X::r(l), X::x(l), l.box.nb, h.nv, S);
     return X::x(l);
   }
+  MgLevel& c = h.L[i + 1];
   TP* x = X::x(l);
   TP* y = X::y(l);
   const TP* r = X::r(l);
-  auto sweep = [&]() {
-    q(k_st_sweep<TP, D>, grid_for(nbv), TPB, X::A(l), X::dinv(l), r, (const TP*)x, y, l.box, nv, 0, S);
-    std::swap(x, y);
+  const TP* A = X::A(l);
+  const TP* dinv = X::dinv(l);
+  auto sweep = [&](int mode, const TP* xc, TP* out) {
+    if (i == 0)
+      sweep_launch<TP, D, true>(q, mode, l, A, dinv, r, x, xc, out, c.box, nv, S);
+    else
+      sweep_launch<TP, D, false>(q, mode, l, A, dinv, r, x, xc, out, c.box, nv, S);
   };
-  q(k_st_scale<TP>, grid_for(nbv), TPB, X::dinv(l), r, x, (unsigned)l.box.nb, nv, S);
-  for (int s = 1; s < h.sweeps; ++s) sweep();
-  q(k_st_sweep<TP, D>, grid_for(nbv), TPB, X::A(l), X::dinv(l), r, (const TP*)x, X::e(l), l.box, nv, 1, S);
-  MgLevel& c = h.L[i + 1];
+  // pre-smoothing from x = 0: x = dinv r, then sweeps - 1 sweeps (the first fused with it)
+  if (h.sweeps == 1) {
+    q(k_st_scale<TP>, grid_for(nbv), TPB, dinv, r, x, (unsigned)l.box.nb, nv, S);
+  } else {
+    sweep(2, nullptr, y);
+    std::swap(x, y);
+    for (int s = 2; s < h.sweeps; ++s) {
+      sweep(0, nullptr, y);
+      std::swap(x, y);
+    }
+  }
+  sweep(1, nullptr, X::e(l));
   q(k_st_restrict<TP, D>, grid_for((int64_t)c.box.nb * nv), TPB, (const TP*)X::e(l), X::r(c),
     (const uint8_t*)c.alive.p, l.box, c.box, nv, S);
   TP* xc = vcycle<TP, D>(q, h, i + 1, S);
-  q(k_st_prolong<TP, D>, grid_for(nbv), TPB, (const TP*)xc, x, X::dinv(l), l.box, c.box, nv, S);
-  for (int s = 0; s < h.sweeps; ++s) sweep();
+  // post-smoothing: the first sweep starts from x + P xc
+  sweep(3, xc, y);
+  std::swap(x, y);
+  for (int s = 1; s < h.sweeps; ++s) {
+    sweep(0, nullptr, y);
+    std::swap(x, y);
+  }
   return x;
 }
 
@@ -1641,6 +1700,98 @@ void hx_spectrum(Hx& h) {
 }
 
 }  // namespace
+
+// Mean seconds per launch, replayed from a graph of `reps` launches on the
+// context's stream (or launched one by one): [0] a finest-level Jacobi sweep (FP32), [1] one block
+// preconditioner application as an FP64_PC32 solve applies it (FP64 in and
+// out, the cycles and Chebyshev steps in FP32), [2] the Chebyshev steps'
+// product by the preconditioner matrix (FP32), [3] one B application (FP32).
+void hx_kernel_times(Hx& h, int reps, bool use_graph, double out[4]) {
+  Ctx* ctx = h.ctx;
+  hipStream_t st = ctx->stream;
+  reps = std::max(1, reps);
+  MgLevel& l0 = h.L[0];
+  const unsigned nv = (unsigned)h.nv;
+  for (DBuf<float>* b : {&l0.r32, &l0.x32, &l0.y32})
+    FEEC_HIP(hipMemsetAsync(b->p, 0, b->n * sizeof(float), st));
+  DBuf<double> r(ctx, std::max<int64_t>(h.N, 1)), z(ctx, std::max<int64_t>(h.N, 1));
+  DBuf<double> part(ctx, NBLK);
+  DBuf<float> r32(ctx, std::max<int64_t>(h.N, 1));
+  FEEC_HIP(hipMemsetAsync(r.p, 0, r.n * 8, st));
+  FEEC_HIP(hipMemsetAsync(r32.p, 0, r32.n * 4, st));
+  if (h.cheb_m > 1)
+    for (DBuf<float>* b : {&h.d32, &h.w32}) FEEC_HIP(hipMemsetAsync(b->p, 0, b->n * 4, st));
+  auto timed = [&](auto&& body) -> double {
+    if (!use_graph) {  // launched one by one (for tracing and counter tools)
+      Enq q{st};
+      body(q);
+      hipEvent_t e0, e1;
+      FEEC_HIP(hipEventCreate(&e0));
+      FEEC_HIP(hipEventCreate(&e1));
+      FEEC_HIP(hipEventRecord(e0, st));
+      for (int i = 0; i < reps; ++i) body(q);
+      FEEC_HIP(hipEventRecord(e1, st));
+      FEEC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return ms * 1e-3 / reps;
+    }
+    std::vector<std::shared_ptr<void>> keep;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    FEEC_HIP(hipGraphCreate(&graph, 0));
+    double sec = 0.0;
+    try {
+      Enq q{st, graph, &keep, nullptr};
+      for (int i = 0; i < reps; ++i) body(q);
+      FEEC_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      FEEC_HIP(hipGraphLaunch(exec, st));  // warm
+      hipEvent_t e0, e1;
+      FEEC_HIP(hipEventCreate(&e0));
+      FEEC_HIP(hipEventCreate(&e1));
+      FEEC_HIP(hipEventRecord(e0, st));
+      FEEC_HIP(hipGraphLaunch(exec, st));
+      FEEC_HIP(hipEventRecord(e1, st));
+      FEEC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      sec = ms * 1e-3 / reps;
+    } catch (...) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      throw;
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    return sec;
+  };
+  const Box cb = h.L.size() > 1 ? h.L[1].box : l0.box;
+  out[0] = h.L.size() > 1 ? timed([&](Enq& q) {
+    switch (h.d) {
+      case 1: sweep_launch<float, 1, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, nullptr, l0.y32.p, cb, nv, nullptr); break;
+      case 2: sweep_launch<float, 2, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, nullptr, l0.y32.p, cb, nv, nullptr); break;
+      case 3: sweep_launch<float, 3, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, nullptr, l0.y32.p, cb, nv, nullptr); break;
+      default: sweep_launch<float, 4, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, nullptr, l0.y32.p, cb, nv, nullptr); break;
+    }
+  }) : 0.0;
+  out[1] = timed([&](Enq& q) { hx_apply<double, float, double>(q, h, (const double*)r.p, z.p, nullptr, part.p); });
+  out[2] = h.cheb_m > 1 ? timed([&](Enq& q) {
+    q(k_pc_spmv<float, float, 4>, grid_for(h.n * 4), TPB, h.P_rp, h.P_col, (const float*)h.P32.p,
+      (const float*)h.d32.p, h.w32.p, h.n, (const MinresS*)nullptr);
+  }) : 0.0;
+  out[3] = timed([&](Enq& q) {
+    switch (h.d) {
+      case 1: hx_core_d<float, float, 1>(q, h, (const float*)r32.p, nullptr); break;
+      case 2: hx_core_d<float, float, 2>(q, h, (const float*)r32.p, nullptr); break;
+      case 3: hx_core_d<float, float, 3>(q, h, (const float*)r32.p, nullptr); break;
+      default: hx_core_d<float, float, 4>(q, h, (const float*)r32.p, nullptr); break;
+    }
+  });
+}
 
 void hx_info(const Hx& h, int* cheb_m, double lam[2]) {
   if (cheb_m) *cheb_m = h.cheb_m;

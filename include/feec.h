@@ -324,12 +324,17 @@ typedef enum {
                              offsets in 64-row SELL slices */
   FEEC_PC_SELL64_F16_C32 = 5, /* as FEEC_PC_SELL64_F16 with FP16 values and int32 columns apart
                                  (some column offset exceeds +-32767) */
-  FEEC_PC_SELL64_F64_C32 = 6  /* as FEEC_PC_SELL64_F64 with int32 columns */
+  FEEC_PC_SELL64_F64_C32 = 6, /* as FEEC_PC_SELL64_F64 with int32 columns */
+  FEEC_PC_MULTIGRID_F32 = 7,  /* the multigrid / HX blocks (FEEC_PC_MULTIGRID): V-cycles on FP32
+                                 Kuhn-stencil diagonals, Chebyshev steps in FP32 */
+  FEEC_PC_MULTIGRID_F64 = 8   /* the same in FP64 (the FP64 path) */
 } feec_pc_format;
 
 /* The block preconditioner blockdiag(hdif_gram(k-1), hdif_gram(k)) as built:
- * its nonzeros, the Chebyshev degree in use (0 = Jacobi), the estimated
- * largest eigenvalue of D^-1 hdif_gram per block, the bytes per stored CSR
+ * its nonzeros, the Chebyshev degree in use (0 = Jacobi; with the multigrid
+ * preconditioner, the B applications per block application), the estimated
+ * largest eigenvalue of D^-1 hdif_gram (with the multigrid preconditioner, of
+ * B hdif_gram) per block, the bytes per stored CSR
  * column index (2 when every |col - row| fits a 16-bit offset, else 4), the
  * format a solve with the problem's options applies (feec_pc_format), the
  * entries that format stores (with SELL padding) and the off-diagonal
@@ -356,6 +361,14 @@ feec_status feec_problem_upload_source(feec_problem *prob, const double *source_
  * the same in FP64}; the last four replayed from a graph of `reps` launches,
  * as a solve replays them. */
 feec_status feec_problem_kernel_times(feec_problem *prob, int reps, double out[6]);
+/* Diagnostics (no counterpart in the reference) for a problem whose
+ * preconditioner is the multigrid one (else FEEC_E_UNSUPPORTED): average
+ * seconds per launch, replayed from a graph of `reps` launches on the context
+ * stream (use_graph != 0) or launched one by one, of {a finest-level Jacobi sweep (FP32), one block preconditioner
+ * application as an FP64_PC32 solve applies it, one product by the
+ * preconditioner matrix in the Chebyshev steps (FP32), one HX application B
+ * (FP32)}. */
+feec_status feec_problem_mg_kernel_times(feec_problem *prob, int reps, int use_graph, double out[4]);
 
 /* Diagnostics (no counterpart in the reference): average seconds per launch of
  * the all-grade Hodge-Dirac element kernel -- M_k for k = 0..dim and

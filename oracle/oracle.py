@@ -373,10 +373,12 @@ def minres(a, b, inv_diag=None, rtol=1e-10, max_iters=10_000):
 class WhitneyComplex:
     """HilbertComplex over a Whitney complex (full or relative)."""
 
-    def __init__(self, cx: Complex, lengths, relative=False):
+    def __init__(self, cx: Complex, lengths, relative=False, memo=False):
         self.cx = cx
         self.lengths = np.ascontiguousarray(lengths, dtype=np.float64)
         self.relative = relative
+        # memo: keep each assembled (kind, grade) block for later requests
+        self._memo = {} if memo else None
         self.dim = cx.dim
         self._interior = [None] * (cx.dim + 1)
         if relative:
@@ -402,9 +404,13 @@ class WhitneyComplex:
         rows, cols = self.ndofs(tg), self.ndofs(rg)
         if rows == 0 or cols == 0:
             return Csr.zeros(rows, cols)
+        if self._memo is not None and (kind, grade) in self._memo:
+            return self._memo[(kind, grade)]
         a = assemble(self.cx, self.lengths, kind, grade)
         if self.relative:
             a = a.select(self._interior[tg], self._interior[rg])
+        if self._memo is not None:
+            self._memo[(kind, grade)] = a
         return a
 
     def mass(self, k):
@@ -1025,14 +1031,21 @@ def tridiagonal_lmax(diag, off):
     return hi
 
 
-def hx_block_lambdas(c: WhitneyComplex, coords, k, n, steps=HX_LANCZOS_STEPS, sweeps=2):
+def hx_blocks(c: WhitneyComplex, coords, k, n, sweeps=2):
+    """The HX preconditioners (B_{k-1} or None, B_k) of the two KKT blocks,
+    sharing one nodal V-cycle."""
+    hu = HxPreconditioner(c, coords, k, n, sweeps)
+    hs = HxPreconditioner(c, coords, k - 1, n, sweeps, hu.nodal) if c.ndofs(k - 1) else None
+    return hs, hu
+
+
+def hx_block_lambdas(c: WhitneyComplex, coords, k, n, steps=HX_LANCZOS_STEPS, sweeps=2, blocks=None):
     """lambda_max of B_j hdif_gram(j) per block (sigma, u): `steps` steps of
     CG on hdif_gram(j) preconditioned by B_j from the xorshift start, and the
     largest eigenvalue of the Lanczos tridiagonal their coefficients make
     (T_jj = 1/alpha_j + beta_{j-1}/alpha_{j-1}, T_{j,j+1} = sqrt(beta_j)/alpha_j)."""
     ns, nu = c.ndofs(k - 1), c.ndofs(k)
-    hu = HxPreconditioner(c, coords, k, n, sweeps)
-    hs = HxPreconditioner(c, coords, k - 1, n, sweeps, hu.nodal) if ns else None
+    hs, hu = blocks or hx_blocks(c, coords, k, n, sweeps)
     blocks = [(hs, c.hdif_gram(k - 1).to_scipy().tocsr() if ns else None, 0, ns),
               (hu, c.hdif_gram(k).to_scipy().tocsr(), ns, ns + nu)]
     v = _xorshift_start(ns + nu)
@@ -1062,17 +1075,17 @@ def hx_block_lambdas(c: WhitneyComplex, coords, k, n, steps=HX_LANCZOS_STEPS, sw
 
 
 def hx_block_preconditioner(c: WhitneyComplex, coords, k, n, nh, sweeps=2, cheb_m=1, lam=None,
-                            frac=HX_CHEB_FRACTION):
+                            frac=HX_CHEB_FRACTION, blocks=None):
     """The KKT block preconditioner diag(B_{k-1}, B_k, I) (elliptic.rs:29-47)
     with HX blocks; a callable r -> z. With cheb_m > 1 each block is
     p(B A) B with p the degree-(cheb_m - 1) Chebyshev polynomial of the
     interval [frac b, b], b = 1.05 lambda_max(B A) (Saad, Algorithm 12.1,
     with B in place of the Jacobi diagonal): cheb_m applications of B and
     cheb_m - 1 of A = hdif_gram per block. `lam`: lambda_max per block
-    (sigma, u), by default from hx_block_lambdas."""
+    (sigma, u), by default from hx_block_lambdas. `blocks`: prebuilt
+    (B_{k-1}, B_k) from hx_blocks."""
     ns, nu = c.ndofs(k - 1), c.ndofs(k)
-    hu = HxPreconditioner(c, coords, k, n, sweeps)
-    hs = HxPreconditioner(c, coords, k - 1, n, sweeps, hu.nodal) if ns else None
+    hs, hu = blocks or hx_blocks(c, coords, k, n, sweeps)
     if cheb_m <= 1:
         def prec(r):
             parts = [hs.apply(r[:ns])] if ns else []
@@ -1080,7 +1093,7 @@ def hx_block_preconditioner(c: WhitneyComplex, coords, k, n, nh, sweeps=2, cheb_
             return np.concatenate(parts)
         return prec
     if lam is None:
-        lam = hx_block_lambdas(c, coords, k, n, sweeps=sweeps)
+        lam = hx_block_lambdas(c, coords, k, n, sweeps=sweeps, blocks=(hs, hu))
     As = c.hdif_gram(k - 1).to_scipy().tocsr() if ns else None
     Au = c.hdif_gram(k).to_scipy().tocsr()
 
