@@ -135,37 +135,21 @@ __global__ void __launch_bounds__(TPB) k_st_scale(const TP* __restrict__ dinv, c
 // symmetric, so a row's -delta entries are read from the +delta diagonals at
 // the neighbour (A(i, i - o) = A(i - o, i)): only the diagonal and the 2^d - 1
 // +delta diagonals are fetched. MODE selects the iterate the sweep starts from:
-//   0: out = x + dinv (r - A x)                        (a weighted Jacobi sweep)
-//   1: out = r - A x, 0 on dead rows                   (the residual)
-//   2: the sweep from x = dinv r                       (the first two pre-sweeps)
-//   3: the sweep from x + P xc (xc on the coarser box) (prolongation, first post-sweep)
+//   0: out = x + dinv (r - A x)                  (a weighted Jacobi sweep)
+//   1: out = r - A x, 0 on dead rows             (the residual)
+//   2: the sweep from x = dinv r                 (the first two pre-sweeps in one)
 // FINE marks the finest level's launches, which stream the operator from HBM,
 // as their own instantiation (the coarse levels are cache-resident).
-template <class TP, int D, int MODE>
+template <class TP, int MODE>
 __device__ __forceinline__ TP st_iterate(int j, unsigned nv, unsigned v, const TP* __restrict__ x,
-                                         const TP* __restrict__ dinv, const TP* __restrict__ r,
-                                         const TP* __restrict__ xc, const int* cj, const Box& cb) {
-  if (MODE == 2) return dinv[j] * r[(unsigned)j * nv + v];
-  TP val = x[(unsigned)j * nv + v];
-  if (MODE == 3 && dinv[j] != TP(0)) {
-    int lo = 0, hi = 0;
-    bool odd = false;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      lo += (cj[a] >> 1) * cb.st[a];
-      hi += ((cj[a] >> 1) + (cj[a] & 1)) * cb.st[a];
-      odd = odd || (cj[a] & 1);
-    }
-    val += odd ? TP(0.5) * (xc[(unsigned)lo * nv + v] + xc[(unsigned)hi * nv + v]) : xc[(unsigned)lo * nv + v];
-  }
-  return val;
+                                         const TP* __restrict__ dinv, const TP* __restrict__ r) {
+  return MODE == 2 ? dinv[j] * r[(unsigned)j * nv + v] : x[(unsigned)j * nv + v];
 }
 
 template <class TP, int D, int MODE, bool FINE>
 __global__ void __launch_bounds__(TPB) k_st_sweep(const TP* __restrict__ A, const TP* __restrict__ dinv,
                                                   const TP* __restrict__ r, const TP* __restrict__ x,
-                                                  const TP* __restrict__ xc, TP* __restrict__ out, Box b, Box cb,
-                                                  unsigned nv, const MinresS* S) {
+                                                  TP* __restrict__ out, Box b, unsigned nv, const MinresS* S) {
   constexpr int H = (1 << D) - 1;
   if (skip(S)) return;
   const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -173,41 +157,20 @@ __global__ void __launch_bounds__(TPB) k_st_sweep(const TP* __restrict__ A, cons
   if (t >= nb * nv) return;
   const unsigned i = t / nv;
   const unsigned v = t - i * nv;
-  int ci[D];  // grid coordinates (prolongation only)
-  if (MODE == 3) {
-    unsigned rr = i;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      ci[a] = (int)(rr % (unsigned)(b.g + 1));
-      rr /= (unsigned)(b.g + 1);
-    }
-  }
-  const TP xi = st_iterate<TP, D, MODE>((int)i, nv, v, x, dinv, r, xc, ci, cb);
+  const TP xi = st_iterate<TP, MODE>((int)i, nv, v, x, dinv, r);
   TP acc = A[i] * xi;
 #pragma unroll
   for (int m = 1; m <= H; ++m) {
     const int o = stencil_offset<D>(m, b.st);
-    int cp[D], cm[D];
-    if (MODE == 3) {
-#pragma unroll
-      for (int a = 0; a < D; ++a) {
-        const int dl = (m >> a) & 1;
-        cp[a] = min(ci[a] + dl, b.g);  // clamped with the index; the weight is 0 there
-        cm[a] = max(ci[a] - dl, 0);
-      }
-    }
-    // +delta: A(i, i + o) at row i; -delta: A(i, i - o) = A(i - o, i) at row i - o
+    // +delta: A(i, i + o) at row i; -delta: A(i, i - o) = A(i - o, i) at row i - o.
+    // A neighbour outside the box has weight 0; its read is kept in range.
     int jp = (int)i + o, jm = (int)i - o;
     const TP ap = jp < (int)nb ? A[(size_t)m * nb + i] : TP(0);
     const TP am = jm >= 0 ? A[(size_t)m * nb + jm] : TP(0);
     jp = jp < (int)nb ? jp : (int)i;
     jm = jm >= 0 ? jm : (int)i;
-    if (MODE == 3) {
-      if (ap == TP(0)) jp = (int)i;  // a box-edge wrap: the weight is 0, keep the read in range
-      if (am == TP(0)) jm = (int)i;
-    }
-    acc += ap * st_iterate<TP, D, MODE>(jp, nv, v, x, dinv, r, xc, jp == (int)i ? ci : cp, cb);
-    acc += am * st_iterate<TP, D, MODE>(jm, nv, v, x, dinv, r, xc, jm == (int)i ? ci : cm, cb);
+    acc += ap * st_iterate<TP, MODE>(jp, nv, v, x, dinv, r);
+    acc += am * st_iterate<TP, MODE>(jm, nv, v, x, dinv, r);
   }
   const TP d = dinv[i];
   const TP res = r[t] - acc;
@@ -257,6 +220,33 @@ __global__ void __launch_bounds__(TPB) k_st_restrict(const TP* __restrict__ ef, 
   rc[t] = acc;
 }
 
+// xf += P xc on the live fine vertices (dinv_f != 0).
+template <class TP, int D>
+__global__ void __launch_bounds__(TPB) k_st_prolong(const TP* __restrict__ xc, TP* __restrict__ xf,
+                                                    const TP* __restrict__ dinv_f, Box fb, Box cb,
+                                                    unsigned nv, const MinresS* S) {
+  if (skip(S)) return;
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)fb.nb * nv) return;
+  const unsigned k = t / nv;
+  const unsigned v = t - k * nv;
+  if (dinv_f[k] == TP(0)) return;
+  unsigned r = k;
+  int lo = 0, hi = 0;
+  bool odd = false;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const int c = (int)(r % (unsigned)(fb.g + 1));
+    r /= (unsigned)(fb.g + 1);
+    lo += (c >> 1) * cb.st[a];
+    hi += ((c >> 1) + (c & 1)) * cb.st[a];
+    odd = odd || (c & 1);
+  }
+  const TP add = odd ? TP(0.5) * (xc[(unsigned)lo * nv + v] + xc[(unsigned)hi * nv + v])
+                     : xc[(unsigned)lo * nv + v];
+  xf[t] += add;
+}
+
 // x = C r on the coarsest box, C dense (nb x nb, row-major, zero on dead
 // rows and columns): one block per row, all slots.
 template <class TP>
@@ -293,10 +283,11 @@ __global__ void __launch_bounds__(TPB) k_st_dense(const TP* __restrict__ C, cons
 // ---------------------------------------------------------------------------
 // operators of the hierarchy (FP64)
 // ---------------------------------------------------------------------------
-__global__ void k_scatter_map(const double* __restrict__ val, const int32_t* __restrict__ map, int64_t n,
-                              double* __restrict__ out) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p < n) out[map[p]] = val[p];
+// out[map[q]] = val[src ? src[q] : q]
+__global__ void k_scatter_map(const double* __restrict__ val, const int32_t* __restrict__ src,
+                              const int32_t* __restrict__ map, int64_t n, double* __restrict__ out) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < n) out[map[q]] = val[src ? src[q] : q];
 }
 
 // dinv = omega / A_ii on live rows, 0 on dead ones.
@@ -564,29 +555,32 @@ __global__ void __launch_bounds__(TPB) k_hxc_final(const TIN* __restrict__ r, co
 }
 
 // partial sums of x[i] y[i] over [lo, hi) (red_grid(hi - lo) blocks)
-__global__ void __launch_bounds__(TPB) k_dotr(const double* __restrict__ x, const double* __restrict__ y,
+template <class TP>
+__global__ void __launch_bounds__(TPB) k_dotr(const TP* __restrict__ x, const TP* __restrict__ y,
                                               int64_t lo, int64_t hi, double* __restrict__ part) {
   __shared__ double sh[TPB / 64];
   double acc = 0.0;
   for (int64_t i = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hi;
        i += (int64_t)gridDim.x * blockDim.x)
-    acc += x[i] * y[i];
+    acc += (double)x[i] * (double)y[i];
   const double t = dev_block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 // y += a_b x, a_0 on [0, split) and a_1 on [split, n)
-__global__ void k_block_axpy(const double* __restrict__ x, double* __restrict__ y, int64_t n, int64_t split,
-                             double a0, double a1) {
+template <class TP>
+__global__ void k_block_axpy(const TP* __restrict__ x, TP* __restrict__ y, int64_t n, int64_t split, double a0,
+                             double a1) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) y[i] += (i < split ? a0 : a1) * x[i];
+  if (i < n) y[i] += (TP)(i < split ? a0 : a1) * x[i];
 }
 
 // p = z + b_b p
-__global__ void k_block_xpby(const double* __restrict__ z, double* __restrict__ p, int64_t n, int64_t split,
-                             double b0, double b1) {
+template <class TP>
+__global__ void k_block_xpby(const TP* __restrict__ z, TP* __restrict__ p, int64_t n, int64_t split, double b0,
+                             double b1) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) p[i] = z[i] + (i < split ? b0 : b1) * p[i];
+  if (i < n) p[i] = z[i] + (TP)(i < split ? b0 : b1) * p[i];
 }
 
 
@@ -749,8 +743,12 @@ struct Hx {
   int64_t n = 0, N = 0;
   int nv = 0;              // slots
   std::vector<MgLevel> L;  // finest first; the last is solved densely
-  DBuf<int32_t> l0_map;    // level-0 operator CSR entry -> stencil position
+  // level-0 operator values: entry q lands on stencil position l0_map[q] and
+  // comes from P's value l0_src[q] (k <= 1: the nodal operator is P's block)
+  // or from the q-th value of the assembled nodal operator (k = 2)
+  DBuf<int32_t> l0_map, l0_src;
   int64_t l0_nnz = 0;
+  bool l0_from_p = false;
   DBuf<double> cinv;
   DBuf<float> cinv32;
   std::vector<HxBlock> blocks;
@@ -759,6 +757,7 @@ struct Hx {
   // Chebyshev acceleration (cheb_m applications of B per block application)
   int cheb_m = 1;
   double frac = 0.25;
+  bool spectrum_fp32 = false;  // estimate lambda_max with the FP32 cycle (mixed-precision problems)
   double lam[2] = {0.0, 0.0};  // lambda_max of B P per block (sigma, u)
   int64_t split = 0;
   const int64_t* P_rp = nullptr;
@@ -1002,14 +1001,14 @@ void level_derived(Ctx* ctx, int d, MgLevel& l) {
 // Level operators from the finest one's values: the level-0 stencil from the
 // nodal CSR, Galerkin products down the hierarchy, the coarsest inverse.
 // Buffers stay where they are, so captured graphs stay valid.
-void hierarchy_values(Hx& h, const Csr& A0, bool check) {
+void hierarchy_values(Hx& h, const double* vals, bool check) {
   Ctx* ctx = h.ctx;
   const int d = h.d;
   MgLevel& l0 = h.L[0];
-  FEEC_REQUIRE(A0.nnz == h.l0_nnz, FEEC_E_DEVICE, "nodal operator pattern changed");
   FEEC_HIP(hipMemsetAsync(l0.A.p, 0, l0.A.n * sizeof(double), ctx->stream));
-  if (A0.nnz)
-    k_scatter_map<<<grid_for(A0.nnz), TPB, 0, ctx->stream>>>(A0.val.p, h.l0_map.p, A0.nnz, l0.A.p);
+  if (h.l0_nnz)
+    k_scatter_map<<<grid_for(h.l0_nnz), TPB, 0, ctx->stream>>>(vals, h.l0_from_p ? h.l0_src.p : nullptr,
+                                                                h.l0_map.p, h.l0_nnz, l0.A.p);
   FEEC_HIP(hipGetLastError());
   level_derived(ctx, d, l0);
   for (size_t i = 0; i + 1 < h.L.size(); ++i) {
@@ -1108,7 +1107,7 @@ void bind_pc_matrix(Hx& h, const KktSystem& s) {
 }  // namespace
 
 Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m, double frac,
-              std::string* why) {
+              bool spectrum_fp32, std::string* why) {
   Mesh& m = *g.mesh;
   Ctx* ctx = m.ctx;
   auto no = [&](const std::string& msg) -> Hx* {
@@ -1136,6 +1135,7 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
   h->split = s.split;
   h->cheb_m = std::max(1, cheb_m);
   h->frac = frac;
+  h->spectrum_fp32 = spectrum_fp32;
   if (h->cheb_m > 1 && s.P.nnz == 0) return no("no preconditioner matrix for the Chebyshev steps");
   if (!(frac > 0.0 && frac < 1.0)) return no("the Chebyshev interval fraction must lie in (0, 1)");
   const bool rel = bc == FEEC_RELATIVE;
@@ -1196,37 +1196,63 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
   }
   // the level-0 operator's stencil positions, then the hierarchy's values
   {
-    Csr A0;
-    nodal_operator(g, bc, s, k, A0);
-    HostCsr a = host_of(ctx, A0);
     const Box& b0 = h->L[0].box;
-    std::vector<int32_t> box_of(a.nrows, 0);
+    std::vector<int32_t> box_of(m.n[0], 0);
     for (int64_t v = 0; v < m.n[0]; ++v)
-      if (dof0[v] >= 0 && dof0[v] < a.nrows) box_of[dof0[v]] = (int32_t)v;
-    std::vector<int32_t> map(a.col.size());
-    for (int64_t r = 0; r < a.nrows; ++r)
-      for (int64_t p = a.rp[r]; p < a.rp[r + 1]; ++p) {
-        int ri = box_of[r], rj = box_of[a.col[p]];
-        const int bi = ri;
-        int diff[4] = {0, 0, 0, 0};
-        bool ok = true;
-        for (int q = 0; q < d; ++q) {
-          diff[q] = rj % (b0.g + 1) - ri % (b0.g + 1);
-          ok = ok && diff[q] >= -1 && diff[q] <= 1;
-          ri /= (b0.g + 1);
-          rj /= (b0.g + 1);
-        }
-        const int sp = ok ? host_stencil_point(d, diff) : -1;
-        if (sp < 0) return no("the nodal operator is not a Kuhn stencil");
-        map[p] = sp * b0.nb + bi;
+      if (dof0[v] >= 0) box_of[dof0[v]] = (int32_t)v;
+    // stencil position of the entry (row DOF r, column DOF c), -1 off the stencil
+    auto position = [&](int64_t r, int64_t c) -> int64_t {
+      int ri = box_of[r], rj = box_of[c];
+      const int bi = ri;
+      int diff[4] = {0, 0, 0, 0};
+      for (int q = 0; q < d; ++q) {
+        diff[q] = rj % (b0.g + 1) - ri % (b0.g + 1);
+        if (diff[q] < -1 || diff[q] > 1) return -1;
+        ri /= (b0.g + 1);
+        rj /= (b0.g + 1);
       }
+      const int sp = host_stencil_point(d, diff);
+      return sp < 0 ? -1 : (int64_t)sp * b0.nb + bi;
+    };
+    std::vector<int32_t> map, src;
+    Csr A0;
+    if (k <= 1 && s.P.nnz) {  // the nodal operator is P's block [lo, hi)
+      const int64_t lo = k == 1 ? 0 : s.split, hi = k == 1 ? s.split : s.n;
+      std::vector<int64_t> rp(s.P.nrows + 1);
+      std::vector<int32_t> col(s.P.nnz);
+      copy_to_host(ctx, rp.data(), s.P.row_ptr.p, rp.size() * 8);
+      copy_to_host(ctx, col.data(), s.P.col.p, col.size() * 4);
+      for (int64_t r = lo; r < hi; ++r)
+        for (int64_t p = rp[r]; p < rp[r + 1]; ++p)
+          if (col[p] >= lo && col[p] < hi) {
+            const int64_t pos = position(r - lo, col[p] - lo);
+            if (pos < 0) return no("the nodal operator is not a Kuhn stencil");
+            map.push_back((int32_t)pos);
+            src.push_back((int32_t)p);
+          }
+      h->l0_from_p = true;
+    } else {
+      nodal_operator(g, bc, s, k, A0);
+      HostCsr a = host_of(ctx, A0);
+      map.resize(a.col.size());
+      for (int64_t r = 0; r < a.nrows; ++r)
+        for (int64_t p = a.rp[r]; p < a.rp[r + 1]; ++p) {
+          const int64_t pos = position(r, a.col[p]);
+          if (pos < 0) return no("the nodal operator is not a Kuhn stencil");
+          map[p] = (int32_t)pos;
+        }
+    }
     h->l0_nnz = (int64_t)map.size();
     h->l0_map.alloc(ctx, std::max<int64_t>(h->l0_nnz, 1));
     if (h->l0_nnz) copy_to_device(ctx, h->l0_map.p, map.data(), h->l0_nnz * 4);
+    if (h->l0_from_p) {
+      h->l0_src.alloc(ctx, std::max<int64_t>(h->l0_nnz, 1));
+      if (h->l0_nnz) copy_to_device(ctx, h->l0_src.p, src.data(), h->l0_nnz * 4);
+    }
     const int nbc = h->L.back().box.nb;
     h->cinv.alloc(ctx, (size_t)nbc * nbc);
     try {
-      hierarchy_values(*h, A0, true);
+      hierarchy_values(*h, h->l0_from_p ? s.P.val.p : A0.val.p, true);
     } catch (const Error& e) {
       return no(e.what());
     }
@@ -1326,7 +1352,7 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
     const int64_t nbv = (int64_t)l.box.nb * h->nv;
     if (i + 1 < h->L.size()) {
       bytes += (int64_t)(2 * h->sweeps) * ((int64_t)(NS / 2 + 1) * l.box.nb * 4 + 12 * nbv) + 16 * nbv;
-      launches += 1 + 2 * h->sweeps;
+      launches += 2 + 2 * h->sweeps;
     } else {
       bytes += (int64_t)l.box.nb * l.box.nb * 4;
       launches += 1;
@@ -1350,11 +1376,16 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
 
 
 void hx_refresh(Hx& h, Geom& g, const KktSystem& s) {
-  Csr A0;
-  nodal_operator(g, h.bc, s, h.k, A0);
-  hierarchy_values(h, A0, false);
-  smoother_diag(h, s);
   FEEC_REQUIRE(s.P.nnz == h.P_nnz, FEEC_E_DEVICE, "preconditioner matrix pattern changed");
+  if (h.l0_from_p) {
+    hierarchy_values(h, s.P.val.p, false);
+  } else {
+    Csr A0;
+    nodal_operator(g, h.bc, s, h.k, A0);
+    FEEC_REQUIRE(A0.nnz == h.l0_nnz, FEEC_E_DEVICE, "nodal operator pattern changed");
+    hierarchy_values(h, A0.val.p, false);
+  }
+  smoother_diag(h, s);
   bind_pc_matrix(h, s);
   hx_spectrum(h);
   h.ctx->sync();
@@ -1411,16 +1442,17 @@ struct Sel<float> {
 // the result (x(i) or y(i)).
 template <class TP, int D, bool FINE>
 void sweep_launch(Enq& q, int mode, const MgLevel& l, const TP* A, const TP* dinv, const TP* r, const TP* x,
-                  const TP* xc, TP* out, const Box& cb, unsigned nv, const MinresS* S) {
+                  TP* out, unsigned nv, const MinresS* S) {
   const unsigned g = grid_for((int64_t)l.box.nb * nv);
   switch (mode) {
-    case 0: q(k_st_sweep<TP, D, 0, FINE>, g, TPB, A, dinv, r, x, xc, out, l.box, cb, nv, S); break;
-    case 1: q(k_st_sweep<TP, D, 1, FINE>, g, TPB, A, dinv, r, x, xc, out, l.box, cb, nv, S); break;
-    case 2: q(k_st_sweep<TP, D, 2, FINE>, g, TPB, A, dinv, r, x, xc, out, l.box, cb, nv, S); break;
-    default: q(k_st_sweep<TP, D, 3, FINE>, g, TPB, A, dinv, r, x, xc, out, l.box, cb, nv, S); break;
+    case 0: q(k_st_sweep<TP, D, 0, FINE>, g, TPB, A, dinv, r, x, out, l.box, nv, S); break;
+    case 1: q(k_st_sweep<TP, D, 1, FINE>, g, TPB, A, dinv, r, x, out, l.box, nv, S); break;
+    default: q(k_st_sweep<TP, D, 2, FINE>, g, TPB, A, dinv, r, x, out, l.box, nv, S); break;
   }
 }
 
+// One V-cycle on level i with its rhs in r(i); returns the buffer holding
+// the result (x(i) or y(i)).
 template <class TP, int D>
 TP* vcycle(Enq& q, Hx& h, size_t i, const MinresS* S) {
   using X = Sel<TP>;
@@ -1437,32 +1469,30 @@ TP* vcycle(Enq& q, Hx& h, size_t i, const MinresS* S) {
   const TP* r = X::r(l);
   const TP* A = X::A(l);
   const TP* dinv = X::dinv(l);
-  auto sweep = [&](int mode, const TP* xc, TP* out) {
+  auto sweep = [&](int mode, TP* out) {
     if (i == 0)
-      sweep_launch<TP, D, true>(q, mode, l, A, dinv, r, x, xc, out, c.box, nv, S);
+      sweep_launch<TP, D, true>(q, mode, l, A, dinv, r, x, out, nv, S);
     else
-      sweep_launch<TP, D, false>(q, mode, l, A, dinv, r, x, xc, out, c.box, nv, S);
+      sweep_launch<TP, D, false>(q, mode, l, A, dinv, r, x, out, nv, S);
   };
   // pre-smoothing from x = 0: x = dinv r, then sweeps - 1 sweeps (the first fused with it)
   if (h.sweeps == 1) {
     q(k_st_scale<TP>, grid_for(nbv), TPB, dinv, r, x, (unsigned)l.box.nb, nv, S);
   } else {
-    sweep(2, nullptr, y);
+    sweep(2, y);
     std::swap(x, y);
     for (int s = 2; s < h.sweeps; ++s) {
-      sweep(0, nullptr, y);
+      sweep(0, y);
       std::swap(x, y);
     }
   }
-  sweep(1, nullptr, X::e(l));
+  sweep(1, X::e(l));
   q(k_st_restrict<TP, D>, grid_for((int64_t)c.box.nb * nv), TPB, (const TP*)X::e(l), X::r(c),
     (const uint8_t*)c.alive.p, l.box, c.box, nv, S);
   TP* xc = vcycle<TP, D>(q, h, i + 1, S);
-  // post-smoothing: the first sweep starts from x + P xc
-  sweep(3, xc, y);
-  std::swap(x, y);
-  for (int s = 1; s < h.sweeps; ++s) {
-    sweep(0, nullptr, y);
+  q(k_st_prolong<TP, D>, grid_for(nbv), TPB, (const TP*)xc, x, dinv, l.box, c.box, nv, S);
+  for (int s = 0; s < h.sweeps; ++s) {
+    sweep(0, y);
     std::swap(x, y);
   }
   return x;
@@ -1596,15 +1626,18 @@ double tridiagonal_lmax(const std::vector<double>& diag, const std::vector<doubl
   return hi;
 }
 
-void hx_spectrum(Hx& h) {
+// kLanczosSteps of B-preconditioned CG on P per block in precision TP; the
+// largest eigenvalue of each block's Lanczos tridiagonal into h.lam.
+template <class TP, int D>
+void lanczos(Hx& h) {
+  using X = Sel<TP>;
   Ctx* ctx = h.ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = h.n, split = h.split;
-  if (h.cheb_m <= 1 || n == 0) return;
-  double* r = h.pvec.p;
-  double* pd = h.d64.p;
-  double* w = h.w64.p;
-  double* z = h.zt64.p;
+  TP* r = X::res(h);
+  TP* pd = X::d(h);
+  TP* w = X::w(h);
+  TP* z = X::zt(h);
   {
     std::vector<double> x(n);
     uint64_t v = 0x2545F4914F6CDD1DULL;
@@ -1614,17 +1647,21 @@ void hx_spectrum(Hx& h) {
       v ^= v << 17;
       x[i] = 0.5 + (double)(v >> 11) * (1.0 / 9007199254740992.0);
     }
-    copy_to_device(ctx, r, x.data(), n * 8);
+    copy_to_device(ctx, h.pvec.p, x.data(), n * 8);
+    if constexpr (std::is_same_v<TP, float>)
+      k_convert<double, float><<<grid_for(n), TPB, 0, st>>>(h.pvec.p, r, n);
+    else
+      FEEC_HIP(hipMemcpyAsync(r, h.pvec.p, n * 8, hipMemcpyDeviceToDevice, st));
   }
   Enq q{st};
   const int64_t lo[2] = {0, split}, hi[2] = {split, n};
   const bool has[2] = {split > 0, n > split};
   std::vector<double> part(2 * (size_t)NBLK);
-  auto dots = [&](const double* x, const double* y, double out[2]) {
+  auto dots = [&](const TP* x, const TP* y, double out[2]) {
     unsigned g[2];
     for (int b = 0; b < 2; ++b) {
       g[b] = red_grid(hi[b] - lo[b]);
-      if (has[b]) k_dotr<<<g[b], TPB, 0, st>>>(x, y, lo[b], hi[b], h.part.p + b * NBLK);
+      if (has[b]) k_dotr<TP><<<g[b], TPB, 0, st>>>(x, y, lo[b], hi[b], h.part.p + b * NBLK);
     }
     FEEC_HIP(hipGetLastError());
     copy_to_host(ctx, part.data(), h.part.p, part.size() * 8);
@@ -1634,13 +1671,13 @@ void hx_spectrum(Hx& h) {
         for (unsigned i = 0; i < g[b]; ++i) out[b] += part[b * NBLK + i];
     }
   };
-  hx_core(q, h, r);
-  FEEC_HIP(hipMemcpyAsync(pd, z, n * 8, hipMemcpyDeviceToDevice, st));
+  hx_core_d<TP, TP, D>(q, h, (const TP*)r, nullptr);
+  FEEC_HIP(hipMemcpyAsync(pd, z, n * sizeof(TP), hipMemcpyDeviceToDevice, st));
   double rho[2], pw[2], rn[2];
   dots(r, z, rho);
   std::vector<double> al[2], be[2];
   for (int j = 0; j < kLanczosSteps; ++j) {
-    k_pc_spmv<double, double, 4><<<grid_for(n * 4), TPB, 0, st>>>(h.P_rp, h.P_col, h.P_val, pd, w, n, nullptr);
+    k_pc_spmv<TP, TP, 4><<<grid_for(n * 4), TPB, 0, st>>>(h.P_rp, h.P_col, X::pval(h), pd, w, n, nullptr);
     FEEC_HIP(hipGetLastError());
     dots(pd, w, pw);
     double a[2] = {0.0, 0.0};
@@ -1651,9 +1688,9 @@ void hx_spectrum(Hx& h) {
         a[b] = rho[b] / pw[b];
         al[b].push_back(a[b]);
       }
-    k_block_axpy<<<grid_for(n), TPB, 0, st>>>(w, r, n, split, -a[0], -a[1]);
+    k_block_axpy<TP><<<grid_for(n), TPB, 0, st>>>(w, r, n, split, -a[0], -a[1]);
     FEEC_HIP(hipGetLastError());
-    hx_core(q, h, r);
+    hx_core_d<TP, TP, D>(q, h, (const TP*)r, nullptr);
     dots(r, z, rn);
     double bt[2] = {0.0, 0.0};
     for (int b = 0; b < 2; ++b)
@@ -1662,7 +1699,7 @@ void hx_spectrum(Hx& h) {
         be[b].push_back(bt[b]);
         rho[b] = rn[b];
       }
-    k_block_xpby<<<grid_for(n), TPB, 0, st>>>(z, pd, n, split, bt[0], bt[1]);
+    k_block_xpby<TP><<<grid_for(n), TPB, 0, st>>>(z, pd, n, split, bt[0], bt[1]);
     FEEC_HIP(hipGetLastError());
   }
   for (int b = 0; b < 2; ++b) {
@@ -1677,6 +1714,25 @@ void hx_spectrum(Hx& h) {
     }
     h.lam[b] = tridiagonal_lmax(diag, off);
   }
+}
+
+template <class TP>
+void lanczos_d(Hx& h) {
+  switch (h.d) {
+    case 1: lanczos<TP, 1>(h); break;
+    case 2: lanczos<TP, 2>(h); break;
+    case 3: lanczos<TP, 3>(h); break;
+    default: lanczos<TP, 4>(h); break;
+  }
+}
+
+void hx_spectrum(Hx& h) {
+  if (h.cheb_m <= 1 || h.n == 0) return;
+  if (h.spectrum_fp32)
+    lanczos_d<float>(h);
+  else
+    lanczos_d<double>(h);
+  Ctx* ctx = h.ctx;
   // coefficients on [frac b, b], b = 1.05 lambda_max, per block
   std::vector<double> c(2 + 3 * (size_t)(h.cheb_m - 1));
   double theta[2], delta[2];
@@ -1769,13 +1825,12 @@ void hx_kernel_times(Hx& h, int reps, bool use_graph, double out[4]) {
     (void)hipGraphDestroy(graph);
     return sec;
   };
-  const Box cb = h.L.size() > 1 ? h.L[1].box : l0.box;
   out[0] = h.L.size() > 1 ? timed([&](Enq& q) {
     switch (h.d) {
-      case 1: sweep_launch<float, 1, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, nullptr, l0.y32.p, cb, nv, nullptr); break;
-      case 2: sweep_launch<float, 2, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, nullptr, l0.y32.p, cb, nv, nullptr); break;
-      case 3: sweep_launch<float, 3, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, nullptr, l0.y32.p, cb, nv, nullptr); break;
-      default: sweep_launch<float, 4, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, nullptr, l0.y32.p, cb, nv, nullptr); break;
+      case 1: sweep_launch<float, 1, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, l0.y32.p, nv, nullptr); break;
+      case 2: sweep_launch<float, 2, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, l0.y32.p, nv, nullptr); break;
+      case 3: sweep_launch<float, 3, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, l0.y32.p, nv, nullptr); break;
+      default: sweep_launch<float, 4, true>(q, 0, l0, l0.A32.p, l0.dinv32.p, l0.r32.p, l0.x32.p, l0.y32.p, nv, nullptr); break;
     }
   }) : 0.0;
   out[1] = timed([&](Enq& q) { hx_apply<double, float, double>(q, h, (const double*)r.p, z.p, nullptr, part.p); });

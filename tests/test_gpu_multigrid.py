@@ -35,7 +35,7 @@ def test_hx_application_matches_oracle(d, n, amp, k, rel, m):
     and the coarse inverse's method differ), FP32 to 1e-5 (m = 1) or 1e-4
     (m = 4: four cycles and three products by hdif_gram in FP32). The
     lambda_max estimates (6 Lanczos steps from the same start) equal the
-    oracle's to 1e-9."""
+    oracle's to 1e-9 on the FP64 path and to 1e-4 with the FP32 cycle."""
     ocx, oL, gcx, geo, cells, coords = both(d, n, amp, seed=n)
     gc = fa.WhitneyComplex(geo, relative=rel)
     prob = fa.Problem(gc, k, mg_opts(precision=fa.FP64, cheb_degree=m))
@@ -51,6 +51,12 @@ def test_hx_application_matches_oracle(d, n, amp, k, rel, m):
         for b in range(2 if k > 0 else 1):
             i = b if k > 0 else 1
             assert abs(lam[i] - want_lam[i]) <= 1e-9 * want_lam[i], (lam, want_lam)
+        # a mixed-precision problem estimates with the FP32 cycle
+        lam32 = fa.Problem(gc, k, mg_opts(precision=fa.FP64_PC32, cheb_degree=m))
+        lam32.assemble()
+        lam32 = lam32.preconditioner()["lambda_max"]
+        for i in ([0, 1] if k > 0 else [1]):
+            assert abs(lam32[i] - want_lam[i]) <= 1e-4 * want_lam[i], (lam32, want_lam)
     prec = o.hx_block_preconditioner(oc, coords, k, n, nh, cheb_m=m, lam=lam)
     r = np.sin(np.arange(ns + nu + nh) * 0.7 + 0.3)
     want = prec(r)
