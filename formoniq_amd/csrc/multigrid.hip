@@ -54,7 +54,10 @@
 // are computed from grid coordinates, without matrices.
 //
 // Precision. The FP64 path runs every piece in FP64; the mixed paths in FP32
-// (the FP64_PC32 solve rounds r to FP32 on entry and widens z on exit).
+// (the FP64_PC32 solve rounds r to FP32 on entry and widens z on exit), where
+// the Chebyshev steps multiply by an FP32 copy of P symmetrized exactly
+// ((P_ij + P_ji)/2 rounded once for both), so the preconditioner stays a
+// symmetric operator.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -503,6 +506,36 @@ __global__ void __launch_bounds__(TPB) k_pc_spmv(const int64_t* __restrict__ rp,
   if (row < nrows && lane == 0) y[row] = acc;
 }
 
+// P32[q] = FP32 of (P[q] + P[tpos[q]]) / 2, tpos[q] the entry of the
+// transpose: an exactly symmetric single-precision copy of P. (Half precision
+// is not enough: on the gradients, where the curl-curl part vanishes, P is
+// its small mass part, which FP16 rounding of the large entries swamps.)
+__global__ void k_sym_single(const double* __restrict__ val, const int32_t* __restrict__ tpos, int64_t n,
+                             float* __restrict__ out) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < n) out[q] = (float)(0.5 * (val[q] + val[tpos[q]]));
+}
+
+// y = P32 x, columns as int16 offsets from the row or int32; LPR lanes per row.
+template <class TC, int LPR>
+__global__ void __launch_bounds__(TPB) k_pc_spmv32(const int64_t* __restrict__ rp, const TC* __restrict__ col,
+                                                   const float* __restrict__ val, const float* __restrict__ x,
+                                                   float* __restrict__ y, int64_t nrows, const MinresS* S) {
+  if (skip(S)) return;
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gid / LPR;
+  const int lane = (int)(gid % LPR);
+  float acc = 0.f;
+  if (row < nrows)
+    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LPR) {
+      const int64_t c = std::is_same_v<TC, int16_t> ? row + (int64_t)col[p] : (int64_t)col[p];
+      acc += val[p] * x[c];
+    }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (row < nrows && lane == 0) y[row] = acc;
+}
+
 // res = zt, d = res / theta, z = 0
 template <class TP>
 __global__ void __launch_bounds__(TPB) k_hxc_first(const TP* __restrict__ zt, TP* __restrict__ res,
@@ -764,7 +797,9 @@ struct Hx {
   const int32_t* P_col = nullptr;
   const double* P_val = nullptr;
   int64_t P_nnz = 0;
-  DBuf<float> P32;
+  DBuf<float> P32;              // exactly symmetric FP32 copy: the FP32 Chebyshev steps' P
+  DBuf<int32_t> P_tpos;         // entry of P's transpose, per entry
+  DBuf<int16_t> P_col16;        // column offsets from the row, when all fit 16 bits
   DBuf<double> coef, pvec, part;
   DBuf<double> res64, d64, zc64, w64;
   DBuf<float> res32, d32, zc32, w32;
@@ -1096,13 +1131,50 @@ namespace {
 void hx_spectrum(Hx& h);
 
 // The KKT preconditioner matrix P (blockdiag(hdif_gram(k-1), hdif_gram(k)))
-// the Chebyshev steps multiply by, and its FP32 copy.
+// the Chebyshev steps multiply by, and its symmetric FP32 copy.
 void bind_pc_matrix(Hx& h, const KktSystem& s) {
   h.P_rp = s.P.row_ptr.p;
   h.P_col = s.P.col.p;
   h.P_val = s.P.val.p;
   h.P_nnz = s.P.nnz;
-  if (h.cheb_m > 1) to32(h.ctx, s.P.val.p, s.P.nnz, h.P32);
+  if (h.cheb_m > 1 && h.P_tpos.p && s.P.nnz)
+    k_sym_single<<<grid_for(s.P.nnz), TPB, 0, h.ctx->stream>>>(s.P.val.p, h.P_tpos.p, s.P.nnz, h.P32.p);
+  FEEC_HIP(hipGetLastError());
+}
+
+// The pattern side of P's FP32 copy: the transpose entry of every entry (P is
+// structurally symmetric with sorted rows, so the counting-sort transpose
+// lists (j, i) exactly where P's row j holds it) and, when every column
+// lies within +-32767 of its row, the 16-bit column offsets.
+void build_pc_single(Hx& h, const KktSystem& s) {
+  Ctx* ctx = h.ctx;
+  const int64_t n = s.P.nrows, nnz = s.P.nnz;
+  std::vector<int64_t> rp(n + 1);
+  std::vector<int32_t> col(nnz);
+  copy_to_host(ctx, rp.data(), s.P.row_ptr.p, (n + 1) * 8);
+  copy_to_host(ctx, col.data(), s.P.col.p, nnz * 4);
+  std::vector<int64_t> at(rp.begin(), rp.end() - 1);
+  std::vector<int32_t> tpos(nnz);
+  bool fit16 = true;
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+      const int64_t q = at[col[p]]++;
+      FEEC_REQUIRE(q < rp[col[p] + 1] && col[q] == i, FEEC_E_DEVICE,
+                   "preconditioner matrix is not structurally symmetric");
+      tpos[p] = (int32_t)q;
+      const int64_t off = (int64_t)col[p] - i;
+      fit16 = fit16 && off >= -32767 && off <= 32767;
+    }
+  h.P_tpos.alloc(ctx, std::max<int64_t>(nnz, 1));
+  h.P32.alloc(ctx, std::max<int64_t>(nnz, 1));
+  if (nnz) copy_to_device(ctx, h.P_tpos.p, tpos.data(), nnz * 4);
+  if (fit16 && nnz) {
+    std::vector<int16_t> c16(nnz);
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t p = rp[i]; p < rp[i + 1]; ++p) c16[p] = (int16_t)(col[p] - i);
+    h.P_col16.alloc(ctx, nnz);
+    copy_to_device(ctx, h.P_col16.p, c16.data(), nnz * 2);
+  }
 }
 }  // namespace
 
@@ -1332,6 +1404,13 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
   h->zt32.alloc(ctx, std::max<int64_t>(h->N, 1));
   smoother_diag(*h, s);
   // the Chebyshev steps
+  if (h->cheb_m > 1 && s.P.nnz) {
+    try {
+      build_pc_single(*h, s);
+    } catch (const Error& e) {
+      return no(e.what());
+    }
+  }
   bind_pc_matrix(*h, s);
   if (h->cheb_m > 1) {
     const int64_t nn = std::max<int64_t>(h->n, 1);
@@ -1365,7 +1444,7 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
         launches += 1;
       }
   if (h->cheb_m > 1) {
-    bytes = bytes * h->cheb_m + (h->cheb_m - 1) * (s.P.nnz * 8 + h->n * 28) + h->n * 12;
+    bytes = bytes * h->cheb_m + (h->cheb_m - 1) * (s.P.nnz * (h->P_col16.p ? 6 : 8) + h->n * 28) + h->n * 12;
     launches = launches * h->cheb_m + 2 * (h->cheb_m - 1) + 2;
   }
   h->bytes32 = bytes;
@@ -1414,7 +1493,6 @@ struct Sel<double> {
   static double* d(Hx& h) { return h.d64.p; }
   static double* zc(Hx& h) { return h.zc64.p; }
   static double* w(Hx& h) { return h.w64.p; }
-  static const double* pval(const Hx& h) { return h.P_val; }
 };
 template <>
 struct Sel<float> {
@@ -1435,7 +1513,6 @@ struct Sel<float> {
   static float* d(Hx& h) { return h.d32.p; }
   static float* zc(Hx& h) { return h.zc32.p; }
   static float* w(Hx& h) { return h.w32.p; }
-  static const float* pval(const Hx& h) { return h.P32.p; }
 };
 
 // One V-cycle on level i with its rhs in r(i); returns the buffer holding
@@ -1549,6 +1626,21 @@ void hx_core_d(Enq& q, Hx& h, const TIN* r, const MinresS* S) {
   }
 }
 
+// y = P x: FP64 values on the FP64 path, the symmetric FP32 copy otherwise.
+template <class TP>
+void pc_spmv(Enq& q, Hx& h, const TP* x, TP* y, const MinresS* S) {
+  const unsigned g = grid_for(h.n * 4);
+  if constexpr (std::is_same_v<TP, double>) {
+    q(k_pc_spmv<double, double, 4>, g, TPB, h.P_rp, h.P_col, h.P_val, x, y, h.n, S);
+  } else {
+    if (h.P_col16.p)
+      q(k_pc_spmv32<int16_t, 4>, g, TPB, h.P_rp, (const int16_t*)h.P_col16.p, (const float*)h.P32.p, x, y,
+        h.n, S);
+    else
+      q(k_pc_spmv32<int32_t, 4>, g, TPB, h.P_rp, h.P_col, (const float*)h.P32.p, x, y, h.n, S);
+  }
+}
+
 // z = B r (cheb_m = 1) or p(B P) B r per block, z = r on the border rows,
 // and the partial sums of r . z.
 template <class TIN, class TP, class TOUT, int D>
@@ -1566,7 +1658,7 @@ void hx_apply_d(Enq& q, Hx& h, const TIN* r, TOUT* z, const MinresS* S, double* 
   const double* coef = h.coef.p;
   q(k_hxc_first<TP>, grid_for(h.n), TPB, (const TP*)X::zt(h), res, d, zc, h.n, h.split, coef, S);
   for (int k = 0; k + 1 < h.cheb_m; ++k) {
-    q(k_pc_spmv<TP, TP, 4>, grid_for(h.n * 4), TPB, h.P_rp, h.P_col, X::pval(h), (const TP*)d, w, h.n, S);
+    pc_spmv<TP>(q, h, (const TP*)d, w, S);
     hx_core_d<TP, TP, D>(q, h, (const TP*)w, S);
     q(k_hxc_step<TP>, grid_for(h.n), TPB, (const TP*)X::zt(h), res, d, zc, h.n, h.split, coef, k, S);
   }
@@ -1597,6 +1689,11 @@ void hx_core(Enq& q, Hx& h, const double* r) {
 // deterministic start every time, and the largest eigenvalue of the Lanczos
 // tridiagonal their coefficients make; then the Chebyshev coefficients.
 constexpr int kLanczosSteps = 6;
+// a block's CG stops early once r.Br falls below this fraction of its start,
+// near the roundoff floor of the arithmetic (the estimate from fewer steps is
+// still a Ritz value, a lower bound)
+template <class TP>
+constexpr double kLanczosStop = std::is_same_v<TP, float> ? 1e-10 : 1e-20;
 constexpr double kLmaxMargin = 1.05;
 
 // Largest eigenvalue of the symmetric tridiagonal (diag, off): bisection on
@@ -1673,32 +1770,43 @@ void lanczos(Hx& h) {
   };
   hx_core_d<TP, TP, D>(q, h, (const TP*)r, nullptr);
   FEEC_HIP(hipMemcpyAsync(pd, z, n * sizeof(TP), hipMemcpyDeviceToDevice, st));
-  double rho[2], pw[2], rn[2];
+  double rho[2], rho0[2], pw[2], rn[2];
   dots(r, z, rho);
+  bool live[2];
+  for (int b = 0; b < 2; ++b) {
+    rho0[b] = rho[b];
+    live[b] = has[b] && rho[b] > 0.0 && std::isfinite(rho[b]);
+    FEEC_REQUIRE(live[b] || !has[b], FEEC_E_NOT_SPD, "multigrid preconditioner is not positive");
+  }
   std::vector<double> al[2], be[2];
-  for (int j = 0; j < kLanczosSteps; ++j) {
-    k_pc_spmv<TP, TP, 4><<<grid_for(n * 4), TPB, 0, st>>>(h.P_rp, h.P_col, X::pval(h), pd, w, n, nullptr);
-    FEEC_HIP(hipGetLastError());
+  for (int j = 0; j < kLanczosSteps && (live[0] || live[1]); ++j) {
+    pc_spmv<TP>(q, h, (const TP*)pd, w, nullptr);
     dots(pd, w, pw);
     double a[2] = {0.0, 0.0};
-    for (int b = 0; b < 2; ++b)
-      if (has[b]) {
-        FEEC_REQUIRE(pw[b] > 0.0 && rho[b] > 0.0 && std::isfinite(rho[b]), FEEC_E_NOT_SPD,
-                     "multigrid preconditioner spectrum estimate failed");
-        a[b] = rho[b] / pw[b];
-        al[b].push_back(a[b]);
+    for (int b = 0; b < 2; ++b) {
+      if (!live[b]) continue;
+      if (!(pw[b] > 0.0) || !std::isfinite(pw[b])) {  // the block's CG has run into roundoff
+        live[b] = false;
+        continue;
       }
+      a[b] = rho[b] / pw[b];
+      al[b].push_back(a[b]);
+    }
     k_block_axpy<TP><<<grid_for(n), TPB, 0, st>>>(w, r, n, split, -a[0], -a[1]);
     FEEC_HIP(hipGetLastError());
     hx_core_d<TP, TP, D>(q, h, (const TP*)r, nullptr);
     dots(r, z, rn);
     double bt[2] = {0.0, 0.0};
-    for (int b = 0; b < 2; ++b)
-      if (has[b]) {
-        bt[b] = rn[b] / rho[b];
-        be[b].push_back(bt[b]);
-        rho[b] = rn[b];
+    for (int b = 0; b < 2; ++b) {
+      if (!live[b]) continue;
+      if (!(rn[b] > kLanczosStop<TP> * rho0[b]) || !std::isfinite(rn[b])) {  // converged: stop this block
+        live[b] = false;
+        continue;
       }
+      bt[b] = rn[b] / rho[b];
+      be[b].push_back(bt[b]);
+      rho[b] = rn[b];
+    }
     k_block_xpby<TP><<<grid_for(n), TPB, 0, st>>>(z, pd, n, split, bt[0], bt[1]);
     FEEC_HIP(hipGetLastError());
   }
@@ -1706,12 +1814,10 @@ void lanczos(Hx& h) {
     h.lam[b] = 0.0;
     if (!has[b]) continue;
     const int m = (int)al[b].size();
+    FEEC_REQUIRE(m > 0, FEEC_E_NOT_SPD, "multigrid preconditioner spectrum estimate failed");
     std::vector<double> diag(m), off(std::max(m - 1, 0));
     for (int j = 0; j < m; ++j) diag[j] = 1.0 / al[b][j] + (j ? be[b][j - 1] / al[b][j - 1] : 0.0);
-    for (int j = 0; j + 1 < m; ++j) {
-      FEEC_REQUIRE(be[b][j] >= 0.0, FEEC_E_NOT_SPD, "multigrid preconditioner is not positive");
-      off[j] = std::sqrt(be[b][j]) / al[b][j];
-    }
+    for (int j = 0; j + 1 < m; ++j) off[j] = std::sqrt(be[b][j]) / al[b][j];
     h.lam[b] = tridiagonal_lmax(diag, off);
   }
 }
@@ -1835,8 +1941,7 @@ void hx_kernel_times(Hx& h, int reps, bool use_graph, double out[4]) {
   }) : 0.0;
   out[1] = timed([&](Enq& q) { hx_apply<double, float, double>(q, h, (const double*)r.p, z.p, nullptr, part.p); });
   out[2] = h.cheb_m > 1 ? timed([&](Enq& q) {
-    q(k_pc_spmv<float, float, 4>, grid_for(h.n * 4), TPB, h.P_rp, h.P_col, (const float*)h.P32.p,
-      (const float*)h.d32.p, h.w32.p, h.n, (const MinresS*)nullptr);
+    pc_spmv<float>(q, h, (const float*)h.d32.p, h.w32.p, nullptr);
   }) : 0.0;
   out[3] = timed([&](Enq& q) {
     switch (h.d) {

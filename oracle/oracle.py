@@ -1008,6 +1008,8 @@ HX_CHEB_FRACTION = 0.25  # lower end of the interval, a fraction of its upper en
 
 
 HX_LANCZOS_STEPS = 6
+# a block stops once r.Br < this fraction of its start (FP32 path, FP64 path)
+HX_LANCZOS_STOP = {True: 1e-10, False: 1e-20}
 
 
 def tridiagonal_lmax(diag, off):
@@ -1031,6 +1033,14 @@ def tridiagonal_lmax(diag, off):
     return hi
 
 
+def single_symmetric(A):
+    """The FP32 path's copy of a preconditioner block for its Chebyshev
+    steps: (A + A^T)/2 rounded to FP32, returned in FP64."""
+    S = (0.5 * (A + A.T)).tocsr()
+    S.data = S.data.astype(np.float32).astype(np.float64)
+    return S
+
+
 def hx_blocks(c: WhitneyComplex, coords, k, n, sweeps=2):
     """The HX preconditioners (B_{k-1} or None, B_k) of the two KKT blocks,
     sharing one nodal V-cycle."""
@@ -1039,15 +1049,20 @@ def hx_blocks(c: WhitneyComplex, coords, k, n, sweeps=2):
     return hs, hu
 
 
-def hx_block_lambdas(c: WhitneyComplex, coords, k, n, steps=HX_LANCZOS_STEPS, sweeps=2, blocks=None):
+def hx_block_lambdas(c: WhitneyComplex, coords, k, n, steps=HX_LANCZOS_STEPS, sweeps=2, blocks=None,
+                     single=False):
     """lambda_max of B_j hdif_gram(j) per block (sigma, u): `steps` steps of
-    CG on hdif_gram(j) preconditioned by B_j from the xorshift start, and the
-    largest eigenvalue of the Lanczos tridiagonal their coefficients make
-    (T_jj = 1/alpha_j + beta_{j-1}/alpha_{j-1}, T_{j,j+1} = sqrt(beta_j)/alpha_j)."""
+    CG on hdif_gram(j) preconditioned by B_j from the xorshift start (fewer
+    once r.Br < HX_LANCZOS_STOP of its start or p.Ap stops being positive;
+    `single`: the FP32 path's estimate, on its FP32 copy of hdif_gram),
+    and the largest eigenvalue of the Lanczos tridiagonal their coefficients
+    make (T_jj = 1/alpha_j + beta_{j-1}/alpha_{j-1},
+    T_{j,j+1} = sqrt(beta_j)/alpha_j)."""
     ns, nu = c.ndofs(k - 1), c.ndofs(k)
     hs, hu = blocks or hx_blocks(c, coords, k, n, sweeps)
-    blocks = [(hs, c.hdif_gram(k - 1).to_scipy().tocsr() if ns else None, 0, ns),
-              (hu, c.hdif_gram(k).to_scipy().tocsr(), ns, ns + nu)]
+    conv = single_symmetric if single else (lambda A: A)
+    blocks = [(hs, conv(c.hdif_gram(k - 1).to_scipy().tocsr()) if ns else None, 0, ns),
+              (hu, conv(c.hdif_gram(k).to_scipy().tocsr()), ns, ns + nu)]
     v = _xorshift_start(ns + nu)
     lam = [0.0, 0.0]
     for b, (B, A, lo, hi) in enumerate(blocks):
@@ -1056,26 +1071,32 @@ def hx_block_lambdas(c: WhitneyComplex, coords, k, n, steps=HX_LANCZOS_STEPS, sw
         r = v[lo:hi].copy()
         z = B.apply(r)
         p = z.copy()
-        rho = r @ z
+        rho0 = rho = r @ z
         al, be = [], []
         for _ in range(steps):
             q = A @ p
-            a = rho / (p @ q)
+            pw = p @ q
+            if not pw > 0:  # the block's CG has run into roundoff
+                break
+            a = rho / pw
+            al.append(a)
             r = r - a * q
             z = B.apply(r)
             rn = r @ z
-            al.append(a)
+            if not rn > HX_LANCZOS_STOP[single] * rho0:  # converged
+                break
             be.append(rn / rho)
             rho = rn
             p = z + be[-1] * p
-        diag = [1.0 / al[j] + (be[j - 1] / al[j - 1] if j else 0.0) for j in range(steps)]
-        off = [math.sqrt(be[j]) / al[j] for j in range(steps - 1)]
+        m = len(al)
+        diag = [1.0 / al[j] + (be[j - 1] / al[j - 1] if j else 0.0) for j in range(m)]
+        off = [math.sqrt(be[j]) / al[j] for j in range(m - 1)]
         lam[b] = tridiagonal_lmax(diag, off)
     return lam
 
 
 def hx_block_preconditioner(c: WhitneyComplex, coords, k, n, nh, sweeps=2, cheb_m=1, lam=None,
-                            frac=HX_CHEB_FRACTION, blocks=None):
+                            frac=HX_CHEB_FRACTION, blocks=None, single=False):
     """The KKT block preconditioner diag(B_{k-1}, B_k, I) (elliptic.rs:29-47)
     with HX blocks; a callable r -> z. With cheb_m > 1 each block is
     p(B A) B with p the degree-(cheb_m - 1) Chebyshev polynomial of the
@@ -1083,7 +1104,8 @@ def hx_block_preconditioner(c: WhitneyComplex, coords, k, n, nh, sweeps=2, cheb_
     with B in place of the Jacobi diagonal): cheb_m applications of B and
     cheb_m - 1 of A = hdif_gram per block. `lam`: lambda_max per block
     (sigma, u), by default from hx_block_lambdas. `blocks`: prebuilt
-    (B_{k-1}, B_k) from hx_blocks."""
+    (B_{k-1}, B_k) from hx_blocks. `single`: the Chebyshev steps multiply by
+    the FP32 path's symmetric FP32 copy of each block (single_symmetric)."""
     ns, nu = c.ndofs(k - 1), c.ndofs(k)
     hs, hu = blocks or hx_blocks(c, coords, k, n, sweeps)
     if cheb_m <= 1:
@@ -1093,9 +1115,10 @@ def hx_block_preconditioner(c: WhitneyComplex, coords, k, n, nh, sweeps=2, cheb_
             return np.concatenate(parts)
         return prec
     if lam is None:
-        lam = hx_block_lambdas(c, coords, k, n, sweeps=sweeps, blocks=(hs, hu))
-    As = c.hdif_gram(k - 1).to_scipy().tocsr() if ns else None
-    Au = c.hdif_gram(k).to_scipy().tocsr()
+        lam = hx_block_lambdas(c, coords, k, n, sweeps=sweeps, blocks=(hs, hu), single=single)
+    conv = single_symmetric if single else (lambda A: A)
+    As = conv(c.hdif_gram(k - 1).to_scipy().tocsr()) if ns else None
+    Au = conv(c.hdif_gram(k).to_scipy().tocsr())
 
     def cheb(B, A, lmax, r):
         b = CHEB_LMAX_MARGIN * lmax

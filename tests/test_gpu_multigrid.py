@@ -32,10 +32,12 @@ def test_hx_application_matches_oracle(d, n, amp, k, rel, m):
     """z = B r with B = diag(HX(k-1), HX(k), I) (m = 1), or each block
     accelerated by the Chebyshev polynomial in B hdif_gram with m
     applications of B: FP64 equal to the oracle's to 1e-10 (summation order
-    and the coarse inverse's method differ), FP32 to 1e-5 (m = 1) or 1e-4
-    (m = 4: four cycles and three products by hdif_gram in FP32). The
-    lambda_max estimates (6 Lanczos steps from the same start) equal the
-    oracle's to 1e-9 on the FP64 path and to 1e-4 with the FP32 cycle."""
+    and the coarse inverse's method differ). The FP32 path (its Chebyshev
+    steps multiply by the exactly symmetric FP32 copy of hdif_gram, which the
+    oracle models with single_symmetric) to 1e-5 (m = 1) or 1e-4 (m = 4: four
+    cycles and three products in FP32). The lambda_max estimates (6 Lanczos
+    steps from the same start) equal the oracle's to 1e-9 on the FP64 path
+    and to 1e-4 on the FP32 path."""
     ocx, oL, gcx, geo, cells, coords = both(d, n, amp, seed=n)
     gc = fa.WhitneyComplex(geo, relative=rel)
     prob = fa.Problem(gc, k, mg_opts(precision=fa.FP64, cheb_degree=m))
@@ -51,23 +53,27 @@ def test_hx_application_matches_oracle(d, n, amp, k, rel, m):
         for b in range(2 if k > 0 else 1):
             i = b if k > 0 else 1
             assert abs(lam[i] - want_lam[i]) <= 1e-9 * want_lam[i], (lam, want_lam)
-        # a mixed-precision problem estimates with the FP32 cycle
-        lam32 = fa.Problem(gc, k, mg_opts(precision=fa.FP64_PC32, cheb_degree=m))
-        lam32.assemble()
-        lam32 = lam32.preconditioner()["lambda_max"]
-        for i in ([0, 1] if k > 0 else [1]):
-            assert abs(lam32[i] - want_lam[i]) <= 1e-4 * want_lam[i], (lam32, want_lam)
     prec = o.hx_block_preconditioner(oc, coords, k, n, nh, cheb_m=m, lam=lam)
     r = np.sin(np.arange(ns + nu + nh) * 0.7 + 0.3)
     want = prec(r)
     got = prob.apply_preconditioner(r, precision=fa.FP64)
     scale = np.abs(want).max()
     assert np.abs(got - want).max() <= 1e-10 * scale, np.abs(got - want).max() / scale
+    # the FP32 path: its own lambda_max (FP32 cycle) and the FP32 copy of P
+    prob32 = fa.Problem(gc, k, mg_opts(precision=fa.FP64_PC32, cheb_degree=m))
+    prob32.assemble()
+    want32 = want
+    if m > 1:
+        lam32 = prob32.preconditioner()["lambda_max"]
+        want_lam32 = o.hx_block_lambdas(oc, coords, k, n, single=True)
+        for i in ([0, 1] if k > 0 else [1]):
+            assert abs(lam32[i] - want_lam32[i]) <= 1e-4 * want_lam32[i], (lam32, want_lam32)
+        want32 = o.hx_block_preconditioner(oc, coords, k, n, nh, cheb_m=m, lam=lam32, single=True)(r)
     tol32 = 1e-5 if m == 1 else 1e-4
-    got32 = prob.apply_preconditioner(r, precision=fa.MIXED)
-    assert np.abs(got32 - want).max() <= tol32 * scale, np.abs(got32 - want).max() / scale
-    got_pc32 = prob.apply_preconditioner(r, precision=fa.FP64_PC32)
-    assert np.abs(got_pc32 - want).max() <= tol32 * scale
+    got32 = prob32.apply_preconditioner(r, precision=fa.MIXED)
+    assert np.abs(got32 - want32).max() <= tol32 * scale, np.abs(got32 - want32).max() / scale
+    got_pc32 = prob32.apply_preconditioner(r, precision=fa.FP64_PC32)
+    assert np.abs(got_pc32 - want32).max() <= tol32 * scale
 
 
 def _iters(d, n, k, rel, amp, precision=fa.FP64, m=1):
@@ -174,8 +180,8 @@ def test_multigrid_needs_a_kuhn_grid_and_coordinates():
 
 def test_c2_full_size_multigrid():
     """The 64^3 k = 1 system: FP64_PC32 and FP64 with the multigrid
-    preconditioner certified to 1e-10, and the same solution as the
-    polynomial's (to the solve tolerance)."""
+    preconditioner certified to 1e-10 in at most 30 MINRES iterations, and
+    the same solution as the polynomial's (to the solve tolerance)."""
     _, _, gcx, geo, _, _ = both(3, 64, 0.05, seed=2)
     wc = fa.WhitneyComplex(geo)
     f = fa.normal_vector(gcx.nsimplices(1), 2)
@@ -190,6 +196,8 @@ def test_c2_full_size_multigrid():
           % tuple(np.diff(t)), flush=True)
     for rep in (rep_mg, rep64, rep_ch):
         assert rep.converged and rep.residual <= 1e-10, rep.as_dict()
+    # the mesh-independent count (15-17 with the default degree), far below the polynomial's
+    assert rep_mg.iters <= 30 and rep64.iters <= 30, (rep_mg.iters, rep64.iters)
     assert np.linalg.norm(u_mg - u_ch) <= 1e-8 * np.linalg.norm(u_ch)
     assert np.linalg.norm(u64 - u_ch) <= 1e-8 * np.linalg.norm(u_ch)
     print("64^3 iterations: multigrid FP64_PC32", rep_mg.iters, "FP64", rep64.iters,

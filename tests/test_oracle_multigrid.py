@@ -167,13 +167,15 @@ def test_vector_nodal_interpolates_constant_forms(d, k):
     assert constant_form_error(d, k) <= 1e-14
 
 
-@pytest.mark.parametrize("m", [1, 4])
+@pytest.mark.parametrize("m,single", [(1, False), (4, False), (4, True)])
 @pytest.mark.parametrize("d,n,k,rel", [(2, 8, 1, True), (2, 6, 2, False), (3, 3, 1, False)])
-def test_hx_block_preconditioner_is_spd(d, n, k, rel, m):
+def test_hx_block_preconditioner_is_spd(d, n, k, rel, m, single):
+    """Plain, Chebyshev-accelerated, and accelerated with the FP32 path's
+    symmetric FP32 copy of hdif_gram in the Chebyshev steps."""
     cx, coords, L = grid(d, n, 0.1, 5)
     c = o.WhitneyComplex(cx, L, relative=rel)
     nh = c.harmonics(k).shape[1]
-    prec = o.hx_block_preconditioner(c, coords, k, n, nh, cheb_m=m)
+    prec = o.hx_block_preconditioner(c, coords, k, n, nh, cheb_m=m, single=single)
     N = c.ndofs(k - 1) + c.ndofs(k) + nh
     B = np.column_stack([prec(e) for e in np.eye(N)])
     assert np.abs(B - B.T).max() <= 1e-12 * np.abs(B).max()
