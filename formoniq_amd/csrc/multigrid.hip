@@ -442,6 +442,32 @@ __global__ void __launch_bounds__(TPB) k_hx_out(const int64_t* __restrict__ rp, 
   y[i] = acc;
 }
 
+// The grade-1 block's scatter without a matrix: row e is the edge (v0, v1),
+// v0 < v1 (box vertices), D_0 has -1 at v0 and +1 at v1 and Pi_1 the same D
+// values eh = (x(v1) - x(v0)) / 2 at both ends:
+//   z[e] = s[e] r[e] + (G(v1) - G(v0)) + sum_c eh[e, c] (X_c(v0) + X_c(v1)),
+// G(v) = x0[v, slot0] + dl[v] r0[v, slot0] and X_c(v) = x0[v, slot0 + 1 + c].
+// Vertices that are not unknowns hold zeros in every slot, so both ends are
+// always read.
+template <class TP, class TB, int D>
+__global__ void __launch_bounds__(TPB) k_hx_out_edge(const int32_t* __restrict__ ev, const TP* __restrict__ eh,
+                                                     const TP* __restrict__ x0, const TP* __restrict__ r0,
+                                                     const TP* __restrict__ dl, int nv, int slot0,
+                                                     const TP* __restrict__ s, const TB* __restrict__ r,
+                                                     TP* __restrict__ z, int64_t nrows, const MinresS* S) {
+  if (skip(S)) return;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= nrows) return;
+  const int v0 = ev[2 * e], v1 = ev[2 * e + 1];
+  const TP* a = x0 + (int64_t)v0 * nv + slot0;
+  const TP* b = x0 + (int64_t)v1 * nv + slot0;
+  TP acc = s[e] * (TP)r[e];
+  acc += (b[0] + dl[v1] * r0[(int64_t)v1 * nv + slot0]) - (a[0] + dl[v0] * r0[(int64_t)v0 * nv + slot0]);
+#pragma unroll
+  for (int c = 0; c < D; ++c) acc += eh[e * D + c] * (a[1 + c] + b[1 + c]);
+  z[e] = acc;
+}
+
 __device__ __forceinline__ double dev_block_sum(double v, double* sh) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -768,6 +794,12 @@ struct HxBlock {
   int slot_tl = 0;
   DBuf<double> sl, t64, u64;  // sl = S_1 on the grade-1 DOFs
   DBuf<float> sl32, t32, u32;
+  // j = 1: the scatter without a matrix (k_hx_out_edge) when D_0's signs follow
+  // the sorted ends: the ends per row and the Pi_1 values
+  bool edge_free = false;
+  DBuf<int32_t> ev;
+  DBuf<double> eh64;
+  DBuf<float> eh32;
 };
 
 struct Hx {
@@ -1368,7 +1400,42 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
       HostCsr Pi = vector_nodal(m, g.coords, 1, map_j, live_col.data(), b.n, nb0);
       HostCsr DP = interleave(D, Pi);
       upload(ctx, transpose(DP), b.g_main);
-      upload(ctx, DP, b.o_main);
+      // the matrix-free scatter: ends and Pi values per row, after checking
+      // that every D_0 entry is -1 at the smaller end and +1 at the larger
+      std::vector<int64_t> sv(m.n[1] * 2);
+      if (feec_mesh_skeleton(static_cast<feec_mesh*>(&m), 1, sv.data()) != FEEC_OK)
+        fail(FEEC_E_DEVICE, feec_last_error());
+      std::vector<int64_t> edge_of(b.n, -1);
+      for (int64_t e = 0; e < m.n[1]; ++e) {
+        const int32_t r = map_j ? map_j[e] : (int32_t)e;
+        if (r >= 0 && r < b.n) edge_of[r] = e;
+      }
+      bool ok = true;
+      std::vector<int32_t> ev(2 * b.n);
+      std::vector<double> eh((size_t)b.n * d);
+      for (int64_t r = 0; r < b.n && ok; ++r) {
+        const int64_t e = edge_of[r];
+        ok = e >= 0 && sv[2 * e] < sv[2 * e + 1];
+        if (!ok) break;
+        ev[2 * r] = (int32_t)sv[2 * e];
+        ev[2 * r + 1] = (int32_t)sv[2 * e + 1];
+        for (int64_t p = D.rp[r]; p < D.rp[r + 1]; ++p)
+          ok = ok && ((D.col[p] == ev[2 * r] && D.val[p] == -1.0) || (D.col[p] == ev[2 * r + 1] && D.val[p] == 1.0));
+        for (int c = 0; c < d; ++c)  // vector_nodal's row values: (1/2!) (x(v1) - x(v0))
+          eh[r * d + c] = (1.0 / 2.0) * (g.coords[ev[2 * r + 1] * d + c] - g.coords[ev[2 * r] * d + c]);
+      }
+      if (ok && d <= 4) {
+        b.edge_free = true;
+        b.ev.alloc(ctx, std::max<int64_t>(2 * b.n, 1));
+        b.eh64.alloc(ctx, std::max<int64_t>(b.n * d, 1));
+        if (b.n) {
+          copy_to_device(ctx, b.ev.p, ev.data(), 2 * b.n * 4);
+          copy_to_device(ctx, b.eh64.p, eh.data(), b.n * d * 8);
+        }
+        to32(ctx, b.eh64.p, b.n * d, b.eh32);
+      } else {
+        upload(ctx, DP, b.o_main);
+      }
       b.slot_main = slot;
       slot += DP.C;
     } else {
@@ -1489,6 +1556,7 @@ struct Sel<double> {
   static const double* sl(const HxBlock& b) { return b.sl.p; }
   static double* t(HxBlock& b) { return b.t64.p; }
   static double* u(HxBlock& b) { return b.u64.p; }
+  static const double* eh(const HxBlock& b) { return b.eh64.p; }
   static double* res(Hx& h) { return h.res64.p; }
   static double* d(Hx& h) { return h.d64.p; }
   static double* zc(Hx& h) { return h.zc64.p; }
@@ -1509,6 +1577,7 @@ struct Sel<float> {
   static const float* sl(const HxBlock& b) { return b.sl32.p; }
   static float* t(HxBlock& b) { return b.t32.p; }
   static float* u(HxBlock& b) { return b.u32.p; }
+  static const float* eh(const HxBlock& b) { return b.eh32.p; }
   static float* res(Hx& h) { return h.res32.p; }
   static float* d(Hx& h) { return h.d32.p; }
   static float* zc(Hx& h) { return h.zc32.p; }
@@ -1619,6 +1688,9 @@ void hx_core_d(Enq& q, Hx& h, const TIN* r, const MinresS* S) {
       q(k_hx_out<TP, TP, TIN>, grid_for(om.m.nrows), TPB, om.m.row_ptr.p, om.m.col.p, X::v(om), om.C, 0, xs,
         (const TP*)slots, X::dinv(l0), nv, b.slot_main, (const TP*)nullptr, (const TIN*)nullptr, zb,
         om.m.nrows, 1, S);
+    } else if (b.edge_free) {
+      q(k_hx_out_edge<TP, TIN, D>, grid_for(b.n), TPB, (const int32_t*)b.ev.p, X::eh(b), xs, (const TP*)slots,
+        X::dinv(l0), nv, b.slot_main, sb, rb, zb, b.n, S);
     } else {
       q(k_hx_out<TP, TP, TIN>, grid_for(om.m.nrows), TPB, om.m.row_ptr.p, om.m.col.p, X::v(om), om.C,
         b.j == 1 ? 1 : 0, xs, (const TP*)slots, X::dinv(l0), nv, b.slot_main, sb, rb, zb, om.m.nrows, 0, S);
