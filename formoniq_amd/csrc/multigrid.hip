@@ -87,6 +87,15 @@ inline unsigned red_grid(int64_t n) {
 
 __device__ __forceinline__ bool skip(const MinresS* S) { return S && S->done; }
 
+// The block's position in the index space, remapped so that each of the 8
+// XCDs (to which blocks are dealt round-robin) works on one contiguous range:
+// stencil neighbours and gathered vector entries then mostly hit that XCD's
+// own L2 instead of being fetched once per XCD. A bijection on [0, gridDim.x).
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned b = blockIdx.x, per = gridDim.x / 8;
+  return b < per * 8 ? (b % 8) * per + b / 8 : b;
+}
+
 // A level's vertex box, passed by value.
 struct Box {
   int g = 0;   // cells per axis
@@ -129,7 +138,7 @@ __global__ void __launch_bounds__(TPB) k_st_scale(const TP* __restrict__ dinv, c
                                                   TP* __restrict__ x, unsigned nb, unsigned nv,
                                                   const MinresS* S) {
   if (skip(S)) return;
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nb * nv) return;
   x[t] = dinv[t / nv] * r[t];
 }
@@ -155,7 +164,7 @@ __global__ void __launch_bounds__(TPB) k_st_sweep(const TP* __restrict__ A, cons
                                                   TP* __restrict__ out, Box b, unsigned nv, const MinresS* S) {
   constexpr int H = (1 << D) - 1;
   if (skip(S)) return;
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned t = xcd_block() * blockDim.x + threadIdx.x;
   const unsigned nb = (unsigned)b.nb;
   if (t >= nb * nv) return;
   const unsigned i = t / nv;
@@ -188,7 +197,7 @@ __global__ void __launch_bounds__(TPB) k_st_restrict(const TP* __restrict__ ef, 
                                                      unsigned nv, const MinresS* S) {
   constexpr int H = (1 << D) - 1;
   if (skip(S)) return;
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= (unsigned)cb.nb * nv) return;
   const unsigned w = t / nv;
   const unsigned v = t - w * nv;
@@ -229,7 +238,7 @@ __global__ void __launch_bounds__(TPB) k_st_prolong(const TP* __restrict__ xc, T
                                                     const TP* __restrict__ dinv_f, Box fb, Box cb,
                                                     unsigned nv, const MinresS* S) {
   if (skip(S)) return;
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= (unsigned)fb.nb * nv) return;
   const unsigned k = t / nv;
   const unsigned v = t - k * nv;
@@ -390,7 +399,7 @@ __global__ void __launch_bounds__(TPB) k_hx_gather(const int64_t* __restrict__ r
                                                    const TX* __restrict__ x, TP* __restrict__ y, int nv,
                                                    int slot0, int64_t nrows, const MinresS* S) {
   if (skip(S)) return;
-  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t gid = xcd_block() * (int64_t)blockDim.x + threadIdx.x;
   const int64_t row = gid / LPR;
   const int lane = (int)(gid % LPR);
   TP acc[MAXC];
@@ -429,7 +438,7 @@ __global__ void __launch_bounds__(TPB) k_hx_out(const int64_t* __restrict__ rp, 
                                                 const TA* __restrict__ a, const TB* __restrict__ b,
                                                 TP* __restrict__ y, int64_t nrows, int add, const MinresS* S) {
   if (skip(S)) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block() * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nrows) return;
   TP acc = a ? (TP)a[i] * (TP)b[i] : (add ? y[i] : TP(0));
   for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
@@ -456,7 +465,7 @@ __global__ void __launch_bounds__(TPB) k_hx_out_edge(const int32_t* __restrict__
                                                      const TP* __restrict__ s, const TB* __restrict__ r,
                                                      TP* __restrict__ z, int64_t nrows, const MinresS* S) {
   if (skip(S)) return;
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t e = xcd_block() * (int64_t)blockDim.x + threadIdx.x;
   if (e >= nrows) return;
   const int v0 = ev[2 * e], v1 = ev[2 * e + 1];
   const TP* a = x0 + (int64_t)v0 * nv + slot0;
@@ -548,7 +557,7 @@ __global__ void __launch_bounds__(TPB) k_pc_spmv32(const int64_t* __restrict__ r
                                                    const float* __restrict__ val, const float* __restrict__ x,
                                                    float* __restrict__ y, int64_t nrows, const MinresS* S) {
   if (skip(S)) return;
-  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t gid = xcd_block() * (int64_t)blockDim.x + threadIdx.x;
   const int64_t row = gid / LPR;
   const int lane = (int)(gid % LPR);
   float acc = 0.f;
