@@ -635,6 +635,92 @@ __global__ void __launch_bounds__(TPB) k_dotr(const TP* __restrict__ x, const TP
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// Lanczos state on the device (doubles): [0..1] rho, [2..3] rho0, [4..5] live,
+// [6..7] alpha of the step, [8..9] beta of the step, [10..11] alphas kept,
+// [12..13] betas kept, then alphas at 14 + 2 j + b and betas at
+// 14 + 2 kMaxLanczos + 2 j + b.
+constexpr int kMaxLanczos = 16;
+// the sum of np partials, on a launch of one wavefront (every lane gets it)
+__device__ __forceinline__ double lz_sum(const double* __restrict__ part, int np) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// rho = rho0 = r.Br per block; live when positive and finite
+__global__ void k_lz_init(const double* __restrict__ part, int g0, int g1, double* __restrict__ lz) {
+  for (int b = 0; b < 2; ++b) {
+    const double v = lz_sum(part + b * NBLK, b ? g1 : g0);
+    if (threadIdx.x == 0) {
+      const bool has = (b ? g1 : g0) > 0;
+      lz[b] = v;
+      lz[2 + b] = v;
+      lz[4 + b] = has && v > 0.0 && isfinite(v) ? 1.0 : 0.0;
+      lz[10 + b] = 0.0;
+      lz[12 + b] = 0.0;
+    }
+  }
+}
+
+// alpha = rho / p.Pp per live block (a block whose p.Pp is not positive stops)
+__global__ void k_lz_alpha(const double* __restrict__ part, int g0, int g1, double* __restrict__ lz) {
+  for (int b = 0; b < 2; ++b) {
+    const double pw = lz_sum(part + b * NBLK, b ? g1 : g0);
+    if (threadIdx.x == 0) {
+      double a = 0.0;
+      if (lz[4 + b] != 0.0) {
+        if (!(pw > 0.0) || !isfinite(pw)) {
+          lz[4 + b] = 0.0;
+        } else {
+          a = lz[b] / pw;
+          const int j = (int)lz[10 + b];
+          lz[14 + 2 * j + b] = a;
+          lz[10 + b] = j + 1;
+        }
+      }
+      lz[6 + b] = a;
+    }
+  }
+}
+
+// beta = r.Br (new) / rho per live block (a block whose r.Br fell below stop
+// of its start stops), rho = r.Br
+__global__ void k_lz_beta(const double* __restrict__ part, int g0, int g1, double stop, double* __restrict__ lz) {
+  for (int b = 0; b < 2; ++b) {
+    const double rn = lz_sum(part + b * NBLK, b ? g1 : g0);
+    if (threadIdx.x == 0) {
+      double bt = 0.0;
+      if (lz[4 + b] != 0.0) {
+        if (!(rn > stop * lz[2 + b]) || !isfinite(rn)) {
+          lz[4 + b] = 0.0;
+        } else {
+          bt = rn / lz[b];
+          const int j = (int)lz[12 + b];
+          lz[14 + 2 * kMaxLanczos + 2 * j + b] = bt;
+          lz[12 + b] = j + 1;
+          lz[b] = rn;
+        }
+      }
+      lz[8 + b] = bt;
+    }
+  }
+}
+
+// y -= alpha_b x and p = z + beta_b p with the step's scalars from lz
+template <class TP>
+__global__ void k_lz_axpy(const TP* __restrict__ x, TP* __restrict__ y, int64_t n, int64_t split,
+                          const double* __restrict__ lz) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] -= (TP)lz[6 + (i < split ? 0 : 1)] * x[i];
+}
+template <class TP>
+__global__ void k_lz_xpby(const TP* __restrict__ z, TP* __restrict__ p, int64_t n, int64_t split,
+                          const double* __restrict__ lz) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + (TP)lz[8 + (i < split ? 0 : 1)] * p[i];
+}
+
 // y += a_b x, a_0 on [0, split) and a_1 on [split, n)
 template <class TP>
 __global__ void k_block_axpy(const TP* __restrict__ x, TP* __restrict__ y, int64_t n, int64_t split, double a0,
@@ -841,10 +927,18 @@ struct Hx {
   DBuf<float> P32;              // exactly symmetric FP32 copy: the FP32 Chebyshev steps' P
   DBuf<int32_t> P_tpos;         // entry of P's transpose, per entry
   DBuf<int16_t> P_col16;        // column offsets from the row, when all fit 16 bits
-  DBuf<double> coef, pvec, part;
+  DBuf<double> coef, pvec, part, lz;
   DBuf<double> res64, d64, zc64, w64;
   DBuf<float> res32, d32, zc32, w32;
   int64_t bytes32 = 0, launches = 0;
+  // the spectrum estimate's launches, captured once and replayed per refresh
+  hipGraph_t lz_graph = nullptr;
+  hipGraphExec_t lz_exec = nullptr;
+  std::vector<std::shared_ptr<void>> lz_keep;
+  ~Hx() {
+    if (lz_exec) (void)hipGraphExecDestroy(lz_exec);
+    if (lz_graph) (void)hipGraphDestroy(lz_graph);
+  }
 };
 
 void hx_free(Hx* h) { delete h; }
@@ -1494,6 +1588,7 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
     for (DBuf<float>* b : {&h->res32, &h->d32, &h->zc32, &h->w32}) b->alloc(ctx, nn);
     h->coef.alloc(ctx, 2 + 3 * (h->cheb_m - 1));
     h->part.alloc(ctx, 4 * (size_t)NBLK);
+    h->lz.alloc(ctx, 14 + 4 * kMaxLanczos);
     try {
       hx_spectrum(*h);
     } catch (const Error& e) {
@@ -1806,99 +1901,75 @@ double tridiagonal_lmax(const std::vector<double>& diag, const std::vector<doubl
 
 // kLanczosSteps of B-preconditioned CG on P per block in precision TP; the
 // largest eigenvalue of each block's Lanczos tridiagonal into h.lam.
+// The estimate's launches: start vector, B r, then kLanczosSteps steps; every
+// scalar stays on the device (lz), so the whole sequence is one graph.
 template <class TP, int D>
-void lanczos(Hx& h) {
+void lanczos_enqueue(Enq& q, Hx& h) {
   using X = Sel<TP>;
-  Ctx* ctx = h.ctx;
-  hipStream_t st = ctx->stream;
   const int64_t n = h.n, split = h.split;
   TP* r = X::res(h);
   TP* pd = X::d(h);
   TP* w = X::w(h);
   TP* z = X::zt(h);
-  {
+  const int64_t lo[2] = {0, split}, hi[2] = {split, n};
+  const int g[2] = {split > 0 ? (int)red_grid(hi[0] - lo[0]) : 0, n > split ? (int)red_grid(hi[1] - lo[1]) : 0};
+  auto dots = [&](const TP* x, const TP* y) {
+    for (int b = 0; b < 2; ++b)
+      if (g[b]) q(k_dotr<TP>, (unsigned)g[b], TPB, x, y, lo[b], hi[b], h.part.p + b * NBLK);
+  };
+  q(k_convert<double, TP>, grid_for(n), TPB, (const double*)h.pvec.p, r, n);
+  hx_core_d<TP, TP, D>(q, h, (const TP*)r, nullptr);
+  q(k_convert<TP, TP>, grid_for(n), TPB, (const TP*)z, pd, n);
+  dots(r, z);
+  q(k_lz_init, 1, 64, (const double*)h.part.p, g[0], g[1], h.lz.p);
+  for (int j = 0; j < kLanczosSteps; ++j) {
+    pc_spmv<TP>(q, h, (const TP*)pd, w, nullptr);
+    dots(pd, w);
+    q(k_lz_alpha, 1, 64, (const double*)h.part.p, g[0], g[1], h.lz.p);
+    q(k_lz_axpy<TP>, grid_for(n), TPB, (const TP*)w, r, n, split, (const double*)h.lz.p);
+    hx_core_d<TP, TP, D>(q, h, (const TP*)r, nullptr);
+    dots(r, z);
+    q(k_lz_beta, 1, 64, (const double*)h.part.p, g[0], g[1], kLanczosStop<TP>, h.lz.p);
+    q(k_lz_xpby<TP>, grid_for(n), TPB, (const TP*)z, pd, n, split, (const double*)h.lz.p);
+  }
+}
+
+template <class TP, int D>
+void lanczos(Hx& h) {
+  static_assert(kLanczosSteps <= kMaxLanczos, "Lanczos history size");
+  Ctx* ctx = h.ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = h.n, split = h.split;
+  if (!h.lz_exec) {
+    // the start vector, 0.5 + U[0,1) from xorshift64 (13, 7, 17), the same every time
     std::vector<double> x(n);
     uint64_t v = 0x2545F4914F6CDD1DULL;
-    for (int64_t i = 0; i < n; ++i) {  // 0.5 + U[0,1) from xorshift64 (13, 7, 17)
+    for (int64_t i = 0; i < n; ++i) {
       v ^= v << 13;
       v ^= v >> 7;
       v ^= v << 17;
       x[i] = 0.5 + (double)(v >> 11) * (1.0 / 9007199254740992.0);
     }
     copy_to_device(ctx, h.pvec.p, x.data(), n * 8);
-    if constexpr (std::is_same_v<TP, float>)
-      k_convert<double, float><<<grid_for(n), TPB, 0, st>>>(h.pvec.p, r, n);
-    else
-      FEEC_HIP(hipMemcpyAsync(r, h.pvec.p, n * 8, hipMemcpyDeviceToDevice, st));
+    FEEC_HIP(hipGraphCreate(&h.lz_graph, 0));
+    Enq q{st, h.lz_graph, &h.lz_keep, nullptr};
+    lanczos_enqueue<TP, D>(q, h);
+    FEEC_HIP(hipGraphInstantiate(&h.lz_exec, h.lz_graph, nullptr, nullptr, 0));
   }
-  Enq q{st};
-  const int64_t lo[2] = {0, split}, hi[2] = {split, n};
+  FEEC_HIP(hipGraphLaunch(h.lz_exec, st));
+  std::vector<double> lz(14 + 4 * kMaxLanczos);
+  copy_to_host(ctx, lz.data(), h.lz.p, lz.size() * 8);
   const bool has[2] = {split > 0, n > split};
-  std::vector<double> part(2 * (size_t)NBLK);
-  auto dots = [&](const TP* x, const TP* y, double out[2]) {
-    unsigned g[2];
-    for (int b = 0; b < 2; ++b) {
-      g[b] = red_grid(hi[b] - lo[b]);
-      if (has[b]) k_dotr<TP><<<g[b], TPB, 0, st>>>(x, y, lo[b], hi[b], h.part.p + b * NBLK);
-    }
-    FEEC_HIP(hipGetLastError());
-    copy_to_host(ctx, part.data(), h.part.p, part.size() * 8);
-    for (int b = 0; b < 2; ++b) {
-      out[b] = 0.0;
-      if (has[b])
-        for (unsigned i = 0; i < g[b]; ++i) out[b] += part[b * NBLK + i];
-    }
-  };
-  hx_core_d<TP, TP, D>(q, h, (const TP*)r, nullptr);
-  FEEC_HIP(hipMemcpyAsync(pd, z, n * sizeof(TP), hipMemcpyDeviceToDevice, st));
-  double rho[2], rho0[2], pw[2], rn[2];
-  dots(r, z, rho);
-  bool live[2];
-  for (int b = 0; b < 2; ++b) {
-    rho0[b] = rho[b];
-    live[b] = has[b] && rho[b] > 0.0 && std::isfinite(rho[b]);
-    FEEC_REQUIRE(live[b] || !has[b], FEEC_E_NOT_SPD, "multigrid preconditioner is not positive");
-  }
-  std::vector<double> al[2], be[2];
-  for (int j = 0; j < kLanczosSteps && (live[0] || live[1]); ++j) {
-    pc_spmv<TP>(q, h, (const TP*)pd, w, nullptr);
-    dots(pd, w, pw);
-    double a[2] = {0.0, 0.0};
-    for (int b = 0; b < 2; ++b) {
-      if (!live[b]) continue;
-      if (!(pw[b] > 0.0) || !std::isfinite(pw[b])) {  // the block's CG has run into roundoff
-        live[b] = false;
-        continue;
-      }
-      a[b] = rho[b] / pw[b];
-      al[b].push_back(a[b]);
-    }
-    k_block_axpy<TP><<<grid_for(n), TPB, 0, st>>>(w, r, n, split, -a[0], -a[1]);
-    FEEC_HIP(hipGetLastError());
-    hx_core_d<TP, TP, D>(q, h, (const TP*)r, nullptr);
-    dots(r, z, rn);
-    double bt[2] = {0.0, 0.0};
-    for (int b = 0; b < 2; ++b) {
-      if (!live[b]) continue;
-      if (!(rn[b] > kLanczosStop<TP> * rho0[b]) || !std::isfinite(rn[b])) {  // converged: stop this block
-        live[b] = false;
-        continue;
-      }
-      bt[b] = rn[b] / rho[b];
-      be[b].push_back(bt[b]);
-      rho[b] = rn[b];
-    }
-    k_block_xpby<TP><<<grid_for(n), TPB, 0, st>>>(z, pd, n, split, bt[0], bt[1]);
-    FEEC_HIP(hipGetLastError());
-  }
   for (int b = 0; b < 2; ++b) {
     h.lam[b] = 0.0;
     if (!has[b]) continue;
-    const int m = (int)al[b].size();
+    const int m = (int)lz[10 + b];
     FEEC_REQUIRE(m > 0, FEEC_E_NOT_SPD, "multigrid preconditioner spectrum estimate failed");
+    auto al = [&](int j) { return lz[14 + 2 * j + b]; };
+    auto be = [&](int j) { return lz[14 + 2 * kMaxLanczos + 2 * j + b]; };
     std::vector<double> diag(m), off(std::max(m - 1, 0));
-    for (int j = 0; j < m; ++j) diag[j] = 1.0 / al[b][j] + (j ? be[b][j - 1] / al[b][j - 1] : 0.0);
-    for (int j = 0; j + 1 < m; ++j) off[j] = std::sqrt(be[b][j]) / al[b][j];
+    for (int j = 0; j < m; ++j) diag[j] = 1.0 / al(j) + (j ? be(j - 1) / al(j - 1) : 0.0);
+    for (int j = 0; j + 1 < m; ++j) off[j] = std::sqrt(be(j)) / al(j);
     h.lam[b] = tridiagonal_lmax(diag, off);
   }
 }
