@@ -487,7 +487,7 @@ void problem_multigrid(Problem& P) {
   const int m = P.opts.cheb_degree < 0 ? kMultigridChebDefault : std::max(1, P.opts.cheb_degree);
   const double frac = P.opts.cheb_lower > 0.0 ? P.opts.cheb_lower : kMultigridChebFraction;
   P.hx = hx_create(*P.g, P.bc, P.k, P.sys, P.opts.mg_sweeps, m, frac, P.opts.precision != FEEC_PREC_FP64,
-                   &why);
+                   P.opts.use_graph != 0, &why);
   FEEC_REQUIRE(P.hx || pc != FEEC_PC_MULTIGRID, FEEC_E_UNSUPPORTED,
                "multigrid preconditioner unavailable: " + why);
   P.sys.hx = P.hx;

@@ -386,11 +386,12 @@ int kuhn_grid_n(Mesh& m);
 // The HX block preconditioner diag(B_{k-1}, B_k, I) of a KKT system of grade k
 // (values from s's preconditioner blocks), each block accelerated by a
 // Chebyshev polynomial in B P with cheb_m applications of B (1: plain B) on
-// [frac b, b] (lambda_max estimated with the FP32 cycle when spectrum_fp32);
+// [frac b, b] (lambda_max estimated with the FP32 cycle when spectrum_fp32,
+// its launches replayed from a graph when use_graph);
 // nullptr, with the reason, when it does not apply (no Kuhn
 // grid, no coordinates, non-Euclidean, k > 2).
 Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m, double frac,
-              bool spectrum_fp32, std::string* why);
+              bool spectrum_fp32, bool use_graph, std::string* why);
 void hx_info(const Hx& h, int* cheb_m, double lam[2]);
 // Seconds per launch: finest-level sweep, block preconditioner application
 // (FP64_PC32), Chebyshev product by P, one B application (FP32).

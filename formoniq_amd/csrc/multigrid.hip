@@ -932,6 +932,8 @@ struct Hx {
   DBuf<float> res32, d32, zc32, w32;
   int64_t bytes32 = 0, launches = 0;
   // the spectrum estimate's launches, captured once and replayed per refresh
+  bool use_graph = true;         // replay the estimate from a graph (else launch one by one)
+  bool lz_direct_ready = false;  // the start vector is on the device
   hipGraph_t lz_graph = nullptr;
   hipGraphExec_t lz_exec = nullptr;
   std::vector<std::shared_ptr<void>> lz_keep;
@@ -1314,7 +1316,7 @@ void build_pc_single(Hx& h, const KktSystem& s) {
 }  // namespace
 
 Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m, double frac,
-              bool spectrum_fp32, std::string* why) {
+              bool spectrum_fp32, bool use_graph, std::string* why) {
   Mesh& m = *g.mesh;
   Ctx* ctx = m.ctx;
   auto no = [&](const std::string& msg) -> Hx* {
@@ -1343,6 +1345,7 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
   h->cheb_m = std::max(1, cheb_m);
   h->frac = frac;
   h->spectrum_fp32 = spectrum_fp32;
+  h->use_graph = use_graph;
   if (h->cheb_m > 1 && s.P.nnz == 0) return no("no preconditioner matrix for the Chebyshev steps");
   if (!(frac > 0.0 && frac < 1.0)) return no("the Chebyshev interval fraction must lie in (0, 1)");
   const bool rel = bc == FEEC_RELATIVE;
@@ -1940,7 +1943,7 @@ void lanczos(Hx& h) {
   Ctx* ctx = h.ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = h.n, split = h.split;
-  if (!h.lz_exec) {
+  if (!h.lz_exec && !h.lz_direct_ready) {
     // the start vector, 0.5 + U[0,1) from xorshift64 (13, 7, 17), the same every time
     std::vector<double> x(n);
     uint64_t v = 0x2545F4914F6CDD1DULL;
@@ -1951,12 +1954,20 @@ void lanczos(Hx& h) {
       x[i] = 0.5 + (double)(v >> 11) * (1.0 / 9007199254740992.0);
     }
     copy_to_device(ctx, h.pvec.p, x.data(), n * 8);
-    FEEC_HIP(hipGraphCreate(&h.lz_graph, 0));
-    Enq q{st, h.lz_graph, &h.lz_keep, nullptr};
-    lanczos_enqueue<TP, D>(q, h);
-    FEEC_HIP(hipGraphInstantiate(&h.lz_exec, h.lz_graph, nullptr, nullptr, 0));
+    if (h.use_graph) {
+      FEEC_HIP(hipGraphCreate(&h.lz_graph, 0));
+      Enq q{st, h.lz_graph, &h.lz_keep, nullptr};
+      lanczos_enqueue<TP, D>(q, h);
+      FEEC_HIP(hipGraphInstantiate(&h.lz_exec, h.lz_graph, nullptr, nullptr, 0));
+    }
+    h.lz_direct_ready = true;
   }
-  FEEC_HIP(hipGraphLaunch(h.lz_exec, st));
+  if (h.lz_exec) {
+    FEEC_HIP(hipGraphLaunch(h.lz_exec, st));
+  } else {  // the same launches one by one (for tracing tools)
+    Enq q{st};
+    lanczos_enqueue<TP, D>(q, h);
+  }
   std::vector<double> lz(14 + 4 * kMaxLanczos);
   copy_to_host(ctx, lz.data(), h.lz.p, lz.size() * 8);
   const bool has[2] = {split > 0, n > split};
