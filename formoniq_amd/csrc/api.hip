@@ -641,7 +641,7 @@ extern "C" {
 const char* feec_last_error(void) { return g_last_error.c_str(); }
 
 const char* feec_status_string(feec_status s) {
-  switch (s) {
+  switch (abi_int(s)) {
     case FEEC_OK: return "ok";
     case FEEC_E_ARG: return "invalid argument";
     case FEEC_E_DEVICE: return "device error";
@@ -927,6 +927,8 @@ feec_status feec_mesh_orientation(feec_mesh* m, int8_t* signs) {
 
 feec_status feec_mesh_coboundary(feec_mesh* m, feec_bc bc, int k, feec_csr** out) {
   return guard([&] {
+    FEEC_REQUIRE(m && out, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
     FEEC_HIP(hipSetDevice(m->ctx->device));
     auto* a = new feec_csr_s();
     a->ctx = m->ctx;
@@ -972,6 +974,8 @@ feec_status feec_geom_from_coords(feec_mesh* m, const double* coords, feec_ambie
                                   feec_geom** out) {
   return guard([&] {
     FEEC_REQUIRE(m && coords && out, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(abi_int(amb) == FEEC_EUCLIDEAN || abi_int(amb) == FEEC_MINKOWSKI, FEEC_E_ARG,
+                 "ambient must be a feec_ambient value");
     FEEC_HIP(hipSetDevice(m->ctx->device));
     auto* g = new feec_geom_s();
     g->mesh = m;
@@ -1019,7 +1023,9 @@ feec_status feec_causal_census(feec_geom* g, int64_t counts[3], uint8_t* riem) {
 // ---- element matrices and assembly ---------------------------------------------
 feec_status feec_element_shape(int dim, feec_form_kind kind, int grade, int* rows, int* cols) {
   return guard([&] {
-    FEEC_REQUIRE(kind >= FEEC_MASS && kind <= FEEC_DIF_BOTH, FEEC_E_ARG, "not an element form");
+    FEEC_REQUIRE(rows && cols, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(abi_int(kind) >= FEEC_MASS && abi_int(kind) <= FEEC_DIF_BOTH, FEEC_E_ARG,
+                 "not an element form");
     *rows = (grade < 0 || grade > dim) ? 0 : element_rows(dim, kind, grade);
     *cols = (grade < 0 || grade > dim) ? 0 : element_cols(dim, kind, grade);
   });
@@ -1027,8 +1033,10 @@ feec_status feec_element_shape(int dim, feec_form_kind kind, int grade, int* row
 
 feec_status feec_element_matrices(feec_geom* g, feec_form_kind kind, int grade, double* out) {
   return guard([&] {
+    FEEC_REQUIRE(g && out, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(abi_int(kind) >= FEEC_MASS && abi_int(kind) <= FEEC_DIF_BOTH, FEEC_E_ARG,
+                 "not an element form");
     Mesh& m = *g->mesh;
-    FEEC_REQUIRE(kind >= FEEC_MASS && kind <= FEEC_DIF_BOTH, FEEC_E_ARG, "not an element form");
     FEEC_REQUIRE(grade >= 0 && grade <= m.d, FEEC_E_ARG, "grade out of range");
     int r = element_rows(m.d, kind, grade), c = element_cols(m.d, kind, grade);
     int64_t nc = m.n[m.d], total = nc * r * c;
@@ -1046,7 +1054,9 @@ feec_status feec_assemble(feec_geom* g, feec_bc bc, feec_form_kind kind, int gra
                           feec_csr** out) {
   return guard([&] {
     FEEC_REQUIRE(g && out, FEEC_E_ARG, "null argument");
-    FEEC_REQUIRE(kind >= FEEC_MASS && kind <= FEEC_HDIF_GRAM, FEEC_E_ARG, "bad form kind");
+    FEEC_REQUIRE(abi_int(kind) >= FEEC_MASS && abi_int(kind) <= FEEC_HDIF_GRAM, FEEC_E_ARG,
+                 "bad form kind");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
     FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
     auto* a = new feec_csr_s();
     try {
@@ -1210,6 +1220,8 @@ feec_status feec_csr_cg(feec_csr* a, const double* b, double* x, double rtol, in
 // ---- problems ---------------------------------------------------------------------
 feec_status feec_harmonics(feec_geom* g, feec_bc bc, int grade, int64_t* count, double* basis) {
   return guard([&] {
+    FEEC_REQUIRE(g, FEEC_E_ARG, "null geometry");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
     std::vector<double> hb;
     int64_t c = 0;
     harmonic_basis(*g, bc, grade, hb, c);
@@ -1220,6 +1232,8 @@ feec_status feec_harmonics(feec_geom* g, feec_bc bc, int grade, int64_t* count, 
 
 feec_status feec_mixed_kkt(feec_geom* g, feec_bc bc, int grade, feec_csr** out, int64_t sizes[3]) {
   return guard([&] {
+    FEEC_REQUIRE(g && out, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
     Problem P;
     P.g = g;
     P.bc = bc;
@@ -1325,6 +1339,7 @@ feec_status feec_problem_create(feec_geom* g, feec_bc bc, int grade, const feec_
                                 feec_problem** out) {
   return guard([&] {
     FEEC_REQUIRE(g && out, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
     FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
     auto* P = new feec_problem_s();
     P->g = g;

@@ -170,6 +170,7 @@ feec_status feec_hodge_dirac_assemble(feec_geom* g, feec_bc bc, double sign, fee
                                       feec_csr** op, int64_t* offsets) {
   return guard([&] {
     FEEC_REQUIRE(g, FEEC_E_ARG, "null geometry");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
     FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
     Dirac D;
     dirac_assemble(*g, bc, sign, D);
@@ -453,7 +454,7 @@ feec_status feec_dirac_irk(feec_geom* g, feec_bc bc, int stages, double dt, int6
                            double* y, double* states, double* energy, feec_report* report) {
   return guard([&] {
     FEEC_REQUIRE(g && y, FEEC_E_ARG, "null argument");
-    FEEC_REQUIRE(bc == FEEC_FULL || bc == FEEC_RELATIVE, FEEC_E_ARG, "bc must be a feec_bc value");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
     FEEC_REQUIRE(stages == 1 || stages == 2, FEEC_E_ARG, "stages must be 1 or 2");
     FEEC_REQUIRE(nsteps >= 0 && std::isfinite(dt), FEEC_E_ARG, "nsteps >= 0 and a finite dt");
     FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
@@ -465,7 +466,7 @@ feec_status feec_dirac_leapfrog(feec_geom* g, feec_bc bc, double dt, int64_t nst
                                 double* states, double* energy, feec_report* report) {
   return guard([&] {
     FEEC_REQUIRE(g && y, FEEC_E_ARG, "null argument");
-    FEEC_REQUIRE(bc == FEEC_FULL || bc == FEEC_RELATIVE, FEEC_E_ARG, "bc must be a feec_bc value");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
     FEEC_REQUIRE(nsteps >= 0 && std::isfinite(dt), FEEC_E_ARG, "nsteps >= 0 and a finite dt");
     FEEC_HIP(hipSetDevice(g->mesh->ctx->device));
     dirac_leapfrog(*g, bc, dt, nsteps, y, states, energy, report, 1e-14);

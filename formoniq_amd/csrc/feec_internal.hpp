@@ -40,6 +40,23 @@ void set_last_error(const std::string& msg);
     }                                                                                       \
   } while (0)
 
+// Enumerations cross the C ABI as ints, so a caller can pass any value, and in
+// C++ reading an enum object that holds a value outside its declared range is
+// undefined. Entry points therefore read such an argument's bits (no enum
+// load) and check them against the declared values before anything else
+// reads the argument.
+template <class E>
+inline int abi_int(const E& e) {
+  static_assert(sizeof(E) == sizeof(int), "C enumerations are int-sized");
+  int v;
+  std::memcpy(&v, &e, sizeof v);
+  return v;
+}
+inline bool abi_bc_ok(const feec_bc& b) {
+  const int v = abi_int(b);
+  return v == FEEC_FULL || v == FEEC_RELATIVE;
+}
+
 #define FEEC_REQUIRE(cond, status, msg) \
   do {                                      \
     if (!(cond)) ::feec::fail(status, msg); \
@@ -396,6 +413,14 @@ void hx_info(const Hx& h, int* cheb_m, double lam[2]);
 // Seconds per launch: finest-level sweep, block preconditioner application
 // (FP64_PC32), Chebyshev product by P, one B application (FP32).
 void hx_kernel_times(Hx& h, int reps, bool use_graph, double out[4]);
+
+// Host numerical helpers of the multigrid set-up.
+// a (n x n, row-major, SPD) is replaced by its inverse, by Cholesky; false if
+// a pivot is not positive.
+bool spd_inverse(int64_t n, std::vector<double>& a);
+// Largest eigenvalue of the symmetric tridiagonal matrix with diagonal diag
+// and off-diagonal off (size n-1), by Sturm-count bisection.
+double tridiagonal_lmax(const std::vector<double>& diag, const std::vector<double>& off);
 // New values after re-assembly (same pattern).
 void hx_refresh(Hx& h, Geom& g, const KktSystem& s);
 void hx_free(Hx* h);

@@ -776,6 +776,8 @@ int64_t ipow(int64_t b, int e) {
   return r;
 }
 
+}  // namespace
+
 // Dense inverse of an SPD matrix by Cholesky; false if it is not SPD.
 bool spd_inverse(int64_t n, std::vector<double>& a) {
   std::vector<double> L(n * n, 0.0);
@@ -808,8 +810,6 @@ bool spd_inverse(int64_t n, std::vector<double>& a) {
   a.swap(inv);
   return true;
 }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // Kuhn grid detection
@@ -1644,6 +1644,33 @@ void hx_refresh(Hx& h, Geom& g, const KktSystem& s) {
   h.ctx->sync();
 }
 
+// Largest eigenvalue of the symmetric tridiagonal (diag, off): bisection on
+// Sturm counts inside the Gershgorin bound.
+double tridiagonal_lmax(const std::vector<double>& diag, const std::vector<double>& off) {
+  const int n = (int)diag.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i ? std::fabs(off[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(off[i]) : 0.0);
+    lo = std::min(lo, diag[i] - r);
+    hi = std::max(hi, diag[i] + r);
+  }
+  for (int it = 0; it < 60; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    int count = 0;
+    double q = 1.0;
+    for (int i = 0; i < n; ++i) {  // eigenvalues below mid = negative pivots of T - mid I
+      q = diag[i] - mid - (i ? off[i - 1] * off[i - 1] / q : 0.0);
+      if (q == 0.0) q = -1e-300;
+      count += q < 0.0;
+    }
+    if (count == n)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return hi;
+}
+
 namespace {
 
 template <class TP>
@@ -1875,32 +1902,6 @@ template <class TP>
 constexpr double kLanczosStop = std::is_same_v<TP, float> ? 1e-10 : 1e-20;
 constexpr double kLmaxMargin = 1.05;
 
-// Largest eigenvalue of the symmetric tridiagonal (diag, off): bisection on
-// Sturm counts inside the Gershgorin bound.
-double tridiagonal_lmax(const std::vector<double>& diag, const std::vector<double>& off) {
-  const int n = (int)diag.size();
-  double lo = 1e300, hi = -1e300;
-  for (int i = 0; i < n; ++i) {
-    const double r = (i ? std::fabs(off[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(off[i]) : 0.0);
-    lo = std::min(lo, diag[i] - r);
-    hi = std::max(hi, diag[i] + r);
-  }
-  for (int it = 0; it < 60; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    int count = 0;
-    double q = 1.0;
-    for (int i = 0; i < n; ++i) {  // eigenvalues below mid = negative pivots of T - mid I
-      q = diag[i] - mid - (i ? off[i - 1] * off[i - 1] / q : 0.0);
-      if (q == 0.0) q = -1e-300;
-      count += q < 0.0;
-    }
-    if (count == n)
-      hi = mid;
-    else
-      lo = mid;
-  }
-  return hi;
-}
 
 // kLanczosSteps of B-preconditioned CG on P per block in precision TP; the
 // largest eigenvalue of each block's Lanczos tridiagonal into h.lam.

@@ -1,0 +1,131 @@
+// Exercises the host-only code of libfeec (the meshers, option defaults,
+// status strings, element shapes, argument checks, and the multigrid set-up's
+// dense Cholesky inverse and tridiagonal eigenvalue bisection) in a build
+// instrumented by AddressSanitizer and UndefinedBehaviorSanitizer. No GPU is
+// touched. Exit status 0 and the line "host_check ok" mean every check held.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "feec.h"
+#include "../../formoniq_amd/csrc/feec_internal.hpp"
+
+static int failures = 0;
+#define EXPECT(c)                                                     \
+  do {                                                                \
+    if (!(c)) {                                                       \
+      std::fprintf(stderr, "FAIL %s:%d: %s\n", __FILE__, __LINE__, #c); \
+      ++failures;                                                     \
+    }                                                                 \
+  } while (0)
+
+static void meshers() {
+  const int sizes[5] = {0, 7, 5, 3, 2};
+  for (int d = 1; d <= 4; ++d) {
+    int64_t nc = 0, nv = 0;
+    EXPECT(feec_kuhn_counts(d, sizes[d], &nc, &nv) == FEEC_OK);
+    std::vector<int64_t> cells((size_t)nc * (d + 1));
+    std::vector<double> coords((size_t)nv * d);
+    EXPECT(feec_kuhn_grid(d, sizes[d], cells.data(), coords.data()) == FEEC_OK);
+    bool in_range = true;
+    for (int64_t v : cells) in_range &= v >= 0 && v < nv;
+    EXPECT(in_range);
+    // each cell's vertices step along distinct axes: a Kuhn simplex
+    EXPECT(feec_jitter_interior(d, sizes[d], coords.data(), 0.2, 7, 0xFu) == FEEC_OK);
+    for (double c : coords) EXPECT(c >= -0.25 && c <= 1.25);
+    EXPECT(feec_kuhn_grid(d, sizes[d], nullptr, coords.data()) == FEEC_OK);
+    EXPECT(feec_kuhn_grid(d, sizes[d], cells.data(), nullptr) == FEEC_OK);
+  }
+  int64_t nc = 0, nv = 0;
+  EXPECT(feec_kuhn_counts(5, 2, &nc, &nv) == FEEC_E_ARG);
+  EXPECT(std::strlen(feec_last_error()) > 0);
+  EXPECT(feec_kuhn_counts(2, 0, &nc, &nv) == FEEC_E_ARG);
+  EXPECT(feec_jitter_interior(2, 3, nullptr, 0.1, 1, 3u) == FEEC_E_ARG);
+  for (int64_t n : {0, 1, 2, 7, 1000}) {
+    std::vector<double> x((size_t)n + 1, 12345.0);
+    EXPECT(feec_normal_vector(n, 42, n ? x.data() : nullptr) == FEEC_OK);
+    for (int64_t i = 0; i < n; ++i) EXPECT(std::isfinite(x[i]));
+    EXPECT(x[n] == 12345.0);  // nothing written past n
+  }
+  EXPECT(feec_normal_vector(-1, 0, nullptr) == FEEC_E_ARG);
+}
+
+static void options_and_strings() {
+  feec_solve_opts o;
+  std::memset(&o, 0xA5, sizeof(o));
+  feec_solve_opts_default(&o);
+  EXPECT(o.rtol > 0.0 && o.rtol < 1.0);
+  EXPECT(o.precision == FEEC_PREC_FP64_PC32);
+  EXPECT(o.preconditioner == FEEC_PC_AUTO);
+  // values outside the enumerations, as a C or Rust caller may pass them
+  for (int s : {0, 1, 5, 10, 11, 99, -3}) {
+    const char* t = feec_status_string((feec_status)s);
+    EXPECT(t != nullptr && std::strlen(t) > 0);
+  }
+  EXPECT(std::strcmp(feec_status_string((feec_status)99), "unknown") == 0);
+  for (int d = 0; d <= 5; ++d)
+    for (int kind = FEEC_MASS; kind <= FEEC_DIF_BOTH; ++kind)
+      for (int k = -1; k <= d + 1; ++k) {
+        int r = -1, c = -1;
+        EXPECT(feec_element_shape(d, (feec_form_kind)kind, k, &r, &c) == FEEC_OK);
+        EXPECT(r >= 0 && c >= 0);
+      }
+  int r, c;
+  EXPECT(feec_element_shape(3, (feec_form_kind)99, 1, &r, &c) == FEEC_E_ARG);
+  EXPECT(feec_element_shape(3, (feec_form_kind)-1, 1, &r, &c) == FEEC_E_ARG);
+  EXPECT(feec_element_shape(3, FEEC_MASS, 1, nullptr, &c) == FEEC_E_ARG);
+  EXPECT(feec_assemble(nullptr, (feec_bc)7, FEEC_MASS, 1, nullptr) == FEEC_E_ARG);
+}
+
+static void dense_inverse() {
+  // SPD: A = L L^T + n I from a deterministic L; check A * inv(A) = I
+  for (int n : {1, 2, 5, 17, 64}) {
+    std::vector<double> a((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        double s = i == j ? n : 0.0;
+        for (int k = 0; k <= std::min(i, j); ++k) s += std::sin(1.0 + i + 3.0 * k) * std::sin(1.0 + j + 3.0 * k);
+        a[(size_t)i * n + j] = s;
+      }
+    std::vector<double> inv = a;
+    EXPECT(feec::spd_inverse(n, inv));
+    double err = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s += a[(size_t)i * n + k] * inv[(size_t)k * n + j];
+        err = std::max(err, std::fabs(s - (i == j ? 1.0 : 0.0)));
+      }
+    EXPECT(err < 1e-10);
+  }
+  std::vector<double> indef = {1.0, 2.0, 2.0, 1.0};  // eigenvalues 3, -1
+  EXPECT(!feec::spd_inverse(2, indef));
+  std::vector<double> empty;
+  EXPECT(feec::spd_inverse(0, empty));
+}
+
+static void tridiagonal() {
+  // 1D Laplacian tridiag(-1, 2, -1) of size n: largest eigenvalue 2 - 2 cos(n pi / (n+1))
+  for (int n : {1, 2, 3, 6, 16}) {
+    std::vector<double> dg(n, 2.0), off(n > 0 ? n - 1 : 0, -1.0);
+    const double want = 2.0 - 2.0 * std::cos(n * M_PI / (n + 1));
+    EXPECT(std::fabs(feec::tridiagonal_lmax(dg, off) - want) < 1e-12 * want);
+  }
+  // a diagonal matrix with a zero off-diagonal: the largest diagonal entry
+  std::vector<double> dg = {0.5, 3.0, 1.0}, off = {0.0, 0.0};
+  EXPECT(std::fabs(feec::tridiagonal_lmax(dg, off) - 3.0) < 1e-12);
+}
+
+int main() {
+  meshers();
+  options_and_strings();
+  dense_inverse();
+  tridiagonal();
+  if (failures) {
+    std::fprintf(stderr, "host_check: %d failures\n", failures);
+    return 1;
+  }
+  std::printf("host_check ok\n");
+  return 0;
+}
