@@ -19,6 +19,7 @@ library is not built.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -45,7 +46,9 @@ __all__ = [
     "normal_vector", "MASS", "DIF_TRIAL", "DIF_TEST", "DIF_BOTH", "HDIF_GRAM", "FULL",
     "RELATIVE", "EUCLIDEAN", "MINKOWSKI", "FP64", "MIXED", "FP64_PC32", "Report", "Request", "Result",
     "ServerOpts", "SolveOpts", "Server", "batch_run", "load_vector", "PC_AUTO", "PC_CHEBYSHEV",
-    "PC_MULTIGRID",
+    "PC_MULTIGRID", "FRAME_REFERENCE", "FRAME_AMBIENT", "SimplexQuadRule",
+    "cell_quadrature_points", "simplex_quadrature_points", "SourceForm", "source_load",
+    "l2_projection", "whitney_at_nodes", "fe_l2_error", "derham_map", "DeviceArray",
 ]
 
 
@@ -145,6 +148,219 @@ def load_vector(geometry: "MeshLengthsSq", grade, cochain):
     c = np.ascontiguousarray(cochain, dtype=np.float64)
     out = np.zeros_like(c)
     check(load().feec_load_vector(geometry._h, grade, _ptr(c, _lib.f64p), _ptr(out, _lib.f64p)))
+    return out
+
+
+# ---- analytic data: quadrature loads, L2 projection and error, de Rham map ------------
+# feec_frame: components in each cell's reference frame, or in the mesh's coordinates
+FRAME_REFERENCE, FRAME_AMBIENT = 0, 1
+
+
+def _frame(frame):
+    if frame in ("reference", FRAME_REFERENCE):
+        return FRAME_REFERENCE
+    if frame in ("ambient", FRAME_AMBIENT):
+        return FRAME_AMBIENT
+    raise ValueError(f"frame must be 'reference' or 'ambient', not {frame!r}")
+
+
+def _array_in(a, shape):
+    """(address, on_device, keepalive) of an input array of the given shape: a numpy
+    array (host) or an object with data_ptr() and is_cuda (a torch tensor on the
+    context's device). The library reads exactly that many doubles."""
+    if isinstance(a, DeviceArray):
+        if a.shape != tuple(shape):
+            raise ValueError(f"expected shape {tuple(shape)}, got {a.shape}")
+        return C.c_void_p(a.data_ptr()), 1, a
+    if hasattr(a, "data_ptr") and getattr(a, "is_cuda", False):
+        if not a.is_contiguous() or str(a.dtype) != "torch.float64":
+            raise ValueError("device arrays must be contiguous float64")
+        if tuple(a.shape) != tuple(shape):
+            raise ValueError(f"expected shape {tuple(shape)}, got {tuple(a.shape)}")
+        return C.c_void_p(a.data_ptr()), 1, a
+    h = np.ascontiguousarray(a, dtype=np.float64)
+    if h.shape != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {h.shape}")
+    return C.c_void_p(h.ctypes.data), 0, h
+
+
+def _cell_values_shape(geometry, grade, degree):
+    d = geometry.complex.dim
+    if not 0 <= grade <= d:
+        raise FeecError(1, "grade out of range")
+    return (geometry.complex.nsimplices(d), _npoints(d, degree), math.comb(d, grade))
+
+
+_HIP = None
+
+
+def _hip_runtime():
+    """The HIP runtime libfeec links (libamdhip64), for DeviceArray."""
+    global _HIP
+    if _HIP is None:
+        load()
+        h = C.CDLL("libamdhip64.so", mode=C.RTLD_GLOBAL)
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipFree.argtypes = [C.c_void_p]
+        _HIP = h
+    return _HIP
+
+
+class DeviceArray:
+    """A float64 array in device memory (hipMalloc on the current device), for the
+    on_device entry points; torch tensors on the device are accepted as well."""
+    is_cuda = True
+
+    def __init__(self, shape):
+        self.shape = tuple(int(s) for s in shape)
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self._p = C.c_void_p()
+        hip = _hip_runtime()
+        if hip.hipMalloc(C.byref(self._p), max(8 * self.size, 8)) != 0:
+            raise FeecError(3, "hipMalloc failed")
+
+    @staticmethod
+    def from_numpy(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = DeviceArray(a.shape)
+        if a.size and _hip_runtime().hipMemcpy(d._p, a.ctypes.data, a.nbytes, 1) != 0:
+            raise FeecError(2, "hipMemcpy failed")
+        return d
+
+    def numpy(self):
+        out = np.zeros(self.shape)
+        if out.size and _hip_runtime().hipMemcpy(out.ctypes.data, self._p, out.nbytes, 2) != 0:
+            raise FeecError(2, "hipMemcpy failed")
+        return out
+
+    def data_ptr(self):
+        return self._p.value or 0
+
+    def __del__(self):
+        if getattr(self, "_p", None) and self._p.value and _HIP is not None:
+            _HIP.hipFree(self._p)
+            self._p = C.c_void_p()
+
+
+def _array_out(like_device, shape):
+    """An output array of the given shape on the same side as the input."""
+    if like_device:
+        d = DeviceArray(shape)
+        return d, C.c_void_p(d.data_ptr())
+    h = np.zeros(shape)
+    return h, C.c_void_p(h.ctypes.data)
+
+
+class SimplexQuadRule:
+    """SimplexQuadRule::degree (simplicial/src/atlas/quadrature.rs:37-83): the
+    Grundmann-Moller rule exact for polynomials of the given degree on the
+    dim-simplex; `points` barycentric [np, dim+1], `weights` summing to one."""
+
+    def __init__(self, dim, degree):
+        n = C.c_int()
+        check(load().feec_quadrature_rule(dim, degree, C.byref(n), None, None))
+        self.dim, self.degree_ = dim, degree
+        self.points = np.zeros((n.value, dim + 1))
+        self.weights = np.zeros(n.value)
+        check(load().feec_quadrature_rule(dim, degree, C.byref(n), _ptr(self.points, _lib.f64p),
+                                          _ptr(self.weights, _lib.f64p)))
+
+    @staticmethod
+    def degree(dim, degree):
+        return SimplexQuadRule(dim, degree)
+
+    def npoints(self):
+        return len(self.weights)
+
+
+def _npoints(dim, degree):
+    n = C.c_int()
+    check(load().feec_quadrature_rule(dim, degree, C.byref(n), None, None))
+    return n.value
+
+
+def cell_quadrature_points(geometry: "MeshLengthsSq", degree, device=False):
+    """Positions of every cell's quadrature nodes [N_d, np, dim] (the MeshPoints an
+    integrand sees); needs a geometry made from coordinates."""
+    d, nc = geometry.complex.dim, geometry.complex.nsimplices(geometry.complex.dim)
+    out, addr = _array_out(device, (nc, _npoints(d, degree), d))
+    check(load().feec_cell_quadrature_points(geometry._h, degree, addr, int(device)))
+    return out
+
+
+def simplex_quadrature_points(geometry: "MeshLengthsSq", grade, degree, device=False):
+    """Positions of the quadrature nodes on every grade-k simplex [N_k, np_k, dim]."""
+    d, ns = geometry.complex.dim, geometry.complex.nsimplices(grade)
+    out, addr = _array_out(device, (ns, _npoints(grade, degree), d))
+    check(load().feec_simplex_quadrature_points(geometry._h, grade, degree, addr, int(device)))
+    return out
+
+
+def source_load(geometry: "MeshLengthsSq", grade, values, degree=1, frame="reference"):
+    """SourceForm(...).assemble (operators.rs:599-635, galerkin.rs:205-312): the load
+    vector of a k-form given at every cell's degree-`degree` nodes, values
+    [N_d, np, C(d, k)] (numpy, or a float64 torch tensor on the device)."""
+    addr, dev, keep = _array_in(values, _cell_values_shape(geometry, grade, degree))
+    out, oaddr = _array_out(dev, (geometry.complex.nsimplices(grade),))
+    check(load().feec_source_load(geometry._h, grade, degree, _frame(frame), addr, oaddr, dev))
+    return out
+
+
+class SourceForm:
+    """SourceForm::new(source, Some(qr)) (operators.rs:599-635) for a field given by
+    its values at the cell nodes of the degree-`degree` rule."""
+
+    def __init__(self, values, grade, degree=1, frame="reference"):
+        self.values, self.grade, self.degree, self.frame = values, grade, degree, frame
+
+    def assemble(self, geometry: "MeshLengthsSq"):
+        return source_load(geometry, self.grade, self.values, self.degree, self.frame)
+
+
+def l2_projection(geometry: "MeshLengthsSq", grade, values, degree=1, frame="reference"):
+    """l2_projection (fe.rs:87-123): (cochain, Report) of the mass solve M_k c = load."""
+    addr, dev, keep = _array_in(values, _cell_values_shape(geometry, grade, degree))
+    out, oaddr = _array_out(dev, (geometry.complex.nsimplices(grade),))
+    rep = Report()
+    check(load().feec_l2_projection(geometry._h, grade, degree, _frame(frame), addr, oaddr,
+                                    C.byref(rep), dev))
+    return out, rep
+
+
+def whitney_at_nodes(geometry: "MeshLengthsSq", grade, cochain, degree=3):
+    """WhitneyInterpolant::eval (interpolant.rs:55-62) at every cell's nodes, in the
+    cells' reference frames: [N_d, np, C(d, k)]."""
+    shape = _cell_values_shape(geometry, grade, degree)
+    addr, dev, keep = _array_in(cochain, (geometry.complex.nsimplices(grade),))
+    out, oaddr = _array_out(dev, shape)
+    check(load().feec_whitney_at_nodes(geometry._h, grade, degree, addr, oaddr, dev))
+    return out
+
+
+def fe_l2_error(cochain, exact, geometry: "MeshLengthsSq", grade, frame="reference"):
+    """fe_l2_error (fe.rs:50-74): |exact - W c|_(L^2) with exact given at the
+    degree-3 cell nodes."""
+    shape = _cell_values_shape(geometry, grade, 3)
+    caddr, cdev, ckeep = _array_in(cochain, (geometry.complex.nsimplices(grade),))
+    eaddr, edev, ekeep = _array_in(exact, shape)
+    if cdev != edev:
+        raise ValueError("cochain and exact values must both be host or both device arrays")
+    err = C.c_double()
+    check(load().feec_l2_error(geometry._h, grade, caddr, _frame(frame), eaddr, C.byref(err), cdev))
+    return err.value
+
+
+def derham_map(geometry: "MeshLengthsSq", grade, values, degree=1):
+    """derham_map (project.rs:55-96): the cochain of integrals of an ambient k-form
+    given at the degree-`degree` nodes of every k-simplex [N_k, np_k, C(d, k)]."""
+    d = geometry.complex.dim
+    if not 0 <= grade <= d:
+        raise FeecError(1, "grade out of range")
+    ns = geometry.complex.nsimplices(grade)
+    addr, dev, keep = _array_in(values, (ns, _npoints(grade, degree), math.comb(d, grade)))
+    out, oaddr = _array_out(dev, (ns,))
+    check(load().feec_derham_map(geometry._h, grade, degree, addr, oaddr, dev))
     return out
 
 

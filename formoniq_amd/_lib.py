@@ -122,6 +122,14 @@ SIGNATURES = {
     "feec_vec_dot_device": (S, [vp, C.c_int64, vp, vp, f64p]),
     "feec_vec_axpy_device": (S, [vp, C.c_int64, C.c_double, vp, vp]),
     "feec_load_vector": (S, [vp, C.c_int, f64p, f64p]),
+    "feec_quadrature_rule": (S, [C.c_int, C.c_int, C.POINTER(C.c_int), f64p, f64p]),
+    "feec_cell_quadrature_points": (S, [vp, C.c_int, vp, C.c_int]),
+    "feec_simplex_quadrature_points": (S, [vp, C.c_int, C.c_int, vp, C.c_int]),
+    "feec_source_load": (S, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int]),
+    "feec_l2_projection": (S, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.POINTER(Report), C.c_int]),
+    "feec_whitney_at_nodes": (S, [vp, C.c_int, C.c_int, vp, vp, C.c_int]),
+    "feec_l2_error": (S, [vp, C.c_int, vp, C.c_int, vp, f64p, C.c_int]),
+    "feec_derham_map": (S, [vp, C.c_int, C.c_int, vp, vp, C.c_int]),
     "feec_problem_solution": (S, [vp, f64p, f64p, f64p]),
     "feec_problem_sizes": (S, [vp, i64p, i64p]),
     "feec_problem_source_buffer": (S, [vp, C.POINTER(vp)]),

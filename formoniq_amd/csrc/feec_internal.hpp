@@ -234,6 +234,7 @@ struct Geom {
   // vertex coordinates [N_0][d] when the geometry came from coordinates (host
   // copy; the auxiliary-space preconditioner reads its frame off them)
   std::vector<double> coords;
+  DBuf<double> dcoords;  // device copy of coords, made on first use (under mesh->mu)
   bool euclidean = false;
 };
 

@@ -248,6 +248,53 @@ feec_status feec_csr_cg_noalloc(feec_csr *a, const double *b, double *x, double 
                                 int64_t max_iters, feec_report *report);
 void feec_csr_destroy(feec_csr *a);
 
+/* ---- analytic data: quadrature loads, L2 projection and error, de Rham map ------------
+ * A differential form reaches the library as its values at quadrature nodes, evaluated by
+ * the caller (on the host or on the device) at the positions that
+ * feec_cell_quadrature_points / feec_simplex_quadrature_points report. A k-form value is its
+ * C(dim,k) components on the colex basis dx^I. frame FEEC_FRAME_REFERENCE: components in each
+ * cell's reference frame, a Section's value (derham/src/section.rs:67); FEEC_FRAME_AMBIENT:
+ * components in the coordinates the geometry was made from (feec_geom_from_coords), pulled
+ * back cell by cell as Pullback::at does (section.rs:182-195). on_device = 1: every array
+ * argument is device memory of the geometry's context, else host memory. Full complex only.
+ * Arrays: cell nodes [N_dim][npoints][...], simplex nodes [N_k][npoints_k][...]. */
+typedef enum feec_frame { FEEC_FRAME_REFERENCE = 0, FEEC_FRAME_AMBIENT = 1 } feec_frame;
+
+/* SimplexQuadRule::degree (simplicial/src/atlas/quadrature.rs:37-83): the Grundmann-Moller
+ * rule of index degree/2 on the dim-simplex (0 <= dim <= 4, 0 <= degree <= 7), barycentric
+ * nodes [npoints][dim+1] and weights summing to one. bary and weights may be NULL. */
+feec_status feec_quadrature_rule(int dim, int degree, int *npoints, double *bary,
+                                 double *weights);
+/* The embedded position x = sum_i lambda_i x_(v_i) of every cell's nodes: [N_dim][np][dim]. */
+feec_status feec_cell_quadrature_points(feec_geom *geom, int degree, double *points,
+                                        int on_device);
+/* The same for the nodes of the rule on the k-simplices: [N_k][np_k][dim]. */
+feec_status feec_simplex_quadrature_points(feec_geom *geom, int grade, int degree,
+                                           double *points, int on_device);
+/* SourceForm (operators.rs:599-635) assembled into its load vector (galerkin.rs:205-312):
+ * load_sigma = sum_K vol_K sum_n w_n <f(x_n), W_sigma(x_n)>_(Lambda^k g_K^-1), N_k entries,
+ * summed over cells in cell order; field at the cell nodes of `degree`. */
+feec_status feec_source_load(feec_geom *geom, int grade, int degree, feec_frame frame,
+                             const double *field, double *load, int on_device);
+/* l2_projection (fe.rs:87-123): M_k c = load, by Jacobi CG to rtol 1e-12 when every cell is
+ * Riemannian, else by MINRES (the reference factorizes). report may be NULL. */
+feec_status feec_l2_projection(feec_geom *geom, int grade, int degree, feec_frame frame,
+                               const double *field, double *cochain, feec_report *report,
+                               int on_device);
+/* WhitneyInterpolant::eval (derham/src/interpolate/interpolant.rs:55-62): the reference-frame
+ * value of W c at every cell's nodes of `degree`, [N_dim][np][C(dim,k)]. */
+feec_status feec_whitney_at_nodes(feec_geom *geom, int grade, int degree, const double *cochain,
+                                  double *values, int on_device);
+/* fe_l2_error (fe.rs:50-74): |exact - W c|_(L^2 Lambda^k) by the degree-3 rule, measured in
+ * each cell's frame by Lambda^k g^-1; exact at the degree-3 cell nodes. *error is a host double. */
+feec_status feec_l2_error(feec_geom *geom, int grade, const double *cochain, feec_frame frame,
+                          const double *exact, double *error, int on_device);
+/* derham_map (derham/src/project.rs:55-96): c_s = integral over the k-simplex s of the form,
+ * the pairing with its tangent blade by the degree-`degree` rule on the simplex; field in
+ * ambient components at the simplex nodes [N_k][np_k][C(dim,k)]. */
+feec_status feec_derham_map(feec_geom *geom, int grade, int degree, const double *field,
+                            double *cochain, int on_device);
+
 /* ---- problems ------------------------------------------------------------ */
 /* b_k and an M_k-orthonormal harmonic basis [ndofs][count] (basis may be NULL).
  * Closed forms on a ball (full k=0: the constant; relative k=n: M_n^-1 times the

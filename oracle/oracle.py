@@ -43,6 +43,13 @@ MASS, DIF_TRIAL, DIF_TEST, DIF_BOTH = 0, 1, 2, 3
 #   31 the midpoint prolongation weights (1/2, 1/2) replaced by (3/4, 1/4)
 #   32 the post-smoothing sweeps of the V-cycle left out
 #   33 the vector-nodal factor 1/(k+1)! replaced by 1/k!
+# and in the analytic-data functions:
+#   41 the Grundmann-Moeller weight sign (-1)^i left off
+#   42 the k! of the Whitney values left off
+#   43 the load paired by the Euclidean product (Lambda^k g^-1 replaced by I)
+#   44 the ambient pullback by Lambda^k(J^T) (J transposed)
+#   45 the de Rham map's 1/k! (the unit simplex volume) left off
+#   46 the L2 error without the cell volume
 MUTATION = 0
 
 i64p = C.POINTER(C.c_int64)
@@ -1142,3 +1149,236 @@ def hx_block_preconditioner(c: WhitneyComplex, coords, k, n, nh, sweeps=2, cheb_
         parts += [cheb(hu, Au, lam[1], r[ns:ns + nu]), r[ns + nu:]]
         return np.concatenate(parts)
     return prec
+
+
+# ---------------------------------------------------------------------------
+# Analytic data: quadrature, source loads, L2 projection and error, de Rham map
+# ---------------------------------------------------------------------------
+# A field reaches these functions as its values at quadrature nodes, a k-form
+# value being its C(dim, k) components on the colex basis dx^I. "reference":
+# components in each cell's reference frame (a Section's value,
+# derham/src/section.rs:67); "ambient": components in the coordinates of the
+# mesh, pulled back cell by cell (Pullback::at, section.rs:182-195).
+
+def _colex_multisets(nsym, degree):
+    """Weakly increasing words of length `degree` over range(nsym) in colex
+    order (Composition::all, multiindex/src/composition.rs:115-119, through
+    Repetition::Allowed.words, multiindex/src/monotone.rs:161-179)."""
+    import itertools
+    words = list(itertools.combinations_with_replacement(range(nsym), degree))
+    words.sort(key=lambda w: tuple(reversed(w)))
+    return words
+
+
+def gm_rule(dim, degree):
+    """SimplexQuadRule::degree (simplicial/src/atlas/quadrature.rs:37-83): the
+    Grundmann-Moeller rule of index s = degree // 2 on the dim-simplex, exact
+    for polynomials of degree 2s + 1; barycentric nodes [np, dim + 1] and
+    weights normalized to sum to one."""
+    n, s = dim, degree // 2
+    d = 2 * s + 1
+    pts, ws = [], []
+    for i in range(s + 1):
+        den = float(d + n - 2 * i)
+        sign = 1.0 if MUTATION == 41 else (-1.0) ** i
+        w = sign * 2.0 ** (-2 * s) * den ** d / (math.factorial(i) * math.factorial(d + n - i))
+        for word in _colex_multisets(n + 1, s - i):
+            parts = [0] * (n + 1)
+            for sym in word:
+                parts[sym] += 1
+            pts.append([(2 * b + 1) / den for b in parts])
+            ws.append(w)
+    ws = np.array(ws)
+    total = 0.0
+    for w in ws:
+        total += w
+    return np.array(pts), ws / total
+
+
+def _subsets(n, k):
+    """k-subsets of range(n) in colex order."""
+    import itertools
+    out = list(itertools.combinations(range(n), k))
+    out.sort(key=lambda t: tuple(reversed(t)))
+    return out
+
+
+def _minors(A, rows, cols):
+    """[len(rows), len(cols)] determinants of A's minors on the given index
+    subsets (Lambda^k of a matrix); A is [..., m, n]."""
+    A = np.asarray(A)
+    lead = A.shape[:-2]
+    out = np.empty(lead + (len(rows), len(cols)))
+    for a, R in enumerate(rows):
+        for b, Cc in enumerate(cols):
+            if len(R) == 0:
+                out[..., a, b] = 1.0
+            else:
+                out[..., a, b] = np.linalg.det(A[..., list(R), :][..., list(Cc)])
+    return out
+
+
+def bary_pullback_table(d, k):
+    """P = Lambda^k(Dlambda) [C(d+1, k), C(d, k)]: row J a k-subset of the cell's
+    vertices, column I an axis k-subset; Dlambda has row 0 = -1 and row j = e_(j-1)
+    (unit_difbarys, simplicial/src/atlas.rs)."""
+    Dl = np.zeros((d + 1, d))
+    Dl[0, :] = -1.0
+    Dl[1:, :] = np.eye(d)
+    return _minors(Dl, _subsets(d + 1, k), _subsets(d, k))
+
+
+def whitney_values(d, k, bary):
+    """WhitneyLsf::at_bary (derham/src/interpolate/form.rs:109-114) for every
+    DOF of grade k at barycentric points bary [np, d+1]: [np, C(d+1, k+1), C(d, k)],
+    W_sigma = k! sum_p (-1)^p lambda_(sigma_p) P[sigma minus sigma_p]."""
+    P = bary_pullback_table(d, k)
+    vsub = _subsets(d + 1, k)
+    rank = {s: i for i, s in enumerate(vsub)}
+    dofs = _subsets(d + 1, k + 1)
+    out = np.zeros((len(bary), len(dofs), P.shape[1]))
+    for i, sig in enumerate(dofs):
+        for p, v in enumerate(sig):
+            rest = tuple(x for x in sig if x != v)
+            kf = 1.0 if MUTATION == 42 else math.factorial(k)
+            out[:, i, :] += ((-1.0) ** p * kf) * bary[:, v:v + 1] * P[rank[rest]][None, :]
+    return out
+
+
+def cell_frames(cx: Complex, lengths):
+    """Per cell: vol, g^-1 [nc, d, d] (SimplexLengthsSq::metric,
+    regge/src/lengths/simplex.rs:308-326; cell_volume, regge/src/lib.rs:26-28)."""
+    d = cx.dim
+    edges = cx.cell_faces(1)
+    s = np.asarray(lengths)[edges]
+    loc = local_subsets(d, 1)
+    eidx = {(int(a), int(b)): i for i, (a, b) in enumerate(loc)}
+    g = np.zeros((len(edges), d, d))
+    for i in range(d):
+        g[:, i, i] = s[:, eidx[(0, i + 1)]]
+        for j in range(i + 1, d):
+            v = 0.5 * ((s[:, eidx[(0, i + 1)]] + s[:, eidx[(0, j + 1)]]) - s[:, eidx[(i + 1, j + 1)]])
+            g[:, i, j] = g[:, j, i] = v
+    det = np.linalg.det(g)
+    vol = np.sqrt(np.abs(det)) / math.factorial(d)
+    return vol, np.linalg.inv(g)
+
+
+def multiform_gramian(ginv, k):
+    """Lambda^k g^-1 [nc, C(d,k), C(d,k)] (multiform_metric,
+    metric/src/tensor.rs:127-131): minors of g^-1 on colex axis subsets."""
+    d = ginv.shape[-1]
+    sub = _subsets(d, k)
+    return _minors(ginv, sub, sub)
+
+
+def cell_jacobians(cx: Complex, coords):
+    """Columns x_(v_(a+1)) - x_(v_0) of each cell's affine map [nc, D, d]
+    (the parametrization's linear_transform, section.rs:184-194)."""
+    cells = cx.cells()
+    X = np.asarray(coords)[cells]
+    return np.transpose(X[:, 1:, :] - X[:, :1, :], (0, 2, 1))
+
+
+def pullback_ambient(J, f, k):
+    """Pullback of ambient k-form components f [nc, np, C(D,k)] along J [nc, D, d]:
+    f_ref[I] = sum_K det(J[K, I]) f[K] (Tensor::pullback, the contravariant
+    action of Lambda^k)."""
+    D, d = J.shape[1], J.shape[2]
+    if MUTATION == 44:
+        J = np.transpose(J, (0, 2, 1))
+    PB = _minors(J, _subsets(D, k), _subsets(d, k))  # [nc, C(D,k), C(d,k)]
+    return np.einsum("cKI,cnK->cnI", PB, f)
+
+
+def cell_quadrature_points(cx: Complex, coords, degree):
+    """The embedded position of every cell's quadrature nodes [nc, np, D]:
+    x = sum_i lambda_i x_(v_i) (chart.point, the MeshPoint the integrand sees)."""
+    bary, _ = gm_rule(cx.dim, degree)
+    X = np.asarray(coords)[cx.cells()]
+    return np.einsum("ni,cia->cna", bary, X)
+
+
+def source_load(cx: Complex, lengths, k, degree, field, frame="reference", coords=None):
+    """SourceForm (formoniq/src/operators.rs:599-635) assembled as a load vector
+    (LinearForm assembly, galerkin.rs:205-312): per cell
+    b_sigma = vol sum_n w_n <f(x_n), W_sigma(x_n)>_(Lambda^k g^-1), summed into
+    the global vector in cell order. field [nc, np, C]."""
+    d = cx.dim
+    bary, w = gm_rule(d, degree)
+    vol, ginv = cell_frames(cx, lengths)
+    L = multiform_gramian(ginv, k)
+    f = np.asarray(field, dtype=np.float64)
+    if frame == "ambient":
+        f = pullback_ambient(cell_jacobians(cx, coords), f, k)
+    W = whitney_values(d, k, bary)  # [np, nf, NA]
+    if MUTATION == 43:
+        L = np.broadcast_to(np.eye(L.shape[-1]), L.shape)
+    h = np.einsum("cij,cnj->cni", L, f)
+    el = np.einsum("n,nsi,cni->cs", w, W, h) * vol[:, None]
+    b = np.zeros(cx.nsimplices(k))
+    faces = cx.cell_faces(k) if k < d else np.arange(len(vol))[:, None]
+    np.add.at(b, faces, el)
+    return b
+
+
+def whitney_at_nodes(cx: Complex, k, cochain, degree):
+    """WhitneyInterpolant::eval (derham/src/interpolate/interpolant.rs:55-62) at
+    every cell's quadrature nodes, reference frame: [nc, np, C(d, k)]."""
+    d = cx.dim
+    bary, _ = gm_rule(d, degree)
+    W = whitney_values(d, k, bary)
+    faces = cx.cell_faces(k) if k < d else np.arange(cx.nsimplices(d))[:, None]
+    cv = np.asarray(cochain)[faces]  # [nc, nf]
+    return np.einsum("cs,nsi->cni", cv, W)
+
+
+def fe_l2_error(cx: Complex, lengths, k, cochain, exact, frame="reference", coords=None):
+    """fe_l2_error (formoniq/src/fe.rs:50-74): sqrt(sum_K vol_K sum_n w_n
+    <e, e>_(Lambda^k g^-1)), e = exact - W c at the degree-3 nodes."""
+    d = cx.dim
+    _, w = gm_rule(d, 3)
+    vol, ginv = cell_frames(cx, lengths)
+    L = multiform_gramian(ginv, k)
+    f = np.asarray(exact, dtype=np.float64)
+    if frame == "ambient":
+        f = pullback_ambient(cell_jacobians(cx, coords), f, k)
+    e = f - whitney_at_nodes(cx, k, cochain, 3)
+    per = np.einsum("n,cni,cij,cnj->c", w, e, L, e) * (1.0 if MUTATION == 46 else vol)
+    return math.sqrt(float(np.sum(per)))
+
+
+def l2_projection(c: WhitneyComplex, k, degree, field, frame="reference", coords=None):
+    """l2_projection (formoniq/src/fe.rs:87-123): M_k x = b with b the source
+    load, by Jacobi CG to rtol 1e-12 on a Riemannian geometry, else a direct
+    solve (the reference's LU)."""
+    b = source_load(c.cx, c.lengths, k, degree, field, frame, coords)
+    M = c.mass(k)
+    _, ginv = cell_frames(c.cx, c.lengths)
+    riem = bool(np.all(np.linalg.eigvalsh(ginv) > 0))
+    if riem:
+        x, rep = cg(M, b, inv_diag=1.0 / M.diagonal(), rtol=1e-12, max_iters=100 * len(b) + 100)
+        return x
+    import scipy.sparse.linalg as spla
+    return spla.spsolve(M.to_scipy().tocsc(), b)
+
+
+def simplex_quadrature_points(cx: Complex, coords, k, degree):
+    """Positions of the quadrature nodes of every k-simplex [N_k, np, D]."""
+    bary, _ = gm_rule(k, degree)
+    X = np.asarray(coords)[cx.skeleton(k)]
+    return np.einsum("ni,sia->sna", bary, X)
+
+
+def derham_map(cx: Complex, coords, k, degree, field):
+    """derham_map (derham/src/project.rs:55-96): c_s = (1/k!) sum_n w_n
+    <f(x_n), e_1 ^ ... ^ e_k>, the pairing of the ambient k-form with the
+    tangent blade of the simplex (edge vectors e_a = x_(s_a) - x_(s_0)), by the
+    degree-`degree` rule on the k-simplex. field [N_k, np, C(D, k)]."""
+    D = np.asarray(coords).shape[1]
+    _, w = gm_rule(k, degree)
+    X = np.asarray(coords)[cx.skeleton(k)]
+    E = np.transpose(X[:, 1:, :] - X[:, :1, :], (0, 2, 1))  # [N_k, D, k]
+    blade = _minors(E, _subsets(D, k), [tuple(range(k))])[..., 0]  # [N_k, C(D,k)]
+    integrand = np.einsum("snK,sK->sn", np.asarray(field, dtype=np.float64), blade)
+    return (integrand @ w) / (1 if MUTATION == 45 else math.factorial(k))
