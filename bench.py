@@ -191,6 +191,61 @@ def c3_measure(n=16, reps=3, sample=2048):
     }
 
 
+def analytic_measure(fa, topo, geo, reps=5):
+    """The analytic-data path on the headline mesh (3D Kuhn 64^3, 1,572,864
+    tets): a smooth ambient 1-form sampled at the degree-3 cell nodes (5 per
+    tet) and resident on the device; per call, its SourceForm load (the
+    element kernel plus the per-edge sum), its L2 projection (load, mass
+    assembly, Jacobi CG to 1e-12; and with the mass assembled once and
+    reused) and the fe_l2_error of the projection; the
+    de Rham map of the same form on the edges (degree-3 rule, 3 nodes per
+    edge). Wall time per call with the device synchronized, median of reps."""
+    nc = topo.nsimplices(3)
+    ne = topo.nsimplices(1)
+
+    def field(p):
+        return np.stack([np.sin(np.pi * p[..., 1]), np.cos(np.pi * p[..., 2]),
+                         np.sin(np.pi * p[..., 0]) * p[..., 1]], -1)
+    vals = fa.DeviceArray.from_numpy(field(fa.cell_quadrature_points(geo, 3)))
+    evals = fa.DeviceArray.from_numpy(field(fa.simplex_quadrature_points(geo, 1, 3)))
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), out
+    t_load, _ = timed(lambda: fa.source_load(geo, 1, vals, 3, "ambient"))
+    t_proj, (x, rep) = timed(lambda: fa.l2_projection(geo, 1, vals, 3, "ambient"))
+    mass = fa.WhitneyComplex(geo).mass(1)
+    t_proj_m, _ = timed(lambda: fa.l2_projection(geo, 1, vals, 3, "ambient", mass=mass))
+    t_err, err = timed(lambda: fa.fe_l2_error(x, vals, geo, 1, "ambient"))
+    t_dr, _ = timed(lambda: fa.derham_map(geo, 1, evals, 3))
+    # per tet: 5 nodes x 3 components of the field, 6 edge ids, 4 vertex ids,
+    # 6 element loads out; once per mesh: the squared lengths and coordinates
+    load_bytes = nc * (5 * 3 * 8 + 6 * 4 + 4 * 4 + 6 * 8) + ne * 8 + topo.nsimplices(0) * 3 * 8
+    return {
+        "workload": f"ambient 1-form (sin pi y, cos pi z, y sin pi x) on Kuhn 64^3 ({nc} tets), "
+                    "degree-3 cell rule (5 nodes), field resident on the device",
+        "source_load_ms": t_load * 1e3, "source_load_cells_per_s": nc / t_load,
+        "source_load_algorithmic_gbs": load_bytes / t_load / 1e9,
+        "l2_projection_ms": t_proj * 1e3, "l2_projection_cg_iters": int(rep.iters),
+        "l2_projection_reused_mass_ms": t_proj_m * 1e3,
+        "fe_l2_error_ms": t_err * 1e3, "projection_l2_error": err,
+        "derham_map_edges_ms": t_dr * 1e3, "derham_map_edges_per_s": ne / t_dr,
+        "dtype": "f64",
+    }
+
+
+def run_analytic(args):
+    import formoniq_amd as fa
+    ctx = fa.Context(0)
+    topo, geo, wc = build_problem(fa, args.n, fa.FP64_PC32, ctx)
+    print(json.dumps({"config": "analytic", **analytic_measure(fa, topo, geo, args.steps)}), flush=True)
+
+
 def run_c3(args):
     n = args.n if args.n != 64 else 16
     print(json.dumps({"config": "C3", **c3_measure(n, args.steps)}), flush=True)
@@ -440,6 +495,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="leave out the C3 sub-object")
     ap.add_argument("--no-c5", action="store_true", help="leave out the C5 sub-object")
+    ap.add_argument("--no-analytic", action="store_true", help="leave out the analytic-data sub-object")
     ap.add_argument("--c5-requests", type=int, default=16, help="requests of the C5 sub-object's stream")
     ap.add_argument("--c5-rate", type=float, default=2.0, help="arrival rate of the C5 sub-object's stream")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "c2_kernel_traffic.json"))
@@ -450,7 +506,7 @@ def main():
                     help="MINRES iterations per replayed graph segment")
     ap.add_argument("--no-warmup", action="store_true",
                     help="c5: start the timed stream on a cold server (no start-up requests)")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "kernels", "leapfrog"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "kernels", "leapfrog", "analytic"],
                     help="c2 (default): the headline line, with C3 and C5 sub-objects; c3/c4/c5: those "
                          "workloads in full, each printed as its own JSON line")
     ap.add_argument("--batch", type=int, default=512, help="c4: number of meshes")
@@ -464,7 +520,7 @@ def main():
         sys.exit(2)
     if args.config != "c2":
         {"c3": run_c3, "c4": run_c4, "c5": run_c5, "kernels": run_kernels,
-         "leapfrog": run_leapfrog}[args.config](args)
+         "leapfrog": run_leapfrog, "analytic": run_analytic}[args.config](args)
         return
 
     import formoniq_amd as fa
@@ -611,6 +667,11 @@ def main():
                         "achieved_gbs": spmv64_bytes / times["spmv64"] / 1e9},
         "roofline": roof,
     }
+    if not args.no_analytic:
+        try:
+            line["analytic"] = analytic_measure(fa, topo, geo)
+        except Exception as e:
+            line["analytic"] = {"error": repr(e)}
     if not args.no_c3:
         try:
             line["c3"] = c3_measure(16, 3, 256)

@@ -318,13 +318,17 @@ class SourceForm:
         return source_load(geometry, self.grade, self.values, self.degree, self.frame)
 
 
-def l2_projection(geometry: "MeshLengthsSq", grade, values, degree=1, frame="reference"):
-    """l2_projection (fe.rs:87-123): (cochain, Report) of the mass solve M_k c = load."""
+def l2_projection(geometry: "MeshLengthsSq", grade, values, degree=1, frame="reference",
+                  mass: "Csr" = None):
+    """l2_projection (fe.rs:87-123): (cochain, Report) of the mass solve M_k c = load.
+    mass: the grade-k mass (WhitneyComplex(geometry).mass(grade)) to reuse, or None
+    to assemble it inside the call."""
     addr, dev, keep = _array_in(values, _cell_values_shape(geometry, grade, degree))
     out, oaddr = _array_out(dev, (geometry.complex.nsimplices(grade),))
     rep = Report()
-    check(load().feec_l2_projection(geometry._h, grade, degree, _frame(frame), addr, oaddr,
-                                    C.byref(rep), dev))
+    check(load().feec_l2_projection(geometry._h, grade, degree, _frame(frame), addr,
+                                    mass._h if mass is not None else None, oaddr, C.byref(rep),
+                                    dev))
     return out, rep
 
 

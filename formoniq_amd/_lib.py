@@ -126,7 +126,7 @@ SIGNATURES = {
     "feec_cell_quadrature_points": (S, [vp, C.c_int, vp, C.c_int]),
     "feec_simplex_quadrature_points": (S, [vp, C.c_int, C.c_int, vp, C.c_int]),
     "feec_source_load": (S, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int]),
-    "feec_l2_projection": (S, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.POINTER(Report), C.c_int]),
+    "feec_l2_projection": (S, [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.POINTER(Report), C.c_int]),
     "feec_whitney_at_nodes": (S, [vp, C.c_int, C.c_int, vp, vp, C.c_int]),
     "feec_l2_error": (S, [vp, C.c_int, vp, C.c_int, vp, f64p, C.c_int]),
     "feec_derham_map": (S, [vp, C.c_int, C.c_int, vp, vp, C.c_int]),

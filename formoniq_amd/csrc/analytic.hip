@@ -33,6 +33,10 @@
 #include "cell_geometry.hpp"
 #include "feec_internal.hpp"
 
+// The node loops carry "#pragma unroll" for the fixed-rule instances; in the
+// generic instance their trip count is a run-time value and stays a loop.
+#pragma clang diagnostic ignored "-Wpass-failed"
+
 namespace feec {
 
 namespace {
@@ -236,11 +240,41 @@ __device__ __forceinline__ void cell_cochain(const int32_t* __restrict__ faces,
   for (int i = 0; i < ND; ++i) cv[i] = cochain[K == D ? c : (int64_t)faces[c * ND + i]];
 }
 
+// Number of nodes of the Grundmann-Moller rule of index S on the n-simplex.
+constexpr int gm_count(int n, int S) {
+  int c = 0;
+  for (int i = 0; i <= S; ++i) c += binom(S - i + n, n);
+  return c;
+}
+constexpr int kGenericRule = -1;
+template <int N, int S>
+__host__ __device__ constexpr int rule_nodes(int np) { return S >= 0 ? gm_count(N, S) : np; }
+// rows of at most this many doubles are held in registers from the start
+template <int S, int W>
+constexpr bool kPreload = S >= 0 && W <= 24;
+
+// The block's rows of an output [row][W] array leave LDS as one contiguous
+// chunk, consecutive lanes on consecutive doubles (each wavefront store covers
+// 512 contiguous bytes). Every thread of the block calls it.
+__device__ __forceinline__ void stage_out(double* __restrict__ dst, int64_t row0, int64_t nrows,
+                                          int W, const double* lds) {
+  __syncthreads();
+  const int64_t total = min((int64_t)TPB, nrows - row0) * W;
+  double* d = dst + row0 * W;
+  for (int64_t i = threadIdx.x; i < total; i += TPB) d[i] = lds[i];
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+// Kernels are instantiated for the rule indices S = 0 and 1 (degrees up to 3,
+// the reference's defaults), whose node loops unroll completely: a lane issues
+// all of its field loads and the rule's (uniform, scalar) loads at once
+// instead of waiting on them node by node. S = -1 is the generic instance for
+// the higher rules, with the node count np at run time.
+//
 // Element loads vol * sum_n w_n <f_n, W_sigma(x_n)>_(Lambda^K g^-1), out[sigma][cell].
-template <int D, int K>
+template <int D, int K, int S>
 __global__ void __launch_bounds__(TPB)
     k_source_load(const int32_t* __restrict__ edges, const double* __restrict__ len,
                   const int32_t* __restrict__ verts, const double* __restrict__ X,
@@ -248,8 +282,19 @@ __global__ void __launch_bounds__(TPB)
                   int64_t nc, double* __restrict__ out) {
   constexpr Grade<D, K> T{};
   constexpr int NA = Grade<D, K>::NA, NP = Grade<D, K>::NP, ND = Grade<D, K>::ND;
+  constexpr int NQ = S >= 0 ? gm_count(D, S) : 1;
+  const int nq = rule_nodes<D, S>(np);
   const int64_t c = blockIdx.x * (int64_t)TPB + threadIdx.x;
   if (c >= nc) return;
+  const double* f = field + c * (int64_t)nq * NA;
+  // small rows are loaded up front, overlapping the metric's latency
+  double fv[kPreload<S, NQ * NA> ? NQ : 1][NA];
+  if constexpr (kPreload<S, NQ * NA>) {
+#pragma unroll
+    for (int n = 0; n < nq; ++n)
+#pragma unroll
+      for (int I = 0; I < NA; ++I) fv[n][I] = f[n * NA + I];
+  }
   CellGeom<D> cg;
   double vol;
   lane_geometry<D>(edges, len, c, cg, vol);
@@ -262,14 +307,14 @@ __global__ void __launch_bounds__(TPB)
     cell_jacobian<D>(verts, X, c, J);
     lambda_k<D, K>(J, PB);
   }
-  const double* w = rule + (int64_t)np * (D + 1);
-  const double* f = field + c * (int64_t)np * NA;
+  const double* w = rule + nq * (D + 1);
   double acc[ND];
 #pragma unroll
   for (int i = 0; i < ND; ++i) acc[i] = 0.0;
-  for (int n = 0; n < np; ++n) {
+#pragma unroll
+  for (int n = 0; n < nq; ++n) {
     double fr[NA];
-    node_value<D, K>(f + n * NA, ambient, PB, fr);
+    node_value<D, K>(kPreload<S, NQ * NA> ? fv[kPreload<S, NQ * NA> ? n : 0] : f + n * NA, ambient, PB, fr);
     double h[NA];
 #pragma unroll
     for (int I = 0; I < NA; ++I) {
@@ -304,27 +349,36 @@ __global__ void __launch_bounds__(TPB)
   for (int i = 0; i < ND; ++i) out[i * nc + c] = vol * acc[i];
 }
 
-// W c at every cell node, reference frame: out[cell][node][I].
-template <int D, int K>
+// W c at every cell node, reference frame: out[cell][node][I], written through
+// LDS (dynamic LDS = TPB * NQ * NA doubles when stage).
+template <int D, int K, int S>
 __global__ void __launch_bounds__(TPB)
     k_whitney_nodes(const int32_t* __restrict__ faces, const double* __restrict__ cochain,
-                    const double* __restrict__ rule, int np, int64_t nc, double* __restrict__ out) {
+                    const double* __restrict__ rule, int np, int64_t nc, bool stage,
+                    double* __restrict__ out) {
   constexpr int NA = Grade<D, K>::NA, ND = Grade<D, K>::ND;
-  const int64_t c = blockIdx.x * (int64_t)TPB + threadIdx.x;
-  if (c >= nc) return;
-  double cv[ND];
-  cell_cochain<D, K>(faces, cochain, c, cv);
-  for (int n = 0; n < np; ++n) {
-    double u[NA];
-    whitney_value<D, K>(rule + n * (D + 1), cv, u);
+  const int nq = rule_nodes<D, S>(np);
+  extern __shared__ double lds[];
+  const int64_t c0 = blockIdx.x * (int64_t)TPB;
+  const int64_t c = c0 + threadIdx.x;
+  double* o = stage ? lds + threadIdx.x * nq * NA : out + c * (int64_t)nq * NA;
+  if (c < nc) {
+    double cv[ND];
+    cell_cochain<D, K>(faces, cochain, c, cv);
 #pragma unroll
-    for (int I = 0; I < NA; ++I) out[(c * np + n) * NA + I] = u[I];
+    for (int n = 0; n < nq; ++n) {
+      double u[NA];
+      whitney_value<D, K>(rule + n * (D + 1), cv, u);
+#pragma unroll
+      for (int I = 0; I < NA; ++I) o[n * NA + I] = u[I];
+    }
   }
+  if (stage) stage_out(out, c0, nc, nq * NA, lds);
 }
 
 // Per cell vol * sum_n w_n <e_n, e_n>_(Lambda^K g^-1), e = f - W c, summed
 // per block (fixed order) into part[blockIdx.x].
-template <int D, int K>
+template <int D, int K, int S>
 __global__ void __launch_bounds__(TPB)
     k_l2_error(const int32_t* __restrict__ edges, const double* __restrict__ len,
                const int32_t* __restrict__ verts, const double* __restrict__ X,
@@ -332,10 +386,22 @@ __global__ void __launch_bounds__(TPB)
                const double* __restrict__ rule, int np, const double* __restrict__ field,
                int64_t nc, double* __restrict__ part) {
   constexpr int NA = Grade<D, K>::NA, ND = Grade<D, K>::ND;
+  constexpr int NQ = S >= 0 ? gm_count(D, S) : 1;
+  const int nq = rule_nodes<D, S>(np);
   __shared__ double red[TPB];
   const int64_t c = blockIdx.x * (int64_t)TPB + threadIdx.x;
   double cell = 0.0;
   if (c < nc) {
+    const double* f = field + c * (int64_t)nq * NA;
+    double fv[kPreload<S, NQ * NA> ? NQ : 1][NA];
+    if constexpr (kPreload<S, NQ * NA>) {
+#pragma unroll
+      for (int n = 0; n < nq; ++n)
+#pragma unroll
+        for (int I = 0; I < NA; ++I) fv[n][I] = f[n * NA + I];
+    }
+    double cv[ND];
+    cell_cochain<D, K>(faces, cochain, c, cv);
     CellGeom<D> cg;
     double vol;
     lane_geometry<D>(edges, len, c, cg, vol);
@@ -348,14 +414,13 @@ __global__ void __launch_bounds__(TPB)
       cell_jacobian<D>(verts, X, c, J);
       lambda_k<D, K>(J, PB);
     }
-    double cv[ND];
-    cell_cochain<D, K>(faces, cochain, c, cv);
-    const double* w = rule + (int64_t)np * (D + 1);
-    const double* f = field + c * (int64_t)np * NA;
+    const double* w = rule + nq * (D + 1);
     double acc = 0.0;
-    for (int n = 0; n < np; ++n) {
+#pragma unroll
+    for (int n = 0; n < nq; ++n) {
       double fr[NA], u[NA];
-      node_value<D, K>(f + n * NA, ambient, PB, fr);
+      node_value<D, K>(kPreload<S, NQ * NA> ? fv[kPreload<S, NQ * NA> ? n : 0] : f + n * NA, ambient, PB,
+                       fr);
       whitney_value<D, K>(rule + n * (D + 1), cv, u);
       double e[NA];
 #pragma unroll
@@ -402,39 +467,54 @@ __device__ __forceinline__ void simplex_vertices(const int32_t* __restrict__ ver
   }
 }
 
-// Node positions sum_i lam_i x_(v_i) of every grade-K simplex (K = D: cells).
-template <int D, int K>
+// Node positions sum_i lam_i x_(v_i) of every grade-K simplex (K = D: cells),
+// written through LDS like k_whitney_nodes.
+template <int D, int K, int S>
 __global__ void __launch_bounds__(TPB)
     k_simplex_points(const int32_t* __restrict__ verts, const int64_t* __restrict__ rep,
                      const double* __restrict__ X, const double* __restrict__ rule, int np,
-                     int64_t ns, double* __restrict__ out) {
-  const int64_t s = blockIdx.x * (int64_t)TPB + threadIdx.x;
-  if (s >= ns) return;
-  int64_t v[K + 1];
-  simplex_vertices<D, K>(verts, rep, s, v);
-  for (int n = 0; n < np; ++n) {
-    const double* lam = rule + n * (K + 1);
+                     int64_t ns, bool stage, double* __restrict__ out) {
+  const int nq = rule_nodes<K, S>(np);
+  extern __shared__ double lds[];
+  const int64_t s0 = blockIdx.x * (int64_t)TPB;
+  const int64_t s = s0 + threadIdx.x;
+  double* o = stage ? lds + threadIdx.x * nq * D : out + s * (int64_t)nq * D;
+  if (s < ns) {
+    int64_t v[K + 1];
+    simplex_vertices<D, K>(verts, rep, s, v);
+    double x[K + 1][D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-      double x = 0.0;
+    for (int i = 0; i <= K; ++i)
 #pragma unroll
-      for (int i = 0; i <= K; ++i) x += lam[i] * X[v[i] * D + a];
-      out[(s * np + n) * D + a] = x;
+      for (int a = 0; a < D; ++a) x[i][a] = X[v[i] * D + a];
+#pragma unroll
+    for (int n = 0; n < nq; ++n) {
+      const double* lam = rule + n * (K + 1);
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i <= K; ++i) t += lam[i] * x[i][a];
+        o[n * D + a] = t;
+      }
     }
   }
+  if (stage) stage_out(out, s0, ns, nq * D, lds);
 }
 
 // c_s = (1/K!) sum_n w_n sum_Kamb f[s][n][Kamb] det(E[ax Kamb, :]), E the
 // simplex's edge vectors x_(s_(b+1)) - x_(s_0) (D x K).
-template <int D, int K>
+template <int D, int K, int S>
 __global__ void __launch_bounds__(TPB)
     k_derham(const int32_t* __restrict__ verts, const int64_t* __restrict__ rep,
              const double* __restrict__ X, const double* __restrict__ rule, int np,
              const double* __restrict__ field, int64_t ns, double* __restrict__ out) {
   constexpr Grade<D, K> T{};
   constexpr int NA = Grade<D, K>::NA;
+  const int nq = rule_nodes<K, S>(np);
   const int64_t s = blockIdx.x * (int64_t)TPB + threadIdx.x;
   if (s >= ns) return;
+  const double* f = field + s * (int64_t)nq * NA;
   int64_t v[K + 1];
   simplex_vertices<D, K>(verts, rep, s, v);
   double blade[NA];
@@ -450,10 +530,10 @@ __global__ void __launch_bounds__(TPB)
       }
     blade[A] = leibniz<K>(m);
   }
-  const double* w = rule + (int64_t)np * (K + 1);
-  const double* f = field + s * (int64_t)np * NA;
+  const double* w = rule + nq * (K + 1);
   double acc = 0.0;
-  for (int n = 0; n < np; ++n) {
+#pragma unroll
+  for (int n = 0; n < nq; ++n) {
     double t = 0.0;
 #pragma unroll
     for (int A = 0; A < NA; ++A) t += f[n * NA + A] * blade[A];
@@ -478,14 +558,23 @@ __global__ void k_inv_abs_diag(const double* d, int64_t n, bool absolute, double
 }
 
 // ---------------------------------------------------------------------------
-// dispatch over (D, K)
+// dispatch over (D, K) and the rule index S
 // ---------------------------------------------------------------------------
-template <template <int, int> class F, class... A>
-void by_dk(int d, int k, A&&... a) {
+template <template <int, int, int> class F, int D, int K, class... A>
+void by_s(int S, A&&... a) {
+  switch (S) {
+    case 0: F<D, K, 0>::run(a...); return;
+    case 1: F<D, K, 1>::run(a...); return;
+    default: F<D, K, kGenericRule>::run(a...); return;
+  }
+}
+
+template <template <int, int, int> class F, class... A>
+void by_dks(int d, int k, int S, A&&... a) {
   switch (d * 8 + k) {
-#define FEEC_DK(D, K) \
-  case D * 8 + K:     \
-    F<D, K>::run(a...); \
+#define FEEC_DK(D, K)                \
+  case D * 8 + K:                    \
+    by_s<F, D, K>(S, a...);          \
     return;
     FEEC_DK(1, 0) FEEC_DK(1, 1)
     FEEC_DK(2, 0) FEEC_DK(2, 1) FEEC_DK(2, 2)
@@ -496,50 +585,66 @@ void by_dk(int d, int k, A&&... a) {
   fail(FEEC_E_UNSUPPORTED, "dimension 1..4 and grade 0..dim");
 }
 
-template <int D, int K>
+// the dynamic LDS of a block that writes rows of W doubles through LDS, 0 when
+// that chunk would exceed kStageBytes (then the rows are written directly)
+constexpr int kStageBytes = 48 * 1024;
+inline size_t stage_bytes(int W) {
+  const size_t b = (size_t)TPB * W * 8;
+  return b <= (size_t)kStageBytes ? b : 0;
+}
+
+template <int D, int K, int S>
 struct RunLoad {
   static void run(Ctx* ctx, const int32_t* edges, const double* len, const int32_t* verts,
                   const double* X, const DeviceRule& r, const double* field, int64_t nc,
                   double* out) {
-    k_source_load<D, K><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, len, verts, X, r.t.p, r.np,
-                                                                field, nc, out);
+    FEEC_REQUIRE((r.np == rule_nodes<D, S>(r.np)), FEEC_E_ARG, "rule does not match its index");
+    k_source_load<D, K, S><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, len, verts, X, r.t.p,
+                                                                   r.np, field, nc, out);
     FEEC_HIP(hipGetLastError());
   }
 };
-template <int D, int K>
+template <int D, int K, int S>
 struct RunWhitney {
   static void run(Ctx* ctx, const int32_t* faces, const double* cochain, const DeviceRule& r,
                   int64_t nc, double* out) {
-    k_whitney_nodes<D, K><<<grid_for(nc), TPB, 0, ctx->stream>>>(faces, cochain, r.t.p, r.np, nc,
-                                                                  out);
+    FEEC_REQUIRE((r.np == rule_nodes<D, S>(r.np)), FEEC_E_ARG, "rule does not match its index");
+    const size_t lds = stage_bytes(r.np * Grade<D, K>::NA);
+    k_whitney_nodes<D, K, S><<<grid_for(nc), TPB, lds, ctx->stream>>>(faces, cochain, r.t.p, r.np,
+                                                                       nc, lds != 0, out);
     FEEC_HIP(hipGetLastError());
   }
 };
-template <int D, int K>
+template <int D, int K, int S>
 struct RunError {
   static void run(Ctx* ctx, const int32_t* edges, const double* len, const int32_t* verts,
                   const double* X, const int32_t* faces, const double* cochain,
                   const DeviceRule& r, const double* field, int64_t nc, double* part) {
-    k_l2_error<D, K><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, len, verts, X, faces, cochain,
-                                                             r.t.p, r.np, field, nc, part);
+    FEEC_REQUIRE((r.np == rule_nodes<D, S>(r.np)), FEEC_E_ARG, "rule does not match its index");
+    k_l2_error<D, K, S><<<grid_for(nc), TPB, 0, ctx->stream>>>(edges, len, verts, X, faces,
+                                                                cochain, r.t.p, r.np, field, nc,
+                                                                part);
     FEEC_HIP(hipGetLastError());
   }
 };
-template <int D, int K>
+template <int D, int K, int S>
 struct RunPoints {
   static void run(Ctx* ctx, const int32_t* verts, const int64_t* rep, const double* X,
                   const DeviceRule& r, int64_t ns, double* out) {
-    k_simplex_points<D, K><<<grid_for(ns), TPB, 0, ctx->stream>>>(verts, rep, X, r.t.p, r.np, ns,
-                                                                   out);
+    FEEC_REQUIRE((r.np == rule_nodes<K, S>(r.np)), FEEC_E_ARG, "rule does not match its index");
+    const size_t lds = stage_bytes(r.np * D);
+    k_simplex_points<D, K, S><<<grid_for(ns), TPB, lds, ctx->stream>>>(verts, rep, X, r.t.p, r.np,
+                                                                        ns, lds != 0, out);
     FEEC_HIP(hipGetLastError());
   }
 };
-template <int D, int K>
+template <int D, int K, int S>
 struct RunDerham {
   static void run(Ctx* ctx, const int32_t* verts, const int64_t* rep, const double* X,
                   const DeviceRule& r, const double* field, int64_t ns, double* out) {
-    k_derham<D, K><<<grid_for(ns), TPB, 0, ctx->stream>>>(verts, rep, X, r.t.p, r.np, field, ns,
-                                                           out);
+    FEEC_REQUIRE((r.np == rule_nodes<K, S>(r.np)), FEEC_E_ARG, "rule does not match its index");
+    k_derham<D, K, S><<<grid_for(ns), TPB, 0, ctx->stream>>>(verts, rep, X, r.t.p, r.np, field, ns,
+                                                              out);
     FEEC_HIP(hipGetLastError());
   }
 };
@@ -646,15 +751,33 @@ void source_load_dev(Geom& g, int k, int degree, bool ambient, const double* fie
   const double* X = ambient ? device_coords(g) : nullptr;
   const int nf = m.nfaces(k);
   if (k == d) {
-    by_dk<RunLoad>(d, k, ctx, m.faces[1].p, g.lengths.p, m.faces[0].p, X, dr, field, nc, out);
+    by_dks<RunLoad>(d, k, degree / 2, ctx, m.faces[1].p, g.lengths.p, m.faces[0].p, X, dr, field, nc, out);
     ctx->sync();
     return;
   }
   DBuf<double> el(ctx, std::max<int64_t>(nc * nf, 1));
-  by_dk<RunLoad>(d, k, ctx, m.faces[1].p, g.lengths.p, m.faces[0].p, X, dr, field, nc, el.p);
+  by_dks<RunLoad>(d, k, degree / 2, ctx, m.faces[1].p, g.lengths.p, m.faces[0].p, X, dr, field, nc, el.p);
   const CooPlan& plan = load_plan(m, k);
   coo_plan_values(ctx, plan, el.p, out, nullptr);
   ctx->sync();
+}
+
+// A device copy of a CSR (same context).
+void csr_clone(const Csr& a, Csr& out) {
+  Ctx* ctx = a.ctx;
+  out.ctx = ctx;
+  out.nrows = a.nrows;
+  out.ncols = a.ncols;
+  out.nnz = a.nnz;
+  out.row_ptr.alloc(ctx, a.nrows + 1);
+  out.col.alloc(ctx, a.nnz);
+  out.val.alloc(ctx, a.nnz);
+  FEEC_HIP(hipMemcpyAsync(out.row_ptr.p, a.row_ptr.p, (a.nrows + 1) * 8, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  if (a.nnz) {
+    FEEC_HIP(hipMemcpyAsync(out.col.p, a.col.p, a.nnz * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    FEEC_HIP(hipMemcpyAsync(out.val.p, a.val.p, a.nnz * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  }
 }
 
 }  // namespace
@@ -689,7 +812,7 @@ feec_status feec_cell_quadrature_points(feec_geom* g, int degree, double* points
     const double* X = device_coords(*g);
     const int64_t nc = m.n[m.d];
     Out o(ctx, points, nc * r.np * m.d, on_device != 0);
-    by_dk<RunPoints>(m.d, m.d, ctx, m.faces[0].p, (const int64_t*)nullptr, X, dr, nc, o.p);
+    by_dks<RunPoints>(m.d, m.d, degree / 2, ctx, m.faces[0].p, (const int64_t*)nullptr, X, dr, nc, o.p);
     o.done(ctx);
   });
 }
@@ -709,7 +832,7 @@ feec_status feec_simplex_quadrature_points(feec_geom* g, int grade, int degree, 
     const int64_t ns = m.n[grade];
     Out o(ctx, points, ns * r.np * m.d, on_device != 0);
     const int64_t* rep = (grade > 0 && grade < m.d) ? m.rep[grade].p : nullptr;
-    by_dk<RunPoints>(m.d, grade, ctx, m.faces[0].p, rep, X, dr, ns, o.p);
+    by_dks<RunPoints>(m.d, grade, degree / 2, ctx, m.faces[0].p, rep, X, dr, ns, o.p);
     o.done(ctx);
   });
 }
@@ -732,8 +855,8 @@ feec_status feec_source_load(feec_geom* g, int grade, int degree, feec_frame fra
 }
 
 feec_status feec_l2_projection(feec_geom* g, int grade, int degree, feec_frame frame,
-                               const double* field, double* cochain, feec_report* report,
-                               int on_device) {
+                               const double* field, const feec_csr* mass, double* cochain,
+                               feec_report* report, int on_device) {
   return guard([&] {
     FEEC_REQUIRE(g && field && cochain, FEEC_E_ARG, "null argument");
     const bool ambient = frame_ambient(frame);
@@ -746,8 +869,14 @@ feec_status feec_l2_projection(feec_geom* g, int grade, int degree, feec_frame f
     In f(ctx, field, m.n[m.d] * np * ncomp(m.d, grade), on_device != 0);
     DBuf<double> b(ctx, std::max<int64_t>(n, 1));
     source_load_dev(*g, grade, degree, ambient, f.p, b.p);
-    Csr M;
-    assemble_form(*g, FEEC_FULL, FEEC_MASS, grade, M);
+    Csr own;
+    if (mass) {
+      FEEC_REQUIRE(mass->ctx == ctx && mass->nrows == n && mass->ncols == n, FEEC_E_ARG,
+                   "mass must be this geometry's N_k x N_k matrix");
+    } else {
+      assemble_form(*g, FEEC_FULL, FEEC_MASS, grade, own);
+    }
+    const Csr& M = mass ? *static_cast<const Csr*>(mass) : own;
     // Riemannian on every cell: the mass is SPD and Jacobi CG is the reference's solve
     bool riemannian = g->euclidean;
     if (!riemannian) {
@@ -776,7 +905,11 @@ feec_status feec_l2_projection(feec_geom* g, int grade, int degree, feec_frame f
       sys.ctx = ctx;
       sys.n = n;
       sys.nh = 0;
-      sys.A = std::move(M);
+      if (mass) {
+        csr_clone(M, sys.A);
+      } else {
+        sys.A = std::move(own);
+      }
       sys.inv_diag = std::move(dinv);
       feec_solve_opts opts;
       feec_solve_opts_default(&opts);
@@ -808,7 +941,7 @@ feec_status feec_whitney_at_nodes(feec_geom* g, int grade, int degree, const dou
     In c(ctx, cochain, m.n[grade], on_device != 0);
     Out o(ctx, values, nc * r.np * ncomp(m.d, grade), on_device != 0);
     const int32_t* faces = grade < m.d ? m.faces[grade].p : nullptr;
-    by_dk<RunWhitney>(m.d, grade, ctx, faces, c.p, dr, nc, o.p);
+    by_dks<RunWhitney>(m.d, grade, degree / 2, ctx, faces, c.p, dr, nc, o.p);
     o.done(ctx);
   });
 }
@@ -833,7 +966,7 @@ feec_status feec_l2_error(feec_geom* g, int grade, const double* cochain, feec_f
     const unsigned nb = grid_for(nc);
     DBuf<double> part(ctx, nb);
     const int32_t* faces = grade < m.d ? m.faces[grade].p : nullptr;
-    by_dk<RunError>(m.d, grade, ctx, m.faces[1].p, g->lengths.p, m.faces[0].p, X, faces, c.p, dr,
+    by_dks<RunError>(m.d, grade, 1, ctx, m.faces[1].p, g->lengths.p, m.faces[0].p, X, faces, c.p, dr,
                     f.p, nc, part.p);
     std::vector<double> h(nb);
     copy_to_host(ctx, h.data(), part.p, nb * 8);
@@ -859,7 +992,7 @@ feec_status feec_derham_map(feec_geom* g, int grade, int degree, const double* f
     In f(ctx, field, ns * r.np * ncomp(m.d, grade), on_device != 0);
     Out o(ctx, cochain, ns, on_device != 0);
     const int64_t* rep = (grade > 0 && grade < m.d) ? m.rep[grade].p : nullptr;
-    by_dk<RunDerham>(m.d, grade, ctx, m.faces[0].p, rep, X, dr, f.p, ns, o.p);
+    by_dks<RunDerham>(m.d, grade, degree / 2, ctx, m.faces[0].p, rep, X, dr, f.p, ns, o.p);
     o.done(ctx);
   });
 }

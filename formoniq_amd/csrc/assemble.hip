@@ -122,19 +122,24 @@ __global__ void k_rows_of(const int64_t* row_ptr, int64_t nrows, int32_t* rows) 
   for (int64_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) rows[p] = (int32_t)r;
 }
 
-__global__ void k_compact(const int64_t* row_ptr, int64_t nrows, const int32_t* col,
-                          const double* val, const uint8_t* keep, const int64_t* newpos,
-                          int32_t* ocol, double* oval, unsigned long long* rowcnt) {
+// Compaction of a pattern to its kept entries. Entry p moves to newpos[p]
+// (the exclusive scan of keep), one lane per entry so loads and stores run in
+// entry order; a row starts at the new position of its first old entry, or at
+// the end when it has none left in the old array.
+__global__ void k_compact_entries(const int32_t* col, const double* val, const uint8_t* keep,
+                                  const int64_t* newpos, int64_t n, int32_t* ocol, double* oval) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p < n && keep[p]) {
+    ocol[newpos[p]] = col[p];
+    oval[newpos[p]] = val[p];
+  }
+}
+__global__ void k_compact_rows(const int64_t* row_ptr, int64_t nrows, const int64_t* newpos,
+                               int64_t n, int64_t nnz_kept, int64_t* orow) {
   int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  unsigned long long c = 0;
-  for (int64_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
-    if (keep[p]) {
-      ocol[newpos[p]] = col[p];
-      oval[newpos[p]] = val[p];
-      ++c;
-    }
-  rowcnt[r] = c;
+  if (r > nrows) return;
+  const int64_t p = row_ptr[r];
+  orow[r] = p < n ? newpos[p] : nnz_kept;
 }
 
 __global__ void k_u8_to_u64(const uint8_t* a, int64_t n, unsigned long long* b) {
@@ -277,12 +282,10 @@ void coo_plan_apply(Ctx* ctx, const CooPlan& plan, const double* vals, bool zero
   out.col.alloc(ctx, nnz);
   out.val.alloc(ctx, nnz);
   out.row_ptr.alloc(ctx, plan.nrows + 1);
-  DBuf<unsigned long long> rowcnt(ctx, std::max<int64_t>(plan.nrows, 1));
-  k_compact<<<grid_for(plan.nrows), TPB, 0, ctx->stream>>>(plan.row_ptr.p, plan.nrows, plan.col.p,
-                                                           v.p, keep.p, pos.p, out.col.p,
-                                                           out.val.p, rowcnt.p);
-  exclusive_scan_u64_to_i64<int64_t>(ctx, rowcnt.p, out.row_ptr.p, plan.nrows);
-  k_set_last<<<1, 1, 0, ctx->stream>>>(out.row_ptr.p, plan.nrows, nnz);
+  k_compact_entries<<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.col.p, v.p, keep.p, pos.p,
+                                                                 plan.nnz, out.col.p, out.val.p);
+  k_compact_rows<<<grid_for(plan.nrows + 1), TPB, 0, ctx->stream>>>(
+      plan.row_ptr.p, plan.nrows, pos.p, plan.nnz, nnz, out.row_ptr.p);
   FEEC_HIP(hipGetLastError());
   ctx->sync();
 }
@@ -658,12 +661,11 @@ bool assemble_compacted(Ctx* ctx, const CooPlan& plan, const double* vals, Compa
       k_u8_to_u64<<<grid_for(ns), TPB, 0, ctx->stream>>>(keep.p, ns, k64.p);
       exclusive_scan_u64_to_i64<int64_t>(ctx, k64.p, cm.pos.p, ns);
     }
-    DBuf<unsigned long long> rowcnt(ctx, std::max<int64_t>(plan.nrows, 1));
-    k_compact<<<grid_for(plan.nrows), TPB, 0, ctx->stream>>>(plan.row_ptr.p, plan.nrows,
-                                                             plan.col.p, v.p, keep.p, cm.pos.p,
-                                                             A.col.p, A.val.p, rowcnt.p);
-    exclusive_scan_u64_to_i64<int64_t>(ctx, rowcnt.p, A.row_ptr.p, plan.nrows);
-    k_set_last<<<1, 1, 0, ctx->stream>>>(A.row_ptr.p, plan.nrows, A.nnz);
+    if (ns)
+      k_compact_entries<<<grid_for(ns), TPB, 0, ctx->stream>>>(plan.col.p, v.p, keep.p, cm.pos.p,
+                                                               ns, A.col.p, A.val.p);
+    k_compact_rows<<<grid_for(plan.nrows + 1), TPB, 0, ctx->stream>>>(
+        plan.row_ptr.p, plan.nrows, cm.pos.p, ns, A.nnz, A.row_ptr.p);
     cm.keep = std::move(keep);
     cm.valid = true;
   } else if (ns) {

@@ -277,10 +277,12 @@ feec_status feec_simplex_quadrature_points(feec_geom *geom, int grade, int degre
 feec_status feec_source_load(feec_geom *geom, int grade, int degree, feec_frame frame,
                              const double *field, double *load, int on_device);
 /* l2_projection (fe.rs:87-123): M_k c = load, by Jacobi CG to rtol 1e-12 when every cell is
- * Riemannian, else by MINRES (the reference factorizes). report may be NULL. */
+ * Riemannian, else by MINRES (the reference factorizes). mass: the grade-k mass from
+ * feec_assemble(geom, FEEC_FULL, FEEC_MASS, grade), reused across calls, or NULL to assemble
+ * it inside the call as the reference does. report may be NULL. */
 feec_status feec_l2_projection(feec_geom *geom, int grade, int degree, feec_frame frame,
-                               const double *field, double *cochain, feec_report *report,
-                               int on_device);
+                               const double *field, const feec_csr *mass, double *cochain,
+                               feec_report *report, int on_device);
 /* WhitneyInterpolant::eval (derham/src/interpolate/interpolant.rs:55-62): the reference-frame
  * value of W c at every cell's nodes of `degree`, [N_dim][np][C(dim,k)]. */
 feec_status feec_whitney_at_nodes(feec_geom *geom, int grade, int degree, const double *cochain,

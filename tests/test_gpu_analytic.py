@@ -88,6 +88,9 @@ def test_l2_projection_reproduces_whitney_forms(d, n):
         x, rep = fa.l2_projection(geo, k, vals, 3)
         assert rep.converged and np.abs(x - c).max() < 1e-9, (k, rep.as_dict())
         assert np.abs(x - o.l2_projection(owc, k, 3, vals)).max() < 1e-9
+        # a reused mass gives the same bits
+        xm, _ = fa.l2_projection(geo, k, vals, 3, mass=fa.WhitneyComplex(geo).mass(k))
+        np.testing.assert_array_equal(xm, x)
 
 
 @pytest.mark.gpu
@@ -102,6 +105,8 @@ def test_l2_projection_on_a_lorentzian_mesh_uses_minres():
         x, rep = fa.l2_projection(geo, k, vals, 3)
         assert rep.converged and np.abs(x - c).max() < 1e-8, (k, rep.as_dict())
         assert np.abs(x - o.l2_projection(owc, k, 3, vals)).max() < 1e-8
+        xm, _ = fa.l2_projection(geo, k, vals, 3, mass=fa.WhitneyComplex(geo).mass(k))
+        np.testing.assert_array_equal(xm, x)
 
 
 @pytest.mark.gpu
@@ -146,6 +151,9 @@ def test_argument_errors():
         fa.source_load(geo, 3, vals, 1)
     with pytest.raises(fa.FeecError):
         fa.source_load(geo, 1, vals, 9)
+    # a mass of the wrong grade is refused
+    with pytest.raises(fa.FeecError):
+        fa.l2_projection(geo, 1, vals, 1, mass=fa.WhitneyComplex(geo).mass(0))
     # a lengths-only geometry takes reference-frame fields
     assert np.array_equal(fa.source_load(geo_l, 1, vals, 1), fa.source_load(geo, 1, vals, 1))
 
