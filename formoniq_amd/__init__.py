@@ -49,6 +49,8 @@ __all__ = [
     "PC_MULTIGRID", "FRAME_REFERENCE", "FRAME_AMBIENT", "SimplexQuadRule",
     "cell_quadrature_points", "simplex_quadrature_points", "SourceForm", "source_load",
     "l2_projection", "whitney_at_nodes", "fe_l2_error", "derham_map", "DeviceArray",
+    "gmsh2coord_cells", "load_cbor_cells", "load_cbor_lengths", "save_cbor_cells",
+    "save_cbor_lengths",
 ]
 
 
@@ -366,6 +368,63 @@ def derham_map(geometry: "MeshLengthsSq", grade, values, degree=1):
     out, oaddr = _array_out(dev, (ns,))
     check(load().feec_derham_map(geometry._h, grade, degree, addr, oaddr, dev))
     return out
+
+
+# ---- request ingestion: gmsh and CBOR ------------------------------------------------
+def gmsh2coord_cells(data: bytes):
+    """gmsh2coord_cells_ordered (regge/src/io/gmsh.rs:84-175) on MSH 4.1 ASCII bytes:
+    (dim, cells [ncells, dim+1] as the file words them, renumbered; coords [nverts, 2 or 3])."""
+    data = bytes(data)
+    d, cd, nv, nc = C.c_int(), C.c_int(), C.c_int64(), C.c_int64()
+    check(load().feec_gmsh_read(data, len(data), C.byref(d), C.byref(cd), C.byref(nv),
+                                C.byref(nc), None, None))
+    cells = np.zeros((nc.value, d.value + 1), np.int64)
+    coords = np.zeros((nv.value, cd.value))
+    check(load().feec_gmsh_read(data, len(data), C.byref(d), C.byref(cd), C.byref(nv),
+                                C.byref(nc), _ptr(cells, _lib.i64p), _ptr(coords, _lib.f64p)))
+    return d.value, cells, coords
+
+
+def load_cbor_cells(data: bytes):
+    """Complex::load (complex.rs:91-99) from CBOR bytes: (dim, cells [ncells, dim+1])."""
+    data = bytes(data)
+    d, nc = C.c_int(), C.c_int64()
+    check(load().feec_cbor_read_cells(data, len(data), C.byref(d), C.byref(nc), None))
+    cells = np.zeros((nc.value, d.value + 1), np.int64)
+    check(load().feec_cbor_read_cells(data, len(data), C.byref(d), C.byref(nc),
+                                      _ptr(cells, _lib.i64p)))
+    return d.value, cells
+
+
+def load_cbor_lengths(data: bytes):
+    """MeshLengthsSq::load (lengths/mesh.rs:244-251) from CBOR bytes: the squared lengths."""
+    data = bytes(data)
+    n = C.c_int64()
+    check(load().feec_cbor_read_lengths(data, len(data), C.byref(n), None))
+    out = np.zeros(n.value)
+    check(load().feec_cbor_read_lengths(data, len(data), C.byref(n), _ptr(out, _lib.f64p)))
+    return out
+
+
+def _cbor_out(fn, *args):
+    n = C.c_int64(0)
+    check(fn(*args, None, C.byref(n)))
+    buf = C.create_string_buffer(n.value)
+    check(fn(*args, buf, C.byref(n)))
+    return buf.raw[: n.value]
+
+
+def save_cbor_cells(cells):
+    """Complex::save: the CBOR of the cell skeleton (cells [ncells, dim+1])."""
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    return _cbor_out(load().feec_cbor_write_cells, cells.shape[1] - 1, cells.shape[0],
+                     _ptr(cells, _lib.i64p))
+
+
+def save_cbor_lengths(lengths_sq):
+    """MeshLengthsSq::save: the CBOR of a squared-length vector."""
+    v = np.ascontiguousarray(lengths_sq, dtype=np.float64)
+    return _cbor_out(load().feec_cbor_write_lengths, v.size, _ptr(v, _lib.f64p))
 
 
 def normal_vector(n, seed):

@@ -130,6 +130,12 @@ SIGNATURES = {
     "feec_whitney_at_nodes": (S, [vp, C.c_int, C.c_int, vp, vp, C.c_int]),
     "feec_l2_error": (S, [vp, C.c_int, vp, C.c_int, vp, f64p, C.c_int]),
     "feec_derham_map": (S, [vp, C.c_int, C.c_int, vp, vp, C.c_int]),
+    "feec_gmsh_read": (S, [C.c_char_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int), i64p, i64p,
+                           i64p, f64p]),
+    "feec_cbor_read_cells": (S, [C.c_char_p, C.c_int64, C.POINTER(C.c_int), i64p, i64p]),
+    "feec_cbor_read_lengths": (S, [C.c_char_p, C.c_int64, i64p, f64p]),
+    "feec_cbor_write_cells": (S, [C.c_int, C.c_int64, i64p, C.c_char_p, i64p]),
+    "feec_cbor_write_lengths": (S, [C.c_int64, f64p, C.c_char_p, i64p]),
     "feec_problem_solution": (S, [vp, f64p, f64p, f64p]),
     "feec_problem_sizes": (S, [vp, i64p, i64p]),
     "feec_problem_source_buffer": (S, [vp, C.POINTER(vp)]),

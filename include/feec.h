@@ -297,6 +297,33 @@ feec_status feec_l2_error(feec_geom *geom, int grade, const double *cochain, fee
 feec_status feec_derham_map(feec_geom *geom, int grade, int degree, const double *field,
                             double *cochain, int on_device);
 
+/* ---- request ingestion: gmsh and CBOR --------------------------------------------------
+ * Host functions on byte buffers. Each reader is called twice: with the output arrays NULL
+ * it reports the sizes, then it fills them. A file that is not what it claims is FEEC_E_ARG
+ * with the reason in feec_last_error(), never a crash. */
+/* gmsh2coord_cells_ordered (regge/src/io/gmsh.rs:84-175): MSH 4.1 ASCII. dim: the highest
+ * simplex dimension present (0..3; its elements are the cells). cells [ncells][dim+1]: each
+ * element's nodes in the order the file lists them, renumbered to the nodes the cells use by
+ * increasing tag (node tag t is node t-1 of the file's node list, as the reference reads it).
+ * coords [nverts][coord_dim]: coord_dim 2 when z is 0 on every node, else 3. Binary MSH and
+ * parametric node blocks are FEEC_E_UNSUPPORTED. */
+feec_status feec_gmsh_read(const char *bytes, int64_t len, int *dim, int *coord_dim,
+                           int64_t *nverts, int64_t *ncells, int64_t *cells, double *coords);
+/* Complex::load (simplicial/src/topology/complex.rs:91-99): the CBOR of the cell skeleton,
+ * an array of {"vertices": [...]} (serde through ciborium 0.2.2). cells [ncells][dim+1]. */
+feec_status feec_cbor_read_cells(const uint8_t *bytes, int64_t len, int *dim, int64_t *ncells,
+                                 int64_t *cells);
+/* MeshLengthsSq::load (regge/src/lengths/mesh.rs:244-251): {"vector": {"data": [...],
+ * "nrows": n, "ncols": null}} (a plain array of numbers as the vector is read too). */
+feec_status feec_cbor_read_lengths(const uint8_t *bytes, int64_t len, int64_t *nedges,
+                                   double *lengths_sq);
+/* The writers of the same encodings (floats in the shortest exact IEEE width, as ciborium
+ * writes them). *len: the buffer's size in, the encoding's size out; out NULL to query. */
+feec_status feec_cbor_write_cells(int dim, int64_t ncells, const int64_t *cells, uint8_t *out,
+                                  int64_t *len);
+feec_status feec_cbor_write_lengths(int64_t nedges, const double *lengths_sq, uint8_t *out,
+                                    int64_t *len);
+
 /* ---- problems ------------------------------------------------------------ */
 /* b_k and an M_k-orthonormal harmonic basis [ndofs][count] (basis may be NULL).
  * Closed forms on a ball (full k=0: the constant; relative k=n: M_n^-1 times the

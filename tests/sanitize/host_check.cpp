@@ -117,7 +117,78 @@ static void tridiagonal() {
   EXPECT(std::fabs(feec::tridiagonal_lmax(dg, off) - 3.0) < 1e-12);
 }
 
+// The ingestion readers on valid files, on every truncation of them and on
+// byte-flipped copies: each call returns a status, never reads out of bounds.
+static void ingestion() {
+  const char* square =
+      "$MeshFormat\n4.1 0 8\n$EndMeshFormat\n$Nodes\n1 4 1 4\n2 1 0 4\n1\n2\n3\n4\n"
+      "0 0 0\n1 0 0\n0 1 0\n1 1 0\n$EndNodes\n$Elements\n1 2 1 2\n2 1 2 2\n1 1 2 3\n"
+      "2 2 4 3\n$EndElements\n";
+  const int64_t n = (int64_t)std::strlen(square);
+  int dim = -1, cd = -1;
+  int64_t nv = 0, nc = 0;
+  EXPECT(feec_gmsh_read(square, n, &dim, &cd, &nv, &nc, nullptr, nullptr) == FEEC_OK);
+  EXPECT(dim == 2 && cd == 2 && nv == 4 && nc == 2);
+  std::vector<int64_t> cells(nc * (dim + 1));
+  std::vector<double> coords(nv * cd);
+  EXPECT(feec_gmsh_read(square, n, &dim, &cd, &nv, &nc, cells.data(), coords.data()) == FEEC_OK);
+  EXPECT(cells[3] == 1 && cells[4] == 3 && cells[5] == 2);
+  for (int64_t cut = 0; cut < n; ++cut) {
+    std::vector<char> part(square, square + cut);  // exactly cut bytes, no terminator
+    feec_gmsh_read(part.data(), cut, &dim, &cd, &nv, &nc, nullptr, nullptr);
+  }
+  uint64_t state = 12345;
+  auto next = [&] {
+    state = state * 6364136223846793005ull + 1442695040888963407ull;
+    return (uint32_t)(state >> 33);
+  };
+  for (int trial = 0; trial < 2000; ++trial) {
+    std::vector<char> m(square, square + n);
+    m[next() % n] = (char)(next() & 0xff);
+    if (feec_gmsh_read(m.data(), n, &dim, &cd, &nv, &nc, nullptr, nullptr) == FEEC_OK) {
+      std::vector<int64_t> c2(nc * (dim + 1));
+      std::vector<double> x2(nv * cd);
+      EXPECT(feec_gmsh_read(m.data(), n, &dim, &cd, &nv, &nc, c2.data(), x2.data()) == FEEC_OK);
+    }
+  }
+  // CBOR: a skeleton and a length vector written, then read whole, cut and flipped
+  const int64_t kc[6] = {0, 1, 2, 1, 2, 300};
+  int64_t len = 0;
+  EXPECT(feec_cbor_write_cells(2, 2, kc, nullptr, &len) == FEEC_OK);
+  std::vector<uint8_t> cb(len);
+  EXPECT(feec_cbor_write_cells(2, 2, kc, cb.data(), &len) == FEEC_OK);
+  const double lv[5] = {1.0, 0.1, 65504.0, 1e300, -0.0};
+  int64_t llen = 0;
+  EXPECT(feec_cbor_write_lengths(5, lv, nullptr, &llen) == FEEC_OK);
+  std::vector<uint8_t> lb(llen);
+  EXPECT(feec_cbor_write_lengths(5, lv, lb.data(), &llen) == FEEC_OK);
+  int64_t ne = 0;
+  std::vector<double> back(5);
+  EXPECT(feec_cbor_read_lengths(lb.data(), llen, &ne, back.data()) == FEEC_OK && ne == 5);
+  for (int i = 0; i < 5; ++i) EXPECT(back[i] == lv[i]);
+  for (const std::vector<uint8_t>* buf : {&cb, &lb}) {
+    for (size_t cut = 0; cut < buf->size(); ++cut) {
+      std::vector<uint8_t> part(buf->begin(), buf->begin() + cut);
+      feec_cbor_read_cells(part.data(), (int64_t)cut, &dim, &nc, nullptr);
+      feec_cbor_read_lengths(part.data(), (int64_t)cut, &ne, nullptr);
+    }
+    for (int trial = 0; trial < 2000; ++trial) {
+      std::vector<uint8_t> m(*buf);
+      m[next() % m.size()] = (uint8_t)next();
+      if (feec_cbor_read_cells(m.data(), (int64_t)m.size(), &dim, &nc, nullptr) == FEEC_OK) {
+        std::vector<int64_t> c2(nc * (dim + 1));
+        feec_cbor_read_cells(m.data(), (int64_t)m.size(), &dim, &nc, c2.data());
+      }
+      if (feec_cbor_read_lengths(m.data(), (int64_t)m.size(), &ne, nullptr) == FEEC_OK) {
+        std::vector<double> v2(ne);
+        feec_cbor_read_lengths(m.data(), (int64_t)m.size(), &ne, v2.data());
+      }
+    }
+  }
+}
+
 int main() {
+  ingestion();
   meshers();
   options_and_strings();
   dense_inverse();
