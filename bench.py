@@ -247,8 +247,10 @@ def run_analytic(args):
 
 
 def run_c3(args):
+    import formoniq_amd as fa
     n = args.n if args.n != 64 else 16
-    print(json.dumps({"config": "C3", **c3_measure(n, args.steps)}), flush=True)
+    print(json.dumps({"config": "C3", **c3_measure(n, args.steps), "tuning": fa.tune_table()}),
+          flush=True)
 
 
 def run_leapfrog(args):
@@ -683,6 +685,9 @@ def main():
                                     args.graph_iters, not args.no_warmup)
         except Exception as e:
             line["c5"] = {"error": repr(e)}
+    # the launch choices the autotuner made during this run (element kernels fused or
+    # split by grade, per size class; the batched grid): key, choice, each candidate's ms
+    line["tuning"] = fa.tune_table()
     if not args.no_cpu_baseline:
         try:
             cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -50,7 +50,7 @@ __all__ = [
     "cell_quadrature_points", "simplex_quadrature_points", "SourceForm", "source_load",
     "l2_projection", "whitney_at_nodes", "fe_l2_error", "derham_map", "DeviceArray",
     "gmsh2coord_cells", "load_cbor_cells", "load_cbor_lengths", "save_cbor_cells",
-    "save_cbor_lengths",
+    "save_cbor_lengths", "tune_mode", "tune_table",
 ]
 
 
@@ -425,6 +425,22 @@ def save_cbor_lengths(lengths_sq):
     """MeshLengthsSq::save: the CBOR of a squared-length vector."""
     v = np.ascontiguousarray(lengths_sq, dtype=np.float64)
     return _cbor_out(load().feec_cbor_write_lengths, v.size, _ptr(v, _lib.f64p))
+
+
+# ---- autotuning ----------------------------------------------------------------------
+def tune_mode(mode: int):
+    """0: tune bit-neutral launch choices by timing (default); c >= 1: always take
+    candidate c-1 (the element kernels: 1 fused, 2 split by grade)."""
+    check(load().feec_tune_mode(int(mode)))
+
+
+def tune_table():
+    """The per-device table of tuned launch choices: a list of text lines."""
+    n = C.c_int64(0)
+    check(load().feec_tune_table(None, C.byref(n)))
+    buf = C.create_string_buffer(n.value)
+    check(load().feec_tune_table(buf, C.byref(n)))
+    return [line for line in buf.value.decode().splitlines() if line]
 
 
 def normal_vector(n, seed):

@@ -443,6 +443,20 @@ __global__ void __launch_bounds__(WAVE) k_elmat_dirac(const int32_t* __restrict_
   dirac_cell<D>(c, o, nc);
 }
 
+// One grade's blocks of a KKT or Dirac launch on its own: the split variant of
+// the fused kernels above. Each entry is computed by the same operations as in
+// the fused kernel (bit-identical); a lane holds one grade's state only, so
+// more wavefronts fit per SIMD, at the cost of recomputing the cell metric
+// once per grade.
+template <int D, int K, unsigned KINDS>
+__global__ void __launch_bounds__(WAVE) k_elmat_grade(const int32_t* __restrict__ edges,
+                                                      const double* __restrict__ len, int64_t nc,
+                                                      Outs o, int* degenerate) {
+  CellIn<D> c;
+  cell_in<D>(edges, len, nc, blockIdx.x * (int64_t)WAVE, degenerate, c);
+  grade_blocks<D, K, KINDS>(c.cg.ginv, c.vol, o, nc, c.cell, c.active);
+}
+
 // Many meshes of mixed dimension and grade in one persistent launch: every
 // wavefront takes (mesh, 64-cell range) items from a global counter until the
 // queue is empty, so the grid drains whatever the mix of sizes.
@@ -588,10 +602,6 @@ unsigned wave_grid(int64_t nc) { return (unsigned)std::max<int64_t>(1, (nc + WAV
 
 void reset_degenerate(Ctx* ctx) { FEEC_HIP(hipMemsetAsync(ctx->dflag, 0, sizeof(int), ctx->stream)); }
 
-// Autotuned persistent grid (in wavefronts) per device, chosen by timing.
-std::mutex g_tune_mu;
-std::map<int, unsigned> g_tuned_waves;
-
 KktOuts kkt_outs(int k, double* const* outs) {
   KktOuts o{};
   if (k >= 1) o.g[0].p[0] = outs[0];
@@ -604,18 +614,70 @@ KktOuts kkt_outs(int k, double* const* outs) {
   return o;
 }
 
+// Element-kernel variants, tuned per (d, k, size) by tune_run: 0 fused (one
+// launch, every grade's blocks per lane), 1 split (one launch per grade).
+constexpr int kElementVariants = 2;
+
+template <int D, int k>
+void launch_kkt_variant(Ctx* ctx, const int32_t* edges, const double* len, int64_t nc,
+                        const KktOuts& o, int variant) {
+  const unsigned grid = wave_grid(nc);
+  if (variant == 0) {
+    k_elmat_kkt<D, k><<<grid, WAVE, 0, ctx->stream>>>(edges, len, nc, o, ctx->dflag);
+    return;
+  }
+  if constexpr (k >= 1) {
+    k_elmat_grade<D, k - 1, B_MASS><<<grid, WAVE, 0, ctx->stream>>>(edges, len, nc, o.g[0], ctx->dflag);
+    k_elmat_grade<D, k, B_MASS | B_DMASS | B_TEST | B_BOTH | B_DBOTH>
+        <<<grid, WAVE, 0, ctx->stream>>>(edges, len, nc, o.g[1], ctx->dflag);
+  } else {
+    k_elmat_grade<D, k, B_MASS | B_DMASS><<<grid, WAVE, 0, ctx->stream>>>(edges, len, nc, o.g[1],
+                                                                          ctx->dflag);
+  }
+  if constexpr (k + 1 <= D)
+    k_elmat_grade<D, k + 1, B_BOTH><<<grid, WAVE, 0, ctx->stream>>>(edges, len, nc, o.g[2], ctx->dflag);
+}
+
 template <int D, int k = 0>
 void launch_kkt(Ctx* ctx, const int32_t* edges, const double* len, int64_t nc, int grade,
                 const KktOuts& o) {
   if constexpr (k <= D) {
     if (grade == k) {
-      k_elmat_kkt<D, k><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, len, nc, o, ctx->dflag);
+      tune_run(ctx, "elmat_kkt", D, k, nc, kElementVariants, [&](int v) {
+        launch_kkt_variant<D, k>(ctx, edges, len, nc, o, v);
+        FEEC_HIP(hipGetLastError());
+      });
       return;
     }
     launch_kkt<D, k + 1>(ctx, edges, len, nc, grade, o);
   } else {
     fail(FEEC_E_ARG, "grade out of range for a KKT system");
   }
+}
+
+template <int D, int K = 0>
+void launch_dirac_split(Ctx* ctx, const int32_t* edges, const double* len, int64_t nc,
+                        const DiracOuts& o) {
+  if constexpr (K <= D) {
+    const unsigned grid = wave_grid(nc);
+    if constexpr (K == 0)
+      k_elmat_grade<D, 0, B_MASS><<<grid, WAVE, 0, ctx->stream>>>(edges, len, nc, o.g[0], ctx->dflag);
+    else
+      k_elmat_grade<D, K, B_MASS | B_TRIAL><<<grid, WAVE, 0, ctx->stream>>>(edges, len, nc, o.g[K],
+                                                                             ctx->dflag);
+    launch_dirac_split<D, K + 1>(ctx, edges, len, nc, o);
+  }
+}
+
+template <int D>
+void launch_dirac(Ctx* ctx, const int32_t* edges, const double* len, int64_t nc, const DiracOuts& o) {
+  tune_run(ctx, "elmat_dirac", D, -1, nc, kElementVariants, [&](int v) {
+    if (v == 0)
+      k_elmat_dirac<D><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, len, nc, o, ctx->dflag);
+    else
+      launch_dirac_split<D>(ctx, edges, len, nc, o);
+    FEEC_HIP(hipGetLastError());
+  });
 }
 
 }  // namespace
@@ -687,36 +749,16 @@ double kkt_element_blocks_batched(Ctx* ctx, const std::vector<BatchElementReques
                                                      ctx->dflag);
     FEEC_HIP(hipGetLastError());
   };
-  unsigned waves = 0;
-  {
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    auto it = g_tuned_waves.find(ctx->device);
-    if (it != g_tuned_waves.end()) waves = it->second;
-  }
+  // the persistent grid, in wavefronts, tuned per device and work size
+  static const unsigned kWaves[] = {1024, 2048, 4096, 8192, 16384};
+  const int c = tune_run(ctx, "elmat_batched_waves", -1, -1, items, 5,
+                         [&](int i) { launch(kWaves[i]); });
   hipEvent_t e0, e1;
   FEEC_HIP(hipEventCreate(&e0));
   FEEC_HIP(hipEventCreate(&e1));
   float best_ms = 0.f;
-  if (waves == 0) {
-    const unsigned cand[] = {1024, 2048, 4096, 8192, 16384};
-    float best = 1e30f;
-    for (unsigned c : cand) {
-      FEEC_HIP(hipEventRecord(e0, ctx->stream));
-      launch(c);
-      FEEC_HIP(hipEventRecord(e1, ctx->stream));
-      FEEC_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        waves = c;
-      }
-    }
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_tuned_waves[ctx->device] = waves;
-  }
   FEEC_HIP(hipEventRecord(e0, ctx->stream));
-  launch(waves);
+  launch(kWaves[c]);
   FEEC_HIP(hipEventRecord(e1, ctx->stream));
   FEEC_HIP(hipEventSynchronize(e1));
   FEEC_HIP(hipEventElapsedTime(&best_ms, e0, e1));
@@ -755,10 +797,10 @@ void dirac_element_matrices(Geom& g, double* out_dev, bool check) {
   reset_degenerate(ctx);
   const int32_t* edges = m.faces[1].p;
   switch (d) {
-    case 1: k_elmat_dirac<1><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, o, ctx->dflag); break;
-    case 2: k_elmat_dirac<2><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, o, ctx->dflag); break;
-    case 3: k_elmat_dirac<3><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, o, ctx->dflag); break;
-    case 4: k_elmat_dirac<4><<<wave_grid(nc), WAVE, 0, ctx->stream>>>(edges, g.lengths.p, nc, o, ctx->dflag); break;
+    case 1: launch_dirac<1>(ctx, edges, g.lengths.p, nc, o); break;
+    case 2: launch_dirac<2>(ctx, edges, g.lengths.p, nc, o); break;
+    case 3: launch_dirac<3>(ctx, edges, g.lengths.p, nc, o); break;
+    case 4: launch_dirac<4>(ctx, edges, g.lengths.p, nc, o); break;
   }
   FEEC_HIP(hipGetLastError());
   if (check) check_degenerate(ctx, ctx->dflag);

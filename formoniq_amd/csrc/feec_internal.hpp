@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -282,6 +283,13 @@ void form_contributions(Mesh& m, int bc, int rg, int cg, int r, int c, int64_t r
                         bool transpose, uint32_t base, bool negate, int32_t* rows, int32_t* cols,
                         uint32_t* srcs);
 int64_t ndofs(Mesh& m, int bc, int k);
+
+// tune.hip
+// Launches candidate `choice` of a bit-neutral launch choice through `launch`,
+// the choice taken from the per-device table, or found by timing every
+// candidate on the first call for (family, d, k, size class of n); returns it.
+int tune_run(Ctx* ctx, const char* family, int d, int k, int64_t n, int ncand,
+             const std::function<void(int)>& launch);
 
 // elmat.hip
 // Element matrices of one pairing, SoA [entry][cell].

@@ -324,6 +324,18 @@ feec_status feec_cbor_write_cells(int dim, int64_t ncells, const int64_t *cells,
 feec_status feec_cbor_write_lengths(int64_t nedges, const double *lengths_sq, uint8_t *out,
                                     int64_t *len);
 
+/* ---- autotuning (no counterpart in the reference) --------------------------------------
+ * Launch choices that change no result bit are tuned per device and per (kernel family, dim,
+ * grade, size class ceil(log2 n)) by timing every candidate on the first call that needs
+ * them; the table lives as long as the process. Tuned now: the element kernels of the KKT
+ * and Hodge-Dirac blocks fused in one launch or split into one launch per grade, and the
+ * persistent grid of the batched element launch. */
+/* mode 0: tune (default); mode c >= 1: always take candidate c-1 (tests, comparisons). */
+feec_status feec_tune_mode(int mode);
+/* The table as text, one line per entry: the key, the choice and each candidate's median
+ * time in ms. *len: the buffer's size in, the text's size with its NUL out; out NULL to query. */
+feec_status feec_tune_table(char *out, int64_t *len);
+
 /* ---- problems ------------------------------------------------------------ */
 /* b_k and an M_k-orthonormal harmonic basis [ndofs][count] (basis may be NULL).
  * Closed forms on a ball (full k=0: the constant; relative k=n: M_n^-1 times the
