@@ -180,6 +180,9 @@ def c3_measure(n=16, reps=3, sample=2048):
             worst = max(worst, float(np.abs(el[c] - want).max() / scale))
         del el
     bytes_per_cell = 8 * 10 + 4 * 10 + 8 * per
+    # the kernel writes 30x what it reads: read its rate against what plain
+    # streaming writes sustain on this device, beside the 8 TB/s peak
+    stream = topo.ctx.stream_rates(1 << 31)
     return {
         "workload": f"4D Lorentzian Hodge-Dirac element matrices, Kuhn {n}^4 ({nc} cells), axes 1..3 "
                     "jittered 0.02h seed 3, Minkowski (-,+,+,+)",
@@ -187,6 +190,9 @@ def c3_measure(n=16, reps=3, sample=2048):
         "seconds_per_launch": sec, "values_per_cell": per,
         "element_matrices_per_s": len(blocks) * nc / sec, "cells_per_s": nc / sec,
         "hbm_gbs": bytes_per_cell * nc / sec / 1e9,
+        "frac_of_peak": bytes_per_cell * nc / sec / 1e9 / HBM_PEAK_GBS,
+        "stream_rates_gbs": stream,
+        "frac_of_write_stream": bytes_per_cell * nc / sec / 1e9 / stream["write_gbs"],
         "max_rel_err_vs_oracle": worst, "sampled_cells": int(pick.size), "dtype": "f64",
     }
 

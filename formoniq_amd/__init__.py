@@ -99,6 +99,13 @@ class Context:
         check(load().feec_ctx_allocations(self._h, C.byref(c)))
         return c.value
 
+    def stream_rates(self, nbytes: int = 1 << 30):
+        """Sustained (write, read, copy) GB/s of plain streaming kernels on this
+        device (feec_ctx_stream_rates)."""
+        r = np.zeros(3)
+        check(load().feec_ctx_stream_rates(self._h, int(nbytes), _ptr(r, _lib.f64p)))
+        return {"write_gbs": r[0], "read_gbs": r[1], "copy_gbs": r[2]}
+
 
 _DEFAULT_CTX = None
 

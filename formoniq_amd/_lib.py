@@ -138,6 +138,7 @@ SIGNATURES = {
     "feec_cbor_write_lengths": (S, [C.c_int64, f64p, C.c_char_p, i64p]),
     "feec_tune_mode": (S, [C.c_int]),
     "feec_tune_table": (S, [C.c_char_p, i64p]),
+    "feec_ctx_stream_rates": (S, [vp, C.c_int64, f64p]),
     "feec_problem_solution": (S, [vp, f64p, f64p, f64p]),
     "feec_problem_sizes": (S, [vp, i64p, i64p]),
     "feec_problem_source_buffer": (S, [vp, C.POINTER(vp)]),

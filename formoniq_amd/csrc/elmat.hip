@@ -129,7 +129,9 @@ __host__ __device__ constexpr double pair_weight(int p, int q) {
 
 __device__ __forceinline__ void put(double* __restrict__ out, int64_t entry, int64_t nc, int64_t cell,
                                     bool active, double v) {
-  if (active) out[entry * nc + cell] = v;
+  // element values are written once and read later by another kernel: a
+  // non-temporal store streams them to HBM without allocating in the caches
+  if (active) __builtin_nontemporal_store(v, out + entry * nc + cell);
 }
 
 // The blocks KINDS of grade K for one cell. M is produced one row at a time

@@ -335,6 +335,11 @@ feec_status feec_tune_mode(int mode);
 /* The table as text, one line per entry: the key, the choice and each candidate's median
  * time in ms. *len: the buffer's size in, the text's size with its NUL out; out NULL to query. */
 feec_status feec_tune_table(char *out, int64_t *len);
+/* Sustained HBM rates of this device with plain streaming kernels over `bytes`-sized buffers
+ * (16 bytes per lane, non-temporal writes, best of 5): rates[0] write, rates[1] read,
+ * rates[2] copy (bytes read + written), in GB/s. The achievable ceiling beside the published
+ * 8 TB/s peak when reading a kernel's roofline fraction. */
+feec_status feec_ctx_stream_rates(feec_ctx *ctx, int64_t bytes, double rates[3]);
 
 /* ---- problems ------------------------------------------------------------ */
 /* b_k and an M_k-orthonormal harmonic basis [ndofs][count] (basis may be NULL).
