@@ -292,6 +292,42 @@ __global__ void __launch_bounds__(TPB) k_st_dense(const TP* __restrict__ C, cons
   }
 }
 
+// The pattern side of P's FP32 copy, one thread per row i of the
+// structurally symmetric P (sorted rows): for each entry (i, j) the position of
+// (j, i) in row j (binary search), and the 16-bit offset j - i. flags[0] is
+// set if a transpose entry is missing, flags[1] if an offset needs more than
+// 16 bits.
+__global__ void k_pc_pattern(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                             int64_t n, int32_t* __restrict__ tpos, int16_t* __restrict__ c16,
+                             int* __restrict__ flags) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool missing = false, wide = false;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+    const int64_t j = col[p];
+    int64_t lo = rp[j], hi = rp[j + 1];
+    while (lo < hi) {  // first position in row j with column >= i
+      const int64_t mid = (lo + hi) / 2;
+      if (col[mid] < i)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const bool found = lo < rp[j + 1] && col[lo] == i;
+    missing = missing || !found;
+    tpos[p] = found ? (int32_t)lo : 0;
+    const int64_t off = j - i;
+    wide = wide || off < -32767 || off > 32767;
+    c16[p] = (int16_t)(wide ? 0 : off);
+  }
+  if (missing) atomicOr(&flags[0], 1);
+  if (wide) atomicOr(&flags[1], 1);
+}
+
+__global__ void k_zero_i32(int* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // operators of the hierarchy (FP64)
 // ---------------------------------------------------------------------------
@@ -1048,15 +1084,21 @@ HostCsr vector_nodal(Mesh& m, const std::vector<double>& coords, int j, const in
       }
       w[I] = factor * det;
     }
-    // entries sorted by column
-    std::vector<std::pair<int32_t, int>> ents;
+    // entries sorted by column (at most 5: insertion sort in place)
+    int32_t ec[5];
+    int ne = 0;
     for (int p = 0; p <= j; ++p) {
       const int32_t c = colmap ? colmap[v[p]] : (int32_t)v[p];
-      if (c >= 0) ents.push_back({c, p});
+      if (c < 0) continue;
+      int q = ne++;
+      while (q > 0 && ec[q - 1] > c) {
+        ec[q] = ec[q - 1];
+        --q;
+      }
+      ec[q] = c;
     }
-    std::sort(ents.begin(), ents.end());
-    for (auto& e : ents) {
-      Pi.col.push_back(e.first);
+    for (int e = 0; e < ne; ++e) {
+      Pi.col.push_back(ec[e]);
       for (int I = 0; I < C; ++I) Pi.val.push_back(w[I]);
     }
     Pi.rp[r + 1] = (int64_t)Pi.col.size();
@@ -1098,17 +1140,30 @@ HostCsr host_coboundary(Mesh& m, int j, const int32_t* rowmap, int64_t nrows, co
     const int32_t r = rowmap ? rowmap[s] : (int32_t)s;
     if (r >= 0) order[r] = s;
   }
+  o.col.reserve(f.col.size());
+  o.val.reserve(f.col.size());
   for (int64_t r = 0; r < nrows; ++r) {
     const int64_t s = order[r];
-    std::vector<std::pair<int32_t, double>> ents;
+    // a j-simplex has j + 1 <= 5 facets: insertion sort in place
+    int32_t ec[5];
+    double ev[5];
+    int ne = 0;
     for (int64_t p = f.rp[s]; p < f.rp[s + 1]; ++p) {
       const int32_t c = colmap ? colmap[f.col[p]] : f.col[p];
-      if (c >= 0) ents.push_back({c, f.val[p]});
+      if (c < 0) continue;
+      FEEC_REQUIRE(ne < 5, FEEC_E_DEVICE, "a coboundary row with more than 5 entries");
+      int q = ne++;
+      while (q > 0 && ec[q - 1] > c) {
+        ec[q] = ec[q - 1];
+        ev[q] = ev[q - 1];
+        --q;
+      }
+      ec[q] = c;
+      ev[q] = f.val[p];
     }
-    std::sort(ents.begin(), ents.end());
-    for (auto& e : ents) {
-      o.col.push_back(e.first);
-      o.val.push_back(e.second);
+    for (int e = 0; e < ne; ++e) {
+      o.col.push_back(ec[e]);
+      o.val.push_back(ev[e]);
     }
     o.rp[r + 1] = (int64_t)o.col.size();
   }
@@ -1286,32 +1341,21 @@ void bind_pc_matrix(Hx& h, const KktSystem& s) {
 void build_pc_single(Hx& h, const KktSystem& s) {
   Ctx* ctx = h.ctx;
   const int64_t n = s.P.nrows, nnz = s.P.nnz;
-  std::vector<int64_t> rp(n + 1);
-  std::vector<int32_t> col(nnz);
-  copy_to_host(ctx, rp.data(), s.P.row_ptr.p, (n + 1) * 8);
-  copy_to_host(ctx, col.data(), s.P.col.p, nnz * 4);
-  std::vector<int64_t> at(rp.begin(), rp.end() - 1);
-  std::vector<int32_t> tpos(nnz);
-  bool fit16 = true;
-  for (int64_t i = 0; i < n; ++i)
-    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-      const int64_t q = at[col[p]]++;
-      FEEC_REQUIRE(q < rp[col[p] + 1] && col[q] == i, FEEC_E_DEVICE,
-                   "preconditioner matrix is not structurally symmetric");
-      tpos[p] = (int32_t)q;
-      const int64_t off = (int64_t)col[p] - i;
-      fit16 = fit16 && off >= -32767 && off <= 32767;
-    }
   h.P_tpos.alloc(ctx, std::max<int64_t>(nnz, 1));
   h.P32.alloc(ctx, std::max<int64_t>(nnz, 1));
-  if (nnz) copy_to_device(ctx, h.P_tpos.p, tpos.data(), nnz * 4);
-  if (fit16 && nnz) {
-    std::vector<int16_t> c16(nnz);
-    for (int64_t i = 0; i < n; ++i)
-      for (int64_t p = rp[i]; p < rp[i + 1]; ++p) c16[p] = (int16_t)(col[p] - i);
-    h.P_col16.alloc(ctx, nnz);
-    copy_to_device(ctx, h.P_col16.p, c16.data(), nnz * 2);
-  }
+  h.P_col16.reset();
+  if (!nnz) return;
+  DBuf<int16_t> c16(ctx, nnz);
+  // flags[0]: some entry has no transpose; flags[1]: some offset is beyond 16 bits
+  DBuf<int> flags(ctx, 2);
+  k_zero_i32<<<1, 64, 0, ctx->stream>>>(flags.p, 2);
+  k_pc_pattern<<<grid_for(n), TPB, 0, ctx->stream>>>(s.P.row_ptr.p, s.P.col.p, n, h.P_tpos.p, c16.p,
+                                                      flags.p);
+  FEEC_HIP(hipGetLastError());
+  int hf[2] = {0, 0};
+  copy_to_host(ctx, hf, flags.p, sizeof(hf));
+  FEEC_REQUIRE(hf[0] == 0, FEEC_E_DEVICE, "preconditioner matrix is not structurally symmetric");
+  if (hf[1] == 0) h.P_col16 = std::move(c16);
 }
 }  // namespace
 
