@@ -654,6 +654,8 @@ struct RunDerham {
 // ---------------------------------------------------------------------------
 int64_t ncomp(int d, int k) { return binom(d, k); }
 
+}  // namespace
+
 // the geometry's coordinates on the device, uploaded on first use
 const double* device_coords(Geom& g) {
   Mesh& m = *g.mesh;
@@ -666,6 +668,8 @@ const double* device_coords(Geom& g) {
   }
   return g.dcoords.p;
 }
+
+namespace {
 
 // An input array: the caller's device pointer, or a device copy of its host array.
 struct In {

@@ -394,6 +394,11 @@ void csr_block(Ctx* ctx, int nb, const BlockRef* blocks, const int64_t* row_size
   coo_plan_apply(ctx, plan, vals.p, false, out);
 }
 
+// Exclusive prefix sum of n counts (n >= 1) into out[0..n).
+void exclusive_scan_counts(Ctx* ctx, const unsigned long long* in, int64_t* out, int64_t n) {
+  exclusive_scan_u64_to_i64<int64_t>(ctx, in, out, n);
+}
+
 void csr_transpose(const Csr& a, Csr& out) {
   BlockRef b{&a, 1.0, true};
   int64_t r = a.ncols, c = a.nrows;

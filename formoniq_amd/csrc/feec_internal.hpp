@@ -255,6 +255,7 @@ void coo_plan_build(Ctx* ctx, CooPlan& plan, int64_t nrows, int64_t ncols, int64
 void coo_plan_apply(Ctx* ctx, const CooPlan& plan, const double* vals, bool zero_drop, Csr& out);
 // Sparse utilities on device CSRs (nalgebra-sparse semantics).
 void csr_transpose(const Csr& a, Csr& out);
+void exclusive_scan_counts(Ctx* ctx, const unsigned long long* in, int64_t* out, int64_t n);
 void csr_add(const Csr& a, const Csr& b, Csr& out);
 void csr_mul(const Csr& a, const Csr& b, Csr& out);
 void csr_select(const Csr& a, const int32_t* row_map, int64_t nrows, const int32_t* col_map,
@@ -283,6 +284,11 @@ void form_contributions(Mesh& m, int bc, int rg, int cg, int r, int c, int64_t r
                         bool transpose, uint32_t base, bool negate, int32_t* rows, int32_t* cols,
                         uint32_t* srcs);
 int64_t ndofs(Mesh& m, int bc, int k);
+
+// analytic.hip
+// The geometry's vertex coordinates on the device, uploaded on first use
+// (takes the mesh's lock; a geometry without coordinates is an argument error).
+const double* device_coords(Geom& g);
 
 // tune.hip
 // Launches candidate `choice` of a bit-neutral launch choice through `launch`,

@@ -66,6 +66,7 @@
 #include <vector>
 
 #include "enqueue.hpp"
+#include "cell_geometry.hpp"
 #include "feec_internal.hpp"
 
 namespace feec {
@@ -1027,147 +1028,301 @@ void to32(Ctx* ctx, const double* a, int64_t n, DBuf<float>& out) {
   FEEC_HIP(hipGetLastError());
 }
 
-// Vector-nodal interpolation of grade j (formoniq/src/hx.rs:86-112): row of
-// the j-simplex sigma, entry per vertex a of sigma with C = C(d, j) values,
-// value I = (1/(j+1)!) det(E_I), E the edge vectors of sigma from its first
-// vertex and E_I their components I (colex j-subsets of the d axes).
-// rowmap: the block's DOF of each j-simplex (-1: none; null: identity);
-// colmap: the column of each vertex (-1: dropped; null: the vertex itself).
-HostCsr vector_nodal(Mesh& m, const std::vector<double>& coords, int j, const int32_t* rowmap,
-                     const int32_t* colmap, int64_t nrows, int64_t ncols) {
-  const int d = m.d;
-  const int64_t ns = m.n[j];
-  std::vector<int64_t> sv(ns * (j + 1));
-  if (feec_mesh_skeleton(static_cast<feec_mesh*>(&m), j, sv.data()) != FEEC_OK)
-    fail(FEEC_E_DEVICE, feec_last_error());
-  std::vector<std::vector<int>> subsets;
-  for (unsigned mask = 0; mask < (1u << d); ++mask)
-    if (__builtin_popcount(mask) == j) {
-      std::vector<int> s;
-      for (int a = 0; a < d; ++a)
-        if (mask & (1u << a)) s.push_back(a);
-      subsets.push_back(s);
+// ---------------------------------------------------------------------------
+// transfer operators of the auxiliary spaces, built on the device
+// ---------------------------------------------------------------------------
+// The ascending vertices of the grade-J simplex s (J >= 1), decoded from its
+// cell slot rep[s] = cell * C(d+1, J+1) + local face (faces in mask order).
+template <int J>
+__device__ inline void simplex_verts(const int32_t* cells, const int64_t* rep, int d, int64_t s,
+                                     int32_t (&v)[J + 1]) {
+  const int nv = d + 1;
+  if (J == d) {
+    for (int p = 0; p <= J; ++p) v[p] = cells[s * nv + p];
+    return;
+  }
+  const int nf = binom(nv, J + 1);
+  const int64_t slot = rep[s];
+  const unsigned msk = nth_subset(nv, J + 1, (int)(slot % nf));
+  const int64_t cell = slot / nf;
+  for (int p = 0; p <= J; ++p) v[p] = cells[cell * nv + nth_elem(msk, p)];
+}
+
+__global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// order[map[s]] = s: the simplex behind each block row (map < 0: no row)
+__global__ void k_invert_map(const int32_t* map, int64_t ns, int64_t nrows, int32_t* order) {
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const int32_t r = map[s];
+  if (r >= 0 && r < nrows) order[r] = (int32_t)s;
+}
+
+// the box column of each vertex: itself when it carries a DOF, else -1
+__global__ void k_live_col(const int32_t* dof0, int64_t nv, int32_t* live) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v < nv) live[v] = dof0[v] >= 0 ? (int32_t)v : -1;
+}
+
+// entries of each vector-nodal row (its vertices that keep a column); the
+// extra entry at nrows is 0 so that the exclusive scan ends on the total
+template <int J>
+__global__ void k_vnodal_count(const int32_t* cells, const int64_t* rep, int d, const int32_t* order,
+                               const int32_t* colmap, int64_t nrows, unsigned long long* cnt) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r > nrows) return;
+  const int64_t s = r == nrows ? -1 : (order ? order[r] : r);
+  unsigned long long c = 0;
+  if (s >= 0) {
+    int32_t v[J + 1];
+    simplex_verts<J>(cells, rep, d, s, v);
+    for (int p = 0; p <= J; ++p) c += (colmap ? colmap[v[p]] : v[p]) >= 0;
+  }
+  cnt[r] = c;
+}
+
+// Vector-nodal interpolation of grade J (formoniq/src/hx.rs:86-112): the row
+// of the J-simplex sigma has an entry per vertex of sigma with C = C(d, J)
+// values, value I = (1/(J+1)!) det(E_I), E the edge vectors of sigma from its
+// first vertex and E_I their components I (colex J-subsets of the d axes);
+// entries sorted by column.
+template <int J>
+__global__ void k_vnodal_fill(const int32_t* cells, const int64_t* rep, int d, const int32_t* order,
+                              const int32_t* colmap, int64_t nrows, const double* X, double factor,
+                              const int64_t* rp, int32_t* col, double* val) {
+#pragma clang fp contract(off)
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int64_t s = order ? order[r] : r;
+  if (s < 0) return;
+  int32_t v[J + 1];
+  simplex_verts<J>(cells, rep, d, s, v);
+  double E[J][4];
+  for (int i = 0; i < J; ++i)
+    for (int a = 0; a < d; ++a) E[i][a] = X[(int64_t)v[i + 1] * d + a] - X[(int64_t)v[0] * d + a];
+  double w[6];
+  int C = 0;
+  for (unsigned mask = 1; mask < (1u << d); ++mask) {  // colex order of subsets = increasing mask
+    if (popc(mask) != J) continue;
+    const int a0 = nth_elem(mask, 0);
+    double det;
+    if constexpr (J == 1) {
+      det = E[0][a0];
+    } else {
+      const int a1 = nth_elem(mask, 1);
+      det = E[0][a0] * E[1][a1] - E[0][a1] * E[1][a0];
     }
-  // colex order of subsets = increasing mask
-  const int C = (int)subsets.size();
+    w[C++] = factor * det;
+  }
+  int32_t ec[J + 1];
+  int ne = 0;
+  for (int p = 0; p <= J; ++p) {
+    const int32_t c = colmap ? colmap[v[p]] : v[p];
+    if (c < 0) continue;
+    int q = ne++;
+    while (q > 0 && ec[q - 1] > c) {
+      ec[q] = ec[q - 1];
+      --q;
+    }
+    ec[q] = c;
+  }
+  const int64_t p0 = rp[r];
+  for (int e = 0; e < ne; ++e) {
+    col[p0 + e] = ec[e];
+    for (int I = 0; I < C; ++I) val[(p0 + e) * C + I] = w[I];
+  }
+}
+
+// COO of the transpose: entry p of row r becomes (col[p], r) from source p
+__global__ void k_transpose_coo(const int64_t* rp, const int32_t* col, int64_t nrows, int32_t* orow,
+                                int32_t* ocol, uint32_t* osrc) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  for (int64_t p = rp[r]; p < rp[r + 1]; ++p) {
+    orow[p] = col[p];
+    ocol[p] = (int32_t)r;
+    osrc[p] = (uint32_t)p;
+  }
+}
+
+// the C values of each (duplicate-free) plan entry from its one source
+__global__ void k_gather_entries(const int64_t* seg, const uint32_t* src, const double* in, int C,
+                                 int64_t nnz, double* out) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= nnz * C) return;
+  const int64_t e = q / C;
+  out[q] = in[(int64_t)src[seg[e]] * C + (q - e * C)];
+}
+
+// [a | b] on one pattern: a's C_a values, then b's C_b values, per entry;
+// bad[0] set when the patterns differ
+__global__ void k_interleave(const int64_t* arp, const int32_t* acol, const double* av, int ca,
+                             const int64_t* brp, const int32_t* bcol, const double* bv, int cb,
+                             int64_t nrows, double* ov, int* bad) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  bool same = arp[r] == brp[r] && arp[r + 1] == brp[r + 1];
+  const int co = ca + cb;
+  for (int64_t p = arp[r]; p < arp[r + 1]; ++p) {
+    same = same && acol[p] == bcol[p];
+    for (int v = 0; v < ca; ++v) ov[p * co + v] = av[p * ca + v];
+    for (int v = 0; v < cb; ++v) ov[p * co + ca + v] = bv[p * cb + v];
+  }
+  if (!same) atomicOr(bad, 1);
+}
+
+// The matrix-free edge scatter: the two ends of each edge row and Pi_1's row
+// values (1/2!) (x(v1) - x(v0)); bad[0] set unless every D_0 entry of the row
+// is -1 at the smaller end or +1 at the larger one.
+__global__ void k_edge_ends(const int32_t* cells, const int64_t* rep, int d, const int32_t* order,
+                            int64_t nrows, const int64_t* drp, const int32_t* dcol, const double* dval,
+                            const double* X, int32_t* ev, double* eh, int* bad) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int64_t s = order ? order[r] : r;
+  if (s < 0) {
+    atomicOr(bad, 1);
+    return;
+  }
+  int32_t v[2];
+  simplex_verts<1>(cells, rep, d, s, v);
+  bool ok = v[0] < v[1];
+  for (int64_t p = drp[r]; p < drp[r + 1]; ++p)
+    ok = ok && ((dcol[p] == v[0] && dval[p] == -1.0) || (dcol[p] == v[1] && dval[p] == 1.0));
+  if (!ok) atomicOr(bad, 1);
+  ev[2 * r] = v[0];
+  ev[2 * r + 1] = v[1];
+  for (int c = 0; c < d; ++c)
+    eh[r * d + c] = (1.0 / 2.0) * (X[(int64_t)v[1] * d + c] - X[(int64_t)v[0] * d + c]);
+}
+
+// avg_row and the FP32 copy of a finished transfer operator
+void finish_x(Ctx* ctx, XCsr& out) {
+  Csr& c = out.m;
+  if (!c.col.p) c.col.alloc(ctx, 1);
+  if (!c.val.p) c.val.alloc(ctx, 1);
+  const int64_t nv = c.nnz * out.C;
+  out.avg_row = c.nrows ? (double)c.nnz / (double)c.nrows : 0.0;
+  to32(ctx, c.val.p, nv, out.v32);
+}
+
+// the simplex behind each of the nrows block rows of grade j (null: identity)
+void row_order(Mesh& m, const int32_t* map, int j, int64_t nrows, DBuf<int32_t>& order) {
+  Ctx* ctx = m.ctx;
+  order.reset();
+  if (!map) return;
+  order.alloc(ctx, std::max<int64_t>(nrows, 1));
+  if (nrows) k_fill_i32<<<grid_for(nrows), TPB, 0, ctx->stream>>>(order.p, nrows, -1);
+  if (m.n[j]) k_invert_map<<<grid_for(m.n[j]), TPB, 0, ctx->stream>>>(map, m.n[j], nrows, order.p);
+  FEEC_HIP(hipGetLastError());
+}
+
+// Pi_j on the block rows `order` (null: all j-simplices), columns colmap (null: the vertices).
+void vector_nodal_dev(Mesh& m, const double* X, int j, const int32_t* order, const int32_t* colmap,
+                      int64_t nrows, int64_t ncols, XCsr& out) {
+  Ctx* ctx = m.ctx;
+  const int d = m.d;
+  FEEC_REQUIRE(j == 1 || j == 2, FEEC_E_DEVICE, "vector-nodal interpolation of an unsupported grade");
+  const int C = binom(d, j);
+  Csr& c = out.m;
+  c.ctx = ctx;
+  c.nrows = nrows;
+  c.ncols = ncols;
+  c.row_ptr.alloc(ctx, nrows + 1);
+  DBuf<unsigned long long> cnt(ctx, nrows + 1);
+  const int32_t* cells = m.faces[0].p;
+  const int64_t* rep = m.rep[j].p;
+  const unsigned gc = grid_for(nrows + 1), gf = grid_for(std::max<int64_t>(nrows, 1));
+  if (j == 1)
+    k_vnodal_count<1><<<gc, TPB, 0, ctx->stream>>>(cells, rep, d, order, colmap, nrows, cnt.p);
+  else
+    k_vnodal_count<2><<<gc, TPB, 0, ctx->stream>>>(cells, rep, d, order, colmap, nrows, cnt.p);
+  exclusive_scan_counts(ctx, cnt.p, c.row_ptr.p, nrows + 1);
+  copy_to_host(ctx, &c.nnz, c.row_ptr.p + nrows, 8);
+  c.col.alloc(ctx, std::max<int64_t>(c.nnz, 1));
+  c.val.alloc(ctx, std::max<int64_t>(c.nnz * C, 1));
   double fact = 1.0;
   for (int i = 2; i <= j + 1; ++i) fact *= i;
-  const double factor = 1.0 / fact;
-  HostCsr Pi;
-  Pi.nrows = nrows;
-  Pi.ncols = ncols;
-  Pi.C = C;
-  Pi.rp.assign(nrows + 1, 0);
-  std::vector<int64_t> order(nrows, -1);
-  for (int64_t s = 0; s < ns; ++s) {
-    const int32_t r = rowmap ? rowmap[s] : (int32_t)s;
-    if (r >= 0) order[r] = s;
+  if (nrows) {
+    if (j == 1)
+      k_vnodal_fill<1><<<gf, TPB, 0, ctx->stream>>>(cells, rep, d, order, colmap, nrows, X, 1.0 / fact,
+                                                    c.row_ptr.p, c.col.p, c.val.p);
+    else
+      k_vnodal_fill<2><<<gf, TPB, 0, ctx->stream>>>(cells, rep, d, order, colmap, nrows, X, 1.0 / fact,
+                                                    c.row_ptr.p, c.col.p, c.val.p);
   }
-  for (int64_t r = 0; r < nrows; ++r) {
-    const int64_t s = order[r];
-    const int64_t* v = &sv[s * (j + 1)];
-    double E[4][4];
-    for (int i = 0; i < j; ++i)
-      for (int a = 0; a < d; ++a) E[i][a] = coords[v[i + 1] * d + a] - coords[v[0] * d + a];
-    double w[6];
-    for (int I = 0; I < C; ++I) {
-      const auto& ax = subsets[I];
-      double det;
-      if (j == 1) {
-        det = E[0][ax[0]];
-      } else if (j == 2) {
-        det = E[0][ax[0]] * E[1][ax[1]] - E[0][ax[1]] * E[1][ax[0]];
-      } else {  // j == 3
-        det = E[0][ax[0]] * (E[1][ax[1]] * E[2][ax[2]] - E[1][ax[2]] * E[2][ax[1]]) -
-              E[0][ax[1]] * (E[1][ax[0]] * E[2][ax[2]] - E[1][ax[2]] * E[2][ax[0]]) +
-              E[0][ax[2]] * (E[1][ax[0]] * E[2][ax[1]] - E[1][ax[1]] * E[2][ax[0]]);
-      }
-      w[I] = factor * det;
-    }
-    // entries sorted by column (at most 5: insertion sort in place)
-    int32_t ec[5];
-    int ne = 0;
-    for (int p = 0; p <= j; ++p) {
-      const int32_t c = colmap ? colmap[v[p]] : (int32_t)v[p];
-      if (c < 0) continue;
-      int q = ne++;
-      while (q > 0 && ec[q - 1] > c) {
-        ec[q] = ec[q - 1];
-        --q;
-      }
-      ec[q] = c;
-    }
-    for (int e = 0; e < ne; ++e) {
-      Pi.col.push_back(ec[e]);
-      for (int I = 0; I < C; ++I) Pi.val.push_back(w[I]);
-    }
-    Pi.rp[r + 1] = (int64_t)Pi.col.size();
-  }
-  return Pi;
+  FEEC_HIP(hipGetLastError());
+  out.C = C;
+  finish_x(ctx, out);
 }
 
-// [a | b]: the entries of two CSRs of the same pattern, a's values first.
-HostCsr interleave(const HostCsr& a, const HostCsr& b) {
-  FEEC_REQUIRE(a.nrows == b.nrows && a.col == b.col && a.rp == b.rp, FEEC_E_DEVICE,
-               "transfer patterns differ");
-  HostCsr o;
-  o.nrows = a.nrows;
-  o.ncols = a.ncols;
-  o.rp = a.rp;
-  o.col = a.col;
-  o.C = a.C + b.C;
-  o.val.resize(o.col.size() * o.C);
-  for (size_t p = 0; p < o.col.size(); ++p) {
-    for (int v = 0; v < a.C; ++v) o.val[p * o.C + v] = a.val[p * a.C + v];
-    for (int v = 0; v < b.C; ++v) o.val[p * o.C + a.C + v] = b.val[p * b.C + v];
-  }
-  return o;
+// the transpose of a transfer operator, C values per entry carried along
+void transpose_dev(Ctx* ctx, const XCsr& a, XCsr& out) {
+  const Csr& m = a.m;
+  const int64_t nnz = m.nnz;
+  DBuf<int32_t> rows(ctx, std::max<int64_t>(nnz, 1)), cols(ctx, std::max<int64_t>(nnz, 1));
+  DBuf<uint32_t> srcs(ctx, std::max<int64_t>(nnz, 1));
+  if (nnz)
+    k_transpose_coo<<<grid_for(m.nrows), TPB, 0, ctx->stream>>>(m.row_ptr.p, m.col.p, m.nrows, rows.p,
+                                                                 cols.p, srcs.p);
+  CooPlan plan;
+  coo_plan_build(ctx, plan, m.ncols, m.nrows, nnz, rows.p, cols.p, srcs.p);
+  FEEC_REQUIRE(plan.nnz == nnz, FEEC_E_DEVICE, "a transfer operator with repeated entries");
+  Csr& c = out.m;
+  c.ctx = ctx;
+  c.nrows = m.ncols;
+  c.ncols = m.nrows;
+  c.nnz = nnz;
+  c.row_ptr = std::move(plan.row_ptr);
+  c.col = std::move(plan.col);
+  c.val.alloc(ctx, std::max<int64_t>(nnz * a.C, 1));
+  if (nnz)
+    k_gather_entries<<<grid_for(nnz * a.C), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, m.val.p, a.C,
+                                                                    nnz, c.val.p);
+  FEEC_HIP(hipGetLastError());
+  out.C = a.C;
+  finish_x(ctx, out);
 }
 
-// The coboundary D_{j-1} on the block's DOFs: rows the j-DOFs (rowmap),
-// columns mapped by colmap (-1: dropped), entries sorted by column.
-HostCsr host_coboundary(Mesh& m, int j, const int32_t* rowmap, int64_t nrows, const int32_t* colmap,
-                        int64_t ncols) {
+// The coboundary D_{j-1} on the block's DOFs: rows the j-DOFs (rowmap: the
+// DOF of each j-simplex, null: identity), columns mapped by colmap (-1:
+// dropped), entries sorted by column.
+void coboundary_dev(Mesh& m, int j, const int32_t* rowmap, int64_t nrows, const int32_t* colmap,
+                    int64_t ncols, XCsr& out) {
   Csr full;
   mesh_coboundary(m, j - 1, full);
-  HostCsr f = host_of(m.ctx, full);
-  HostCsr o;
-  o.nrows = nrows;
-  o.ncols = ncols;
-  o.rp.assign(nrows + 1, 0);
-  std::vector<int64_t> order(nrows, -1);
-  for (int64_t s = 0; s < f.nrows; ++s) {
-    const int32_t r = rowmap ? rowmap[s] : (int32_t)s;
-    if (r >= 0) order[r] = s;
+  csr_select(full, rowmap, nrows, colmap, ncols, out.m);
+  out.C = 1;
+  finish_x(m.ctx, out);
+}
+
+// [a | b]: the entries of two operators of one pattern, a's values first;
+// raises bad[0] on the device when the patterns differ.
+void interleave_dev(Ctx* ctx, const XCsr& a, const XCsr& b, XCsr& out, int* bad) {
+  FEEC_REQUIRE(a.m.nrows == b.m.nrows && a.m.nnz == b.m.nnz, FEEC_E_DEVICE, "transfer patterns differ");
+  const Csr& am = a.m;
+  Csr& c = out.m;
+  c.ctx = ctx;
+  c.nrows = am.nrows;
+  c.ncols = am.ncols;
+  c.nnz = am.nnz;
+  out.C = a.C + b.C;
+  c.row_ptr.alloc(ctx, am.nrows + 1);
+  c.col.alloc(ctx, std::max<int64_t>(am.nnz, 1));
+  c.val.alloc(ctx, std::max<int64_t>(am.nnz * out.C, 1));
+  FEEC_HIP(hipMemcpyAsync(c.row_ptr.p, am.row_ptr.p, (am.nrows + 1) * 8, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  if (am.nnz) {
+    FEEC_HIP(hipMemcpyAsync(c.col.p, am.col.p, am.nnz * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    k_interleave<<<grid_for(am.nrows), TPB, 0, ctx->stream>>>(am.row_ptr.p, am.col.p, am.val.p, a.C,
+                                                               b.m.row_ptr.p, b.m.col.p, b.m.val.p, b.C,
+                                                               am.nrows, c.val.p, bad);
   }
-  o.col.reserve(f.col.size());
-  o.val.reserve(f.col.size());
-  for (int64_t r = 0; r < nrows; ++r) {
-    const int64_t s = order[r];
-    // a j-simplex has j + 1 <= 5 facets: insertion sort in place
-    int32_t ec[5];
-    double ev[5];
-    int ne = 0;
-    for (int64_t p = f.rp[s]; p < f.rp[s + 1]; ++p) {
-      const int32_t c = colmap ? colmap[f.col[p]] : f.col[p];
-      if (c < 0) continue;
-      FEEC_REQUIRE(ne < 5, FEEC_E_DEVICE, "a coboundary row with more than 5 entries");
-      int q = ne++;
-      while (q > 0 && ec[q - 1] > c) {
-        ec[q] = ec[q - 1];
-        ev[q] = ev[q - 1];
-        --q;
-      }
-      ec[q] = c;
-      ev[q] = f.val[p];
-    }
-    for (int e = 0; e < ne; ++e) {
-      o.col.push_back(ec[e]);
-      o.val.push_back(ev[e]);
-    }
-    o.rp[r + 1] = (int64_t)o.col.size();
-  }
-  return o;
+  FEEC_HIP(hipGetLastError());
+  finish_x(ctx, out);
 }
 
 // The nodal operator hdif_gram(0) on the complex's grade-0 DOFs: P's block
@@ -1395,13 +1550,12 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
   const bool rel = bc == FEEC_RELATIVE;
   if (rel) m.ensure_interior();
   const int NS = nstencil(d);
-  // vertex -> its grade-0 DOF (-1: none) and -> its box column (-1 off the unknowns)
-  std::vector<int32_t> dof0(m.n[0]), live_col(m.n[0]);
-  if (rel) copy_to_host(ctx, dof0.data(), m.interior[0].p, m.n[0] * 4);
-  for (int64_t v = 0; v < m.n[0]; ++v) {
-    if (!rel) dof0[v] = (int32_t)v;
-    live_col[v] = dof0[v] >= 0 ? (int32_t)v : -1;
-  }
+  // vertex -> its grade-0 DOF (-1: none)
+  std::vector<int32_t> dof0(m.n[0]);
+  if (rel)
+    copy_to_host(ctx, dof0.data(), m.interior[0].p, m.n[0] * 4);
+  else
+    for (int64_t v = 0; v < m.n[0]; ++v) dof0[v] = (int32_t)v;
   // the levels
   {
     auto add_level = [&](int gcells, const std::vector<uint8_t>& al, int64_t nlive) {
@@ -1518,19 +1672,23 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
   if (k >= 1) grades.push_back(k - 1);
   grades.push_back(k);
   int slot = 0;
+  // the vertices' box columns (null: every vertex keeps its own) and coordinates
+  DBuf<int32_t> live_dev;
+  const int32_t* live_col_dev = nullptr;
+  if (rel) {
+    live_dev.alloc(ctx, std::max<int64_t>(m.n[0], 1));
+    k_live_col<<<grid_for(m.n[0]), TPB, 0, ctx->stream>>>(m.interior[0].p, m.n[0], live_dev.p);
+    FEEC_HIP(hipGetLastError());
+    live_col_dev = live_dev.p;
+  }
+  const double* X = k >= 1 ? device_coords(g) : nullptr;
   for (int j : grades) {
     HxBlock b;
     b.j = j;
     b.off = (j == k) ? ns : 0;
     b.n = (j == k) ? s.n - ns : ns;
-    std::vector<int32_t> hmap_j, hmap_l;
-    auto host_map = [&](int grade, std::vector<int32_t>& out) -> const int32_t* {
-      if (!rel) return nullptr;
-      out.resize(m.n[grade]);
-      copy_to_host(ctx, out.data(), m.interior[grade].p, m.n[grade] * 4);
-      return out.data();
-    };
-    const int32_t* map_j = host_map(j, hmap_j);
+    // the block's DOF of each j-simplex (null: identity) and its inverse
+    const int32_t* map_j = rel ? m.interior[j].p : nullptr;
     if (j == 0) {
       HostCsr o;  // block DOF -> its vertex
       o.nrows = b.n;
@@ -1546,64 +1704,53 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
       b.slot_main = slot;
       slot += 1;
     } else if (j == 1) {
-      HostCsr D = host_coboundary(m, 1, map_j, b.n, live_col.data(), nb0);
-      HostCsr Pi = vector_nodal(m, g.coords, 1, map_j, live_col.data(), b.n, nb0);
-      HostCsr DP = interleave(D, Pi);
-      upload(ctx, transpose(DP), b.g_main);
+      DBuf<int32_t> order;
+      row_order(m, map_j, 1, b.n, order);
+      XCsr D, Pi, DP;
+      coboundary_dev(m, 1, map_j, b.n, live_col_dev, nb0, D);
+      vector_nodal_dev(m, X, 1, order.p, live_col_dev, b.n, nb0, Pi);
+      DBuf<int> bad(ctx, 2);
+      k_zero_i32<<<1, 64, 0, ctx->stream>>>(bad.p, 2);
+      interleave_dev(ctx, D, Pi, DP, bad.p);
+      transpose_dev(ctx, DP, b.g_main);
       // the matrix-free scatter: ends and Pi values per row, after checking
       // that every D_0 entry is -1 at the smaller end and +1 at the larger
-      std::vector<int64_t> sv(m.n[1] * 2);
-      if (feec_mesh_skeleton(static_cast<feec_mesh*>(&m), 1, sv.data()) != FEEC_OK)
-        fail(FEEC_E_DEVICE, feec_last_error());
-      std::vector<int64_t> edge_of(b.n, -1);
-      for (int64_t e = 0; e < m.n[1]; ++e) {
-        const int32_t r = map_j ? map_j[e] : (int32_t)e;
-        if (r >= 0 && r < b.n) edge_of[r] = e;
-      }
-      bool ok = true;
-      std::vector<int32_t> ev(2 * b.n);
-      std::vector<double> eh((size_t)b.n * d);
-      for (int64_t r = 0; r < b.n && ok; ++r) {
-        const int64_t e = edge_of[r];
-        ok = e >= 0 && sv[2 * e] < sv[2 * e + 1];
-        if (!ok) break;
-        ev[2 * r] = (int32_t)sv[2 * e];
-        ev[2 * r + 1] = (int32_t)sv[2 * e + 1];
-        for (int64_t p = D.rp[r]; p < D.rp[r + 1]; ++p)
-          ok = ok && ((D.col[p] == ev[2 * r] && D.val[p] == -1.0) || (D.col[p] == ev[2 * r + 1] && D.val[p] == 1.0));
-        for (int c = 0; c < d; ++c)  // vector_nodal's row values: (1/2!) (x(v1) - x(v0))
-          eh[r * d + c] = (1.0 / 2.0) * (g.coords[ev[2 * r + 1] * d + c] - g.coords[ev[2 * r] * d + c]);
-      }
-      if (ok && d <= 4) {
-        b.edge_free = true;
-        b.ev.alloc(ctx, std::max<int64_t>(2 * b.n, 1));
-        b.eh64.alloc(ctx, std::max<int64_t>(b.n * d, 1));
-        if (b.n) {
-          copy_to_device(ctx, b.ev.p, ev.data(), 2 * b.n * 4);
-          copy_to_device(ctx, b.eh64.p, eh.data(), b.n * d * 8);
-        }
-        to32(ctx, b.eh64.p, b.n * d, b.eh32);
-      } else {
-        upload(ctx, DP, b.o_main);
-      }
+      b.ev.alloc(ctx, std::max<int64_t>(2 * b.n, 1));
+      b.eh64.alloc(ctx, std::max<int64_t>(b.n * d, 1));
+      if (b.n)
+        k_edge_ends<<<grid_for(b.n), TPB, 0, ctx->stream>>>(m.faces[0].p, m.rep[1].p, d, order.p, b.n,
+                                                            D.m.row_ptr.p, D.m.col.p, D.m.val.p, X, b.ev.p,
+                                                            b.eh64.p, bad.p + 1);
+      FEEC_HIP(hipGetLastError());
+      int hb[2] = {0, 0};
+      copy_to_host(ctx, hb, bad.p, sizeof(hb));
+      FEEC_REQUIRE(hb[0] == 0, FEEC_E_DEVICE, "transfer patterns differ");
       b.slot_main = slot;
       slot += DP.C;
+      if (hb[1] == 0 && d <= 4) {
+        b.edge_free = true;
+        to32(ctx, b.eh64.p, b.n * d, b.eh32);
+      } else {
+        b.ev.reset();
+        b.eh64.reset();
+        b.o_main = std::move(DP);
+      }
     } else {
-      const int32_t* map_l = host_map(1, hmap_l);
+      const int32_t* map_l = rel ? m.interior[1].p : nullptr;
       const int64_t nl = rel ? m.n_interior[1] : m.n[1];
-      HostCsr D = host_coboundary(m, 2, map_j, b.n, map_l, nl);
-      HostCsr Pi = vector_nodal(m, g.coords, 2, map_j, live_col.data(), b.n, nb0);
-      HostCsr Pl = vector_nodal(m, g.coords, 1, map_l, live_col.data(), nl, nb0);
-      upload(ctx, transpose(D), b.g_t);
-      upload(ctx, D, b.o_d);
-      upload(ctx, transpose(Pl), b.g_tl);
-      upload(ctx, Pl, b.o_ul);
-      upload(ctx, transpose(Pi), b.g_main);
-      upload(ctx, Pi, b.o_main);
+      DBuf<int32_t> order_j, order_l;
+      row_order(m, map_j, 2, b.n, order_j);
+      row_order(m, map_l, 1, nl, order_l);
+      coboundary_dev(m, 2, map_j, b.n, map_l, nl, b.o_d);
+      vector_nodal_dev(m, X, 2, order_j.p, live_col_dev, b.n, nb0, b.o_main);
+      vector_nodal_dev(m, X, 1, order_l.p, live_col_dev, nl, nb0, b.o_ul);
+      transpose_dev(ctx, b.o_d, b.g_t);
+      transpose_dev(ctx, b.o_ul, b.g_tl);
+      transpose_dev(ctx, b.o_main, b.g_main);
       b.slot_main = slot;
-      slot += Pi.C;
+      slot += b.o_main.C;
       b.slot_tl = slot;
-      slot += Pl.C;
+      slot += b.o_ul.C;
       b.t64.alloc(ctx, std::max<int64_t>(nl, 1));
       b.u64.alloc(ctx, std::max<int64_t>(nl, 1));
       b.t32.alloc(ctx, std::max<int64_t>(nl, 1));
