@@ -1007,21 +1007,6 @@ void upload(Ctx* ctx, const HostCsr& h, XCsr& out) {
   FEEC_HIP(hipGetLastError());
 }
 
-HostCsr host_of(Ctx* ctx, const Csr& a) {
-  HostCsr h;
-  h.nrows = a.nrows;
-  h.ncols = a.ncols;
-  h.rp.resize(a.nrows + 1);
-  h.col.resize(a.nnz);
-  h.val.resize(a.nnz);
-  copy_to_host(ctx, h.rp.data(), a.row_ptr.p, (a.nrows + 1) * 8);
-  if (a.nnz) {
-    copy_to_host(ctx, h.col.data(), a.col.p, a.nnz * 4);
-    copy_to_host(ctx, h.val.data(), a.val.p, a.nnz * 8);
-  }
-  return h;
-}
-
 void to32(Ctx* ctx, const double* a, int64_t n, DBuf<float>& out) {
   if ((int64_t)out.n != std::max<int64_t>(n, 1)) out.alloc(ctx, std::max<int64_t>(n, 1));
   if (n) k_convert<double, float><<<grid_for(n), TPB, 0, ctx->stream>>>(a, out.p, n);
@@ -1352,13 +1337,52 @@ int host_stencil_offset(int d, int s, const int* st) {
   }
 }
 
-int host_stencil_point(int d, const int* diff) {
-  switch (d) {
-    case 1: return stencil_point<1>(diff);
-    case 2: return stencil_point<2>(diff);
-    case 3: return stencil_point<3>(diff);
-    default: return stencil_point<4>(diff);
+// Entries of the nodal block of rows and columns [lo, hi) of a CSR, per
+// row; the extra entry at hi - lo is 0 so that the exclusive scan ends on the total.
+__global__ void k_l0_count(const int64_t* rp, const int32_t* col, int64_t lo, int64_t hi,
+                           unsigned long long* cnt) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t n = hi - lo;
+  if (i > n) return;
+  unsigned long long c = 0;
+  if (i < n)
+    for (int64_t p = rp[lo + i]; p < rp[lo + i + 1]; ++p) c += col[p] >= lo && col[p] < hi;
+  cnt[i] = c;
+}
+
+// The level-0 stencil slot (point * nb + row vertex) of each entry of the
+// nodal block, in row order, with its source entry when src is given;
+// bad[0] set for an entry off the Kuhn stencil. box_of: the vertex of each
+// nodal DOF (null: the DOF is the vertex).
+template <int D>
+__global__ void k_l0_fill(const int64_t* rp, const int32_t* col, int64_t lo, int64_t hi,
+                          const int32_t* box_of, Box b0, const int64_t* start, int32_t* map,
+                          int32_t* src, int* bad) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= hi - lo) return;
+  const int g1 = b0.g + 1;
+  const int bi = box_of ? box_of[i] : (int)i;
+  int64_t q = start[i];
+  bool ok = bi >= 0;
+  for (int64_t p = rp[lo + i]; p < rp[lo + i + 1]; ++p) {
+    const int32_t c = col[p];
+    if (c < lo || c >= hi) continue;
+    int ri = bi, rj = box_of ? box_of[c - lo] : (int)(c - lo);
+    ok = ok && rj >= 0;
+    int diff[D];
+    for (int a = 0; a < D; ++a) {
+      diff[a] = rj % g1 - ri % g1;
+      ok = ok && diff[a] >= -1 && diff[a] <= 1;
+      ri /= g1;
+      rj /= g1;
+    }
+    const int sp = stencil_point<D>(diff);
+    ok = ok && sp >= 0;
+    map[q] = ok ? (int32_t)((int64_t)sp * b0.nb + bi) : 0;
+    if (src) src[q] = (int32_t)p;
+    ++q;
   }
+  if (!ok) atomicOr(bad, 1);
 }
 
 void launch_galerkin(Ctx* ctx, int d, MgLevel& f, MgLevel& c) {
@@ -1605,58 +1629,57 @@ Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m
   // the level-0 operator's stencil positions, then the hierarchy's values
   {
     const Box& b0 = h->L[0].box;
-    std::vector<int32_t> box_of(m.n[0], 0);
-    for (int64_t v = 0; v < m.n[0]; ++v)
-      if (dof0[v] >= 0) box_of[dof0[v]] = (int32_t)v;
-    // stencil position of the entry (row DOF r, column DOF c), -1 off the stencil
-    auto position = [&](int64_t r, int64_t c) -> int64_t {
-      int ri = box_of[r], rj = box_of[c];
-      const int bi = ri;
-      int diff[4] = {0, 0, 0, 0};
-      for (int q = 0; q < d; ++q) {
-        diff[q] = rj % (b0.g + 1) - ri % (b0.g + 1);
-        if (diff[q] < -1 || diff[q] > 1) return -1;
-        ri /= (b0.g + 1);
-        rj /= (b0.g + 1);
-      }
-      const int sp = host_stencil_point(d, diff);
-      return sp < 0 ? -1 : (int64_t)sp * b0.nb + bi;
-    };
-    std::vector<int32_t> map, src;
+    // the vertex of each nodal DOF (null: the DOF is the vertex)
+    DBuf<int32_t> box_of;
+    if (rel) {
+      box_of.alloc(ctx, std::max<int64_t>(m.n_interior[0], 1));
+      if (m.n[0])
+        k_invert_map<<<grid_for(m.n[0]), TPB, 0, ctx->stream>>>(m.interior[0].p, m.n[0], m.n_interior[0],
+                                                                 box_of.p);
+      FEEC_HIP(hipGetLastError());
+    }
     Csr A0;
+    const Csr* src_op = &s.P;
+    int64_t lo = 0, hi = 0;
     if (k <= 1 && s.P.nnz) {  // the nodal operator is P's block [lo, hi)
-      const int64_t lo = k == 1 ? 0 : s.split, hi = k == 1 ? s.split : s.n;
-      std::vector<int64_t> rp(s.P.nrows + 1);
-      std::vector<int32_t> col(s.P.nnz);
-      copy_to_host(ctx, rp.data(), s.P.row_ptr.p, rp.size() * 8);
-      copy_to_host(ctx, col.data(), s.P.col.p, col.size() * 4);
-      for (int64_t r = lo; r < hi; ++r)
-        for (int64_t p = rp[r]; p < rp[r + 1]; ++p)
-          if (col[p] >= lo && col[p] < hi) {
-            const int64_t pos = position(r - lo, col[p] - lo);
-            if (pos < 0) return no("the nodal operator is not a Kuhn stencil");
-            map.push_back((int32_t)pos);
-            src.push_back((int32_t)p);
-          }
+      lo = k == 1 ? 0 : s.split;
+      hi = k == 1 ? s.split : s.n;
       h->l0_from_p = true;
     } else {
       nodal_operator(g, bc, s, k, A0);
-      HostCsr a = host_of(ctx, A0);
-      map.resize(a.col.size());
-      for (int64_t r = 0; r < a.nrows; ++r)
-        for (int64_t p = a.rp[r]; p < a.rp[r + 1]; ++p) {
-          const int64_t pos = position(r, a.col[p]);
-          if (pos < 0) return no("the nodal operator is not a Kuhn stencil");
-          map[p] = (int32_t)pos;
-        }
+      src_op = &A0;
+      hi = A0.nrows;
     }
-    h->l0_nnz = (int64_t)map.size();
+    const int64_t nn = hi - lo;
+    FEEC_REQUIRE(nn == (rel ? m.n_interior[0] : m.n[0]), FEEC_E_DEVICE, "nodal block of the wrong size");
+    DBuf<unsigned long long> cnt(ctx, nn + 1);
+    DBuf<int64_t> start(ctx, nn + 1);
+    k_l0_count<<<grid_for(nn + 1), TPB, 0, ctx->stream>>>(src_op->row_ptr.p, src_op->col.p, lo, hi, cnt.p);
+    FEEC_HIP(hipGetLastError());
+    exclusive_scan_counts(ctx, cnt.p, start.p, nn + 1);
+    copy_to_host(ctx, &h->l0_nnz, start.p + nn, 8);
     h->l0_map.alloc(ctx, std::max<int64_t>(h->l0_nnz, 1));
-    if (h->l0_nnz) copy_to_device(ctx, h->l0_map.p, map.data(), h->l0_nnz * 4);
-    if (h->l0_from_p) {
-      h->l0_src.alloc(ctx, std::max<int64_t>(h->l0_nnz, 1));
-      if (h->l0_nnz) copy_to_device(ctx, h->l0_src.p, src.data(), h->l0_nnz * 4);
+    if (h->l0_from_p) h->l0_src.alloc(ctx, std::max<int64_t>(h->l0_nnz, 1));
+    DBuf<int> bad(ctx, 1);
+    k_zero_i32<<<1, 64, 0, ctx->stream>>>(bad.p, 1);
+    if (nn) {
+      const unsigned gl = grid_for(nn);
+      int32_t* srcp = h->l0_from_p ? h->l0_src.p : nullptr;
+#define FEEC_L0(D)                                                                                       \
+  k_l0_fill<D><<<gl, TPB, 0, ctx->stream>>>(src_op->row_ptr.p, src_op->col.p, lo, hi, box_of.p, b0, start.p, \
+                                            h->l0_map.p, srcp, bad.p)
+      switch (d) {
+        case 1: FEEC_L0(1); break;
+        case 2: FEEC_L0(2); break;
+        case 3: FEEC_L0(3); break;
+        default: FEEC_L0(4); break;
+      }
+#undef FEEC_L0
+      FEEC_HIP(hipGetLastError());
     }
+    int hbad = 0;
+    copy_to_host(ctx, &hbad, bad.p, sizeof(int));
+    if (hbad) return no("the nodal operator is not a Kuhn stencil");
     const int nbc = h->L.back().box.nb;
     h->cinv.alloc(ctx, (size_t)nbc * nbc);
     try {
