@@ -599,6 +599,35 @@ class Complex:
         check(load().feec_mesh_coboundary(self._h, bc, k, C.byref(h)))
         return Csr(h, self)
 
+    def _ndofs(self, k, bc):
+        if bc == RELATIVE:
+            return int((self.boundary_flags(k) == 0).sum())
+        return self.nsimplices(k)
+
+    def betti_number(self, k, bc=FULL):
+        """b_k (Complex::betti_number), or relative to the boundary with
+        bc=RELATIVE (relative_betti_number), from exact ranks."""
+        b = C.c_int64()
+        check(load().feec_betti_number(self._h, bc, k, C.byref(b)))
+        return int(b.value)
+
+    def _generators(self, fn, k, bc):
+        cnt = C.c_int64()
+        check(fn(self._h, bc, k, C.byref(cnt), None))
+        out = np.zeros((cnt.value, self._ndofs(k, bc)), np.int64)
+        if cnt.value:
+            check(fn(self._h, bc, k, C.byref(cnt), _ptr(out, _lib.i64p)))
+        return out
+
+    def cohomology_generators(self, k, bc=FULL):
+        """Integral cocycles of a basis of the free part of H^k, one per row, on
+        the complex's DOFs (relative: interior numbering)."""
+        return self._generators(load().feec_cohomology_generators, k, bc)
+
+    def homology_generators(self, k, bc=FULL):
+        """Integral cycles of a basis of the free part of H_k, one per row."""
+        return self._generators(load().feec_homology_generators, k, bc)
+
 
 class MeshLengthsSq:
     """Signed squared edge lengths on the device (a Regge metric)."""
@@ -706,14 +735,22 @@ class WhitneyComplex:
     def dif(self, k):
         return self.topology.coboundary(k, self.bc)
 
-    def harmonics(self, k):
+    def _harmonic_reading(self, fn, k):
         cnt = C.c_int64()
-        check(load().feec_harmonics(self.geometry._h, self.bc, k, C.byref(cnt), None))
+        check(fn(self.geometry._h, self.bc, k, C.byref(cnt), None))
         n = self.ndofs(k)
         out = np.zeros((max(cnt.value, 1), n))
         if cnt.value:
-            check(load().feec_harmonics(self.geometry._h, self.bc, k, C.byref(cnt), _ptr(out, _lib.f64p)))
+            check(fn(self.geometry._h, self.bc, k, C.byref(cnt), _ptr(out, _lib.f64p)))
         return out[: cnt.value].T.copy()
+
+    def harmonics(self, k):
+        """An M_k-orthonormal harmonic basis, ndofs x b_k (solve_harmonics)."""
+        return self._harmonic_reading(load().feec_harmonics, k)
+
+    def harmonics_integral(self, k):
+        """The period-normalized harmonic basis, ndofs x b_k (Harmonics::integral)."""
+        return self._harmonic_reading(load().feec_harmonics_integral, k)
 
     def mixed_kkt(self, k):
         h = C.c_void_p()
@@ -737,7 +774,7 @@ def solve_source(complex_: WhitneyComplex, source_galvec, grade, opts: SolveOpts
                                   _ptr(u, _lib.f64p), _ptr(p, _lib.f64p), C.byref(rep))
     if raise_on_failure:
         check(st)
-    nh = complex_.harmonics(grade).shape[1] if grade in (0, complex_.dim) else 0
+    nh = topo.betti_number(grade, complex_.bc)
     return sigma[: topo.nsimplices(grade - 1)], u, p[:nh], rep
 
 

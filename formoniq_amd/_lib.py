@@ -100,6 +100,10 @@ SIGNATURES = {
     "feec_csr_cg_noalloc": (S, [vp, f64p, f64p, C.c_double, C.c_int64, C.POINTER(Report)]),
     "feec_csr_destroy": (None, [vp]),
     "feec_harmonics": (S, [vp, C.c_int, C.c_int, i64p, f64p]),
+    "feec_harmonics_integral": (S, [vp, C.c_int, C.c_int, i64p, f64p]),
+    "feec_betti_number": (S, [vp, C.c_int, C.c_int, i64p]),
+    "feec_cohomology_generators": (S, [vp, C.c_int, C.c_int, i64p, i64p]),
+    "feec_homology_generators": (S, [vp, C.c_int, C.c_int, i64p, i64p]),
     "feec_mixed_kkt": (S, [vp, C.c_int, C.c_int, C.POINTER(vp), i64p]),
     "feec_solve_source": (S, [vp, C.c_int, C.c_int, f64p, C.POINTER(SolveOpts), f64p, f64p, f64p,
                               C.POINTER(Report)]),

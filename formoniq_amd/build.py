@@ -19,7 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfeec.so")
 
 SOURCES = ["topology.hip", "elmat.hip", "assemble.hip", "krylov.hip", "multigrid.hip", "api.hip",
-           "dirac.hip", "serve.hip", "analytic.hip", "ingest.hip", "tune.hip"]
+           "dirac.hip", "serve.hip", "analytic.hip", "ingest.hip", "tune.hip", "homology.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FEEC_OFFLOAD_ARCH", "gfx950")

@@ -231,36 +231,50 @@ __global__ void k_fill_border(const double* mh, int64_t nu, int64_t ns, int64_t 
 
 }  // namespace
 
-int64_t euler_characteristic(const Mesh& m) {
-  int64_t chi = 0;
-  for (int k = 0; k <= m.d; ++k) chi += (k & 1) ? -m.n[k] : m.n[k];
-  return chi;
-}
-
-// Harmonic basis on a ball (elliptic.rs:197-220): b_0 = 1 on the full complex
-// (the normalized constant), b_n = 1 on the relative complex (M_n^-1 times the
-// orientation, dirac.rs:553-568); every other space is trivial. The closed
-// forms are used only when the mesh passes the ball test -- one connected
-// component and Euler characteristic 1 -- and every grade of any other mesh
-// (an annulus, a disconnected union) is FEEC_E_UNSUPPORTED, so no problem is
-// built without the harmonic border it needs. (The test is necessary, not
-// sufficient: in 3D a connected mesh with chi = 1 can still have b_1 = b_2 > 0.)
-void harmonic_basis(Geom& g, int bc, int k, std::vector<double>& basis, int64_t& count) {
+// Harmonic basis (elliptic.rs:197-220), both readings [count][ndofs]: the
+// period-normalized one (Harmonics::integral) and the M_k-orthonormal one the
+// KKT border uses. Its dimension is b_k. On a Kuhn box (a ball) the closed
+// forms: b_0 = 1 on the full complex, the constant; b_n = 1 on the relative
+// complex, M_n^-1 times the orientation o (dirac.rs:553-568); every other
+// space is trivial. On any other mesh b_k is computed exactly (modular ranks);
+// the same closed forms are taken where b_k = 1 on a connected mesh, and
+// otherwise the projected integral cocycles (harmonics_exact). A closed form
+// is what harmonics_exact returns there: the cocycle projects to a multiple of
+// 1 (resp. M_n^-1 o), whose period on the cycle (vertex 0, resp. o with its
+// first cell positive) is made 1, then its M-norm.
+void harmonic_bases(Geom& g, int bc, int k, std::vector<double>& orthonormal, std::vector<double>& integral,
+                    int64_t& count) {
   Mesh& m = *g.mesh;
+  FEEC_REQUIRE(k >= 0 && k <= m.d, FEEC_E_ARG, "grade out of range");
   count = 0;
-  basis.clear();
-  m.ensure_components();
-  FEEC_REQUIRE(m.components == 1 && euler_characteristic(m) == 1, FEEC_E_UNSUPPORTED,
-               "harmonic forms are implemented in closed form for balls (one component, Euler "
-               "characteristic 1)");
-  bool candidate = (bc == FEEC_FULL && k == 0) || (bc == FEEC_RELATIVE && k == m.d);
-  if (!candidate) return;
+  orthonormal.clear();
+  integral.clear();
+  const bool candidate = (bc == FEEC_FULL && k == 0) || (bc == FEEC_RELATIVE && k == m.d);
+  bool closed = false;
+  if (kuhn_grid_n(m) > 0) {
+    if (!candidate) return;
+    closed = true;
+  } else {
+    const int64_t b = betti_number(m, bc, k);
+    if (b == 0) return;
+    m.ensure_components();
+    if (candidate && b == 1 && m.components == 1) {
+      if (bc == FEEC_RELATIVE) m.ensure_orientation();
+      closed = bc == FEEC_FULL || m.orientable;
+    }
+    if (!closed) {
+      harmonics_exact(g, bc, k, integral, orthonormal, count);
+      return;
+    }
+  }
   Csr M;
   assemble_form(g, bc, FEEC_MASS, k, M);
   int64_t n = M.nrows;
   std::vector<double> v(n);
+  double period = 0.0;
   if (bc == FEEC_FULL) {
     std::fill(v.begin(), v.end(), 1.0);
+    period = n ? 1.0 : 0.0;  // the cycle is vertex 0
   } else {
     m.ensure_orientation();
     FEEC_REQUIRE(m.orientable, FEEC_E_TOPOLOGY, "the mesh is not orientable");
@@ -270,13 +284,19 @@ void harmonic_basis(Geom& g, int bc, int k, std::vector<double>& basis, int64_t&
     copy_to_host(m.ctx, rp.data(), M.row_ptr.p, (n + 1) * 8);
     copy_to_host(m.ctx, col.data(), M.col.p, M.nnz * 4);
     copy_to_host(m.ctx, val.data(), M.val.p, M.nnz * 8);
+    const double first = n && m.orientation[0] < 0 ? -1.0 : 1.0;
     for (int64_t i = 0; i < n; ++i) {
       double dgl = 0.0;
       for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
         if (col[p] == i) dgl = val[p];
-      v[i] = (double)m.orientation[i] / dgl;  // top-grade Whitney mass is diagonal
+      const double o = first * (double)m.orientation[i];
+      v[i] = o / dgl;  // top-grade Whitney mass is diagonal
+      period += v[i] * o;
     }
   }
+  FEEC_REQUIRE(period != 0.0, FEEC_E_DEVICE, "the harmonic form has no period");
+  for (auto& x : v) x /= period;
+  integral = v;
   // normalize in the M_k inner product
   DBuf<double> dv(m.ctx, n), mv(m.ctx, n);
   copy_to_device(m.ctx, dv.p, v.data(), n * 8);
@@ -286,10 +306,18 @@ void harmonic_basis(Geom& g, int bc, int k, std::vector<double>& basis, int64_t&
   copy_to_host(m.ctx, hmv.data(), mv.p, n * 8);
   double q = 0.0;
   for (int64_t i = 0; i < n; ++i) q += v[i] * hmv[i];
+  FEEC_REQUIRE(q > 0.0, FEEC_E_UNSUPPORTED,
+               "the L2 pairing is not definite on the harmonic space (the reference falls back to an "
+               "eigensolve there, which is not built)");
   double s = 1.0 / std::sqrt(q);
   for (auto& x : v) x *= s;
-  basis = std::move(v);
+  orthonormal = std::move(v);
   count = 1;
+}
+
+void harmonic_basis(Geom& g, int bc, int k, std::vector<double>& basis, int64_t& count) {
+  std::vector<double> integral;
+  harmonic_bases(g, bc, k, basis, integral, count);
 }
 
 void problem_build(Problem& P) {
@@ -429,9 +457,9 @@ void problem_build(Problem& P) {
     // border = M_k H, stored as full-length columns (zero on the sigma rows)
     Csr M;
     assemble_form(g, bc, FEEC_MASS, k, M);
-    DBuf<double> h(ctx, P.nu), mh(ctx, P.nu);
-    copy_to_device(ctx, h.p, P.harmonics_host.data(), P.nu * 8);
-    spmv64(ctx, M, h.p, mh.p);
+    DBuf<double> h(ctx, P.nh * P.nu), mh(ctx, P.nh * P.nu);
+    copy_to_device(ctx, h.p, P.harmonics_host.data(), P.nh * P.nu * 8);
+    for (int64_t c = 0; c < P.nh; ++c) spmv64(ctx, M, h.p + c * P.nu, mh.p + c * P.nu);
     P.sys.border.alloc(ctx, P.nh * n);
     k_fill_border<<<grid_for(n), TPB, 0, ctx->stream>>>(mh.p, P.nu, P.ns, n, (int)P.nh,
                                                         P.sys.border.p);
@@ -1228,6 +1256,52 @@ feec_status feec_harmonics(feec_geom* g, feec_bc bc, int grade, int64_t* count, 
     *count = c;
     if (basis && c) std::copy(hb.begin(), hb.end(), basis);
   });
+}
+
+feec_status feec_harmonics_integral(feec_geom* g, feec_bc bc, int grade, int64_t* count, double* basis) {
+  return guard([&] {
+    FEEC_REQUIRE(g && count, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
+    std::vector<double> ortho, integral;
+    int64_t c = 0;
+    harmonic_bases(*g, bc, grade, ortho, integral, c);
+    *count = c;
+    if (basis && c) std::copy(integral.begin(), integral.end(), basis);
+  });
+}
+
+feec_status feec_betti_number(const feec_mesh* mc, feec_bc bc, int grade, int64_t* b) {
+  return guard([&] {
+    FEEC_REQUIRE(mc && b, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
+    feec_mesh* m = const_cast<feec_mesh*>(mc);
+    FEEC_HIP(hipSetDevice(m->ctx->device));
+    *b = betti_number(*m, bc, grade);
+  });
+}
+
+namespace {
+feec_status generators_out(const feec_mesh* mc, feec_bc bc, int grade, int64_t* count, int64_t* out,
+                           bool cocycles) {
+  return guard([&] {
+    FEEC_REQUIRE(mc && count, FEEC_E_ARG, "null argument");
+    FEEC_REQUIRE(abi_bc_ok(bc), FEEC_E_ARG, "bc must be a feec_bc value");
+    feec_mesh* m = const_cast<feec_mesh*>(mc);
+    FEEC_HIP(hipSetDevice(m->ctx->device));
+    const auto gens = cocycles ? cohomology_generators(*m, bc, grade) : homology_generators(*m, bc, grade);
+    *count = (int64_t)gens.size();
+    if (out)
+      for (size_t i = 0; i < gens.size(); ++i) std::copy(gens[i].begin(), gens[i].end(), out + i * gens[i].size());
+  });
+}
+}  // namespace
+
+feec_status feec_cohomology_generators(const feec_mesh* m, feec_bc bc, int grade, int64_t* count, int64_t* out) {
+  return generators_out(m, bc, grade, count, out, true);
+}
+
+feec_status feec_homology_generators(const feec_mesh* m, feec_bc bc, int grade, int64_t* count, int64_t* out) {
+  return generators_out(m, bc, grade, count, out, false);
 }
 
 feec_status feec_mixed_kkt(feec_geom* g, feec_bc bc, int grade, feec_csr** out, int64_t sizes[3]) {

@@ -318,6 +318,23 @@ void causal_census(Geom& g, int64_t counts[3], uint8_t* riem_dev);
 // assembly driver
 void assemble_form(Geom& g, int bc, int kind, int grade, Csr& out);
 
+// homology.hip: exact (co)homology on the host
+// A sparse integer matrix by rows, each sorted by column with no zeros.
+struct IntMat {
+  int64_t nrows = 0, ncols = 0;
+  std::vector<std::vector<std::pair<int32_t, int64_t>>> rows;
+};
+// the most harmonic forms the KKT border holds (the MINRES border kernels' limit)
+constexpr int64_t kMaxHarmonics = 8;
+IntMat int_transpose(const IntMat& a);
+int64_t exact_rank(const IntMat& a);
+std::vector<std::vector<int64_t>> quotient_generators(const IntMat& A, const IntMat& B);
+int64_t betti_number(Mesh& m, int bc, int k);
+std::vector<std::vector<int64_t>> cohomology_generators(Mesh& m, int bc, int k);
+std::vector<std::vector<int64_t>> homology_generators(Mesh& m, int bc, int k);
+void harmonics_exact(Geom& g, int bc, int k, std::vector<double>& integral, std::vector<double>& orthonormal,
+                     int64_t& count);
+
 // krylov.hip
 struct KrylovStats {
   int64_t iters = 0;

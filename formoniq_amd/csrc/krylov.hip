@@ -35,6 +35,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <tuple>
 #include <type_traits>
@@ -949,6 +950,38 @@ __global__ void k_power_normalize(const double* y, int64_t split, int64_t n, con
   if (ny > 0) x[i] = (1.0 / ny) * y[i];
 }
 
+// The Gershgorin bound of D^-1 P per block: max over the block's rows of
+// sum_j |P_ij| dinv_i, an upper bound on lambda_max(D^-1 P). Rows are >= 0, so
+// the max runs on the bit patterns (order-preserving for non-negative
+// doubles): a wavefront max per block, then one atomic per wavefront.
+__global__ void k_gershgorin(const int64_t* rp, const int32_t* col, const double* val, const double* dinv,
+                             int64_t n, int64_t split, unsigned long long* out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double v0 = 0.0, v1 = 0.0;
+  if (i < n) {
+    double acc = 0.0;
+    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) acc += fabs(val[p]);
+    acc *= fabs(dinv[i]);
+    if (i < split)
+      v0 = acc;
+    else
+      v1 = acc;
+  }
+  (void)col;
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 = fmax(v0, __shfl_xor(v0, off, 64));
+    v1 = fmax(v1, __shfl_xor(v1, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (v0 > 0.0) atomicMax(&out[0], (unsigned long long)__double_as_longlong(v0));
+    if (v1 > 0.0) atomicMax(&out[1], (unsigned long long)__double_as_longlong(v1));
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
 inline unsigned red_grid(int64_t n) {
   return (unsigned)std::min<int64_t>(NBLK, std::max<int64_t>(1, (n + TPB - 1) / TPB));
@@ -1675,6 +1708,9 @@ static void ensure_band(KktSystem& s) {
   }
 }
 
+constexpr double kChebLmaxMargin = 1.05;
+constexpr double kChebGershgorinTight = 1.1;
+
 void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
   Ctx* ctx = s.ctx;
   hipStream_t st = ctx->stream;
@@ -1708,8 +1744,13 @@ void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
     k_power_finish<<<2, TPB, 0, st>>>(py.p, px.p, g0, g1, sq.p);
     k_power_normalize<<<grid_for(n), TPB, 0, st>>>(y.p, s.split, n, sq.p, x.p, dlam.p);
   }
+  DBuf<unsigned long long> gb(ctx, 2);
+  k_zero_u64<<<1, 64, 0, st>>>(gb.p, 2);
+  k_gershgorin<<<grid_for(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p, s.inv_diag.p, n, s.split, gb.p);
   FEEC_HIP(hipGetLastError());
   copy_to_host(ctx, lam, dlam.p, sizeof(lam));
+  unsigned long long gbits[2] = {0, 0};
+  copy_to_host(ctx, gbits, gb.p, sizeof(gbits));
   // interval lower ends (fraction of b) tuned on 3D k=1 Kuhn meshes
   const double frac =
       lower > 0.0 ? lower
@@ -1725,7 +1766,14 @@ void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
   double rho[2], sigma[2], delta[2];
   for (int b = 0; b < 2; ++b) {
     s.lambda_max[b] = lam[b];
-    double bb = 1.05 * lam[b], aa = bb * frac;
+    // the interval's top: the Gershgorin bound where it is within 10% of the
+    // power estimate (it bounds lambda_max, which 30 power steps can
+    // underestimate by more than 5% when the top of the spectrum is
+    // clustered), else 1.05 times the estimate
+    double gersh;
+    std::memcpy(&gersh, &gbits[b], sizeof(double));
+    double bb = (gersh > 0.0 && gersh <= kChebGershgorinTight * lam[b]) ? gersh : kChebLmaxMargin * lam[b];
+    double aa = bb * frac;
     double theta = 0.5 * (bb + aa);
     delta[b] = 0.5 * (bb - aa);
     sigma[b] = theta / delta[b];

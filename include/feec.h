@@ -342,13 +342,33 @@ feec_status feec_tune_table(char *out, int64_t *len);
 feec_status feec_ctx_stream_rates(feec_ctx *ctx, int64_t bytes, double rates[3]);
 
 /* ---- problems ------------------------------------------------------------ */
-/* b_k and an M_k-orthonormal harmonic basis [ndofs][count] (basis may be NULL).
- * Closed forms on a ball (full k=0: the constant; relative k=n: M_n^-1 times the
- * orientation, b_k = 0 otherwise). A mesh that fails the ball test (one connected
- * component, Euler characteristic 1) gives FEEC_E_UNSUPPORTED at every grade, here
- * and in every entry point that needs the harmonic border (feec_mixed_kkt,
- * feec_solve_source, feec_problem_create). */
+/* b_k and an M_k-orthonormal harmonic basis [count][ndofs] (basis may be NULL): the
+ * reading solve_harmonics returns (elliptic.rs:197-220). Kuhn boxes take the closed forms
+ * (full k=0: the constant; relative k=n: M_n^-1 times the orientation; b_k = 0 otherwise).
+ * Other meshes: b_k from exact ranks, then the projected integral cocycles of
+ * harmonics (harmonic.rs:118-175), or the same closed forms where they are the whole
+ * space on a connected mesh. FEEC_E_UNSUPPORTED: more than 8 harmonic forms, an
+ * indefinite L2 pairing on the harmonic space (the reference's eigensolve fallback is
+ * not built), or exact elimination beyond its budget. */
 feec_status feec_harmonics(feec_geom *geom, feec_bc bc, int grade, int64_t *count, double *basis);
+/* The period-normalized harmonic basis [count][ndofs] (Harmonics::integral,
+ * harmonic.rs:118-175): column i pairs to delta_ij with the j-th integral cycle of
+ * feec_homology_generators. */
+feec_status feec_harmonics_integral(feec_geom *geom, feec_bc bc, int grade, int64_t *count,
+                                    double *basis);
+/* Betti number b_k of the complex (FEEC_FULL; Complex::betti_number, homology.rs:47-52)
+ * or relative to its boundary (FEEC_RELATIVE; relative_betti_number, homology.rs:81-98):
+ * n_k - rank d_k - rank d_{k+1}, ranks over two prime fields (exact.rs:97-147). */
+feec_status feec_betti_number(const feec_mesh *mesh, feec_bc bc, int grade, int64_t *b);
+/* Integral representatives of a basis of the free part of H^k (cocycles;
+ * cohomology.rs:55-90) or of H_k (cycles; homology.rs:116-153) on the complex's DOFs
+ * (relative: interior numbering), [count][ndofs] (out may be NULL to query count): the
+ * RREF kernel vectors kept by quotient_generators (exact.rs:178-264), primitive, first
+ * nonzero entry positive. */
+feec_status feec_cohomology_generators(const feec_mesh *mesh, feec_bc bc, int grade,
+                                       int64_t *count, int64_t *out);
+feec_status feec_homology_generators(const feec_mesh *mesh, feec_bc bc, int grade,
+                                     int64_t *count, int64_t *out);
 /* The symmetric mixed KKT (elliptic.rs:54-114), with its harmonic border and
  * the sigma-row sign flip; sizes = {n_sigma, n_u, n_harmonic}. */
 feec_status feec_mixed_kkt(feec_geom *geom, feec_bc bc, int grade, feec_csr **out,

@@ -50,6 +50,11 @@ MASS, DIF_TRIAL, DIF_TEST, DIF_BOTH = 0, 1, 2, 3
 #   44 the ambient pullback by Lambda^k(J^T) (J transposed)
 #   45 the de Rham map's 1/k! (the unit simplex volume) left off
 #   46 the L2 error without the cell volume
+# and in the exact (co)homology and harmonic bases:
+#   51 quotient_generators keeps the kernel vectors that do NOT enlarge the span
+#   52 primitive() leaves the first nonzero entry negative
+#   53 the period normalization left out (integral = H)
+#   54 the harmonic projection h = z + D p (the sign of the exact part flipped)
 MUTATION = 0
 
 i64p = C.POINTER(C.c_int64)
@@ -448,10 +453,19 @@ class WhitneyComplex:
             return np.nonzero(self._interior[k])[0]
         return np.arange(self.ndofs(k))
 
-    def harmonics(self, k):
-        """An M_k-orthonormal harmonic basis on a ball: b_0 = 1 on the full
+    def harmonics(self, k, method="auto"):
+        """An M_k-orthonormal harmonic basis (elliptic.rs:197-220).
+
+        method "closed": the closed forms on a ball: b_0 = 1 on the full
         complex (the constant), b_n = 1 on the relative one (M_n^{-1} times the
-        orientation, dirac.rs:553-568), zero otherwise."""
+        orientation, dirac.rs:553-568), zero otherwise. "exact": the projected
+        integral cocycles of harmonic.rs:118-175 (``harmonics_exact``), for any
+        topology. "auto": "closed" on connected complexes with Euler
+        characteristic 1 (every such test mesh is a ball), else "exact"."""
+        if method == "auto":
+            method = "closed" if _looks_like_ball(self.cx) else "exact"
+        if method == "exact":
+            return harmonics_exact(self, k)[1]
         n = self.ndofs(k)
         if not self.relative and k == 0:
             h = np.ones(n)
@@ -464,6 +478,220 @@ class WhitneyComplex:
         m = self.mass(k)
         h = h / math.sqrt(h @ (m @ h))
         return h.reshape(n, 1)
+
+
+# ---------------------------------------------------------------------------
+# Exact (co)homology and harmonic bases (simplicial/src/linalg/exact.rs,
+# simplicial/src/topology/{homology,cohomology}.rs, formoniq/src/harmonic.rs)
+# ---------------------------------------------------------------------------
+def _looks_like_ball(cx):
+    chi = sum((-1) ** k * cx.nsimplices(k) for k in range(cx.dim + 1))
+    if chi != 1:
+        return False
+    # one component: vertices joined through the cells
+    parent = list(range(cx.nsimplices(0)))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+    for c in cx.cells():
+        for v in c[1:]:
+            ra, rb = find(int(c[0])), find(int(v))
+            if ra != rb:
+                parent[ra] = rb
+    return len({find(v) for v in range(len(parent))}) == 1
+
+
+def integral_coboundary(cx, k, relative=False):
+    """D_k: C^k -> C^{k+1} as a dense integer matrix on the DOFs (relative:
+    the interior simplices), the transpose of integral_boundary(k+1)
+    (cohomology.rs:38-45); empty outside 0 <= k < dim."""
+    def dofs(j):
+        if j < 0 or j > cx.dim:
+            return np.zeros(0, np.int64)
+        if relative:
+            return np.nonzero(cx.boundary_flags(j) == 0)[0]
+        return np.arange(cx.nsimplices(j))
+    rows, cols = dofs(k + 1), dofs(k)
+    if k < 0 or k >= cx.dim or len(rows) == 0 or len(cols) == 0:
+        return np.zeros((len(rows), len(cols)), np.int64)
+    full = np.rint(cx.coboundary(k).todense()).astype(np.int64)
+    return full[np.ix_(rows, cols)]
+
+
+def rank_mod(a, p):
+    """exact.rs:117-147: rows reduced in order against pivot rows keyed by
+    their leading column, each pivot normalized to a leading 1 mod p."""
+    pivots = {}
+    rank = 0
+    for row in np.asarray(a, dtype=np.int64) % p:
+        row = row.copy()
+        while True:
+            nz = np.nonzero(row)[0]
+            if len(nz) == 0:
+                break
+            lead = int(nz[0])
+            if lead not in pivots:
+                inv = pow(int(row[lead]), p - 2, p)
+                pivots[lead] = (row * inv) % p
+                rank += 1
+                break
+            row = (row - int(row[lead]) * pivots[lead]) % p
+    return rank
+
+
+def exact_rank(a):
+    """exact.rs:106-115: the larger of the ranks over F_p for two primes."""
+    a = np.asarray(a, dtype=np.int64)
+    if a.size == 0:
+        return 0
+    return max(rank_mod(a, 1_000_000_007), rank_mod(a, 1_000_000_009))
+
+
+def _rref(mat, ncols):
+    """exact.rs:266-298 over Fractions: column by column, the first row at or
+    below the current one with a nonzero entry is swapped up, normalized and
+    eliminated from every other row."""
+    from fractions import Fraction
+    mat = [[Fraction(int(x)) for x in r] for r in mat]
+    nrows = len(mat)
+    pivot_cols = []
+    row = 0
+    for col in range(ncols):
+        if row >= nrows:
+            break
+        sel = next((r for r in range(row, nrows) if mat[r][col] != 0), None)
+        if sel is None:
+            continue
+        mat[row], mat[sel] = mat[sel], mat[row]
+        inv = 1 / mat[row][col]
+        mat[row] = [x * inv for x in mat[row]]
+        for r in range(nrows):
+            if r != row and mat[r][col] != 0:
+                f = mat[r][col]
+                mat[r] = [x - f * y for x, y in zip(mat[r], mat[row])]
+        pivot_cols.append(col)
+        row += 1
+    return mat, pivot_cols
+
+
+def exact_kernel(a):
+    """exact.rs:149-175: one kernel vector per free column of the RREF, a 1
+    there and minus the RREF entries at the pivot columns."""
+    from fractions import Fraction
+    a = np.asarray(a, dtype=np.int64)
+    ncols = a.shape[1]
+    mat, pivots = _rref(a.tolist(), ncols)
+    is_pivot = set(pivots)
+    out = []
+    for free in range(ncols):
+        if free in is_pivot:
+            continue
+        v = [Fraction(0)] * ncols
+        v[free] = Fraction(1)
+        for r, pc in enumerate(pivots):
+            v[pc] = -mat[r][free]
+        out.append(v)
+    return out
+
+
+def _primitive(v):
+    """exact.rs:300-330: denominators cleared, gcd divided out, the first
+    nonzero entry positive."""
+    lcm = 1
+    for x in v:
+        lcm = lcm * x.denominator // math.gcd(lcm, x.denominator)
+    ints = [int(x * lcm) for x in v]
+    g = 0
+    for x in ints:
+        g = math.gcd(g, x)
+    if g:
+        ints = [x // g for x in ints]
+    first = next((x for x in ints if x != 0), 0)
+    if first < 0 and MUTATION != 52:
+        ints = [-x for x in ints]
+    return ints
+
+
+def quotient_generators(a, b):
+    """exact.rs:178-264: representatives of ker A / im B. An echelon span is
+    seeded with B's columns; each RREF kernel vector of A, in free-column
+    order, is reduced against it and kept (primitive) when it enlarges it."""
+    from fractions import Fraction
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    assert a.shape[1] == b.shape[0]
+    pivots = {}
+
+    def insert(vec):
+        red = list(vec)
+        while True:
+            lead = next((i for i, x in enumerate(red) if x != 0), None)
+            if lead is None:
+                return False
+            if lead not in pivots:
+                inv = 1 / red[lead]
+                pivots[lead] = [x * inv for x in red]
+                return True
+            f = red[lead]
+            red = [x - f * y for x, y in zip(red, pivots[lead])]
+
+    for j in range(b.shape[1]):
+        insert([Fraction(int(x)) for x in b[:, j]])
+    out = []
+    for v in exact_kernel(a):
+        grew = insert(v)
+        if grew != (MUTATION == 51):
+            out.append(_primitive(v))
+    return out
+
+
+def betti_number(cx, k, relative=False):
+    """homology.rs:47-52 / 81-98: n_k - rank d_k - rank d_{k+1}, d_k = D_{k-1}^T."""
+    lo, hi = integral_coboundary(cx, k - 1, relative), integral_coboundary(cx, k, relative)
+    return lo.shape[0] - exact_rank(lo) - exact_rank(hi)
+
+
+def cohomology_generators(cx, k, relative=False):
+    """cohomology.rs:55-90 on the DOFs: ker D_k / im D_{k-1}."""
+    return quotient_generators(integral_coboundary(cx, k, relative), integral_coboundary(cx, k - 1, relative))
+
+
+def homology_generators(cx, k, relative=False):
+    """homology.rs:116-153 on the DOFs: ker d_k / im d_{k+1}."""
+    return quotient_generators(integral_coboundary(cx, k - 1, relative).T,
+                               integral_coboundary(cx, k, relative).T)
+
+
+def harmonics_exact(c: WhitneyComplex, k):
+    """harmonic.rs:118-175: (integral, orthonormal), each ndofs x b_k. Every
+    integral cocycle z is projected to h = z - D p, D = dif(k-1), p minimizing
+    |z - D p|_M (the reference's CG on D^T M D p = D^T M z; any solution gives
+    the same h), then integral = H (P^T)^-1 with P_ij = <z_i, c_j> against the
+    integral cycles, and orthonormal = integral L^-T for G = L L^T the Gram
+    matrix in M. Raises ValueError where G is not positive definite."""
+    z = cohomology_generators(c.cx, k, c.relative)
+    cyc = homology_generators(c.cx, k, c.relative)
+    n = c.ndofs(k)
+    if not z:
+        return np.zeros((n, 0)), np.zeros((n, 0))
+    M = c.mass(k).todense()
+    Z = np.array(z, dtype=np.float64).T
+    D = c.dif(k - 1).todense() if k >= 1 else np.zeros((n, 0))
+    if D.shape[1]:
+        Lm = np.linalg.cholesky(M)
+        p, *_ = np.linalg.lstsq(Lm.T @ D, Lm.T @ Z, rcond=None)
+        H = Z + D @ p if MUTATION == 54 else Z - D @ p
+    else:
+        H = Z
+    P = np.array([[sum(int(x) * int(y) for x, y in zip(zi, cj)) for cj in cyc] for zi in z], dtype=np.float64)
+    integral = H if MUTATION == 53 else H @ np.linalg.inv(P.T)
+    G = integral.T @ M @ integral
+    L = np.linalg.cholesky(G)
+    ortho = np.linalg.solve(L, integral.T).T
+    return integral, ortho
 
 
 def hodge_blocks(c: WhitneyComplex, k):
@@ -550,6 +778,16 @@ def _minres_direct_blocks(c, k, kkt, rhs, nh, rtol, max_iters):
 
 CHEB_POWER_ITERS = 30
 CHEB_LMAX_MARGIN = 1.05
+CHEB_GERSHGORIN_TIGHT = 1.1
+
+
+def cheb_interval_top(lmax, gershgorin):
+    """The Chebyshev interval's upper end: the Gershgorin bound of D^-1 P
+    (max_i sum_j |P_ij| / P_ii, never below lambda_max) where it lies within
+    10% of the power estimate, else 1.05 times the estimate."""
+    if 0.0 < gershgorin <= CHEB_GERSHGORIN_TIGHT * lmax:
+        return gershgorin
+    return CHEB_LMAX_MARGIN * lmax
 
 
 def cheb_interval_fraction(m):
@@ -571,11 +809,11 @@ def _xorshift_start(n):
     return out
 
 
-def chebyshev_coefficients(lmax, m):
+def chebyshev_coefficients(lmax, m, top=None):
     """(1/theta, [(rho_{k+1} rho_k, 2 rho_{k+1}/delta)] for k < m-1) of the
-    Chebyshev iteration on [f b, b], b = 1.05 lmax (Saad, Iterative Methods
-    for Sparse Linear Systems, 2nd ed., Algorithm 12.1)."""
-    b = CHEB_LMAX_MARGIN * lmax
+    Chebyshev iteration on [f b, b], b = top, or 1.05 lmax without it (Saad,
+    Iterative Methods for Sparse Linear Systems, 2nd ed., Algorithm 12.1)."""
+    b = CHEB_LMAX_MARGIN * lmax if top is None else top
     a = b * cheb_interval_fraction(m)
     theta, delta = 0.5 * (b + a), 0.5 * (b - a)
     sigma = theta / delta
@@ -623,7 +861,8 @@ def chebyshev_jacobi(c: WhitneyComplex, k, nh, m):
     """Build-defined stand-in for the reference's direct block preconditioner
     diag(hdif_gram(k-1)^-1, hdif_gram(k)^-1, I) (elliptic.rs:29-47): per
     block, a degree-(m-1) Chebyshev polynomial in D^-1 hdif_gram with D its
-    diagonal and lambda_max from 30 power iterations. Returns prec(r)."""
+    diagonal, lambda_max from 30 power iterations and the interval's top
+    from cheb_interval_top. Returns prec(r)."""
     import scipy.sparse as sp
     blocks = ([c.hdif_gram(k - 1).to_scipy()] if k > 0 else []) + [c.hdif_gram(k).to_scipy()]
     P = sp.block_diag(blocks, format="csr")
@@ -643,7 +882,9 @@ def chebyshev_jacobi(c: WhitneyComplex, k, nh, m):
                 lam[b] = ny / nx
             if ny > 0:
                 x[lo:hi] = y[lo:hi] / ny
-    coefs = [chebyshev_coefficients(lam[b], m) for b in range(2)]
+    rows = np.abs(P).sum(axis=1).A1 * dinv
+    gersh = [rows[lo:hi].max() if hi > lo else 0.0 for lo, hi in bounds]
+    coefs = [chebyshev_coefficients(lam[b], m, cheb_interval_top(lam[b], gersh[b])) for b in range(2)]
     Pb = [blocks[0] if k > 0 else None, blocks[-1]]
 
     def prec(r):

@@ -1,6 +1,7 @@
 // Exercises the host-only code of libfeec (the meshers, option defaults,
-// status strings, element shapes, argument checks, and the multigrid set-up's
-// dense Cholesky inverse and tridiagonal eigenvalue bisection) in a build
+// status strings, element shapes, argument checks, the multigrid set-up's
+// dense Cholesky inverse and tridiagonal eigenvalue bisection, and the exact
+// (co)homology eliminations) in a build
 // instrumented by AddressSanitizer and UndefinedBehaviorSanitizer. No GPU is
 // touched. Exit status 0 and the line "host_check ok" mean every check held.
 #include <cmath>
@@ -117,6 +118,59 @@ static void tridiagonal() {
   EXPECT(std::fabs(feec::tridiagonal_lmax(dg, off) - 3.0) < 1e-12);
 }
 
+static feec::IntMat dense_int(int64_t r, int64_t c, const std::vector<int64_t>& v) {
+  feec::IntMat a;
+  a.nrows = r;
+  a.ncols = c;
+  a.rows.assign(r, {});
+  for (int64_t i = 0; i < r; ++i)
+    for (int64_t j = 0; j < c; ++j)
+      if (v[i * c + j]) a.rows[i].push_back({(int32_t)j, v[i * c + j]});
+  return a;
+}
+
+// The exact (co)homology routines on incidence matrices built here: the
+// triangle's boundary (a circle), a long path (a tree) and a long cycle.
+static void exact_topology() {
+  using feec::IntMat;
+  const IntMat d0 = dense_int(3, 3, {-1, 1, 0, -1, 0, 1, 0, -1, 1});  // edges 01, 02, 12
+  const IntMat no_rows = dense_int(0, 3, {}), no_cols = dense_int(3, 0, {});
+  EXPECT(feec::exact_rank(d0) == 2);
+  EXPECT(feec::exact_rank(no_rows) == 0);
+  const auto h0 = feec::quotient_generators(d0, no_cols);  // H^0: the constant
+  EXPECT(h0.size() == 1 && h0[0] == std::vector<int64_t>({1, 1, 1}));
+  const auto h1 = feec::quotient_generators(no_rows, d0);  // H^1: the first edge's cochain
+  EXPECT(h1.size() == 1 && h1[0] == std::vector<int64_t>({1, 0, 0}));
+  const auto c1 = feec::quotient_generators(feec::int_transpose(d0), no_cols);  // H_1: the loop
+  EXPECT(c1.size() == 1 && c1[0] == std::vector<int64_t>({1, -1, 1}));
+  const auto c0 = feec::quotient_generators(no_rows, feec::int_transpose(d0));  // H_0: a vertex
+  EXPECT(c0.size() == 1 && c0[0] == std::vector<int64_t>({1, 0, 0}));
+  for (bool closed : {false, true}) {
+    const int64_t n = 3000, ne = closed ? n : n - 1;
+    IntMat d;  // edge i joins vertices i and i + 1 (mod n)
+    d.nrows = ne;
+    d.ncols = n;
+    d.rows.assign(ne, {});
+    for (int64_t e = 0; e < ne; ++e) {
+      const int32_t a = (int32_t)e, b = (int32_t)((e + 1) % n);
+      if (a < b)
+        d.rows[e] = {{a, -1}, {b, 1}};
+      else
+        d.rows[e] = {{b, 1}, {a, -1}};
+    }
+    EXPECT(feec::exact_rank(d) == n - 1);
+    const auto z0 = feec::quotient_generators(d, dense_int(n, 0, {}));
+    EXPECT(z0.size() == 1 && z0[0] == std::vector<int64_t>(n, 1));
+    const auto cyc = feec::quotient_generators(feec::int_transpose(d), dense_int(ne, 0, {}));
+    EXPECT(cyc.size() == (closed ? 1u : 0u));
+    if (closed && cyc.size() == 1) {
+      bool unit = true;
+      for (int64_t v : cyc[0]) unit = unit && (v == 1 || v == -1);
+      EXPECT(unit && cyc[0][0] == 1);
+    }
+  }
+}
+
 // The ingestion readers on valid files, on every truncation of them and on
 // byte-flipped copies: each call returns a status, never reads out of bounds.
 static void ingestion() {
@@ -188,6 +242,7 @@ static void ingestion() {
 }
 
 int main() {
+  exact_topology();
   ingestion();
   meshers();
   options_and_strings();

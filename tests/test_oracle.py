@@ -514,3 +514,44 @@ def test_known_answers_catch_planted_mistakes(m, golden, tmp_path):
                       (o.DIF_TEST, "dif_test_n2_k1")):
         failures += not np.allclose(o.element_matrix(2, kind, 1, s, L=L), golden[key]["value"], rtol=1e-12)
     assert failures > 0, MUTATIONS[m]
+
+
+def test_chebyshev_interval_top_bounds_lambda_max():
+    """The interval's top is never below lambda_max(D^-1 hdif_gram) (dense
+    eigensolve), also on the solid-torus slab whose vertex block 30 power
+    steps underestimate by 6% (1.05 times that estimate would not cover it),
+    and the Chebyshev-preconditioned MINRES there needs more than one step
+    (a polynomial that is not positive on the spectrum breaks the Lanczos
+    recurrence at once)."""
+    from _homology_meshes import FIXTURES, squared_lengths
+    cases = []
+    dim, cells, geom = FIXTURES["tunnel"]()
+    cx = o.Complex(dim, cells)
+    cases.append((cx, squared_lengths(cx, geom), dim))
+    cells, coords = jittered(2, 8, 0.1, 1)
+    cx2 = o.Complex(2, cells)
+    cases.append((cx2, cx2.lengths_from_coords(coords), 2))
+    for cx, L, dim in cases:
+        c = o.WhitneyComplex(cx, L)
+        for k in range(dim + 1):
+            blocks = ([c.hdif_gram(k - 1)] if k else []) + [c.hdif_gram(k)]
+            P = [g.todense() for g in blocks]
+            for A in P:
+                d = np.diag(A)
+                lmax = np.linalg.eigvals(A / d[:, None]).real.max()
+                rows = (np.abs(A).sum(1) / d).max()
+                assert rows >= lmax * (1 - 1e-12)
+            prec = o.chebyshev_jacobi(c, k, 0, 16)
+            for b, A in enumerate(P):
+                bi = b if k else 1
+                d = np.diag(A)
+                lmax = np.linalg.eigvals(A / d[:, None]).real.max()
+                top = o.cheb_interval_top(prec.lambda_max[bi], (np.abs(A).sum(1) / d).max())
+                assert top >= lmax * (1 - 1e-12), (k, b, top, lmax)
+    c = o.WhitneyComplex(cases[0][0], cases[0][1])
+    f = np.cos(1.0 + np.arange(cases[0][0].nsimplices(0)))
+    kkt, rhs, ns, nu = o.assemble_mixed_kkt(c, f, 0, c.harmonics(0))
+    x, rep = o.minres_cheb(kkt, rhs, o.chebyshev_jacobi(c, 0, 1, 16))
+    assert rep.converged and rep.iters > 1
+    r = kkt.to_scipy() @ x - rhs
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(rhs)
