@@ -222,6 +222,11 @@ struct Mesh {
   // cached assembly plans keyed by (bc, row grade, col grade)
   std::map<std::tuple<int, int, int>, std::unique_ptr<CooPlan>> plans;
   std::mutex mu;
+  // exact topology on first request: Betti numbers by (bc, grade), integral
+  // generators by (bc, grade, 0 cocycles / 1 cycles); guarded by topo_mu
+  std::map<std::pair<int, int>, int64_t> betti;
+  std::map<std::tuple<int, int, int>, std::vector<std::vector<int64_t>>> generators;
+  std::mutex topo_mu;
 
   int nfaces(int k) const { return binom(d + 1, k + 1); }
   void ensure_interior();

@@ -235,6 +235,46 @@ IntMat coboundary_int(Mesh& m, int bc, int k) {
   return a;
 }
 
+// The rank of D_0 on the DOFs by union-find. Its rows are a graph's
+// incidence rows (-1 at the smaller live end, +1 at the larger; one entry
+// where the other end is a removed boundary vertex, none where both are), so
+// its rank over every field is the vertex count minus the components that do
+// not reach a removed vertex (one extra "ground" vertex stands for those).
+// -1 when a row is not of that form.
+int64_t graph_rank(const IntMat& d0) {
+  const int64_t n = d0.ncols;
+  std::vector<int64_t> parent(n + 1);
+  for (int64_t v = 0; v <= n; ++v) parent[v] = v;
+  auto find = [&](int64_t v) {
+    while (parent[v] != v) {
+      parent[v] = parent[parent[v]];
+      v = parent[v];
+    }
+    return v;
+  };
+  int64_t merges = 0;
+  auto unite = [&](int64_t a, int64_t b) {
+    a = find(a);
+    b = find(b);
+    if (a != b) {
+      parent[a] = b;
+      ++merges;
+    }
+  };
+  for (const auto& r : d0.rows) {
+    if (r.size() == 2) {
+      if (r[0].second != -1 || r[1].second != 1) return -1;
+      unite(r[0].first, r[1].first);
+    } else if (r.size() == 1) {
+      if (r[0].second != -1 && r[0].second != 1) return -1;
+      unite(r[0].first, n);
+    } else if (!r.empty()) {
+      return -1;
+    }
+  }
+  return merges;  // (n + 1) - components of the graph with its ground vertex
+}
+
 }  // namespace
 
 IntMat int_transpose(const IntMat& a) {
@@ -362,24 +402,57 @@ std::vector<std::vector<int64_t>> quotient_generators(const IntMat& A, const Int
 int64_t betti_number(Mesh& m, int bc, int k) {
   FEEC_REQUIRE(k >= 0 && k <= m.d, FEEC_E_ARG, "grade out of range");
   if (kuhn_grid_n(m) > 0) return (bc == FEEC_RELATIVE ? k == m.d : k == 0) ? 1 : 0;
+  {
+    std::lock_guard<std::mutex> lock(m.topo_mu);
+    auto it = m.betti.find({bc, k});
+    if (it != m.betti.end()) return it->second;
+  }
+  // rank d_j = rank D_{j-1}; D_0's by union-find
+  auto rank_of = [&](int j, const IntMat& d) -> int64_t {
+    if (j == 0) {
+      const int64_t g = graph_rank(d);
+      if (g >= 0) return g;
+    }
+    return exact_rank(int_transpose(d));
+  };
   const IntMat lo = coboundary_int(m, bc, k - 1), hi = coboundary_int(m, bc, k);
   const int64_t nk = lo.nrows;
-  return nk - exact_rank(int_transpose(lo)) - exact_rank(int_transpose(hi));
+  const int64_t b = nk - rank_of(k - 1, lo) - rank_of(k, hi);
+  std::lock_guard<std::mutex> lock(m.topo_mu);
+  m.betti[{bc, k}] = b;
+  return b;
 }
 
 // Integral cocycles of grade k on the complex's DOFs (cohomology.rs:55-90):
 // ker D_k / im D_{k-1}.
+namespace {
+// generators of one kind (0 cocycles, 1 cycles), computed once per mesh
+std::vector<std::vector<int64_t>> cached_generators(Mesh& m, int bc, int k, int kind) {
+  {
+    std::lock_guard<std::mutex> lock(m.topo_mu);
+    auto it = m.generators.find({bc, k, kind});
+    if (it != m.generators.end()) return it->second;
+  }
+  std::vector<std::vector<int64_t>> g =
+      kind == 0 ? quotient_generators(coboundary_int(m, bc, k), coboundary_int(m, bc, k - 1))
+                : quotient_generators(int_transpose(coboundary_int(m, bc, k - 1)),
+                                      int_transpose(coboundary_int(m, bc, k)));
+  std::lock_guard<std::mutex> lock(m.topo_mu);
+  m.generators[{bc, k, kind}] = g;
+  return g;
+}
+}  // namespace
+
 std::vector<std::vector<int64_t>> cohomology_generators(Mesh& m, int bc, int k) {
   FEEC_REQUIRE(k >= 0 && k <= m.d, FEEC_E_ARG, "grade out of range");
-  return quotient_generators(coboundary_int(m, bc, k), coboundary_int(m, bc, k - 1));
+  return cached_generators(m, bc, k, 0);
 }
 
 // Integral cycles of grade k on the complex's DOFs (homology.rs:116-153):
 // ker d_k / im d_{k+1}, d_k = D_{k-1}^T.
 std::vector<std::vector<int64_t>> homology_generators(Mesh& m, int bc, int k) {
   FEEC_REQUIRE(k >= 0 && k <= m.d, FEEC_E_ARG, "grade out of range");
-  return quotient_generators(int_transpose(coboundary_int(m, bc, k - 1)),
-                             int_transpose(coboundary_int(m, bc, k)));
+  return cached_generators(m, bc, k, 1);
 }
 
 // The harmonic bases from integral cohomology (harmonic.rs:118-175): each
