@@ -425,6 +425,10 @@ int kkt_preconditioner_format(const KktSystem& s, int precision, int storage);
 double time_cheb_step(KktSystem& s, bool fp32, int reps, int storage);
 // Jacobi-preconditioned CG (krylov.rs:48-95: x0 = 0, stop when |r| <= rtol |b|)
 // with its scalars on the device (host read every 8 iterations).
+// The spectrum estimates' start vector (n entries on the device): 0.5 + U[0,1)
+// from xorshift64 (13, 7, 17) seeded 0x2545F4914F6CDD1D, the i-th entry from
+// the generator's (i+1)-th state.
+void xorshift_start(Ctx* ctx, double* out, int64_t n);
 KrylovStats cg_solve_device(Ctx* ctx, const Csr& a, const double* dinv, const double* b, double* x,
                             double rtol, int64_t max_iters);
 // Sets up (once per device, that device current) the address of the scratch

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -978,6 +979,23 @@ __global__ void k_gershgorin(const int64_t* rp, const int32_t* col, const double
   }
 }
 
+// Chunks of the power-iteration start vector 0.5 + U[0,1) from xorshift64
+// (13, 7, 17): thread j steps from the generator's state after j * chunk
+// steps, so the vector equals the sequential one bit for bit.
+__global__ void k_xorshift_chunks(const unsigned long long* starts, int64_t n, int chunk, double* out) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i0 = j * chunk;
+  if (i0 >= n) return;
+  unsigned long long v = starts[j];
+  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  for (int64_t i = i0; i < i1; ++i) {
+    v ^= v << 13;
+    v ^= v >> 7;
+    v ^= v << 17;
+    out[i] = 0.5 + (double)(v >> 11) * (1.0 / 9007199254740992.0);
+  }
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
@@ -1708,6 +1726,50 @@ static void ensure_band(KktSystem& s) {
   }
 }
 
+// The start vector of the spectrum estimates: x_i = 0.5 + U[0,1) from the
+// (i+1)-th state of xorshift64 (13, 7, 17) seeded 0x2545F4914F6CDD1D. The
+// generator is linear over GF(2), so the state after k steps is T^k applied to
+// the seed, T the 64x64 bit matrix of one step; the host steps through the
+// chunk starts with T^chunk and the device expands the chunks.
+void xorshift_start(Ctx* ctx, double* out, int64_t n) {
+  if (n <= 0) return;
+  constexpr int kChunk = 256;
+  using Mat = std::array<uint64_t, 64>;  // column b = the image of bit b
+  auto step = [](uint64_t v) {
+    v ^= v << 13;
+    v ^= v >> 7;
+    v ^= v << 17;
+    return v;
+  };
+  auto apply = [](const Mat& m, uint64_t v) {
+    uint64_t r = 0;
+    for (int b = 0; b < 64; ++b)
+      if ((v >> b) & 1) r ^= m[b];
+    return r;
+  };
+  Mat t, p;
+  for (int b = 0; b < 64; ++b) t[b] = step(uint64_t(1) << b);
+  // p = T^kChunk by squaring (kChunk is a power of two)
+  p = t;
+  for (int e = 1; e < kChunk; e *= 2) {
+    Mat q;
+    for (int b = 0; b < 64; ++b) q[b] = apply(p, p[b]);
+    p = q;
+  }
+  const int64_t nchunks = (n + kChunk - 1) / kChunk;
+  std::vector<unsigned long long> starts(nchunks);
+  uint64_t v = 0x2545F4914F6CDD1Dull;
+  for (int64_t j = 0; j < nchunks; ++j) {
+    starts[j] = v;
+    v = apply(p, v);
+  }
+  DBuf<unsigned long long> d(ctx, nchunks);
+  copy_to_device(ctx, d.p, starts.data(), nchunks * 8);
+  k_xorshift_chunks<<<grid_for(nchunks), TPB, 0, ctx->stream>>>(d.p, n, kChunk, out);
+  FEEC_HIP(hipGetLastError());
+  ctx->sync();
+}
+
 constexpr double kChebLmaxMargin = 1.05;
 constexpr double kChebGershgorinTight = 1.1;
 
@@ -1719,15 +1781,7 @@ void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
   ensure_band(s);
   const int64_t n = s.n;
   DBuf<double> x(ctx, n), y(ctx, n), part(ctx, NBLK + 1);
-  std::vector<double> hx(n);
-  uint64_t st_rng = 0x2545F4914F6CDD1Dull;
-  for (int64_t i = 0; i < n; ++i) {
-    st_rng ^= st_rng << 13;
-    st_rng ^= st_rng >> 7;
-    st_rng ^= st_rng << 17;
-    hx[i] = 0.5 + (double)(st_rng >> 11) * (1.0 / 9007199254740992.0);
-  }
-  copy_to_device(ctx, x.p, hx.data(), n * 8);
+  xorshift_start(ctx, x.p, n);
   // per block: |y|^2 and |x|^2 (fixed-order sums) and lambda, all on the device
   // so the 30 iterations run without a host round trip
   // four launches per iteration: y = D^-1 P x; per block |y|^2, |x|^2 (fixed

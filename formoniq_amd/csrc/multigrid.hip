@@ -2160,15 +2160,7 @@ void lanczos(Hx& h) {
   const int64_t n = h.n, split = h.split;
   if (!h.lz_exec && !h.lz_direct_ready) {
     // the start vector, 0.5 + U[0,1) from xorshift64 (13, 7, 17), the same every time
-    std::vector<double> x(n);
-    uint64_t v = 0x2545F4914F6CDD1DULL;
-    for (int64_t i = 0; i < n; ++i) {
-      v ^= v << 13;
-      v ^= v >> 7;
-      v ^= v << 17;
-      x[i] = 0.5 + (double)(v >> 11) * (1.0 / 9007199254740992.0);
-    }
-    copy_to_device(ctx, h.pvec.p, x.data(), n * 8);
+    xorshift_start(ctx, h.pvec.p, n);
     if (h.use_graph) {
       FEEC_HIP(hipGraphCreate(&h.lz_graph, 0));
       Enq q{st, h.lz_graph, &h.lz_keep, nullptr};
