@@ -208,6 +208,66 @@ __global__ void k_soa_to_aos(const double* soa, int64_t nc, int per, double* aos
 }
 
 // vector contributions: entry rows[t] = base_row + map(faces[cell][i]), col 0
+// The preconditioner's plan from the KKT plan. blockdiag(hdif_gram(k-1),
+// hdif_gram(k)) has the pattern of the KKT's two diagonal blocks, whose plan
+// entries hold one contribution per (cell, local pair): -M_{k-1} on the sigma
+// block, dif_both(k+1) on the u block. The sort of P's own contributions (M
+// pushed before dif_both) would give each entry the M contributions of the
+// same (cell, pair)s in the same order, then the dif_both ones: these kernels
+// write that plan without sorting. Per row: the entries of the row's own block
+// (a contiguous run of the KKT row) and where the run starts.
+__global__ void k_pc_rows(const int64_t* krp, const int32_t* kcol, int64_t n, int64_t ns,
+                          unsigned long long* cnt, int64_t* kfirst) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r > n) return;
+  if (r == n) {
+    cnt[n] = 0;
+    return;
+  }
+  const bool sig = r < ns;
+  int64_t c = 0, first = -1;
+  for (int64_t p = krp[r]; p < krp[r + 1]; ++p)
+    if ((kcol[p] < ns) == sig) {
+      if (first < 0) first = p;
+      ++c;
+    }
+  cnt[r] = (unsigned long long)c;
+  kfirst[r] = first < 0 ? krp[r] : first;
+}
+
+// each P entry's column and contribution count (twice the KKT entry's)
+__global__ void k_pc_entries(const int64_t* prp, const int64_t* kfirst, const int32_t* kcol,
+                             const int64_t* kseg, int64_t n, int32_t* pcol, unsigned long long* plen) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int64_t t = 0; t < prp[r + 1] - prp[r]; ++t) {
+    const int64_t e = kfirst[r] + t, q = prp[r] + t;
+    pcol[q] = kcol[e];
+    plen[q] = 2ull * (unsigned long long)(kseg[e + 1] - kseg[e]);
+  }
+}
+
+// each P entry's sources: the mass planes, then the dif_both planes, of the
+// KKT entry's (cell, pair)s
+__global__ void k_pc_srcs(const int64_t* prp, const int64_t* kfirst, const int64_t* kseg,
+                          const uint32_t* ksrc, const int64_t* pseg, int64_t n, int64_t ns,
+                          uint32_t off_mlo, uint32_t off_dlo, uint32_t off_mhi, uint32_t off_dhi,
+                          uint32_t* psrc) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const bool sig = r < ns;
+  for (int64_t t = 0; t < prp[r + 1] - prp[r]; ++t) {
+    const int64_t e = kfirst[r] + t, q = prp[r] + t;
+    const int64_t L = kseg[e + 1] - kseg[e], base = pseg[q];
+    for (int64_t i = 0; i < L; ++i) {
+      const uint32_t s = ksrc[kseg[e] + i] & ~kCooNegate;
+      const uint32_t local = s - (sig ? off_mlo : off_dhi);  // (pair * ncells + cell)
+      psrc[base + i] = local + (sig ? off_mlo : off_mhi);
+      psrc[base + L + i] = local + (sig ? off_dlo : off_dhi);
+    }
+  }
+}
+
 __global__ void k_diag_coo(const int32_t* faces, int nf, const int32_t* map, int64_t nc, int64_t roff,
                            int stride_diag /* element entry of (i,i) = i*stride_diag */,
                            uint32_t base, int32_t* rows, int32_t* cols, uint32_t* srcs) {
@@ -408,7 +468,39 @@ void problem_build(Problem& P) {
   }
   // preconditioner matrix blockdiag(hdif_gram(k-1), hdif_gram(k)) with
   // hdif_gram(j) = M_j + dif_both(j+1) (whitney_complex.rs:180-183)
-  {
+  if (has_up) {
+    // from the KKT plan (k_pc_rows): no sort of P's contributions
+    CooPlan& pp = P.pc_plan;
+    const CooPlan& kp = P.kkt_plan;
+    pp.nrows = n;
+    pp.ncols = n;
+    pp.row_ptr.alloc(ctx, n + 1);
+    DBuf<unsigned long long> cnt(ctx, n + 1);
+    DBuf<int64_t> kfirst(ctx, std::max<int64_t>(n, 1));
+    k_pc_rows<<<grid_for(n + 1), TPB, 0, ctx->stream>>>(kp.row_ptr.p, kp.col.p, n, P.ns, cnt.p, kfirst.p);
+    FEEC_HIP(hipGetLastError());
+    exclusive_scan_counts(ctx, cnt.p, pp.row_ptr.p, n + 1);
+    copy_to_host(ctx, &pp.nnz, pp.row_ptr.p + n, 8);
+    pp.col.alloc(ctx, pp.nnz);
+    pp.seg.alloc(ctx, pp.nnz + 1);
+    DBuf<unsigned long long> plen(ctx, pp.nnz + 1);
+    FEEC_HIP(hipMemsetAsync(plen.p + pp.nnz, 0, sizeof(unsigned long long), ctx->stream));
+    if (n)
+      k_pc_entries<<<grid_for(n), TPB, 0, ctx->stream>>>(pp.row_ptr.p, kfirst.p, kp.col.p, kp.seg.p, n, pp.col.p,
+                                                         plen.p);
+    FEEC_HIP(hipGetLastError());
+    exclusive_scan_counts(ctx, plen.p, pp.seg.p, pp.nnz + 1);
+    int64_t nsrc = 0;
+    copy_to_host(ctx, &nsrc, pp.seg.p + pp.nnz, 8);
+    FEEC_REQUIRE(nsrc == sizes[0] + sizes[6] + sizes[5] + sizes[2] || bc == FEEC_RELATIVE, FEEC_E_DEVICE,
+                 "preconditioner plan: contribution count");
+    pp.src.alloc(ctx, std::max<int64_t>(nsrc, 1));
+    if (n)
+      k_pc_srcs<<<grid_for(n), TPB, 0, ctx->stream>>>(
+          pp.row_ptr.p, kfirst.p, kp.seg.p, kp.src.p, pp.seg.p, n, P.ns, (uint32_t)P.off[0], (uint32_t)P.off[6],
+          (uint32_t)P.off[5], (uint32_t)P.off[2], pp.src.p);
+    FEEC_HIP(hipGetLastError());
+  } else {
     int64_t total_pc = sizes[0] + sizes[6] + sizes[5] + sizes[2];
     DBuf<int32_t> rows(ctx, std::max<int64_t>(total_pc, 1)), cols(ctx, std::max<int64_t>(total_pc, 1));
     DBuf<uint32_t> srcs(ctx, std::max<int64_t>(total_pc, 1));
@@ -430,6 +522,8 @@ void problem_build(Problem& P) {
       at += sizes[2];
     }
     coo_plan_build(ctx, P.pc_plan, n, n, at, rows.p, cols.p, srcs.p);
+  }
+  {
     Csr& Pm = P.sys.P;
     Pm.ctx = ctx;
     Pm.nrows = n;

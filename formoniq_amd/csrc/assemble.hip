@@ -28,7 +28,7 @@ namespace feec {
 namespace {
 
 constexpr int TPB = 256;
-constexpr uint32_t NEG = 0x80000000u;
+constexpr uint32_t NEG = kCooNegate;
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
 

@@ -178,6 +178,9 @@ void copy_to_host(Ctx* c, void* dst, const void* src, size_t bytes);
 
 // Device COO -> CSR plan: the sorted order of (row, col) keys, kept so that a
 // later value pass is a segmented sum (duplicates summed in input order).
+// A COO source index with this bit set contributes its value negated.
+constexpr uint32_t kCooNegate = 0x80000000u;
+
 struct CooPlan {
   int64_t nrows = 0, ncols = 0;
   int64_t nnz = 0;        // unique (row, col) entries
