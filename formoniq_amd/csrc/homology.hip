@@ -293,6 +293,115 @@ int64_t exact_rank(const IntMat& a) {
   return std::max(rank_mod(a, 1000000007, w), rank_mod(a, 1000000009, w));
 }
 
+namespace {
+
+// A's columns as graph edges (tail, head), when every column is an incidence
+// column: -1 at its tail and +1 at its head, or one of them alone (the other
+// end is the "ground" vertex A.nrows), or empty (a loop at ground). False
+// otherwise. D_0^T (the boundary d_1) has this form, absolute and relative.
+bool graph_columns(const IntMat& A, std::vector<int64_t>& tail, std::vector<int64_t>& head) {
+  const int64_t g = A.nrows;
+  tail.assign(A.ncols, g);
+  head.assign(A.ncols, g);
+  std::vector<int8_t> seen(A.ncols, 0);  // bit 0: a -1 seen, bit 1: a +1 seen
+  for (int64_t r = 0; r < A.nrows; ++r)
+    for (const auto& e : A.rows[r]) {
+      const int bit = e.second == -1 ? 1 : e.second == 1 ? 2 : 0;
+      if (!bit || (seen[e.first] & bit)) return false;
+      seen[e.first] |= bit;
+      (bit == 1 ? tail : head)[e.first] = r;
+    }
+  return true;
+}
+
+// The RREF's free columns and the kernel vectors v_f of the kept ones, for a
+// graph-shaped A. The pivot columns are the lowest-index column basis, i.e.
+// the edges Kruskal keeps in index order (union-find); v_f is the one vector
+// of ker A with a 1 at f on f and pivots before it, i.e. f's fundamental cycle
+// in that forest (the forest's path between f's ends is the same in the final
+// forest), +1 on the edges it traverses forward.
+struct GraphKernel {
+  std::vector<int64_t> tail, head, parent_edge, up, depth;
+  std::vector<char> pivot;
+
+  void build(int64_t nv) {  // nv vertices including ground
+    const int64_t ne = (int64_t)tail.size();
+    std::vector<int64_t> uf(nv);
+    for (int64_t v = 0; v < nv; ++v) uf[v] = v;
+    auto find = [&](int64_t v) {
+      while (uf[v] != v) {
+        uf[v] = uf[uf[v]];
+        v = uf[v];
+      }
+      return v;
+    };
+    pivot.assign(ne, 0);
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> adj(nv);  // (neighbour, edge)
+    for (int64_t e = 0; e < ne; ++e) {
+      const int64_t a = find(tail[e]), b = find(head[e]);
+      if (a == b) continue;
+      uf[a] = b;
+      pivot[e] = 1;
+      adj[tail[e]].push_back({head[e], e});
+      adj[head[e]].push_back({tail[e], e});
+    }
+    // root every tree; parent edge and depth per vertex
+    parent_edge.assign(nv, -1);
+    up.assign(nv, -1);
+    depth.assign(nv, -1);
+    std::vector<int64_t> stack;
+    for (int64_t r = 0; r < nv; ++r) {
+      if (depth[r] >= 0) continue;
+      depth[r] = 0;
+      stack.push_back(r);
+      while (!stack.empty()) {
+        const int64_t v = stack.back();
+        stack.pop_back();
+        for (const auto& [u, e] : adj[v])
+          if (depth[u] < 0) {
+            depth[u] = depth[v] + 1;
+            up[u] = v;
+            parent_edge[u] = e;
+            stack.push_back(u);
+          }
+      }
+    }
+  }
+
+  // v_f: +1 on f, then the forest path from head(f) back to tail(f)
+  std::vector<Rat> cycle(int64_t f) const {
+    std::vector<Rat> x(tail.size());
+    x[f] = Rat{1, 1};
+    if (tail[f] == head[f]) return x;  // a loop: the column is zero
+    // walk x from head(f) and y from tail(f) up to their common ancestor; the
+    // edge above a vertex v on the head side is traversed v -> up(v), on the
+    // tail side up(v) -> v
+    int64_t a = head[f], b = tail[f];
+    auto mark = [&](int64_t v, bool from_v) {
+      const int64_t e = parent_edge[v];
+      const int64_t from = from_v ? v : up[v], to = from_v ? up[v] : v;
+      x[e] = Rat{(tail[e] == from && head[e] == to) ? 1 : -1, 1};
+    };
+    while (depth[a] > depth[b]) {
+      mark(a, true);
+      a = up[a];
+    }
+    while (depth[b] > depth[a]) {
+      mark(b, false);
+      b = up[b];
+    }
+    while (a != b) {
+      mark(a, true);
+      a = up[a];
+      mark(b, false);
+      b = up[b];
+    }
+    return x;
+  }
+};
+
+}  // namespace
+
 // Representatives of ker A / im B over Q, chosen as exact.rs:178-264 chooses
 // them: the RREF kernel vector v_f of every free column f of A, in column
 // order, kept when it enlarges span(im B, the v before it), each returned
@@ -302,15 +411,23 @@ int64_t exact_rank(const IntMat& a) {
 // read on the free columns, can be combined to end (in free-column order) at
 // f. Those ends are the pivots of B's free-row columns eliminated by their
 // last entry; the kept v_f are the other free columns, and each is formed by
-// one back-substitution through A's echelon rows.
+// one back-substitution through A's echelon rows (or, when A is a graph's
+// incidence, as a fundamental cycle).
 std::vector<std::vector<int64_t>> quotient_generators(const IntMat& A, const IntMat& B) {
   FEEC_REQUIRE(A.ncols == B.nrows, FEEC_E_DEVICE, "quotient_generators: the maps do not compose");
   const int64_t n = A.ncols;
   Work w;
+  GraphKernel gk;
+  const bool graph = graph_columns(A, gk.tail, gk.head);
   // A's echelon rows over Q, leading entry 1, keyed by their leading column
+  // (general A), or the Kruskal forest (graph A)
   std::vector<int32_t> pivot_of(std::max<int64_t>(n, 1), -1);
   std::vector<Row<Rat>> piv;
-  {
+  if (graph) {
+    gk.build(A.nrows + 1);
+    for (int64_t c = 0; c < n; ++c)
+      if (gk.pivot[c]) pivot_of[c] = 0;
+  } else {
     Row<Rat> row, tmp;
     for (const auto& r0 : A.rows) {
       row.clear();
@@ -373,7 +490,7 @@ std::vector<std::vector<int64_t>> quotient_generators(const IntMat& A, const Int
       }
     }
   }
-  // the kept kernel vectors, by back-substitution through the echelon rows
+  // the kept kernel vectors
   std::vector<int32_t> order(piv.size());
   for (size_t i = 0; i < piv.size(); ++i) order[i] = (int32_t)i;
   std::sort(order.begin(), order.end(),
@@ -381,6 +498,10 @@ std::vector<std::vector<int64_t>> quotient_generators(const IntMat& A, const Int
   std::vector<std::vector<int64_t>> out;
   for (int64_t fi = 0; fi < nf; ++fi) {
     if (ends[fi]) continue;
+    if (graph) {
+      out.push_back(primitive(gk.cycle(free_cols[fi])));
+      continue;
+    }
     std::vector<Rat> x(n);
     x[free_cols[fi]] = Rat{1, 1};
     for (int32_t i : order) {
