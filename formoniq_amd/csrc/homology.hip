@@ -391,6 +391,7 @@ struct GraphKernel {
       b = up[b];
     }
     while (a != b) {
+      FEEC_REQUIRE(up[a] >= 0 && up[b] >= 0, FEEC_E_DEVICE, "a free edge's ends lie in different trees");
       mark(a, true);
       a = up[a];
       mark(b, false);

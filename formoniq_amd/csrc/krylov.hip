@@ -955,8 +955,8 @@ __global__ void k_power_normalize(const double* y, int64_t split, int64_t n, con
 // sum_j |P_ij| dinv_i, an upper bound on lambda_max(D^-1 P). Rows are >= 0, so
 // the max runs on the bit patterns (order-preserving for non-negative
 // doubles): a wavefront max per block, then one atomic per wavefront.
-__global__ void k_gershgorin(const int64_t* rp, const int32_t* col, const double* val, const double* dinv,
-                             int64_t n, int64_t split, unsigned long long* out) {
+__global__ void k_gershgorin(const int64_t* rp, const double* val, const double* dinv, int64_t n, int64_t split,
+                             unsigned long long* out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   double v0 = 0.0, v1 = 0.0;
   if (i < n) {
@@ -968,7 +968,6 @@ __global__ void k_gershgorin(const int64_t* rp, const int32_t* col, const double
     else
       v1 = acc;
   }
-  (void)col;
   for (int off = 32; off > 0; off >>= 1) {
     v0 = fmax(v0, __shfl_xor(v0, off, 64));
     v1 = fmax(v1, __shfl_xor(v1, off, 64));
@@ -1800,7 +1799,7 @@ void kkt_set_chebyshev(KktSystem& s, int m, double lower) {
   }
   DBuf<unsigned long long> gb(ctx, 2);
   k_zero_u64<<<1, 64, 0, st>>>(gb.p, 2);
-  k_gershgorin<<<grid_for(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.col.p, s.P.val.p, s.inv_diag.p, n, s.split, gb.p);
+  k_gershgorin<<<grid_for(n), TPB, 0, st>>>(s.P.row_ptr.p, s.P.val.p, s.inv_diag.p, n, s.split, gb.p);
   FEEC_HIP(hipGetLastError());
   copy_to_host(ctx, lam, dlam.p, sizeof(lam));
   unsigned long long gbits[2] = {0, 0};
