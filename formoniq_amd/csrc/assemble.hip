@@ -116,6 +116,54 @@ __global__ void k_segsum(const int64_t* seg, const uint32_t* src, int64_t nnz, c
   }
 }
 
+// The same sums with the gathers shared by the wavefront: each wavefront
+// takes 64 consecutive entries, whose contributions are contiguous in src;
+// its lanes fetch all of them together (every lane busy, 64 gathers in
+// flight) into LDS, then each lane adds its own entry's values in order, so
+// every sum is the one k_segsum forms. A run longer than the LDS stage falls
+// back to the per-lane gathers.
+constexpr int kSegStage = 1024;  // contributions staged per wavefront
+__global__ void __launch_bounds__(TPB) k_segsum_wave(const int64_t* seg, const uint32_t* src, int64_t nnz,
+                                                     const double* vals, double* out, uint8_t* keep,
+                                                     unsigned long long* ndropped) {
+  __shared__ double sv[TPB / 64][kSegStage];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t e0 = ((int64_t)blockIdx.x * (TPB / 64) + w) * 64;
+  const bool active = e0 < nnz;
+  const int64_t e1 = active ? (e0 + 64 < nnz ? e0 + 64 : nnz) : e0;
+  const int64_t s0 = active ? seg[e0] : 0, s1 = active ? seg[e1] : 0;
+  const bool staged = active && s1 - s0 <= kSegStage;
+  if (staged)
+    for (int64_t q = s0 + lane; q < s1; q += 64) {
+      const uint32_t j = src[q];
+      const double v = vals[j & ~NEG];
+      sv[w][q - s0] = (j & NEG) ? -v : v;
+    }
+  __syncthreads();
+  const int64_t e = e0 + lane;
+  if (!active || e >= e1) return;
+  const int64_t b = seg[e], en = seg[e + 1];
+  double acc = 0.0;
+  bool nz = false;
+  for (int64_t q = b; q < en; ++q) {
+    double v;
+    if (staged) {
+      v = sv[w][q - s0];
+    } else {
+      const uint32_t j = src[q];
+      v = vals[j & ~NEG];
+      if (j & NEG) v = -v;
+    }
+    nz |= (v != 0.0);
+    acc = (q == b) ? v : acc + v;
+  }
+  out[e] = acc;
+  if (keep) {
+    keep[e] = nz ? 1 : 0;
+    if (!nz) atomicAdd(ndropped, 1ull);
+  }
+}
+
 __global__ void k_rows_of(const int64_t* row_ptr, int64_t nrows, int32_t* rows) {
   int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (r >= nrows) return;
@@ -240,10 +288,16 @@ int64_t coo_plan_values(Ctx* ctx, const CooPlan& plan, const double* vals, doubl
                         uint8_t* keep) {
   if (plan.nnz == 0) return 0;
   unsigned long long* nd = ctx->dcount;
-  if (keep) FEEC_HIP(hipMemsetAsync(nd, 0, sizeof(unsigned long long), ctx->stream));
-  k_segsum<<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals,
-                                                        out, keep, nd);
-  FEEC_HIP(hipGetLastError());
+  // per-lane gathers, or the wavefront's shared ones: the same sums, bit for bit
+  tune_run(ctx, "segsum", 0, 0, plan.nnz, 2, [&](int c) {
+    if (keep) FEEC_HIP(hipMemsetAsync(nd, 0, sizeof(unsigned long long), ctx->stream));
+    if (c == 0)
+      k_segsum<<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
+    else
+      k_segsum_wave<<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep,
+                                                                 nd);
+    FEEC_HIP(hipGetLastError());
+  });
   if (!keep) return 0;
   unsigned long long h = 0;
   FEEC_HIP(hipMemcpyAsync(&h, nd, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
