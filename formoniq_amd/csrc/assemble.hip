@@ -122,13 +122,24 @@ __global__ void k_segsum(const int64_t* seg, const uint32_t* src, int64_t nnz, c
 // flight) into LDS, then each lane adds its own entry's values in order, so
 // every sum is the one k_segsum forms. A run longer than the LDS stage falls
 // back to the per-lane gathers.
+// XCD_CHUNK > 0: blocks are dealt to the 8 XCDs round-robin; the grid (a
+// multiple of 8 * XCD_CHUNK) is renumbered so that each XCD takes runs of
+// XCD_CHUNK consecutive blocks spread over the whole range, and an element
+// value's readers a few hundred blocks apart (the neighbouring layer of the
+// mesh) share that XCD's L2.
 constexpr int kSegStage = 1024;  // contributions staged per wavefront
+template <int XCD_CHUNK>
 __global__ void __launch_bounds__(TPB) k_segsum_wave(const int64_t* seg, const uint32_t* src, int64_t nnz,
                                                      const double* vals, double* out, uint8_t* keep,
                                                      unsigned long long* ndropped) {
   __shared__ double sv[TPB / 64][kSegStage];
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int64_t e0 = ((int64_t)blockIdx.x * (TPB / 64) + w) * 64;
+  int64_t blk = blockIdx.x;
+  if (XCD_CHUNK > 0) {
+    const int64_t x = blockIdx.x % 8, k = blockIdx.x / 8;
+    blk = ((k / XCD_CHUNK) * 8 + x) * XCD_CHUNK + k % XCD_CHUNK;
+  }
+  const int64_t e0 = (blk * (TPB / 64) + w) * 64;
   const bool active = e0 < nnz;
   const int64_t e1 = active ? (e0 + 64 < nnz ? e0 + 64 : nnz) : e0;
   const int64_t s0 = active ? seg[e0] : 0, s1 = active ? seg[e1] : 0;
@@ -289,13 +300,18 @@ int64_t coo_plan_values(Ctx* ctx, const CooPlan& plan, const double* vals, doubl
   if (plan.nnz == 0) return 0;
   unsigned long long* nd = ctx->dcount;
   // per-lane gathers, or the wavefront's shared ones: the same sums, bit for bit
-  tune_run(ctx, "segsum", 0, 0, plan.nnz, 2, [&](int c) {
+  tune_run(ctx, "segsum", 0, 0, plan.nnz, 3, [&](int c) {
     if (keep) FEEC_HIP(hipMemsetAsync(nd, 0, sizeof(unsigned long long), ctx->stream));
-    if (c == 0)
+    if (c == 0) {
       k_segsum<<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
-    else
-      k_segsum_wave<<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep,
-                                                                 nd);
+    } else if (c == 1) {
+      k_segsum_wave<0><<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out,
+                                                                    keep, nd);
+    } else {
+      constexpr int C = 512;
+      const unsigned g = (grid_for(plan.nnz) + 8 * C - 1) / (8 * C) * (8 * C);
+      k_segsum_wave<C><<<g, TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
+    }
     FEEC_HIP(hipGetLastError());
   });
   if (!keep) return 0;

@@ -35,13 +35,16 @@ def test_element_variants_are_bit_identical(d, n, k):
     _, _, gcx, geo, _, _ = both(d, n, 0.1, seed=d)
     wc = fa.WhitneyComplex(geo)
     out = {}
-    for mode in (1, 2):  # fused, split by grade
+    # mode 1: fused elements, per-lane segmented sums; 2: split by grade, the
+    # wavefront's shared gathers; 3: split, shared gathers in XCD-chunked order
+    for mode in (1, 2, 3):
         fa.tune_mode(mode)
         kkt, _ = wc.mixed_kkt(k)
         hd = fa.HodgeDirac.assemble(wc)
         out[mode] = (_csr(kkt), _csr(hd.mass_block), _csr(hd.op))
-    for a, b in zip(out[1], out[2]):
-        _same(a, b)
+    for m in (2, 3):
+        for a, b in zip(out[1], out[m]):
+            _same(a, b)
 
 
 def test_batched_grid_candidates_are_bit_identical():
