@@ -124,9 +124,8 @@ __global__ void k_segsum(const int64_t* seg, const uint32_t* src, int64_t nnz, c
 // back to the per-lane gathers.
 // XCD_CHUNK > 0: blocks are dealt to the 8 XCDs round-robin; the grid (a
 // multiple of 8 * XCD_CHUNK) is renumbered so that each XCD takes runs of
-// XCD_CHUNK consecutive blocks spread over the whole range, and an element
-// value's readers a few hundred blocks apart (the neighbouring layer of the
-// mesh) share that XCD's L2.
+// XCD_CHUNK consecutive blocks spread over the whole range, so that an
+// element value's nearby readers share that XCD's L2.
 constexpr int kSegStage = 1024;  // contributions staged per wavefront
 template <int XCD_CHUNK>
 __global__ void __launch_bounds__(TPB) k_segsum_wave(const int64_t* seg, const uint32_t* src, int64_t nnz,
@@ -308,7 +307,7 @@ int64_t coo_plan_values(Ctx* ctx, const CooPlan& plan, const double* vals, doubl
       k_segsum_wave<0><<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out,
                                                                     keep, nd);
     } else {
-      constexpr int C = 512;
+      constexpr int C = 128;
       const unsigned g = (grid_for(plan.nnz) + 8 * C - 1) / (8 * C) * (8 * C);
       k_segsum_wave<C><<<g, TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
     }
