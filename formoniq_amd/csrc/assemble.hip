@@ -65,14 +65,24 @@ __global__ void k_heads(const uint64_t* keys, int64_t n, int32_t* head) {
 }
 
 __global__ void k_entries(const uint64_t* keys, const int32_t* head, const int32_t* scan, int64_t n,
-                          int64_t ncols, int64_t* seg, int32_t* col, unsigned long long* rowcnt) {
+                          int64_t ncols, int64_t* seg, int32_t* col) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n || !head[i]) return;
   int64_t e = scan[i] - 1;
   seg[e] = i;
-  uint64_t k = keys[i];
-  col[e] = (int32_t)(k % (uint64_t)ncols);
-  atomicAdd(&rowcnt[k / (uint64_t)ncols], 1ull);
+  col[e] = (int32_t)(keys[i] % (uint64_t)ncols);
+}
+
+// row_ptr from the sorted entries: entry e (or the end, e = nnz) starts every
+// row after the previous entry's row up to its own, so each of the nrows + 1
+// pointers is written once (empty rows included), without counting atomics.
+__global__ void k_row_starts(const uint64_t* keys, const int64_t* seg, int64_t nnz, int64_t ncols,
+                             int64_t nrows, int64_t* row_ptr) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e > nnz) return;
+  const int64_t prev = e == 0 ? -1 : (int64_t)(keys[seg[e - 1]] / (uint64_t)ncols);
+  const int64_t cur = e == nnz ? nrows : (int64_t)(keys[seg[e]] / (uint64_t)ncols);
+  for (int64_t r = prev + 1; r <= cur; ++r) row_ptr[r] = e;
 }
 
 template <class T>
@@ -281,15 +291,13 @@ void coo_plan_build(Ctx* ctx, CooPlan& plan, int64_t nrows, int64_t ncols, int64
   plan.nnz = nnz;
   plan.seg.alloc(ctx, nnz + 1);
   plan.col.alloc(ctx, nnz);
-  DBuf<unsigned long long> rowcnt(ctx, nrows);
-  FEEC_HIP(hipMemsetAsync(rowcnt.p, 0, nrows * sizeof(unsigned long long), ctx->stream));
   if (nv > 0)
     k_entries<<<grid_for(nv), TPB, 0, ctx->stream>>>(k1.p, head.p, scan.p, nv,
                                                      std::max<int64_t>(ncols, 1), plan.seg.p,
-                                                     plan.col.p, rowcnt.p);
+                                                     plan.col.p);
   k_set_last<<<1, 1, 0, ctx->stream>>>(plan.seg.p, nnz, nv);
-  exclusive_scan_u64_to_i64<int64_t>(ctx, rowcnt.p, plan.row_ptr.p, nrows);
-  k_set_last<<<1, 1, 0, ctx->stream>>>(plan.row_ptr.p, nrows, nnz);
+  k_row_starts<<<grid_for(nnz + 1), TPB, 0, ctx->stream>>>(k1.p, plan.seg.p, nnz, std::max<int64_t>(ncols, 1),
+                                                           nrows, plan.row_ptr.p);
   FEEC_HIP(hipGetLastError());
   ctx->sync();
 }

@@ -32,6 +32,8 @@ std::mutex g_mu;
 std::map<std::tuple<int, std::string, int, int, int>, Entry> g_table;
 std::atomic<int> g_mode{0};
 
+constexpr float kLongLaunchMs = 0.5f;
+
 int size_class(int64_t n) {
   int c = 0;
   while (c < 62 && (int64_t(1) << c) < n) ++c;
@@ -76,16 +78,23 @@ int tune_run(Ctx* ctx, const char* family, int d, int k, int64_t n, int ncand,
   try {
     for (int c = 0; c < ncand; ++c) {
       launch(c);  // warm-up (first-launch code loading, caches)
+      // the median of three timed launches; one suffices for launches of
+      // kLongLaunchMs or more, whose run-to-run spread is a few percent
       float t[3];
-      for (int r = 0; r < 3; ++r) {
+      int reps = 0;
+      for (; reps < 3; ++reps) {
         FEEC_HIP(hipEventRecord(e0, ctx->stream));
         launch(c);
         FEEC_HIP(hipEventRecord(e1, ctx->stream));
         FEEC_HIP(hipEventSynchronize(e1));
-        FEEC_HIP(hipEventElapsedTime(&t[r], e0, e1));
+        FEEC_HIP(hipEventElapsedTime(&t[reps], e0, e1));
+        if (reps == 0 && t[0] >= kLongLaunchMs) {
+          reps = 1;
+          break;
+        }
       }
-      std::sort(t, t + 3);
-      e.ms[c] = t[1];
+      std::sort(t, t + reps);
+      e.ms[c] = t[reps / 2];
     }
   } catch (...) {
     (void)hipEventDestroy(e0);
