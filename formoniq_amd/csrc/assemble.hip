@@ -306,7 +306,10 @@ int64_t coo_plan_values(Ctx* ctx, const CooPlan& plan, const double* vals, doubl
                         uint8_t* keep) {
   if (plan.nnz == 0) return 0;
   unsigned long long* nd = ctx->dcount;
-  // per-lane gathers, or the wavefront's shared ones: the same sums, bit for bit
+  // per-lane gathers, or the wavefront's shared ones: the same sums, bit for
+  // bit. Below 2^22 entries the per-lane gathers were the fastest at every
+  // size measured, so only larger plans are timed.
+  const bool timed = plan.nnz >= (int64_t(1) << 22);
   tune_run(ctx, "segsum", 0, 0, plan.nnz, 3, [&](int c) {
     if (keep) FEEC_HIP(hipMemsetAsync(nd, 0, sizeof(unsigned long long), ctx->stream));
     if (c == 0) {
@@ -320,7 +323,7 @@ int64_t coo_plan_values(Ctx* ctx, const CooPlan& plan, const double* vals, doubl
       k_segsum_wave<C><<<g, TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
     }
     FEEC_HIP(hipGetLastError());
-  });
+  }, timed);
   if (!keep) return 0;
   unsigned long long h = 0;
   FEEC_HIP(hipMemcpyAsync(&h, nd, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));

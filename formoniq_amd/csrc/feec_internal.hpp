@@ -302,8 +302,9 @@ const double* device_coords(Geom& g);
 // Launches candidate `choice` of a bit-neutral launch choice through `launch`,
 // the choice taken from the per-device table, or found by timing every
 // candidate on the first call for (family, d, k, size class of n); returns it.
+// timed = false launches candidate 0 unless a mode forces another.
 int tune_run(Ctx* ctx, const char* family, int d, int k, int64_t n, int ncand,
-             const std::function<void(int)>& launch);
+             const std::function<void(int)>& launch, bool timed = true);
 
 // elmat.hip
 // Element matrices of one pairing, SoA [entry][cell].

@@ -49,13 +49,17 @@ bool capturing(hipStream_t s) {
 }  // namespace
 
 int tune_run(Ctx* ctx, const char* family, int d, int k, int64_t n, int ncand,
-             const std::function<void(int)>& launch) {
+             const std::function<void(int)>& launch, bool timed) {
   FEEC_REQUIRE(ncand >= 1, FEEC_E_ARG, "no candidates");
   const int mode = g_mode.load();
   if (mode > 0) {
     const int c = std::min(mode - 1, ncand - 1);
     launch(c);
     return c;
+  }
+  if (!timed) {
+    launch(0);
+    return 0;
   }
   const auto key = std::make_tuple(ctx->device, std::string(family), d, k, size_class(n));
   {
