@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <future>
 #include <vector>
 
 #include "feec_internal.hpp"
@@ -288,9 +289,15 @@ IntMat int_transpose(const IntMat& a) {
 }
 
 int64_t exact_rank(const IntMat& a) {
+  // the larger of the ranks over two primes near 2^30 (exact.rs:106-115), the
+  // two eliminations on two host threads
+  auto second = std::async(std::launch::async, [&a] {
+    Work w;
+    return rank_mod(a, 1000000009, w);
+  });
   Work w;
-  // the larger of the ranks over two primes near 2^30 (exact.rs:106-115)
-  return std::max(rank_mod(a, 1000000007, w), rank_mod(a, 1000000009, w));
+  const int64_t r1 = rank_mod(a, 1000000007, w);
+  return std::max(r1, second.get());
 }
 
 namespace {
@@ -539,7 +546,9 @@ int64_t betti_number(Mesh& m, int bc, int k) {
   };
   const IntMat lo = coboundary_int(m, bc, k - 1), hi = coboundary_int(m, bc, k);
   const int64_t nk = lo.nrows;
-  const int64_t b = nk - rank_of(k - 1, lo) - rank_of(k, hi);
+  auto r_hi = std::async(std::launch::async, [&] { return rank_of(k, hi); });
+  const int64_t r_lo = rank_of(k - 1, lo);
+  const int64_t b = nk - r_lo - r_hi.get();
   std::lock_guard<std::mutex> lock(m.topo_mu);
   m.betti[{bc, k}] = b;
   return b;
@@ -562,6 +571,23 @@ std::vector<std::vector<int64_t>> cached_generators(Mesh& m, int bc, int k, int 
   std::lock_guard<std::mutex> lock(m.topo_mu);
   m.generators[{bc, k, kind}] = g;
   return g;
+}
+
+// Both kinds at once: the two coboundaries are copied once, and the two
+// eliminations (host-only) run on two threads.
+void cache_both_generators(Mesh& m, int bc, int k) {
+  {
+    std::lock_guard<std::mutex> lock(m.topo_mu);
+    if (m.generators.count({bc, k, 0}) && m.generators.count({bc, k, 1})) return;
+  }
+  const IntMat lo = coboundary_int(m, bc, k - 1), hi = coboundary_int(m, bc, k);
+  auto cyc = std::async(std::launch::async,
+                        [&] { return quotient_generators(int_transpose(lo), int_transpose(hi)); });
+  std::vector<std::vector<int64_t>> co = quotient_generators(hi, lo);
+  std::vector<std::vector<int64_t>> ho = cyc.get();
+  std::lock_guard<std::mutex> lock(m.topo_mu);
+  m.generators[{bc, k, 0}] = std::move(co);
+  m.generators[{bc, k, 1}] = std::move(ho);
 }
 }  // namespace
 
@@ -587,6 +613,7 @@ void harmonics_exact(Geom& g, int bc, int k, std::vector<double>& integral, std:
                      int64_t& count) {
   Mesh& m = *g.mesh;
   Ctx* ctx = m.ctx;
+  cache_both_generators(m, bc, k);
   const std::vector<std::vector<int64_t>> z = cohomology_generators(m, bc, k);
   const std::vector<std::vector<int64_t>> cyc = homology_generators(m, bc, k);
   const int64_t b = (int64_t)z.size();
