@@ -473,6 +473,33 @@ def run_kernels(args):
                       "calibration_bytes_written": b.numel() * 4}), flush=True)
 
 
+def run_step(args):
+    """One headline step (64^3 k=1, FP64_PC32, the default preconditioner) with
+    graphs off, for kernel tracing: a warm-up step, then one assemble() and one
+    solve(), each between 0.3 s pauses so that a trace splits at them."""
+    import formoniq_amd as fa
+    ctx = fa.Context(0)
+    topo, geo, wc = build_problem(fa, args.n, fa.FP64_PC32, ctx)
+    f = fa.normal_vector(topo.nsimplices(1), 2)
+    prob = fa.Problem(wc, 1, fa.solve_opts(precision=fa.FP64_PC32, use_graph=0))
+    prob.upload_source(f)
+    prob.assemble()
+    prob.solve()
+    ctx.synchronize()
+    time.sleep(0.3)
+    t0 = time.perf_counter()
+    prob.assemble()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    time.sleep(0.3)
+    t2 = time.perf_counter()
+    r = prob.solve()
+    ctx.synchronize()
+    t3 = time.perf_counter()
+    print(json.dumps({"config": "step", "assemble_ms": (t1 - t0) * 1e3, "solve_ms": (t3 - t2) * 1e3,
+                      "iters": int(r.iters), "residual": float(r.residual)}), flush=True)
+
+
 def timed_steps(ctx, prob, steps, label):
     """Time `steps` steps (assemble + solve) of a resident problem."""
     lat, reps = [], []
@@ -514,7 +541,8 @@ def main():
                     help="MINRES iterations per replayed graph segment")
     ap.add_argument("--no-warmup", action="store_true",
                     help="c5: start the timed stream on a cold server (no start-up requests)")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "kernels", "leapfrog", "analytic"],
+    ap.add_argument("--config", default="c2",
+                    choices=["c2", "c3", "c4", "c5", "kernels", "step", "leapfrog", "analytic"],
                     help="c2 (default): the headline line, with C3 and C5 sub-objects; c3/c4/c5: those "
                          "workloads in full, each printed as its own JSON line")
     ap.add_argument("--batch", type=int, default=512, help="c4: number of meshes")
@@ -527,7 +555,7 @@ def main():
               "sharding of assembly, MINRES and the batches is designed but not built)", file=sys.stderr)
         sys.exit(2)
     if args.config != "c2":
-        {"c3": run_c3, "c4": run_c4, "c5": run_c5, "kernels": run_kernels,
+        {"c3": run_c3, "c4": run_c4, "c5": run_c5, "kernels": run_kernels, "step": run_step,
          "leapfrog": run_leapfrog, "analytic": run_analytic}[args.config](args)
         return
 
