@@ -235,36 +235,36 @@ __global__ void k_pc_rows(const int64_t* krp, const int32_t* kcol, int64_t n, in
   kfirst[r] = first < 0 ? krp[r] : first;
 }
 
-// each P entry's column and contribution count (twice the KKT entry's)
+// each P entry's column, contribution count (twice the KKT entry's) and KKT entry
 __global__ void k_pc_entries(const int64_t* prp, const int64_t* kfirst, const int32_t* kcol,
-                             const int64_t* kseg, int64_t n, int32_t* pcol, unsigned long long* plen) {
+                             const int64_t* kseg, int64_t n, int32_t* pcol, unsigned long long* plen,
+                             int64_t* pe) {
   const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (r >= n) return;
   for (int64_t t = 0; t < prp[r + 1] - prp[r]; ++t) {
     const int64_t e = kfirst[r] + t, q = prp[r] + t;
     pcol[q] = kcol[e];
     plen[q] = 2ull * (unsigned long long)(kseg[e + 1] - kseg[e]);
+    pe[q] = e;
   }
 }
 
-// each P entry's sources: the mass planes, then the dif_both planes, of the
-// KKT entry's (cell, pair)s
-__global__ void k_pc_srcs(const int64_t* prp, const int64_t* kfirst, const int64_t* kseg,
-                          const uint32_t* ksrc, const int64_t* pseg, int64_t n, int64_t ns,
-                          uint32_t off_mlo, uint32_t off_dlo, uint32_t off_mhi, uint32_t off_dhi,
-                          uint32_t* psrc) {
-  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const bool sig = r < ns;
-  for (int64_t t = 0; t < prp[r + 1] - prp[r]; ++t) {
-    const int64_t e = kfirst[r] + t, q = prp[r] + t;
-    const int64_t L = kseg[e + 1] - kseg[e], base = pseg[q];
-    for (int64_t i = 0; i < L; ++i) {
-      const uint32_t s = ksrc[kseg[e] + i] & ~kCooNegate;
-      const uint32_t local = s - (sig ? off_mlo : off_dhi);  // (pair * ncells + cell)
-      psrc[base + i] = local + (sig ? off_mlo : off_mhi);
-      psrc[base + L + i] = local + (sig ? off_dlo : off_dhi);
-    }
+// each P entry's sources (one lane per entry): the mass planes, then the
+// dif_both planes, of the KKT entry's (cell, pair)s. An entry lies in the
+// sigma block when its column does.
+__global__ void k_pc_srcs(const int64_t* pe, const int32_t* pcol, const int64_t* kseg, const uint32_t* ksrc,
+                          const int64_t* pseg, int64_t nnz, int64_t ns, uint32_t off_mlo, uint32_t off_dlo,
+                          uint32_t off_mhi, uint32_t off_dhi, uint32_t* psrc) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  const bool sig = pcol[q] < ns;
+  const int64_t e = pe[q];
+  const int64_t L = kseg[e + 1] - kseg[e], base = pseg[q];
+  for (int64_t i = 0; i < L; ++i) {
+    const uint32_t s = ksrc[kseg[e] + i] & ~kCooNegate;
+    const uint32_t local = s - (sig ? off_mlo : off_dhi);  // (pair * ncells + cell)
+    psrc[base + i] = local + (sig ? off_mlo : off_mhi);
+    psrc[base + L + i] = local + (sig ? off_dlo : off_dhi);
   }
 }
 
@@ -484,10 +484,11 @@ void problem_build(Problem& P) {
     pp.col.alloc(ctx, pp.nnz);
     pp.seg.alloc(ctx, pp.nnz + 1);
     DBuf<unsigned long long> plen(ctx, pp.nnz + 1);
+    DBuf<int64_t> pe(ctx, std::max<int64_t>(pp.nnz, 1));
     FEEC_HIP(hipMemsetAsync(plen.p + pp.nnz, 0, sizeof(unsigned long long), ctx->stream));
     if (n)
       k_pc_entries<<<grid_for(n), TPB, 0, ctx->stream>>>(pp.row_ptr.p, kfirst.p, kp.col.p, kp.seg.p, n, pp.col.p,
-                                                         plen.p);
+                                                         plen.p, pe.p);
     FEEC_HIP(hipGetLastError());
     exclusive_scan_counts(ctx, plen.p, pp.seg.p, pp.nnz + 1);
     int64_t nsrc = 0;
@@ -495,9 +496,9 @@ void problem_build(Problem& P) {
     FEEC_REQUIRE(nsrc == sizes[0] + sizes[6] + sizes[5] + sizes[2] || bc == FEEC_RELATIVE, FEEC_E_DEVICE,
                  "preconditioner plan: contribution count");
     pp.src.alloc(ctx, std::max<int64_t>(nsrc, 1));
-    if (n)
-      k_pc_srcs<<<grid_for(n), TPB, 0, ctx->stream>>>(
-          pp.row_ptr.p, kfirst.p, kp.seg.p, kp.src.p, pp.seg.p, n, P.ns, (uint32_t)P.off[0], (uint32_t)P.off[6],
+    if (pp.nnz)
+      k_pc_srcs<<<grid_for(pp.nnz), TPB, 0, ctx->stream>>>(
+          pe.p, pp.col.p, kp.seg.p, kp.src.p, pp.seg.p, pp.nnz, P.ns, (uint32_t)P.off[0], (uint32_t)P.off[6],
           (uint32_t)P.off[5], (uint32_t)P.off[2], pp.src.p);
     FEEC_HIP(hipGetLastError());
   } else {
