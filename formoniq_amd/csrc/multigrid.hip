@@ -299,29 +299,34 @@ __global__ void __launch_bounds__(TPB) k_st_dense(const TP* __restrict__ C, cons
 // set if a transpose entry is missing, flags[1] if an offset needs more than
 // 16 bits.
 __global__ void k_pc_pattern(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
-                             int64_t n, int32_t* __restrict__ tpos, int16_t* __restrict__ c16,
+                             int64_t n, int64_t nnz, int32_t* __restrict__ tpos, int16_t* __restrict__ c16,
                              int* __restrict__ flags) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  bool missing = false, wide = false;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-    const int64_t j = col[p];
-    int64_t lo = rp[j], hi = rp[j + 1];
-    while (lo < hi) {  // first position in row j with column >= i
-      const int64_t mid = (lo + hi) / 2;
-      if (col[mid] < i)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    const bool found = lo < rp[j + 1] && col[lo] == i;
-    missing = missing || !found;
-    tpos[p] = found ? (int32_t)lo : 0;
-    const int64_t off = j - i;
-    wide = wide || off < -32767 || off > 32767;
-    c16[p] = (int16_t)(wide ? 0 : off);
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // one lane per entry
+  if (p >= nnz) return;
+  int64_t lo = 0, hi = n;  // the entry's row: the last i with rp[i] <= p
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if (rp[mid] <= p)
+      lo = mid;
+    else
+      hi = mid;
   }
-  if (missing) atomicOr(&flags[0], 1);
+  const int64_t i = lo, j = col[p];
+  lo = rp[j];
+  hi = rp[j + 1];
+  while (lo < hi) {  // first position in row j with column >= i
+    const int64_t mid = (lo + hi) / 2;
+    if (col[mid] < i)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  const bool found = lo < rp[j + 1] && col[lo] == i;
+  tpos[p] = found ? (int32_t)lo : 0;
+  const int64_t off = j - i;
+  const bool wide = off < -32767 || off > 32767;
+  c16[p] = (int16_t)(wide ? 0 : off);
+  if (!found) atomicOr(&flags[0], 1);
   if (wide) atomicOr(&flags[1], 1);
 }
 
@@ -1528,8 +1533,8 @@ void build_pc_single(Hx& h, const KktSystem& s) {
   // flags[0]: some entry has no transpose; flags[1]: some offset is beyond 16 bits
   DBuf<int> flags(ctx, 2);
   k_zero_i32<<<1, 64, 0, ctx->stream>>>(flags.p, 2);
-  k_pc_pattern<<<grid_for(n), TPB, 0, ctx->stream>>>(s.P.row_ptr.p, s.P.col.p, n, h.P_tpos.p, c16.p,
-                                                      flags.p);
+  k_pc_pattern<<<grid_for(nnz), TPB, 0, ctx->stream>>>(s.P.row_ptr.p, s.P.col.p, n, nnz, h.P_tpos.p, c16.p,
+                                                        flags.p);
   FEEC_HIP(hipGetLastError());
   int hf[2] = {0, 0};
   copy_to_host(ctx, hf, flags.p, sizeof(hf));
