@@ -114,3 +114,31 @@ def test_more_harmonic_forms_than_the_border_holds_are_unsupported():
         wc.harmonics(0)
     assert e.value.name == "FEEC_E_UNSUPPORTED"
     assert wc.harmonics(1).shape == (gcx.nsimplices(1), 0)
+
+
+def test_4d_slab_with_a_hole_through_two_axes():
+    """4D: the Kuhn 3^4 grid without the boxes (1, 1, z, w), a square annulus
+    times a square: b = (1, 1, 0, 0, 0), relative (0, 0, 0, 1, 1) (Lefschetz
+    duality), and the harmonic 1-form closed, co-closed and M-normalized with
+    the library's own operators (the oracle's dense rational elimination is too
+    slow at this size)."""
+    cells, coords = o.kuhn(4, 3)
+    box = np.arange(len(cells)) // 24
+    keep = ~((box % 3 == 1) & ((box // 3) % 3 == 1))
+    used, inv = np.unique(cells[keep], return_inverse=True)
+    cells, coords = inv.reshape(-1, 5).astype(np.int64), coords[used]
+    gcx = fa.Complex(4, cells, nverts=len(coords))
+    chi = sum((-1) ** k * gcx.nsimplices(k) for k in range(5))
+    assert chi == 0
+    assert [gcx.betti_number(k) for k in range(5)] == [1, 1, 0, 0, 0]
+    assert [gcx.betti_number(k, fa.RELATIVE) for k in range(5)] == [0, 0, 0, 1, 1]
+    wc = fa.WhitneyComplex(fa.MeshLengthsSq.from_coords(gcx, coords))
+    h = wc.harmonics(1)[:, 0]
+    M = wc.mass(1).to_scipy()
+    assert abs(h @ (M @ h) - 1.0) <= 1e-12
+    assert np.abs(gcx.coboundary(1).to_scipy() @ h).max() <= 1e-12 * np.abs(h).max()
+    assert np.abs(gcx.coboundary(0).to_scipy().T @ (M @ h)).max() <= 1e-9 * np.abs(M @ h).max()
+    c = gcx.homology_generators(1)
+    assert c.shape[0] == 1 and abs(wc.harmonics_integral(1)[:, 0] @ c[0] - 1.0) <= 1e-10
+    # every cycle generator is a cycle: its boundary (D_0^T c) vanishes exactly
+    assert not np.any(gcx.coboundary(0).to_scipy().T @ c[0])
