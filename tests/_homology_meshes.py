@@ -78,6 +78,21 @@ def circle():
     return _periodic(1, 3)
 
 
+def mobius(nx=4):
+    """A Moebius strip: nx unit boxes in a row, the last one glued to the first
+    with a flip, two triangles a box. Non-orientable: b = (1, 1, 0), relative
+    (0, 0, 0) over Q (H_1(M, dM) is Z/2)."""
+    def v(i, j):
+        return (1 - j) if i == nx else 2 * i + j
+    cells, sq = [], {}
+    for i in range(nx):
+        a, b, c, d = v(i, 0), v(i + 1, 0), v(i, 1), v(i + 1, 1)
+        cells += [sorted([a, b, d]), sorted([a, c, d])]
+        for e, l2 in (((a, b), 1.0), ((c, d), 1.0), ((a, c), 1.0), ((b, d), 1.0), ((a, d), 2.0)):
+            sq[tuple(sorted(e))] = l2
+    return 2, np.array(cells, np.int64), lambda cx: np.array([sq[tuple(e)] for e in cx.skeleton(1).tolist()])
+
+
 def squared_lengths(cx, geom):
     if callable(geom):
         return np.ascontiguousarray(geom(cx), dtype=np.float64)
@@ -87,5 +102,5 @@ def squared_lengths(cx, geom):
 FIXTURES = {
     "cube1": lambda: cube(1), "cube2": lambda: cube(2), "cube3": lambda: cube(3),
     "two_sphere": two_sphere, "annulus": annulus, "circle": circle, "torus": torus,
-    "tunnel": tunnel, "two_disks": two_disks,
+    "tunnel": tunnel, "two_disks": two_disks, "mobius": mobius,
 }

@@ -40,7 +40,7 @@ def test_betti_numbers_and_generators_match_the_oracle(name):
             assert c.tolist() == o.homology_generators(ocx, k, rel), (rel, k)
 
 
-@pytest.mark.parametrize("name", ["annulus", "torus", "circle", "two_sphere", "tunnel", "two_disks"])
+@pytest.mark.parametrize("name", ["annulus", "torus", "circle", "two_sphere", "tunnel", "two_disks", "mobius"])
 def test_harmonic_bases_match_the_oracle(name):
     for rel in (False, True):
         dim, ocx, owc, gcx, gwc = _both(name, rel)

@@ -35,6 +35,7 @@ KNOWN = {  # (absolute, relative) Betti numbers
     "torus": ([1, 2, 1], [1, 2, 1]),      # unit_torus(2, 3): b_k = binom(2, k)
     "tunnel": ([1, 1, 0, 0], [0, 0, 1, 1]),
     "two_disks": ([2, 0, 0], [0, 0, 2]),
+    "mobius": ([1, 1, 0], [0, 0, 0]),     # non-orientable: no relative top class over Q
 }
 
 
@@ -83,7 +84,7 @@ def _sympy_quotient(a, b):
     return out
 
 
-@pytest.mark.parametrize("name", ["annulus", "torus", "two_sphere", "circle", "cube2", "two_disks"])
+@pytest.mark.parametrize("name", ["annulus", "torus", "two_sphere", "circle", "cube2", "two_disks", "mobius"])
 @pytest.mark.parametrize("rel", [False, True])
 def test_generators_match_sympy(name, rel):
     cx = _cx(name)
@@ -135,7 +136,7 @@ def _harmonic_laws(wc, k, integral, ortho, cyc):
     assert np.linalg.matrix_rank(np.hstack([integral, ortho]), tol=1e-9) == ortho.shape[1]
 
 
-@pytest.mark.parametrize("name", ["annulus", "torus", "two_sphere", "tunnel"])
+@pytest.mark.parametrize("name", ["annulus", "torus", "two_sphere", "tunnel", "mobius"])
 def test_harmonic_bases_obey_the_laws(name):
     dim, cells, geom = FIXTURES[name]()
     for rel in (False, True):
