@@ -136,12 +136,13 @@ __global__ void k_segsum(const int64_t* seg, const uint32_t* src, int64_t nnz, c
 // multiple of 8 * XCD_CHUNK) is renumbered so that each XCD takes runs of
 // XCD_CHUNK consecutive blocks spread over the whole range, so that an
 // element value's nearby readers share that XCD's L2.
-constexpr int kSegStage = 1024;  // contributions staged per wavefront
-template <int XCD_CHUNK>
+// STAGE: contributions staged per wavefront (a smaller stage leaves room for
+// more resident wavefronts; runs longer than it take the per-lane gathers).
+template <int XCD_CHUNK, int STAGE>
 __global__ void __launch_bounds__(TPB) k_segsum_wave(const int64_t* seg, const uint32_t* src, int64_t nnz,
                                                      const double* vals, double* out, uint8_t* keep,
                                                      unsigned long long* ndropped) {
-  __shared__ double sv[TPB / 64][kSegStage];
+  __shared__ double sv[TPB / 64][STAGE];
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   int64_t blk = blockIdx.x;
   if (XCD_CHUNK > 0) {
@@ -152,7 +153,7 @@ __global__ void __launch_bounds__(TPB) k_segsum_wave(const int64_t* seg, const u
   const bool active = e0 < nnz;
   const int64_t e1 = active ? (e0 + 64 < nnz ? e0 + 64 : nnz) : e0;
   const int64_t s0 = active ? seg[e0] : 0, s1 = active ? seg[e1] : 0;
-  const bool staged = active && s1 - s0 <= kSegStage;
+  const bool staged = active && s1 - s0 <= STAGE;
   if (staged)
     for (int64_t q = s0 + lane; q < s1; q += 64) {
       const uint32_t j = src[q];
@@ -306,22 +307,21 @@ int64_t coo_plan_values(Ctx* ctx, const CooPlan& plan, const double* vals, doubl
                         uint8_t* keep) {
   if (plan.nnz == 0) return 0;
   unsigned long long* nd = ctx->dcount;
-  // per-lane gathers, or the wavefront's shared ones: the same sums, bit for
-  // bit. Below 2^22 entries the per-lane gathers were the fastest at every
+  // per-lane gathers, or the wavefront's shared ones in XCD runs with a stage
+  // of 512 or 1,024 contributions: the same sums, bit for bit. Below 2^22 entries the per-lane gathers were the fastest at every
   // size measured, so only larger plans are timed.
   const bool timed = plan.nnz >= (int64_t(1) << 22);
   tune_run(ctx, "segsum", 0, 0, plan.nnz, 3, [&](int c) {
     if (keep) FEEC_HIP(hipMemsetAsync(nd, 0, sizeof(unsigned long long), ctx->stream));
-    if (c == 0) {
-      k_segsum<<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
-    } else if (c == 1) {
-      k_segsum_wave<0><<<grid_for(plan.nnz), TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out,
-                                                                    keep, nd);
-    } else {
-      constexpr int C = 128;
-      const unsigned g = (grid_for(plan.nnz) + 8 * C - 1) / (8 * C) * (8 * C);
-      k_segsum_wave<C><<<g, TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
-    }
+    const unsigned g = grid_for(plan.nnz);
+    constexpr int C = 128;
+    const unsigned gx = (g + 8 * C - 1) / (8 * C) * (8 * C);
+    if (c == 0)
+      k_segsum<<<g, TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
+    else if (c == 1)  // about 2-3 contributions per entry (the KKT)
+      k_segsum_wave<C, 512><<<gx, TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
+    else  // about 5 (mass plus stiffness: the preconditioner matrix)
+      k_segsum_wave<C, 1024><<<gx, TPB, 0, ctx->stream>>>(plan.seg.p, plan.src.p, plan.nnz, vals, out, keep, nd);
     FEEC_HIP(hipGetLastError());
   }, timed);
   if (!keep) return 0;

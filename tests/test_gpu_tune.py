@@ -36,7 +36,8 @@ def test_element_variants_are_bit_identical(d, n, k):
     wc = fa.WhitneyComplex(geo)
     out = {}
     # mode 1: fused elements, per-lane segmented sums; 2: split by grade, the
-    # wavefront's shared gathers; 3: split, shared gathers in XCD-chunked order
+    # wavefront's shared gathers in XCD runs (stage 512); 3: split, the same
+    # with a stage of 1,024
     for mode in (1, 2, 3):
         fa.tune_mode(mode)
         kkt, _ = wc.mixed_kkt(k)
