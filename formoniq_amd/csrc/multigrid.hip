@@ -566,8 +566,9 @@ __global__ void k_scale(const double* __restrict__ a, int64_t n, double s, doubl
 // coef: [1/theta_sigma, 1/theta_u, then per step k: rho_{k+1} rho_k,
 // 2 rho_{k+1}/delta_sigma, 2 rho_{k+1}/delta_u]; row i < split is sigma's.
 
-// y = P x, P the KKT preconditioner matrix (CSR); LPR lanes per row.
-template <class TV, class TP, int LPR>
+// y = P x, P the KKT preconditioner matrix (CSR); LPR lanes per row, each
+// taking its entries U at a time as k_pc_spmv32 below does (the same sum).
+template <class TV, class TP, int LPR, int U>
 __global__ void __launch_bounds__(TPB) k_pc_spmv(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                  const TV* __restrict__ val, const TP* __restrict__ x,
                                                  TP* __restrict__ y, int64_t nrows, const MinresS* S) {
@@ -576,8 +577,24 @@ __global__ void __launch_bounds__(TPB) k_pc_spmv(const int64_t* __restrict__ rp,
   const int64_t row = gid / LPR;
   const int lane = (int)(gid % LPR);
   TP acc = TP(0);
-  if (row < nrows)
-    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LPR) acc += (TP)val[p] * x[col[p]];
+  if (row < nrows) {
+    const int64_t e = rp[row + 1];
+    int64_t p = rp[row] + lane;
+    for (; p + (U - 1) * LPR < e; p += U * LPR) {
+      int32_t c[U];
+      TP v[U], xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        c[u] = col[p + u * LPR];
+        v[u] = (TP)val[p + u * LPR];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc += v[u] * xv[u];
+    }
+    for (; p < e; p += LPR) acc += (TP)val[p] * x[col[p]];
+  }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if (row < nrows && lane == 0) y[row] = acc;
@@ -594,7 +611,10 @@ __global__ void k_sym_single(const double* __restrict__ val, const int32_t* __re
 }
 
 // y = P32 x, columns as int16 offsets from the row or int32; LPR lanes per row.
-template <class TC, int LPR>
+// Each lane takes its entries U at a time (loads, then gathers, then the
+// products added in entry order): more loads in flight per lane and the same
+// sum, bit for bit, as one entry at a time.
+template <class TC, int LPR, int U>
 __global__ void __launch_bounds__(TPB) k_pc_spmv32(const int64_t* __restrict__ rp, const TC* __restrict__ col,
                                                    const float* __restrict__ val, const float* __restrict__ x,
                                                    float* __restrict__ y, int64_t nrows, const MinresS* S) {
@@ -602,12 +622,26 @@ __global__ void __launch_bounds__(TPB) k_pc_spmv32(const int64_t* __restrict__ r
   const int64_t gid = xcd_block() * (int64_t)blockDim.x + threadIdx.x;
   const int64_t row = gid / LPR;
   const int lane = (int)(gid % LPR);
+  const auto column = [row](TC c) { return std::is_same_v<TC, int16_t> ? row + (int64_t)c : (int64_t)c; };
   float acc = 0.f;
-  if (row < nrows)
-    for (int64_t p = rp[row] + lane; p < rp[row + 1]; p += LPR) {
-      const int64_t c = std::is_same_v<TC, int16_t> ? row + (int64_t)col[p] : (int64_t)col[p];
-      acc += val[p] * x[c];
+  if (row < nrows) {
+    const int64_t e = rp[row + 1];
+    int64_t p = rp[row] + lane;
+    for (; p + (U - 1) * LPR < e; p += U * LPR) {
+      TC c[U];
+      float v[U], xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        c[u] = col[p + u * LPR];
+        v[u] = val[p + u * LPR];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = x[column(c[u])];
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc += v[u] * xv[u];
     }
+    for (; p < e; p += LPR) acc += val[p] * x[column(col[p])];
+  }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if (row < nrows && lane == 0) y[row] = acc;
@@ -2056,13 +2090,13 @@ template <class TP>
 void pc_spmv(Enq& q, Hx& h, const TP* x, TP* y, const MinresS* S) {
   const unsigned g = grid_for(h.n * 4);
   if constexpr (std::is_same_v<TP, double>) {
-    q(k_pc_spmv<double, double, 4>, g, TPB, h.P_rp, h.P_col, h.P_val, x, y, h.n, S);
+    q(k_pc_spmv<double, double, 4, 2>, g, TPB, h.P_rp, h.P_col, h.P_val, x, y, h.n, S);
   } else {
     if (h.P_col16.p)
-      q(k_pc_spmv32<int16_t, 4>, g, TPB, h.P_rp, (const int16_t*)h.P_col16.p, (const float*)h.P32.p, x, y,
+      q(k_pc_spmv32<int16_t, 4, 2>, g, TPB, h.P_rp, (const int16_t*)h.P_col16.p, (const float*)h.P32.p, x, y,
         h.n, S);
     else
-      q(k_pc_spmv32<int32_t, 4>, g, TPB, h.P_rp, h.P_col, (const float*)h.P32.p, x, y, h.n, S);
+      q(k_pc_spmv32<int32_t, 4, 2>, g, TPB, h.P_rp, h.P_col, (const float*)h.P32.p, x, y, h.n, S);
   }
 }
 
