@@ -630,28 +630,42 @@ def main():
             "degree": int(prob.preconditioner()["degree"])}
     del prob
 
-    # roofline of the dominant kernel: the finest-level Jacobi sweep of the
-    # V-cycle (the largest share of a step's kernel time: 2 sweeps before and
-    # after each coarse correction, several cycles per MINRES iteration)
-    roof = None
+    # roofline of the dominant kernel: the product by the preconditioner
+    # matrix P in FP32 (the single kernel with the largest share of a solve's
+    # kernel time: one launch per Chebyshev step, 5 per preconditioner
+    # application); beside it the finest-level Jacobi sweep of the V-cycle
+    roof, sweep_roof = None, None
+    tr = {}
+    if os.path.exists(args.traffic):
+        with open(args.traffic) as fh:
+            tr = json.load(fh)
+        if tr.get("n_kkt") != n_kkt:
+            tr = {}
     if mg is not None:
+        # int16 column offsets (or int32) and FP32 values per entry, 8-byte row
+        # pointers, x read and y written once per row
+        p_rows, p_nnz, cb = n_kkt, int(pc["nnz"]), int(pc["column_bytes"])
+        spmv_bytes = p_nnz * (cb + 4) + 8 * (p_rows + 1) + 8 * p_rows
+        achieved = spmv_bytes / mg["pc_spmv32"] / 1e9
+        roof = {
+            "kernel": "k_pc_spmv32<short,4,2> (y = P x, the block-diagonal preconditioner matrix in FP32, "
+                      "4 lanes per row)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": tr.get("pc_spmv32_hbm_bytes_per_launch"),
+            "algorithmic_bytes_per_launch": spmv_bytes, "launch_seconds": mg["pc_spmv32"],
+            "rows": p_rows, "nnz": p_nnz, "column_bytes": cb,
+        }
         nb0 = (args.n + 1) ** 3
         nv = 5  # slots of one 3D k = 1 cycle: sigma's V, u's gradient V and its 3 vector copies
         # 8 stencil diagonals (the diagonal and the 7 +delta ones; the -delta entries
         # are the same numbers), dinv, r and x read once, the new iterate written
         sweep_bytes = nb0 * (8 * 4 + 4 + 3 * 4 * nv)
         achieved = sweep_bytes / mg["sweep_fine"] / 1e9
-        traffic = None
-        if os.path.exists(args.traffic):
-            with open(args.traffic) as fh:
-                tr = json.load(fh)
-            if tr.get("n_kkt") == n_kkt and "sweep_fine_hbm_bytes_per_launch" in tr:
-                traffic = tr["sweep_fine_hbm_bytes_per_launch"]
-        roof = {
+        sweep_roof = {
             "kernel": "k_st_sweep<float,3,0,true> (finest-level weighted-Jacobi sweep of the multigrid V-cycle "
                       "on the Kuhn-stencil diagonals, 5 interleaved slots, FP32)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": tr.get("sweep_fine_hbm_bytes_per_launch"),
             "algorithmic_bytes_per_launch": sweep_bytes, "launch_seconds": mg["sweep_fine"],
             "finest_level_vertices": nb0, "slots": nv,
         }
@@ -702,6 +716,7 @@ def main():
         "mr_spmv_f64": {"launch_seconds": times["spmv64"], "algorithmic_bytes_per_launch": spmv64_bytes,
                         "achieved_gbs": spmv64_bytes / times["spmv64"] / 1e9},
         "roofline": roof,
+        "roofline_sweep_fine": sweep_roof,
     }
     if not args.no_analytic:
         try:

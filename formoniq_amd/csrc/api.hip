@@ -1575,7 +1575,7 @@ feec_status feec_problem_preconditioner(const feec_problem* p, int64_t* nnz, int
       lambda_max[0] = p->sys.lambda_max[0];
       lambda_max[1] = p->sys.lambda_max[1];
     }
-    if (s.hx) hx_info(*s.hx, cheb_degree, lambda_max);
+    if (s.hx) hx_info(*s.hx, cheb_degree, lambda_max, column_bytes);
   });
 }
 

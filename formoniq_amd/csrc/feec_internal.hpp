@@ -454,7 +454,7 @@ int kuhn_grid_n(Mesh& m);
 // grid, no coordinates, non-Euclidean, k > 2).
 Hx* hx_create(Geom& g, int bc, int k, const KktSystem& s, int sweeps, int cheb_m, double frac,
               bool spectrum_fp32, bool use_graph, std::string* why);
-void hx_info(const Hx& h, int* cheb_m, double lam[2]);
+void hx_info(const Hx& h, int* cheb_m, double lam[2], int* column_bytes);
 // Seconds per launch: finest-level sweep, block preconditioner application
 // (FP64_PC32), Chebyshev product by P, one B application (FP32).
 void hx_kernel_times(Hx& h, int reps, bool use_graph, double out[4]);

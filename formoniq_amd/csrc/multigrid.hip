@@ -2349,21 +2349,45 @@ void hx_kernel_times(Hx& h, int reps, bool use_graph, double out[4]) {
     }
   }) : 0.0;
   out[1] = timed([&](Enq& q) { hx_apply<double, float, double>(q, h, (const double*)r.p, z.p, nullptr, part.p); });
-  out[2] = h.cheb_m > 1 ? timed([&](Enq& q) {
-    pc_spmv<float>(q, h, (const float*)h.d32.p, h.w32.p, nullptr);
-  }) : 0.0;
-  out[3] = timed([&](Enq& q) {
+  const auto core = [&](Enq& q) {
     switch (h.d) {
       case 1: hx_core_d<float, float, 1>(q, h, (const float*)r32.p, nullptr); break;
       case 2: hx_core_d<float, float, 2>(q, h, (const float*)r32.p, nullptr); break;
       case 3: hx_core_d<float, float, 3>(q, h, (const float*)r32.p, nullptr); break;
       default: hx_core_d<float, float, 4>(q, h, (const float*)r32.p, nullptr); break;
     }
-  });
+  };
+  out[3] = timed(core);
+  // the product by P as a Chebyshev step meets it: each launch timed on its own
+  // events right after an (untimed) HX core has streamed its operands through
+  // the caches. Back to back with itself the product finds part of P still
+  // cached (about 20% faster at a 206 MB P), which a solve never sees.
+  out[2] = 0.0;
+  if (h.cheb_m > 1) {
+    Enq q{st};
+    hipEvent_t e0, e1;
+    FEEC_HIP(hipEventCreate(&e0));
+    FEEC_HIP(hipEventCreate(&e1));
+    double sec = 0.0;
+    for (int i = 0; i < reps; ++i) {
+      core(q);
+      FEEC_HIP(hipEventRecord(e0, st));
+      pc_spmv<float>(q, h, (const float*)h.d32.p, h.w32.p, nullptr);
+      FEEC_HIP(hipEventRecord(e1, st));
+      FEEC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FEEC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      sec += ms * 1e-3;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out[2] = sec / reps;
+  }
 }
 
-void hx_info(const Hx& h, int* cheb_m, double lam[2]) {
+void hx_info(const Hx& h, int* cheb_m, double lam[2], int* column_bytes) {
   if (cheb_m) *cheb_m = h.cheb_m;
+  if (column_bytes) *column_bytes = h.P_col16.p ? 2 : 4;
   if (lam) {
     lam[0] = h.lam[0];
     lam[1] = h.lam[1];

@@ -479,8 +479,9 @@ feec_status feec_problem_kernel_times(feec_problem *prob, int reps, double out[6
  * seconds per launch, replayed from a graph of `reps` launches on the context
  * stream (use_graph != 0) or launched one by one, of {a finest-level Jacobi sweep (FP32), one block preconditioner
  * application as an FP64_PC32 solve applies it, one product by the
- * preconditioner matrix in the Chebyshev steps (FP32), one HX application B
- * (FP32)}. */
+ * preconditioner matrix in the Chebyshev steps (FP32; each launch timed on
+ * its own events right after an HX application, as a Chebyshev step meets it),
+ * one HX application B (FP32)}. */
 feec_status feec_problem_mg_kernel_times(feec_problem *prob, int reps, int use_graph, double out[4]);
 
 /* Diagnostics (no counterpart in the reference): average seconds per launch of
